@@ -1,0 +1,604 @@
+/*
+ * ORACLE -- test infrastructure only (see oracle.h).  Plain-C restatement of
+ * the reference's generic 3D nonlinear ADER-DG kernels.  Loop nests, index
+ * layouts and the grouping of every floating-point expression follow the
+ * cited reference lines so that results agree to rounding.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#include "oracle.h"
+#include "../exahype_amd/csrc/dg_tables_data.h"
+
+#define W(n, i) EXA_TAB_gl_weights[n][i]
+#define MAXB 10
+
+/* ---- user PDE: 3D Euler, E/tests/kernels/c/GenericEulerKernelTest3D.cpp:26-55 ---- */
+void orc_flux(const orc_cfg* c, const double* Q, double* F) {
+  const int nv = c->nvar;
+  double* f = F;
+  double* g = F + nv;
+  double* h = F + 2 * nv;
+  if (c->pde == 1) { /* linear advection test PDE: F_d = a_d Q */
+    for (int v = 0; v < nv; ++v) {
+      f[v] = c->adv[0] * Q[v];
+      g[v] = c->adv[1] * Q[v];
+      h[v] = c->adv[2] * Q[v];
+    }
+    return;
+  }
+  const double GAMMA = c->gamma;
+  const double irho = 1.0 / Q[0];
+  const double p = (GAMMA - 1) * (Q[4] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) * irho);
+  f[0] = Q[1];
+  f[1] = irho * Q[1] * Q[1] + p;
+  f[2] = irho * Q[1] * Q[2];
+  f[3] = irho * Q[1] * Q[3];
+  f[4] = irho * Q[1] * (Q[4] + p);
+  g[0] = Q[2];
+  g[1] = irho * Q[2] * Q[1];
+  g[2] = irho * Q[2] * Q[2] + p;
+  g[3] = irho * Q[2] * Q[3];
+  g[4] = irho * Q[2] * (Q[4] + p);
+  h[0] = Q[3];
+  h[1] = irho * Q[3] * Q[1];
+  h[2] = irho * Q[3] * Q[2];
+  h[3] = irho * Q[3] * Q[3] + p;
+  h[4] = irho * Q[3] * (Q[4] + p);
+}
+
+/* E/tests/kernels/c/GenericEulerKernelTest3D.cpp:65-82 */
+void orc_eigenvalues(const orc_cfg* c, const double* Q, int d, double* lambda) {
+  if (c->pde == 1) {
+    for (int v = 0; v < c->nvar; ++v) lambda[v] = c->adv[d];
+    return;
+  }
+  const double GAMMA = c->gamma;
+  double irho = 1.0 / Q[0];
+  double p = (GAMMA - 1) * (Q[4] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) * irho);
+  double u_n = Q[d + 1] * irho;
+  double cs = sqrt(GAMMA * p * irho);
+  lambda[0] = u_n - cs;
+  lambda[1] = u_n;
+  lambda[2] = u_n;
+  lambda[3] = u_n;
+  lambda[4] = u_n + cs;
+}
+
+/* per-thread scratch of the predictor, sized for N=9 and nData<=8 */
+static _Thread_local double* tl_lQi;
+static _Thread_local double* tl_lQi_old;
+static _Thread_local double* tl_rhs;
+static _Thread_local double* tl_lFi;
+static _Thread_local double* tl_lQhi;
+static _Thread_local double* tl_lFhi;
+
+static void ensure_scratch(void) {
+  if (tl_lQi) return;
+  const size_t b4 = MAXB * MAXB * MAXB * MAXB;
+  tl_lQi = (double*)calloc(b4 * 8, sizeof(double));
+  tl_lQi_old = (double*)calloc(b4 * 8, sizeof(double));
+  tl_rhs = (double*)calloc(b4 * 8, sizeof(double));
+  tl_lFi = (double*)calloc(b4 * 4 * 8, sizeof(double));
+  tl_lQhi = (double*)calloc(MAXB * MAXB * MAXB * 8, sizeof(double));
+  tl_lFhi = (double*)calloc(MAXB * MAXB * MAXB * 4 * 8, sizeof(double));
+}
+
+/*
+ * spaceTimePredictorNonlinear<useSource=false,useFlux=true,useNCP=false>
+ * kernels/aderdg/generic/c/3d/spaceTimePredictorNonlinear.cpph:524-577
+ * with aderPicardLoopNonlinear (:43-317), aderPredictorNonlinear (:326-397)
+ * and aderExtrapolatorNonlinear (:400-519).  lQi/lFi may be NULL (then
+ * internal scratch is used).  Returns the number of Picard iterations.
+ */
+int orc_space_time_predictor(const orc_cfg* c, double* lQhbnd, double* lFhbnd,
+                             double* lQi, double* lFi, double* lQhi, double* lFhi,
+                             const double* luh, const double dx[3], double dt) {
+  ensure_scratch();
+  const int N = c->order, B = N + 1, nv = c->nvar, nd = c->nvar + c->npar;
+  const int B2 = B * B, B3 = B2 * B, B4 = B2 * B2;
+  if (!lQi) lQi = tl_lQi;
+  if (!lFi) lFi = tl_lFi;
+  if (!lQhi) lQhi = tl_lQhi;
+  if (!lFhi) lFhi = tl_lFhi;
+  double* lQi_old = tl_lQi_old;
+  double* rhs = tl_rhs;
+/* idx_luh(z,y,x,m); idx_lQi(z,y,x,t,m); idx_rhs(t,z,y,x,m); idx_lFi(t,z,y,x,d,m) */
+#define LUH(z, y, x, m) luh[(((z) * B + (y)) * B + (x)) * nd + (m)]
+#define LQI(A, z, y, x, t, m) A[((((z) * B + (y)) * B + (x)) * B + (t)) * nd + (m)]
+#define RHS(t, z, y, x, m) rhs[((((t) * B + (z)) * B + (y)) * B + (x)) * nv + (m)]
+#define LFI(t, z, y, x, d, m) lFi[(((((t) * B + (z)) * B + (y)) * B + (x)) * 4 + (d)) * nv + (m)]
+  const double* Kxi = &EXA_TAB_Kxi[N][0][0]; /* Kxi[a][b] = Kxi[a*10+b] */
+  const double* iK1 = &EXA_TAB_iK1[N][0][0];
+#define KXI(a, b) Kxi[(a) * MAXB + (b)]
+#define IK1(a, b) iK1[(a) * MAXB + (b)]
+
+  /* 1. trivial initial guess (:68-78) */
+  for (int i = 0; i < B; i++)
+    for (int j = 0; j < B; j++)
+      for (int k = 0; k < B; k++)
+        for (int l = 0; l < B; l++)
+          memcpy(&LQI(lQi, i, j, k, l, 0), &LUH(i, j, k, 0), nd * sizeof(double));
+
+  /* 3. discrete Picard iterations (:82-316) */
+  const int MaxIterations = 2 * (N + 1);
+  int iter;
+  int executed = 0;
+  for (iter = 0; iter < MaxIterations; iter++) {
+    executed++;
+    memcpy(lQi_old, lQi, (size_t)B4 * nd * sizeof(double));
+    for (int i = 0; i < B; i++) { /* time DOF */
+      /* fluxes (:96-115) */
+      for (int j = 0; j < B; j++)
+        for (int k = 0; k < B; k++)
+          for (int l = 0; l < B; l++) {
+            double F[3 * 8];
+            orc_flux(c, &LQI(lQi, j, k, l, i, 0), F);
+            for (int d = 0; d < 3; ++d)
+              for (int m = 0; m < nv; ++m) LFI(i, j, k, l, d, m) = F[d * nv + m];
+          }
+      /* rhs0 = w^3 F0 luh (:118-133) */
+      for (int j = 0; j < B; j++)
+        for (int k = 0; k < B; k++)
+          for (int l = 0; l < B; l++) {
+            const double weight = W(N, j) * W(N, k) * W(N, l);
+            for (int m = 0; m < nv; m++)
+              RHS(i, j, k, l, m) = weight * EXA_TAB_F0[N][i] * LUH(j, k, l, m);
+          }
+      /* x direction (:143-166) */
+      for (int j = 0; j < B; j++)
+        for (int k = 0; k < B; k++) {
+          const double weight = W(N, i) * W(N, j) * W(N, k);
+          const double updateSize = weight * dt / dx[0];
+          for (int l = 0; l < B; l++)
+            for (int m = 0; m < nv; m++)
+              for (int n = 0; n < B; n++)
+                RHS(i, j, k, l, m) -= updateSize * LFI(i, j, k, n, 0, m) * KXI(n, l);
+        }
+      /* y direction (:169-194) */
+      for (int j = 0; j < B; j++)
+        for (int k = 0; k < B; k++) {
+          const double weight = W(N, i) * W(N, j) * W(N, k);
+          const double updateSize = weight * dt / dx[1];
+          for (int l = 0; l < B; l++)
+            for (int m = 0; m < nv; m++)
+              for (int n = 0; n < B; n++)
+                RHS(i, j, l, k, m) -= updateSize * LFI(i, j, n, k, 1, m) * KXI(n, l);
+        }
+      /* z direction (:197-222) */
+      for (int j = 0; j < B; j++)
+        for (int k = 0; k < B; k++) {
+          const double weight = W(N, i) * W(N, j) * W(N, k);
+          const double updateSize = weight * dt / dx[2];
+          for (int l = 0; l < B; l++)
+            for (int m = 0; m < nv; m++)
+              for (int n = 0; n < B; n++)
+                RHS(i, l, j, k, m) -= updateSize * LFI(i, n, j, k, 2, m) * KXI(n, l);
+        }
+    }
+    /* zero the variables of lQi (:255-268) */
+    for (int i = 0; i < B; i++)
+      for (int j = 0; j < B; j++)
+        for (int k = 0; k < B; k++)
+          for (int l = 0; l < B; l++)
+            for (int m = 0; m < nv; m++) LQI(lQi, i, j, k, l, m) = 0.0;
+    /* 4. multiply with iK1 (:270-296) */
+    for (int i = 0; i < B; i++)
+      for (int j = 0; j < B; j++)
+        for (int k = 0; k < B; k++) {
+          const double weight = W(N, i) * W(N, j) * W(N, k);
+          const double iweight = 1.0 / weight;
+          for (int l = 0; l < B; l++)
+            for (int m = 0; m < nv; m++)
+              for (int n = 0; n < B; n++)
+                LQI(lQi, i, j, k, l, m) += iweight * RHS(n, i, j, k, m) * IK1(l, n);
+        }
+    /* 5. exit condition (:299-306) */
+    const double tol = 1e-7;
+    double sq_res = 0.0;
+    for (int q = 0; q < B4 * nd; q++) sq_res += (lQi_old[q] - lQi[q]) * (lQi_old[q] - lQi[q]);
+    if (sq_res < tol * tol) break;
+  }
+
+  /* time averages (:326-397) */
+  double* lFhi_x = lFhi;
+  double* lFhi_y = lFhi + B3 * nv;
+  double* lFhi_z = lFhi + 2 * B3 * nv;
+  double* lShi = lFhi + 3 * B3 * nv;
+  memset(lQhi, 0, (size_t)B3 * nd * sizeof(double));
+  memset(lFhi, 0, (size_t)4 * B3 * nv * sizeof(double));
+  (void)lShi;
+#define LFHI(A, a, b, cc, m) A[(((a) * B + (b)) * B + (cc)) * nv + (m)]
+#define LQHI(z, y, x, m) lQhi[(((z) * B + (y)) * B + (x)) * nd + (m)]
+  for (int i = 0; i < B; i++)
+    for (int j = 0; j < B; j++)
+      for (int k = 0; k < B; k++) {
+        for (int l = 0; l < nv; l++)
+          for (int m = 0; m < B; m++) {
+            LFHI(lFhi_x, i, j, k, l) += LFI(m, i, j, k, 0, l) * W(N, m);
+            LFHI(lFhi_y, i, k, j, l) += LFI(m, i, j, k, 1, l) * W(N, m);
+            LFHI(lFhi_z, j, k, i, l) += LFI(m, i, j, k, 2, l) * W(N, m);
+          }
+        for (int l = 0; l < nd; l++)
+          for (int m = 0; m < B; m++) LQHI(i, j, k, l) += LQI(lQi, i, j, k, m, l) * W(N, m);
+      }
+
+  /* face extrapolation (:400-519) */
+  memset(lQhbnd, 0, (size_t)6 * B2 * nd * sizeof(double));
+  memset(lFhbnd, 0, (size_t)6 * B2 * nv * sizeof(double));
+  const double* FL = EXA_TAB_FLCoeff[N];
+  const double* FR = EXA_TAB_FRCoeff[N];
+#define QB(f, a, b, m) lQhbnd[(((f) * B + (a)) * B + (b)) * nd + (m)]
+#define FB(f, a, b, m) lFhbnd[(((f) * B + (a)) * B + (b)) * nv + (m)]
+  for (int i = 0; i < B; i++)
+    for (int j = 0; j < B; j++) {
+      for (int k = 0; k < nv; k++)
+        for (int l = 0; l < B; l++) {
+          FB(0, i, j, k) += LFHI(lFhi_x, i, j, l, k) * FL[l];
+          FB(1, i, j, k) += LFHI(lFhi_x, i, j, l, k) * FR[l];
+        }
+      for (int k = 0; k < nd; k++)
+        for (int l = 0; l < B; l++) {
+          QB(0, i, j, k) += LQHI(i, j, l, k) * FL[l];
+          QB(1, i, j, k) += LQHI(i, j, l, k) * FR[l];
+        }
+    }
+  for (int i = 0; i < B; i++)
+    for (int j = 0; j < B; j++) {
+      for (int k = 0; k < nv; k++)
+        for (int l = 0; l < B; l++) {
+          FB(2, i, j, k) += LFHI(lFhi_y, i, j, l, k) * FL[l];
+          FB(3, i, j, k) += LFHI(lFhi_y, i, j, l, k) * FR[l];
+        }
+      for (int k = 0; k < nd; k++)
+        for (int l = 0; l < B; l++) {
+          QB(2, i, j, k) += LQHI(i, l, j, k) * FL[l];
+          QB(3, i, j, k) += LQHI(i, l, j, k) * FR[l];
+        }
+    }
+  for (int i = 0; i < B; i++)
+    for (int j = 0; j < B; j++) {
+      for (int k = 0; k < nv; k++)
+        for (int l = 0; l < B; l++) {
+          FB(4, i, j, k) += LFHI(lFhi_z, i, j, l, k) * FL[l];
+          FB(5, i, j, k) += LFHI(lFhi_z, i, j, l, k) * FR[l];
+        }
+      for (int k = 0; k < nd; k++)
+        for (int l = 0; l < B; l++) {
+          QB(4, i, j, k) += LQHI(l, i, j, k) * FL[l];
+          QB(5, i, j, k) += LQHI(l, i, j, k) * FR[l];
+        }
+    }
+  return executed;
+#undef LUH
+#undef LQI
+#undef RHS
+#undef LFI
+#undef QB
+#undef FB
+}
+
+/* volumeIntegralNonlinear<useSourceOrNCP=false,useFlux=true>,
+ * kernels/aderdg/generic/c/3d/volumeIntegralNonlinear.cpph:26-125 */
+void orc_volume_integral(const orc_cfg* c, double* lduh, const double* lFhi, const double dx[3]) {
+  const int N = c->order, B = N + 1, nv = c->nvar;
+  const int B3 = B * B * B;
+  const double* Kxi = &EXA_TAB_Kxi[N][0][0];
+#define IDX(a, b, cc, m) ((((a) * B + (b)) * B + (cc)) * nv + (m))
+  memset(lduh, 0, (size_t)B3 * nv * sizeof(double));
+  const int x_off = 0, y_off = B3 * nv, z_off = 2 * B3 * nv;
+  for (int i = 0; i < B; i++)
+    for (int j = 0; j < B; j++) {
+      const double weight = W(N, i) * W(N, j);
+      const double updateSize = weight / dx[0];
+      for (int k = 0; k < B; k++)
+        for (int l = 0; l < nv; l++)
+          for (int m = 0; m < B; m++)
+            lduh[IDX(i, j, k, l)] += KXI(k, m) * lFhi[x_off + IDX(i, j, m, l)] * updateSize;
+    }
+  for (int i = 0; i < B; i++)
+    for (int j = 0; j < B; j++) {
+      const double weight = W(N, i) * W(N, j);
+      const double updateSize = weight / dx[1];
+      for (int k = 0; k < B; k++)
+        for (int l = 0; l < nv; l++)
+          for (int m = 0; m < B; m++)
+            lduh[IDX(i, k, j, l)] += KXI(k, m) * lFhi[y_off + IDX(i, j, m, l)] * updateSize;
+    }
+  for (int i = 0; i < B; i++)
+    for (int j = 0; j < B; j++) {
+      const double weight = W(N, i) * W(N, j);
+      const double updateSize = weight / dx[2];
+      for (int k = 0; k < B; k++)
+        for (int l = 0; l < nv; l++)
+          for (int m = 0; m < B; m++)
+            lduh[IDX(k, i, j, l)] += KXI(k, m) * lFhi[z_off + IDX(i, j, m, l)] * updateSize;
+    }
+#undef IDX
+}
+
+/* surfaceIntegralNonlinear, kernels/aderdg/generic/c/3d/surfaceIntegralNonlinear.cpph:21-86 */
+void orc_surface_integral(const orc_cfg* c, double* lduh, const double* lFbnd, const double dx[3]) {
+  const int N = c->order, B = N + 1, nv = c->nvar;
+  const double* FL = EXA_TAB_FLCoeff[N];
+  const double* FR = EXA_TAB_FRCoeff[N];
+#define IL(a, b, cc, m) ((((a) * B + (b)) * B + (cc)) * nv + (m))
+#define IF(f, a, b, m) ((((f) * B + (a)) * B + (b)) * nv + (m))
+  for (int i = 0; i < B; i++)
+    for (int j = 0; j < B; j++) {
+      const double weight = W(N, i) * W(N, j);
+      const double updateSize = weight / dx[0];
+      for (int k = 0; k < B; k++)
+        for (int l = 0; l < nv; l++)
+          lduh[IL(i, j, k, l)] -= (lFbnd[IF(1, i, j, l)] * FR[k] - lFbnd[IF(0, i, j, l)] * FL[k]) * updateSize;
+    }
+  for (int i = 0; i < B; i++)
+    for (int j = 0; j < B; j++)
+      for (int k = 0; k < B; k++) {
+        const double weight = W(N, i) * W(N, k);
+        const double updateSize = weight / dx[1];
+        for (int l = 0; l < nv; l++)
+          lduh[IL(i, j, k, l)] -= (lFbnd[IF(3, i, k, l)] * FR[j] - lFbnd[IF(2, i, k, l)] * FL[j]) * updateSize;
+      }
+  for (int i = 0; i < B; i++)
+    for (int j = 0; j < B; j++)
+      for (int k = 0; k < B; k++) {
+        const double weight = W(N, j) * W(N, k);
+        const double updateSize = weight / dx[2];
+        for (int l = 0; l < nv; l++)
+          lduh[IL(i, j, k, l)] -= (lFbnd[IF(5, j, k, l)] * FR[i] - lFbnd[IF(4, j, k, l)] * FL[i]) * updateSize;
+      }
+#undef IL
+#undef IF
+}
+
+/* riemannSolverNonlinear<useNCP=false>, kernels/aderdg/generic/c/3d/riemannSolverNonlinear.cpph:38-120 */
+void orc_riemann_solver(const orc_cfg* c, double* FL, double* FR, const double* QL,
+                        const double* QR, double dt, int direction) {
+  (void)dt;
+  const int N = c->order, B = N + 1, nv = c->nvar, nd = c->nvar + c->npar;
+  double QavL[16] = {0.0}, QavR[16] = {0.0};
+  for (int i = 0; i < B; i++)
+    for (int j = 0; j < B; j++) {
+      const double weight = W(N, i) * W(N, j);
+      for (int k = 0; k < nd; k++) {
+        QavL[k] += weight * QL[(i * B + j) * nd + k];
+        QavR[k] += weight * QR[(i * B + j) * nd + k];
+      }
+    }
+  double LL[16] = {0.0}, LR[16] = {0.0};
+  orc_eigenvalues(c, QavL, direction, LL);
+  orc_eigenvalues(c, QavR, direction, LR);
+  double smax_L = fabs(LL[0]), smax_R = fabs(LR[0]);
+  for (int v = 1; v < nv; ++v) {
+    if (fabs(LL[v]) > smax_L) smax_L = fabs(LL[v]);
+    if (fabs(LR[v]) > smax_R) smax_R = fabs(LR[v]);
+  }
+  const double smax = (smax_L < smax_R) ? smax_R : smax_L; /* std::max(a,b) */
+  for (int i = 0; i < B; i++)
+    for (int j = 0; j < B; j++)
+      for (int k = 0; k < nv; k++) {
+        const int f = (i * B + j) * nv + k, q = (i * B + j) * nd + k;
+        FL[f] = 0.5 * (FR[f] + FL[f]) - 0.5 * smax * (QR[q] - QL[q]);
+        FR[f] = FL[f];
+      }
+}
+
+/* solutionUpdate, kernels/aderdg/generic/c/3d/solutionUpdate.cpph:23-47 */
+void orc_solution_update(const orc_cfg* c, double* luh, const double* lduh, double dt) {
+  const int N = c->order, B = N + 1, nv = c->nvar, nd = c->nvar + c->npar;
+  for (int i = 0; i < B; i++)
+    for (int j = 0; j < B; j++)
+      for (int k = 0; k < B; k++) {
+        const double weight = W(N, i) * W(N, j) * W(N, k);
+        const double updateSize = dt / weight;
+        for (int l = 0; l < nv; l++)
+          luh[((i * B + j) * B + k) * nd + l] += lduh[((i * B + j) * B + k) * nv + l] * updateSize;
+      }
+}
+
+/* stableTimeStepSize, kernels/aderdg/generic/c/3d/stableTimeStepSize.cpph:21-61 */
+double orc_stable_time_step_size(const orc_cfg* c, const double* luh, const double dx[3]) {
+  static const double PNPM[10] = {1.0, 0.33, 0.17, 0.1, 0.069, 0.045, 0.038, 0.03, 0.02, 0.015};
+  const int N = c->order, B = N + 1, nv = c->nvar, nd = c->nvar + c->npar;
+  double dt = DBL_MAX;
+  double lambda[16];
+  for (int i = 0; i < B; i++)
+    for (int j = 0; j < B; j++)
+      for (int k = 0; k < B; k++) {
+        double denominator = 0.0;
+        for (int d = 0; d < 3; d++) {
+          orc_eigenvalues(c, luh + ((i * B + j) * B + k) * nd, d, lambda);
+          double maxEigenvalue = 0.0;
+          for (int ivar = 0; ivar < nv; ivar++) {
+            const double a = fabs(lambda[ivar]);
+            maxEigenvalue = (maxEigenvalue < a) ? a : maxEigenvalue; /* std::max(a, m) */
+          }
+          denominator += maxEigenvalue / dx[d];
+        }
+        const double cand = c->cfl * PNPM[N] / denominator;
+        dt = (cand < dt) ? cand : dt; /* std::min(dt, cand) */
+      }
+  return dt;
+}
+
+/* ---- initial conditions (build-defined user adjustPointSolution) ---- */
+void orc_ic_point(const orc_cfg* c, const orc_ic* ic, const double x[3], double t, double* Q) {
+  const double g = c->gamma;
+  double rho, u[3], p;
+  if (ic->kind == 2) {
+    for (int v = 0; v < c->nvar; ++v) Q[v] = ic->q0[v];
+    return;
+  }
+  if (ic->kind == 3) { /* sine wave for the linear-advection test PDE */
+    double s = 0.0;
+    for (int m = 0; m < ic->nmodes; ++m) {
+      const double ph = 2.0 * M_PI * (ic->k[m][0] * (x[0] - c->adv[0] * t) + ic->k[m][1] * (x[1] - c->adv[1] * t) +
+                                      ic->k[m][2] * (x[2] - c->adv[2] * t)) + ic->phi[m];
+      s += ic->a[m] * sin(ph);
+    }
+    for (int v = 0; v < c->nvar; ++v) Q[v] = 1.0 + (v + 1) * s;
+    return;
+  }
+  if (ic->kind == 4) { /* entropy wave: density modes advected with uinf, constant u and p */
+    double s = 0.0;
+    for (int m = 0; m < ic->nmodes; ++m)
+      s += ic->a[m] * sin(2.0 * M_PI * (ic->k[m][0] * (x[0] - ic->uinf[0] * t) + ic->k[m][1] * (x[1] - ic->uinf[1] * t) +
+                                        ic->k[m][2] * (x[2] - ic->uinf[2] * t)) + ic->phi[m]);
+    rho = ic->rho_inf + ic->amp * s;
+    p = ic->p_inf;
+    u[0] = ic->uinf[0];
+    u[1] = ic->uinf[1];
+    u[2] = ic->uinf[2];
+  } else if (ic->kind == 1) { /* Sod sphere */
+    const double dx0 = x[0] - ic->xc[0], dx1 = x[1] - ic->xc[1], dx2 = x[2] - ic->xc[2];
+    const double r = sqrt(dx0 * dx0 + dx1 * dx1 + dx2 * dx2);
+    rho = r < ic->r0 ? ic->rho_in : ic->rho_out;
+    p = r < ic->r0 ? ic->p_in : ic->p_out;
+    u[0] = u[1] = u[2] = 0.0;
+  } else { /* isentropic vortex advected with uinf (periodic in x,y) + density modes */
+    double xs = x[0] - ic->uinf[0] * t, ys = x[1] - ic->uinf[1] * t;
+    xs = xs - floor(xs);
+    ys = ys - floor(ys);
+    const double rx = (xs - ic->xc[0]) / ic->L, ry = (ys - ic->xc[1]) / ic->L;
+    const double r2 = rx * rx + ry * ry;
+    const double e1 = exp(0.5 * (1.0 - r2));
+    const double du = ic->eps / (2.0 * M_PI) * e1;
+    const double dT = -(g - 1.0) * ic->eps * ic->eps / (8.0 * g * M_PI * M_PI) * exp(1.0 - r2);
+    const double T = 1.0 + dT;
+    rho = ic->rho_inf * pow(T, 1.0 / (g - 1.0));
+    p = ic->p_inf * pow(T, g / (g - 1.0));
+    u[0] = ic->uinf[0] - du * ry;
+    u[1] = ic->uinf[1] + du * rx;
+    u[2] = ic->uinf[2];
+    double s = 0.0;
+    for (int m = 0; m < ic->nmodes; ++m)
+      s += ic->a[m] * sin(2.0 * M_PI * (ic->k[m][0] * x[0] + ic->k[m][1] * x[1] + ic->k[m][2] * x[2]) + ic->phi[m]);
+    rho += ic->amp * s;
+  }
+  Q[0] = rho;
+  Q[1] = rho * u[0];
+  Q[2] = rho * u[1];
+  Q[3] = rho * u[2];
+  Q[4] = p / (g - 1.0) + 0.5 * rho * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+}
+
+/* solutionAdjustment, kernels/aderdg/generic/c/3d/solutionAdjustment.cpph:17-52 */
+void orc_solution_adjustment(const orc_cfg* c, const orc_ic* ic, double* luh,
+                             const double centre[3], const double dx[3], double t) {
+  const int N = c->order, B = N + 1, nd = c->nvar + c->npar;
+  double x[3];
+  for (int i = 0; i < B; i++) {
+    const double qi = EXA_TAB_gl_nodes[N][i];
+    for (int j = 0; j < B; j++) {
+      const double qj = EXA_TAB_gl_nodes[N][j];
+      for (int k = 0; k < B; k++) {
+        const double qk = EXA_TAB_gl_nodes[N][k];
+        x[2] = centre[2] + dx[2] * (qi - 0.5);
+        x[1] = centre[1] + dx[1] * (qj - 0.5);
+        x[0] = centre[0] + dx[0] * (qk - 0.5);
+        orc_ic_point(c, ic, x, t, luh + ((i * B + j) * B + k) * nd);
+      }
+    }
+  }
+}
+
+/* ---- periodic grid driver ---- */
+int orc_grid_alloc(const orc_cfg* c, orc_grid* g, int nx, int ny, int nz) {
+  const int B = c->order + 1, nd = c->nvar + c->npar, nv = c->nvar;
+  const size_t nc = (size_t)nx * ny * nz;
+  memset(g, 0, sizeof(*g));
+  g->nx = nx; g->ny = ny; g->nz = nz;
+  for (int d = 0; d < 3; ++d) { g->offset[d] = 0.0; g->width[d] = 1.0; }
+  g->luh = (double*)calloc(nc * B * B * B * nd, sizeof(double));
+  g->lduh = (double*)calloc(nc * B * B * B * nv, sizeof(double));
+  g->lQhbnd = (double*)calloc(nc * 6 * B * B * nd, sizeof(double));
+  g->lFhbnd = (double*)calloc(nc * 6 * B * B * nv, sizeof(double));
+  g->iters = (int*)calloc(nc, sizeof(int));
+  return (g->luh && g->lduh && g->lQhbnd && g->lFhbnd && g->iters) ? 0 : -1;
+}
+
+void orc_grid_free(orc_grid* g) {
+  free(g->luh); free(g->lduh); free(g->lQhbnd); free(g->lFhbnd); free(g->iters);
+  memset(g, 0, sizeof(*g));
+}
+
+static void cell_geom(const orc_grid* g, int cidx, double centre[3], double dx[3]) {
+  const int ix = cidx % g->nx, iy = (cidx / g->nx) % g->ny, iz = cidx / (g->nx * g->ny);
+  dx[0] = g->width[0] / g->nx; dx[1] = g->width[1] / g->ny; dx[2] = g->width[2] / g->nz;
+  centre[0] = g->offset[0] + (ix + 0.5) * dx[0];
+  centre[1] = g->offset[1] + (iy + 0.5) * dx[1];
+  centre[2] = g->offset[2] + (iz + 0.5) * dx[2];
+}
+
+void orc_grid_init(const orc_cfg* c, orc_grid* g, const orc_ic* ic) {
+  const int B = c->order + 1, nd = c->nvar + c->npar;
+  const int nc = g->nx * g->ny * g->nz;
+  for (int q = 0; q < nc; ++q) {
+    double centre[3], dx[3];
+    cell_geom(g, q, centre, dx);
+    orc_solution_adjustment(c, ic, g->luh + (size_t)q * B * B * B * nd, centre, dx, 0.0);
+  }
+}
+
+double orc_grid_stable_dt(const orc_cfg* c, const orc_grid* g, int nthreads) {
+  const int B = c->order + 1, nd = c->nvar + c->npar;
+  const int nc = g->nx * g->ny * g->nz;
+  double dxv[3] = {g->width[0] / g->nx, g->width[1] / g->ny, g->width[2] / g->nz};
+  double dtmin = DBL_MAX;
+#pragma omp parallel for num_threads(nthreads) reduction(min : dtmin) schedule(static)
+  for (int q = 0; q < nc; ++q) {
+    const double d = orc_stable_time_step_size(c, g->luh + (size_t)q * B * B * B * nd, dxv);
+    if (d < dtmin) dtmin = d;
+  }
+  return dtmin;
+}
+
+void orc_grid_predict_cells(const orc_cfg* c, orc_grid* g, double dt, int first, int count, int nthreads) {
+  const int B = c->order + 1, nd = c->nvar + c->npar, nv = c->nvar;
+  const int B2 = B * B, B3 = B2 * B;
+  double dxv[3] = {g->width[0] / g->nx, g->width[1] / g->ny, g->width[2] / g->nz};
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 16)
+  for (int q = first; q < first + count; ++q) {
+    ensure_scratch();
+    g->iters[q] = orc_space_time_predictor(c, g->lQhbnd + (size_t)q * 6 * B2 * nd,
+                                           g->lFhbnd + (size_t)q * 6 * B2 * nv, NULL, NULL, NULL, tl_lFhi,
+                                           g->luh + (size_t)q * B3 * nd, dxv, dt);
+    orc_volume_integral(c, g->lduh + (size_t)q * B3 * nv, tl_lFhi, dxv);
+  }
+}
+
+double orc_grid_step(const orc_cfg* c, orc_grid* g, double dt, int nthreads) {
+  const int B = c->order + 1, nd = c->nvar + c->npar, nv = c->nvar;
+  const int B2 = B * B, B3 = B2 * B;
+  const int nx = g->nx, ny = g->ny, nz = g->nz, nc = nx * ny * nz;
+  double dxv[3] = {g->width[0] / nx, g->width[1] / ny, g->width[2] / nz};
+  /* Prediction sweep (E/solvers/ADERDGSolver.cpp:1818-1946) */
+  orc_grid_predict_cells(c, g, dt, 0, nc, nthreads);
+  /* NeighbourDataMerging: Riemann problem on every face, periodic wrap
+   * (E/solvers/ADERDGSolver.cpp:2744-2825; left cell face 2d+1, right cell face 2d) */
+  for (int d = 0; d < 3; ++d) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int q = 0; q < nc; ++q) {
+      const int ix = q % nx, iy = (q / nx) % ny, iz = q / (nx * ny);
+      int jx = ix, jy = iy, jz = iz;
+      if (d == 0) jx = (ix + 1) % nx;
+      if (d == 1) jy = (iy + 1) % ny;
+      if (d == 2) jz = (iz + 1) % nz;
+      const int r = (jz * ny + jy) * nx + jx;
+      orc_riemann_solver(c, g->lFhbnd + ((size_t)q * 6 + 2 * d + 1) * B2 * nv,
+                         g->lFhbnd + ((size_t)r * 6 + 2 * d) * B2 * nv,
+                         g->lQhbnd + ((size_t)q * 6 + 2 * d + 1) * B2 * nd,
+                         g->lQhbnd + ((size_t)r * 6 + 2 * d) * B2 * nd, dt, d);
+    }
+  }
+  /* SolutionUpdate (E/solvers/ADERDGSolver.cpp:2100-2179) */
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (int q = 0; q < nc; ++q) {
+    orc_surface_integral(c, g->lduh + (size_t)q * B3 * nv, g->lFhbnd + (size_t)q * 6 * B2 * nv, dxv);
+    orc_solution_update(c, g->luh + (size_t)q * B3 * nd, g->lduh + (size_t)q * B3 * nv, dt);
+  }
+  /* TimeStepSizeComputation (E/solvers/ADERDGSolver.cpp:1948-2001) */
+  return orc_grid_stable_dt(c, g, nthreads);
+}

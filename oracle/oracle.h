@@ -1,0 +1,106 @@
+/*
+ * ORACLE -- test infrastructure only.
+ *
+ * A plain-C restatement of the reference's generic CPU kernels for the 3D
+ * nonlinear ADER-DG scheme and the finite-volume (Godunov / MUSCL-Hancock)
+ * schemes, used to check the HIP product path.  Only the tests, the smoke
+ * check and the CPU-baseline leg of bench.py may load it; the product never
+ * calls it.  Every function keeps the reference's array layout (row-major,
+ * variable index fastest) and its loop and summation order, so that on the
+ * same inputs it reproduces the reference's arithmetic.  The file:line each
+ * function follows is cited beside it (paths relative to the reference repo).
+ *
+ * Pins (what checks each function): see tests/test_oracle_*.py.
+ *   flux/eigenvalues      golden testPDEFluxes (generic_euler_testdata.cpp)
+ *   volume integral       golden testVolumeIntegralNonlinear (N=3)
+ *   surface integral      golden testSurfaceIntegralNonlinear (N=3)
+ *   riemann solver        golden testRiemannSolverNonlinear (N=3, random states)
+ *   solution update       golden testSolutionUpdate (N=3)
+ *   space-time predictor  golden testSpaceTimePredictorNonlinear (N=3, constant
+ *                         state) + exactness for linear advection (closed form)
+ *   stable dt             closed form
+ *   whole scheme          convergence order on the isentropic vortex
+ *   FV kernels            closed forms, conservation, symmetry
+ */
+#ifndef EXA_ORACLE_H
+#define EXA_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int order;      /* N; B = N+1 nodes per axis                       */
+  int nvar;       /* conserved variables (5 for 3D Euler)            */
+  int npar;       /* material parameters carried in luh (0 for Euler) */
+  double gamma;   /* ratio of specific heats                         */
+  double cfl;     /* CFL factor (0.9 for ADER-DG)                    */
+  int pde;        /* 0: 3D Euler (linear advection test PDE when 1)  */
+  double adv[3];  /* advection velocity for pde==1                   */
+} orc_cfg;
+
+/* user PDE (E/tests/kernels/c/GenericEulerKernelTest3D.cpp:26-82) */
+void orc_flux(const orc_cfg* c, const double* Q, double* F /* [3][nvar] */);
+void orc_eigenvalues(const orc_cfg* c, const double* Q, int d, double* lambda);
+
+/* ADER-DG kernels, kernels/aderdg/generic/c/3d/ */
+int orc_space_time_predictor(const orc_cfg* c, double* lQhbnd, double* lFhbnd,
+                             double* lQi, double* lFi, double* lQhi, double* lFhi,
+                             const double* luh, const double dx[3], double dt);
+void orc_volume_integral(const orc_cfg* c, double* lduh, const double* lFhi, const double dx[3]);
+void orc_surface_integral(const orc_cfg* c, double* lduh, const double* lFhbnd, const double dx[3]);
+void orc_riemann_solver(const orc_cfg* c, double* FL, double* FR, const double* QL,
+                        const double* QR, double dt, int direction);
+void orc_solution_update(const orc_cfg* c, double* luh, const double* lduh, double dt);
+double orc_stable_time_step_size(const orc_cfg* c, const double* luh, const double dx[3]);
+
+/* initial conditions at GL nodes (kernels/aderdg/generic/c/3d/solutionAdjustment.cpph:17-52) */
+typedef struct {
+  int kind;            /* 0 vortex+modes, 1 Sod sphere, 2 constant, 3 advected sine, 4 entropy wave */
+  double xc[3];        /* vortex centre / sphere centre                  */
+  double L, eps;       /* vortex length scale and strength               */
+  double uinf[3];      /* background velocity                            */
+  double rho_inf, p_inf;
+  double amp;          /* density-mode amplitude                         */
+  int nmodes;
+  double k[8][3], a[8], phi[8];
+  double r0, rho_in, p_in, rho_out, p_out;  /* Sod sphere */
+  double q0[8];        /* constant state (kind 2)                        */
+} orc_ic;
+void orc_ic_point(const orc_cfg* c, const orc_ic* ic, const double x[3], double t, double* Q);
+void orc_solution_adjustment(const orc_cfg* c, const orc_ic* ic, double* luh,
+                             const double centre[3], const double dx[3], double t);
+
+/* periodic regular grid driver (non-fused stepping, E/runners/Runner.cpp:1125-1190) */
+typedef struct {
+  int nx, ny, nz;
+  double offset[3], width[3];
+  double* luh;      /* [cell][B^3][nData]   */
+  double* lduh;     /* [cell][B^3][nvar]    */
+  double* lQhbnd;   /* [cell][6][B^2][nData]*/
+  double* lFhbnd;   /* [cell][6][B^2][nvar] */
+  int* iters;       /* Picard iterations of the last predictor per cell */
+} orc_grid;
+int orc_grid_alloc(const orc_cfg* c, orc_grid* g, int nx, int ny, int nz);
+void orc_grid_free(orc_grid* g);
+void orc_grid_init(const orc_cfg* c, orc_grid* g, const orc_ic* ic);
+double orc_grid_stable_dt(const orc_cfg* c, const orc_grid* g, int nthreads);
+/* one step with dt; returns the stable dt of the new state */
+double orc_grid_step(const orc_cfg* c, orc_grid* g, double dt, int nthreads);
+/* predictor+volume integral on a subset of cells (CPU baseline sample) */
+void orc_grid_predict_cells(const orc_cfg* c, orc_grid* g, double dt, int first, int count, int nthreads);
+
+/* FV kernels, kernels/finitevolumes/ */
+double orc_fv_rusanov(const orc_cfg* c, int dim, double* fL, double* fR,
+                      const double* qL, const double* qR, int normal);
+double orc_fv_stable_dt(const orc_cfg* c, int dim, int patch, int ghost,
+                        const double* luh, const double* dx, double cfl);
+double orc_godunov_update(const orc_cfg* c, int dim, int patch, double* luh_new,
+                          const double* luh, const double* dx, double dt, double cfl);
+double orc_musclhancock_update(const orc_cfg* c, int patch, double* luh_new,
+                               const double* luh, const double* dx, double dt, double cfl);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

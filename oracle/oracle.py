@@ -1,0 +1,223 @@
+"""ctypes binding of the oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import
+this module; the product package never does.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+class Cfg(C.Structure):
+    _fields_ = [("order", C.c_int), ("nvar", C.c_int), ("npar", C.c_int),
+                ("gamma", C.c_double), ("cfl", C.c_double), ("pde", C.c_int),
+                ("adv", C.c_double * 3)]
+
+
+class IC(C.Structure):
+    _fields_ = [("kind", C.c_int), ("xc", C.c_double * 3), ("L", C.c_double), ("eps", C.c_double),
+                ("uinf", C.c_double * 3), ("rho_inf", C.c_double), ("p_inf", C.c_double),
+                ("amp", C.c_double), ("nmodes", C.c_int), ("k", (C.c_double * 3) * 8),
+                ("a", C.c_double * 8), ("phi", C.c_double * 8), ("r0", C.c_double),
+                ("rho_in", C.c_double), ("p_in", C.c_double), ("rho_out", C.c_double),
+                ("p_out", C.c_double), ("q0", C.c_double * 8)]
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("offset", C.c_double * 3), ("width", C.c_double * 3),
+                ("luh", C.POINTER(C.c_double)), ("lduh", C.POINTER(C.c_double)),
+                ("lQhbnd", C.POINTER(C.c_double)), ("lFhbnd", C.POINTER(C.c_double)),
+                ("iters", C.POINTER(C.c_int))]
+
+
+_lib = None
+_D = C.POINTER(C.c_double)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        L.orc_flux.argtypes = [P(Cfg), _D, _D]
+        L.orc_eigenvalues.argtypes = [P(Cfg), _D, C.c_int, _D]
+        L.orc_space_time_predictor.argtypes = [P(Cfg), _D, _D, _D, _D, _D, _D, _D, _D, C.c_double]
+        L.orc_space_time_predictor.restype = C.c_int
+        L.orc_volume_integral.argtypes = [P(Cfg), _D, _D, _D]
+        L.orc_surface_integral.argtypes = [P(Cfg), _D, _D, _D]
+        L.orc_riemann_solver.argtypes = [P(Cfg), _D, _D, _D, _D, C.c_double, C.c_int]
+        L.orc_solution_update.argtypes = [P(Cfg), _D, _D, C.c_double]
+        L.orc_stable_time_step_size.argtypes = [P(Cfg), _D, _D]
+        L.orc_stable_time_step_size.restype = C.c_double
+        L.orc_ic_point.argtypes = [P(Cfg), P(IC), _D, C.c_double, _D]
+        L.orc_solution_adjustment.argtypes = [P(Cfg), P(IC), _D, _D, _D, C.c_double]
+        L.orc_grid_alloc.argtypes = [P(Cfg), P(Grid), C.c_int, C.c_int, C.c_int]
+        L.orc_grid_alloc.restype = C.c_int
+        L.orc_grid_free.argtypes = [P(Grid)]
+        L.orc_grid_init.argtypes = [P(Cfg), P(Grid), P(IC)]
+        L.orc_grid_stable_dt.argtypes = [P(Cfg), P(Grid), C.c_int]
+        L.orc_grid_stable_dt.restype = C.c_double
+        L.orc_grid_step.argtypes = [P(Cfg), P(Grid), C.c_double, C.c_int]
+        L.orc_grid_step.restype = C.c_double
+        L.orc_grid_predict_cells.argtypes = [P(Cfg), P(Grid), C.c_double, C.c_int, C.c_int, C.c_int]
+        L.orc_fv_rusanov.argtypes = [P(Cfg), C.c_int, _D, _D, _D, _D, C.c_int]
+        L.orc_fv_rusanov.restype = C.c_double
+        L.orc_fv_stable_dt.argtypes = [P(Cfg), C.c_int, C.c_int, C.c_int, _D, _D, C.c_double]
+        L.orc_fv_stable_dt.restype = C.c_double
+        L.orc_godunov_update.argtypes = [P(Cfg), C.c_int, C.c_int, _D, _D, _D, C.c_double, C.c_double]
+        L.orc_godunov_update.restype = C.c_double
+        L.orc_musclhancock_update.argtypes = [P(Cfg), C.c_int, _D, _D, _D, C.c_double, C.c_double]
+        L.orc_musclhancock_update.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+def cfg(order, nvar=5, npar=0, gamma=1.4, cfl=0.9, pde=0, adv=(0.0, 0.0, 0.0)):
+    c = Cfg(order, nvar, npar, gamma, cfl, pde)
+    for i in range(3):
+        c.adv[i] = adv[i]
+    return c
+
+
+def dvec(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+def flux(c, Q):
+    F = np.zeros(3 * c.nvar)
+    lib().orc_flux(C.byref(c), ptr(dvec(Q)), ptr(F))
+    return F.reshape(3, c.nvar)
+
+
+def eigenvalues(c, Q, d):
+    lam = np.zeros(16)
+    lib().orc_eigenvalues(C.byref(c), ptr(dvec(Q)), d, ptr(lam))
+    return lam[:c.nvar]
+
+
+def space_time_predictor(c, luh, dx, dt):
+    B = c.order + 1
+    nd, nv = c.nvar + c.npar, c.nvar
+    luh = dvec(luh)
+    out = dict(lQhbnd=np.zeros(6 * B * B * nd), lFhbnd=np.zeros(6 * B * B * nv),
+               lQi=np.zeros(B ** 4 * nd), lFi=np.zeros(B ** 4 * 4 * nv),
+               lQhi=np.zeros(B ** 3 * nd), lFhi=np.zeros(4 * B ** 3 * nv))
+    it = lib().orc_space_time_predictor(C.byref(c), ptr(out["lQhbnd"]), ptr(out["lFhbnd"]), ptr(out["lQi"]),
+                                        ptr(out["lFi"]), ptr(out["lQhi"]), ptr(out["lFhi"]), ptr(luh),
+                                        ptr(dvec(dx)), dt)
+    out["iters"] = it
+    return out
+
+
+def volume_integral(c, lFhi, dx):
+    B = c.order + 1
+    lduh = np.zeros(B ** 3 * c.nvar)
+    lib().orc_volume_integral(C.byref(c), ptr(lduh), ptr(dvec(lFhi)), ptr(dvec(dx)))
+    return lduh
+
+
+def surface_integral(c, lduh, lFhbnd, dx):
+    lduh = dvec(lduh).copy()
+    lib().orc_surface_integral(C.byref(c), ptr(lduh), ptr(dvec(lFhbnd)), ptr(dvec(dx)))
+    return lduh
+
+
+def riemann_solver(c, FL, FR, QL, QR, dt, direction):
+    FL, FR = dvec(FL).copy(), dvec(FR).copy()
+    lib().orc_riemann_solver(C.byref(c), ptr(FL), ptr(FR), ptr(dvec(QL)), ptr(dvec(QR)), dt, direction)
+    return FL, FR
+
+
+def solution_update(c, luh, lduh, dt):
+    luh = dvec(luh).copy()
+    lib().orc_solution_update(C.byref(c), ptr(luh), ptr(dvec(lduh)), dt)
+    return luh
+
+
+def stable_time_step_size(c, luh, dx):
+    return lib().orc_stable_time_step_size(C.byref(c), ptr(dvec(luh)), ptr(dvec(dx)))
+
+
+def make_ic(kind=0, **kw):
+    ic = IC()
+    ic.kind = kind
+    for k, v in kw.items():
+        cur = getattr(ic, k)
+        if isinstance(cur, (int, float)):
+            setattr(ic, k, v)
+        elif k == "k":
+            for m, row in enumerate(v):
+                for j in range(3):
+                    ic.k[m][j] = row[j]
+        else:
+            for i, x in enumerate(v):
+                cur[i] = x
+    return ic
+
+
+def solution_adjustment(c, ic, centre, dx, t=0.0):
+    B = c.order + 1
+    luh = np.zeros(B ** 3 * (c.nvar + c.npar))
+    lib().orc_solution_adjustment(C.byref(c), C.byref(ic), ptr(luh), ptr(dvec(centre)), ptr(dvec(dx)), t)
+    return luh
+
+
+class OracleGrid:
+    """Periodic regular grid of ADER-DG cells driven by the oracle."""
+
+    def __init__(self, c, nx, ny, nz):
+        self.c = c
+        self.g = Grid()
+        if lib().orc_grid_alloc(C.byref(c), C.byref(self.g), nx, ny, nz) != 0:
+            raise MemoryError("oracle grid allocation failed")
+        self.ncells = nx * ny * nz
+        B = c.order + 1
+        self.cell_size = B ** 3 * (c.nvar + c.npar)
+
+    def __del__(self):
+        try:
+            lib().orc_grid_free(C.byref(self.g))
+        except Exception:
+            pass
+
+    def init(self, ic):
+        lib().orc_grid_init(C.byref(self.c), C.byref(self.g), C.byref(ic))
+
+    def luh(self):
+        n = self.ncells * self.cell_size
+        return np.ctypeslib.as_array(self.g.luh, shape=(n,))
+
+    def iters(self):
+        return np.ctypeslib.as_array(self.g.iters, shape=(self.ncells,))
+
+    def set_luh(self, a):
+        self.luh()[:] = np.asarray(a, dtype=np.float64).ravel()
+
+    def stable_dt(self, nthreads=0):
+        return lib().orc_grid_stable_dt(C.byref(self.c), C.byref(self.g), nthreads or os.cpu_count())
+
+    def step(self, dt, nthreads=0):
+        return lib().orc_grid_step(C.byref(self.c), C.byref(self.g), dt, nthreads or os.cpu_count())
+
+    def predict_cells(self, dt, first, count, nthreads=0):
+        lib().orc_grid_predict_cells(C.byref(self.c), C.byref(self.g), dt, first, count,
+                                     nthreads or os.cpu_count())

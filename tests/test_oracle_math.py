@@ -1,0 +1,217 @@
+"""Pin the oracle where the golden vectors leave it open.
+
+* Space-time predictor: for linear advection (F_d = a_d q) and initial data
+  that is a polynomial of total degree <= N, the exact solution q(x - a t) lies
+  in the space-time tensor space and the GL quadrature is exact, so the
+  converged predictor must reproduce it at every space-time node, together
+  with its time average and face values.  A transposed iK1/Kxi, a wrong sign,
+  a swapped direction or weight, or a face mix-up all break this.
+* Predictor on Euler: the result must satisfy the discrete space-time
+  fixed-point equation built independently from first-principles operators
+  (exahype_amd/tables/dg_math.py) with numpy einsum.
+* Whole scheme: conservation of the weighted cell sums on a periodic grid and
+  the convergence order on the isentropic vortex.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from exahype_amd.tables import dg_math
+
+
+def _poly_ic(rng, N):
+    # a few linear forms raised to powers <= N: total degree <= N
+    forms = [(rng.normal(size=3), rng.normal(), rng.integers(1, N + 1)) for _ in range(3)]
+
+    def u0(x):
+        x = np.asarray(x)
+        return sum(((x @ c) + b) ** p for c, b, p in forms) + 2.0
+    return u0
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5, 7])
+def test_predictor_exact_for_linear_advection(N):
+    rng = np.random.default_rng(100 + N)
+    a = np.array([0.3, -0.5, 0.7])
+    c = O.cfg(N, 5, 0, pde=1, adv=a)
+    B = N + 1
+    xi, w = dg_math.gl_nodes_weights(N)
+    centre = np.array([0.3, 0.55, 0.2])
+    dx = np.array([0.1, 0.07, 0.13])
+    dt = 0.05
+    u0 = _poly_ic(rng, N)
+    scale = np.arange(1, 6)
+    # node coordinates in reference order [z][y][x]
+    Z, Y, X = np.meshgrid(xi, xi, xi, indexing="ij")
+    pts = np.stack([centre[0] + dx[0] * (X - 0.5), centre[1] + dx[1] * (Y - 0.5),
+                    centre[2] + dx[2] * (Z - 0.5)], axis=-1)
+    luh = (u0(pts)[..., None] * scale).ravel()
+    out = O.space_time_predictor(c, luh, dx, dt)
+    assert out["iters"] <= 2 * B
+    lQi = out["lQi"].reshape(B, B, B, B, 5)
+    for t in range(B):
+        exact = u0(pts - a * xi[t] * dt)[..., None] * scale
+        np.testing.assert_allclose(lQi[:, :, :, t, :], exact, rtol=1e-9, atol=1e-9)
+    # time average and face values
+    tq, tw = dg_math.gl_nodes_weights(N + 4)
+    avg = sum(wt * u0(pts - a * tt * dt) for tt, wt in zip(tq, tw))[..., None] * scale
+    np.testing.assert_allclose(out["lQhi"].reshape(B, B, B, 5), avg, rtol=1e-9, atol=1e-9)
+    lQhbnd = out["lQhbnd"].reshape(6, B, B, 5)
+    lFhbnd = out["lFhbnd"].reshape(6, B, B, 5)
+    # x faces: (a,b)=(z,y); y faces: (z,x); z faces: (y,x)
+    qh = out["lQhi"].reshape(B, B, B, 5)
+    for d in range(3):
+        for side, s in ((0, 0.0), (1, 1.0)):
+            face = 2 * d + side
+            FLR = dg_math.operators(N)["FLCoeff" if side == 0 else "FRCoeff"]
+            if d == 0:
+                ref = np.einsum("zyxv,x->zyv", qh, FLR)
+            elif d == 1:
+                ref = np.einsum("zyxv,y->zxv", qh, FLR)
+            else:
+                ref = np.einsum("zyxv,z->yxv", qh, FLR)
+            np.testing.assert_allclose(lQhbnd[face], ref, rtol=1e-9, atol=1e-9)
+            np.testing.assert_allclose(lFhbnd[face], a[d] * ref, rtol=1e-9, atol=1e-9)
+
+
+def _euler_state(rng, B, nnodes):
+    rho = 1.0 + 0.2 * rng.random(nnodes)
+    u = 0.3 * rng.normal(size=(nnodes, 3))
+    p = 1.0 + 0.2 * rng.random(nnodes)
+    E = p / 0.4 + 0.5 * rho * (u ** 2).sum(1)
+    return np.column_stack([rho, rho * u[:, 0], rho * u[:, 1], rho * u[:, 2], E])
+
+
+@pytest.mark.parametrize("N", [2, 3, 5])
+def test_predictor_satisfies_discrete_fixed_point(N):
+    """lQi = (1/w^3) iK1 . (w^3 F0 luh - sum_d c_d Kxi^T F_d(lQi)) up to the Picard tolerance."""
+    rng = np.random.default_rng(7 + N)
+    B = N + 1
+    c = O.cfg(N, 5, 0)
+    luh = _euler_state(rng, B, B ** 3)
+    dx = np.array([0.05, 0.04, 0.06])
+    dt = 0.1 * O.stable_time_step_size(c, luh.ravel(), dx)
+    out = O.space_time_predictor(c, luh.ravel(), dx, dt)
+    assert out["iters"] < 2 * B  # converged before the cap
+    M = dg_math.operators(N)
+    w, K, iK1, F0 = M["gl_weights"], M["Kxi"], M["iK1"], M["F0"]
+    q = out["lQi"].reshape(B, B, B, B, 5)  # z y x t v
+    F = np.zeros((B, B, B, B, 3, 5))
+    for idx in np.ndindex(B, B, B, B):
+        F[idx] = O.flux(c, q[idx])
+    w3 = np.einsum("z,y,x->zyx", w, w, w)
+    rhs = np.einsum("zyx,t,zyxv->tzyxv", w3, F0, luh.reshape(B, B, B, 5))
+    rhs -= np.einsum("t,z,y,zyntv,nl->tzylv", w * dt / dx[0], w, w, F[..., 0, :], K)
+    rhs -= np.einsum("t,z,x,zlxtv,ly->tzyxv", w * dt / dx[1], w, w, F[..., 1, :], K)
+    rhs -= np.einsum("t,y,x,lyxtv,lz->tzyxv", w * dt / dx[2], w, w, F[..., 2, :], K)
+    q_next = np.einsum("lt,tzyxv->zyxlv", iK1, rhs) / w3[..., None, None]
+    # one more Picard step moves the converged iterate by less than the tolerance
+    assert np.sqrt(np.sum((q_next - q) ** 2)) < 1e-7 * 1.01
+
+
+def _vortex_ic(amp=0.0, seed=0):
+    rng = np.random.default_rng(seed)
+    k = rng.integers(1, 4, size=(8, 3)).astype(float)
+    return O.make_ic(0, xc=(0.5, 0.5, 0.5), L=0.1, eps=5.0, uinf=(1.0, 1.0, 0.0), rho_inf=1.0,
+                     p_inf=1.0, amp=amp, nmodes=8, k=k, a=rng.uniform(-1, 1, 8),
+                     phi=rng.uniform(0, 2 * np.pi, 8))
+
+
+def test_grid_conservation():
+    N = 3
+    c = O.cfg(N)
+    g = O.OracleGrid(c, 4, 3, 2)
+    g.init(_vortex_ic(amp=1e-2, seed=3))
+    B = N + 1
+    w = dg_math.gl_nodes_weights(N)[1]
+    w3 = np.einsum("z,y,x->zyx", w, w, w).ravel()
+
+    def totals():
+        u = g.luh().reshape(-1, B ** 3, 5)
+        return np.einsum("n,cnv->v", w3, u)
+    t0 = totals()
+    dt = g.stable_dt(2)
+    for _ in range(3):
+        dt = g.step(dt, 2)
+    t1 = totals()
+    np.testing.assert_allclose(t1, t0, rtol=0, atol=1e-12 * np.abs(t0).max())
+
+
+def _entropy_wave_ic():
+    return O.make_ic(4, uinf=(0.7, 0.4, 0.3), rho_inf=1.0, p_inf=1.0, amp=0.2, nmodes=2,
+                     k=[(1, 1, 0), (0, 1, 1)], a=(1.0, 0.5), phi=(0.3, 1.1))
+
+
+def _l2_error(N, n, nz, T, ic):
+    c = O.cfg(N)
+    g = O.OracleGrid(c, n, n, nz)
+    g.init(ic)
+    t = 0.0
+    dt = g.stable_dt()
+    while t < T - 1e-14:
+        dt = min(dt, T - t)
+        nxt = g.step(dt)
+        t += dt
+        dt = nxt
+    B = N + 1
+    xi, w = dg_math.gl_nodes_weights(N)
+    err = 0.0
+    u = g.luh().reshape(nz, n, n, B, B, B, 5)
+    h, hz = 1.0 / n, 1.0 / nz
+    for iz in range(nz):
+        for iy in range(n):
+            for ix in range(n):
+                centre = ((ix + 0.5) * h, (iy + 0.5) * h, (iz + 0.5) * hz)
+                ex = O.solution_adjustment(c, ic, centre, (h, h, hz), t).reshape(B, B, B, 5)
+                d = u[iz, iy, ix] - ex
+                err += np.einsum("z,y,x,zyx->", w, w, w, d[..., 0] ** 2) * h * h * hz
+    return np.sqrt(err)
+
+
+def _l2_error_1d(N, n, T, ic):
+    """Same as _l2_error on an n x 1 x 1 periodic grid (wave along x)."""
+    c = O.cfg(N)
+    g = O.OracleGrid(c, n, 1, 1)
+    g.init(ic)
+    t, dt = 0.0, g.stable_dt()
+    while t < T - 1e-14:
+        dt = min(dt, T - t)
+        nxt = g.step(dt)
+        t += dt
+        dt = nxt
+    B = N + 1
+    xi, w = dg_math.gl_nodes_weights(N)
+    u = g.luh().reshape(n, B, B, B, 5)
+    err, h = 0.0, 1.0 / n
+    for ix in range(n):
+        ex = O.solution_adjustment(c, ic, ((ix + 0.5) * h, 0.5, 0.5), (h, 1.0, 1.0), t).reshape(B, B, B, 5)
+        err += np.einsum("z,y,x,zyx->", w, w, w, (u[ix] - ex)[..., 0] ** 2) * h
+    return np.sqrt(err)
+
+
+@pytest.mark.parametrize("N,n", [(1, 16), (2, 16), (3, 8), (4, 8)])
+def test_entropy_wave_convergence_order(N, n):
+    """Smooth, exactly periodic Euler solution (density wave along x, u and p
+    constant, transverse velocity nonzero): order N+1 in h."""
+    ic = O.make_ic(4, uinf=(0.7, 0.4, 0.3), rho_inf=1.0, p_inf=1.0, amp=0.2, nmodes=1,
+                   k=[(1, 0, 0)], a=(1.0,), phi=(0.3,))
+    e1 = _l2_error_1d(N, n, 0.1, ic)
+    e2 = _l2_error_1d(N, 2 * n, 0.1, ic)
+    order = np.log2(e1 / e2)
+    assert order > N + 1 - 0.35, (e1, e2, order)
+
+
+def test_entropy_wave_diagonal_3d():
+    """Two oblique modes on a 3D grid at order 3."""
+    ic = _entropy_wave_ic()
+    e1 = _l2_error(3, 4, 4, 0.05, ic)
+    e2 = _l2_error(3, 8, 8, 0.05, ic)
+    assert np.log2(e1 / e2) > 3.7, (e1, e2)
+
+
+def test_vortex_convergence_order():
+    """Isentropic vortex (length 0.1, extruded in z) at order 3 on 16^2 and 32^2 cells."""
+    e1 = _l2_error(3, 16, 1, 0.02, _vortex_ic())
+    e2 = _l2_error(3, 32, 1, 0.02, _vortex_ic())
+    order = np.log2(e1 / e2)
+    assert order > 3.5, (e1, e2, order)
