@@ -1,0 +1,192 @@
+"""Benchmark: DoF-updates/s of 3D Euler ADER-DG at order 5 on one MI355X.
+
+Workload: 81^3 periodic cells on the unit cube, order N=5 (6^3 Gauss-Legendre
+nodes per cell, 5 variables), isentropic vortex plus 8 seeded density modes,
+CFL 0.9; one step = predictor (Picard + time averages + face extrapolation +
+volume integral), Rusanov on every face, surface integral + update + global
+CFL minimum, with inputs resident in HBM.
+
+Prints ONE JSON line (rank 0, N=1).  Fields beyond the driver contract:
+  roofline     dominant kernel (the predictor): algorithmic FLOP per launch
+               divided by its HIP-event-timed average launch duration
+  cpu_baseline the C oracle (OpenMP) timed on this host on a bounded sample
+               of the same workload
+  precisions   per-precision throughput and the L2 difference of the mixed
+               path against the FP64 path after the timed steps
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DoF-updates/s, 3D Euler ADER-DG N=5, mixed vs FP64 GPU at L2 err ≤1e-6"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+PHI_FLUX = 40   # flops of one 3D Euler flux evaluation (1 division counted as 1)
+NVAR = 5
+
+
+def predictor_flops_per_cell(N, iters):
+    """Algorithmic FLOP of one predictor launch per cell: I Picard sweeps of
+    8 V B^5 contraction flops + B^4 (flux + 5V), then time averages and the
+    volume integral (14 V B^4) and the face extrapolation (24 V B^3)."""
+    B = N + 1
+    return iters * (8 * NVAR * B ** 5 + B ** 4 * (PHI_FLUX + 5 * NVAR)) + 14 * NVAR * B ** 4 + 24 * NVAR * B ** 3
+
+
+def vortex_ic_kwargs(seed=0):
+    rng = np.random.default_rng(seed)
+    return dict(xc=(0.5, 0.5, 0.5), L=0.1, eps=5.0, uinf=(1.0, 1.0, 0.0), rho_inf=1.0, p_inf=1.0, amp=1e-2,
+                nmodes=8, k=rng.integers(1, 4, size=(8, 3)).astype(float), a=rng.uniform(-1, 1, 8),
+                phi=rng.uniform(0, 2 * np.pi, 8))
+
+
+def run_gpu(order, n, steps, warmup, precision, waves_per_simd=0):
+    import exahype_amd as E
+    s = E.ADERDGSolver(order, precision=precision, fp64_sweeps=1, waves_per_simd=waves_per_simd)
+    g = E.Grid(s, n, n, n)
+    g.set_initial_conditions(E.make_ic(0, **vortex_ic_kwargs()))
+    g.compute_dt()
+    g.step(warmup)
+    g.sync()
+    g.kernel_times()  # discard warm-up timings
+    g.total_picard_iterations()  # reset the device sweep counter
+    g.enable_timing(True)
+    E.device_synchronize()
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.sync()
+    E.device_synchronize()
+    wall = time.perf_counter() - t0
+    times, counts = g.kernel_times()
+    sweeps = g.total_picard_iterations() / (g.ncells * steps)
+    state = g.download()
+    g.close()
+    s.close()
+    return dict(wall=wall, times=times, counts=counts, iters=sweeps, state=state)
+
+
+def l2_per_variable(a, b, N, ncells):
+    """Domain L2 norm (GL quadrature per cell, unit cube) of a-b per variable."""
+    from exahype_amd.tables import dg_math
+    w = dg_math.gl_nodes_weights(N)[1]
+    w3 = np.einsum("z,y,x->zyx", w, w, w).ravel()
+    d = (a - b).reshape(ncells, (N + 1) ** 3, NVAR)
+    return np.sqrt(np.einsum("n,cnv->v", w3, d ** 2) / ncells)
+
+
+def cpu_baseline(order, n_full, threads, sample_cells=27, steps=1):
+    """Oracle (the C restatement, OpenMP) on a sample_cells^3 sub-box with the
+    same cell size, order and initial data; returns DoF-updates/s."""
+    from oracle import oracle as O
+    c = O.cfg(order)
+    og = O.OracleGrid(c, sample_cells, sample_cells, sample_cells)
+    for d in range(3):
+        og.g.width[d] = sample_cells / n_full
+    og.init(O.make_ic(0, **vortex_ic_kwargs()))
+    dt = og.stable_dt(threads)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dt = og.step(dt, threads)
+    wall = time.perf_counter() - t0
+    B = order + 1
+    return og.ncells * NVAR * B ** 3 * steps / wall, wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--order", type=int, default=5)
+    ap.add_argument("--cells", type=int, default=81)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--waves-per-simd", type=int, default=0, help="predictor occupancy target (0 = tuned)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the path runs on a single MI355X in this build "
+              "(multi-GPU sharding is designed, not built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import exahype_amd as E
+    N, n = args.order, args.cells
+    ncells = n ** 3
+    dof = ncells * NVAR * (N + 1) ** 3
+
+    r64 = run_gpu(N, n, args.steps, args.warmup, E.PREC_FP64, args.waves_per_simd)
+    rate64 = dof * args.steps / r64["wall"]
+    rmx = run_gpu(N, n, args.steps, args.warmup, E.PREC_FP32, args.waves_per_simd)
+    ratemx = dof * args.steps / rmx["wall"]
+    l2 = l2_per_variable(rmx["state"], r64["state"], N, ncells)
+    mixed_ok = bool(l2.max() <= 1e-6)
+    head = rmx if mixed_ok else r64
+    rate = ratemx if mixed_ok else rate64
+
+    B = N + 1
+    pred_ms = head["times"][0] / max(head["counts"][0], 1)
+    sweeps = head["iters"]
+    if mixed_ok:
+        # FP32 sweeps at the FP32 peak, one FP64 correction sweep and the FP64
+        # tail at the FP64 peak: peak = total flop / minimum time
+        f_sweep = 8 * NVAR * B ** 5 + B ** 4 * (PHI_FLUX + 5 * NVAR)
+        f32 = (sweeps - 1.0) * f_sweep * ncells
+        f64 = (predictor_flops_per_cell(N, sweeps) * ncells) - f32
+        flops = f32 + f64
+        peak = flops / (f32 / PEAK_FP32_TFLOPS + f64 / PEAK_FP64_TFLOPS)
+        kname = "k_predictor<%d, mixed FP32 + 1 FP64 sweep>" % N
+    else:
+        flops = predictor_flops_per_cell(N, sweeps) * ncells
+        peak = PEAK_FP64_TFLOPS
+        kname = "k_predictor<%d, FP64>" % N
+    achieved = flops / (pred_ms * 1e-3) / 1e12
+
+    out = {
+        "metric": METRIC,
+        "value": rate,
+        "unit": "DoF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["wall"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if mixed_ok else "f64",
+        "data": "synthetic",
+        "config": {"workload": "3D Euler ADER-DG, order %d, %d^3 periodic cells, vortex + 8 density modes, CFL 0.9"
+                               % (N, n), "cells": ncells, "order": N, "dof_per_step": dof,
+                   "predictor": "mixed: FP32 Picard sweeps + 1 FP64 correction sweep" if mixed_ok else "fp64",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": kname, "avg_launch_ms": pred_ms,
+                     "picard_sweeps_per_cell_step": sweeps, "flop_per_launch": flops},
+        "phase_ms_per_step": {"predictor": pred_ms,
+                              "riemann": head["times"][1] / max(head["counts"][1], 1),
+                              "update_surface_dt": head["times"][2] / max(head["counts"][2], 1)},
+        "precisions": {
+            "fp64": {"dof_updates_per_s": rate64, "ms_per_step": r64["wall"] / args.steps * 1e3,
+                     "predictor_ms": r64["times"][0] / max(r64["counts"][0], 1), "sweeps_per_cell": r64["iters"]},
+            "mixed": {"dof_updates_per_s": ratemx, "ms_per_step": rmx["wall"] / args.steps * 1e3,
+                      "predictor_ms": rmx["times"][0] / max(rmx["counts"][0], 1), "sweeps_per_cell": rmx["iters"],
+                      "l2_vs_fp64_per_variable": l2.tolist(), "speedup_vs_fp64": ratemx / rate64,
+                      "l2_ok": mixed_ok}},
+    }
+    if not args.no_cpu_baseline:
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        v, wall = cpu_baseline(N, n, threads, sample_cells=27, steps=2)
+        v1, wall1 = cpu_baseline(N, n, 1, sample_cells=9, steps=1)
+        out["cpu_baseline"] = {"value": v, "unit": "DoF-updates/s", "cores": threads, "kind": "port",
+                               "sample": "oracle, order %d, 27^3 cells of size 1/%d, 2 steps (%.1f s wall)" % (N, n, wall),
+                               "single_thread": {"value": v1, "sample": "9^3 cells, 1 step (%.1f s)" % wall1}}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

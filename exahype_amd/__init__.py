@@ -1,0 +1,292 @@
+"""exahype_amd: MI355X (gfx950) ADER-DG / finite-volume kernels.
+
+Python host side over the C ABI in include/exahype_amd.h.  The classes mirror
+the kernel surface of the reference's solver plugins:
+
+* ``ADERDGSolver``: spaceTimePredictor, volumeIntegral, surfaceIntegral,
+  riemannSolver, solutionUpdate, stableTimeStepSize, adjustSolution -- the
+  virtuals of exahype::solvers::ADERDGSolver (E/solvers/ADERDGSolver.h:994-1135),
+  same argument meaning and array layouts.
+* ``Grid``: a device-resident periodic mesh advanced by whole time steps.
+
+The HIP library is required: importing works everywhere, but creating a solver
+raises ``RuntimeError`` when the library is missing or no gfx950 GPU is usable.
+There is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libexahype_amd.so")
+
+EXA_OK = 0
+PREC_FP64 = 0
+PREC_FP32 = 1
+PDE_EULER = 0
+
+_D = C.POINTER(C.c_double)
+_lib = None
+
+
+class AderdgConfig(C.Structure):
+    _fields_ = [("order", C.c_int), ("nvar", C.c_int), ("npar", C.c_int), ("cfl", C.c_double),
+                ("gamma", C.c_double), ("pde", C.c_int), ("precision", C.c_int), ("fp64_sweeps", C.c_int),
+                ("waves_per_simd", C.c_int)]
+
+
+class InitialCondition(C.Structure):
+    """exa_ic: pointwise initial condition evaluated at the Gauss-Legendre nodes."""
+    _fields_ = [("kind", C.c_int), ("xc", C.c_double * 3), ("L", C.c_double), ("eps", C.c_double),
+                ("uinf", C.c_double * 3), ("rho_inf", C.c_double), ("p_inf", C.c_double),
+                ("amp", C.c_double), ("nmodes", C.c_int), ("k", (C.c_double * 3) * 8),
+                ("a", C.c_double * 8), ("phi", C.c_double * 8), ("r0", C.c_double),
+                ("rho_in", C.c_double), ("p_in", C.c_double), ("rho_out", C.c_double),
+                ("p_out", C.c_double), ("q0", C.c_double * 8)]
+
+
+# every symbol declared in include/exahype_amd.h
+EXPORTS = [
+    "exa_last_error", "exa_version", "exa_init", "exa_device_synchronize", "exa_aderdg_create", "exa_aderdg_destroy",
+    "exa_aderdg_space_time_predictor", "exa_aderdg_volume_integral", "exa_aderdg_surface_integral",
+    "exa_aderdg_riemann_solver", "exa_aderdg_solution_update", "exa_aderdg_stable_time_step_size",
+    "exa_aderdg_adjust_solution", "exa_grid_create", "exa_grid_destroy", "exa_grid_num_cells",
+    "exa_grid_upload", "exa_grid_download", "exa_grid_set_initial_conditions", "exa_grid_compute_dt",
+    "exa_grid_set_dt", "exa_grid_step", "exa_grid_sync", "exa_grid_picard_iterations",
+    "exa_grid_use_graph", "exa_grid_enable_timing", "exa_grid_kernel_times",
+    "exa_grid_total_picard_iterations",
+]
+
+
+def load_library(path=LIB_PATH):
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("exahype_amd HIP library not built: %s (run python -m exahype_amd.build)" % path)
+    L = C.CDLL(path)
+    P, I, LL = C.POINTER, C.c_int, C.c_longlong
+    L.exa_last_error.restype = C.c_char_p
+    L.exa_version.restype = C.c_char_p
+    L.exa_init.argtypes = [I]
+    L.exa_device_synchronize.argtypes = []
+    L.exa_aderdg_create.argtypes = [P(AderdgConfig), P(C.c_void_p)]
+    L.exa_aderdg_destroy.argtypes = [C.c_void_p]
+    L.exa_aderdg_space_time_predictor.argtypes = [C.c_void_p, _D, _D, _D, _D, _D, _D, _D, _D, C.c_double, P(I)]
+    L.exa_aderdg_volume_integral.argtypes = [C.c_void_p, _D, _D, _D]
+    L.exa_aderdg_surface_integral.argtypes = [C.c_void_p, _D, _D, _D]
+    L.exa_aderdg_riemann_solver.argtypes = [C.c_void_p, _D, _D, _D, _D, C.c_double, I]
+    L.exa_aderdg_solution_update.argtypes = [C.c_void_p, _D, _D, C.c_double]
+    L.exa_aderdg_stable_time_step_size.argtypes = [C.c_void_p, _D, _D, _D]
+    L.exa_aderdg_adjust_solution.argtypes = [C.c_void_p, _D, _D, _D, C.c_double, P(InitialCondition)]
+    L.exa_grid_create.argtypes = [C.c_void_p, I, I, I, _D, _D, P(C.c_void_p)]
+    L.exa_grid_destroy.argtypes = [C.c_void_p]
+    L.exa_grid_num_cells.argtypes = [C.c_void_p, P(LL)]
+    L.exa_grid_upload.argtypes = [C.c_void_p, _D]
+    L.exa_grid_download.argtypes = [C.c_void_p, _D]
+    L.exa_grid_set_initial_conditions.argtypes = [C.c_void_p, P(InitialCondition)]
+    L.exa_grid_compute_dt.argtypes = [C.c_void_p, _D]
+    L.exa_grid_set_dt.argtypes = [C.c_void_p, C.c_double]
+    L.exa_grid_step.argtypes = [C.c_void_p, I]
+    L.exa_grid_sync.argtypes = [C.c_void_p, _D, _D]
+    L.exa_grid_picard_iterations.argtypes = [C.c_void_p, P(I)]
+    L.exa_grid_use_graph.argtypes = [C.c_void_p, I]
+    L.exa_grid_enable_timing.argtypes = [C.c_void_p, I]
+    L.exa_grid_kernel_times.argtypes = [C.c_void_p, _D, P(LL)]
+    L.exa_grid_total_picard_iterations.argtypes = [C.c_void_p, P(LL)]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != EXA_OK:
+        raise RuntimeError("exahype_amd error %d: %s" % (rc, _lib.exa_last_error().decode()))
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous, "float64 C-contiguous arrays required"
+    return a.ctypes.data_as(_D)
+
+
+def _vec3(x):
+    return np.ascontiguousarray(np.broadcast_to(np.asarray(x, dtype=np.float64), (3,)))
+
+
+def device_synchronize():
+    """Wait for all queued GPU work of this process."""
+    _check(load_library().exa_device_synchronize())
+
+
+def make_ic(kind=0, **kw):
+    """Build an InitialCondition: kind 0 vortex+modes, 1 Sod sphere, 2 constant, 4 entropy wave."""
+    ic = InitialCondition()
+    ic.kind = kind
+    for k, v in kw.items():
+        cur = getattr(ic, k)
+        if isinstance(cur, (int, float)):
+            setattr(ic, k, v)
+        elif k == "k":
+            for m, row in enumerate(v):
+                for j in range(3):
+                    ic.k[m][j] = row[j]
+        else:
+            for i, x in enumerate(v):
+                cur[i] = x
+    return ic
+
+
+class ADERDGSolver:
+    """Per-cell kernel surface of exahype::solvers::ADERDGSolver on the GPU."""
+
+    def __init__(self, order, nvar=5, npar=0, cfl=0.9, gamma=1.4, precision=PREC_FP64, fp64_sweeps=1,
+                 waves_per_simd=0, device=0):
+        L = load_library()
+        _check(L.exa_init(device))
+        self.cfg = AderdgConfig(order, nvar, npar, cfl, gamma, PDE_EULER, precision, fp64_sweeps, waves_per_simd)
+        h = C.c_void_p()
+        _check(L.exa_aderdg_create(C.byref(self.cfg), C.byref(h)))
+        self._h = h
+        self.order, self.nvar, self.npar = order, nvar, npar
+        self.B = order + 1
+        self.ndata = nvar + npar
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.exa_aderdg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- virtual kernel entry points (reference names) ---
+    def spaceTimePredictor(self, luh, dx, dt):
+        B, nd, nv = self.B, self.ndata, self.nvar
+        luh = np.ascontiguousarray(luh, dtype=np.float64)
+        out = dict(lQhbnd=np.zeros(6 * B * B * nd), lFhbnd=np.zeros(6 * B * B * nv), lQi=np.zeros(B ** 4 * nd),
+                   lFi=np.zeros(B ** 4 * 4 * nv), lQhi=np.zeros(B ** 3 * nd), lFhi=np.zeros(4 * B ** 3 * nv))
+        it = C.c_int(0)
+        _check(_lib.exa_aderdg_space_time_predictor(self._h, _p(out["lQhbnd"]), _p(out["lFhbnd"]), _p(out["lQi"]),
+                                                    _p(out["lFi"]), _p(out["lQhi"]), _p(out["lFhi"]), _p(luh),
+                                                    _p(_vec3(dx)), float(dt), C.byref(it)))
+        out["iters"] = it.value
+        return out
+
+    def volumeIntegral(self, lFhi, dx):
+        lduh = np.zeros(self.B ** 3 * self.nvar)
+        _check(_lib.exa_aderdg_volume_integral(self._h, _p(lduh), _p(np.ascontiguousarray(lFhi, np.float64)),
+                                               _p(_vec3(dx))))
+        return lduh
+
+    def surfaceIntegral(self, lduh, lFhbnd, dx):
+        lduh = np.array(lduh, dtype=np.float64, copy=True)
+        _check(_lib.exa_aderdg_surface_integral(self._h, _p(lduh), _p(np.ascontiguousarray(lFhbnd, np.float64)),
+                                                _p(_vec3(dx))))
+        return lduh
+
+    def riemannSolver(self, FL, FR, QL, QR, dt, normalNonZero):
+        FL = np.array(FL, dtype=np.float64, copy=True)
+        FR = np.array(FR, dtype=np.float64, copy=True)
+        _check(_lib.exa_aderdg_riemann_solver(self._h, _p(FL), _p(FR), _p(np.ascontiguousarray(QL, np.float64)),
+                                              _p(np.ascontiguousarray(QR, np.float64)), float(dt),
+                                              int(normalNonZero)))
+        return FL, FR
+
+    def solutionUpdate(self, luh, lduh, dt):
+        luh = np.array(luh, dtype=np.float64, copy=True)
+        _check(_lib.exa_aderdg_solution_update(self._h, _p(luh), _p(np.ascontiguousarray(lduh, np.float64)),
+                                               float(dt)))
+        return luh
+
+    def stableTimeStepSize(self, luh, dx):
+        dt = C.c_double(0.0)
+        _check(_lib.exa_aderdg_stable_time_step_size(self._h, _p(np.ascontiguousarray(luh, np.float64)),
+                                                     _p(_vec3(dx)), C.byref(dt)))
+        return dt.value
+
+    def adjustSolution(self, centre, dx, t, ic):
+        luh = np.zeros(self.B ** 3 * self.ndata)
+        _check(_lib.exa_aderdg_adjust_solution(self._h, _p(luh), _p(_vec3(centre)), _p(_vec3(dx)), float(t),
+                                               C.byref(ic)))
+        return luh
+
+
+class Grid:
+    """Regular periodic mesh of ADER-DG cells resident in HBM."""
+
+    def __init__(self, solver, nx, ny, nz, offset=(0.0, 0.0, 0.0), width=(1.0, 1.0, 1.0)):
+        self.solver = solver
+        h = C.c_void_p()
+        _check(_lib.exa_grid_create(solver._h, nx, ny, nz, _p(_vec3(offset)), _p(_vec3(width)), C.byref(h)))
+        self._h = h
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.ncells = nx * ny * nz
+        self.cell_size = solver.B ** 3 * solver.nvar
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.exa_grid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, luh):
+        luh = np.ascontiguousarray(luh, dtype=np.float64).ravel()
+        assert luh.size == self.ncells * self.cell_size
+        _check(_lib.exa_grid_upload(self._h, _p(luh)))
+
+    def download(self):
+        out = np.zeros(self.ncells * self.cell_size)
+        _check(_lib.exa_grid_download(self._h, _p(out)))
+        return out
+
+    def set_initial_conditions(self, ic):
+        _check(_lib.exa_grid_set_initial_conditions(self._h, C.byref(ic)))
+
+    def compute_dt(self):
+        dt = C.c_double(0.0)
+        _check(_lib.exa_grid_compute_dt(self._h, C.byref(dt)))
+        return dt.value
+
+    def set_dt(self, dt):
+        _check(_lib.exa_grid_set_dt(self._h, float(dt)))
+
+    def step(self, nsteps=1):
+        _check(_lib.exa_grid_step(self._h, int(nsteps)))
+
+    def sync(self):
+        t, dt = C.c_double(0.0), C.c_double(0.0)
+        _check(_lib.exa_grid_sync(self._h, C.byref(t), C.byref(dt)))
+        return t.value, dt.value
+
+    def picard_iterations(self):
+        it = np.zeros(self.ncells, dtype=np.int32)
+        _check(_lib.exa_grid_picard_iterations(self._h, it.ctypes.data_as(C.POINTER(C.c_int))))
+        return it
+
+    def total_picard_iterations(self):
+        t = C.c_longlong(0)
+        _check(_lib.exa_grid_total_picard_iterations(self._h, C.byref(t)))
+        return t.value
+
+    def use_graph(self, enable=True):
+        _check(_lib.exa_grid_use_graph(self._h, 1 if enable else 0))
+
+    def enable_timing(self, enable=True):
+        _check(_lib.exa_grid_enable_timing(self._h, 1 if enable else 0))
+
+    def kernel_times(self):
+        t = np.zeros(3)
+        c = np.zeros(3, dtype=np.int64)
+        _check(_lib.exa_grid_kernel_times(self._h, _p(t), c.ctypes.data_as(C.POINTER(C.c_longlong))))
+        return t, c

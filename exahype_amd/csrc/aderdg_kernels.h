@@ -1,0 +1,560 @@
+// ADER-DG kernels for gfx950 (3D, nonlinear, conservative flux only).
+//
+// Device layout of a cell (structure of arrays inside the cell block, cells
+// contiguous in HBM):
+//   luh, lduh : [cell][v][node]            node = (z*B + y)*B + x
+//   qf, ff    : [cell][face][v][a*B + b]   face order x-,x+,y-,y+,z-,z+,
+//                                          (a,b) = (z,y) | (z,x) | (y,x)
+// One workgroup owns one cell; thread s owns spatial node s.  All summation
+// orders and expression groupings follow the reference generic kernels so
+// the FP64 results agree with them to rounding.
+#pragma once
+#include "common.h"
+#include "pde_euler.h"
+#include <type_traits>
+
+namespace exa {
+
+constexpr int V = 5;
+
+template <int N>
+struct Ord {
+  static constexpr int B = N + 1, B2 = B * B, B3 = B2 * B, B4 = B2 * B2;
+  static constexpr int NT = round_up64(B3);          // threads per cell
+  static constexpr int LDS_DOUBLES = 6 * V * B3 + 16;  // flux slice + partials / averages + reduction scratch
+};
+
+// ---------------------------------------------------------------------------
+// Volume integral and face extrapolation from the time-averaged predictor held
+// in LDS: sQh[v][node], sFh[d][v][node] (d = x,y,z) -- shared by the fused
+// predictor and the stand-alone volume-integral entry point.
+// volumeIntegralNonlinear.cpph:26-125, aderExtrapolatorNonlinear :400-519.
+template <int N>
+__device__ inline void volume_integral_node(const double* sFh, int s, int z, int y, int x,
+                                            double idx0, double idx1, double idx2, double out[V]) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
+  const DGTables& T = c_tab[N];
+  const double usx = (T.w[z] * T.w[y]) / idx0;
+  const double usy = (T.w[z] * T.w[x]) / idx1;
+  const double usz = (T.w[y] * T.w[x]) / idx2;
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    double acc = 0.0;
+#pragma unroll
+    for (int m = 0; m < B; ++m) acc += (T.Kxi[x][m] * sFh[(0 * V + v) * B3 + z * B2 + y * B + m]) * usx;
+#pragma unroll
+    for (int m = 0; m < B; ++m) acc += (T.Kxi[y][m] * sFh[(1 * V + v) * B3 + z * B2 + m * B + x]) * usy;
+#pragma unroll
+    for (int m = 0; m < B; ++m) acc += (T.Kxi[z][m] * sFh[(2 * V + v) * B3 + m * B2 + y * B + x]) * usz;
+    out[v] = acc;
+  }
+}
+
+template <int N>
+__device__ inline void extrapolate_faces(const double* sQh, const double* sFh, int s, double* qf_cell,
+                                         double* ff_cell) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
+  if (s >= 3 * B2) return;
+  const DGTables& T = c_tab[N];
+  const int d = s / B2, ab = s % B2, a = ab / B, b = ab % B;
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    double qL = 0.0, qR = 0.0, fL = 0.0, fR = 0.0;
+#pragma unroll
+    for (int l = 0; l < B; ++l) {
+      const int idx = d == 0 ? a * B2 + b * B + l : (d == 1 ? a * B2 + l * B + b : l * B2 + a * B + b);
+      const double qv = sQh[v * B3 + idx];
+      const double fv = sFh[(d * V + v) * B3 + idx];
+      fL += fv * T.FL[l];
+      fR += fv * T.FR[l];
+      qL += qv * T.FL[l];
+      qR += qv * T.FR[l];
+    }
+    qf_cell[((2 * d) * V + v) * B2 + ab] = qL;
+    qf_cell[((2 * d + 1) * V + v) * B2 + ab] = qR;
+    ff_cell[((2 * d) * V + v) * B2 + ab] = fL;
+    ff_cell[((2 * d + 1) * V + v) * B2 + ab] = fR;
+  }
+}
+
+// Debug outputs of the per-cell entry point, reference layouts with nVar=5.
+struct PredictorDebug {
+  double* lQi;   // [z][y][x][t][V]
+  double* lFi;   // [t][z][y][x][4][V]
+  double* lQhi;  // [z][y][x][V]
+  double* lFhi;  // 4 blocks of B^3*V
+};
+
+// ---------------------------------------------------------------------------
+// Picard sweeps of the space-time predictor for one cell held by one
+// workgroup (aderPicardLoopNonlinear, spaceTimePredictorNonlinear.cpph:43-317).
+//
+// Thread s owns spatial node s and keeps its whole time history q[t][v] in
+// registers.  Per time slice t, three phases separated by barriers:
+//   F  node owners evaluate the flux and store F_d in direction-contiguous
+//      layouts (x: [z][y][x], y: [z][x][y], z: [y][x][z]), so every pencil is
+//      one contiguous run of B values;
+//   P  pencil owners (3*V*B^2 items: direction, variable, pencil) read their
+//      pencil once and produce all B outputs of the contraction with Kxi,
+//      whose entries are uniform across the workgroup (scalar operands);
+//   A  node owners combine rhs = w^3 F0[t] u - Px - Py - Pz and accumulate the
+//      thread-local time contraction qn[t'] += (rhs / w^3) iK1[t'][t].
+// T is the arithmetic type of the sweep: double for the plain method and the
+// correction sweeps, float for the low-precision sweeps of the mixed
+// predictor.  Returns the sweeps executed; stops once
+// sum((q_new-q_old)^2) < tol2 over the cell (block-uniform decision).
+template <int N, typename T, bool ACC_FH, bool DEBUG>
+__device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], T* sF, T* sP, double* red,
+                                    bool act, int s, int z, int y, int x, double dt, double dx0, double dx1,
+                                    double dx2, double gamma, int max_sweeps, double tol2, double (&fh)[3][V],
+                                    PredictorDebug dbg) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
+  constexpr int NITEMS = 3 * V * B2;
+  constexpr bool DP = std::is_same<T, double>::value;
+  const DGTables& Td = c_tab[N];
+  const DGTablesF& Tf = c_tabf[N];
+  auto W = [&](int i) -> T { return DP ? (T)Td.w[i] : (T)Tf.w[i]; };
+  auto KXI = [&](int a, int b) -> T { return DP ? (T)Td.Kxi[a][b] : (T)Tf.Kxi[a][b]; };
+  auto IK1 = [&](int a, int b) -> T { return DP ? (T)Td.iK1[a][b] : (T)Tf.iK1[a][b]; };
+  auto F0 = [&](int i) -> T { return DP ? (T)Td.F0[i] : (T)Tf.F0[i]; };
+
+  T uu[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) uu[v] = (T)u[v];
+  const T w3 = W(z) * W(y) * W(x);
+  const T iw3 = T(1.0) / w3;
+  const T tdt = (T)dt, tg = (T)gamma;
+  const T tdx[3] = {(T)dx0, (T)dx1, (T)dx2};
+  // flux store offsets of this node in the three direction layouts
+  const int ox = (z * B + y) * B + x, oy = (z * B + x) * B + y, oz = (y * B + x) * B + z;
+
+  int it = 0;
+  while (it < max_sweeps) {
+    ++it;
+    T qn[B][V];
+#pragma unroll
+    for (int t = 0; t < B; ++t)
+#pragma unroll
+      for (int v = 0; v < V; ++v) qn[t][v] = T(0);
+#pragma unroll
+    for (int t = 0; t < B; ++t) {
+      // phase F
+      if (act) {
+        T F[3][V];
+        EulerPDE::flux<T>(q[t], F, tg);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          sF[(0 * V + v) * B3 + ox] = F[0][v];
+          sF[(1 * V + v) * B3 + oy] = F[1][v];
+          sF[(2 * V + v) * B3 + oz] = F[2][v];
+        }
+      }
+      __syncthreads();
+      // phase P
+      const T wt = W(t);
+      for (int i = s; i < NITEMS; i += O::NT) {
+        const int d = i / (V * B2), v = (i / B2) % V, ab = i % B2, a = ab / B, b = ab % B;
+        const T c = ((wt * W(a)) * W(b)) * tdt / tdx[d];
+        const T* pf = sF + (d * V + v) * B3 + ab * B;
+        T f[B];
+#pragma unroll
+        for (int n = 0; n < B; ++n) f[n] = c * pf[n];
+        T* pp = sP + (d * V + v) * B3;
+#pragma unroll
+        for (int l = 0; l < B; ++l) {
+          T acc = T(0);
+#pragma unroll
+          for (int n = 0; n < B; ++n) acc += f[n] * KXI(n, l);
+          const int node = d == 0 ? ab * B + l : (d == 1 ? (a * B + l) * B + b : (l * B + a) * B + b);
+          pp[node] = acc;
+        }
+      }
+      __syncthreads();
+      // phase A
+      if (act) {
+        const T c0 = w3 * F0(t);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const T r = ((c0 * uu[v] - sP[(0 * V + v) * B3 + s]) - sP[(1 * V + v) * B3 + s]) - sP[(2 * V + v) * B3 + s];
+          const T ri = iw3 * r;
+#pragma unroll
+          for (int t2 = 0; t2 < B; ++t2) qn[t2][v] += ri * IK1(t2, t);
+        }
+      }
+    }
+    double res = 0.0;
+    if (act) {
+      T rt = T(0);
+#pragma unroll
+      for (int t = 0; t < B; ++t)
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const T dd = q[t][v] - qn[t][v];
+          rt += dd * dd;
+        }
+      res = (double)rt;
+    }
+    res = block_sum(res, red);
+    const bool done = res < tol2 || it == max_sweeps;
+    if (done && ACC_FH && act) {
+      // time-averaged fluxes of this last sweep (aderPredictorNonlinear
+      // :326-397): the fluxes of its input iterate q, summed over t in order
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int v = 0; v < V; ++v) fh[d][v] = 0.0;
+#pragma unroll
+      for (int t = 0; t < B; ++t) {
+        T F[3][V];
+        EulerPDE::flux<T>(q[t], F, tg);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            fh[d][v] += (double)F[d][v] * Td.w[t];
+            if (DEBUG) dbg.lFi[((t * B3 + s) * 4 + d) * V + v] = (double)F[d][v];
+          }
+        if (DEBUG)
+#pragma unroll
+          for (int v = 0; v < V; ++v) dbg.lFi[((t * B3 + s) * 4 + 3) * V + v] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < B; ++t)
+#pragma unroll
+      for (int v = 0; v < V; ++v) q[t][v] = qn[t][v];
+    if (done) break;
+  }
+  return it;
+}
+
+// ---------------------------------------------------------------------------
+// Space-time predictor + time averages + face extrapolation + volume
+// integral.  spaceTimePredictorNonlinear.cpph:43-577 and
+// volumeIntegralNonlinear.cpph:26-125 fused into one pass over the cell.
+// MIXED=false: the plain FP64 method (Picard to sum(dq^2) < 1e-14 or 2(N+1)
+// sweeps, as the reference).  MIXED=true: FP32 sweeps until the residual
+// reaches the FP32 rounding floor of the cell, then `fp64_sweeps` FP64
+// correction sweeps (stopping early on the FP64 criterion); the averages,
+// extrapolation and volume integral are FP64 from the last FP64 sweep.
+template <int N, bool DEBUG, bool MIXED, int WPE>
+__global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_predictor(
+    const double* __restrict__ luh, double* __restrict__ lduh, double* __restrict__ qf,
+    double* __restrict__ ff, int* __restrict__ iters, const double* __restrict__ dtp, double dx0,
+    double dx1, double dx2, double gamma, int fp64_sweeps, unsigned long long* __restrict__ sweep_count,
+    PredictorDebug dbg) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
+  extern __shared__ double sm[];
+  double* red = sm + 6 * V * B3;   // reduction scratch
+  const DGTables& T = c_tab[N];
+
+  const size_t cell = blockIdx.x;
+  const int s = threadIdx.x;
+  const bool act = s < B3;
+  const int z = act ? s / B2 : 0, y = act ? (s / B) % B : 0, x = act ? s % B : 0;
+  const double dt = *dtp;
+
+  double u[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) u[v] = act ? luh[cell * V * B3 + v * B3 + s] : 1.0;
+
+  double q[B][V];
+  double fh[3][V];
+  constexpr int MaxIterations = 2 * (N + 1);
+  int it = 0;
+  if (MIXED) {
+    float qs[B][V];
+#pragma unroll
+    for (int t = 0; t < B; ++t)
+#pragma unroll
+      for (int v = 0; v < V; ++v) qs[t][v] = (float)u[v];
+    // FP32 rounding floor of sum((dq)^2) over the cell: count * (4 eps_f * |u|max)^2
+    double um = 0.0;
+#pragma unroll
+    for (int v = 0; v < V; ++v) um = fmax(um, fabs(u[v]));
+    um = -block_min(-um, red);
+    const double floor_lp = (double)(V * O::B4) * (4.0 * 5.96e-8 * um) * (4.0 * 5.96e-8 * um);
+    const double tol_lp = fmax(1e-7 * 1e-7, floor_lp);
+    it = picard_sweeps<N, float, false, false>(qs, u, reinterpret_cast<float*>(sm),
+                                               reinterpret_cast<float*>(sm + 3 * V * B3), red, act, s, z, y, x, dt,
+                                               dx0, dx1, dx2, gamma, MaxIterations, tol_lp, fh, dbg);
+#pragma unroll
+    for (int t = 0; t < B; ++t)
+#pragma unroll
+      for (int v = 0; v < V; ++v) q[t][v] = (double)qs[t][v];
+    it += picard_sweeps<N, double, true, false>(q, u, sm, sm + 3 * V * B3, red, act, s, z, y, x, dt, dx0, dx1, dx2,
+                                                gamma, fp64_sweeps, 1e-7 * 1e-7, fh, dbg);
+  } else {
+#pragma unroll
+    for (int t = 0; t < B; ++t)
+#pragma unroll
+      for (int v = 0; v < V; ++v) q[t][v] = u[v];
+    it = picard_sweeps<N, double, true, DEBUG>(q, u, sm, sm + 3 * V * B3, red, act, s, z, y, x, dt, dx0, dx1, dx2,
+                                               gamma, MaxIterations, 1e-7 * 1e-7, fh, dbg);
+  }
+
+  // time averages (aderPredictorNonlinear :326-397)
+  double* sQh = sm;
+  double* sFh = sm + V * B3;
+  if (act) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      double a = 0.0;
+#pragma unroll
+      for (int t = 0; t < B; ++t) a += q[t][v] * T.w[t];
+      sQh[v * B3 + s] = a;
+      if (DEBUG) {
+        dbg.lQhi[s * V + v] = a;
+#pragma unroll
+        for (int t = 0; t < B; ++t) dbg.lQi[(s * B + t) * V + v] = q[t][v];
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int v = 0; v < V; ++v) sFh[(d * V + v) * B3 + s] = fh[d][v];
+    if (DEBUG) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        dbg.lFhi[0 * B3 * V + ((z * B + y) * B + x) * V + v] = fh[0][v];
+        dbg.lFhi[1 * B3 * V + ((z * B + x) * B + y) * V + v] = fh[1][v];
+        dbg.lFhi[2 * B3 * V + ((y * B + x) * B + z) * V + v] = fh[2][v];
+        dbg.lFhi[3 * B3 * V + s * V + v] = 0.0;
+      }
+    }
+  }
+  __syncthreads();
+  if (act) {
+    double out[V];
+    volume_integral_node<N>(sFh, s, z, y, x, dx0, dx1, dx2, out);
+#pragma unroll
+    for (int v = 0; v < V; ++v) lduh[cell * V * B3 + v * B3 + s] = out[v];
+  }
+  extrapolate_faces<N>(sQh, sFh, s, qf + cell * 6 * V * B2, ff + cell * 6 * V * B2);
+  if (s == 0) {
+    if (iters) iters[cell] = it;
+    if (sweep_count) atomicAdd(sweep_count, (unsigned long long)it);
+  }
+}
+
+// Stand-alone volume integral from lFhi in the reference layout.
+template <int N>
+__global__ __launch_bounds__(Ord<N>::NT) void k_volume_integral_ref(const double* __restrict__ lFhi,
+                                                                    double* __restrict__ lduh, double dx0,
+                                                                    double dx1, double dx2) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B3 = O::B3;
+  extern __shared__ double sm[];
+  const int s = threadIdx.x;
+  const bool act = s < B3;
+  const int z = act ? s / O::B2 : 0, y = act ? (s / B) % B : 0, x = act ? s % B : 0;
+  double* sFh = sm + V * B3;
+  if (act) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      sFh[(0 * V + v) * B3 + s] = lFhi[0 * B3 * V + ((z * B + y) * B + x) * V + v];
+      sFh[(1 * V + v) * B3 + s] = lFhi[1 * B3 * V + ((z * B + x) * B + y) * V + v];
+      sFh[(2 * V + v) * B3 + s] = lFhi[2 * B3 * V + ((y * B + x) * B + z) * V + v];
+    }
+  }
+  __syncthreads();
+  if (act) {
+    double out[V];
+    volume_integral_node<N>(sFh, s, z, y, x, dx0, dx1, dx2, out);
+#pragma unroll
+    for (int v = 0; v < V; ++v) lduh[v * B3 + s] = out[v];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Rusanov flux on every face: one wavefront per face.
+// riemannSolverNonlinear.cpph:38-120; caller solveRiemannProblemAtInterface
+// (E/solvers/ADERDGSolver.cpp:2744-2825): left cell face 2d+1, right cell face 2d.
+// nbr[d*ncells + c] = neighbour of cell c in +d direction.
+template <int N>
+__global__ __launch_bounds__(256) void k_riemann(const double* __restrict__ qf, double* __restrict__ ff,
+                                                 const int* __restrict__ nbr, int ncells, double gamma,
+                                                 long long wave0, long long nwaves) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2;
+  constexpr int NPL = (B2 + 63) / 64;
+  const DGTables& T = c_tab[N];
+  const long long local = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (local >= nwaves) return;
+  const long long wave = wave0 + local;
+  const int d = (int)(wave / ncells);
+  const int cl = (int)(wave % ncells);
+  const int cr = nbr[(size_t)d * ncells + cl];
+  const double* QL = qf + ((size_t)cl * 6 + 2 * d + 1) * V * B2;
+  const double* QR = qf + ((size_t)cr * 6 + 2 * d) * V * B2;
+  double* FL = ff + ((size_t)cl * 6 + 2 * d + 1) * V * B2;
+  double* FR = ff + ((size_t)cr * 6 + 2 * d) * V * B2;
+
+  double aL[V], aR[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) aL[v] = aR[v] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NPL; ++k) {
+    const int node = lane + 64 * k;
+    if (node < B2) {
+      const double w = T.w[node / B] * T.w[node % B];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        aL[v] += w * QL[v * B2 + node];
+        aR[v] += w * QR[v * B2 + node];
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    aL[v] = wave_sum(aL[v]);
+    aR[v] = wave_sum(aR[v]);
+  }
+  const double sL = EulerPDE::max_abs_eigenvalue(aL, d, gamma);
+  const double sR = EulerPDE::max_abs_eigenvalue(aR, d, gamma);
+  const double smax = sL < sR ? sR : sL;
+#pragma unroll
+  for (int k = 0; k < NPL; ++k) {
+    const int node = lane + 64 * k;
+    if (node < B2) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const int i = v * B2 + node;
+        const double F = 0.5 * (FR[i] + FL[i]) - 0.5 * smax * (QR[i] - QL[i]);
+        FL[i] = F;
+        FR[i] = F;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Surface integral + solution update + per-node CFL dt with a block minimum
+// and one atomic per cell.  surfaceIntegralNonlinear.cpph:21-86,
+// solutionUpdate.cpph:23-47, stableTimeStepSize.cpph:21-61.
+enum : int { UPD_SURFACE = 1, UPD_UPDATE = 2, UPD_DT = 4, UPD_WRITE_LDUH = 8 };
+
+template <int N, int MODE>
+__global__ __launch_bounds__(Ord<N>::NT) void k_update(double* __restrict__ luh, double* __restrict__ lduh,
+                                                       const double* __restrict__ ff,
+                                                       const double* __restrict__ dtp,
+                                                       unsigned long long* __restrict__ dt_next, double dx0,
+                                                       double dx1, double dx2, double gamma, double cfl) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
+  __shared__ double red[16];
+  const DGTables& T = c_tab[N];
+  const size_t cell = blockIdx.x;
+  const int s = threadIdx.x;
+  const bool act = s < B3;
+  const int z = act ? s / B2 : 0, y = act ? (s / B) % B : 0, x = act ? s % B : 0;
+  double dtn = 1.0e300;
+  if (act) {
+    double du[V], u[V];
+    if (MODE & (UPD_SURFACE | UPD_UPDATE)) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) du[v] = lduh[cell * V * B3 + v * B3 + s];
+    }
+    if (MODE & UPD_SURFACE) {
+      const double* F = ff + cell * 6 * V * B2;
+      const double usx = (T.w[z] * T.w[y]) / dx0;
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        du[v] -= (F[(1 * V + v) * B2 + z * B + y] * T.FR[x] - F[(0 * V + v) * B2 + z * B + y] * T.FL[x]) * usx;
+      const double usy = (T.w[z] * T.w[x]) / dx1;
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        du[v] -= (F[(3 * V + v) * B2 + z * B + x] * T.FR[y] - F[(2 * V + v) * B2 + z * B + x] * T.FL[y]) * usy;
+      const double usz = (T.w[y] * T.w[x]) / dx2;
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        du[v] -= (F[(5 * V + v) * B2 + y * B + x] * T.FR[z] - F[(4 * V + v) * B2 + y * B + x] * T.FL[z]) * usz;
+    }
+    if (MODE & UPD_WRITE_LDUH) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) lduh[cell * V * B3 + v * B3 + s] = du[v];
+    }
+    if (MODE & (UPD_UPDATE | UPD_DT)) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) u[v] = luh[cell * V * B3 + v * B3 + s];
+    }
+    if (MODE & UPD_UPDATE) {
+      const double weight = T.w[z] * T.w[y] * T.w[x];
+      const double updateSize = *dtp / weight;
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        u[v] += du[v] * updateSize;
+        luh[cell * V * B3 + v * B3 + s] = u[v];
+      }
+    }
+    if (MODE & UPD_DT) {
+      double den = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double m = EulerPDE::max_abs_eigenvalue(u, d, gamma);
+        den += m / (d == 0 ? dx0 : (d == 1 ? dx1 : dx2));
+      }
+      dtn = cfl * exa_pnpm(N) / den;
+    }
+  }
+  if (MODE & UPD_DT) {
+    dtn = block_min(dtn, red);
+    if (s == 0) atomic_min_pos_double(dt_next, dtn);
+  }
+}
+
+// Initial conditions at the GL nodes (solutionAdjustment.cpph:17-52).
+template <int N>
+__global__ __launch_bounds__(Ord<N>::NT) void k_initial_conditions(double* __restrict__ luh, exa_ic ic,
+                                                                   double gamma, int nx, int ny, double ox,
+                                                                   double oy, double oz, double dx0,
+                                                                   double dx1, double dx2) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
+  const DGTables& T = c_tab[N];
+  const size_t cell = blockIdx.x;
+  const int s = threadIdx.x;
+  if (s >= B3) return;
+  const int z = s / B2, y = (s / B) % B, x = s % B;
+  const int ix = (int)(cell % nx), iy = (int)((cell / nx) % ny), iz = (int)(cell / ((size_t)nx * ny));
+  const double c0 = ox + (ix + 0.5) * dx0, c1 = oy + (iy + 0.5) * dx1, c2 = oz + (iz + 0.5) * dx2;
+  double X[3];
+  X[2] = c2 + dx2 * (T.xi[z] - 0.5);
+  X[1] = c1 + dx1 * (T.xi[y] - 0.5);
+  X[0] = c0 + dx0 * (T.xi[x] - 0.5);
+  double Q[V];
+  exa_ic_point(ic, gamma, X, 0.0, Q);
+#pragma unroll
+  for (int v = 0; v < V; ++v) luh[cell * V * B3 + v * B3 + s] = Q[v];
+}
+
+// dt bookkeeping between steps: dt_cur <- min over cells, time += dt used.
+__global__ void k_dt_advance(double* dt_cur, unsigned long long* dt_next, double* time) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *time += *dt_cur;
+    *dt_cur = __longlong_as_double((long long)*dt_next);
+    *dt_next = (unsigned long long)__double_as_longlong(1.0e300);
+  }
+}
+
+__global__ void k_dt_reset(unsigned long long* dt_next) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dt_next = (unsigned long long)__double_as_longlong(1.0e300);
+}
+
+__global__ void k_dt_take(double* dt_cur, const unsigned long long* dt_next) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dt_cur = __longlong_as_double((long long)*dt_next);
+}
+
+__global__ void k_sum_iters(const int* __restrict__ iters, int n, unsigned long long* out) {
+  unsigned long long acc = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += iters[i];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+}
+
+}  // namespace exa

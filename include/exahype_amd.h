@@ -1,0 +1,176 @@
+/*
+ * exahype_amd -- MI355X (gfx950) ADER-DG and finite-volume kernels behind a
+ * C ABI.
+ *
+ * Two layers:
+ *
+ *  1. Per-cell / per-face drop-ins for the kernel entry points of the
+ *     reference's solver plugin surface.  Arrays are host pointers in the
+ *     reference's own layouts (row-major, variable index fastest, nData =
+ *     nVar + nPar entries per node for state arrays, nVar for fluxes):
+ *       luh    [z][y][x][nData]            lduh [z][y][x][nVar]
+ *       lQhbnd [face][a][b][nData]         lFhbnd [face][a][b][nVar]
+ *       face order x-,x+,y-,y+,z-,z+; (a,b) = (z,y), (z,x), (y,x)
+ *       lQi [z][y][x][t][nData]            lFi [t][z][y][x][4][nVar]
+ *       lQhi [z][y][x][nData]              lFhi = x|y|z|source blocks of B^3*nVar,
+ *                                          x:[z][y][x], y:[z][x][y], z:[y][x][z]
+ *     Each call is synchronous and re-entrant (one stream per solver handle,
+ *     serialised by a mutex).
+ *
+ *  2. A device-resident grid: cells of a regular periodic mesh kept in HBM in
+ *     structure-of-arrays blocks, advanced by whole time steps (predictor,
+ *     Riemann solves, surface integral + update + CFL reduction) without host
+ *     round trips.  The step is the reference's non-fused stepping
+ *     (E/runners/Runner.cpp:1125-1190).
+ *
+ * Every entry point returns EXA_OK (0) or a negative status; exa_last_error()
+ * gives the message of the last failure on the calling thread.  No entry
+ * point falls back to a CPU implementation: without a usable gfx950 device
+ * they return EXA_ERR_HIP.
+ */
+#ifndef EXAHYPE_AMD_H
+#define EXAHYPE_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EXA_OK 0
+#define EXA_ERR_ARG (-1)
+#define EXA_ERR_HIP (-2)
+#define EXA_ERR_UNSUPPORTED (-3)
+#define EXA_ERR_NOMEM (-4)
+
+/* predictor precision */
+#define EXA_PREC_FP64 0   /* plain method: every operation in FP64             */
+#define EXA_PREC_FP32 1   /* Picard sweeps in FP32, then FP64 correction sweeps */
+
+/* PDEs compiled into the kernels */
+#define EXA_PDE_EULER 0   /* 3D compressible Euler, 5 variables (gamma from config) */
+
+const char* exa_last_error(void);
+
+/* Version string of the library build. */
+const char* exa_version(void);
+
+/* Selects the device for the calling thread and uploads the DG tables
+ * (idempotent per device). */
+int exa_init(int device);
+
+/* Waits for all work on the current device (every stream). */
+int exa_device_synchronize(void);
+
+/* ---------------------------------------------------------------------- */
+/* Solver handle: the compile-time constants of the reference's generated
+ * Abstract<Solver> (TT/AbstractGenericADERDGSolverHeader.template:31-34). */
+typedef struct exa_solver exa_solver;
+
+typedef struct {
+  int order;        /* N, 1..7                                     */
+  int nvar;         /* must be 5 for EXA_PDE_EULER                 */
+  int npar;         /* material parameters (passed through)        */
+  double cfl;       /* CFL factor (0.9 in the reference)           */
+  double gamma;     /* ratio of specific heats                     */
+  int pde;          /* EXA_PDE_*                                   */
+  int precision;    /* EXA_PREC_*                                  */
+  int fp64_sweeps;  /* EXA_PREC_FP32: FP64 Picard sweeps after the FP32 phase (>=1) */
+  int waves_per_simd; /* predictor occupancy target: 0 = tuned default per order, 1 or 2 */
+} exa_aderdg_config;
+
+int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out);
+int exa_aderdg_destroy(exa_solver* s);
+
+/* ADERDGSolver::spaceTimePredictor (E/solvers/ADERDGSolver.h:1049-1059;
+ * kernel kernels/aderdg/generic/c/3d/spaceTimePredictorNonlinear.cpph:524-577).
+ * lQhi/lFhi receive what the reference writes into tempUnknowns and
+ * tempFluxUnknowns.  lQi, lFi, lQhi, lFhi and picard_iterations may be NULL. */
+int exa_aderdg_space_time_predictor(exa_solver* s, double* lQhbnd, double* lFhbnd,
+                                    double* lQi, double* lFi, double* lQhi, double* lFhi,
+                                    const double* luh, const double dx[3], double dt,
+                                    int* picard_iterations);
+
+/* ADERDGSolver::volumeIntegral (E/solvers/ADERDGSolver.h:1015-1017;
+ * kernels/aderdg/generic/c/3d/volumeIntegralNonlinear.cpph:26-125) */
+int exa_aderdg_volume_integral(exa_solver* s, double* lduh, const double* lFhi, const double dx[3]);
+
+/* ADERDGSolver::surfaceIntegral (E/solvers/ADERDGSolver.h:1027-1029;
+ * kernels/aderdg/generic/c/3d/surfaceIntegralNonlinear.cpph:21-86) */
+int exa_aderdg_surface_integral(exa_solver* s, double* lduh, const double* lFhbnd, const double dx[3]);
+
+/* ADERDGSolver::riemannSolver (E/solvers/ADERDGSolver.h:1073-1080;
+ * kernels/aderdg/generic/c/3d/riemannSolverNonlinear.cpph:38-120).  FL and FR
+ * enter holding the extrapolated normal fluxes and leave holding the
+ * Rusanov flux. */
+int exa_aderdg_riemann_solver(exa_solver* s, double* FL, double* FR, const double* QL,
+                              const double* QR, double dt, int normalNonZero);
+
+/* ADERDGSolver::solutionUpdate (E/solvers/ADERDGSolver.h:1005;
+ * kernels/aderdg/generic/c/3d/solutionUpdate.cpph:23-47) */
+int exa_aderdg_solution_update(exa_solver* s, double* luh, const double* lduh, double dt);
+
+/* ADERDGSolver::stableTimeStepSize (E/solvers/ADERDGSolver.h:1112-1115;
+ * kernels/aderdg/generic/c/3d/stableTimeStepSize.cpph:21-61) */
+int exa_aderdg_stable_time_step_size(exa_solver* s, const double* luh, const double dx[3], double* dt);
+
+/* ---------------------------------------------------------------------- */
+/* Initial conditions evaluated at the Gauss-Legendre nodes, the user
+ * adjustPointSolution of the reference (E/solvers/ADERDGSolver.h:1183;
+ * kernels/aderdg/generic/c/3d/solutionAdjustment.cpph:17-52). */
+typedef struct {
+  int kind;              /* 0 isentropic vortex + density modes, 1 Sod sphere,
+                            2 constant state, 4 entropy wave            */
+  double xc[3];          /* vortex / sphere centre                       */
+  double L, eps;         /* vortex length scale and strength             */
+  double uinf[3];        /* background velocity                          */
+  double rho_inf, p_inf; /* background density and pressure              */
+  double amp;            /* density-mode amplitude                       */
+  int nmodes;            /* <= 8                                         */
+  double k[8][3], a[8], phi[8];
+  double r0, rho_in, p_in, rho_out, p_out;
+  double q0[8];
+} exa_ic;
+
+/* ADERDGSolver::adjustSolution with a pointwise initial condition. */
+int exa_aderdg_adjust_solution(exa_solver* s, double* luh, const double centre[3],
+                               const double dx[3], double t, const exa_ic* ic);
+
+/* ---------------------------------------------------------------------- */
+/* Device-resident periodic grid. */
+typedef struct exa_grid exa_grid;
+
+int exa_grid_create(exa_solver* s, int nx, int ny, int nz, const double offset[3],
+                    const double width[3], exa_grid** out);
+int exa_grid_destroy(exa_grid* g);
+int exa_grid_num_cells(const exa_grid* g, long long* n);
+/* luh of every cell, cells ordered (z,y,x) with x fastest, each in the
+ * reference layout [z][y][x][nData]. */
+int exa_grid_upload(exa_grid* g, const double* luh);
+int exa_grid_download(exa_grid* g, double* luh);
+int exa_grid_set_initial_conditions(exa_grid* g, const exa_ic* ic);
+/* Stable dt of the current state (global minimum over cells); also
+ * becomes the dt of the next step.  Synchronous. */
+int exa_grid_compute_dt(exa_grid* g, double* dt);
+/* Overrides the dt of the next step. */
+int exa_grid_set_dt(exa_grid* g, double dt);
+/* Enqueues nsteps time steps; each uses the stable dt of its start state. */
+int exa_grid_step(exa_grid* g, int nsteps);
+/* Waits for queued work; returns simulated time and the dt of the next step. */
+int exa_grid_sync(exa_grid* g, double* time, double* next_dt);
+/* Picard iterations executed by the last predictor of each cell (int per cell). */
+int exa_grid_picard_iterations(exa_grid* g, int* iters);
+/* 1: replay each time step as one captured hipGraph; 0: plain launches. */
+int exa_grid_use_graph(exa_grid* g, int enable);
+/* Kernel timing: when enabled, HIP events bracket every launch of each
+ * phase on the grid's stream; times[k] = summed milliseconds, counts[k] =
+ * launches, for k = 0 predictor, 1 riemann, 2 update(+surface+dt).
+ * Only meaningful with graphs disabled. */
+int exa_grid_enable_timing(exa_grid* g, int enable);
+int exa_grid_kernel_times(exa_grid* g, double times[3], long long counts[3]);
+/* Picard sweeps summed over all cells and all steps since the previous call
+ * (a device counter; the call resets it). */
+int exa_grid_total_picard_iterations(exa_grid* g, long long* total);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
