@@ -25,6 +25,9 @@ EXA_OK = 0
 PREC_FP64 = 0
 PREC_FP32 = 1
 PDE_EULER = 0
+PDE_EULERFV2D = 1
+FV_GODUNOV = 0
+FV_MUSCLHANCOCK = 1
 
 _D = C.POINTER(C.c_double)
 _lib = None
@@ -34,6 +37,11 @@ class AderdgConfig(C.Structure):
     _fields_ = [("order", C.c_int), ("nvar", C.c_int), ("npar", C.c_int), ("cfl", C.c_double),
                 ("gamma", C.c_double), ("pde", C.c_int), ("precision", C.c_int), ("fp64_sweeps", C.c_int),
                 ("waves_per_simd", C.c_int)]
+
+
+class FVConfig(C.Structure):
+    _fields_ = [("dim", C.c_int), ("patch", C.c_int), ("scheme", C.c_int), ("nvar", C.c_int), ("npar", C.c_int),
+                ("cfl", C.c_double), ("gamma", C.c_double), ("pde", C.c_int)]
 
 
 class InitialCondition(C.Structure):
@@ -56,6 +64,10 @@ EXPORTS = [
     "exa_grid_set_dt", "exa_grid_step", "exa_grid_sync", "exa_grid_picard_iterations",
     "exa_grid_use_graph", "exa_grid_enable_timing", "exa_grid_kernel_times",
     "exa_grid_total_picard_iterations",
+    "exa_fv_create", "exa_fv_destroy", "exa_fv_solution_update", "exa_fv_stable_time_step_size",
+    "exa_fv_riemann_solver", "exa_fv_grid_create", "exa_fv_grid_destroy", "exa_fv_grid_upload",
+    "exa_fv_grid_download", "exa_fv_grid_set_initial_conditions", "exa_fv_grid_compute_dt", "exa_fv_grid_step",
+    "exa_fv_grid_sync", "exa_limiter_project_on_fv", "exa_limiter_project_on_dg", "exa_limiter_dmp",
 ]
 
 
@@ -96,6 +108,22 @@ def load_library(path=LIB_PATH):
     L.exa_grid_enable_timing.argtypes = [C.c_void_p, I]
     L.exa_grid_kernel_times.argtypes = [C.c_void_p, _D, P(LL)]
     L.exa_grid_total_picard_iterations.argtypes = [C.c_void_p, P(LL)]
+    L.exa_fv_create.argtypes = [P(FVConfig), P(C.c_void_p)]
+    L.exa_fv_destroy.argtypes = [C.c_void_p]
+    L.exa_fv_solution_update.argtypes = [C.c_void_p, _D, _D, _D, C.c_double, _D]
+    L.exa_fv_stable_time_step_size.argtypes = [C.c_void_p, _D, _D, _D]
+    L.exa_fv_riemann_solver.argtypes = [C.c_void_p, _D, _D, _D, _D, I, _D]
+    L.exa_fv_grid_create.argtypes = [C.c_void_p, I, I, I, _D, _D, P(C.c_void_p)]
+    L.exa_fv_grid_destroy.argtypes = [C.c_void_p]
+    L.exa_fv_grid_upload.argtypes = [C.c_void_p, _D]
+    L.exa_fv_grid_download.argtypes = [C.c_void_p, _D]
+    L.exa_fv_grid_set_initial_conditions.argtypes = [C.c_void_p, P(InitialCondition)]
+    L.exa_fv_grid_compute_dt.argtypes = [C.c_void_p, _D]
+    L.exa_fv_grid_step.argtypes = [C.c_void_p, I]
+    L.exa_fv_grid_sync.argtypes = [C.c_void_p, _D, _D, _D]
+    L.exa_limiter_project_on_fv.argtypes = [I, _D, I, _D]
+    L.exa_limiter_project_on_dg.argtypes = [I, _D, I, _D]
+    L.exa_limiter_dmp.argtypes = [I, _D, I, C.c_double, C.c_double, _D, _D, P(I)]
     _lib = L
     return L
 
@@ -290,3 +318,131 @@ class Grid:
         c = np.zeros(3, dtype=np.int64)
         _check(_lib.exa_grid_kernel_times(self._h, _p(t), c.ctypes.data_as(C.POINTER(C.c_longlong))))
         return t, c
+
+
+class FiniteVolumesSolver:
+    """Per-patch kernel surface of exahype::solvers::FiniteVolumesSolver on the GPU
+    (E/solvers/FiniteVolumesSolver.h:235-400).  Patches include their ghost layers
+    in the reference layout [(z)][y][x][nData]."""
+
+    def __init__(self, dim, patch, scheme=FV_GODUNOV, cfl=0.9, gamma=1.4, pde=None, device=0):
+        L = load_library()
+        _check(L.exa_init(device))
+        if pde is None:
+            pde = PDE_EULER if dim == 3 else PDE_EULERFV2D
+        self.cfg = FVConfig(dim, patch, scheme, 5, 0, cfl, gamma, pde)
+        h = C.c_void_p()
+        _check(L.exa_fv_create(C.byref(self.cfg), C.byref(h)))
+        self._h = h
+        self.dim, self.patch = dim, patch
+        self.ghost = 2 if scheme == FV_MUSCLHANCOCK else 1
+        self.side = patch + 2 * self.ghost
+        self.patch_size = self.side ** dim * 5
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.exa_fv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solutionUpdate(self, luh_new, luh, dx, dt):
+        luh_new = np.array(luh_new, dtype=np.float64, copy=True)
+        adm = C.c_double(0.0)
+        _check(_lib.exa_fv_solution_update(self._h, _p(luh_new), _p(np.ascontiguousarray(luh, np.float64)),
+                                           _p(_vec3(dx)), float(dt), C.byref(adm)))
+        return luh_new, adm.value
+
+    def stableTimeStepSize(self, luh, dx):
+        dt = C.c_double(0.0)
+        _check(_lib.exa_fv_stable_time_step_size(self._h, _p(np.ascontiguousarray(luh, np.float64)), _p(_vec3(dx)),
+                                                 C.byref(dt)))
+        return dt.value
+
+    def riemannSolver(self, qL, qR, normalNonZero):
+        fL, fR, smax = np.zeros(5), np.zeros(5), C.c_double(0.0)
+        _check(_lib.exa_fv_riemann_solver(self._h, _p(fL), _p(fR), _p(np.ascontiguousarray(qL, np.float64)),
+                                          _p(np.ascontiguousarray(qR, np.float64)), int(normalNonZero),
+                                          C.byref(smax)))
+        return fL, fR, smax.value
+
+
+class FVGrid:
+    """Periodic grid of finite-volume patches resident in HBM."""
+
+    def __init__(self, solver, nx, ny, nz=1, offset=(0.0, 0.0, 0.0), width=(1.0, 1.0, 1.0)):
+        self.solver = solver
+        h = C.c_void_p()
+        _check(_lib.exa_fv_grid_create(solver._h, nx, ny, nz, _p(_vec3(offset)), _p(_vec3(width)), C.byref(h)))
+        self._h = h
+        self.npatch = nx * ny * nz
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.exa_fv_grid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, luh):
+        luh = np.ascontiguousarray(luh, dtype=np.float64).ravel()
+        assert luh.size == self.npatch * self.solver.patch_size
+        _check(_lib.exa_fv_grid_upload(self._h, _p(luh)))
+
+    def download(self):
+        out = np.zeros(self.npatch * self.solver.patch_size)
+        _check(_lib.exa_fv_grid_download(self._h, _p(out)))
+        return out
+
+    def set_initial_conditions(self, ic):
+        _check(_lib.exa_fv_grid_set_initial_conditions(self._h, C.byref(ic)))
+
+    def compute_dt(self):
+        dt = C.c_double(0.0)
+        _check(_lib.exa_fv_grid_compute_dt(self._h, C.byref(dt)))
+        return dt.value
+
+    def step(self, nsteps=1):
+        _check(_lib.exa_fv_grid_step(self._h, int(nsteps)))
+
+    def sync(self):
+        t, dt, adm = C.c_double(0.0), C.c_double(0.0), C.c_double(0.0)
+        _check(_lib.exa_fv_grid_sync(self._h, C.byref(t), C.byref(dt), C.byref(adm)))
+        return t.value, dt.value, adm.value
+
+
+def projectOnFVLimiterSpace(luh, order, ghost=0, lim=None):
+    """Subcell averages of the DG polynomial (2N+1 subcells per direction)."""
+    L = load_library()
+    _check(L.exa_init(0))
+    S = 2 * order + 1 + 2 * ghost
+    lim = np.zeros(S ** 3 * 5) if lim is None else np.array(lim, dtype=np.float64, copy=True)
+    _check(L.exa_limiter_project_on_fv(order, _p(np.ascontiguousarray(luh, np.float64)), ghost, _p(lim)))
+    return lim
+
+
+def projectOnDGSpace(lim, order, ghost=0):
+    L = load_library()
+    _check(L.exa_init(0))
+    luh = np.zeros((order + 1) ** 3 * 5)
+    _check(L.exa_limiter_project_on_dg(order, _p(np.ascontiguousarray(lim, np.float64)), ghost, _p(luh)))
+    return luh
+
+
+def discreteMaximumPrincipleAndMinAndMaxSearch(luh, order, bmin, bmax, relax, scaling, nobs=5):
+    L = load_library()
+    _check(L.exa_init(0))
+    bmin = np.array(bmin, dtype=np.float64, copy=True)
+    bmax = np.array(bmax, dtype=np.float64, copy=True)
+    ok = C.c_int(0)
+    _check(L.exa_limiter_dmp(order, _p(np.ascontiguousarray(luh, np.float64)), nobs, float(relax), float(scaling),
+                             _p(bmin), _p(bmax), C.byref(ok)))
+    return bool(ok.value), bmin, bmax

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "aderdg_kernels.h"
+#include "host_util.h"
 #include "dg_tables_data.h"
 
 __constant__ DGTables c_tab[EXA_MAX_ORDER + 1];
@@ -25,20 +26,6 @@ void exa_set_error(const char* fmt, ...) {
   va_end(ap);
   g_err = buf;
 }
-
-#define EXA_ARG(cond, msg)                      \
-  do {                                          \
-    if (!(cond)) {                              \
-      exa_set_error("invalid argument: %s", msg); \
-      return EXA_ERR_ARG;                       \
-    }                                           \
-  } while (0)
-
-#define EXA_TRY(expr)          \
-  do {                         \
-    int r__ = (expr);          \
-    if (r__ != EXA_OK) return r__; \
-  } while (0)
 
 // Order dispatch: instantiates a lambda-like body for the supported orders.
 #define EXA_ORDER_SWITCH(N, BODY)                 \
@@ -145,25 +132,6 @@ extern "C" int exa_device_synchronize(void) {
 
 // ---------------------------------------------------------------------------
 // solver handle
-struct DevBuf {
-  void* p = nullptr;
-  size_t n = 0;
-  int ensure(size_t bytes) {
-    if (bytes <= n) return EXA_OK;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-    EXA_CHECK(hipMalloc(&p, bytes));
-    n = bytes;
-    return EXA_OK;
-  }
-  template <typename T>
-  T* as() const { return static_cast<T*>(p); }
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-};
-
 struct exa_solver {
   exa_aderdg_config cfg;
   int device = 0;

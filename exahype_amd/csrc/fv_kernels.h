@@ -1,0 +1,385 @@
+// Finite-volume kernels for gfx950: Godunov (2D/3D), MUSCL-Hancock (3D),
+// Rusanov flux, FV CFL time step, ghost-layer filling of periodic patch grids.
+//
+// Device layout of a patch: structure of arrays over the patch including
+// its ghost layers, [patch][v][cell], cell = (k*S + j)*S + i (3D) or j*S + i
+// (2D), S = patch + 2*ghost.  Updates are written in gather form: one thread
+// per interior cell evaluates the Rusanov fluxes of its 2*DIM faces and
+// applies them in the reference's order (x-, x+, y-, y+, z-, z+), so each
+// cell's accumulation matches the reference's face loops
+// (kernels/finitevolumes/godunov/c/{2d,3d}/godunov.cpph,
+// kernels/finitevolumes/musclhancock/c/3d/musclhancock.cpph).
+#pragma once
+#include "common.h"
+#include "pde_euler.h"
+
+namespace exa {
+namespace fv {
+
+constexpr int V = 5;
+
+struct PatchGeom {
+  int P, G, S, dim;
+  __host__ __device__ int ncells_box() const { return dim == 3 ? S * S * S : S * S; }
+  __host__ __device__ int stride(int d) const { return d == 0 ? 1 : (d == 1 ? S : S * S); }
+};
+
+// rusanov<useNCP=false,useFlux=true>, kernels/finitevolumes/riemannsolvers/c/rusanov.cpph:24-156
+template <class PDE>
+__device__ inline double rusanov(const double* qL, const double* qR, int d, double gamma, double* f) {
+  const double sL = PDE::max_abs_eigenvalue(qL, d, gamma);
+  const double sR = PDE::max_abs_eigenvalue(qR, d, gamma);
+  double s_max = -1.0;
+  s_max = sL < s_max ? s_max : sL;
+  s_max = sR < s_max ? s_max : sR;
+  double FL[V], FR[V];
+  PDE::flux_dir(qL, d, FL, gamma);
+  PDE::flux_dir(qR, d, FR, gamma);
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    f[i] = 0.5 * s_max * (qL[i] - qR[i]);
+    f[i] += 0.5 * (FL[i] + FR[i]);
+  }
+  return s_max;
+}
+
+__device__ inline void load_q(const double* base, int nbox, int cell, double* q) {
+#pragma unroll
+  for (int v = 0; v < V; ++v) q[v] = base[v * nbox + cell];
+}
+
+__device__ inline void interior_coords(const PatchGeom& g, int c, int& i, int& j, int& k) {
+  i = c % g.P + g.G;
+  j = (c / g.P) % g.P + g.G;
+  k = g.dim == 3 ? c / (g.P * g.P) + g.G : 0;
+}
+
+__device__ inline int box_index(const PatchGeom& g, int i, int j, int k) {
+  return g.dim == 3 ? (k * g.S + j) * g.S + i : j * g.S + i;
+}
+
+// The reference's face loops also run over the two patch-boundary faces of
+// every line, so the first ghost layer of luh_new receives one face update:
+// -= at the lower ghost (face ghost|first), += at the upper ghost (last|ghost).
+// Enumerates those cells: item -> (direction d, side, transverse position).
+__device__ inline void ghost_layer_cell(const PatchGeom& g, int item, int& d, int& side, int& idx, int& inner) {
+  const int trans = g.dim == 3 ? g.P * g.P : g.P;
+  d = item / (2 * trans);
+  side = (item / trans) % 2;
+  const int t = item % trans;
+  const int a = g.dim == 3 ? t / g.P + g.G : 0, b = t % g.P + g.G;
+  int c[3];
+  const int o1 = d == 0 ? 1 : 0, o2 = d == 2 ? 1 : 2;
+  c[o1] = b;
+  c[o2] = a;
+  c[d] = side ? g.G + g.P : g.G - 1;
+  idx = box_index(g, c[0], c[1], c[2]);
+  c[d] = side ? g.G + g.P - 1 : g.G;
+  inner = box_index(g, c[0], c[1], c[2]);
+}
+
+// Godunov update of every patch: luh_new(interior) = luh + flux differences.
+// Returns (per patch, and as a global minimum) the admissible dt of the face
+// wave speeds: min CFL/DIM * h / s_max (godunov.cpph:33-143).
+template <int DIM, class PDE>
+__global__ __launch_bounds__(256) void k_godunov(const double* __restrict__ luh, double* __restrict__ luh_new,
+                                                 double* __restrict__ dt_patch, unsigned long long* dt_min,
+                                                 PatchGeom g, double dx0, double dx1, double dx2,
+                                                 const double* __restrict__ dtp, double gamma, double cfl) {
+  __shared__ double red[8];
+  const double dt = *dtp;
+  const int nbox = g.ncells_box();
+  const double* u = luh + (size_t)blockIdx.x * V * nbox;
+  double* un = luh_new + (size_t)blockIdx.x * V * nbox;
+  const double h[3] = {dx0 / g.P, dx1 / g.P, dx2 / g.P};
+  const int ninterior = DIM == 3 ? g.P * g.P * g.P : g.P * g.P;
+  double dtm = 1.0e300;
+  for (int c = threadIdx.x; c < ninterior; c += blockDim.x) {
+    int i, j, k;
+    interior_coords(g, c, i, j, k);
+    const int idx = box_index(g, i, j, k);
+    double q[V], acc[V], qn[V], f[V];
+    load_q(u, nbox, idx, q);
+#pragma unroll
+    for (int v = 0; v < V; ++v) acc[v] = q[v];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const double fac = dt / h[d];
+      load_q(u, nbox, idx - g.stride(d), qn);
+      double s = rusanov<PDE>(qn, q, d, gamma, f);
+      double cand = cfl / DIM * h[d] / s;
+      dtm = cand < dtm ? cand : dtm;
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[v] += fac * f[v];
+      load_q(u, nbox, idx + g.stride(d), qn);
+      s = rusanov<PDE>(q, qn, d, gamma, f);
+      cand = cfl / DIM * h[d] / s;
+      dtm = cand < dtm ? cand : dtm;
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[v] -= fac * f[v];
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) un[v * nbox + idx] = acc[v];
+  }
+  const int nghost = 2 * DIM * (DIM == 3 ? g.P * g.P : g.P);
+  for (int it = threadIdx.x; it < nghost; it += blockDim.x) {
+    int d, side, idx, inner;
+    ghost_layer_cell(g, it, d, side, idx, inner);
+    double ql[V], qr[V], f[V];
+    load_q(u, nbox, side ? inner : idx, ql);
+    load_q(u, nbox, side ? idx : inner, qr);
+    rusanov<PDE>(ql, qr, d, gamma, f);
+    const double fac = dt / h[d];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const double o = un[v * nbox + idx];
+      un[v * nbox + idx] = side ? o + fac * f[v] : o - fac * f[v];
+    }
+  }
+  dtm = block_min(dtm, red);
+  if (threadIdx.x == 0) {
+    if (dt_patch) dt_patch[blockIdx.x] = dtm;
+    if (dt_min) atomic_min_pos_double(dt_min, dtm);
+  }
+}
+
+__device__ inline double minmod(double a, double b) {
+  if (a * b < 0.0) return 0.0;
+  return fabs(a) < fabs(b) ? a : b;
+}
+
+// MUSCL-Hancock, phase 1 (musclhancock.cpph:62-213): minmod slopes, boundary
+// extrapolated states w, and for interior cells the half-step evolution with
+// the fluxes of the cell's own states.  The first ghost layer gets its
+// extrapolated states in the normal direction only and is not evolved.
+// W: [patch][6][v][cell], states (L,R) per direction x,y,z.
+template <class PDE>
+__global__ __launch_bounds__(256) void k_muscl_states(const double* __restrict__ luh, double* __restrict__ W,
+                                                      PatchGeom g, double dx0, double dx1, double dx2,
+                                                      const double* __restrict__ dtp, double gamma) {
+  const double dt = *dtp;
+  const int nbox = g.ncells_box();
+  const double* u = luh + (size_t)blockIdx.x * V * nbox;
+  double* w = W + (size_t)blockIdx.x * 6 * V * nbox;
+  const double h[3] = {dx0 / g.P, dx1 / g.P, dx2 / g.P};
+  const int b = g.G, e = g.G + g.P;
+  for (int c = threadIdx.x; c < nbox; c += blockDim.x) {
+    const int i = c % g.S, j = (c / g.S) % g.S, k = c / (g.S * g.S);
+    const bool in[3] = {i >= b && i < e, j >= b && j < e, k >= b && k < e};
+    const bool ghost[3] = {i == b - 1 || i == e, j == b - 1 || j == e, k == b - 1 || k == e};
+    const bool interior = in[0] && in[1] && in[2];
+    bool want[3];
+    want[0] = interior || (ghost[0] && in[1] && in[2]);
+    want[1] = interior || (ghost[1] && in[0] && in[2]);
+    want[2] = interior || (ghost[2] && in[0] && in[1]);
+    if (!(want[0] || want[1] || want[2])) continue;
+    double q[V];
+    load_q(u, nbox, c, q);
+    double wL[3][V], wR[3][V];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (!want[d]) continue;
+      double qm[V], qp[V];
+      load_q(u, nbox, c - g.stride(d), qm);
+      load_q(u, nbox, c + g.stride(d), qp);
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const double slope = minmod(qp[v] - q[v], q[v] - qm[v]) / h[d];
+        wL[d][v] = q[v] - 0.5 * h[d] * slope;
+        wR[d][v] = q[v] + 0.5 * h[d] * slope;
+      }
+    }
+    if (interior) {
+      double Qt[V], F[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) Qt[v] = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        PDE::flux_dir(wL[d], d, F, gamma);
+#pragma unroll
+        for (int v = 0; v < V; ++v) Qt[v] += F[v] / h[d];
+        PDE::flux_dir(wR[d], d, F, gamma);
+#pragma unroll
+        for (int v = 0; v < V; ++v) Qt[v] -= F[v] / h[d];
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          wL[d][v] += 0.5 * dt * Qt[v];
+          wR[d][v] += 0.5 * dt * Qt[v];
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (!want[d]) continue;
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        w[((2 * d) * V + v) * nbox + c] = wL[d][v];
+        w[((2 * d + 1) * V + v) * nbox + c] = wR[d][v];
+      }
+    }
+  }
+}
+
+// MUSCL-Hancock, phase 2 (musclhancock.cpph:215-281): Rusanov on
+// (w_R[i], w_L[i+1]) at every face, applied per interior cell.
+template <class PDE>
+__global__ __launch_bounds__(256) void k_muscl_update(const double* __restrict__ luh, const double* __restrict__ W,
+                                                      double* __restrict__ luh_new, double* __restrict__ dt_patch,
+                                                      unsigned long long* dt_min, PatchGeom g, double dx0, double dx1,
+                                                      double dx2, const double* __restrict__ dtp, double gamma,
+                                                      double cfl) {
+  __shared__ double red[8];
+  const double dt = *dtp;
+  const int nbox = g.ncells_box();
+  const double* u = luh + (size_t)blockIdx.x * V * nbox;
+  const double* w = W + (size_t)blockIdx.x * 6 * V * nbox;
+  double* un = luh_new + (size_t)blockIdx.x * V * nbox;
+  const double h[3] = {dx0 / g.P, dx1 / g.P, dx2 / g.P};
+  const int ninterior = g.P * g.P * g.P;
+  double dtm = 1.0e300;
+  for (int c = threadIdx.x; c < ninterior; c += blockDim.x) {
+    int i, j, k;
+    interior_coords(g, c, i, j, k);
+    const int idx = box_index(g, i, j, k);
+    double acc[V], a[V], b[V], f[V];
+    load_q(u, nbox, idx, acc);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double fac = dt / h[d];
+      const double* wLd = w + (2 * d) * V * nbox;
+      const double* wRd = w + (2 * d + 1) * V * nbox;
+      load_q(wRd, nbox, idx - g.stride(d), a);
+      load_q(wLd, nbox, idx, b);
+      double s = rusanov<PDE>(a, b, d, gamma, f);
+      double cand = cfl / 3 * h[d] / s;
+      dtm = cand < dtm ? cand : dtm;
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[v] += fac * f[v];
+      load_q(wRd, nbox, idx, a);
+      load_q(wLd, nbox, idx + g.stride(d), b);
+      s = rusanov<PDE>(a, b, d, gamma, f);
+      cand = cfl / 3 * h[d] / s;
+      dtm = cand < dtm ? cand : dtm;
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[v] -= fac * f[v];
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) un[v * nbox + idx] = acc[v];
+  }
+  const int nghost = 6 * g.P * g.P;
+  for (int it = threadIdx.x; it < nghost; it += blockDim.x) {
+    int d, side, idx, inner;
+    ghost_layer_cell(g, it, d, side, idx, inner);
+    double a[V], b[V], f[V];
+    const double* wLd = w + (2 * d) * V * nbox;
+    const double* wRd = w + (2 * d + 1) * V * nbox;
+    load_q(wRd, nbox, side ? inner : idx, a);
+    load_q(wLd, nbox, side ? idx : inner, b);
+    rusanov<PDE>(a, b, d, gamma, f);
+    const double fac = dt / h[d];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const double o = un[v * nbox + idx];
+      un[v * nbox + idx] = side ? o + fac * f[v] : o - fac * f[v];
+    }
+  }
+  dtm = block_min(dtm, red);
+  if (threadIdx.x == 0) {
+    if (dt_patch) dt_patch[blockIdx.x] = dtm;
+    if (dt_min) atomic_min_pos_double(dt_min, dtm);
+  }
+}
+
+// FV CFL time step of every patch (commons.cpph:26-67 2D, :410-463 3D).
+template <int DIM, class PDE>
+__global__ __launch_bounds__(256) void k_fv_stable_dt(const double* __restrict__ luh, double* __restrict__ dt_patch,
+                                                      unsigned long long* dt_min, PatchGeom g, double dx0,
+                                                      double dx1, double dx2, double gamma, double cfl) {
+  __shared__ double red[8];
+  const int nbox = g.ncells_box();
+  const double* u = luh + (size_t)blockIdx.x * V * nbox;
+  const double dxs[3] = {dx0, dx1, dx2};
+  const int ninterior = DIM == 3 ? g.P * g.P * g.P : g.P * g.P;
+  double dtm = 1.0e300;
+  for (int c = threadIdx.x; c < ninterior; c += blockDim.x) {
+    int i, j, k;
+    interior_coords(g, c, i, j, k);
+    double q[V];
+    load_q(u, nbox, box_index(g, i, j, k), q);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      const double s = PDE::max_abs_eigenvalue(q, d, gamma);
+      const double cand = cfl / DIM * (dxs[d] / g.P) / s;
+      dtm = cand < dtm ? cand : dtm;
+    }
+  }
+  dtm = block_min(dtm, red);
+  if (threadIdx.x == 0) {
+    if (dt_patch) dt_patch[blockIdx.x] = dtm;
+    if (dt_min) atomic_min_pos_double(dt_min, dtm);
+  }
+}
+
+// Face ghost layers from the neighbours' interiors (commons.cpph:337-402,
+// :876-982).  nbr[(2*d + side)*npatch + p]: neighbour of patch p on side
+// (0: -, 1: +) of direction d.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_fv_ghosts(double* __restrict__ luh, const int* __restrict__ nbr,
+                                                   PatchGeom g, int npatch) {
+  const int p = blockIdx.x;
+  const int nbox = g.ncells_box();
+  const int trans = DIM == 3 ? g.P * g.P : g.P;
+  const int per_side = g.G * trans;
+  const int total = 2 * DIM * per_side;
+  double* me = luh + (size_t)p * V * nbox;
+  for (int it = threadIdx.x; it < total; it += blockDim.x) {
+    const int ds = it / per_side, r = it % per_side;
+    const int d = ds / 2, side = ds % 2;
+    const int kk = r / trans, t = r % trans;
+    const int a = DIM == 3 ? t / g.P + g.G : 0, b = t % g.P + g.G;
+    const int mine = side ? g.G + g.P + kk : kk;
+    const int theirs = side ? g.G + kk : g.P + kk;
+    int ci[3], ni[3];
+    // transverse coordinates: (b, a) fill the two directions other than d
+    const int o1 = d == 0 ? 1 : 0, o2 = d == 2 ? 1 : 2;
+    ci[d] = mine;
+    ni[d] = theirs;
+    ci[o1] = ni[o1] = b;
+    ci[o2] = ni[o2] = a;
+    const double* nb = luh + (size_t)nbr[(2 * d + side) * npatch + p] * V * nbox;
+    const int cidx = box_index(g, ci[0], ci[1], ci[2]);
+    const int nidx = box_index(g, ni[0], ni[1], ni[2]);
+#pragma unroll
+    for (int v = 0; v < V; ++v) me[v * nbox + cidx] = nb[v * nbox + nidx];
+  }
+}
+
+// Initial conditions at the subcell centres (commons.cpph:69-97).
+template <int DIM>
+__global__ __launch_bounds__(256) void k_fv_initial_conditions(double* __restrict__ luh, exa_ic ic, double gamma,
+                                                               PatchGeom g, int nx, int ny, double ox, double oy,
+                                                               double oz, double dx0, double dx1, double dx2) {
+  const int p = blockIdx.x;
+  const int nbox = g.ncells_box();
+  const int px = p % nx, py = (p / nx) % ny, pz = p / (nx * ny);
+  const double cx = ox + (px + 0.5) * dx0, cy = oy + (py + 0.5) * dx1, cz = oz + (pz + 0.5) * dx2;
+  const int ninterior = DIM == 3 ? g.P * g.P * g.P : g.P * g.P;
+  for (int c = threadIdx.x; c < ninterior; c += blockDim.x) {
+    int i, j, k;
+    interior_coords(g, c, i, j, k);
+    const double qr = (2.0 * (i - g.G) + 1.0) / (2.0 * g.P);
+    const double qs = (2.0 * (j - g.G) + 1.0) / (2.0 * g.P);
+    const double qt = (2.0 * (k - g.G) + 1.0) / (2.0 * g.P);
+    double X[3] = {cx + dx0 * (qr - 0.5), cy + dx1 * (qs - 0.5), DIM == 3 ? cz + dx2 * (qt - 0.5) : 0.0};
+    double Q[V];
+    exa_ic_point(ic, gamma, X, 0.0, Q);
+    const int idx = box_index(g, i, j, k);
+#pragma unroll
+    for (int v = 0; v < V; ++v) luh[(size_t)p * V * nbox + v * nbox + idx] = Q[v];
+  }
+}
+
+}  // namespace fv
+}  // namespace exa

@@ -1,0 +1,277 @@
+// A-posteriori subcell limiter on gfx950: DG <-> FV subcell projections and
+// the discrete maximum principle (DMP).
+//
+// The projection matrices are built on the host with the reference's
+// algorithm (kernels/LimiterProjectionMatrices.cpp:39-223): subcell averages
+// of the Lagrange basis by Gauss-Legendre quadrature (uh2lim), basis values at
+// the Gauss-Lobatto nodes (uh2lob), and the constrained least-squares inverse
+// (lim2uh) through the same Gauss-Jordan elimination, whose pivot search keeps
+// the signed value of a new pivot candidate (:181).  The same sequence of
+// operations gives the same matrices bit for bit.
+//
+// Device kernels contract one axis at a time (sum factorisation): one
+// workgroup per cell, the partially contracted tensors in LDS.
+#pragma once
+#include <cmath>
+#include <cstring>
+
+#include "common.h"
+
+namespace exa {
+namespace lim {
+
+constexpr int V = 5;
+constexpr int MAXB = 10, MAXL = 19;
+
+struct LimTables {
+  double u2l[MAXB][MAXL];   // uh2lim[k][i]: average of phi_k over subcell i
+  double u2lob[MAXB][MAXB]; // uh2lob[j][i]: phi_j at Gauss-Lobatto node i
+  double l2u[MAXL][MAXB];   // lim2uh[i][j]
+};
+
+inline void base_func_1d(double* phi, double xi, int B, const double* nodes) {
+  for (int i = 0; i < B; i++) phi[i] = 1;
+  for (int m = 0; m < B; m++)
+    for (int j = 0; j < B; j++) {
+      if (j == m) continue;
+      phi[m] = phi[m] * (xi - nodes[j]) / (nodes[m] - nodes[j]);
+    }
+}
+
+// Gauss-Jordan inverse with the reference's pivot bookkeeping.
+inline bool gauss_jordan_inverse(int n, const double* a, double* ia) {
+  double c[(MAXB + 1) * 2 * (MAXB + 1)];
+  auto C = [&](int i, int j) -> double& { return c[i * 2 * n + j]; };
+  for (int i = 0; i < n; i++) {
+    int j = 0;
+    for (; j < n; j++) C(i, j) = a[j * n + i];
+    for (; j < 2 * n; j++) C(i, j) = 0.;
+    C(i, i + n) = 1.;
+  }
+  for (int i = 0; i < n; i++) {
+    int ml = i;
+    double mlV = std::fabs(C(i, i));
+    for (int j = i + 1; j < n; j++)
+      if (std::fabs(C(j, i)) > mlV) {
+        ml = j;
+        mlV = C(j, i);
+      }
+    for (int k = 0; k < 2 * n; k++) {
+      const double t = C(ml, k);
+      C(ml, k) = C(i, k);
+      C(i, k) = t;
+    }
+    if (C(i, i) == 0) return false;
+    const double piv = 1. / C(i, i);
+    for (int k = 0; k < 2 * n; k++) C(i, k) *= piv;
+    for (int j = i + 1; j < n; j++) {
+      const double t = C(j, i);
+      for (int k = 0; k < 2 * n; k++) C(j, k) -= t * C(i, k);
+    }
+  }
+  for (int i = n - 1; i >= 0; i--)
+    for (int j = i - 1; j >= 0; j--) {
+      const double t = C(j, i);
+      for (int k = 0; k < 2 * n; k++) C(j, k) -= t * C(i, k);
+    }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) ia[j * n + i] = C(i, j + n);
+  return true;
+}
+
+// gl_nodes/gl_weights/gll_nodes: the order-N rows of the quadrature tables.
+inline bool build_tables(int N, const double* gl_nodes, const double* gl_weights, const double* gll_nodes,
+                         LimTables& t) {
+  const int B = N + 1, L = 2 * N + 1, M = B + 1;
+  std::memset(&t, 0, sizeof(t));
+  double phi[MAXB];
+  const double dxi = 1. / L;
+  for (int i = 0; i < L; i++) {
+    const double xLeft = i * dxi;
+    for (int j = 0; j < B; j++) {
+      const double xi = xLeft + dxi * gl_nodes[j];
+      base_func_1d(phi, xi, B, gl_nodes);
+      for (int k = 0; k < B; k++) t.u2l[k][i] += gl_weights[j] * phi[k];
+    }
+  }
+  for (int i = 0; i < B; i++) {
+    base_func_1d(phi, gll_nodes[i], B, gl_nodes);
+    for (int j = 0; j < B; j++) t.u2lob[j][i] = phi[j];
+  }
+  double lsqm[(MAXB + 1) * (MAXB + 1)], ilsqm[(MAXB + 1) * (MAXB + 1)], rhs[MAXL * (MAXB + 1)];
+  for (int i = 0; i < B; i++) {
+    for (int j = 0; j < B; j++) {
+      lsqm[i * M + j] = 0.;
+      for (int k = 0; k < L; k++) lsqm[i * M + j] += 2 * t.u2l[i][k] * t.u2l[j][k];
+    }
+    lsqm[i * M + B] = gl_weights[i];
+  }
+  for (int i = 0; i < B; i++) lsqm[B * M + i] = -gl_weights[i];
+  lsqm[B * M + B] = 0.;
+  if (!gauss_jordan_inverse(M, lsqm, ilsqm)) return false;
+  for (int i = 0; i < L; i++) {
+    for (int j = 0; j < B; j++) rhs[i * M + j] = 2 * t.u2l[j][i];
+    rhs[i * M + B] = dxi;
+  }
+  for (int i = 0; i < L; i++)
+    for (int j = 0; j < B; j++) {
+      t.l2u[i][j] = 0.;
+      for (int k = 0; k < M; k++) t.l2u[i][j] += ilsqm[k * M + j] * rhs[i * M + k];
+    }
+  return true;
+}
+
+// Contract axis `axis` of a [v][n2][n1][n0] tensor (n0 fastest) with the
+// matrix M[a_in][a_out] (row stride ms): in has extent nin along the axis,
+// out has extent nout.  Threads stride over the output elements.
+template <typename MAT>
+__device__ inline void contract_axis(const double* in, double* out, int n0, int n1, int n2, int axis, int nin,
+                                     int nout, MAT mat) {
+  const int o0 = axis == 0 ? nout : n0, o1 = axis == 1 ? nout : n1, o2 = axis == 2 ? nout : n2;
+  const int total = V * o0 * o1 * o2;
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int i0 = e % o0, i1 = (e / o0) % o1, i2 = (e / (o0 * o1)) % o2, v = e / (o0 * o1 * o2);
+    double acc = 0.0;
+    for (int k = 0; k < nin; ++k) {
+      const int j0 = axis == 0 ? k : i0, j1 = axis == 1 ? k : i1, j2 = axis == 2 ? k : i2;
+      const int oo = axis == 0 ? i0 : (axis == 1 ? i1 : i2);
+      acc += in[((v * n2 + j2) * n1 + j1) * n0 + j0] * mat(k, oo);
+    }
+    out[((v * o2 + i2) * o1 + i1) * o0 + i0] = acc;
+  }
+}
+
+// luh [cell][v][B^3] -> lim [cell][v][S^3] interior (S = L + 2 ghost);
+// projectOnFVLimiterSpace, kernels/limiter/generic/c/Limiter.cpp:23-66.
+__global__ void k_project_on_fv(const double* __restrict__ luh, double* __restrict__ lim, const LimTables* tab,
+                                int B, int ghost, const int* __restrict__ cells) {
+  extern __shared__ double sm[];
+  const int L = 2 * B - 1, S = L + 2 * ghost;
+  const int cell = cells ? cells[blockIdx.x] : blockIdx.x;
+  const double* u = luh + (size_t)cell * V * B * B * B;
+  double* a = sm;                      // [v][B][B][L]
+  double* b = sm + V * B * B * L;      // [v][B][L][L]
+  double* c = b + V * B * L * L;       // [v][L][L][L]
+  auto m = [&](int k, int o) { return tab->u2l[k][o]; };
+  contract_axis(u, a, B, B, B, 0, B, L, m);
+  __syncthreads();
+  contract_axis(a, b, L, B, B, 1, B, L, m);
+  __syncthreads();
+  contract_axis(b, c, L, L, B, 2, B, L, m);
+  __syncthreads();
+  double* o = lim + (size_t)blockIdx.x * V * S * S * S;
+  for (int e = threadIdx.x; e < V * L * L * L; e += blockDim.x) {
+    const int x = e % L, y = (e / L) % L, z = (e / (L * L)) % L, v = e / (L * L * L);
+    o[((v * S + z + ghost) * S + y + ghost) * S + x + ghost] = c[e];
+  }
+}
+
+// lim interior -> luh; projectOnDGSpace, kernels/limiter/generic/c/Limiter.cpp:69-112.
+__global__ void k_project_on_dg(const double* __restrict__ lim, double* __restrict__ luh, const LimTables* tab,
+                                int B, int ghost, const int* __restrict__ cells) {
+  extern __shared__ double sm[];
+  const int L = 2 * B - 1, S = L + 2 * ghost;
+  const int cell = cells ? cells[blockIdx.x] : blockIdx.x;
+  const double* in = lim + (size_t)blockIdx.x * V * S * S * S;
+  double* c = sm;                      // [v][L][L][L]
+  double* a = sm + V * L * L * L;      // [v][L][L][B]
+  double* b = a + V * L * L * B;       // [v][L][B][B]
+  for (int e = threadIdx.x; e < V * L * L * L; e += blockDim.x) {
+    const int x = e % L, y = (e / L) % L, z = (e / (L * L)) % L, v = e / (L * L * L);
+    c[e] = in[((v * S + z + ghost) * S + y + ghost) * S + x + ghost];
+  }
+  __syncthreads();
+  auto m = [&](int k, int o) { return tab->l2u[k][o]; };
+  contract_axis(c, a, L, L, L, 0, L, B, m);
+  __syncthreads();
+  contract_axis(a, b, B, L, L, 1, L, B, m);
+  __syncthreads();
+  contract_axis(b, luh + (size_t)cell * V * B * B * B, B, B, L, 2, L, B, m);
+}
+
+// Cell min/max of the observables (the first nobs variables) over the GL
+// nodes, the Gauss-Lobatto nodes and the subcell averages
+// (findCellLocalMinAndMax, kernels/limiter/generic/c/Limiter.cpp:210-368).
+// out: [cell][2][nobs] (min then max).
+__global__ void k_cell_min_max(const double* __restrict__ luh, double* __restrict__ out, const LimTables* tab, int B,
+                               int nobs, int ncells) {
+  extern __shared__ double sm[];
+  const int L = 2 * B - 1;
+  const int cell = blockIdx.x;
+  const double* u = luh + (size_t)cell * V * B * B * B;
+  double* a = sm;                  // scratch for the two contractions chains
+  double* b = a + V * B * B * L;
+  double* c = b + V * B * L * L;   // [v][L][L][L]
+  double* g1 = c + V * L * L * L;  // [v][B][B][B] lobatto chain
+  double* g2 = g1 + V * B * B * B;
+  double* g3 = g2 + V * B * B * B;
+  double* red = g3 + V * B * B * B;  // 2 * nobs * (blockDim/64)
+  auto ml = [&](int k, int o) { return tab->u2l[k][o]; };
+  auto mg = [&](int k, int o) { return tab->u2lob[k][o]; };
+  contract_axis(u, a, B, B, B, 0, B, L, ml);
+  contract_axis(u, g1, B, B, B, 0, B, B, mg);
+  __syncthreads();
+  contract_axis(a, b, L, B, B, 1, B, L, ml);
+  contract_axis(g1, g2, B, B, B, 1, B, B, mg);
+  __syncthreads();
+  contract_axis(b, c, L, L, B, 2, B, L, ml);
+  contract_axis(g2, g3, B, B, B, 2, B, B, mg);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  for (int v = 0; v < nobs; ++v) {
+    double mn = 1.0e300, mx = -1.0e300;
+    for (int e = threadIdx.x; e < B * B * B; e += blockDim.x) {
+      const double p = u[v * B * B * B + e], q = g3[v * B * B * B + e];
+      mn = fmin(mn, fmin(p, q));
+      mx = fmax(mx, fmax(p, q));
+    }
+    for (int e = threadIdx.x; e < L * L * L; e += blockDim.x) {
+      const double q = c[v * L * L * L + e];
+      mn = fmin(mn, q);
+      mx = fmax(mx, q);
+    }
+    mn = wave_min(mn);
+    mx = -wave_min(-mx);
+    if (lane == 0) {
+      red[(2 * v) * nw + wid] = mn;
+      red[(2 * v + 1) * nw + wid] = mx;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < nobs) {
+    const int v = threadIdx.x;
+    double mn = red[(2 * v) * nw], mx = red[(2 * v + 1) * nw];
+    for (int w = 1; w < nw; ++w) {
+      mn = fmin(mn, red[(2 * v) * nw + w]);
+      mx = fmax(mx, red[(2 * v + 1) * nw + w]);
+    }
+    out[((size_t)cell * 2 + 0) * nobs + v] = mn;
+    out[((size_t)cell * 2 + 1) * nobs + v] = mx;
+  }
+}
+
+// DMP of every cell against the min/max of itself and its six face
+// neighbours (discreteMaximumPrincipleAndMinAndMaxSearch,
+// kernels/limiter/generic/c/Limiter.cpp:114-165; the boundary min/max of a
+// cell are the minima/maxima of its neighbours' cell values merged over the
+// faces, E/solvers/LimitingADERDGSolver.cpp:993-1026).  ok[cell] = 1 if
+// admissible.  minmax: [cell][2][nobs]; nbr: [6][ncells] neighbour ids.
+__global__ void k_dmp(const double* __restrict__ minmax, const double* __restrict__ bminmax,
+                      int* __restrict__ ok, int nobs, int ncells, double relax, double scaling) {
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= ncells) return;
+  int good = 1;
+  for (int v = 0; v < nobs; ++v) {
+    const double bmin = bminmax[((size_t)cell * 2 + 0) * nobs + v];
+    const double bmax = bminmax[((size_t)cell * 2 + 1) * nobs + v];
+    double sd = (bmax - bmin) * scaling;
+    sd = sd < relax ? relax : sd;
+    const double lmin = minmax[((size_t)cell * 2 + 0) * nobs + v];
+    const double lmax = minmax[((size_t)cell * 2 + 1) * nobs + v];
+    if (lmin < bmin - sd || lmax > bmax + sd) good = 0;
+  }
+  ok[cell] = good;
+}
+
+}  // namespace lim
+}  // namespace exa

@@ -48,6 +48,19 @@ struct EulerPDE {
     lambda[4] = u_n + c;
   }
 
+  // flux in direction d only (same expressions as flux(); p on the diagonal)
+  __host__ __device__ static inline void flux_dir(const double* Q, int d, double* f, double gamma) {
+    const double irho = 1.0 / Q[0];
+    const double p = (gamma - 1) * (Q[4] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) * irho);
+    const double qd = Q[1 + d];
+    f[0] = qd;
+    f[1] = irho * qd * Q[1];
+    f[2] = irho * qd * Q[2];
+    f[3] = irho * qd * Q[3];
+    f[1 + d] = f[1 + d] + p;
+    f[4] = irho * qd * (Q[4] + p);
+  }
+
   // max_v |lambda_v| accumulated as in stableTimeStepSize.cpph:43-48
   // (std::max(fabs(lambda), maxEigenvalue) starting from 0).
   __host__ __device__ static inline double max_abs_eigenvalue(const double* Q, int d, double gamma) {
@@ -72,7 +85,14 @@ __host__ __device__ inline void exa_ic_point(const exa_ic& ic, double gamma, con
     for (int v = 0; v < 5; ++v) Q[v] = ic.q0[v];
     return;
   }
-  if (ic.kind == 4) {
+  if (ic.kind == 5) {
+    const double d0 = x[0] - ic.xc[0], d1 = x[1] - ic.xc[1], d2 = x[2] - ic.xc[2];
+    rho = ic.rho_inf + ic.amp * exp(-(d0 * d0 + d1 * d1 + d2 * d2) / ic.L);
+    p = ic.p_inf;
+    u0 = ic.uinf[0];
+    u1 = ic.uinf[1];
+    u2 = ic.uinf[2];
+  } else if (ic.kind == 4) {
     double s = 0.0;
     for (int m = 0; m < ic.nmodes; ++m)
       s += ic.a[m] * sin(2.0 * pi * (ic.k[m][0] * (x[0] - ic.uinf[0] * t) + ic.k[m][1] * (x[1] - ic.uinf[1] * t) +
@@ -114,3 +134,33 @@ __host__ __device__ inline void exa_ic_point(const exa_ic& ic, double gamma, con
   Q[3] = rho * u2;
   Q[4] = p / (gamma - 1.0) + 0.5 * rho * (u0 * u0 + u1 * u1 + u2 * u2);
 }
+
+// The EulerFV demonstrator's user PDE (2D, variables rho, j[3], E;
+// Demonstrators/EulerFV/MyEulerSolver.cpp:51-85): p = (g-1)(E - ((0.5/rho) j).j),
+// F[d][1+i] = (1/rho)(j_i j_d) + p delta_id as the generated Fluxes::j(Matrix)
+// writes it (Toolkit/src/eu/exahype/variables/Variables.java:629-636).
+struct EulerFV2DPDE {
+  static constexpr int NVar = 5;
+  __host__ __device__ static inline double pressure(const double* Q, double irho, double gamma) {
+    const double h = 0.5 * irho;
+    const double jj = (h * Q[1]) * Q[1] + (h * Q[2]) * Q[2] + (h * Q[3]) * Q[3];
+    return (gamma - 1) * (Q[4] - jj);
+  }
+  __host__ __device__ static inline void flux_dir(const double* Q, int d, double* f, double gamma) {
+    const double irho = 1. / Q[0];
+    const double p = pressure(Q, irho, gamma);
+    f[0] = Q[1 + d];
+    for (int i = 0; i < 3; ++i) f[1 + i] = irho * (Q[1 + i] * Q[1 + d]) + p * (i == d ? 1.0 : 0.0);
+    f[4] = (irho * (Q[4] + p)) * Q[1 + d];
+  }
+  __host__ __device__ static inline double max_abs_eigenvalue(const double* Q, int d, double gamma) {
+    const double irho = 1. / Q[0];
+    const double p = pressure(Q, irho, gamma);
+    const double u_n = Q[d + 1] * irho;
+    const double c = sqrt(gamma * p * irho);
+    const double lam[5] = {u_n - c, u_n, u_n, u_n, u_n + c};
+    double m = fabs(lam[0]);
+    for (int v = 1; v < 5; ++v) m = (m < fabs(lam[v])) ? fabs(lam[v]) : m;
+    return m;
+  }
+};

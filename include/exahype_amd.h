@@ -118,7 +118,8 @@ int exa_aderdg_stable_time_step_size(exa_solver* s, const double* luh, const dou
  * kernels/aderdg/generic/c/3d/solutionAdjustment.cpph:17-52). */
 typedef struct {
   int kind;              /* 0 isentropic vortex + density modes, 1 Sod sphere,
-                            2 constant state, 4 entropy wave            */
+                            2 constant state, 4 entropy wave,
+                            5 Gaussian density pulse rho_inf + amp exp(-|x-xc|^2/L) */
   double xc[3];          /* vortex / sphere centre                       */
   double L, eps;         /* vortex length scale and strength             */
   double uinf[3];        /* background velocity                          */
@@ -169,6 +170,87 @@ int exa_grid_kernel_times(exa_grid* g, double times[3], long long counts[3]);
 /* Picard sweeps summed over all cells and all steps since the previous call
  * (a device counter; the call resets it). */
 int exa_grid_total_picard_iterations(exa_grid* g, long long* total);
+
+/* ---------------------------------------------------------------------- */
+/* Finite volumes: the kernel surface of exahype::solvers::FiniteVolumesSolver
+ * (E/solvers/FiniteVolumesSolver.h:235-400).  A patch is stored with its
+ * ghost layers, reference layout [(z)][y][x][nData], side patch+2*ghost. */
+#define EXA_FV_GODUNOV 0        /* first order, ghost width 1 (2D and 3D)        */
+#define EXA_FV_MUSCLHANCOCK 1   /* second order, ghost width 2, minmod (3D only) */
+#define EXA_PDE_EULERFV2D 1     /* the EulerFV demonstrator's 2D Euler (rho, j[3], E) */
+
+typedef struct exa_fv_solver exa_fv_solver;
+
+typedef struct {
+  int dim;        /* 2 or 3                                        */
+  int patch;      /* subcells per direction (PatchSize)            */
+  int scheme;     /* EXA_FV_GODUNOV | EXA_FV_MUSCLHANCOCK          */
+  int nvar;       /* 5                                             */
+  int npar;       /* 0                                             */
+  double cfl;     /* CFL of stableTimeStepSize (0.9 in the reference) */
+  double gamma;
+  int pde;        /* EXA_PDE_EULER (3D) | EXA_PDE_EULERFV2D (2D)   */
+} exa_fv_config;
+
+int exa_fv_create(const exa_fv_config* cfg, exa_fv_solver** out);
+int exa_fv_destroy(exa_fv_solver* s);
+
+/* FiniteVolumesSolver::solutionUpdate (E/solvers/FiniteVolumesSolver.h:300-306;
+ * kernels/finitevolumes/godunov/c/{2d,3d}/godunov.cpph,
+ * kernels/finitevolumes/musclhancock/c/3d/musclhancock.cpph).  luh_new holds a
+ * copy of luh on entry, as FiniteVolumesSolver::updateSolution passes it
+ * (E/solvers/FiniteVolumesSolver.cpp:519-583); max_admissible_dt receives the
+ * kernel's face-speed time step (MUSCL-Hancock uses its own CFL 0.99). */
+int exa_fv_solution_update(exa_fv_solver* s, double* luh_new, const double* luh, const double dx[3], double dt,
+                           double* max_admissible_dt);
+
+/* FiniteVolumesSolver::stableTimeStepSize (E/solvers/FiniteVolumesSolver.h:252-255;
+ * kernels/finitevolumes/commons/c/commons.cpph:26-67, :410-463) */
+int exa_fv_stable_time_step_size(exa_fv_solver* s, const double* luh, const double dx[3], double* dt);
+
+/* FiniteVolumesSolver::riemannSolver (E/solvers/FiniteVolumesSolver.h:333-336;
+ * kernels/finitevolumes/riemannsolvers/c/rusanov.cpph:24-156); returns s_max. */
+int exa_fv_riemann_solver(exa_fv_solver* s, double* fL, double* fR, const double* qL, const double* qR,
+                          int normalNonZero, double* s_max);
+
+/* Device-resident periodic grid of patches (nz = 1 in 2D).  One step is the
+ * reference's FV step: ghost-layer filling from the neighbours
+ * (E/solvers/FiniteVolumesSolver.cpp:659-687), solution update with the dt of
+ * the step (:519-583), then the stable dt of the new state (:443-466). */
+typedef struct exa_fv_grid exa_fv_grid;
+
+int exa_fv_grid_create(exa_fv_solver* s, int nx, int ny, int nz, const double offset[3], const double width[3],
+                       exa_fv_grid** out);
+int exa_fv_grid_destroy(exa_fv_grid* g);
+/* all patches with ghost layers, patches ordered (z,y,x) x fastest */
+int exa_fv_grid_upload(exa_fv_grid* g, const double* luh);
+int exa_fv_grid_download(exa_fv_grid* g, double* luh);
+int exa_fv_grid_set_initial_conditions(exa_fv_grid* g, const exa_ic* ic);
+int exa_fv_grid_compute_dt(exa_fv_grid* g, double* dt);
+int exa_fv_grid_step(exa_fv_grid* g, int nsteps);
+/* simulated time, dt of the next step, min over the last step's patches of
+ * the kernels' admissible dt */
+int exa_fv_grid_sync(exa_fv_grid* g, double* time, double* next_dt, double* admissible_dt);
+
+/* ---------------------------------------------------------------------- */
+/* Subcell limiter (orders 1..5, 2N+1 subcells per direction, 5 variables).
+ * luh [z][y][x][nVar]; lim [z][y][x][nVar] of side 2N+1+2*ghost, of which the
+ * interior is read or written. */
+
+/* kernels::limiter::generic::c::projectOnFVLimiterSpace
+ * (kernels/limiter/generic/c/Limiter.cpp:23-66) */
+int exa_limiter_project_on_fv(int order, const double* luh, int ghost, double* lim);
+
+/* kernels::limiter::generic::c::projectOnDGSpace (kernels/limiter/generic/c/Limiter.cpp:69-112) */
+int exa_limiter_project_on_dg(int order, const double* lim, int ghost, double* luh);
+
+/* kernels::limiter::generic::c::discreteMaximumPrincipleAndMinAndMaxSearch
+ * (kernels/limiter/generic/c/Limiter.cpp:114-165) with the observables the
+ * first nobs variables.  bmin/bmax: [6][nobs] face-neighbour minima/maxima,
+ * updated as the reference does (block 0 <- cell min/max, block 1 <- boundary
+ * min/max); *admissible = 1 if the DMP holds. */
+int exa_limiter_dmp(int order, const double* luh, int nobs, double relax, double scaling, double* bmin,
+                    double* bmax, int* admissible);
 
 #ifdef __cplusplus
 }

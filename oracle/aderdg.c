@@ -441,7 +441,14 @@ void orc_ic_point(const orc_cfg* c, const orc_ic* ic, const double x[3], double 
     for (int v = 0; v < c->nvar; ++v) Q[v] = 1.0 + (v + 1) * s;
     return;
   }
-  if (ic->kind == 4) { /* entropy wave: density modes advected with uinf, constant u and p */
+  if (ic->kind == 5) { /* rho = rho_inf + amp exp(-|x-xc|^2 / L), velocity uinf, pressure p_inf */
+    const double d0 = x[0] - ic->xc[0], d1 = x[1] - ic->xc[1], d2 = x[2] - ic->xc[2];
+    rho = ic->rho_inf + ic->amp * exp(-(d0 * d0 + d1 * d1 + d2 * d2) / ic->L);
+    p = ic->p_inf;
+    u[0] = ic->uinf[0];
+    u[1] = ic->uinf[1];
+    u[2] = ic->uinf[2];
+  } else if (ic->kind == 4) { /* entropy wave: density modes advected with uinf, constant u and p */
     double s = 0.0;
     for (int m = 0; m < ic->nmodes; ++m)
       s += ic->a[m] * sin(2.0 * M_PI * (ic->k[m][0] * (x[0] - ic->uinf[0] * t) + ic->k[m][1] * (x[1] - ic->uinf[1] * t) +

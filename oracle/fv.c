@@ -295,3 +295,141 @@ double orc_musclhancock_update(const orc_cfg* c, int patch, double* luh_new,
   free(slopet);
   return dt_max;
 }
+
+/* ---- periodic grid of FV patches (FiniteVolumesSolver step, §3.4 of the
+ * survey: ghost filling E/solvers/FiniteVolumesSolver.cpp:659-687, update
+ * :519-583, new dt :443-466) ---- */
+
+/* ghostLayerFilling, kernels/finitevolumes/commons/c/commons.cpph:337-402 (2D), :876-982 (3D):
+ * face ghost layers of width G copied from the neighbour's interior. */
+static void fv_fill_ghosts(const orc_cfg* c, const orc_fv_grid* g) {
+  const int nd = c->nvar + c->npar, P = g->patch, G = g->ghost, S = P + 2 * G;
+  const int dim = g->dim;
+  const size_t psz = (size_t)(dim == 3 ? S * S * S : S * S) * nd;
+  const int nz = dim == 3 ? g->nz : 1;
+  for (int pz = 0; pz < nz; ++pz)
+    for (int py = 0; py < g->ny; ++py)
+      for (int px = 0; px < g->nx; ++px) {
+        double* me = g->luh + (((size_t)pz * g->ny + py) * g->nx + px) * psz;
+        for (int d = 0; d < dim; ++d)
+          for (int side = 0; side < 2; ++side) {
+            int qx = px, qy = py, qz = pz;
+            const int step = side ? 1 : -1;
+            if (d == 0) qx = (px + step + g->nx) % g->nx;
+            if (d == 1) qy = (py + step + g->ny) % g->ny;
+            if (d == 2) qz = (pz + step + nz) % nz;
+            const double* nb = g->luh + (((size_t)qz * g->ny + qy) * g->nx + qx) * psz;
+            const int zb = dim == 3 ? G : 0, ze = dim == 3 ? G + P : 1;
+            for (int a = zb; a < ze; ++a)
+              for (int b = G; b < G + P; ++b)
+                for (int k = 0; k < G; ++k) {
+                  /* my ghost index along d, neighbour's interior index along d */
+                  const int mine = side ? G + P + k : k;
+                  const int theirs = side ? G + k : P + k; /* patchBegin+k | patchEnd-G+k */
+                  int ci[3], ni[3]; /* (x,y,z) */
+                  if (dim == 2) {
+                    /* 2D: a unused; (y,x) with b the transverse index */
+                    if (d == 0) { ci[0] = mine; ci[1] = b; ni[0] = theirs; ni[1] = b; }
+                    else { ci[0] = b; ci[1] = mine; ni[0] = b; ni[1] = theirs; }
+                    memcpy(me + ((size_t)ci[1] * S + ci[0]) * nd, nb + ((size_t)ni[1] * S + ni[0]) * nd,
+                           nd * sizeof(double));
+                  } else {
+                    if (d == 0) { ci[0] = mine; ci[1] = b; ci[2] = a; ni[0] = theirs; ni[1] = b; ni[2] = a; }
+                    else if (d == 1) { ci[0] = b; ci[1] = mine; ci[2] = a; ni[0] = b; ni[1] = theirs; ni[2] = a; }
+                    else { ci[0] = b; ci[1] = a; ci[2] = mine; ni[0] = b; ni[1] = a; ni[2] = theirs; }
+                    memcpy(me + (((size_t)ci[2] * S + ci[1]) * S + ci[0]) * nd,
+                           nb + (((size_t)ni[2] * S + ni[1]) * S + ni[0]) * nd, nd * sizeof(double));
+                  }
+                }
+          }
+      }
+}
+
+int orc_fv_grid_alloc(const orc_cfg* c, orc_fv_grid* g, int dim, int scheme, int patch, int nx, int ny, int nz) {
+  memset(g, 0, sizeof(*g));
+  g->dim = dim;
+  g->scheme = scheme;
+  g->patch = patch;
+  g->ghost = scheme == 1 ? 2 : 1;
+  g->nx = nx; g->ny = ny; g->nz = dim == 3 ? nz : 1;
+  for (int d = 0; d < 3; ++d) g->width[d] = 1.0;
+  const int S = patch + 2 * g->ghost, nd = c->nvar + c->npar;
+  const size_t psz = (size_t)(dim == 3 ? S * S * S : S * S) * nd;
+  const size_t np = (size_t)nx * ny * g->nz;
+  g->luh = (double*)calloc(np * psz, sizeof(double));
+  g->luh_new = (double*)calloc(np * psz, sizeof(double));
+  return g->luh && g->luh_new ? 0 : -1;
+}
+
+void orc_fv_grid_free(orc_fv_grid* g) {
+  free(g->luh);
+  free(g->luh_new);
+  memset(g, 0, sizeof(*g));
+}
+
+/* solutionAdjustment at subcell centres, kernels/finitevolumes/commons/c/commons.cpph:69-97 (2D) */
+void orc_fv_grid_init(const orc_cfg* c, orc_fv_grid* g, const orc_ic* ic) {
+  const int nd = c->nvar + c->npar, P = g->patch, G = g->ghost, S = P + 2 * G, dim = g->dim;
+  const size_t psz = (size_t)(dim == 3 ? S * S * S : S * S) * nd;
+  const double hx = g->width[0] / g->nx, hy = g->width[1] / g->ny, hz = g->width[2] / g->nz;
+  for (int pz = 0; pz < g->nz; ++pz)
+    for (int py = 0; py < g->ny; ++py)
+      for (int px = 0; px < g->nx; ++px) {
+        double* me = g->luh + (((size_t)pz * g->ny + py) * g->nx + px) * psz;
+        const double cx = g->offset[0] + (px + 0.5) * hx, cy = g->offset[1] + (py + 0.5) * hy,
+                     cz = g->offset[2] + (pz + 0.5) * hz;
+        const int zb = dim == 3 ? G : 0, ze = dim == 3 ? G + P : 1;
+        for (int k = zb; k < ze; ++k)
+          for (int j = G; j < G + P; ++j)
+            for (int i = G; i < G + P; ++i) {
+              const double qr = (2.0 * (i - G) + 1.0) / (2.0 * P);
+              const double qs = (2.0 * (j - G) + 1.0) / (2.0 * P);
+              const double qt = (2.0 * (k - G) + 1.0) / (2.0 * P);
+              double x[3] = {cx + hx * (qr - 0.5), cy + hy * (qs - 0.5), dim == 3 ? cz + hz * (qt - 0.5) : 0.0};
+              double* Q = me + (dim == 3 ? (((size_t)k * S + j) * S + i) : ((size_t)j * S + i)) * nd;
+              orc_ic_point(c, ic, x, 0.0, Q);
+            }
+      }
+}
+
+double orc_fv_grid_stable_dt(const orc_cfg* c, const orc_fv_grid* g, int nthreads) {
+  const int nd = c->nvar + c->npar, S = g->patch + 2 * g->ghost;
+  const size_t psz = (size_t)(g->dim == 3 ? S * S * S : S * S) * nd;
+  const int np = g->nx * g->ny * g->nz;
+  const double dx[3] = {g->width[0] / g->nx, g->width[1] / g->ny, g->width[2] / g->nz};
+  double dtmin = DBL_MAX;
+#pragma omp parallel for num_threads(nthreads) reduction(min : dtmin) schedule(static)
+  for (int p = 0; p < np; ++p) {
+    const double d = orc_fv_stable_dt(c, g->dim, g->patch, g->ghost, g->luh + (size_t)p * psz, dx, c->cfl);
+    if (d < dtmin) dtmin = d;
+  }
+  return dtmin;
+}
+
+/* one step: ghost filling, update with dt (luh_new starts as a copy of luh),
+ * swap; returns the min over patches of the kernels' admissible dt through
+ * *dt_admissible and the stable dt of the new state. */
+double orc_fv_grid_step(const orc_cfg* c, orc_fv_grid* g, double dt, double* dt_admissible, int nthreads) {
+  const int nd = c->nvar + c->npar, S = g->patch + 2 * g->ghost;
+  const size_t psz = (size_t)(g->dim == 3 ? S * S * S : S * S) * nd;
+  const int np = g->nx * g->ny * g->nz;
+  const double dx[3] = {g->width[0] / g->nx, g->width[1] / g->ny, g->width[2] / g->nz};
+  fv_fill_ghosts(c, g);
+  memcpy(g->luh_new, g->luh, np * psz * sizeof(double));
+  double adm = DBL_MAX;
+#pragma omp parallel for num_threads(nthreads) reduction(min : adm) schedule(dynamic, 4)
+  for (int p = 0; p < np; ++p) {
+    double a;
+    if (g->scheme == 1)
+      a = orc_musclhancock_update(c, g->patch, g->luh_new + (size_t)p * psz, g->luh + (size_t)p * psz, dx, dt, 0.99);
+    else
+      a = orc_godunov_update(c, g->dim, g->patch, g->luh_new + (size_t)p * psz, g->luh + (size_t)p * psz, dx, dt,
+                             c->cfl);
+    if (a < adm) adm = a;
+  }
+  double* t = g->luh;
+  g->luh = g->luh_new;
+  g->luh_new = t;
+  if (dt_admissible) *dt_admissible = adm;
+  return orc_fv_grid_stable_dt(c, g, nthreads);
+}

@@ -35,7 +35,7 @@ typedef struct {
   int npar;       /* material parameters carried in luh (0 for Euler) */
   double gamma;   /* ratio of specific heats                         */
   double cfl;     /* CFL factor (0.9 for ADER-DG)                    */
-  int pde;        /* 0: 3D Euler (linear advection test PDE when 1)  */
+  int pde;        /* 0: 3D Euler; 1: linear advection test PDE; 2: EulerFV demonstrator (FV only) */
   double adv[3];  /* advection velocity for pde==1                   */
 } orc_cfg;
 
@@ -56,7 +56,8 @@ double orc_stable_time_step_size(const orc_cfg* c, const double* luh, const doub
 
 /* initial conditions at GL nodes (kernels/aderdg/generic/c/3d/solutionAdjustment.cpph:17-52) */
 typedef struct {
-  int kind;            /* 0 vortex+modes, 1 Sod sphere, 2 constant, 3 advected sine, 4 entropy wave */
+  int kind;            /* 0 vortex+modes, 1 Sod sphere, 2 constant, 3 advected sine, 4 entropy wave,
+                          5 Gaussian density pulse (EulerFV demonstrator setup) */
   double xc[3];        /* vortex centre / sphere centre                  */
   double L, eps;       /* vortex length scale and strength               */
   double uinf[3];      /* background velocity                            */
@@ -99,6 +100,27 @@ double orc_godunov_update(const orc_cfg* c, int dim, int patch, double* luh_new,
                           const double* luh, const double* dx, double dt, double cfl);
 double orc_musclhancock_update(const orc_cfg* c, int patch, double* luh_new,
                                const double* luh, const double* dx, double dt, double cfl);
+
+/* periodic grid of FV patches with face ghost layers (width 1 Godunov, 2 MUSCL-Hancock) */
+typedef struct {
+  int dim, scheme, patch, ghost;  /* scheme 0 Godunov, 1 MUSCL-Hancock */
+  int nx, ny, nz;
+  double offset[3], width[3];
+  double* luh;      /* [patch][(z)][y][x][nData] including ghosts */
+  double* luh_new;
+} orc_fv_grid;
+int orc_fv_grid_alloc(const orc_cfg* c, orc_fv_grid* g, int dim, int scheme, int patch, int nx, int ny, int nz);
+void orc_fv_grid_free(orc_fv_grid* g);
+void orc_fv_grid_init(const orc_cfg* c, orc_fv_grid* g, const orc_ic* ic);
+double orc_fv_grid_stable_dt(const orc_cfg* c, const orc_fv_grid* g, int nthreads);
+double orc_fv_grid_step(const orc_cfg* c, orc_fv_grid* g, double dt, double* dt_admissible, int nthreads);
+
+/* subcell limiter, kernels/limiter/generic/c/Limiter.cpp and kernels/LimiterProjectionMatrices.cpp */
+int orc_limiter_matrices(int N, double* uh2lim, double* uh2lob, double* lim2uh);
+void orc_project_on_fv(int N, int nvar, int ghost, const double* luh, double* lim);
+void orc_project_on_dg(int N, int nvar, int ghost, const double* lim, double* luh);
+void orc_cell_min_max(int N, int nvar, int nobs, const double* luh, double* mn, double* mx);
+int orc_dmp(int N, int nvar, int nobs, const double* luh, double relax, double scaling, double* bmin, double* bmax);
 
 #ifdef __cplusplus
 }

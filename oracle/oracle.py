@@ -40,6 +40,13 @@ class Grid(C.Structure):
                 ("iters", C.POINTER(C.c_int))]
 
 
+class FVGridS(C.Structure):
+    _fields_ = [("dim", C.c_int), ("scheme", C.c_int), ("patch", C.c_int), ("ghost", C.c_int),
+                ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("offset", C.c_double * 3), ("width", C.c_double * 3),
+                ("luh", C.POINTER(C.c_double)), ("luh_new", C.POINTER(C.c_double))]
+
+
 _lib = None
 _D = C.POINTER(C.c_double)
 
@@ -80,6 +87,21 @@ def lib():
         L.orc_godunov_update.restype = C.c_double
         L.orc_musclhancock_update.argtypes = [P(Cfg), C.c_int, _D, _D, _D, C.c_double, C.c_double]
         L.orc_musclhancock_update.restype = C.c_double
+        L.orc_fv_grid_alloc.argtypes = [P(Cfg), P(FVGridS), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_fv_grid_alloc.restype = C.c_int
+        L.orc_fv_grid_free.argtypes = [P(FVGridS)]
+        L.orc_fv_grid_init.argtypes = [P(Cfg), P(FVGridS), P(IC)]
+        L.orc_fv_grid_stable_dt.argtypes = [P(Cfg), P(FVGridS), C.c_int]
+        L.orc_fv_grid_stable_dt.restype = C.c_double
+        L.orc_fv_grid_step.argtypes = [P(Cfg), P(FVGridS), C.c_double, _D, C.c_int]
+        L.orc_fv_grid_step.restype = C.c_double
+        L.orc_limiter_matrices.argtypes = [C.c_int, _D, _D, _D]
+        L.orc_limiter_matrices.restype = C.c_int
+        L.orc_project_on_fv.argtypes = [C.c_int, C.c_int, C.c_int, _D, _D]
+        L.orc_project_on_dg.argtypes = [C.c_int, C.c_int, C.c_int, _D, _D]
+        L.orc_cell_min_max.argtypes = [C.c_int, C.c_int, C.c_int, _D, _D, _D]
+        L.orc_dmp.argtypes = [C.c_int, C.c_int, C.c_int, _D, C.c_double, C.c_double, _D, _D]
+        L.orc_dmp.restype = C.c_int
         _lib = L
     return _lib
 
@@ -221,3 +243,90 @@ class OracleGrid:
     def predict_cells(self, dt, first, count, nthreads=0):
         lib().orc_grid_predict_cells(C.byref(self.c), C.byref(self.g), dt, first, count,
                                      nthreads or os.cpu_count())
+
+
+def fv_rusanov(c, dim, qL, qR, normal):
+    fL, fR = np.zeros(16), np.zeros(16)
+    s = lib().orc_fv_rusanov(C.byref(c), dim, ptr(fL), ptr(fR), ptr(dvec(qL)), ptr(dvec(qR)), normal)
+    return fL[:c.nvar], fR[:c.nvar], s
+
+
+def fv_stable_dt(c, dim, patch, ghost, luh, dx, cfl):
+    return lib().orc_fv_stable_dt(C.byref(c), dim, patch, ghost, ptr(dvec(luh)), ptr(dvec(dx)), cfl)
+
+
+def godunov_update(c, dim, patch, luh_new, luh, dx, dt, cfl):
+    luh_new = dvec(luh_new).copy()
+    a = lib().orc_godunov_update(C.byref(c), dim, patch, ptr(luh_new), ptr(dvec(luh)), ptr(dvec(dx)), dt, cfl)
+    return luh_new, a
+
+
+def musclhancock_update(c, patch, luh_new, luh, dx, dt, cfl=0.99):
+    luh_new = dvec(luh_new).copy()
+    a = lib().orc_musclhancock_update(C.byref(c), patch, ptr(luh_new), ptr(dvec(luh)), ptr(dvec(dx)), dt, cfl)
+    return luh_new, a
+
+
+class OracleFVGrid:
+    """Periodic grid of FV patches driven by the oracle (scheme 0 Godunov, 1 MUSCL-Hancock)."""
+
+    def __init__(self, c, dim, scheme, patch, nx, ny, nz=1):
+        self.c = c
+        self.g = FVGridS()
+        if lib().orc_fv_grid_alloc(C.byref(c), C.byref(self.g), dim, scheme, patch, nx, ny, nz) != 0:
+            raise MemoryError("oracle FV grid allocation failed")
+        S = patch + 2 * self.g.ghost
+        self.patch_size = S ** dim * (c.nvar + c.npar)
+        self.npatch = nx * ny * (nz if dim == 3 else 1)
+
+    def __del__(self):
+        try:
+            lib().orc_fv_grid_free(C.byref(self.g))
+        except Exception:
+            pass
+
+    def init(self, ic):
+        lib().orc_fv_grid_init(C.byref(self.c), C.byref(self.g), C.byref(ic))
+
+    def luh(self):
+        return np.ctypeslib.as_array(self.g.luh, shape=(self.npatch * self.patch_size,))
+
+    def stable_dt(self, nthreads=0):
+        return lib().orc_fv_grid_stable_dt(C.byref(self.c), C.byref(self.g), nthreads or os.cpu_count())
+
+    def step(self, dt, nthreads=0):
+        adm = C.c_double(0.0)
+        nxt = lib().orc_fv_grid_step(C.byref(self.c), C.byref(self.g), dt, C.byref(adm), nthreads or os.cpu_count())
+        return nxt, adm.value
+
+
+def limiter_matrices(N):
+    B, L = N + 1, 2 * N + 1
+    a, b, c = np.zeros(B * L), np.zeros(B * B), np.zeros(L * B)
+    assert lib().orc_limiter_matrices(N, ptr(a), ptr(b), ptr(c)) == 0
+    return a.reshape(B, L), b.reshape(B, B), c.reshape(L, B)
+
+
+def project_on_fv(N, luh, ghost=0, nvar=5):
+    S = 2 * N + 1 + 2 * ghost
+    lim = np.zeros(S ** 3 * nvar)
+    lib().orc_project_on_fv(N, nvar, ghost, ptr(dvec(luh)), ptr(lim))
+    return lim
+
+
+def project_on_dg(N, lim, ghost=0, nvar=5):
+    luh = np.zeros((N + 1) ** 3 * nvar)
+    lib().orc_project_on_dg(N, nvar, ghost, ptr(dvec(lim)), ptr(luh))
+    return luh
+
+
+def cell_min_max(N, luh, nobs=5, nvar=5):
+    mn, mx = np.zeros(nobs), np.zeros(nobs)
+    lib().orc_cell_min_max(N, nvar, nobs, ptr(dvec(luh)), ptr(mn), ptr(mx))
+    return mn, mx
+
+
+def dmp(N, luh, bmin, bmax, relax, scaling, nobs=5, nvar=5):
+    bmin, bmax = dvec(bmin).copy(), dvec(bmax).copy()
+    ok = lib().orc_dmp(N, nvar, nobs, ptr(dvec(luh)), relax, scaling, ptr(bmin), ptr(bmax))
+    return bool(ok), bmin, bmax
