@@ -108,6 +108,8 @@ def main():
     ap.add_argument("--cells", type=int, default=81)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--waves-per-simd", type=int, default=0, help="predictor occupancy target (0 = tuned)")
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="also time a plain device copy of this many GiB (HBM reference, counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the path runs on a single MI355X in this build "
@@ -185,7 +187,26 @@ def main():
         out["cpu_baseline"] = {"value": v, "unit": "DoF-updates/s", "cores": threads, "kind": "port",
                                "sample": "oracle, order %d, 27^3 cells of size 1/%d, 2 steps (%.1f s wall)" % (N, n, wall),
                                "single_thread": {"value": v1, "sample": "9^3 cells, 1 step (%.1f s)" % wall1}}
+    if args.copy_gib > 0:
+        out["hbm_copy"] = copy_reference(args.copy_gib)
     print(json.dumps(out))
+
+
+def copy_reference(gib):
+    """A plain device-to-device copy of `gib` GiB: bytes moved and GB/s."""
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    return {"bytes_read": 8 * n, "bytes_written": 8 * n, "ms": ms, "GBps": 16 * n / (ms * 1e-3) / 1e9}
 
 
 if __name__ == "__main__":

@@ -68,6 +68,9 @@ EXPORTS = [
     "exa_fv_riemann_solver", "exa_fv_grid_create", "exa_fv_grid_destroy", "exa_fv_grid_upload",
     "exa_fv_grid_download", "exa_fv_grid_set_initial_conditions", "exa_fv_grid_compute_dt", "exa_fv_grid_step",
     "exa_fv_grid_sync", "exa_limiter_project_on_fv", "exa_limiter_project_on_dg", "exa_limiter_dmp",
+    "exa_amr_face_unknowns_prolongation", "exa_amr_face_unknowns_restriction",
+    "exa_amr_volume_unknowns_prolongation", "exa_amr_volume_unknowns_restriction",
+    "exa_amr_prolongation_batched", "exa_amr_restriction_batched",
 ]
 
 
@@ -124,6 +127,12 @@ def load_library(path=LIB_PATH):
     L.exa_limiter_project_on_fv.argtypes = [I, _D, I, _D]
     L.exa_limiter_project_on_dg.argtypes = [I, _D, I, _D]
     L.exa_limiter_dmp.argtypes = [I, _D, I, C.c_double, C.c_double, _D, _D, P(I)]
+    for nm in ("exa_amr_face_unknowns_prolongation", "exa_amr_face_unknowns_restriction"):
+        getattr(L, nm).argtypes = [I, I, I, _D, _D, _D, _D, I, I, P(I)]
+    for nm in ("exa_amr_volume_unknowns_prolongation", "exa_amr_volume_unknowns_restriction"):
+        getattr(L, nm).argtypes = [I, I, I, _D, _D, I, I, P(I)]
+    for nm in ("exa_amr_prolongation_batched", "exa_amr_restriction_batched"):
+        getattr(L, nm).argtypes = [I, I, I, C.c_void_p, LL, C.c_void_p, LL, P(AmrTransfer), I]
     _lib = L
     return L
 
@@ -446,3 +455,100 @@ def discreteMaximumPrincipleAndMinAndMaxSearch(luh, order, bmin, bmax, relax, sc
     _check(L.exa_limiter_dmp(order, _p(np.ascontiguousarray(luh, np.float64)), nobs, float(relax), float(scaling),
                              _p(bmin), _p(bmax), C.byref(ok)))
     return bool(ok.value), bmin, bmax
+
+
+# ---------------------------------------------------------------------------
+# AMR level transfers (kernels/aderdg/generic/c/3d/amrRoutines.cpph)
+
+class AmrTransfer(C.Structure):
+    _fields_ = [("fine", C.c_longlong), ("coarse", C.c_longlong), ("levels", C.c_int), ("index", C.c_int * 3)]
+
+
+def _iarr(v, n):
+    a = (C.c_int * n)()
+    for i in range(n):
+        a[i] = int(v[i])
+    return a
+
+
+def faceUnknownsProlongation(lQhbndCoarse, lFhbndCoarse, coarseGridLevel, fineGridLevel, subfaceIndex, order,
+                             nvar=5, npar=0):
+    """Fine face (lQhbnd, lFhbnd) of one sub-face; amrRoutines.cpph:45-144."""
+    L = load_library()
+    _check(L.exa_init(0))
+    B2 = (order + 1) ** 2
+    qf, ff = np.zeros(B2 * (nvar + npar)), np.zeros(B2 * nvar)
+    _check(L.exa_amr_face_unknowns_prolongation(order, nvar, npar, _p(qf), _p(ff),
+                                                _p(np.ascontiguousarray(lQhbndCoarse, np.float64)),
+                                                _p(np.ascontiguousarray(lFhbndCoarse, np.float64)),
+                                                coarseGridLevel, fineGridLevel, _iarr(subfaceIndex, 2)))
+    return qf, ff
+
+
+def faceUnknownsRestriction(lQhbndCoarse, lFhbndCoarse, lQhbndFine, lFhbndFine, coarseGridLevel, fineGridLevel,
+                            subfaceIndex, order, nvar=5, npar=0):
+    """Returns copies of the coarse face data with the fine face's restriction added."""
+    L = load_library()
+    _check(L.exa_init(0))
+    qc = np.array(lQhbndCoarse, dtype=np.float64, copy=True)
+    fc = np.array(lFhbndCoarse, dtype=np.float64, copy=True)
+    _check(L.exa_amr_face_unknowns_restriction(order, nvar, npar, _p(qc), _p(fc),
+                                               _p(np.ascontiguousarray(lQhbndFine, np.float64)),
+                                               _p(np.ascontiguousarray(lFhbndFine, np.float64)),
+                                               coarseGridLevel, fineGridLevel, _iarr(subfaceIndex, 2)))
+    return qc, fc
+
+
+def volumeUnknownsProlongation(luhCoarse, coarseGridLevel, fineGridLevel, subcellIndex, order, nvar=5, npar=0):
+    L = load_library()
+    _check(L.exa_init(0))
+    out = np.zeros((order + 1) ** 3 * (nvar + npar))
+    _check(L.exa_amr_volume_unknowns_prolongation(order, nvar, npar, _p(out),
+                                                  _p(np.ascontiguousarray(luhCoarse, np.float64)),
+                                                  coarseGridLevel, fineGridLevel, _iarr(subcellIndex, 3)))
+    return out
+
+
+def volumeUnknownsRestriction(luhCoarse, luhFine, coarseGridLevel, fineGridLevel, subcellIndex, order, nvar=5,
+                              npar=0):
+    L = load_library()
+    _check(L.exa_init(0))
+    out = np.array(luhCoarse, dtype=np.float64, copy=True)
+    _check(L.exa_amr_volume_unknowns_restriction(order, nvar, npar, _p(out),
+                                                 _p(np.ascontiguousarray(luhFine, np.float64)),
+                                                 coarseGridLevel, fineGridLevel, _iarr(subcellIndex, 3)))
+    return out
+
+
+def _transfers(lst):
+    arr = (AmrTransfer * max(1, len(lst)))()
+    for i, (fine, coarse, levels, idx) in enumerate(lst):
+        arr[i].fine, arr[i].coarse, arr[i].levels = int(fine), int(coarse), int(levels)
+        for a in range(len(idx)):
+            arr[i].index[a] = int(idx[a])
+    return arr
+
+
+def _dev(t):
+    """(pointer, number of doubles) of a contiguous float64 device tensor."""
+    if not (getattr(t, "is_cuda", False) and str(t.dtype) == "torch.float64" and t.is_contiguous()):
+        raise ValueError("expected a contiguous float64 device tensor")
+    return C.c_void_p(t.data_ptr()), t.numel()
+
+
+def amrProlongationBatched(dim, order, ndata, fine, coarse, transfers):
+    """Batched prolongations between device tensors; transfers = [(fine_off, coarse_off, levels, index)]."""
+    L = load_library()
+    _check(L.exa_init(0))
+    pf, nf = _dev(fine)
+    pc, nc = _dev(coarse)
+    _check(L.exa_amr_prolongation_batched(dim, order, ndata, pf, nf, pc, nc, _transfers(transfers), len(transfers)))
+
+
+def amrRestrictionBatched(dim, order, ndata, coarse, fine, transfers):
+    """Batched restrictions (added into coarse); runs of one coarse offset must be consecutive."""
+    L = load_library()
+    _check(L.exa_init(0))
+    pf, nf = _dev(fine)
+    pc, nc = _dev(coarse)
+    _check(L.exa_amr_restriction_batched(dim, order, ndata, pc, nc, pf, nf, _transfers(transfers), len(transfers)))

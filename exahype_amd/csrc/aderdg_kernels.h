@@ -286,7 +286,7 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
     for (int t = 0; t < B; ++t)
 #pragma unroll
       for (int v = 0; v < V; ++v) q[t][v] = (double)qs[t][v];
-    it += picard_sweeps<N, double, true, false>(q, u, sm, sm + 3 * V * B3, red, act, s, z, y, x, dt, dx0, dx1, dx2,
+    it += picard_sweeps<N, double, true, DEBUG>(q, u, sm, sm + 3 * V * B3, red, act, s, z, y, x, dt, dx0, dx1, dx2,
                                                 gamma, fp64_sweeps, 1e-7 * 1e-7, fh, dbg);
   } else {
 #pragma unroll

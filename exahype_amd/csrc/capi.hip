@@ -58,7 +58,10 @@ static hipError_t set_pred_attrs(int bytes) {
   e = hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::k_predictor<N, true, false, WPE>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::k_predictor<N, false, true, WPE>),
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::k_predictor<N, false, true, WPE>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::k_predictor<N, true, true, WPE>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
@@ -228,7 +231,7 @@ extern "C" int exa_aderdg_space_time_predictor(exa_solver* s, double* lQhbnd, do
                          s->iters.as<int>(), s->dt.as<double>(), dx[0], dx[1], dx[2], gamma, s->cfg.fp64_sweeps,
                          nullptr, dbg);
     else
-      hipLaunchKernelGGL((exa::k_predictor<NN, false, true, 1>), dim3(1), dim3(O::NT), O::LDS_DOUBLES * sizeof(double),
+      hipLaunchKernelGGL((exa::k_predictor<NN, true, true, 1>), dim3(1), dim3(O::NT), O::LDS_DOUBLES * sizeof(double),
                          s->stream, s->luh.as<double>(), s->lduh.as<double>(), s->qf.as<double>(), s->ff.as<double>(),
                          s->iters.as<int>(), s->dt.as<double>(), dx[0], dx[1], dx[2], gamma, s->cfg.fp64_sweeps,
                          nullptr, dbg);
@@ -636,8 +639,8 @@ extern "C" int exa_grid_set_dt(exa_grid* g, double dt) {
 // (two waves per SIMD trade a few spilled registers for latency hiding).
 static int predictor_waves_per_simd(const exa_aderdg_config& c, int N) {
   if (c.waves_per_simd == 1 || c.waves_per_simd == 2) return c.waves_per_simd;
-  (void)N;
-  return 1;
+  // measured at 81^3 cells: two waves per SIMD is 1.6x faster at N=5
+  return N <= 5 ? 2 : 1;
 }
 
 static int enqueue_step(exa_grid* g, bool timed) {

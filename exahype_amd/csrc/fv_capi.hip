@@ -361,7 +361,7 @@ extern "C" int exa_fv_grid_step(exa_fv_grid* g, int nsteps) {
     hipLaunchKernelGGL(k_fv_reset, dim3(1), dim3(1), 0, g->stream, g->adm());
     EXA_TRY(launch_fv_update(g->s, g->stream, g->np, g->cur.as<double>(), g->nxt.as<double>(), g->w.as<double>(),
                              nullptr, g->adm(), g->dx, g->dt_cur()));
-    std::swap(g->cur, g->nxt);
+    g->cur.swap(g->nxt);
     EXA_TRY(launch_fv_dt(g->s, g->stream, g->np, g->cur.as<double>(), nullptr, g->dt_next(), g->dx));
     hipLaunchKernelGGL(k_fv_dt_advance, dim3(1), dim3(1), 0, g->stream, g->dt_cur(), g->dt_next(), g->time());
     EXA_CHECK(hipGetLastError());

@@ -18,10 +18,36 @@
     if (r__ != EXA_OK) return r__; \
   } while (0)
 
-// Device buffer that grows on demand and frees itself.
+// Device buffer that grows on demand and frees itself.  Move-only: a copy
+// would free the same allocation twice.
 struct DevBuf {
   void* p = nullptr;
   size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipFree(p);
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  void swap(DevBuf& o) noexcept {
+    void* tp = p;
+    size_t tn = n;
+    p = o.p;
+    n = o.n;
+    o.p = tp;
+    o.n = tn;
+  }
   int ensure(size_t bytes) {
     if (bytes <= n) return EXA_OK;
     if (p) (void)hipFree(p);

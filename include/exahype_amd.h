@@ -252,6 +252,61 @@ int exa_limiter_project_on_dg(int order, const double* lim, int ghost, double* l
 int exa_limiter_dmp(int order, const double* luh, int nobs, double relax, double scaling, double* bmin,
                     double* bmax, int* admissible);
 
+/* ---- AMR level transfers (orders 0..9, 1..10 levels) ---------------------
+ * Between a coarse cell/face and a descendant fineGridLevel - coarseGridLevel
+ * levels finer; the sub-index of the descendant counts 0..3^levels-1 per axis
+ * (x first).  Prolongation overwrites the fine data, restriction ADDS the L2
+ * projection of the fine data into the coarse data, as the reference does.
+ * Face arrays [a][b][n] (b is sub-index axis 0), volume arrays [z][y][x][n]. */
+
+/* faceUnknownsProlongation (E/solvers/ADERDGSolver.h:1229; kernel
+ * kernels/aderdg/generic/c/3d/amrRoutines.cpph:45-144; called from
+ * E/solvers/ADERDGSolver.cpp:2315) */
+int exa_amr_face_unknowns_prolongation(int order, int nvar, int npar, double* lQhbndFine, double* lFhbndFine,
+                                       const double* lQhbndCoarse, const double* lFhbndCoarse, int coarseGridLevel,
+                                       int fineGridLevel, const int subfaceIndex[2]);
+/* faceUnknownsRestriction (E/solvers/ADERDGSolver.h:1244; kernel
+ * kernels/aderdg/generic/c/3d/amrRoutines.cpph:182-258; called from
+ * E/solvers/ADERDGSolver.cpp:2438) */
+int exa_amr_face_unknowns_restriction(int order, int nvar, int npar, double* lQhbndCoarse, double* lFhbndCoarse,
+                                      const double* lQhbndFine, const double* lFhbndFine, int coarseGridLevel,
+                                      int fineGridLevel, const int subfaceIndex[2]);
+/* volumeUnknownsProlongation (E/solvers/ADERDGSolver.h:1259; kernel
+ * kernels/aderdg/generic/c/3d/amrRoutines.cpph:289-374; called from
+ * E/solvers/ADERDGSolver.cpp:1415) */
+int exa_amr_volume_unknowns_prolongation(int order, int nvar, int npar, double* luhFine, const double* luhCoarse,
+                                         int coarseGridLevel, int fineGridLevel, const int subcellIndex[3]);
+/* volumeUnknownsRestriction (E/solvers/ADERDGSolver.h:1272; kernel
+ * kernels/aderdg/generic/c/3d/amrRoutines.cpph:410-493; called from
+ * E/solvers/ADERDGSolver.cpp:1688) */
+int exa_amr_volume_unknowns_restriction(int order, int nvar, int npar, double* luhCoarse, const double* luhFine,
+                                        int coarseGridLevel, int fineGridLevel, const int subcellIndex[3]);
+
+/* One transfer of a batch: block offsets in doubles into the fine and coarse
+ * device arrays, the level difference and the sub-index (index[2] unused for
+ * faces). */
+typedef struct {
+  long long fine;
+  long long coarse;
+  int levels;
+  int index[3];
+} exa_amr_transfer;
+
+/* Batched forms of the four transfers above over DEVICE arrays (d_fine of
+ * n_fine doubles, d_coarse of n_coarse doubles), dim 2 = faces, 3 = volumes,
+ * ndata values per node; the transfer list is a host array.  One workgroup per
+ * output block: prolongations must write disjoint fine blocks; restrictions
+ * into one coarse block must be consecutive in the list and are added in list
+ * order.  Input and output arrays must not overlap.  Synchronous.  (No single
+ * counterpart in the reference, which calls the per-cell kernels from its
+ * mesh traversal, E/solvers/ADERDGSolver.cpp.) */
+int exa_amr_prolongation_batched(int dim, int order, int ndata, double* d_fine, long long n_fine,
+                                 const double* d_coarse, long long n_coarse, const exa_amr_transfer* transfers,
+                                 int count);
+int exa_amr_restriction_batched(int dim, int order, int ndata, double* d_coarse, long long n_coarse,
+                                const double* d_fine, long long n_fine, const exa_amr_transfer* transfers,
+                                int count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -122,6 +122,18 @@ void orc_project_on_dg(int N, int nvar, int ghost, const double* lim, double* lu
 void orc_cell_min_max(int N, int nvar, int nobs, const double* luh, double* mn, double* mx);
 int orc_dmp(int N, int nvar, int nobs, const double* luh, double relax, double scaling, double* bmin, double* bmax);
 
+/* AMR level transfers, kernels/aderdg/generic/c/3d/amrRoutines.cpph */
+void orc_face_unknowns_prolongation(int N, int nvar, int npar, double* lQhbndFine, double* lFhbndFine,
+                                    const double* lQhbndCoarse, const double* lFhbndCoarse, int coarseLevel,
+                                    int fineLevel, const int subfaceIndex[2]);
+void orc_face_unknowns_restriction(int N, int nvar, int npar, double* lQhbndCoarse, double* lFhbndCoarse,
+                                   const double* lQhbndFine, const double* lFhbndFine, int coarseLevel,
+                                   int fineLevel, const int subfaceIndex[2]);
+void orc_volume_unknowns_prolongation(int N, int ndata, double* luhFine, const double* luhCoarse, int coarseLevel,
+                                      int fineLevel, const int subcellIndex[3]);
+void orc_volume_unknowns_restriction(int N, int ndata, double* luhCoarse, const double* luhFine, int coarseLevel,
+                                     int fineLevel, const int subcellIndex[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -102,6 +102,11 @@ def lib():
         L.orc_cell_min_max.argtypes = [C.c_int, C.c_int, C.c_int, _D, _D, _D]
         L.orc_dmp.argtypes = [C.c_int, C.c_int, C.c_int, _D, C.c_double, C.c_double, _D, _D]
         L.orc_dmp.restype = C.c_int
+        _I = C.POINTER(C.c_int)
+        for nm in ("orc_face_unknowns_prolongation", "orc_face_unknowns_restriction"):
+            getattr(L, nm).argtypes = [C.c_int, C.c_int, C.c_int, _D, _D, _D, _D, C.c_int, C.c_int, _I]
+        for nm in ("orc_volume_unknowns_prolongation", "orc_volume_unknowns_restriction"):
+            getattr(L, nm).argtypes = [C.c_int, C.c_int, _D, _D, C.c_int, C.c_int, _I]
         _lib = L
     return _lib
 
@@ -330,3 +335,43 @@ def dmp(N, luh, bmin, bmax, relax, scaling, nobs=5, nvar=5):
     bmin, bmax = dvec(bmin).copy(), dvec(bmax).copy()
     ok = lib().orc_dmp(N, nvar, nobs, ptr(dvec(luh)), relax, scaling, ptr(bmin), ptr(bmax))
     return bool(ok), bmin, bmax
+
+
+def _ivec(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def face_unknowns_prolongation(N, qc, fc, coarse_level, fine_level, subface, nvar=5, npar=0):
+    """Fine face (Q, F) from the coarse face; amrRoutines.cpph:45-144."""
+    B2 = (N + 1) ** 2
+    qf, ff = np.zeros(B2 * (nvar + npar)), np.zeros(B2 * nvar)
+    idx, ip = _ivec(subface)
+    lib().orc_face_unknowns_prolongation(N, nvar, npar, ptr(qf), ptr(ff), ptr(dvec(qc)), ptr(dvec(fc)),
+                                         coarse_level, fine_level, ip)
+    return qf, ff
+
+
+def face_unknowns_restriction(N, qc, fc, qf, ff, coarse_level, fine_level, subface, nvar=5, npar=0):
+    """Adds the restriction of the fine face into copies of (qc, fc); amrRoutines.cpph:182-258."""
+    qc, fc = dvec(qc).copy(), dvec(fc).copy()
+    idx, ip = _ivec(subface)
+    lib().orc_face_unknowns_restriction(N, nvar, npar, ptr(qc), ptr(fc), ptr(dvec(qf)), ptr(dvec(ff)),
+                                        coarse_level, fine_level, ip)
+    return qc, fc
+
+
+def volume_unknowns_prolongation(N, luh_coarse, coarse_level, fine_level, subcell, ndata=5):
+    """amrRoutines.cpph:289-374"""
+    out = np.zeros((N + 1) ** 3 * ndata)
+    idx, ip = _ivec(subcell)
+    lib().orc_volume_unknowns_prolongation(N, ndata, ptr(out), ptr(dvec(luh_coarse)), coarse_level, fine_level, ip)
+    return out
+
+
+def volume_unknowns_restriction(N, luh_coarse, luh_fine, coarse_level, fine_level, subcell, ndata=5):
+    """Adds into a copy of luh_coarse; amrRoutines.cpph:410-493"""
+    out = dvec(luh_coarse).copy()
+    idx, ip = _ivec(subcell)
+    lib().orc_volume_unknowns_restriction(N, ndata, ptr(out), ptr(dvec(luh_fine)), coarse_level, fine_level, ip)
+    return out
