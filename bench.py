@@ -47,9 +47,10 @@ def vortex_ic_kwargs(seed=0):
                 phi=rng.uniform(0, 2 * np.pi, 8))
 
 
-def run_gpu(order, n, steps, warmup, precision, waves_per_simd=0):
+def run_gpu(order, n, steps, warmup, precision, waves_per_simd=0, contraction=0):
     import exahype_amd as E
-    s = E.ADERDGSolver(order, precision=precision, fp64_sweeps=1, waves_per_simd=waves_per_simd)
+    s = E.ADERDGSolver(order, precision=precision, fp64_sweeps=1, waves_per_simd=waves_per_simd,
+                       contraction=contraction)
     g = E.Grid(s, n, n, n)
     g.set_initial_conditions(E.make_ic(0, **vortex_ic_kwargs()))
     g.compute_dt()
@@ -108,6 +109,8 @@ def main():
     ap.add_argument("--cells", type=int, default=81)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--waves-per-simd", type=int, default=0, help="predictor occupancy target (0 = tuned)")
+    ap.add_argument("--contraction", type=int, default=0,
+                    help="predictor sweep contraction: 0 tuned, 1 pencil owners, 2 node owners")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also time a plain device copy of this many GiB (HBM reference, counter calibration)")
     args = ap.parse_args()
@@ -121,9 +124,9 @@ def main():
     ncells = n ** 3
     dof = ncells * NVAR * (N + 1) ** 3
 
-    r64 = run_gpu(N, n, args.steps, args.warmup, E.PREC_FP64, args.waves_per_simd)
+    r64 = run_gpu(N, n, args.steps, args.warmup, E.PREC_FP64, args.waves_per_simd, args.contraction)
     rate64 = dof * args.steps / r64["wall"]
-    rmx = run_gpu(N, n, args.steps, args.warmup, E.PREC_FP32, args.waves_per_simd)
+    rmx = run_gpu(N, n, args.steps, args.warmup, E.PREC_FP32, args.waves_per_simd, args.contraction)
     ratemx = dof * args.steps / rmx["wall"]
     l2 = l2_per_variable(rmx["state"], r64["state"], N, ncells)
     mixed_ok = bool(l2.max() <= 1e-6)
@@ -193,20 +196,18 @@ def main():
 
 
 def copy_reference(gib):
-    """A plain device-to-device copy of `gib` GiB: bytes moved and GB/s."""
-    import torch
+    """A plain streaming device-to-device copy of `gib` GiB by the library's
+    copy kernel: bytes moved and GB/s (HBM reference, counter calibration)."""
+    import exahype_amd as E
     n = int(gib * (1 << 30)) // 8
-    a = torch.ones(n, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    b.copy_(a)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    b.copy_(a)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1)
-    return {"bytes_read": 8 * n, "bytes_written": 8 * n, "ms": ms, "GBps": 16 * n / (ms * 1e-3) / 1e9}
+    a = E.DeviceArray(n)
+    b = E.DeviceArray(n)
+    E.device_copy(b, a, 8 * n)
+    ms = min(E.device_copy(b, a, 8 * n) for _ in range(3))
+    a.close()
+    b.close()
+    return {"kernel": "k_stream_copy", "bytes_read": 8 * n, "bytes_written": 8 * n, "ms": ms,
+            "GBps": 16 * n / (ms * 1e-3) / 1e9}
 
 
 if __name__ == "__main__":

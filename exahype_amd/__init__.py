@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(HERE, "_lib", "libexahype_amd.so")
 EXA_OK = 0
 PREC_FP64 = 0
 PREC_FP32 = 1
+CONTRACT_DEFAULT, CONTRACT_PENCIL, CONTRACT_NODE = 0, 1, 2
 PDE_EULER = 0
 PDE_EULERFV2D = 1
 FV_GODUNOV = 0
@@ -36,7 +37,7 @@ _lib = None
 class AderdgConfig(C.Structure):
     _fields_ = [("order", C.c_int), ("nvar", C.c_int), ("npar", C.c_int), ("cfl", C.c_double),
                 ("gamma", C.c_double), ("pde", C.c_int), ("precision", C.c_int), ("fp64_sweeps", C.c_int),
-                ("waves_per_simd", C.c_int)]
+                ("waves_per_simd", C.c_int), ("contraction", C.c_int)]
 
 
 class FVConfig(C.Structure):
@@ -71,6 +72,7 @@ EXPORTS = [
     "exa_amr_face_unknowns_prolongation", "exa_amr_face_unknowns_restriction",
     "exa_amr_volume_unknowns_prolongation", "exa_amr_volume_unknowns_restriction",
     "exa_amr_prolongation_batched", "exa_amr_restriction_batched",
+    "exa_device_malloc", "exa_device_free", "exa_memcpy_htod", "exa_memcpy_dtoh", "exa_device_copy",
 ]
 
 
@@ -127,6 +129,11 @@ def load_library(path=LIB_PATH):
     L.exa_limiter_project_on_fv.argtypes = [I, _D, I, _D]
     L.exa_limiter_project_on_dg.argtypes = [I, _D, I, _D]
     L.exa_limiter_dmp.argtypes = [I, _D, I, C.c_double, C.c_double, _D, _D, P(I)]
+    L.exa_device_malloc.argtypes = [P(C.c_void_p), LL]
+    L.exa_device_free.argtypes = [C.c_void_p]
+    L.exa_memcpy_htod.argtypes = [C.c_void_p, C.c_void_p, LL]
+    L.exa_memcpy_dtoh.argtypes = [C.c_void_p, C.c_void_p, LL]
+    L.exa_device_copy.argtypes = [C.c_void_p, C.c_void_p, LL, P(C.c_float)]
     for nm in ("exa_amr_face_unknowns_prolongation", "exa_amr_face_unknowns_restriction"):
         getattr(L, nm).argtypes = [I, I, I, _D, _D, _D, _D, I, I, P(I)]
     for nm in ("exa_amr_volume_unknowns_prolongation", "exa_amr_volume_unknowns_restriction"):
@@ -180,10 +187,11 @@ class ADERDGSolver:
     """Per-cell kernel surface of exahype::solvers::ADERDGSolver on the GPU."""
 
     def __init__(self, order, nvar=5, npar=0, cfl=0.9, gamma=1.4, precision=PREC_FP64, fp64_sweeps=1,
-                 waves_per_simd=0, device=0):
+                 waves_per_simd=0, contraction=0, device=0):
         L = load_library()
         _check(L.exa_init(device))
-        self.cfg = AderdgConfig(order, nvar, npar, cfl, gamma, PDE_EULER, precision, fp64_sweeps, waves_per_simd)
+        self.cfg = AderdgConfig(order, nvar, npar, cfl, gamma, PDE_EULER, precision, fp64_sweeps, waves_per_simd,
+                                contraction)
         h = C.c_void_p()
         _check(L.exa_aderdg_create(C.byref(self.cfg), C.byref(h)))
         self._h = h
@@ -529,11 +537,55 @@ def _transfers(lst):
     return arr
 
 
-def _dev(t):
-    """(pointer, number of doubles) of a contiguous float64 device tensor."""
-    if not (getattr(t, "is_cuda", False) and str(t.dtype) == "torch.float64" and t.is_contiguous()):
-        raise ValueError("expected a contiguous float64 device tensor")
-    return C.c_void_p(t.data_ptr()), t.numel()
+class DeviceArray:
+    """float64 array in device memory of the current device (exa_device_malloc)."""
+
+    def __init__(self, n_or_host):
+        L = load_library()
+        _check(L.exa_init(0))
+        host = None if np.isscalar(n_or_host) else np.ascontiguousarray(n_or_host, np.float64).ravel()
+        self.n = int(n_or_host) if host is None else host.size
+        p = C.c_void_p()
+        _check(L.exa_device_malloc(C.byref(p), 8 * max(self.n, 1)))
+        self.ptr = p
+        if host is not None and self.n:
+            _check(L.exa_memcpy_htod(self.ptr, host.ctypes.data_as(C.c_void_p), 8 * self.n))
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, np.float64).ravel()
+        if host.size != self.n:
+            raise ValueError("size mismatch")
+        _check(_lib.exa_memcpy_htod(self.ptr, host.ctypes.data_as(C.c_void_p), 8 * self.n))
+
+    def numpy(self):
+        out = np.empty(self.n)
+        if self.n:
+            _check(_lib.exa_memcpy_dtoh(out.ctypes.data_as(C.c_void_p), self.ptr, 8 * self.n))
+        return out
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            _lib.exa_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_copy(dst, src, nbytes):
+    """Streaming device copy (library kernel); returns its duration in ms."""
+    ms = C.c_float(0.0)
+    _check(load_library().exa_device_copy(dst.ptr, src.ptr, int(nbytes), C.byref(ms)))
+    return ms.value
+
+
+def _dev(a):
+    if not isinstance(a, DeviceArray):
+        raise ValueError("expected a DeviceArray")
+    return a.ptr, a.n
 
 
 def amrProlongationBatched(dim, order, ndata, fine, coarse, transfers):

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libexahype_amd.so")
-SOURCES = ["capi.hip", "fv_capi.hip", "limiter_capi.hip", "amr_capi.hip"]
+SOURCES = ["capi.hip", "fv_capi.hip", "limiter_capi.hip", "amr_capi.hip", "device_capi.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -21,7 +21,7 @@ template <int N>
 struct Ord {
   static constexpr int B = N + 1, B2 = B * B, B3 = B2 * B, B4 = B2 * B2;
   static constexpr int NT = round_up64(B3);          // threads per cell
-  static constexpr int LDS_DOUBLES = 6 * V * B3 + 16;  // flux slice + partials / averages + reduction scratch
+  static constexpr int LDS_DOUBLES = 6 * V * B3 + 3 * B3 + 16;  // flux slices + pencil constants + reduction scratch
 };
 
 // ---------------------------------------------------------------------------
@@ -232,6 +232,151 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
 }
 
 // ---------------------------------------------------------------------------
+// The same Picard sweeps with one barrier per time slice ("node-owner"
+// contraction).  Per slice t:
+//   F  node owners evaluate the flux, scale F_d by the pencil constant
+//      c_d = ((w_t w_a) w_b) dt/dx_d of their own pencil (a table in LDS) and
+//      store c_d F_d in node order [z][y][x] into buffer t&1;
+//   C  after the barrier every node owner contracts its three pencils itself:
+//      P_d = sum_n (c_d F_d)[pencil, n] Kxi[n][l_d] -- lanes of one pencil read
+//      the same LDS words (broadcast), x-neighbours read consecutive words --
+//      and accumulates the time contraction as before.
+// The products and sums are the ones the pencil-owner form evaluates, in the
+// same order.  Buffers alternate between slices, so slice t+1's stores never
+// meet slice t's loads and a single barrier per slice suffices.
+// sF: 2 buffers of 3*V*B^3; sc: c_d table [d][t][a*B+b].
+template <int N, typename T, bool ACC_FH, bool DEBUG>
+__device__ inline int picard_sweeps_node(T (&q)[Ord<N>::B][V], const double (&u)[V], T* sF, const T* sc,
+                                         double* red, bool act, int s, int z, int y, int x, double gamma,
+                                         int max_sweeps, double tol2, double (&fh)[3][V], PredictorDebug dbg) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
+  constexpr bool DP = std::is_same<T, double>::value;
+  const DGTables& Td = c_tab[N];
+  const DGTablesF& Tf = c_tabf[N];
+  auto W = [&](int i) -> T { return DP ? (T)Td.w[i] : (T)Tf.w[i]; };
+  auto KXI = [&](int a, int b) -> T { return DP ? (T)Td.Kxi[a][b] : (T)Tf.Kxi[a][b]; };
+  auto IK1 = [&](int a, int b) -> T { return DP ? (T)Td.iK1[a][b] : (T)Tf.iK1[a][b]; };
+  auto F0 = [&](int i) -> T { return DP ? (T)Td.F0[i] : (T)Tf.F0[i]; };
+
+  T uu[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) uu[v] = (T)u[v];
+  const T w3 = W(z) * W(y) * W(x);
+  const T iw3 = T(1.0) / w3;
+  const T tg = (T)gamma;
+  const int px = (z * B + y) * B, py = z * B2 + x, pz = y * B + x;  // pencil bases
+  const int cxo = z * B + y, cyo = z * B + x, czo = y * B + x;      // pencil (a,b) of this node
+
+  int it = 0;
+  while (it < max_sweeps) {
+    ++it;
+    T qn[B][V];
+#pragma unroll
+    for (int t = 0; t < B; ++t)
+#pragma unroll
+      for (int v = 0; v < V; ++v) qn[t][v] = T(0);
+#pragma unroll
+    for (int t = 0; t < B; ++t) {
+      T* buf = sF + (t & 1) * 3 * V * B3;
+      if (act) {
+        T F[3][V];
+        EulerPDE::flux<T>(q[t], F, tg);
+        const T cx = sc[(0 * B + t) * B2 + cxo], cy = sc[(1 * B + t) * B2 + cyo], cz = sc[(2 * B + t) * B2 + czo];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          buf[(0 * V + v) * B3 + s] = cx * F[0][v];
+          buf[(1 * V + v) * B3 + s] = cy * F[1][v];
+          buf[(2 * V + v) * B3 + s] = cz * F[2][v];
+        }
+      }
+      __syncthreads();
+      if (act) {
+        const T c0 = w3 * F0(t);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const T* bx = buf + (0 * V + v) * B3 + px;
+          const T* by = buf + (1 * V + v) * B3 + py;
+          const T* bz = buf + (2 * V + v) * B3 + pz;
+          T ax = T(0), ay = T(0), az = T(0);
+#pragma unroll
+          for (int n = 0; n < B; ++n) ax += bx[n] * KXI(n, x);
+#pragma unroll
+          for (int n = 0; n < B; ++n) ay += by[n * B] * KXI(n, y);
+#pragma unroll
+          for (int n = 0; n < B; ++n) az += bz[n * B2] * KXI(n, z);
+          const T r = ((c0 * uu[v] - ax) - ay) - az;
+          const T ri = iw3 * r;
+#pragma unroll
+          for (int t2 = 0; t2 < B; ++t2) qn[t2][v] += ri * IK1(t2, t);
+        }
+      }
+    }
+    double res = 0.0;
+    if (act) {
+      T rt = T(0);
+#pragma unroll
+      for (int t = 0; t < B; ++t)
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const T dd = q[t][v] - qn[t][v];
+          rt += dd * dd;
+        }
+      res = (double)rt;
+    }
+    res = block_sum(res, red);
+    const bool done = res < tol2 || it == max_sweeps;
+    if (done && ACC_FH && act) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int v = 0; v < V; ++v) fh[d][v] = 0.0;
+#pragma unroll
+      for (int t = 0; t < B; ++t) {
+        T F[3][V];
+        EulerPDE::flux<T>(q[t], F, tg);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            fh[d][v] += (double)F[d][v] * Td.w[t];
+            if (DEBUG) dbg.lFi[((t * B3 + s) * 4 + d) * V + v] = (double)F[d][v];
+          }
+        if (DEBUG)
+#pragma unroll
+          for (int v = 0; v < V; ++v) dbg.lFi[((t * B3 + s) * 4 + 3) * V + v] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < B; ++t)
+#pragma unroll
+      for (int v = 0; v < V; ++v) q[t][v] = qn[t][v];
+    if (done) break;
+  }
+  return it;
+}
+
+// Pencil constants c_d(t, a, b) = ((w_t w_a) w_b) dt / dx_d of the sweeps,
+// [d][t][a*B+b], in the sweep's arithmetic type.
+template <int N, typename T>
+__device__ inline void pencil_constants(T* sc, double dt, double dx0, double dx1, double dx2) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2;
+  constexpr bool DP = std::is_same<T, double>::value;
+  const DGTables& Td = c_tab[N];
+  const DGTablesF& Tf = c_tabf[N];
+  const T tdx[3] = {(T)dx0, (T)dx1, (T)dx2};
+  const T tdt = (T)dt;
+  for (int i = threadIdx.x; i < 3 * B * B2; i += blockDim.x) {
+    const int d = i / (B * B2), t = (i / B2) % B, a = (i % B2) / B, b = i % B;
+    const T wt = DP ? (T)Td.w[t] : (T)Tf.w[t];
+    const T wa = DP ? (T)Td.w[a] : (T)Tf.w[a];
+    const T wb = DP ? (T)Td.w[b] : (T)Tf.w[b];
+    sc[i] = ((wt * wa) * wb) * tdt / tdx[d];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Space-time predictor + time averages + face extrapolation + volume
 // integral.  spaceTimePredictorNonlinear.cpph:43-577 and
 // volumeIntegralNonlinear.cpph:26-125 fused into one pass over the cell.
@@ -240,7 +385,7 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
 // reaches the FP32 rounding floor of the cell, then `fp64_sweeps` FP64
 // correction sweeps (stopping early on the FP64 criterion); the averages,
 // extrapolation and volume integral are FP64 from the last FP64 sweep.
-template <int N, bool DEBUG, bool MIXED, int WPE>
+template <int N, bool DEBUG, bool MIXED, int WPE, bool NODE>
 __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_predictor(
     const double* __restrict__ luh, double* __restrict__ lduh, double* __restrict__ qf,
     double* __restrict__ ff, int* __restrict__ iters, const double* __restrict__ dtp, double dx0,
@@ -249,7 +394,9 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
   using O = Ord<N>;
   constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
   extern __shared__ double sm[];
-  double* red = sm + 6 * V * B3;   // reduction scratch
+  double* scd = sm + 6 * V * B3;                        // pencil constants, FP64 sweeps
+  float* scf = reinterpret_cast<float*>(sm + 3 * V * B3);  // pencil constants, FP32 sweeps
+  double* red = sm + 6 * V * B3 + 3 * B3;                 // reduction scratch
   const DGTables& T = c_tab[N];
 
   const size_t cell = blockIdx.x;
@@ -257,6 +404,11 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
   const bool act = s < B3;
   const int z = act ? s / B2 : 0, y = act ? (s / B) % B : 0, x = act ? s % B : 0;
   const double dt = *dtp;
+  if (NODE) {
+    pencil_constants<N, double>(scd, dt, dx0, dx1, dx2);
+    if (MIXED) pencil_constants<N, float>(scf, dt, dx0, dx1, dx2);
+    __syncthreads();
+  }
 
   double u[V];
 #pragma unroll
@@ -279,22 +431,34 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
     um = -block_min(-um, red);
     const double floor_lp = (double)(V * O::B4) * (4.0 * 5.96e-8 * um) * (4.0 * 5.96e-8 * um);
     const double tol_lp = fmax(1e-7 * 1e-7, floor_lp);
-    it = picard_sweeps<N, float, false, false>(qs, u, reinterpret_cast<float*>(sm),
-                                               reinterpret_cast<float*>(sm + 3 * V * B3), red, act, s, z, y, x, dt,
-                                               dx0, dx1, dx2, gamma, MaxIterations, tol_lp, fh, dbg);
+    if (NODE)
+      it = picard_sweeps_node<N, float, false, false>(qs, u, reinterpret_cast<float*>(sm), scf, red, act, s, z, y, x,
+                                                      gamma, MaxIterations, tol_lp, fh, dbg);
+    else
+      it = picard_sweeps<N, float, false, false>(qs, u, reinterpret_cast<float*>(sm),
+                                                 reinterpret_cast<float*>(sm + 3 * V * B3), red, act, s, z, y, x, dt,
+                                                 dx0, dx1, dx2, gamma, MaxIterations, tol_lp, fh, dbg);
 #pragma unroll
     for (int t = 0; t < B; ++t)
 #pragma unroll
       for (int v = 0; v < V; ++v) q[t][v] = (double)qs[t][v];
-    it += picard_sweeps<N, double, true, DEBUG>(q, u, sm, sm + 3 * V * B3, red, act, s, z, y, x, dt, dx0, dx1, dx2,
-                                                gamma, fp64_sweeps, 1e-7 * 1e-7, fh, dbg);
+    if (NODE)
+      it += picard_sweeps_node<N, double, true, DEBUG>(q, u, sm, scd, red, act, s, z, y, x, gamma, fp64_sweeps,
+                                                       1e-7 * 1e-7, fh, dbg);
+    else
+      it += picard_sweeps<N, double, true, DEBUG>(q, u, sm, sm + 3 * V * B3, red, act, s, z, y, x, dt, dx0, dx1, dx2,
+                                                  gamma, fp64_sweeps, 1e-7 * 1e-7, fh, dbg);
   } else {
 #pragma unroll
     for (int t = 0; t < B; ++t)
 #pragma unroll
       for (int v = 0; v < V; ++v) q[t][v] = u[v];
-    it = picard_sweeps<N, double, true, DEBUG>(q, u, sm, sm + 3 * V * B3, red, act, s, z, y, x, dt, dx0, dx1, dx2,
-                                               gamma, MaxIterations, 1e-7 * 1e-7, fh, dbg);
+    if (NODE)
+      it = picard_sweeps_node<N, double, true, DEBUG>(q, u, sm, scd, red, act, s, z, y, x, gamma, MaxIterations,
+                                                      1e-7 * 1e-7, fh, dbg);
+    else
+      it = picard_sweeps<N, double, true, DEBUG>(q, u, sm, sm + 3 * V * B3, red, act, s, z, y, x, dt, dx0, dx1, dx2,
+                                                 gamma, MaxIterations, 1e-7 * 1e-7, fh, dbg);
   }
 
   // time averages (aderPredictorNonlinear :326-397)

@@ -50,27 +50,30 @@ extern "C" const char* exa_version(void) { return "exahype_amd 0.1 gfx950"; }
 static std::mutex g_init_mu;
 static unsigned long long g_init_mask = 0;
 
-template <int N, int WPE>
+template <int N, int WPE, bool NODE>
 static hipError_t set_pred_attrs(int bytes) {
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::k_predictor<N, false, false, WPE>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::k_predictor<N, true, false, WPE>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::k_predictor<N, false, true, WPE>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::k_predictor<N, true, true, WPE>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  const void* ks[4] = {reinterpret_cast<const void*>(&exa::k_predictor<N, false, false, WPE, NODE>),
+                       reinterpret_cast<const void*>(&exa::k_predictor<N, false, true, WPE, NODE>),
+                       WPE == 1 ? reinterpret_cast<const void*>(&exa::k_predictor<N, true, false, 1, NODE>) : nullptr,
+                       WPE == 1 ? reinterpret_cast<const void*>(&exa::k_predictor<N, true, true, 1, NODE>) : nullptr};
+  for (const void* k : ks) {
+    if (!k) continue;
+    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 template <int N>
 static hipError_t set_lds_attrs() {
   const int bytes = exa::Ord<N>::LDS_DOUBLES * (int)sizeof(double);
-  hipError_t e = set_pred_attrs<N, 1>(bytes);
+  hipError_t e = set_pred_attrs<N, 1, false>(bytes);
   if (e != hipSuccess) return e;
-  e = set_pred_attrs<N, 2>(bytes);
+  e = set_pred_attrs<N, 2, false>(bytes);
+  if (e != hipSuccess) return e;
+  e = set_pred_attrs<N, 1, true>(bytes);
+  if (e != hipSuccess) return e;
+  e = set_pred_attrs<N, 2, true>(bytes);
   if (e != hipSuccess) return e;
   return hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::k_volume_integral_ref<N>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
@@ -144,6 +147,15 @@ struct exa_solver {
   DevBuf luh, lduh, qf, ff, iters, dt, dbg, misc;
 };
 
+// Contraction form of the predictor's Picard sweeps: EXA_CONTRACT_PENCIL
+// (two barriers per time slice) or EXA_CONTRACT_NODE (one barrier per slice).
+static bool predictor_node_contraction(const exa_aderdg_config& c, int N) {
+  if (c.contraction == EXA_CONTRACT_PENCIL) return false;
+  if (c.contraction == EXA_CONTRACT_NODE) return true;
+  (void)N;
+  return false;
+}
+
 extern "C" int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out) {
   EXA_ARG(cfg && out, "null pointer");
   EXA_ARG(cfg->order >= 1 && cfg->order <= 7, "order must be 1..7");
@@ -152,6 +164,7 @@ extern "C" int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out)
   EXA_ARG(cfg->cfl > 0.0 && cfg->gamma > 1.0, "cfl/gamma");
   EXA_ARG(cfg->precision == EXA_PREC_FP64 || cfg->precision == EXA_PREC_FP32, "precision");
   EXA_ARG(cfg->waves_per_simd >= 0 && cfg->waves_per_simd <= 2, "waves_per_simd must be 0, 1 or 2");
+  EXA_ARG(cfg->contraction >= 0 && cfg->contraction <= 2, "contraction must be 0, 1 or 2");
   int dev = 0;
   EXA_CHECK(hipGetDevice(&dev));
   EXA_TRY(exa_init(dev));
@@ -225,16 +238,14 @@ extern "C" int exa_aderdg_space_time_predictor(exa_solver* s, double* lQhbnd, do
   const double gamma = s->cfg.gamma;
   EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
-    if (s->cfg.precision == EXA_PREC_FP64)
-      hipLaunchKernelGGL((exa::k_predictor<NN, true, false, 1>), dim3(1), dim3(O::NT), O::LDS_DOUBLES * sizeof(double),
-                         s->stream, s->luh.as<double>(), s->lduh.as<double>(), s->qf.as<double>(), s->ff.as<double>(),
-                         s->iters.as<int>(), s->dt.as<double>(), dx[0], dx[1], dx[2], gamma, s->cfg.fp64_sweeps,
-                         nullptr, dbg);
-    else
-      hipLaunchKernelGGL((exa::k_predictor<NN, true, true, 1>), dim3(1), dim3(O::NT), O::LDS_DOUBLES * sizeof(double),
-                         s->stream, s->luh.as<double>(), s->lduh.as<double>(), s->qf.as<double>(), s->ff.as<double>(),
-                         s->iters.as<int>(), s->dt.as<double>(), dx[0], dx[1], dx[2], gamma, s->cfg.fp64_sweeps,
-                         nullptr, dbg);
+    const bool mixed = s->cfg.precision != EXA_PREC_FP64;
+    const bool node = predictor_node_contraction(s->cfg, NN);
+    auto kern = mixed ? (node ? &exa::k_predictor<NN, true, true, 1, true> : &exa::k_predictor<NN, true, true, 1, false>)
+                      : (node ? &exa::k_predictor<NN, true, false, 1, true>
+                              : &exa::k_predictor<NN, true, false, 1, false>);
+    hipLaunchKernelGGL(kern, dim3(1), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), s->stream, s->luh.as<double>(),
+                       s->lduh.as<double>(), s->qf.as<double>(), s->ff.as<double>(), s->iters.as<int>(),
+                       s->dt.as<double>(), dx[0], dx[1], dx[2], gamma, s->cfg.fp64_sweeps, nullptr, dbg);
   });
   EXA_CHECK(hipGetLastError());
   std::vector<double> hq(6 * V * B2), hf(6 * V * B2), hdbg(nQi + nFi + nQhi + nFhi);
@@ -666,8 +677,15 @@ static int enqueue_step(exa_grid* g, bool timed) {
     using O = exa::Ord<NN>;
     const bool mixed = g->s->cfg.precision != EXA_PREC_FP64;
     const int wpe = predictor_waves_per_simd(g->s->cfg, NN);
-    auto kern = mixed ? (wpe == 2 ? &exa::k_predictor<NN, false, true, 2> : &exa::k_predictor<NN, false, true, 1>)
-                      : (wpe == 2 ? &exa::k_predictor<NN, false, false, 2> : &exa::k_predictor<NN, false, false, 1>);
+    const bool node = predictor_node_contraction(g->s->cfg, NN);
+    auto kern = node ? (mixed ? (wpe == 2 ? &exa::k_predictor<NN, false, true, 2, true>
+                                          : &exa::k_predictor<NN, false, true, 1, true>)
+                              : (wpe == 2 ? &exa::k_predictor<NN, false, false, 2, true>
+                                          : &exa::k_predictor<NN, false, false, 1, true>))
+                     : (mixed ? (wpe == 2 ? &exa::k_predictor<NN, false, true, 2, false>
+                                          : &exa::k_predictor<NN, false, true, 1, false>)
+                              : (wpe == 2 ? &exa::k_predictor<NN, false, false, 2, false>
+                                          : &exa::k_predictor<NN, false, false, 1, false>));
     hipLaunchKernelGGL(kern, dim3((unsigned)g->nc), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), g->stream,
                        g->luh.as<double>(), g->lduh.as<double>(), g->qf.as<double>(), g->ff.as<double>(),
                        g->iters.as<int>(), g->dt_cur(), g->dx[0], g->dx[1], g->dx[2], g->s->cfg.gamma,

@@ -60,6 +60,18 @@ int exa_init(int device);
 /* Waits for all work on the current device (every stream). */
 int exa_device_synchronize(void);
 
+/* Device memory of the current device for callers without their own HIP
+ * allocator (the batched entry points take device pointers).  No counterpart
+ * in the reference, whose cell data lives in host heaps. */
+int exa_device_malloc(void** ptr, long long bytes);
+int exa_device_free(void* ptr);
+int exa_memcpy_htod(void* dst, const void* src, long long bytes);
+int exa_memcpy_dtoh(void* dst, const void* src, long long bytes);
+/* Device-to-device copy by a plain streaming kernel (32-byte aligned
+ * pointers); *ms receives its event-timed duration.  The HBM reference of
+ * the benchmarks. */
+int exa_device_copy(void* dst, const void* src, long long bytes, float* ms);
+
 /* ---------------------------------------------------------------------- */
 /* Solver handle: the compile-time constants of the reference's generated
  * Abstract<Solver> (TT/AbstractGenericADERDGSolverHeader.template:31-34). */
@@ -75,7 +87,12 @@ typedef struct {
   int precision;    /* EXA_PREC_*                                  */
   int fp64_sweeps;  /* EXA_PREC_FP32: FP64 Picard sweeps after the FP32 phase (>=1) */
   int waves_per_simd; /* predictor occupancy target: 0 = tuned default per order, 1 or 2 */
+  int contraction;  /* EXA_CONTRACT_*: form of the sweeps' spatial contraction */
 } exa_aderdg_config;
+
+#define EXA_CONTRACT_DEFAULT 0  /* tuned per order                                        */
+#define EXA_CONTRACT_PENCIL 1   /* pencil owners contract, two barriers per time slice    */
+#define EXA_CONTRACT_NODE 2     /* node owners contract their pencils, one barrier per slice */
 
 int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out);
 int exa_aderdg_destroy(exa_solver* s);

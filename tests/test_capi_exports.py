@@ -31,10 +31,9 @@ def test_library_exports_every_declared_symbol():
 
 def test_no_gpu_fails_loudly():
     """Without a GPU the product refuses to run instead of falling back."""
-    import torch
     import exahype_amd
-    if torch.cuda.is_available():
-        pytest.skip("GPU present")
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("GPU driver present")
     if not os.path.exists(exahype_amd.LIB_PATH):
         pytest.skip("library not built")
     with pytest.raises(RuntimeError):

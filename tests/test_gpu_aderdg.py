@@ -106,9 +106,10 @@ def test_golden_volume_surface_riemann_update(s3p1, golden):
 
 
 # ---------------------------------------------------------------- per-cell vs oracle
+@pytest.mark.parametrize("contraction", [1, 2])
 @pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7])
-def test_predictor_vs_oracle(N):
-    s = exa().ADERDGSolver(N)
+def test_predictor_vs_oracle(N, contraction):
+    s = exa().ADERDGSolver(N, contraction=contraction)
     c = O.cfg(N)
     B = N + 1
     for seed in range(2):
@@ -169,10 +170,11 @@ def _ic_kw(seed):
                 phi=rng.uniform(0, 6.28, 8))
 
 
+@pytest.mark.parametrize("contraction", [1, 2])
 @pytest.mark.parametrize("N,n,steps", [(3, 6, 5), (5, 4, 5), (2, 5, 3), (7, 2, 2)])
-def test_grid_steps_vs_oracle(N, n, steps):
+def test_grid_steps_vs_oracle(N, n, steps, contraction):
     E = exa()
-    s = E.ADERDGSolver(N)
+    s = E.ADERDGSolver(N, contraction=contraction)
     c = O.cfg(N)
     og = O.OracleGrid(c, n, n + 1, n - 1)
     og.init(O.make_ic(0, **_ic_kw(11)))

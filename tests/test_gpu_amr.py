@@ -75,7 +75,6 @@ def test_batched_volume_all_children(N):
     """27 prolongations of one coarse cell, then the 27 restrictions back into a
     zeroed coarse cell as one consecutive run: matches the oracle's sequential
     accumulation and returns the coarse data."""
-    import torch
     import exahype_amd as E
     rng = np.random.default_rng(N)
     nd = 5
@@ -83,25 +82,24 @@ def test_batched_volume_all_children(N):
     coarse = rng.normal(size=blk)
     kids = list(itertools.product(range(3), repeat=3))
     tr = [(i * blk, 0, 1, (k[2], k[1], k[0])) for i, k in enumerate(kids)]
-    d_coarse = torch.tensor(coarse, dtype=torch.float64, device="cuda")
-    d_fine = torch.zeros(len(kids) * blk, dtype=torch.float64, device="cuda")
+    d_coarse = E.DeviceArray(coarse)
+    d_fine = E.DeviceArray(np.zeros(len(kids) * blk))
     E.amrProlongationBatched(3, N, nd, d_fine, d_coarse, tr)
-    fine = d_fine.cpu().numpy()
+    fine = d_fine.numpy()
     for i, k in enumerate(kids):
         o = O.volume_unknowns_prolongation(N, coarse, 0, 1, (k[2], k[1], k[0]), ndata=nd)
         assert close(fine[i * blk:(i + 1) * blk], o)
-    d_back = torch.zeros(blk, dtype=torch.float64, device="cuda")
+    d_back = E.DeviceArray(np.zeros(blk))
     E.amrRestrictionBatched(3, N, nd, d_back, d_fine, tr)
     acc = np.zeros(blk)
     for i, k in enumerate(kids):
         acc = O.volume_unknowns_restriction(N, acc, fine[i * blk:(i + 1) * blk], 0, 1, (k[2], k[1], k[0]), ndata=nd)
-    back = d_back.cpu().numpy()
+    back = d_back.numpy()
     assert close(back, acc)
     assert np.abs(back - coarse).max() <= 1e-10 * np.abs(coarse).max()
 
 
 def test_batched_faces_many_pairs_and_checks():
-    import torch
     import exahype_amd as E
     N, nd, levels = 4, 5, 2
     rng = np.random.default_rng(11)
@@ -110,10 +108,10 @@ def test_batched_faces_many_pairs_and_checks():
     coarse = rng.normal(size=ncoarse * blk)
     fine_in = rng.normal(size=nfine * blk)
     tr = [(i * blk, (i % ncoarse) * blk, levels, tuple(rng.integers(0, 9, 2))) for i in range(nfine)]
-    d_c = torch.tensor(coarse, dtype=torch.float64, device="cuda")
-    d_f = torch.zeros(nfine * blk, dtype=torch.float64, device="cuda")
+    d_c = E.DeviceArray(coarse)
+    d_f = E.DeviceArray(np.zeros(nfine * blk))
     E.amrProlongationBatched(2, N, nd, d_f, d_c, tr)
-    f = d_f.cpu().numpy()
+    f = d_f.numpy()
     for i, (fo, co, lv, ix) in enumerate(tr):
         o, _ = O.face_unknowns_prolongation(N, coarse[co:co + blk], np.zeros(blk),
                                             0, lv, ix, nvar=nd, npar=0)
@@ -121,15 +119,15 @@ def test_batched_faces_many_pairs_and_checks():
     # restrictions grouped by coarse block (consecutive runs), accumulated in list order
     order = sorted(range(nfine), key=lambda i: tr[i][1])
     rtr = [tr[i] for i in order]
-    d_f2 = torch.tensor(fine_in, dtype=torch.float64, device="cuda")
-    d_c2 = torch.tensor(coarse, dtype=torch.float64, device="cuda")
+    d_f2 = E.DeviceArray(fine_in)
+    d_c2 = E.DeviceArray(coarse)
     E.amrRestrictionBatched(2, N, nd, d_c2, d_f2, rtr)
     exp = coarse.copy()
     for fo, co, lv, ix in rtr:
         q, _ = O.face_unknowns_restriction(N, exp[co:co + blk], np.zeros((N + 1) ** 2 * nd), fine_in[fo:fo + blk],
                                            np.zeros((N + 1) ** 2 * nd), 0, lv, ix, nvar=nd, npar=0)
         exp[co:co + blk] = q
-    assert close(d_c2.cpu().numpy(), exp)
+    assert close(d_c2.numpy(), exp)
     # a coarse block split into two runs, and a block past the end, are refused
     with pytest.raises(RuntimeError):
         E.amrRestrictionBatched(2, N, nd, d_c2, d_f2, tr)
