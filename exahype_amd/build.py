@@ -13,7 +13,7 @@ LIB = os.path.join(OUT_DIR, "libexahype_amd.so")
 SOURCES = ["capi.hip", "fv_capi.hip", "limiter_capi.hip", "amr_capi.hip", "device_capi.hip"]
 
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
@@ -26,10 +26,25 @@ def hipcc():
             return c
 
 
-def build(verbose=False):
-    os.makedirs(OUT_DIR, exist_ok=True)
+def build(verbose=False, jobs=None):
+    """Compiles every source to an object (in parallel), then links the
+    shared library; always rebuilds."""
+    from concurrent.futures import ThreadPoolExecutor
+    obj_dir = os.path.join(OUT_DIR, "obj")
+    os.makedirs(obj_dir, exist_ok=True)
+    objs = [os.path.join(obj_dir, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+
+    def compile_one(i):
+        cmd = [hipcc()] + HIPCC_FLAGS + ["-I", CSRC, "-c", os.path.join(CSRC, SOURCES[i]), "-o", objs[i]]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+
+    jobs = jobs or min(len(SOURCES), max(1, min(8, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(compile_one, range(len(SOURCES))))
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", CSRC] + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", tmp]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)

@@ -97,8 +97,10 @@ struct PredictorDebug {
 //      layouts (x: [z][y][x], y: [z][x][y], z: [y][x][z]), so every pencil is
 //      one contiguous run of B values;
 //   P  pencil owners (3*V*B^2 items: direction, variable, pencil) read their
-//      pencil once and produce all B outputs of the contraction with Kxi,
-//      whose entries are uniform across the workgroup (scalar operands);
+//      pencil once, scale it by the pencil constant c_d = ((w_t w_a) w_b)
+//      dt/dx_d (a table in LDS), and produce all B outputs of the contraction
+//      with Kxi, whose entries are uniform across the workgroup (scalar
+//      operands), stored in the same direction layout;
 //   A  node owners combine rhs = w^3 F0[t] u - Px - Py - Pz and accumulate the
 //      thread-local time contraction qn[t'] += (rhs / w^3) iK1[t'][t].
 // T is the arithmetic type of the sweep: double for the plain method and the
@@ -106,10 +108,9 @@ struct PredictorDebug {
 // predictor.  Returns the sweeps executed; stops once
 // sum((q_new-q_old)^2) < tol2 over the cell (block-uniform decision).
 template <int N, typename T, bool ACC_FH, bool DEBUG>
-__device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], T* sF, T* sP, double* red,
-                                    bool act, int s, int z, int y, int x, double dt, double dx0, double dx1,
-                                    double dx2, double gamma, int max_sweeps, double tol2, double (&fh)[3][V],
-                                    PredictorDebug dbg) {
+__device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], T* sF, T* sP, const T* sc,
+                                    double* red, bool act, int s, int z, int y, int x, double gamma, int max_sweeps,
+                                    double tol2, double (&fh)[3][V], PredictorDebug dbg) {
   using O = Ord<N>;
   constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
   constexpr int NITEMS = 3 * V * B2;
@@ -126,9 +127,8 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
   for (int v = 0; v < V; ++v) uu[v] = (T)u[v];
   const T w3 = W(z) * W(y) * W(x);
   const T iw3 = T(1.0) / w3;
-  const T tdt = (T)dt, tg = (T)gamma;
-  const T tdx[3] = {(T)dx0, (T)dx1, (T)dx2};
-  // flux store offsets of this node in the three direction layouts
+  const T tg = (T)gamma;
+  // offsets of this node in the three direction layouts (flux and partials)
   const int ox = (z * B + y) * B + x, oy = (z * B + x) * B + y, oz = (y * B + x) * B + z;
 
   int it = 0;
@@ -153,23 +153,23 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
         }
       }
       __syncthreads();
-      // phase P
-      const T wt = W(t);
+      // phase P: item = (d*V + v, pencil ab); pencils and partials are
+      // contiguous runs of B in the direction layouts, so the item needs no
+      // direction-dependent addressing
       for (int i = s; i < NITEMS; i += O::NT) {
-        const int d = i / (V * B2), v = (i / B2) % V, ab = i % B2, a = ab / B, b = ab % B;
-        const T c = ((wt * W(a)) * W(b)) * tdt / tdx[d];
-        const T* pf = sF + (d * V + v) * B3 + ab * B;
+        const int dv = i / B2, ab = i - dv * B2, d = dv / V;
+        const T c = sc[(d * B + t) * B2 + ab];
+        const T* pf = sF + dv * B3 + ab * B;
         T f[B];
 #pragma unroll
         for (int n = 0; n < B; ++n) f[n] = c * pf[n];
-        T* pp = sP + (d * V + v) * B3;
+        T* pp = sP + dv * B3 + ab * B;
 #pragma unroll
         for (int l = 0; l < B; ++l) {
           T acc = T(0);
 #pragma unroll
           for (int n = 0; n < B; ++n) acc += f[n] * KXI(n, l);
-          const int node = d == 0 ? ab * B + l : (d == 1 ? (a * B + l) * B + b : (l * B + a) * B + b);
-          pp[node] = acc;
+          pp[l] = acc;
         }
       }
       __syncthreads();
@@ -178,7 +178,7 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
         const T c0 = w3 * F0(t);
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-          const T r = ((c0 * uu[v] - sP[(0 * V + v) * B3 + s]) - sP[(1 * V + v) * B3 + s]) - sP[(2 * V + v) * B3 + s];
+          const T r = ((c0 * uu[v] - sP[(0 * V + v) * B3 + ox]) - sP[(1 * V + v) * B3 + oy]) - sP[(2 * V + v) * B3 + oz];
           const T ri = iw3 * r;
 #pragma unroll
           for (int t2 = 0; t2 < B; ++t2) qn[t2][v] += ri * IK1(t2, t);
@@ -404,11 +404,9 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
   const bool act = s < B3;
   const int z = act ? s / B2 : 0, y = act ? (s / B) % B : 0, x = act ? s % B : 0;
   const double dt = *dtp;
-  if (NODE) {
-    pencil_constants<N, double>(scd, dt, dx0, dx1, dx2);
-    if (MIXED) pencil_constants<N, float>(scf, dt, dx0, dx1, dx2);
-    __syncthreads();
-  }
+  pencil_constants<N, double>(scd, dt, dx0, dx1, dx2);
+  if (MIXED) pencil_constants<N, float>(scf, dt, dx0, dx1, dx2);
+  __syncthreads();
 
   double u[V];
 #pragma unroll
@@ -436,8 +434,8 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
                                                       gamma, MaxIterations, tol_lp, fh, dbg);
     else
       it = picard_sweeps<N, float, false, false>(qs, u, reinterpret_cast<float*>(sm),
-                                                 reinterpret_cast<float*>(sm + 3 * V * B3), red, act, s, z, y, x, dt,
-                                                 dx0, dx1, dx2, gamma, MaxIterations, tol_lp, fh, dbg);
+                                                 reinterpret_cast<float*>(sm) + 3 * V * B3, scf, red, act, s, z, y,
+                                                 x, gamma, MaxIterations, tol_lp, fh, dbg);
 #pragma unroll
     for (int t = 0; t < B; ++t)
 #pragma unroll
@@ -446,8 +444,8 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
       it += picard_sweeps_node<N, double, true, DEBUG>(q, u, sm, scd, red, act, s, z, y, x, gamma, fp64_sweeps,
                                                        1e-7 * 1e-7, fh, dbg);
     else
-      it += picard_sweeps<N, double, true, DEBUG>(q, u, sm, sm + 3 * V * B3, red, act, s, z, y, x, dt, dx0, dx1, dx2,
-                                                  gamma, fp64_sweeps, 1e-7 * 1e-7, fh, dbg);
+      it += picard_sweeps<N, double, true, DEBUG>(q, u, sm, sm + 3 * V * B3, scd, red, act, s, z, y, x, gamma,
+                                                  fp64_sweeps, 1e-7 * 1e-7, fh, dbg);
   } else {
 #pragma unroll
     for (int t = 0; t < B; ++t)
@@ -457,8 +455,8 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
       it = picard_sweeps_node<N, double, true, DEBUG>(q, u, sm, scd, red, act, s, z, y, x, gamma, MaxIterations,
                                                       1e-7 * 1e-7, fh, dbg);
     else
-      it = picard_sweeps<N, double, true, DEBUG>(q, u, sm, sm + 3 * V * B3, red, act, s, z, y, x, dt, dx0, dx1, dx2,
-                                                 gamma, MaxIterations, 1e-7 * 1e-7, fh, dbg);
+      it = picard_sweeps<N, double, true, DEBUG>(q, u, sm, sm + 3 * V * B3, scd, red, act, s, z, y, x, gamma,
+                                                 MaxIterations, 1e-7 * 1e-7, fh, dbg);
   }
 
   // time averages (aderPredictorNonlinear :326-397)
