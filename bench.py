@@ -167,6 +167,7 @@ def main():
         "config": {"workload": "3D Euler ADER-DG, order %d, %d^3 periodic cells, vortex + 8 density modes, CFL 0.9"
                                % (N, n), "cells": ncells, "order": N, "dof_per_step": dof,
                    "predictor": "mixed: FP32 Picard sweeps + 1 FP64 correction sweep" if mixed_ok else "fp64",
+                   "waves_per_simd": args.waves_per_simd or "tuned", "contraction": args.contraction or "tuned",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None,

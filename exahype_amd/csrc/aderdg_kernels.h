@@ -618,6 +618,11 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_update(double* __restrict__ luh,
   double dtn = 1.0e300;
   if (act) {
     double du[V], u[V];
+    // both state loads issued before any arithmetic
+    if (MODE & (UPD_UPDATE | UPD_DT)) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) u[v] = luh[cell * V * B3 + v * B3 + s];
+    }
     if (MODE & (UPD_SURFACE | UPD_UPDATE)) {
 #pragma unroll
       for (int v = 0; v < V; ++v) du[v] = lduh[cell * V * B3 + v * B3 + s];
@@ -641,10 +646,6 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_update(double* __restrict__ luh,
 #pragma unroll
       for (int v = 0; v < V; ++v) lduh[cell * V * B3 + v * B3 + s] = du[v];
     }
-    if (MODE & (UPD_UPDATE | UPD_DT)) {
-#pragma unroll
-      for (int v = 0; v < V; ++v) u[v] = luh[cell * V * B3 + v * B3 + s];
-    }
     if (MODE & UPD_UPDATE) {
       const double weight = T.w[z] * T.w[y] * T.w[x];
       const double updateSize = *dtp / weight;
@@ -655,12 +656,12 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_update(double* __restrict__ luh,
       }
     }
     if (MODE & UPD_DT) {
+      double m[3];
+      EulerPDE::max_abs_eigenvalues3(u, gamma, m);
       double den = 0.0;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const double m = EulerPDE::max_abs_eigenvalue(u, d, gamma);
-        den += m / (d == 0 ? dx0 : (d == 1 ? dx1 : dx2));
-      }
+      den += m[0] / dx0;
+      den += m[1] / dx1;
+      den += m[2] / dx2;
       dtn = cfl * exa_pnpm(N) / den;
     }
   }

@@ -152,8 +152,9 @@ struct exa_solver {
 static bool predictor_node_contraction(const exa_aderdg_config& c, int N) {
   if (c.contraction == EXA_CONTRACT_PENCIL) return false;
   if (c.contraction == EXA_CONTRACT_NODE) return true;
-  (void)N;
-  return false;
+  // measured at 81^3 cells (FP64 and mixed): pencil owners win at N=5
+  // (1.9x), node owners at N=3 (1.3x) and N=7 (1.1x)
+  return N != 5;
 }
 
 extern "C" int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out) {

@@ -74,6 +74,26 @@ struct EulerPDE {
     }
     return m;
   }
+
+  // max_abs_eigenvalue for d = 0, 1, 2 at once: 1/rho, p and c are the same
+  // values eigenvalues() computes for every direction, evaluated once.
+  __host__ __device__ static inline void max_abs_eigenvalues3(const double* Q, double gamma, double m[3]) {
+    const double irho = 1.0 / Q[0];
+    const double p = (gamma - 1.0) * (Q[4] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) * irho);
+    const double c = sqrt(gamma * p * irho);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double u_n = Q[d + 1] * irho;
+      const double lam[5] = {u_n - c, u_n, u_n, u_n, u_n + c};
+      double mm = 0.0;
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        const double a = fabs(lam[v]);
+        mm = (mm < a) ? a : mm;
+      }
+      m[d] = mm;
+    }
+  }
 };
 
 // Pointwise initial conditions (the adjustPointSolution of the configured
