@@ -28,6 +28,9 @@ sys.path.insert(0, ROOT)
 METRIC = "DoF-updates/s, 3D Euler ADER-DG N=5, mixed vs FP64 GPU at L2 err ≤1e-6"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0
+# mixed-precision variants measured: (name, precision, FP64 correction sweeps)
+MIXED_VARIANTS = [("fp32", 1, 1), ("fp32_mfma", 2, 1), ("bf16_mfma", 3, 2)]
 PHI_FLUX = 40   # flops of one 3D Euler flux evaluation (1 division counted as 1)
 NVAR = 5
 
@@ -47,9 +50,9 @@ def vortex_ic_kwargs(seed=0):
                 phi=rng.uniform(0, 2 * np.pi, 8))
 
 
-def run_gpu(order, n, steps, warmup, precision, waves_per_simd=0, contraction=0):
+def run_gpu(order, n, steps, warmup, precision, waves_per_simd=0, contraction=0, fp64_sweeps=1):
     import exahype_amd as E
-    s = E.ADERDGSolver(order, precision=precision, fp64_sweeps=1, waves_per_simd=waves_per_simd,
+    s = E.ADERDGSolver(order, precision=precision, fp64_sweeps=fp64_sweeps, waves_per_simd=waves_per_simd,
                        contraction=contraction)
     g = E.Grid(s, n, n, n)
     g.set_initial_conditions(E.make_ic(0, **vortex_ic_kwargs()))
@@ -126,25 +129,37 @@ def main():
 
     r64 = run_gpu(N, n, args.steps, args.warmup, E.PREC_FP64, args.waves_per_simd, args.contraction)
     rate64 = dof * args.steps / r64["wall"]
-    rmx = run_gpu(N, n, args.steps, args.warmup, E.PREC_FP32, args.waves_per_simd, args.contraction)
-    ratemx = dof * args.steps / rmx["wall"]
-    l2 = l2_per_variable(rmx["state"], r64["state"], N, ncells)
-    mixed_ok = bool(l2.max() <= 1e-6)
-    head = rmx if mixed_ok else r64
-    rate = ratemx if mixed_ok else rate64
+    mixed = {}
+    for name, prec, sweeps in MIXED_VARIANTS:
+        if prec >= E.PREC_FP32_MFMA and N > 7:
+            continue
+        r = run_gpu(N, n, args.steps, args.warmup, prec, args.waves_per_simd, args.contraction, sweeps)
+        l2 = l2_per_variable(r["state"], r64["state"], N, ncells)
+        r.update(rate=dof * args.steps / r["wall"], l2=l2, ok=bool(l2.max() <= 1e-6), name=name, sweeps_fp64=sweeps,
+                 prec=prec)
+        del r["state"]
+        mixed[name] = r
+    passing = [r for r in mixed.values() if r["ok"]]
+    best = max(passing, key=lambda r: r["rate"]) if passing else None
+    mixed_ok = best is not None
+    head = best if mixed_ok else r64
+    rate = best["rate"] if mixed_ok else rate64
 
     B = N + 1
     pred_ms = head["times"][0] / max(head["counts"][0], 1)
     sweeps = head["iters"]
     if mixed_ok:
-        # FP32 sweeps at the FP32 peak, one FP64 correction sweep and the FP64
-        # tail at the FP64 peak: peak = total flop / minimum time
+        # low-precision sweeps at their peak (FP32 vector ALU 157.3 TF/s with
+        # packed math, BF16 MFMA 2.5 PF/s dense), the FP64 correction sweeps
+        # and the FP64 tail at the FP64 peak: peak = total flop / minimum time
         f_sweep = 8 * NVAR * B ** 5 + B ** 4 * (PHI_FLUX + 5 * NVAR)
-        f32 = (sweeps - 1.0) * f_sweep * ncells
-        f64 = (predictor_flops_per_cell(N, sweeps) * ncells) - f32
-        flops = f32 + f64
-        peak = flops / (f32 / PEAK_FP32_TFLOPS + f64 / PEAK_FP64_TFLOPS)
-        kname = "k_predictor<%d, mixed FP32 + 1 FP64 sweep>" % N
+        n_lp = max(sweeps - best["sweeps_fp64"], 0.0)
+        f_lp = n_lp * f_sweep * ncells
+        f64 = (predictor_flops_per_cell(N, sweeps) * ncells) - f_lp
+        flops = f_lp + f64
+        peak_lp = PEAK_BF16_TFLOPS if best["prec"] == E.PREC_BF16_MFMA else PEAK_FP32_TFLOPS
+        peak = flops / (f_lp / peak_lp + f64 / PEAK_FP64_TFLOPS)
+        kname = "k_predictor<%d, %s + %d FP64 sweep(s)>" % (N, best["name"], best["sweeps_fp64"])
     else:
         flops = predictor_flops_per_cell(N, sweeps) * ncells
         peak = PEAK_FP64_TFLOPS
@@ -162,11 +177,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if mixed_ok else "f64",
+        "dtype": ({"fp32": "f32+f64", "fp32_mfma": "f32+f64", "bf16_mfma": "bf16+f32+f64"}[best["name"]]
+                  if mixed_ok else "f64"),
         "data": "synthetic",
         "config": {"workload": "3D Euler ADER-DG, order %d, %d^3 periodic cells, vortex + 8 density modes, CFL 0.9"
                                % (N, n), "cells": ncells, "order": N, "dof_per_step": dof,
-                   "predictor": "mixed: FP32 Picard sweeps + 1 FP64 correction sweep" if mixed_ok else "fp64",
+                   "predictor": ("mixed: %s Picard sweeps + %d FP64 correction sweep(s)"
+                                 % (best["name"], best["sweeps_fp64"]) if mixed_ok else "fp64"),
                    "waves_per_simd": args.waves_per_simd or "tuned", "contraction": args.contraction or "tuned",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
@@ -176,21 +193,22 @@ def main():
         "phase_ms_per_step": {"predictor": pred_ms,
                               "riemann": head["times"][1] / max(head["counts"][1], 1),
                               "update_surface_dt": head["times"][2] / max(head["counts"][2], 1)},
-        "precisions": {
-            "fp64": {"dof_updates_per_s": rate64, "ms_per_step": r64["wall"] / args.steps * 1e3,
-                     "predictor_ms": r64["times"][0] / max(r64["counts"][0], 1), "sweeps_per_cell": r64["iters"]},
-            "mixed": {"dof_updates_per_s": ratemx, "ms_per_step": rmx["wall"] / args.steps * 1e3,
-                      "predictor_ms": rmx["times"][0] / max(rmx["counts"][0], 1), "sweeps_per_cell": rmx["iters"],
-                      "l2_vs_fp64_per_variable": l2.tolist(), "speedup_vs_fp64": ratemx / rate64,
-                      "l2_ok": mixed_ok}},
+        "precisions": dict(
+            fp64={"dof_updates_per_s": rate64, "ms_per_step": r64["wall"] / args.steps * 1e3,
+                  "predictor_ms": r64["times"][0] / max(r64["counts"][0], 1), "sweeps_per_cell": r64["iters"]},
+            **{k: {"dof_updates_per_s": r["rate"], "ms_per_step": r["wall"] / args.steps * 1e3,
+                   "predictor_ms": r["times"][0] / max(r["counts"][0], 1), "sweeps_per_cell": r["iters"],
+                   "fp64_correction_sweeps": r["sweeps_fp64"], "l2_vs_fp64_per_variable": r["l2"].tolist(),
+                   "speedup_vs_fp64": r["rate"] / rate64, "l2_ok": r["ok"]} for k, r in mixed.items()}),
     }
     if not args.no_cpu_baseline:
         threads = min(len(os.sched_getaffinity(0)), 16)
-        v, wall = cpu_baseline(N, n, threads, sample_cells=27, steps=2)
-        v1, wall1 = cpu_baseline(N, n, 1, sample_cells=9, steps=1)
+        v, wall = cpu_baseline(N, n, threads, sample_cells=27, steps=30)
+        v1, wall1 = cpu_baseline(N, n, 1, sample_cells=9, steps=20)
         out["cpu_baseline"] = {"value": v, "unit": "DoF-updates/s", "cores": threads, "kind": "port",
-                               "sample": "oracle, order %d, 27^3 cells of size 1/%d, 2 steps (%.1f s wall)" % (N, n, wall),
-                               "single_thread": {"value": v1, "sample": "9^3 cells, 1 step (%.1f s)" % wall1}}
+                               "sample": "oracle (C, OpenMP), order %d, 27^3 periodic cells of size 1/%d, same initial "
+                                         "data, 30 steps (%.1f s wall)" % (N, n, wall),
+                               "single_thread": {"value": v1, "sample": "9^3 cells, 20 steps (%.1f s)" % wall1}}
     if args.copy_gib > 0:
         out["hbm_copy"] = copy_reference(args.copy_gib)
     print(json.dumps(out))

@@ -24,6 +24,8 @@ LIB_PATH = os.path.join(HERE, "_lib", "libexahype_amd.so")
 EXA_OK = 0
 PREC_FP64 = 0
 PREC_FP32 = 1
+PREC_FP32_MFMA = 2
+PREC_BF16_MFMA = 3
 CONTRACT_DEFAULT, CONTRACT_PENCIL, CONTRACT_NODE = 0, 1, 2
 PDE_EULER = 0
 PDE_EULERFV2D = 1
@@ -73,6 +75,7 @@ EXPORTS = [
     "exa_amr_volume_unknowns_prolongation", "exa_amr_volume_unknowns_restriction",
     "exa_amr_prolongation_batched", "exa_amr_restriction_batched",
     "exa_device_malloc", "exa_device_free", "exa_memcpy_htod", "exa_memcpy_dtoh", "exa_device_copy",
+    "exa_selftest_mfma",
 ]
 
 
@@ -134,6 +137,7 @@ def load_library(path=LIB_PATH):
     L.exa_memcpy_htod.argtypes = [C.c_void_p, C.c_void_p, LL]
     L.exa_memcpy_dtoh.argtypes = [C.c_void_p, C.c_void_p, LL]
     L.exa_device_copy.argtypes = [C.c_void_p, C.c_void_p, LL, P(C.c_float)]
+    L.exa_selftest_mfma.argtypes = [I, C.c_void_p, C.c_void_p, C.c_void_p]
     for nm in ("exa_amr_face_unknowns_prolongation", "exa_amr_face_unknowns_restriction"):
         getattr(L, nm).argtypes = [I, I, I, _D, _D, _D, _D, I, I, P(I)]
     for nm in ("exa_amr_volume_unknowns_prolongation", "exa_amr_volume_unknowns_restriction"):
@@ -580,6 +584,19 @@ def device_copy(dst, src, nbytes):
     ms = C.c_float(0.0)
     _check(load_library().exa_device_copy(dst.ptr, src.ptr, int(nbytes), C.byref(ms)))
     return ms.value
+
+
+def selftest_mfma(kind, A, Bm):
+    """D = A @ B through the matrix cores (kind 0 FP32, 1 BF16) with the
+    operand layouts of the mixed-precision predictor."""
+    L = load_library()
+    _check(L.exa_init(0))
+    A = np.ascontiguousarray(A, np.float32)
+    Bm = np.ascontiguousarray(Bm, np.float32)
+    D = np.zeros((16, 16), np.float32)
+    _check(L.exa_selftest_mfma(kind, A.ctypes.data_as(C.c_void_p), Bm.ctypes.data_as(C.c_void_p),
+                               D.ctypes.data_as(C.c_void_p)))
+    return D
 
 
 def _dev(a):

@@ -11,6 +11,7 @@
 #pragma once
 #include "common.h"
 #include "pde_euler.h"
+#include "mfma_util.h"
 #include <type_traits>
 
 namespace exa {
@@ -356,6 +357,171 @@ __device__ inline int picard_sweeps_node(T (&q)[Ord<N>::B][V], const double (&u)
   return it;
 }
 
+// ---------------------------------------------------------------------------
+// Low-precision Picard sweeps with the spatial contraction on the matrix
+// cores (mixed-precision predictor only).  Per time slice t:
+//   F  node owners evaluate the FP32 flux, scale F_d by their pencil's
+//      constant c_d and store it in the direction layouts (pencils contiguous);
+//   P  the 3*V*B^2 pencils, taken two at a time, are the rows of
+//      D = A * Kbd with A[m] = [pencil 2m | pencil 2m+1 | 0] (K = 16) and
+//      Kbd = blockdiag(Kxi, Kxi) (16 x 16): one 16x16 MFMA per 32 pencils,
+//      BF16 operands (v_mfma_f32_16x16x16_bf16) or FP32 operands
+//      (ceil(2B/4) x v_mfma_f32_16x16x4_f32), FP32 accumulation; row m,
+//      column n of D is output node n%B of pencil 2m + n/B, so D's valid
+//      columns are again contiguous pencils;
+//   A  node owners combine rhs and accumulate the time contraction (VALU, FP32).
+// Operand layouts of the 16x16 MFMAs (64 lanes, l = lane): A[l%16][4(l/16)+j],
+// B[4(l/16)+j][l%16], D[4(l/16)+i][l%16] (x16 variant: j = 0..3; x4 variant:
+// A[l%16][l/16], B[l/16][l%16]).
+
+// Kbd entry (k, n) for B <= 8.
+template <int B>
+__device__ inline float kbd(const float (&K)[10][10], int k, int n) {
+  if (k < B && n < B) return K[k][n];
+  if (k >= B && k < 2 * B && n >= B && n < 2 * B) return K[k - B][n - B];
+  return 0.0f;
+}
+
+template <int N, bool BF16>
+struct MfmaKxi {
+  exa_s4 b16;    // BF16: Kbd[4(l/16)+j][l%16]
+  float b32[4];  // FP32: Kbd[4s + l/16][l%16], s = 0..3
+  __device__ inline void init(int lane) {
+    constexpr int B = N + 1;
+    const DGTablesF& Tf = c_tabf[N];
+    const int n = lane & 15, kq = lane >> 4;
+    if (BF16) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b16[j] = f32_to_bf16_rne(kbd<B>(Tf.Kxi, 4 * kq + j, n));
+    } else {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) b32[s] = kbd<B>(Tf.Kxi, 4 * s + kq, n);
+    }
+  }
+};
+
+// All pencils of one slice: sF -> sP (both [dv][ab][n], FP32).
+template <int N, bool BF16>
+__device__ inline void contract_mfma(const float* sF, float* sP, const MfmaKxi<N, BF16>& kf) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2;
+  constexpr int NITEMS = 3 * V * B2;
+  constexpr int NTILES = (NITEMS + 31) / 32;
+  constexpr int NWAVES = O::NT / 64;
+  constexpr int KS = (2 * B + 3) / 4;  // FP32 K-steps of 4
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  for (int tile = wave; tile < NTILES; tile += NWAVES) {
+    const int row0 = tile * 32 + 2 * m;  // first pencil of this A row
+    // entry k of the row is node k%B of pencil row0 + k/B; pencils past the
+    // last (NITEMS may be odd) read as zero
+    const int lim = (NITEMS - row0) * B;  // entries of this row inside sF
+    const float* a = sF + row0 * B;
+    exa_f4 acc = {0.f, 0.f, 0.f, 0.f};
+    if (BF16) {
+      exa_s4 av;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k = 4 * kq + j;
+        av[j] = f32_to_bf16_rne((k < 2 * B && k < lim) ? a[k] : 0.0f);
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, kf.b16, acc, 0, 0, 0);
+    } else {
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + kq;
+        const float av = (k < 2 * B && k < lim) ? a[k] : 0.0f;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, kf.b32[s], acc, 0, 0, 0);
+      }
+    }
+    const int n = m;  // output column of this lane: node n%B of pencil 2(row) + n/B
+    if (n < 2 * B) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = tile * 32 + 2 * (4 * kq + i);
+        if (r * B + n < NITEMS * B) sP[r * B + n] = acc[i];
+      }
+    }
+  }
+}
+
+template <int N, bool BF16>
+__device__ inline int picard_sweeps_mfma(float (&q)[Ord<N>::B][V], const double (&u)[V], float* sF, float* sP,
+                                         const float* sc, double* red, bool act, int s, int z, int y, int x,
+                                         double gamma, int max_sweeps, double tol2) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
+  static_assert(B <= 8, "block-diagonal pairing needs 2B <= 16");
+  const DGTablesF& Tf = c_tabf[N];
+  MfmaKxi<N, BF16> kf;
+  kf.init(threadIdx.x & 63);
+
+  float uu[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) uu[v] = (float)u[v];
+  const float w3 = Tf.w[z] * Tf.w[y] * Tf.w[x];
+  const float iw3 = 1.0f / w3;
+  const float tg = (float)gamma;
+  const int ox = (z * B + y) * B + x, oy = (z * B + x) * B + y, oz = (y * B + x) * B + z;
+  const int abx = z * B + y, aby = z * B + x, abz = y * B + x;
+
+  int it = 0;
+  while (it < max_sweeps) {
+    ++it;
+    float qn[B][V];
+#pragma unroll
+    for (int t = 0; t < B; ++t)
+#pragma unroll
+      for (int v = 0; v < V; ++v) qn[t][v] = 0.0f;
+#pragma unroll
+    for (int t = 0; t < B; ++t) {
+      if (act) {
+        float F[3][V];
+        EulerPDE::flux<float>(q[t], F, tg);
+        const float cx = sc[(0 * B + t) * B2 + abx], cy = sc[(1 * B + t) * B2 + aby], cz = sc[(2 * B + t) * B2 + abz];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          sF[(0 * V + v) * B3 + ox] = cx * F[0][v];
+          sF[(1 * V + v) * B3 + oy] = cy * F[1][v];
+          sF[(2 * V + v) * B3 + oz] = cz * F[2][v];
+        }
+      }
+      __syncthreads();
+      contract_mfma<N, BF16>(sF, sP, kf);
+      __syncthreads();
+      if (act) {
+        const float c0 = w3 * Tf.F0[t];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const float r = ((c0 * uu[v] - sP[(0 * V + v) * B3 + ox]) - sP[(1 * V + v) * B3 + oy]) - sP[(2 * V + v) * B3 + oz];
+          const float ri = iw3 * r;
+#pragma unroll
+          for (int t2 = 0; t2 < B; ++t2) qn[t2][v] += ri * Tf.iK1[t2][t];
+        }
+      }
+    }
+    double res = 0.0;
+    if (act) {
+      float rt = 0.0f;
+#pragma unroll
+      for (int t = 0; t < B; ++t)
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const float dd = q[t][v] - qn[t][v];
+          rt += dd * dd;
+        }
+      res = (double)rt;
+    }
+    res = block_sum(res, red);
+#pragma unroll
+    for (int t = 0; t < B; ++t)
+#pragma unroll
+      for (int v = 0; v < V; ++v) q[t][v] = qn[t][v];
+    if (res < tol2) break;
+  }
+  return it;
+}
+
 // Pencil constants c_d(t, a, b) = ((w_t w_a) w_b) dt / dx_d of the sweeps,
 // [d][t][a*B+b], in the sweep's arithmetic type.
 template <int N, typename T>
@@ -376,6 +542,9 @@ __device__ inline void pencil_constants(T* sc, double dt, double dx0, double dx1
   }
 }
 
+// Arithmetic of the predictor's Picard sweeps (kernel template argument).
+enum : int { PRED_FP64 = 0, PRED_FP32 = 1, PRED_FP32_MFMA = 2, PRED_BF16_MFMA = 3 };
+
 // ---------------------------------------------------------------------------
 // Space-time predictor + time averages + face extrapolation + volume
 // integral.  spaceTimePredictorNonlinear.cpph:43-577 and
@@ -385,7 +554,7 @@ __device__ inline void pencil_constants(T* sc, double dt, double dx0, double dx1
 // reaches the FP32 rounding floor of the cell, then `fp64_sweeps` FP64
 // correction sweeps (stopping early on the FP64 criterion); the averages,
 // extrapolation and volume integral are FP64 from the last FP64 sweep.
-template <int N, bool DEBUG, bool MIXED, int WPE, bool NODE>
+template <int N, bool DEBUG, int PREC, int WPE, bool NODE>
 __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_predictor(
     const double* __restrict__ luh, double* __restrict__ lduh, double* __restrict__ qf,
     double* __restrict__ ff, int* __restrict__ iters, const double* __restrict__ dtp, double dx0,
@@ -403,6 +572,7 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
   const int s = threadIdx.x;
   const bool act = s < B3;
   const int z = act ? s / B2 : 0, y = act ? (s / B) % B : 0, x = act ? s % B : 0;
+  constexpr bool MIXED = PREC != PRED_FP64;
   const double dt = *dtp;
   pencil_constants<N, double>(scd, dt, dx0, dx1, dx2);
   if (MIXED) pencil_constants<N, float>(scf, dt, dx0, dx1, dx2);
@@ -416,26 +586,32 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
   double fh[3][V];
   constexpr int MaxIterations = 2 * (N + 1);
   int it = 0;
-  if (MIXED) {
+  if constexpr (MIXED) {
     float qs[B][V];
 #pragma unroll
     for (int t = 0; t < B; ++t)
 #pragma unroll
       for (int v = 0; v < V; ++v) qs[t][v] = (float)u[v];
-    // FP32 rounding floor of sum((dq)^2) over the cell: count * (4 eps_f * |u|max)^2
+    // rounding floor of sum((dq)^2) over the cell in the low precision:
+    // count * (4 eps * |u|max)^2, eps the unit roundoff of the contraction
+    // operands (FP32 2^-24, BF16 2^-8)
     double um = 0.0;
 #pragma unroll
     for (int v = 0; v < V; ++v) um = fmax(um, fabs(u[v]));
     um = -block_min(-um, red);
-    const double floor_lp = (double)(V * O::B4) * (4.0 * 5.96e-8 * um) * (4.0 * 5.96e-8 * um);
+    const double eps = PREC == PRED_BF16_MFMA ? 3.90625e-3 : 5.96e-8;
+    const double floor_lp = (double)(V * O::B4) * (4.0 * eps * um) * (4.0 * eps * um);
     const double tol_lp = fmax(1e-7 * 1e-7, floor_lp);
-    if (NODE)
-      it = picard_sweeps_node<N, float, false, false>(qs, u, reinterpret_cast<float*>(sm), scf, red, act, s, z, y, x,
-                                                      gamma, MaxIterations, tol_lp, fh, dbg);
+    float* sFf = reinterpret_cast<float*>(sm);
+    if constexpr (PREC == PRED_FP32_MFMA || PREC == PRED_BF16_MFMA)
+      it = picard_sweeps_mfma<N, PREC == PRED_BF16_MFMA>(qs, u, sFf, sFf + 3 * V * B3, scf, red, act, s, z, y, x,
+                                                         gamma, MaxIterations, tol_lp);
+    else if (NODE)
+      it = picard_sweeps_node<N, float, false, false>(qs, u, sFf, scf, red, act, s, z, y, x, gamma, MaxIterations,
+                                                      tol_lp, fh, dbg);
     else
-      it = picard_sweeps<N, float, false, false>(qs, u, reinterpret_cast<float*>(sm),
-                                                 reinterpret_cast<float*>(sm) + 3 * V * B3, scf, red, act, s, z, y,
-                                                 x, gamma, MaxIterations, tol_lp, fh, dbg);
+      it = picard_sweeps<N, float, false, false>(qs, u, sFf, sFf + 3 * V * B3, scf, red, act, s, z, y, x, gamma,
+                                                 MaxIterations, tol_lp, fh, dbg);
 #pragma unroll
     for (int t = 0; t < B; ++t)
 #pragma unroll

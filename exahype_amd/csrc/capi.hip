@@ -50,31 +50,43 @@ extern "C" const char* exa_version(void) { return "exahype_amd 0.1 gfx950"; }
 static std::mutex g_init_mu;
 static unsigned long long g_init_mask = 0;
 
-template <int N, int WPE, bool NODE>
-static hipError_t set_pred_attrs(int bytes) {
-  const void* ks[4] = {reinterpret_cast<const void*>(&exa::k_predictor<N, false, false, WPE, NODE>),
-                       reinterpret_cast<const void*>(&exa::k_predictor<N, false, true, WPE, NODE>),
-                       WPE == 1 ? reinterpret_cast<const void*>(&exa::k_predictor<N, true, false, 1, NODE>) : nullptr,
-                       WPE == 1 ? reinterpret_cast<const void*>(&exa::k_predictor<N, true, true, 1, NODE>) : nullptr};
-  for (const void* k : ks) {
-    if (!k) continue;
-    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e != hipSuccess) return e;
+// Predictor kernels instantiated: every precision and contraction at one
+// wave per SIMD (with and without the per-cell debug outputs) and, for
+// N <= 5, at two waves per SIMD.
+using PredFn = void (*)(const double*, double*, double*, double*, int*, const double*, double, double, double, double,
+                        int, unsigned long long*, exa::PredictorDebug);
+
+template <int N, bool DEBUG, int WPE, bool NODE>
+static PredFn pred_by_prec(int prec) {
+  switch (prec) {
+    case EXA_PREC_FP32: return &exa::k_predictor<N, DEBUG, exa::PRED_FP32, WPE, NODE>;
+    case EXA_PREC_FP32_MFMA: return &exa::k_predictor<N, DEBUG, exa::PRED_FP32_MFMA, WPE, NODE>;
+    case EXA_PREC_BF16_MFMA: return &exa::k_predictor<N, DEBUG, exa::PRED_BF16_MFMA, WPE, NODE>;
+    default: return &exa::k_predictor<N, DEBUG, exa::PRED_FP64, WPE, NODE>;
   }
-  return hipSuccess;
+}
+
+template <int N>
+static PredFn predictor_kernel(bool debug, int prec, int wpe, bool node) {
+  if (debug) return node ? pred_by_prec<N, true, 1, true>(prec) : pred_by_prec<N, true, 1, false>(prec);
+  if (N <= 5 && wpe == 2)
+    return node ? pred_by_prec<N, false, (N <= 5 ? 2 : 1), true>(prec)
+                : pred_by_prec<N, false, (N <= 5 ? 2 : 1), false>(prec);
+  return node ? pred_by_prec<N, false, 1, true>(prec) : pred_by_prec<N, false, 1, false>(prec);
 }
 
 template <int N>
 static hipError_t set_lds_attrs() {
   const int bytes = exa::Ord<N>::LDS_DOUBLES * (int)sizeof(double);
-  hipError_t e = set_pred_attrs<N, 1, false>(bytes);
-  if (e != hipSuccess) return e;
-  e = set_pred_attrs<N, 2, false>(bytes);
-  if (e != hipSuccess) return e;
-  e = set_pred_attrs<N, 1, true>(bytes);
-  if (e != hipSuccess) return e;
-  e = set_pred_attrs<N, 2, true>(bytes);
-  if (e != hipSuccess) return e;
+  const int precs[4] = {EXA_PREC_FP64, EXA_PREC_FP32, EXA_PREC_FP32_MFMA, EXA_PREC_BF16_MFMA};
+  for (int dbg = 0; dbg < 2; ++dbg)
+    for (int w = 1; w <= 2; ++w)
+      for (int node = 0; node < 2; ++node)
+        for (int p : precs) {
+          hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(predictor_kernel<N>(dbg, p, w, node)),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+          if (e != hipSuccess) return e;
+        }
   return hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::k_volume_integral_ref<N>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
@@ -163,7 +175,8 @@ extern "C" int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out)
   EXA_ARG(cfg->pde == EXA_PDE_EULER && cfg->nvar == 5, "only 3D Euler with 5 variables is compiled in");
   EXA_ARG(cfg->npar >= 0 && cfg->npar <= 3, "npar must be 0..3");
   EXA_ARG(cfg->cfl > 0.0 && cfg->gamma > 1.0, "cfl/gamma");
-  EXA_ARG(cfg->precision == EXA_PREC_FP64 || cfg->precision == EXA_PREC_FP32, "precision");
+  EXA_ARG(cfg->precision >= EXA_PREC_FP64 && cfg->precision <= EXA_PREC_BF16_MFMA, "precision");
+  EXA_ARG(cfg->precision < EXA_PREC_FP32_MFMA || cfg->order <= 7, "MFMA sweeps need order <= 7");
   EXA_ARG(cfg->waves_per_simd >= 0 && cfg->waves_per_simd <= 2, "waves_per_simd must be 0, 1 or 2");
   EXA_ARG(cfg->contraction >= 0 && cfg->contraction <= 2, "contraction must be 0, 1 or 2");
   int dev = 0;
@@ -239,11 +252,8 @@ extern "C" int exa_aderdg_space_time_predictor(exa_solver* s, double* lQhbnd, do
   const double gamma = s->cfg.gamma;
   EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
-    const bool mixed = s->cfg.precision != EXA_PREC_FP64;
     const bool node = predictor_node_contraction(s->cfg, NN);
-    auto kern = mixed ? (node ? &exa::k_predictor<NN, true, true, 1, true> : &exa::k_predictor<NN, true, true, 1, false>)
-                      : (node ? &exa::k_predictor<NN, true, false, 1, true>
-                              : &exa::k_predictor<NN, true, false, 1, false>);
+    PredFn kern = predictor_kernel<NN>(true, s->cfg.precision, 1, node);
     hipLaunchKernelGGL(kern, dim3(1), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), s->stream, s->luh.as<double>(),
                        s->lduh.as<double>(), s->qf.as<double>(), s->ff.as<double>(), s->iters.as<int>(),
                        s->dt.as<double>(), dx[0], dx[1], dx[2], gamma, s->cfg.fp64_sweeps, nullptr, dbg);
@@ -676,17 +686,9 @@ static int enqueue_step(exa_grid* g, bool timed) {
   nodbg.lQi = nodbg.lFi = nodbg.lQhi = nodbg.lFhi = nullptr;
   EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
-    const bool mixed = g->s->cfg.precision != EXA_PREC_FP64;
     const int wpe = predictor_waves_per_simd(g->s->cfg, NN);
     const bool node = predictor_node_contraction(g->s->cfg, NN);
-    auto kern = node ? (mixed ? (wpe == 2 ? &exa::k_predictor<NN, false, true, 2, true>
-                                          : &exa::k_predictor<NN, false, true, 1, true>)
-                              : (wpe == 2 ? &exa::k_predictor<NN, false, false, 2, true>
-                                          : &exa::k_predictor<NN, false, false, 1, true>))
-                     : (mixed ? (wpe == 2 ? &exa::k_predictor<NN, false, true, 2, false>
-                                          : &exa::k_predictor<NN, false, true, 1, false>)
-                              : (wpe == 2 ? &exa::k_predictor<NN, false, false, 2, false>
-                                          : &exa::k_predictor<NN, false, false, 1, false>));
+    PredFn kern = predictor_kernel<NN>(false, g->s->cfg.precision, wpe, node);
     hipLaunchKernelGGL(kern, dim3((unsigned)g->nc), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), g->stream,
                        g->luh.as<double>(), g->lduh.as<double>(), g->qf.as<double>(), g->ff.as<double>(),
                        g->iters.as<int>(), g->dt_cur(), g->dx[0], g->dx[1], g->dx[2], g->s->cfg.gamma,

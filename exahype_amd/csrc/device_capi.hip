@@ -72,3 +72,54 @@ extern "C" int exa_device_copy(void* dst, const void* src, long long bytes, floa
   if (ms) *ms = t;
   return EXA_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Self-test of the matrix-core operand layouts the predictor's low-precision
+// contraction relies on: D = A * B for 16x16 row-major A, B with the lane
+// mapping A[l%16][4(l/16)+j], B[4(l/16)+j][l%16], D[4(l/16)+i][l%16]
+// (16x16x16 BF16) and A[l%16][l/16 + 4s], B[l/16 + 4s][l%16] over four K-steps
+// (16x16x4 FP32).
+#include "mfma_util.h"
+
+namespace {
+__global__ void __launch_bounds__(64) k_mfma_selftest(int kind, const float* __restrict__ A,
+                                                      const float* __restrict__ Bm, float* __restrict__ D) {
+  const int l = threadIdx.x, r = l & 15, q = l >> 4;
+  exa::exa_f4 acc = {0.f, 0.f, 0.f, 0.f};
+  if (kind == 1) {
+    exa::exa_s4 a, b;
+    for (int j = 0; j < 4; ++j) {
+      a[j] = exa::f32_to_bf16_rne(A[r * 16 + 4 * q + j]);
+      b[j] = exa::f32_to_bf16_rne(Bm[(4 * q + j) * 16 + r]);
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  } else {
+    for (int s = 0; s < 4; ++s)
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(A[r * 16 + q + 4 * s], Bm[(q + 4 * s) * 16 + r], acc, 0, 0, 0);
+  }
+  for (int i = 0; i < 4; ++i) D[(4 * q + i) * 16 + r] = acc[i];
+}
+}  // namespace
+
+extern "C" int exa_selftest_mfma(int kind, const float* A, const float* Bm, float* D) {
+  EXA_ARG(A && Bm && D && (kind == 0 || kind == 1), "arguments");
+  int dev = 0;
+  EXA_CHECK(hipGetDevice(&dev));
+  EXA_TRY(exa_init(dev));
+  float* d = nullptr;
+  EXA_CHECK(hipMalloc(&d, 3 * 256 * sizeof(float)));
+  hipError_t e = hipMemcpy(d, A, 256 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d + 256, Bm, 256 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_mfma_selftest, dim3(1), dim3(64), 0, 0, kind, d, d + 256, d + 512);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(D, d + 512, 256 * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) {
+    exa_set_error("mfma self-test: %s", hipGetErrorString(e));
+    return EXA_ERR_HIP;
+  }
+  return EXA_OK;
+}

@@ -42,8 +42,12 @@ extern "C" {
 #define EXA_ERR_NOMEM (-4)
 
 /* predictor precision */
-#define EXA_PREC_FP64 0   /* plain method: every operation in FP64             */
-#define EXA_PREC_FP32 1   /* Picard sweeps in FP32, then FP64 correction sweeps */
+#define EXA_PREC_FP64 0       /* plain method: every operation in FP64                        */
+#define EXA_PREC_FP32 1       /* Picard sweeps in FP32 (vector ALU), then FP64 correction sweeps */
+#define EXA_PREC_FP32_MFMA 2  /* sweeps' spatial contractions on FP32 MFMA, FP32 elsewhere, then
+                                 FP64 correction sweeps                                        */
+#define EXA_PREC_BF16_MFMA 3  /* as EXA_PREC_FP32_MFMA with BF16 contraction operands (FP32
+                                 accumulation)                                                 */
 
 /* PDEs compiled into the kernels */
 #define EXA_PDE_EULER 0   /* 3D compressible Euler, 5 variables (gamma from config) */
@@ -72,6 +76,12 @@ int exa_memcpy_dtoh(void* dst, const void* src, long long bytes);
  * the benchmarks. */
 int exa_device_copy(void* dst, const void* src, long long bytes, float* ms);
 
+/* Self-test of the matrix-core operand layouts used by the mixed-precision
+ * predictor: D = A * B for 16x16 row-major FP32 matrices through one
+ * wavefront of 16x16x4 FP32 MFMAs (kind 0) or one 16x16x16 BF16 MFMA (kind 1,
+ * operands rounded to BF16).  No counterpart in the reference. */
+int exa_selftest_mfma(int kind, const float* A, const float* B, float* D);
+
 /* ---------------------------------------------------------------------- */
 /* Solver handle: the compile-time constants of the reference's generated
  * Abstract<Solver> (TT/AbstractGenericADERDGSolverHeader.template:31-34). */
@@ -85,7 +95,7 @@ typedef struct {
   double gamma;     /* ratio of specific heats                     */
   int pde;          /* EXA_PDE_*                                   */
   int precision;    /* EXA_PREC_*                                  */
-  int fp64_sweeps;  /* EXA_PREC_FP32: FP64 Picard sweeps after the FP32 phase (>=1) */
+  int fp64_sweeps;  /* mixed precisions: FP64 Picard sweeps after the low-precision phase (>=1) */
   int waves_per_simd; /* predictor occupancy target: 0 = tuned default per order, 1 or 2 */
   int contraction;  /* EXA_CONTRACT_*: form of the sweeps' spatial contraction */
 } exa_aderdg_config;

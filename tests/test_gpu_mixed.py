@@ -1,12 +1,13 @@
-"""Mixed-precision predictor: FP32 Picard sweeps + FP64 correction sweep.
+"""Mixed-precision predictor: low-precision Picard sweeps (FP32 on the vector
+ALU, or the spatial contractions on FP32 / BF16 MFMA) + FP64 correction
+sweeps.
 
 Bar (floating point, stated here): after 50 steps of the 3D Euler vortex
 setup the L2 difference to the FP64 path is <= 1e-6 for every variable
 (L2 over the domain with the Gauss-Legendre quadrature of each cell); the
-per-cell predictor outputs differ from the FP64 oracle by at most 1e-6 of the
-cell's state scale (max over variables of ||b_v||_inf).  One FP64 correction
-sweep removes all but a fraction (the Picard contraction factor) of the
-FP32-level error, so the bound is on absolute error at the state scale.
+per-cell predictor outputs differ from the FP64 oracle by at most 5e-6 of the
+output's scale.  BF16 runs two FP64 correction sweeps (its low-precision
+fixed point is only ~1e-3 accurate), the FP32 forms one.
 """
 import numpy as np
 import pytest
@@ -31,10 +32,33 @@ def l2_per_variable(a, b, N, ncells, h3):
     return np.sqrt(np.einsum("n,cnv->v", w3, d ** 2) * h3)
 
 
-@pytest.mark.parametrize("N", [3, 5])
-def test_mixed_predictor_vs_oracle(N):
+PRECS = [(1, 1), (2, 1), (3, 2)]  # (precision, FP64 correction sweeps)
+
+
+def test_mfma_operand_layouts():
+    """The lane layouts the matrix-core contraction assumes, against numpy."""
     import exahype_amd as E
-    s = E.ADERDGSolver(N, precision=E.PREC_FP32, fp64_sweeps=1)
+    rng = np.random.default_rng(3)
+    A = rng.normal(size=(16, 16)).astype(np.float32)
+    Bm = rng.normal(size=(16, 16)).astype(np.float32)
+    D = E.selftest_mfma(0, A, Bm)
+    ref = A.astype(np.float64) @ Bm.astype(np.float64)
+    assert np.abs(D - ref).max() <= 1e-5 * np.abs(ref).max()
+
+    def bf16(x):
+        u = x.astype(np.float32).view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+        return u.astype(np.uint32).view(np.float32)
+    D = E.selftest_mfma(1, A, Bm)
+    ref = bf16(A).astype(np.float64) @ bf16(Bm).astype(np.float64)
+    assert np.abs(D - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("prec,sweeps", PRECS)
+@pytest.mark.parametrize("N", [3, 5])
+def test_mixed_predictor_vs_oracle(N, prec, sweeps):
+    import exahype_amd as E
+    s = E.ADERDGSolver(N, precision=prec, fp64_sweeps=sweeps)
     c = O.cfg(N)
     rng = np.random.default_rng(1)
     ic = O.make_ic(0, **_ic_kw(3))
@@ -56,12 +80,13 @@ def test_mixed_predictor_vs_oracle(N):
             assert err <= 5e-6, (N, k, err)
 
 
-@pytest.mark.parametrize("N,n", [(3, 12), (5, 9)])
-def test_mixed_50_steps_l2_vs_fp64(N, n):
+@pytest.mark.parametrize("prec,sweeps", PRECS)
+@pytest.mark.parametrize("N,n", [(3, 12), (5, 9), (7, 5)])
+def test_mixed_50_steps_l2_vs_fp64(N, n, prec, sweeps):
     import exahype_amd as E
     out = []
-    for prec in (E.PREC_FP64, E.PREC_FP32):
-        s = E.ADERDGSolver(N, precision=prec, fp64_sweeps=1)
+    for p, sw in ((E.PREC_FP64, 1), (prec, sweeps)):
+        s = E.ADERDGSolver(N, precision=p, fp64_sweeps=sw)
         g = E.Grid(s, n, n, n)
         g.set_initial_conditions(E.make_ic(0, **_ic_kw(7)))
         g.compute_dt()
