@@ -42,6 +42,11 @@ class AderdgConfig(C.Structure):
                 ("waves_per_simd", C.c_int), ("contraction", C.c_int)]
 
 
+class EnsembleMember(C.Structure):
+    _fields_ = [("order", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("offset", C.c_double * 3), ("width", C.c_double * 3), ("gamma", C.c_double)]
+
+
 class FVConfig(C.Structure):
     _fields_ = [("dim", C.c_int), ("patch", C.c_int), ("scheme", C.c_int), ("nvar", C.c_int), ("npar", C.c_int),
                 ("cfl", C.c_double), ("gamma", C.c_double), ("pde", C.c_int)]
@@ -76,6 +81,9 @@ EXPORTS = [
     "exa_amr_prolongation_batched", "exa_amr_restriction_batched",
     "exa_device_malloc", "exa_device_free", "exa_memcpy_htod", "exa_memcpy_dtoh", "exa_device_copy",
     "exa_selftest_mfma",
+    "exa_ensemble_create", "exa_ensemble_destroy", "exa_ensemble_set_initial_conditions", "exa_ensemble_compute_dt",
+    "exa_ensemble_use_graph", "exa_ensemble_step", "exa_ensemble_sync", "exa_ensemble_download",
+    "exa_ensemble_total_picard_iterations",
 ]
 
 
@@ -138,6 +146,15 @@ def load_library(path=LIB_PATH):
     L.exa_memcpy_dtoh.argtypes = [C.c_void_p, C.c_void_p, LL]
     L.exa_device_copy.argtypes = [C.c_void_p, C.c_void_p, LL, P(C.c_float)]
     L.exa_selftest_mfma.argtypes = [I, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.exa_ensemble_create.argtypes = [P(EnsembleMember), I, P(AderdgConfig), P(C.c_void_p)]
+    L.exa_ensemble_destroy.argtypes = [C.c_void_p]
+    L.exa_ensemble_set_initial_conditions.argtypes = [C.c_void_p, P(InitialCondition)]
+    L.exa_ensemble_compute_dt.argtypes = [C.c_void_p]
+    L.exa_ensemble_use_graph.argtypes = [C.c_void_p, I]
+    L.exa_ensemble_step.argtypes = [C.c_void_p, I]
+    L.exa_ensemble_sync.argtypes = [C.c_void_p, _D, _D]
+    L.exa_ensemble_download.argtypes = [C.c_void_p, I, _D]
+    L.exa_ensemble_total_picard_iterations.argtypes = [C.c_void_p, P(LL)]
     for nm in ("exa_amr_face_unknowns_prolongation", "exa_amr_face_unknowns_restriction"):
         getattr(L, nm).argtypes = [I, I, I, _D, _D, _D, _D, I, I, P(I)]
     for nm in ("exa_amr_volume_unknowns_prolongation", "exa_amr_volume_unknowns_restriction"):
@@ -621,3 +638,71 @@ def amrRestrictionBatched(dim, order, ndata, coarse, fine, transfers):
     pf, nf = _dev(fine)
     pc, nc = _dev(coarse)
     _check(L.exa_amr_restriction_batched(dim, order, ndata, pc, nc, pf, nf, _transfers(transfers), len(transfers)))
+
+
+# ---------------------------------------------------------------------------
+# Ensembles of independent grids
+
+class Ensemble:
+    """Independent periodic grids (order, cells per direction, gamma per
+    member) advanced by shared launches, one hipGraph per step."""
+
+    def __init__(self, members, precision=PREC_FP64, fp64_sweeps=1, cfl=0.9, waves_per_simd=0, contraction=0):
+        L = load_library()
+        _check(L.exa_init(0))
+        arr = (EnsembleMember * len(members))()
+        self.members = []
+        for i, m in enumerate(members):
+            order, (nx, ny, nz), gamma = m["order"], m["cells"], m.get("gamma", 1.4)
+            arr[i].order, arr[i].nx, arr[i].ny, arr[i].nz, arr[i].gamma = order, nx, ny, nz, gamma
+            for d in range(3):
+                arr[i].offset[d] = m.get("offset", (0.0, 0.0, 0.0))[d]
+                arr[i].width[d] = m.get("width", (1.0, 1.0, 1.0))[d]
+            self.members.append(dict(order=order, cells=(nx, ny, nz), gamma=gamma))
+        cfg = AderdgConfig(0, 5, 0, cfl, 1.4, PDE_EULER, precision, fp64_sweeps, waves_per_simd, contraction)
+        h = C.c_void_p()
+        _check(L.exa_ensemble_create(arr, len(members), C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.n = len(members)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.exa_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_initial_conditions(self, ics):
+        arr = (InitialCondition * self.n)(*ics)
+        _check(_lib.exa_ensemble_set_initial_conditions(self._h, arr))
+
+    def compute_dt(self):
+        _check(_lib.exa_ensemble_compute_dt(self._h))
+        return self.sync()[1]
+
+    def use_graph(self, enable=True):
+        _check(_lib.exa_ensemble_use_graph(self._h, 1 if enable else 0))
+
+    def step(self, nsteps=1):
+        _check(_lib.exa_ensemble_step(self._h, int(nsteps)))
+
+    def sync(self):
+        t, dt = np.zeros(self.n), np.zeros(self.n)
+        _check(_lib.exa_ensemble_sync(self._h, _p(t), _p(dt)))
+        return t, dt
+
+    def download(self, i):
+        m = self.members[i]
+        nx, ny, nz = m["cells"]
+        out = np.zeros(nx * ny * nz * (m["order"] + 1) ** 3 * 5)
+        _check(_lib.exa_ensemble_download(self._h, i, _p(out)))
+        return out
+
+    def total_picard_iterations(self):
+        n = C.c_longlong(0)
+        _check(_lib.exa_ensemble_total_picard_iterations(self._h, C.byref(n)))
+        return n.value

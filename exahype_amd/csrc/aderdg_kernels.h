@@ -88,6 +88,24 @@ struct PredictorDebug {
   double* lFhi;  // 4 blocks of B^3*V
 };
 
+// Ensemble of independent grids advanced by the same launches: every cell
+// belongs to an instance with its own mesh width, gamma, time and dt.
+struct InstParams {
+  double dx[3];
+  double gamma;
+  double dt_cur;
+  unsigned long long dt_next;  // bit pattern of the running positive minimum
+  double time;
+  double off[3];
+  long long base;  // first cell of the instance in the group's arrays
+  int nx, ny, nz, pad;
+};
+struct Ens {
+  const int* cell_inst;  // instance of every cell; nullptr: a single grid
+  InstParams* inst;
+  const exa_ic* ics;  // per-instance initial conditions
+};
+
 // ---------------------------------------------------------------------------
 // Picard sweeps of the space-time predictor for one cell held by one
 // workgroup (aderPicardLoopNonlinear, spaceTimePredictorNonlinear.cpph:43-317).
@@ -559,7 +577,7 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
     const double* __restrict__ luh, double* __restrict__ lduh, double* __restrict__ qf,
     double* __restrict__ ff, int* __restrict__ iters, const double* __restrict__ dtp, double dx0,
     double dx1, double dx2, double gamma, int fp64_sweeps, unsigned long long* __restrict__ sweep_count,
-    PredictorDebug dbg) {
+    PredictorDebug dbg, Ens ens) {
   using O = Ord<N>;
   constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
   extern __shared__ double sm[];
@@ -573,7 +591,17 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
   const bool act = s < B3;
   const int z = act ? s / B2 : 0, y = act ? (s / B) % B : 0, x = act ? s % B : 0;
   constexpr bool MIXED = PREC != PRED_FP64;
-  const double dt = *dtp;
+  double dt;
+  if (ens.cell_inst) {
+    const InstParams& P = ens.inst[ens.cell_inst[cell]];
+    dt = P.dt_cur;
+    dx0 = P.dx[0];
+    dx1 = P.dx[1];
+    dx2 = P.dx[2];
+    gamma = P.gamma;
+  } else {
+    dt = *dtp;
+  }
   pencil_constants<N, double>(scd, dt, dx0, dx1, dx2);
   if (MIXED) pencil_constants<N, float>(scf, dt, dx0, dx1, dx2);
   __syncthreads();
@@ -716,7 +744,7 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_volume_integral_ref(const double
 template <int N>
 __global__ __launch_bounds__(256) void k_riemann(const double* __restrict__ qf, double* __restrict__ ff,
                                                  const int* __restrict__ nbr, int ncells, double gamma,
-                                                 long long wave0, long long nwaves) {
+                                                 long long wave0, long long nwaves, Ens ens) {
   using O = Ord<N>;
   constexpr int B = O::B, B2 = O::B2;
   constexpr int NPL = (B2 + 63) / 64;
@@ -728,6 +756,7 @@ __global__ __launch_bounds__(256) void k_riemann(const double* __restrict__ qf, 
   const int d = (int)(wave / ncells);
   const int cl = (int)(wave % ncells);
   const int cr = nbr[(size_t)d * ncells + cl];
+  if (ens.cell_inst) gamma = ens.inst[ens.cell_inst[cl]].gamma;
   const double* QL = qf + ((size_t)cl * 6 + 2 * d + 1) * V * B2;
   const double* QR = qf + ((size_t)cr * 6 + 2 * d) * V * B2;
   double* FL = ff + ((size_t)cl * 6 + 2 * d + 1) * V * B2;
@@ -782,12 +811,21 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_update(double* __restrict__ luh,
                                                        const double* __restrict__ ff,
                                                        const double* __restrict__ dtp,
                                                        unsigned long long* __restrict__ dt_next, double dx0,
-                                                       double dx1, double dx2, double gamma, double cfl) {
+                                                       double dx1, double dx2, double gamma, double cfl, Ens ens) {
   using O = Ord<N>;
   constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
   __shared__ double red[16];
   const DGTables& T = c_tab[N];
   const size_t cell = blockIdx.x;
+  if (ens.cell_inst) {
+    InstParams& P = ens.inst[ens.cell_inst[cell]];
+    dtp = &P.dt_cur;
+    dt_next = &P.dt_next;
+    dx0 = P.dx[0];
+    dx1 = P.dx[1];
+    dx2 = P.dx[2];
+    gamma = P.gamma;
+  }
   const int s = threadIdx.x;
   const bool act = s < B3;
   const int z = act ? s / B2 : 0, y = act ? (s / B) % B : 0, x = act ? s % B : 0;
@@ -852,15 +890,31 @@ template <int N>
 __global__ __launch_bounds__(Ord<N>::NT) void k_initial_conditions(double* __restrict__ luh, exa_ic ic,
                                                                    double gamma, int nx, int ny, double ox,
                                                                    double oy, double oz, double dx0,
-                                                                   double dx1, double dx2) {
+                                                                   double dx1, double dx2, Ens ens) {
   using O = Ord<N>;
   constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
   const DGTables& T = c_tab[N];
   const size_t cell = blockIdx.x;
   const int s = threadIdx.x;
   if (s >= B3) return;
+  size_t local = cell;
+  if (ens.cell_inst) {
+    const int i = ens.cell_inst[cell];
+    const InstParams& P = ens.inst[i];
+    ic = ens.ics[i];
+    gamma = P.gamma;
+    nx = P.nx;
+    ny = P.ny;
+    ox = P.off[0];
+    oy = P.off[1];
+    oz = P.off[2];
+    dx0 = P.dx[0];
+    dx1 = P.dx[1];
+    dx2 = P.dx[2];
+    local = cell - (size_t)P.base;
+  }
   const int z = s / B2, y = (s / B) % B, x = s % B;
-  const int ix = (int)(cell % nx), iy = (int)((cell / nx) % ny), iz = (int)(cell / ((size_t)nx * ny));
+  const int ix = (int)(local % nx), iy = (int)((local / nx) % ny), iz = (int)(local / ((size_t)nx * ny));
   const double c0 = ox + (ix + 0.5) * dx0, c1 = oy + (iy + 0.5) * dx1, c2 = oz + (iz + 0.5) * dx2;
   double X[3];
   X[2] = c2 + dx2 * (T.xi[z] - 0.5);
@@ -887,6 +941,28 @@ __global__ void k_dt_reset(unsigned long long* dt_next) {
 
 __global__ void k_dt_take(double* dt_cur, const unsigned long long* dt_next) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *dt_cur = __longlong_as_double((long long)*dt_next);
+}
+
+// Ensemble dt bookkeeping, one thread per instance.
+__global__ void k_ens_dt_advance(InstParams* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  p[i].time += p[i].dt_cur;
+  p[i].dt_cur = __longlong_as_double((long long)p[i].dt_next);
+  p[i].dt_next = (unsigned long long)__double_as_longlong(1.0e300);
+}
+
+__global__ void k_ens_dt_reset(InstParams* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i].dt_next = (unsigned long long)__double_as_longlong(1.0e300);
+}
+
+__global__ void k_ens_dt_take(InstParams* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    p[i].dt_cur = __longlong_as_double((long long)p[i].dt_next);
+    p[i].dt_next = (unsigned long long)__double_as_longlong(1.0e300);
+  }
 }
 
 __global__ void k_sum_iters(const int* __restrict__ iters, int n, unsigned long long* out) {

@@ -54,7 +54,7 @@ static unsigned long long g_init_mask = 0;
 // wave per SIMD (with and without the per-cell debug outputs) and, for
 // N <= 5, at two waves per SIMD.
 using PredFn = void (*)(const double*, double*, double*, double*, int*, const double*, double, double, double, double,
-                        int, unsigned long long*, exa::PredictorDebug);
+                        int, unsigned long long*, exa::PredictorDebug, exa::Ens);
 
 template <int N, bool DEBUG, int WPE, bool NODE>
 static PredFn pred_by_prec(int prec) {
@@ -256,7 +256,7 @@ extern "C" int exa_aderdg_space_time_predictor(exa_solver* s, double* lQhbnd, do
     PredFn kern = predictor_kernel<NN>(true, s->cfg.precision, 1, node);
     hipLaunchKernelGGL(kern, dim3(1), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), s->stream, s->luh.as<double>(),
                        s->lduh.as<double>(), s->qf.as<double>(), s->ff.as<double>(), s->iters.as<int>(),
-                       s->dt.as<double>(), dx[0], dx[1], dx[2], gamma, s->cfg.fp64_sweeps, nullptr, dbg);
+                       s->dt.as<double>(), dx[0], dx[1], dx[2], gamma, s->cfg.fp64_sweeps, nullptr, dbg, exa::Ens{});
   });
   EXA_CHECK(hipGetLastError());
   std::vector<double> hq(6 * V * B2), hf(6 * V * B2), hdbg(nQi + nFi + nQhi + nFhi);
@@ -374,7 +374,7 @@ static int run_update_cell(exa_solver* s, double* luh_ref, double* lduh_ref, con
     using O = exa::Ord<NN>;
     hipLaunchKernelGGL((exa::k_update<NN, MODE>), dim3(1), dim3(O::NT), 0, s->stream, s->luh.as<double>(),
                        s->lduh.as<double>(), s->ff.as<double>(), dtp, dtn, dx[0], dx[1], dx[2], s->cfg.gamma,
-                       s->cfg.cfl);
+                       s->cfg.cfl, exa::Ens{});
   });
   EXA_CHECK(hipGetLastError());
   EXA_TRY(sync_stream(s));
@@ -439,7 +439,7 @@ extern "C" int exa_aderdg_riemann_solver(exa_solver* s, double* FL, double* FR, 
   EXA_CHECK(hipMemcpy(s->misc.p, nbr, sizeof(nbr), hipMemcpyHostToDevice));
   EXA_ORDER_SWITCH(N, {
     hipLaunchKernelGGL((exa::k_riemann<NN>), dim3(1), dim3(64), 0, s->stream, s->qf.as<double>(), s->ff.as<double>(),
-                       s->misc.as<int>(), 2, s->cfg.gamma, (long long)(2 * d), 1LL);
+                       s->misc.as<int>(), 2, s->cfg.gamma, (long long)(2 * d), 1LL, exa::Ens{});
   });
   EXA_CHECK(hipGetLastError());
   EXA_TRY(sync_stream(s));
@@ -461,7 +461,7 @@ extern "C" int exa_aderdg_adjust_solution(exa_solver* s, double* luh, const doub
     using O = exa::Ord<NN>;
     hipLaunchKernelGGL((exa::k_initial_conditions<NN>), dim3(1), dim3(O::NT), 0, s->stream, s->luh.as<double>(), *ic,
                        s->cfg.gamma, 1, 1, centre[0] - 0.5 * dx[0], centre[1] - 0.5 * dx[1],
-                       centre[2] - 0.5 * dx[2], dx[0], dx[1], dx[2]);
+                       centre[2] - 0.5 * dx[2], dx[0], dx[1], dx[2], exa::Ens{});
   });
   EXA_CHECK(hipGetLastError());
   EXA_TRY(sync_stream(s));
@@ -614,7 +614,7 @@ extern "C" int exa_grid_set_initial_conditions(exa_grid* g, const exa_ic* ic) {
     using O = exa::Ord<NN>;
     hipLaunchKernelGGL((exa::k_initial_conditions<NN>), dim3((unsigned)g->nc), dim3(O::NT), 0, g->stream,
                        g->luh.as<double>(), *ic, g->s->cfg.gamma, g->nx, g->ny, g->off[0], g->off[1], g->off[2],
-                       g->dx[0], g->dx[1], g->dx[2]);
+                       g->dx[0], g->dx[1], g->dx[2], exa::Ens{});
   });
   EXA_CHECK(hipGetLastError());
   EXA_CHECK(hipStreamSynchronize(g->stream));
@@ -628,7 +628,7 @@ static int launch_update(exa_grid* g) {
     using O = exa::Ord<NN>;
     hipLaunchKernelGGL((exa::k_update<NN, MODE>), dim3((unsigned)g->nc), dim3(O::NT), 0, g->stream,
                        g->luh.as<double>(), g->lduh.as<double>(), g->ff.as<double>(), g->dt_cur(), g->dt_next(),
-                       g->dx[0], g->dx[1], g->dx[2], g->s->cfg.gamma, g->s->cfg.cfl);
+                       g->dx[0], g->dx[1], g->dx[2], g->s->cfg.gamma, g->s->cfg.cfl, exa::Ens{});
   });
   EXA_CHECK(hipGetLastError());
   return EXA_OK;
@@ -692,7 +692,7 @@ static int enqueue_step(exa_grid* g, bool timed) {
     hipLaunchKernelGGL(kern, dim3((unsigned)g->nc), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), g->stream,
                        g->luh.as<double>(), g->lduh.as<double>(), g->qf.as<double>(), g->ff.as<double>(),
                        g->iters.as<int>(), g->dt_cur(), g->dx[0], g->dx[1], g->dx[2], g->s->cfg.gamma,
-                       g->s->cfg.fp64_sweeps, g->iter_sum(), nodbg);
+                       g->s->cfg.fp64_sweeps, g->iter_sum(), nodbg, exa::Ens{});
   });
   EXA_CHECK(hipGetLastError());
   EXA_TRY(tstop(0));
@@ -701,7 +701,7 @@ static int enqueue_step(exa_grid* g, bool timed) {
   const unsigned blocks = (unsigned)((waves * 64 + 255) / 256);
   EXA_ORDER_SWITCH(N, {
     hipLaunchKernelGGL((exa::k_riemann<NN>), dim3(blocks), dim3(256), 0, g->stream, g->qf.as<double>(),
-                       g->ff.as<double>(), g->nbr.as<int>(), (int)g->nc, g->s->cfg.gamma, 0LL, waves);
+                       g->ff.as<double>(), g->nbr.as<int>(), (int)g->nc, g->s->cfg.gamma, 0LL, waves, exa::Ens{});
   });
   EXA_CHECK(hipGetLastError());
   EXA_TRY(tstop(1));
@@ -796,5 +796,317 @@ extern "C" int exa_grid_kernel_times(exa_grid* g, double times[3], long long cou
     g->times[p] = 0.0;
     g->counts[p] = 0;
   }
+  return EXA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble of independent periodic grids (uneven sizes, orders and gammas):
+// the instances of one order share every launch of a step -- one predictor,
+// one Riemann and one update launch over all their cells, each cell reading
+// its instance's dt, mesh width and gamma -- and the launches of all orders
+// of one step form one hipGraph.
+struct EnsGroup {
+  int order = 0;
+  exa_solver* s = nullptr;  // owned; carries the group's configuration
+  long long nc = 0;
+  std::vector<int> members;  // ensemble indices of the instances of this group
+  std::vector<exa::InstParams> hp;
+  DevBuf luh, lduh, qf, ff, iters, nbr, cell_inst, params, ics, iter_sum;
+};
+
+struct exa_ensemble {
+  int device = 0;
+  std::vector<EnsGroup*> groups;
+  std::vector<std::pair<int, int>> where;  // instance -> (group, index in group)
+  hipStream_t stream = nullptr;
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+};
+
+static exa::Ens ens_of(EnsGroup* gr) {
+  exa::Ens e;
+  e.cell_inst = gr->cell_inst.as<int>();
+  e.inst = gr->params.as<exa::InstParams>();
+  e.ics = gr->ics.as<exa_ic>();
+  return e;
+}
+
+extern "C" int exa_ensemble_destroy(exa_ensemble* e) {
+  if (!e) return EXA_OK;
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  for (EnsGroup* gr : e->groups) {
+    exa_aderdg_destroy(gr->s);
+    delete gr;
+  }
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+  return EXA_OK;
+}
+
+extern "C" int exa_ensemble_create(const exa_ensemble_member* m, int n, const exa_aderdg_config* cfg,
+                                   exa_ensemble** out) {
+  EXA_ARG(m && n > 0 && cfg && out, "arguments");
+  size_t need = 0;
+  for (int i = 0; i < n; ++i) {
+    EXA_ARG(m[i].order >= 1 && m[i].order <= 7, "member order must be 1..7");
+    EXA_ARG(m[i].nx > 0 && m[i].ny > 0 && m[i].nz > 0, "member grid dimensions");
+    EXA_ARG(m[i].gamma > 1.0, "member gamma must exceed 1");
+    const size_t B = m[i].order + 1, nc = (size_t)m[i].nx * m[i].ny * m[i].nz;
+    need += nc * (2 * exa::V * B * B * B + 12 * exa::V * B * B) * sizeof(double) + nc * 5 * sizeof(int);
+  }
+  int dev = 0;
+  EXA_CHECK(hipGetDevice(&dev));
+  EXA_TRY(exa_init(dev));
+  size_t freeb = 0, total = 0;
+  EXA_CHECK(hipMemGetInfo(&freeb, &total));
+  if (need > freeb) {
+    exa_set_error("ensemble needs %.1f GB of device memory, %.1f GB free: advance it in smaller waves",
+                  need / 1e9, freeb / 1e9);
+    return EXA_ERR_NOMEM;
+  }
+  exa_ensemble* e = new exa_ensemble();
+  e->device = dev;
+  e->where.resize(n);
+  for (int N = 1; N <= 7; ++N) {
+    std::vector<int> mem;
+    for (int i = 0; i < n; ++i)
+      if (m[i].order == N) mem.push_back(i);
+    if (mem.empty()) continue;
+    EnsGroup* gr = new EnsGroup();
+    e->groups.push_back(gr);
+    gr->order = N;
+    gr->members = mem;
+    exa_aderdg_config c = *cfg;
+    c.order = N;
+    c.nvar = 5;
+    c.npar = 0;
+    c.pde = EXA_PDE_EULER;
+    if (c.cfl <= 0.0) c.cfl = 0.9;
+    if (c.gamma <= 1.0) c.gamma = 1.4;
+    int r = exa_aderdg_create(&c, &gr->s);
+    if (r != EXA_OK) {
+      exa_ensemble_destroy(e);
+      return r;
+    }
+    long long base = 0;
+    for (size_t j = 0; j < mem.size(); ++j) {
+      const exa_ensemble_member& mm = m[mem[j]];
+      exa::InstParams p{};
+      for (int d = 0; d < 3; ++d) {
+        p.off[d] = mm.offset[d];
+        p.dx[d] = mm.width[d] / (d == 0 ? mm.nx : (d == 1 ? mm.ny : mm.nz));
+      }
+      p.gamma = mm.gamma;
+      p.dt_cur = 0.0;
+      p.dt_next = exa_big_bits();
+      p.time = 0.0;
+      p.base = base;
+      p.nx = mm.nx;
+      p.ny = mm.ny;
+      p.nz = mm.nz;
+      gr->hp.push_back(p);
+      e->where[mem[j]] = {(int)e->groups.size() - 1, (int)j};
+      base += (long long)mm.nx * mm.ny * mm.nz;
+    }
+    gr->nc = base;
+    if (base >= (1LL << 31)) {
+      exa_ensemble_destroy(e);
+      exa_set_error("invalid argument: more than 2^31 cells of one order in an ensemble");
+      return EXA_ERR_ARG;
+    }
+    const int B = N + 1, B2 = B * B, B3 = B2 * B, V = exa::V;
+    if ((r = gr->luh.ensure(sizeof(double) * gr->nc * V * B3)) != EXA_OK ||
+        (r = gr->lduh.ensure(sizeof(double) * gr->nc * V * B3)) != EXA_OK ||
+        (r = gr->qf.ensure(sizeof(double) * gr->nc * 6 * V * B2)) != EXA_OK ||
+        (r = gr->ff.ensure(sizeof(double) * gr->nc * 6 * V * B2)) != EXA_OK ||
+        (r = gr->iters.ensure(sizeof(int) * gr->nc)) != EXA_OK ||
+        (r = gr->nbr.ensure(sizeof(int) * 3 * gr->nc)) != EXA_OK ||
+        (r = gr->cell_inst.ensure(sizeof(int) * gr->nc)) != EXA_OK ||
+        (r = gr->params.ensure(sizeof(exa::InstParams) * mem.size())) != EXA_OK ||
+        (r = gr->ics.ensure(sizeof(exa_ic) * mem.size())) != EXA_OK ||
+        (r = gr->iter_sum.ensure(sizeof(unsigned long long))) != EXA_OK) {
+      exa_ensemble_destroy(e);
+      return r;
+    }
+    std::vector<int> nbr(3 * gr->nc), ci(gr->nc);
+    for (size_t j = 0; j < mem.size(); ++j) {
+      const exa::InstParams& p = gr->hp[j];
+      const long long nci = (long long)p.nx * p.ny * p.nz;
+      for (long long c = 0; c < nci; ++c) {
+        const int ix = (int)(c % p.nx), iy = (int)((c / p.nx) % p.ny), iz = (int)(c / ((long long)p.nx * p.ny));
+        const long long g = p.base + c;
+        nbr[0 * gr->nc + g] = (int)(p.base + ((long long)iz * p.ny + iy) * p.nx + (ix + 1) % p.nx);
+        nbr[1 * gr->nc + g] = (int)(p.base + ((long long)iz * p.ny + (iy + 1) % p.ny) * p.nx + ix);
+        nbr[2 * gr->nc + g] = (int)(p.base + ((long long)((iz + 1) % p.nz) * p.ny + iy) * p.nx + ix);
+        ci[g] = (int)j;
+      }
+    }
+    hipError_t he = hipMemcpy(gr->nbr.p, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(gr->cell_inst.p, ci.data(), sizeof(int) * ci.size(), hipMemcpyHostToDevice);
+    if (he == hipSuccess)
+      he = hipMemcpy(gr->params.p, gr->hp.data(), sizeof(exa::InstParams) * gr->hp.size(), hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemset(gr->iters.p, 0, sizeof(int) * gr->nc);
+    if (he == hipSuccess) he = hipMemset(gr->iter_sum.p, 0, sizeof(unsigned long long));
+    if (he != hipSuccess) {
+      exa_set_error("ensemble setup: %s", hipGetErrorString(he));
+      exa_ensemble_destroy(e);
+      return EXA_ERR_HIP;
+    }
+  }
+  hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) {
+    exa_set_error("hipStreamCreate: %s", hipGetErrorString(he));
+    exa_ensemble_destroy(e);
+    return EXA_ERR_HIP;
+  }
+  *out = e;
+  return EXA_OK;
+}
+
+extern "C" int exa_ensemble_set_initial_conditions(exa_ensemble* e, const exa_ic* ics) {
+  EXA_ARG(e && ics, "null pointer");
+  EXA_CHECK(hipSetDevice(e->device));
+  for (EnsGroup* gr : e->groups) {
+    std::vector<exa_ic> h;
+    for (int i : gr->members) h.push_back(ics[i]);
+    EXA_CHECK(hipMemcpyAsync(gr->ics.p, h.data(), sizeof(exa_ic) * h.size(), hipMemcpyHostToDevice, e->stream));
+    EXA_CHECK(hipStreamSynchronize(e->stream));
+    const exa::Ens en = ens_of(gr);
+    EXA_ORDER_SWITCH(gr->order, {
+      using O = exa::Ord<NN>;
+      hipLaunchKernelGGL((exa::k_initial_conditions<NN>), dim3((unsigned)gr->nc), dim3(O::NT), 0, e->stream,
+                         gr->luh.as<double>(), ics[0], 1.4, 1, 1, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, en);
+    });
+    EXA_CHECK(hipGetLastError());
+  }
+  EXA_CHECK(hipStreamSynchronize(e->stream));
+  return EXA_OK;
+}
+
+extern "C" int exa_ensemble_compute_dt(exa_ensemble* e) {
+  EXA_ARG(e, "null pointer");
+  EXA_CHECK(hipSetDevice(e->device));
+  for (EnsGroup* gr : e->groups) {
+    const int ni = (int)gr->members.size();
+    const exa::Ens en = ens_of(gr);
+    hipLaunchKernelGGL(exa::k_ens_dt_reset, dim3((ni + 255) / 256), dim3(256), 0, e->stream, en.inst, ni);
+    EXA_ORDER_SWITCH(gr->order, {
+      using O = exa::Ord<NN>;
+      hipLaunchKernelGGL((exa::k_update<NN, exa::UPD_DT>), dim3((unsigned)gr->nc), dim3(O::NT), 0, e->stream,
+                         gr->luh.as<double>(), gr->lduh.as<double>(), gr->ff.as<double>(), nullptr, nullptr, 1.0,
+                         1.0, 1.0, 1.4, gr->s->cfg.cfl, en);
+    });
+    hipLaunchKernelGGL(exa::k_ens_dt_take, dim3((ni + 255) / 256), dim3(256), 0, e->stream, en.inst, ni);
+    EXA_CHECK(hipGetLastError());
+  }
+  EXA_CHECK(hipStreamSynchronize(e->stream));
+  return EXA_OK;
+}
+
+static int enqueue_ensemble_step(exa_ensemble* e) {
+  exa::PredictorDebug nodbg;
+  nodbg.lQi = nodbg.lFi = nodbg.lQhi = nodbg.lFhi = nullptr;
+  for (EnsGroup* gr : e->groups) {
+    const exa::Ens en = ens_of(gr);
+    const exa_aderdg_config& cfg = gr->s->cfg;
+    EXA_ORDER_SWITCH(gr->order, {
+      using O = exa::Ord<NN>;
+      const int wpe = predictor_waves_per_simd(cfg, NN);
+      const bool node = predictor_node_contraction(cfg, NN);
+      PredFn kern = predictor_kernel<NN>(false, cfg.precision, wpe, node);
+      hipLaunchKernelGGL(kern, dim3((unsigned)gr->nc), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), e->stream,
+                         gr->luh.as<double>(), gr->lduh.as<double>(), gr->qf.as<double>(), gr->ff.as<double>(),
+                         gr->iters.as<int>(), nullptr, 1.0, 1.0, 1.0, 1.4, cfg.fp64_sweeps,
+                         gr->iter_sum.as<unsigned long long>(), nodbg, en);
+      const long long waves = 3LL * gr->nc;
+      const unsigned blocks = (unsigned)((waves * 64 + 255) / 256);
+      hipLaunchKernelGGL((exa::k_riemann<NN>), dim3(blocks), dim3(256), 0, e->stream, gr->qf.as<double>(),
+                         gr->ff.as<double>(), gr->nbr.as<int>(), (int)gr->nc, 1.4, 0LL, waves, en);
+      hipLaunchKernelGGL((exa::k_update<NN, exa::UPD_SURFACE | exa::UPD_UPDATE | exa::UPD_DT>),
+                         dim3((unsigned)gr->nc), dim3(O::NT), 0, e->stream, gr->luh.as<double>(),
+                         gr->lduh.as<double>(), gr->ff.as<double>(), nullptr, nullptr, 1.0, 1.0, 1.0, 1.4, cfg.cfl,
+                         en);
+    });
+    const int ni = (int)gr->members.size();
+    hipLaunchKernelGGL(exa::k_ens_dt_advance, dim3((ni + 255) / 256), dim3(256), 0, e->stream, en.inst, ni);
+    EXA_CHECK(hipGetLastError());
+  }
+  return EXA_OK;
+}
+
+extern "C" int exa_ensemble_use_graph(exa_ensemble* e, int enable) {
+  EXA_ARG(e, "null pointer");
+  e->use_graph = enable != 0;
+  return EXA_OK;
+}
+
+extern "C" int exa_ensemble_step(exa_ensemble* e, int nsteps) {
+  EXA_ARG(e && nsteps >= 0, "nsteps");
+  EXA_CHECK(hipSetDevice(e->device));
+  if (e->use_graph) {
+    if (!e->gexec) {
+      EXA_CHECK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      int r = enqueue_ensemble_step(e);
+      hipGraph_t graph = nullptr;
+      hipError_t he = hipStreamEndCapture(e->stream, &graph);
+      if (r != EXA_OK) return r;
+      EXA_CHECK(he);
+      e->graph = graph;
+      EXA_CHECK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+    }
+    for (int i = 0; i < nsteps; ++i) EXA_CHECK(hipGraphLaunch(e->gexec, e->stream));
+  } else {
+    for (int i = 0; i < nsteps; ++i) EXA_TRY(enqueue_ensemble_step(e));
+  }
+  EXA_CHECK(hipGetLastError());
+  return EXA_OK;
+}
+
+extern "C" int exa_ensemble_sync(exa_ensemble* e, double* times, double* dts) {
+  EXA_ARG(e, "null pointer");
+  EXA_CHECK(hipSetDevice(e->device));
+  EXA_CHECK(hipStreamSynchronize(e->stream));
+  for (EnsGroup* gr : e->groups) {
+    std::vector<exa::InstParams> p(gr->members.size());
+    EXA_CHECK(hipMemcpy(p.data(), gr->params.p, sizeof(exa::InstParams) * p.size(), hipMemcpyDeviceToHost));
+    for (size_t j = 0; j < p.size(); ++j) {
+      if (times) times[gr->members[j]] = p[j].time;
+      if (dts) dts[gr->members[j]] = p[j].dt_cur;
+    }
+  }
+  return EXA_OK;
+}
+
+extern "C" int exa_ensemble_download(exa_ensemble* e, int instance, double* luh) {
+  EXA_ARG(e && luh && instance >= 0 && instance < (int)e->where.size(), "instance");
+  EXA_CHECK(hipSetDevice(e->device));
+  EXA_CHECK(hipStreamSynchronize(e->stream));
+  EnsGroup* gr = e->groups[e->where[instance].first];
+  const exa::InstParams& p = gr->hp[e->where[instance].second];
+  const int B = gr->order + 1, B3 = B * B * B, V = exa::V;
+  const size_t per = (size_t)V * B3, nci = (size_t)p.nx * p.ny * p.nz;
+  std::vector<double> buf(per * nci);
+  EXA_CHECK(hipMemcpy(buf.data(), gr->luh.as<double>() + p.base * per, sizeof(double) * buf.size(),
+                      hipMemcpyDeviceToHost));
+  for (size_t c = 0; c < nci; ++c) soa_to_ref_cell(buf.data() + c * per, luh + c * per, B3, V);
+  return EXA_OK;
+}
+
+extern "C" int exa_ensemble_total_picard_iterations(exa_ensemble* e, long long* total) {
+  EXA_ARG(e && total, "null pointer");
+  EXA_CHECK(hipSetDevice(e->device));
+  EXA_CHECK(hipStreamSynchronize(e->stream));
+  long long t = 0;
+  for (EnsGroup* gr : e->groups) {
+    unsigned long long h = 0, zero = 0;
+    EXA_CHECK(hipMemcpy(&h, gr->iter_sum.p, sizeof(h), hipMemcpyDeviceToHost));
+    EXA_CHECK(hipMemcpy(gr->iter_sum.p, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    t += (long long)h;
+  }
+  *total = t;
   return EXA_OK;
 }

@@ -199,6 +199,42 @@ int exa_grid_kernel_times(exa_grid* g, double times[3], long long counts[3]);
 int exa_grid_total_picard_iterations(exa_grid* g, long long* total);
 
 /* ---------------------------------------------------------------------- */
+/* Ensemble of independent periodic grids of uneven sizes, orders and gammas.
+ * The instances of one order share every launch of a step (one predictor,
+ * one Riemann and one update launch over all their cells; each cell reads
+ * its instance's dt, mesh width and gamma), and a step of all orders is one
+ * hipGraph.  Each instance keeps its own time and CFL time step.  No single
+ * counterpart in the reference, which runs one grid per process. */
+typedef struct exa_ensemble exa_ensemble;
+
+typedef struct {
+  int order;          /* 1..7                       */
+  int nx, ny, nz;     /* cells per direction        */
+  double offset[3];   /* domain origin              */
+  double width[3];    /* domain extent              */
+  double gamma;       /* ratio of specific heats    */
+} exa_ensemble_member;
+
+/* cfg supplies precision, fp64_sweeps, cfl, waves_per_simd and contraction
+ * (order, gamma and the PDE fields are taken per member).  Fails with
+ * EXA_ERR_NOMEM when the members do not fit the free device memory. */
+int exa_ensemble_create(const exa_ensemble_member* members, int n, const exa_aderdg_config* cfg,
+                        exa_ensemble** out);
+int exa_ensemble_destroy(exa_ensemble* e);
+/* ics: one initial condition per member */
+int exa_ensemble_set_initial_conditions(exa_ensemble* e, const exa_ic* ics);
+/* stable dt of every member's current state becomes its next dt */
+int exa_ensemble_compute_dt(exa_ensemble* e);
+int exa_ensemble_use_graph(exa_ensemble* e, int enable);
+int exa_ensemble_step(exa_ensemble* e, int nsteps);
+/* times/dts: per member (either may be NULL); waits for the ensemble's work */
+int exa_ensemble_sync(exa_ensemble* e, double* times, double* dts);
+/* one member's luh, cells ordered (z,y,x), reference layout per cell */
+int exa_ensemble_download(exa_ensemble* e, int member, double* luh);
+/* Picard sweeps summed over all cells and steps since the last call */
+int exa_ensemble_total_picard_iterations(exa_ensemble* e, long long* total);
+
+/* ---------------------------------------------------------------------- */
 /* Finite volumes: the kernel surface of exahype::solvers::FiniteVolumesSolver
  * (E/solvers/FiniteVolumesSolver.h:235-400).  A patch is stored with its
  * ghost layers, reference layout [(z)][y][x][nData], side patch+2*ghost. */
