@@ -114,9 +114,11 @@ def main():
     ap.add_argument("--waves-per-simd", type=int, default=0, help="predictor occupancy target (0 = tuned)")
     ap.add_argument("--contraction", type=int, default=0,
                     help="predictor sweep contraction: 0 tuned, 1 pencil owners, 2 node owners")
-    ap.add_argument("--workload", default="aderdg", choices=["aderdg", "fv2d"],
+    ap.add_argument("--workload", default="aderdg", choices=["aderdg", "fv2d", "ensemble"],
                     help="aderdg: the headline ADER-DG step; fv2d: the EulerFV demonstrator "
-                         "(2D Godunov, 81x81 patches of 10x10 volumes, 200 steps)")
+                         "(2D Godunov, 81x81 patches of 10x10 volumes, 200 steps); ensemble: independent "
+                         "grids of random order/size/gamma advanced together")
+    ap.add_argument("--instances", type=int, default=512, help="ensemble members (--workload ensemble)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also time a plain device copy of this many GiB (HBM reference, counter calibration)")
     args = ap.parse_args()
@@ -127,6 +129,9 @@ def main():
 
     if args.workload == "fv2d":
         print(json.dumps(run_fv2d(args)))
+        return
+    if args.workload == "ensemble":
+        print(json.dumps(run_ensemble(args)))
         return
     import exahype_amd as E
     N, n = args.order, args.cells
@@ -278,6 +283,90 @@ def run_fv2d(args):
                                                          % (steps, cwall)}
         out["parity_vs_oracle_rel_inf"] = err
     return out
+
+
+def ensemble_members(n, seed=5):
+    """n members: order ~ U{3..7}, cells ~ U{9^3, 27^3, 45^3}, gamma ~ U[1.3, 1.67],
+    smooth seeded initial data (vortex of random strength/drift + 8 density modes)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        side = int(rng.choice([9, 27, 45]))
+        ic = dict(xc=tuple(rng.uniform(0.3, 0.7, 3)), L=0.1, eps=float(rng.uniform(1, 5)),
+                  uinf=tuple(rng.uniform(-1, 1, 3)), rho_inf=1.0, p_inf=1.0, amp=1e-2, nmodes=8,
+                  k=rng.integers(1, 4, size=(8, 3)).astype(float), a=rng.uniform(-1, 1, 8),
+                  phi=rng.uniform(0, 2 * np.pi, 8))
+        out.append(dict(order=int(rng.integers(3, 8)), cells=(side, side, side), gamma=float(rng.uniform(1.3, 1.67)),
+                        ic=ic))
+    return out
+
+
+def member_bytes(m):
+    B = m["order"] + 1
+    nc = m["cells"][0] * m["cells"][1] * m["cells"][2]
+    return nc * (2 * NVAR * B ** 3 + 12 * NVAR * B ** 2) * 8 + nc * 20
+
+
+def run_ensemble(args):
+    """Ensemble batching: members of random order, size and gamma advanced
+    20 steps by shared launches (one hipGraph per step), in waves that fit
+    the device; each wave runs the mixed-precision path and the FP64 path
+    side by side, and every member's L2 difference between them is reported."""
+    import exahype_amd as E
+    steps = args.steps if args.steps != 10 else 20
+    members = ensemble_members(args.instances)
+    cap = 100e9  # bytes per ensemble of a wave (two are resident: mixed and FP64)
+    waves, cur, size = [], [], 0
+    for i, m in enumerate(members):
+        if cur and size + member_bytes(m) > cap:
+            waves.append(cur)
+            cur, size = [], 0
+        cur.append(i)
+        size += member_bytes(m)
+    waves.append(cur)
+    prec, sweeps = E.PREC_FP32_MFMA, 1
+    dof_steps, wall_mx, wall_64, l2s = 0, 0.0, 0.0, [None] * len(members)
+    for w in waves:
+        ms = [members[i] for i in w]
+        ics = [E.make_ic(0, **m["ic"]) for m in ms]
+        runs = {}
+        for p, sw in ((prec, sweeps), (E.PREC_FP64, 1)):
+            ens = E.Ensemble(ms, precision=p, fp64_sweeps=sw)
+            ens.set_initial_conditions(ics)
+            ens.compute_dt()
+            ens.step(1)  # graph capture + warm-up
+            ens.sync()
+            ens.set_initial_conditions(ics)
+            ens.compute_dt()
+            E.device_synchronize()
+            t0 = time.perf_counter()
+            ens.step(steps)
+            ens.sync()
+            wall = time.perf_counter() - t0
+            runs[p] = (ens, wall)
+        wall_mx += runs[prec][1]
+        wall_64 += runs[E.PREC_FP64][1]
+        for j, i in enumerate(w):
+            m = members[i]
+            nc = m["cells"][0] * m["cells"][1] * m["cells"][2]
+            dof_steps += nc * NVAR * (m["order"] + 1) ** 3 * steps
+            a = runs[prec][0].download(j)
+            b = runs[E.PREC_FP64][0].download(j)
+            l2s[i] = float(l2_per_variable(a, b, m["order"], nc).max())
+        for ens, _ in runs.values():
+            ens.close()
+    l2a = np.array(l2s)
+    return {"metric": "DoF-updates/s, ensemble of %d independent 3D Euler ADER-DG grids (N 3..7, 9^3..45^3 cells)"
+                      % len(members), "value": dof_steps / wall_mx, "unit": "DoF-updates/s", "n_gpus": 1,
+            "steps": steps, "warmup": 1, "ms_per_step": wall_mx / steps * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+            "config": {"workload": "ensemble: order ~ U{3..7}, cells ~ U{9^3,27^3,45^3}, gamma ~ U[1.3,1.67], "
+                                   "seeded smooth states, 20 steps, shared launches per order, one hipGraph per step",
+                       "members": len(members), "waves": len(waves), "predictor": "fp32_mfma + 1 FP64 sweep",
+                       "orders": {str(N): sum(1 for m in members if m["order"] == N) for N in range(3, 8)}},
+            "fp64": {"dof_updates_per_s": dof_steps / wall_64, "speedup_mixed_vs_fp64": wall_64 / wall_mx},
+            "l2_mixed_vs_fp64": {"max": float(l2a.max()), "median": float(np.median(l2a)),
+                                 "members_above_1e-6": int((l2a > 1e-6).sum())}}
 
 
 def copy_reference(gib):

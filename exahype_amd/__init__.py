@@ -83,7 +83,7 @@ EXPORTS = [
     "exa_selftest_mfma",
     "exa_ensemble_create", "exa_ensemble_destroy", "exa_ensemble_set_initial_conditions", "exa_ensemble_compute_dt",
     "exa_ensemble_use_graph", "exa_ensemble_step", "exa_ensemble_sync", "exa_ensemble_download",
-    "exa_ensemble_total_picard_iterations",
+    "exa_ensemble_total_picard_iterations", "exa_autotune",
 ]
 
 
@@ -146,6 +146,7 @@ def load_library(path=LIB_PATH):
     L.exa_memcpy_dtoh.argtypes = [C.c_void_p, C.c_void_p, LL]
     L.exa_device_copy.argtypes = [C.c_void_p, C.c_void_p, LL, P(C.c_float)]
     L.exa_selftest_mfma.argtypes = [I, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.exa_autotune.argtypes = [I, I, I, P(I), P(I), _D]
     L.exa_ensemble_create.argtypes = [P(EnsembleMember), I, P(AderdgConfig), P(C.c_void_p)]
     L.exa_ensemble_destroy.argtypes = [C.c_void_p]
     L.exa_ensemble_set_initial_conditions.argtypes = [C.c_void_p, P(InitialCondition)]
@@ -706,3 +707,14 @@ class Ensemble:
         n = C.c_longlong(0)
         _check(_lib.exa_ensemble_total_picard_iterations(self._h, C.byref(n)))
         return n.value
+
+
+def autotune(order, precision=PREC_FP64, ncells=32):
+    """Times the predictor variants of one order/precision on this device and
+    records the fastest; returns (waves_per_simd, contraction, times_ms)."""
+    L = load_library()
+    _check(L.exa_init(0))
+    w, c = C.c_int(0), C.c_int(0)
+    ms = np.zeros(4)
+    _check(L.exa_autotune(order, precision, ncells, C.byref(w), C.byref(c), _p(ms)))
+    return w.value, c.value, ms

@@ -161,9 +161,19 @@ struct exa_solver {
 
 // Contraction form of the predictor's Picard sweeps: EXA_CONTRACT_PENCIL
 // (two barriers per time slice) or EXA_CONTRACT_NODE (one barrier per slice).
-static bool predictor_node_contraction(const exa_aderdg_config& c, int N) {
+// Predictor variants chosen by exa_autotune, per device, order and precision
+// (0: not tuned, the measured defaults below apply).
+struct TunedVariant {
+  int wpe = 0;
+  int contraction = 0;
+};
+static TunedVariant g_tuned[64][EXA_MAX_ORDER + 1][4];
+
+static bool predictor_node_contraction(const exa_aderdg_config& c, int N, int dev) {
   if (c.contraction == EXA_CONTRACT_PENCIL) return false;
   if (c.contraction == EXA_CONTRACT_NODE) return true;
+  const int tc = g_tuned[dev & 63][N][c.precision & 3].contraction;
+  if (tc) return tc == EXA_CONTRACT_NODE;
   // measured at 81^3 cells (FP64 and mixed): pencil owners win at N=5
   // (1.9x), node owners at N=3 (1.3x) and N=7 (1.1x)
   return N != 5;
@@ -252,7 +262,7 @@ extern "C" int exa_aderdg_space_time_predictor(exa_solver* s, double* lQhbnd, do
   const double gamma = s->cfg.gamma;
   EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
-    const bool node = predictor_node_contraction(s->cfg, NN);
+    const bool node = predictor_node_contraction(s->cfg, NN, s->device);
     PredFn kern = predictor_kernel<NN>(true, s->cfg.precision, 1, node);
     hipLaunchKernelGGL(kern, dim3(1), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), s->stream, s->luh.as<double>(),
                        s->lduh.as<double>(), s->qf.as<double>(), s->ff.as<double>(), s->iters.as<int>(),
@@ -659,8 +669,10 @@ extern "C" int exa_grid_set_dt(exa_grid* g, double dt) {
 
 // Occupancy target of the predictor: measured per order and precision
 // (two waves per SIMD trade a few spilled registers for latency hiding).
-static int predictor_waves_per_simd(const exa_aderdg_config& c, int N) {
+static int predictor_waves_per_simd(const exa_aderdg_config& c, int N, int dev) {
   if (c.waves_per_simd == 1 || c.waves_per_simd == 2) return c.waves_per_simd;
+  const int tw = g_tuned[dev & 63][N][c.precision & 3].wpe;
+  if (tw) return tw;
   // measured at 81^3 cells: two waves per SIMD is 1.6x faster at N=5
   return N <= 5 ? 2 : 1;
 }
@@ -686,8 +698,8 @@ static int enqueue_step(exa_grid* g, bool timed) {
   nodbg.lQi = nodbg.lFi = nodbg.lQhi = nodbg.lFhi = nullptr;
   EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
-    const int wpe = predictor_waves_per_simd(g->s->cfg, NN);
-    const bool node = predictor_node_contraction(g->s->cfg, NN);
+    const int wpe = predictor_waves_per_simd(g->s->cfg, NN, g->s->device);
+    const bool node = predictor_node_contraction(g->s->cfg, NN, g->s->device);
     PredFn kern = predictor_kernel<NN>(false, g->s->cfg.precision, wpe, node);
     hipLaunchKernelGGL(kern, dim3((unsigned)g->nc), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), g->stream,
                        g->luh.as<double>(), g->lduh.as<double>(), g->qf.as<double>(), g->ff.as<double>(),
@@ -1015,8 +1027,8 @@ static int enqueue_ensemble_step(exa_ensemble* e) {
     const exa_aderdg_config& cfg = gr->s->cfg;
     EXA_ORDER_SWITCH(gr->order, {
       using O = exa::Ord<NN>;
-      const int wpe = predictor_waves_per_simd(cfg, NN);
-      const bool node = predictor_node_contraction(cfg, NN);
+      const int wpe = predictor_waves_per_simd(cfg, NN, gr->s->device);
+      const bool node = predictor_node_contraction(cfg, NN, gr->s->device);
       PredFn kern = predictor_kernel<NN>(false, cfg.precision, wpe, node);
       hipLaunchKernelGGL(kern, dim3((unsigned)gr->nc), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), e->stream,
                          gr->luh.as<double>(), gr->lduh.as<double>(), gr->qf.as<double>(), gr->ff.as<double>(),
@@ -1108,5 +1120,72 @@ extern "C" int exa_ensemble_total_picard_iterations(exa_ensemble* e, long long* 
     t += (long long)h;
   }
   *total = t;
+  return EXA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner of the predictor's launch variant: times every (waves per SIMD,
+// contraction form) pair of one order and precision on a synthetic vortex
+// grid and records the fastest for this device.
+extern "C" int exa_autotune(int order, int precision, int ncells, int* waves_per_simd, int* contraction,
+                            double ms[4]) {
+  EXA_ARG(order >= 1 && order <= 7, "order must be 1..7");
+  EXA_ARG(precision >= EXA_PREC_FP64 && precision <= EXA_PREC_BF16_MFMA, "precision");
+  EXA_ARG(ncells >= 4 && ncells <= 128, "ncells must be 4..128");
+  int dev = 0;
+  EXA_CHECK(hipGetDevice(&dev));
+  EXA_TRY(exa_init(dev));
+  exa_ic ic{};
+  ic.kind = 0;
+  ic.xc[0] = ic.xc[1] = ic.xc[2] = 0.5;
+  ic.L = 0.1;
+  ic.eps = 5.0;
+  ic.uinf[0] = ic.uinf[1] = 1.0;
+  ic.rho_inf = ic.p_inf = 1.0;
+  double best = 1e300;
+  int bw = 0, bc = 0, k = 0;
+  for (int w = 1; w <= 2; ++w)
+    for (int cform = EXA_CONTRACT_PENCIL; cform <= EXA_CONTRACT_NODE; ++cform, ++k) {
+      exa_aderdg_config cfg{};
+      cfg.order = order;
+      cfg.nvar = 5;
+      cfg.cfl = 0.9;
+      cfg.gamma = 1.4;
+      cfg.pde = EXA_PDE_EULER;
+      cfg.precision = precision;
+      cfg.fp64_sweeps = 1;
+      cfg.waves_per_simd = w;
+      cfg.contraction = cform;
+      exa_solver* s = nullptr;
+      exa_grid* g = nullptr;
+      EXA_TRY(exa_aderdg_create(&cfg, &s));
+      int r = exa_grid_create(s, ncells, ncells, ncells, nullptr, nullptr, &g);
+      double t = 1e300;
+      if (r == EXA_OK) r = exa_grid_set_initial_conditions(g, &ic);
+      if (r == EXA_OK) r = exa_grid_compute_dt(g, nullptr);
+      if (r == EXA_OK) r = exa_grid_step(g, 1);
+      if (r == EXA_OK) r = exa_grid_enable_timing(g, 1);
+      if (r == EXA_OK) {
+        double tt[3];
+        long long cc[3];
+        r = exa_grid_kernel_times(g, tt, cc);
+        if (r == EXA_OK) r = exa_grid_step(g, 3);
+        if (r == EXA_OK) r = exa_grid_kernel_times(g, tt, cc);
+        if (r == EXA_OK && cc[0] > 0) t = tt[0] / cc[0];
+      }
+      exa_grid_destroy(g);
+      exa_aderdg_destroy(s);
+      if (r != EXA_OK) return r;
+      if (ms) ms[k] = t;
+      if (t < best) {
+        best = t;
+        bw = w;
+        bc = cform;
+      }
+    }
+  g_tuned[dev & 63][order][precision].wpe = bw;
+  g_tuned[dev & 63][order][precision].contraction = bc;
+  if (waves_per_simd) *waves_per_simd = bw;
+  if (contraction) *contraction = bc;
   return EXA_OK;
 }

@@ -227,3 +227,21 @@ def test_conservation_large_grid():
     g.sync()
     t1 = tot()
     assert np.abs(t1 - t0).max() <= 1e-11 * np.abs(t0).max()
+
+
+def test_autotune_records_a_variant():
+    """The autotuner times the four launch variants and picks the fastest; a
+    solver left at the tuned defaults still reproduces the oracle."""
+    E = exa()
+    w, c, ms = E.autotune(3, E.PREC_FP64, 12)
+    assert w in (1, 2) and c in (1, 2)
+    assert np.all(np.isfinite(ms)) and np.all(ms > 0)
+    assert ms[(w - 1) * 2 + (c - 1)] == ms.min()
+    s = E.ADERDGSolver(3)
+    c3 = O.cfg(3)
+    luh = smooth_cell_state(3, 0)
+    dx = np.array([1 / 27] * 3)
+    dt = 0.9 * O.stable_time_step_size(c3, luh, dx)
+    g = s.spaceTimePredictor(luh, dx, dt)
+    o = O.space_time_predictor(c3, luh, dx, dt)
+    assert rel_err(g["lQhbnd"], o["lQhbnd"]) <= TOL
