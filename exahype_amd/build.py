@@ -10,7 +10,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libexahype_amd.so")
-SOURCES = ["capi.hip", "fv_capi.hip", "limiter_capi.hip", "amr_capi.hip", "device_capi.hip"]
+SOURCES = ["capi.hip"] + ["pred_n%d.hip" % n for n in range(1, 8)] + [
+    "fv_capi.hip", "limiter_capi.hip", "amr_capi.hip", "device_capi.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -40,7 +41,7 @@ def build(verbose=False, jobs=None):
             print(" ".join(cmd), flush=True)
         subprocess.check_call(cmd)
 
-    jobs = jobs or min(len(SOURCES), max(1, min(8, os.cpu_count() or 1)))
+    jobs = jobs or min(len(SOURCES), max(1, min(12, os.cpu_count() or 1)))
     with ThreadPoolExecutor(jobs) as ex:
         list(ex.map(compile_one, range(len(SOURCES))))
     tmp = LIB + ".tmp"

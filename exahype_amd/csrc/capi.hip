@@ -10,11 +10,11 @@
 #include <vector>
 
 #include "aderdg_kernels.h"
+#include "grid_kernels.h"
+#include "predictor_tu.h"
 #include "host_util.h"
 #include "dg_tables_data.h"
 
-__constant__ DGTables c_tab[EXA_MAX_ORDER + 1];
-__constant__ DGTablesF c_tabf[EXA_MAX_ORDER + 1];
 
 static thread_local std::string g_err;
 
@@ -50,43 +50,23 @@ extern "C" const char* exa_version(void) { return "exahype_amd 0.1 gfx950"; }
 static std::mutex g_init_mu;
 static unsigned long long g_init_mask = 0;
 
-// Predictor kernels instantiated: every precision and contraction at one
-// wave per SIMD (with and without the per-cell debug outputs) and, for
-// N <= 5, at two waves per SIMD.
-using PredFn = void (*)(const double*, double*, double*, double*, int*, const double*, double, double, double, double,
-                        int, unsigned long long*, exa::PredictorDebug, exa::Ens);
-
-template <int N, bool DEBUG, int WPE, bool NODE>
-static PredFn pred_by_prec(int prec) {
-  switch (prec) {
-    case EXA_PREC_FP32: return &exa::k_predictor<N, DEBUG, exa::PRED_FP32, WPE, NODE>;
-    case EXA_PREC_FP32_MFMA: return &exa::k_predictor<N, DEBUG, exa::PRED_FP32_MFMA, WPE, NODE>;
-    case EXA_PREC_BF16_MFMA: return &exa::k_predictor<N, DEBUG, exa::PRED_BF16_MFMA, WPE, NODE>;
-    default: return &exa::k_predictor<N, DEBUG, exa::PRED_FP64, WPE, NODE>;
-  }
-}
-
+// Predictor kernels live in the per-order units pred_n<N>.hip.
 template <int N>
 static PredFn predictor_kernel(bool debug, int prec, int wpe, bool node) {
-  if (debug) return node ? pred_by_prec<N, true, 1, true>(prec) : pred_by_prec<N, true, 1, false>(prec);
-  if (N <= 5 && wpe == 2)
-    return node ? pred_by_prec<N, false, (N <= 5 ? 2 : 1), true>(prec)
-                : pred_by_prec<N, false, (N <= 5 ? 2 : 1), false>(prec);
-  return node ? pred_by_prec<N, false, 1, true>(prec) : pred_by_prec<N, false, 1, false>(prec);
+  switch (N) {
+    case 1: return exa_pred_kernel_1(debug, prec, wpe, node);
+    case 2: return exa_pred_kernel_2(debug, prec, wpe, node);
+    case 3: return exa_pred_kernel_3(debug, prec, wpe, node);
+    case 4: return exa_pred_kernel_4(debug, prec, wpe, node);
+    case 5: return exa_pred_kernel_5(debug, prec, wpe, node);
+    case 6: return exa_pred_kernel_6(debug, prec, wpe, node);
+    default: return exa_pred_kernel_7(debug, prec, wpe, node);
+  }
 }
 
 template <int N>
 static hipError_t set_lds_attrs() {
   const int bytes = exa::Ord<N>::LDS_DOUBLES * (int)sizeof(double);
-  const int precs[4] = {EXA_PREC_FP64, EXA_PREC_FP32, EXA_PREC_FP32_MFMA, EXA_PREC_BF16_MFMA};
-  for (int dbg = 0; dbg < 2; ++dbg)
-    for (int w = 1; w <= 2; ++w)
-      for (int node = 0; node < 2; ++node)
-        for (int p : precs) {
-          hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(predictor_kernel<N>(dbg, p, w, node)),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-          if (e != hipSuccess) return e;
-        }
   return hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::k_volume_integral_ref<N>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
@@ -132,6 +112,13 @@ extern "C" int exa_init(int device) {
       }
     }
   EXA_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_tabf), hostf, sizeof(hostf)));
+  EXA_CHECK(exa_pred_init_1(host, hostf));
+  EXA_CHECK(exa_pred_init_2(host, hostf));
+  EXA_CHECK(exa_pred_init_3(host, hostf));
+  EXA_CHECK(exa_pred_init_4(host, hostf));
+  EXA_CHECK(exa_pred_init_5(host, hostf));
+  EXA_CHECK(exa_pred_init_6(host, hostf));
+  EXA_CHECK(exa_pred_init_7(host, hostf));
   EXA_CHECK(set_lds_attrs<1>());
   EXA_CHECK(set_lds_attrs<2>());
   EXA_CHECK(set_lds_attrs<3>());

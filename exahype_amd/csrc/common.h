@@ -34,7 +34,9 @@ struct DGTables {
   double iK1[EXA_NB][EXA_NB];  // inverse time matrix iK1[t'][t]
 };
 
-extern __constant__ DGTables c_tab[EXA_MAX_ORDER + 1];
+// one copy per translation unit; every unit that launches kernels reading the
+// tables uploads them at exa_init
+static __constant__ DGTables c_tab[EXA_MAX_ORDER + 1];
 
 // The same tables rounded to FP32, for the low-precision predictor sweeps.
 struct DGTablesF {
@@ -44,7 +46,7 @@ struct DGTablesF {
   float iK1[EXA_NB][EXA_NB];
 };
 
-extern __constant__ DGTablesF c_tabf[EXA_MAX_ORDER + 1];
+static __constant__ DGTablesF c_tabf[EXA_MAX_ORDER + 1];
 
 // PNPM factors of the CFL condition, kernels/aderdg/generic/c/3d/stableTimeStepSize.cpph:34-35
 __host__ __device__ inline double exa_pnpm(int N) {

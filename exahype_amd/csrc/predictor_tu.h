@@ -1,0 +1,30 @@
+// Interface of the per-order translation units that instantiate the
+// predictor kernels (pred_n<N>.hip): compiling the orders in parallel units
+// keeps the build short.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+namespace exa {
+struct PredictorDebug;
+struct Ens;
+}  // namespace exa
+
+using PredFn = void (*)(const double*, double*, double*, double*, int*, const double*, double, double, double, double,
+                        int, unsigned long long*, exa::PredictorDebug, exa::Ens);
+
+// Kernel of one variant: per-cell debug outputs (one wave per SIMD only),
+// precision EXA_PREC_*, waves per SIMD (2 is built for N <= 5), node-owner
+// contraction or pencil-owner contraction.
+#define EXA_PRED_TU_DECL(N)                                                   \
+  PredFn exa_pred_kernel_##N(bool debug, int prec, int wpe, bool node);       \
+  hipError_t exa_pred_init_##N(const DGTables* tab, const DGTablesF* tabf);
+EXA_PRED_TU_DECL(1)
+EXA_PRED_TU_DECL(2)
+EXA_PRED_TU_DECL(3)
+EXA_PRED_TU_DECL(4)
+EXA_PRED_TU_DECL(5)
+EXA_PRED_TU_DECL(6)
+EXA_PRED_TU_DECL(7)
+#undef EXA_PRED_TU_DECL
