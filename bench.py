@@ -30,7 +30,7 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_BF16_TFLOPS = 2500.0
 # mixed-precision variants measured: (name, precision, FP64 correction sweeps)
-MIXED_VARIANTS = [("fp32", 1, 1), ("fp32_mfma", 2, 1), ("bf16_mfma", 3, 2)]
+MIXED_VARIANTS = [("fp32", 1, 1), ("fp32_mfma", 2, 1), ("bf16_mfma", 3, 2), ("bf16_mfma_3", 3, 3)]
 PHI_FLUX = 40   # flops of one 3D Euler flux evaluation (1 division counted as 1)
 NVAR = 5
 
@@ -41,6 +41,21 @@ def predictor_flops_per_cell(N, iters):
     volume integral (14 V B^4) and the face extrapolation (24 V B^3)."""
     B = N + 1
     return iters * (8 * NVAR * B ** 5 + B ** 4 * (PHI_FLUX + 5 * NVAR)) + 14 * NVAR * B ** 4 + 24 * NVAR * B ** 3
+
+
+def measured_traffic(kernel, n, N):
+    """HBM bytes per launch of `kernel` from the counter runs summarised in
+    profiles/traffic.json (FETCH_SIZE and WRITE_SIZE, each scaled by what the
+    counter reads for the streaming copy kernel), or None if that kernel and
+    workload were not measured."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            rec = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not rec or rec.get("cells") != n ** 3 or rec.get("order") != N:
+        return None
+    return rec["bytes_per_launch"]
 
 
 def vortex_ic_kwargs(seed=0):
@@ -188,8 +203,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ({"fp32": "f32+f64", "fp32_mfma": "f32+f64", "bf16_mfma": "bf16+f32+f64"}[best["name"]]
-                  if mixed_ok else "f64"),
+        "dtype": (("bf16+f32+f64" if best["prec"] == E.PREC_BF16_MFMA else "f32+f64") if mixed_ok else "f64"),
         "data": "synthetic",
         "config": {"workload": "3D Euler ADER-DG, order %d, %d^3 periodic cells, vortex + 8 density modes, CFL 0.9"
                                % (N, n), "cells": ncells, "order": N, "dof_per_step": dof,
@@ -198,7 +212,7 @@ def main():
                    "waves_per_simd": args.waves_per_simd or "tuned", "contraction": args.contraction or "tuned",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": measured_traffic(kname, n, N),
                      "kernel": kname, "avg_launch_ms": pred_ms,
                      "picard_sweeps_per_cell_step": sweeps, "flop_per_launch": flops},
         "phase_ms_per_step": {"predictor": pred_ms,

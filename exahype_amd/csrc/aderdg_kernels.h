@@ -136,9 +136,13 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
   constexpr bool DP = std::is_same<T, double>::value;
   const DGTables& Td = c_tab[N];
   const DGTablesF& Tf = c_tabf[N];
+  // tz is an opaque zero (see the asm in the slice loop): the Kxi and iK1
+  // reads through it are re-issued every slice as scalar loads instead of
+  // being hoisted out of the sweep into registers the kernel cannot spare
+  int tz = 0;
   auto W = [&](int i) -> T { return DP ? (T)Td.w[i] : (T)Tf.w[i]; };
-  auto KXI = [&](int a, int b) -> T { return DP ? (T)Td.Kxi[a][b] : (T)Tf.Kxi[a][b]; };
-  auto IK1 = [&](int a, int b) -> T { return DP ? (T)Td.iK1[a][b] : (T)Tf.iK1[a][b]; };
+  auto KXI = [&](int a, int b) -> T { return DP ? (T)c_tab[N + tz].Kxi[a][b] : (T)c_tabf[N + tz].Kxi[a][b]; };
+  auto IK1 = [&](int a, int b) -> T { return DP ? (T)c_tab[N + tz].iK1[a][b] : (T)c_tabf[N + tz].iK1[a][b]; };
   auto F0 = [&](int i) -> T { return DP ? (T)Td.F0[i] : (T)Tf.F0[i]; };
 
   T uu[V];
@@ -160,6 +164,7 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
       for (int v = 0; v < V; ++v) qn[t][v] = T(0);
 #pragma unroll
     for (int t = 0; t < B; ++t) {
+      asm volatile("" : "+s"(tz));
       // phase F
       if (act) {
         T F[3][V];
@@ -273,9 +278,13 @@ __device__ inline int picard_sweeps_node(T (&q)[Ord<N>::B][V], const double (&u)
   constexpr bool DP = std::is_same<T, double>::value;
   const DGTables& Td = c_tab[N];
   const DGTablesF& Tf = c_tabf[N];
+  // tz is an opaque zero (see the asm in the slice loop): the Kxi and iK1
+  // reads through it are re-issued every slice as scalar loads instead of
+  // being hoisted out of the sweep into registers the kernel cannot spare
+  int tz = 0;
   auto W = [&](int i) -> T { return DP ? (T)Td.w[i] : (T)Tf.w[i]; };
-  auto KXI = [&](int a, int b) -> T { return DP ? (T)Td.Kxi[a][b] : (T)Tf.Kxi[a][b]; };
-  auto IK1 = [&](int a, int b) -> T { return DP ? (T)Td.iK1[a][b] : (T)Tf.iK1[a][b]; };
+  auto KXI = [&](int a, int b) -> T { return DP ? (T)c_tab[N + tz].Kxi[a][b] : (T)c_tabf[N + tz].Kxi[a][b]; };
+  auto IK1 = [&](int a, int b) -> T { return DP ? (T)c_tab[N + tz].iK1[a][b] : (T)c_tabf[N + tz].iK1[a][b]; };
   auto F0 = [&](int i) -> T { return DP ? (T)Td.F0[i] : (T)Tf.F0[i]; };
 
   T uu[V];
@@ -297,6 +306,7 @@ __device__ inline int picard_sweeps_node(T (&q)[Ord<N>::B][V], const double (&u)
       for (int v = 0; v < V; ++v) qn[t][v] = T(0);
 #pragma unroll
     for (int t = 0; t < B; ++t) {
+      asm volatile("" : "+s"(tz));
       T* buf = sF + (t & 1) * 3 * V * B3;
       if (act) {
         T F[3][V];
@@ -483,6 +493,7 @@ __device__ inline int picard_sweeps_mfma(float (&q)[Ord<N>::B][V], const double 
   const int ox = (z * B + y) * B + x, oy = (z * B + x) * B + y, oz = (y * B + x) * B + z;
   const int abx = z * B + y, aby = z * B + x, abz = y * B + x;
 
+  int tz = 0;  // opaque zero, as in picard_sweeps
   int it = 0;
   while (it < max_sweeps) {
     ++it;
@@ -493,6 +504,7 @@ __device__ inline int picard_sweeps_mfma(float (&q)[Ord<N>::B][V], const double 
       for (int v = 0; v < V; ++v) qn[t][v] = 0.0f;
 #pragma unroll
     for (int t = 0; t < B; ++t) {
+      asm volatile("" : "+s"(tz));
       if (act) {
         float F[3][V];
         EulerPDE::flux<float>(q[t], F, tg);
@@ -514,7 +526,7 @@ __device__ inline int picard_sweeps_mfma(float (&q)[Ord<N>::B][V], const double 
           const float r = ((c0 * uu[v] - sP[(0 * V + v) * B3 + ox]) - sP[(1 * V + v) * B3 + oy]) - sP[(2 * V + v) * B3 + oz];
           const float ri = iw3 * r;
 #pragma unroll
-          for (int t2 = 0; t2 < B; ++t2) qn[t2][v] += ri * Tf.iK1[t2][t];
+          for (int t2 = 0; t2 < B; ++t2) qn[t2][v] += ri * c_tabf[N + tz].iK1[t2][t];
         }
       }
     }

@@ -6,8 +6,9 @@ Bar (floating point, stated here): after 50 steps of the 3D Euler vortex
 setup the L2 difference to the FP64 path is <= 1e-6 for every variable
 (L2 over the domain with the Gauss-Legendre quadrature of each cell); the
 per-cell predictor outputs differ from the FP64 oracle by at most 5e-6 of the
-output's scale.  BF16 runs two FP64 correction sweeps (its low-precision
-fixed point is only ~1e-3 accurate), the FP32 forms one.
+output's scale.  BF16 needs three FP64 correction sweeps to meet the bar (its
+low-precision fixed point is only ~1e-3 accurate; with two the 50-step L2 is
+1e-5..1e-4 on this setup), the FP32 forms one.
 """
 import numpy as np
 import pytest
@@ -32,7 +33,7 @@ def l2_per_variable(a, b, N, ncells, h3):
     return np.sqrt(np.einsum("n,cnv->v", w3, d ** 2) * h3)
 
 
-PRECS = [(1, 1), (2, 1), (3, 2)]  # (precision, FP64 correction sweeps)
+PRECS = [(1, 1), (2, 1), (3, 3)]  # (precision, FP64 correction sweeps)
 
 
 def test_mfma_operand_layouts():
