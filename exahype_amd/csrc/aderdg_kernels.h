@@ -143,7 +143,8 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
   auto W = [&](int i) -> T { return DP ? (T)Td.w[i] : (T)Tf.w[i]; };
   auto KXI = [&](int a, int b) -> T { return DP ? (T)c_tab[N + tz].Kxi[a][b] : (T)c_tabf[N + tz].Kxi[a][b]; };
   auto IK1 = [&](int a, int b) -> T { return DP ? (T)c_tab[N + tz].iK1[a][b] : (T)c_tabf[N + tz].iK1[a][b]; };
-  auto F0 = [&](int i) -> T { return DP ? (T)Td.F0[i] : (T)Tf.F0[i]; };
+  // (F0 too: w^3 F0[t] u would otherwise be hoisted for every t, 30 registers)
+  auto F0 = [&](int i) -> T { return DP ? (T)c_tab[N + tz].F0[i] : (T)c_tabf[N + tz].F0[i]; };
 
   T uu[V];
 #pragma unroll
@@ -285,7 +286,8 @@ __device__ inline int picard_sweeps_node(T (&q)[Ord<N>::B][V], const double (&u)
   auto W = [&](int i) -> T { return DP ? (T)Td.w[i] : (T)Tf.w[i]; };
   auto KXI = [&](int a, int b) -> T { return DP ? (T)c_tab[N + tz].Kxi[a][b] : (T)c_tabf[N + tz].Kxi[a][b]; };
   auto IK1 = [&](int a, int b) -> T { return DP ? (T)c_tab[N + tz].iK1[a][b] : (T)c_tabf[N + tz].iK1[a][b]; };
-  auto F0 = [&](int i) -> T { return DP ? (T)Td.F0[i] : (T)Tf.F0[i]; };
+  // (F0 too: w^3 F0[t] u would otherwise be hoisted for every t, 30 registers)
+  auto F0 = [&](int i) -> T { return DP ? (T)c_tab[N + tz].F0[i] : (T)c_tabf[N + tz].F0[i]; };
 
   T uu[V];
 #pragma unroll
@@ -520,7 +522,7 @@ __device__ inline int picard_sweeps_mfma(float (&q)[Ord<N>::B][V], const double 
       contract_mfma<N, BF16>(sF, sP, kf);
       __syncthreads();
       if (act) {
-        const float c0 = w3 * Tf.F0[t];
+        const float c0 = w3 * c_tabf[N + tz].F0[t];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
           const float r = ((c0 * uu[v] - sP[(0 * V + v) * B3 + ox]) - sP[(1 * V + v) * B3 + oy]) - sP[(2 * V + v) * B3 + oz];
