@@ -55,8 +55,12 @@ __host__ __device__ inline double exa_pnpm(int N) {
 }
 
 // positive doubles order like their bit patterns: min via integer atomics
+// The minimum only decreases: a value not below the current one skips the
+// atomic, so the hundreds of thousands of per-cell candidates of a step cost
+// a plain load each and only the few improving ones contend for the address.
 __device__ inline void atomic_min_pos_double(unsigned long long* addr, double v) {
-  atomicMin(addr, (unsigned long long)__double_as_longlong(v));
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  if (b < __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(addr, b);
 }
 
 template <typename T>
