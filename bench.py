@@ -128,7 +128,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--waves-per-simd", type=int, default=0, help="predictor occupancy target (0 = tuned)")
     ap.add_argument("--contraction", type=int, default=0,
-                    help="predictor sweep contraction: 0 tuned, 1 pencil owners, 2 node owners")
+                    help="predictor sweep contraction: 0 tuned, 1 pencil owners, 2 node owners, "
+                         "3 pencil owners two slices at a time")
     ap.add_argument("--workload", default="aderdg", choices=["aderdg", "fv2d", "ensemble"],
                     help="aderdg: the headline ADER-DG step; fv2d: the EulerFV demonstrator "
                          "(2D Godunov, 81x81 patches of 10x10 volumes, 200 steps); ensemble: independent "
@@ -338,7 +339,7 @@ def run_ensemble(args):
         cur.append(i)
         size += member_bytes(m)
     waves.append(cur)
-    prec, sweeps = E.PREC_FP32_MFMA, 1
+    prec, sweeps = E.PREC_FP32, 1
     dof_steps, wall_mx, wall_64, l2s = 0, 0.0, 0.0, [None] * len(members)
     for w in waves:
         ms = [members[i] for i in w]
@@ -376,7 +377,7 @@ def run_ensemble(args):
             "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
             "config": {"workload": "ensemble: order ~ U{3..7}, cells ~ U{9^3,27^3,45^3}, gamma ~ U[1.3,1.67], "
                                    "seeded smooth states, 20 steps, shared launches per order, one hipGraph per step",
-                       "members": len(members), "waves": len(waves), "predictor": "fp32_mfma + 1 FP64 sweep",
+                       "members": len(members), "waves": len(waves), "predictor": "fp32 + 1 FP64 sweep",
                        "orders": {str(N): sum(1 for m in members if m["order"] == N) for N in range(3, 8)}},
             "fp64": {"dof_updates_per_s": dof_steps / wall_64, "speedup_mixed_vs_fp64": wall_64 / wall_mx},
             "l2_mixed_vs_fp64": {"max": float(l2a.max()), "median": float(np.median(l2a)),

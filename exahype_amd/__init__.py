@@ -26,7 +26,7 @@ PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_FP32_MFMA = 2
 PREC_BF16_MFMA = 3
-CONTRACT_DEFAULT, CONTRACT_PENCIL, CONTRACT_NODE = 0, 1, 2
+CONTRACT_DEFAULT, CONTRACT_PENCIL, CONTRACT_NODE, CONTRACT_PENCIL_PAIR = 0, 1, 2, 3
 PDE_EULER = 0
 PDE_EULERFV2D = 1
 FV_GODUNOV = 0
@@ -715,6 +715,6 @@ def autotune(order, precision=PREC_FP64, ncells=32):
     L = load_library()
     _check(L.exa_init(0))
     w, c = C.c_int(0), C.c_int(0)
-    ms = np.zeros(4)
+    ms = np.zeros(6)
     _check(L.exa_autotune(order, precision, ncells, C.byref(w), C.byref(c), _p(ms)))
     return w.value, c.value, ms

@@ -22,7 +22,12 @@ template <int N>
 struct Ord {
   static constexpr int B = N + 1, B2 = B * B, B3 = B2 * B, B4 = B2 * B2;
   static constexpr int NT = round_up64(B3);          // threads per cell
-  static constexpr int LDS_DOUBLES = 6 * V * B3 + 3 * B3 + 16;  // flux slices + pencil constants + reduction scratch
+  // LDS stride between the pencils of a direction layout: odd, so that lanes
+  // addressing consecutive pencils (phase P) or the same node of consecutive
+  // pencils (phases F and A of the y and z layouts) fall into distinct banks
+  static constexpr int PS = (B % 2 == 0) ? B + 1 : B;
+  static constexpr int BLK = B2 * PS;                // one (direction, variable) block of pencils
+  static constexpr int LDS_DOUBLES = 6 * V * BLK + 3 * B3 + 16;  // flux slices + pencil constants + reduction scratch
 };
 
 // ---------------------------------------------------------------------------
@@ -126,12 +131,12 @@ struct Ens {
 // correction sweeps, float for the low-precision sweeps of the mixed
 // predictor.  Returns the sweeps executed; stops once
 // sum((q_new-q_old)^2) < tol2 over the cell (block-uniform decision).
-template <int N, typename T, bool ACC_FH, bool DEBUG>
+template <int N, typename T, bool ACC_FH, bool DEBUG, bool PAIR = false>
 __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], T* sF, T* sP, const T* sc,
                                     double* red, bool act, int s, int z, int y, int x, double gamma, int max_sweeps,
                                     double tol2, double (&fh)[3][V], PredictorDebug dbg) {
   using O = Ord<N>;
-  constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
+  constexpr int B = O::B, B2 = O::B2, B3 = O::B3, PS = O::PS, BLK = O::BLK;
   constexpr int NITEMS = 3 * V * B2;
   constexpr bool DP = std::is_same<T, double>::value;
   const DGTables& Td = c_tab[N];
@@ -153,7 +158,7 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
   const T iw3 = T(1.0) / w3;
   const T tg = (T)gamma;
   // offsets of this node in the three direction layouts (flux and partials)
-  const int ox = (z * B + y) * B + x, oy = (z * B + x) * B + y, oz = (y * B + x) * B + z;
+  const int ox = (z * B + y) * PS + x, oy = (z * B + x) * PS + y, oz = (y * B + x) * PS + z;
 
   int it = 0;
   while (it < max_sweeps) {
@@ -163,51 +168,115 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
     for (int t = 0; t < B; ++t)
 #pragma unroll
       for (int v = 0; v < V; ++v) qn[t][v] = T(0);
+    if constexpr (!PAIR) {
 #pragma unroll
-    for (int t = 0; t < B; ++t) {
-      asm volatile("" : "+s"(tz));
-      // phase F
-      if (act) {
-        T F[3][V];
-        EulerPDE::flux<T>(q[t], F, tg);
+      for (int t = 0; t < B; ++t) {
+        asm volatile("" : "+s"(tz));
+        // phase F
+        if (act) {
+          T F[3][V];
+          EulerPDE::flux<T>(q[t], F, tg);
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-          sF[(0 * V + v) * B3 + ox] = F[0][v];
-          sF[(1 * V + v) * B3 + oy] = F[1][v];
-          sF[(2 * V + v) * B3 + oz] = F[2][v];
+          for (int v = 0; v < V; ++v) {
+            sF[(0 * V + v) * BLK + ox] = F[0][v];
+            sF[(1 * V + v) * BLK + oy] = F[1][v];
+            sF[(2 * V + v) * BLK + oz] = F[2][v];
+          }
+        }
+        __syncthreads();
+        // phase P: item = (d*V + v, pencil ab); pencils and partials are
+        // contiguous runs of B in the direction layouts, so the item needs no
+        // direction-dependent addressing
+        for (int i = s; i < NITEMS; i += O::NT) {
+          const int dv = i / B2, ab = i - dv * B2, d = dv / V;
+          const T c = sc[(d * B + t) * B2 + ab];
+          const T* pf = sF + dv * BLK + ab * PS;
+          T f[B];
+#pragma unroll
+          for (int n = 0; n < B; ++n) f[n] = c * pf[n];
+          T* pp = sP + dv * BLK + ab * PS;
+#pragma unroll
+          for (int l = 0; l < B; ++l) {
+            T acc = T(0);
+#pragma unroll
+            for (int n = 0; n < B; ++n) acc += f[n] * KXI(n, l);
+            pp[l] = acc;
+          }
+        }
+        __syncthreads();
+        // phase A
+        if (act) {
+          const T c0 = w3 * F0(t);
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            const T r = ((c0 * uu[v] - sP[(0 * V + v) * BLK + ox]) - sP[(1 * V + v) * BLK + oy]) - sP[(2 * V + v) * BLK + oz];
+            const T ri = iw3 * r;
+#pragma unroll
+            for (int t2 = 0; t2 < B; ++t2) qn[t2][v] += ri * IK1(t2, t);
+          }
         }
       }
-      __syncthreads();
-      // phase P: item = (d*V + v, pencil ab); pencils and partials are
-      // contiguous runs of B in the direction layouts, so the item needs no
-      // direction-dependent addressing
-      for (int i = s; i < NITEMS; i += O::NT) {
-        const int dv = i / B2, ab = i - dv * B2, d = dv / V;
-        const T c = sc[(d * B + t) * B2 + ab];
-        const T* pf = sF + dv * B3 + ab * B;
-        T f[B];
+    } else {
+      // two slices per phase, contraction in place (an item reads its whole
+      // pencil before writing it): slice t0 in sF, slice t0+1 in sP; the
+      // last slice of an odd B goes alone.  Three barriers per pair.
 #pragma unroll
-        for (int n = 0; n < B; ++n) f[n] = c * pf[n];
-        T* pp = sP + dv * B3 + ab * B;
+      for (int t0 = 0; t0 < B; t0 += 2) {
+        asm volatile("" : "+s"(tz));
+        const int ns = t0 + 1 < B ? 2 : 1;
+        if (act) {
 #pragma unroll
-        for (int l = 0; l < B; ++l) {
-          T acc = T(0);
+          for (int k = 0; k < 2; ++k) {
+            if (k < ns) {
+              T* buf = k == 0 ? sF : sP;
+              T F[3][V];
+              EulerPDE::flux<T>(q[t0 + k], F, tg);
 #pragma unroll
-          for (int n = 0; n < B; ++n) acc += f[n] * KXI(n, l);
-          pp[l] = acc;
+              for (int v = 0; v < V; ++v) {
+                buf[(0 * V + v) * BLK + ox] = F[0][v];
+                buf[(1 * V + v) * BLK + oy] = F[1][v];
+                buf[(2 * V + v) * BLK + oz] = F[2][v];
+              }
+            }
+          }
         }
-      }
-      __syncthreads();
-      // phase A
-      if (act) {
-        const T c0 = w3 * F0(t);
+        __syncthreads();
+        for (int i = s; i < ns * NITEMS; i += O::NT) {
+          const int k = i >= NITEMS ? 1 : 0;
+          const int ii = i - k * NITEMS;
+          const int dv = ii / B2, ab = ii - dv * B2, d = dv / V;
+          const T c = sc[(d * B + t0 + k) * B2 + ab];
+          T* pf = (k == 0 ? sF : sP) + dv * BLK + ab * PS;
+          T f[B];
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-          const T r = ((c0 * uu[v] - sP[(0 * V + v) * B3 + ox]) - sP[(1 * V + v) * B3 + oy]) - sP[(2 * V + v) * B3 + oz];
-          const T ri = iw3 * r;
+          for (int n = 0; n < B; ++n) f[n] = c * pf[n];
 #pragma unroll
-          for (int t2 = 0; t2 < B; ++t2) qn[t2][v] += ri * IK1(t2, t);
+          for (int l = 0; l < B; ++l) {
+            T acc = T(0);
+#pragma unroll
+            for (int n = 0; n < B; ++n) acc += f[n] * KXI(n, l);
+            pf[l] = acc;
+          }
         }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+          for (int k = 0; k < 2; ++k) {
+            if (k < ns) {
+              const T* buf = k == 0 ? sF : sP;
+              const T c0 = w3 * F0(t0 + k);
+#pragma unroll
+              for (int v = 0; v < V; ++v) {
+                const T r =
+                    ((c0 * uu[v] - buf[(0 * V + v) * BLK + ox]) - buf[(1 * V + v) * BLK + oy]) - buf[(2 * V + v) * BLK + oz];
+                const T ri = iw3 * r;
+#pragma unroll
+                for (int t2 = 0; t2 < B; ++t2) qn[t2][v] += ri * IK1(t2, t0 + k);
+              }
+            }
+          }
+        }
+        __syncthreads();
       }
     }
     double res = 0.0;
@@ -586,7 +655,10 @@ enum : int { PRED_FP64 = 0, PRED_FP32 = 1, PRED_FP32_MFMA = 2, PRED_BF16_MFMA = 
 // reaches the FP32 rounding floor of the cell, then `fp64_sweeps` FP64
 // correction sweeps (stopping early on the FP64 criterion); the averages,
 // extrapolation and volume integral are FP64 from the last FP64 sweep.
-template <int N, bool DEBUG, int PREC, int WPE, bool NODE>
+// Contraction form of the sweeps (kernel template argument).
+enum : int { FORM_PENCIL = 0, FORM_NODE = 1, FORM_PENCIL_PAIR = 2 };
+
+template <int N, bool DEBUG, int PREC, int WPE, int FORM>
 __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_predictor(
     const double* __restrict__ luh, double* __restrict__ lduh, double* __restrict__ qf,
     double* __restrict__ ff, int* __restrict__ iters, const double* __restrict__ dtp, double dx0,
@@ -595,9 +667,10 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
   using O = Ord<N>;
   constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
   extern __shared__ double sm[];
-  double* scd = sm + 6 * V * B3;                        // pencil constants, FP64 sweeps
-  float* scf = reinterpret_cast<float*>(sm + 3 * V * B3);  // pencil constants, FP32 sweeps
-  double* red = sm + 6 * V * B3 + 3 * B3;                 // reduction scratch
+  constexpr int BLK = O::BLK;
+  double* scd = sm + 6 * V * BLK;                         // pencil constants, FP64 sweeps
+  float* scf = reinterpret_cast<float*>(sm + 3 * V * BLK);  // pencil constants, FP32 sweeps
+  double* red = sm + 6 * V * BLK + 3 * B3;                 // reduction scratch
   const DGTables& T = c_tab[N];
 
   const size_t cell = blockIdx.x;
@@ -648,33 +721,36 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
     if constexpr (PREC == PRED_FP32_MFMA || PREC == PRED_BF16_MFMA)
       it = picard_sweeps_mfma<N, PREC == PRED_BF16_MFMA>(qs, u, sFf, sFf + 3 * V * B3, scf, red, act, s, z, y, x,
                                                          gamma, MaxIterations, tol_lp);
-    else if (NODE)
+    else if constexpr (FORM == FORM_NODE)
       it = picard_sweeps_node<N, float, false, false>(qs, u, sFf, scf, red, act, s, z, y, x, gamma, MaxIterations,
                                                       tol_lp, fh, dbg);
     else
-      it = picard_sweeps<N, float, false, false>(qs, u, sFf, sFf + 3 * V * B3, scf, red, act, s, z, y, x, gamma,
-                                                 MaxIterations, tol_lp, fh, dbg);
+      it = picard_sweeps<N, float, false, false, FORM == FORM_PENCIL_PAIR>(qs, u, sFf, sFf + 3 * V * BLK, scf, red,
+                                                                           act, s, z, y, x, gamma, MaxIterations,
+                                                                           tol_lp, fh, dbg);
 #pragma unroll
     for (int t = 0; t < B; ++t)
 #pragma unroll
       for (int v = 0; v < V; ++v) q[t][v] = (double)qs[t][v];
-    if (NODE)
+    if constexpr (FORM == FORM_NODE)
       it += picard_sweeps_node<N, double, true, DEBUG>(q, u, sm, scd, red, act, s, z, y, x, gamma, fp64_sweeps,
                                                        1e-7 * 1e-7, fh, dbg);
     else
-      it += picard_sweeps<N, double, true, DEBUG>(q, u, sm, sm + 3 * V * B3, scd, red, act, s, z, y, x, gamma,
-                                                  fp64_sweeps, 1e-7 * 1e-7, fh, dbg);
+      it += picard_sweeps<N, double, true, DEBUG, FORM == FORM_PENCIL_PAIR>(q, u, sm, sm + 3 * V * BLK, scd, red, act,
+                                                                            s, z, y, x, gamma, fp64_sweeps,
+                                                                            1e-7 * 1e-7, fh, dbg);
   } else {
 #pragma unroll
     for (int t = 0; t < B; ++t)
 #pragma unroll
       for (int v = 0; v < V; ++v) q[t][v] = u[v];
-    if (NODE)
+    if constexpr (FORM == FORM_NODE)
       it = picard_sweeps_node<N, double, true, DEBUG>(q, u, sm, scd, red, act, s, z, y, x, gamma, MaxIterations,
                                                       1e-7 * 1e-7, fh, dbg);
     else
-      it = picard_sweeps<N, double, true, DEBUG>(q, u, sm, sm + 3 * V * B3, scd, red, act, s, z, y, x, gamma,
-                                                 MaxIterations, 1e-7 * 1e-7, fh, dbg);
+      it = picard_sweeps<N, double, true, DEBUG, FORM == FORM_PENCIL_PAIR>(q, u, sm, sm + 3 * V * BLK, scd, red, act, s,
+                                                                           z, y, x, gamma, MaxIterations, 1e-7 * 1e-7,
+                                                                           fh, dbg);
   }
 
   // time averages (aderPredictorNonlinear :326-397)

@@ -52,15 +52,15 @@ static unsigned long long g_init_mask = 0;
 
 // Predictor kernels live in the per-order units pred_n<N>.hip.
 template <int N>
-static PredFn predictor_kernel(bool debug, int prec, int wpe, bool node) {
+static PredFn predictor_kernel(bool debug, int prec, int wpe, int form) {
   switch (N) {
-    case 1: return exa_pred_kernel_1(debug, prec, wpe, node);
-    case 2: return exa_pred_kernel_2(debug, prec, wpe, node);
-    case 3: return exa_pred_kernel_3(debug, prec, wpe, node);
-    case 4: return exa_pred_kernel_4(debug, prec, wpe, node);
-    case 5: return exa_pred_kernel_5(debug, prec, wpe, node);
-    case 6: return exa_pred_kernel_6(debug, prec, wpe, node);
-    default: return exa_pred_kernel_7(debug, prec, wpe, node);
+    case 1: return exa_pred_kernel_1(debug, prec, wpe, form);
+    case 2: return exa_pred_kernel_2(debug, prec, wpe, form);
+    case 3: return exa_pred_kernel_3(debug, prec, wpe, form);
+    case 4: return exa_pred_kernel_4(debug, prec, wpe, form);
+    case 5: return exa_pred_kernel_5(debug, prec, wpe, form);
+    case 6: return exa_pred_kernel_6(debug, prec, wpe, form);
+    default: return exa_pred_kernel_7(debug, prec, wpe, form);
   }
 }
 
@@ -156,14 +156,19 @@ struct TunedVariant {
 };
 static TunedVariant g_tuned[64][EXA_MAX_ORDER + 1][4];
 
-static bool predictor_node_contraction(const exa_aderdg_config& c, int N, int dev) {
-  if (c.contraction == EXA_CONTRACT_PENCIL) return false;
-  if (c.contraction == EXA_CONTRACT_NODE) return true;
+static int predictor_form(const exa_aderdg_config& c, int N, int dev) {
+  switch (c.contraction) {
+    case EXA_CONTRACT_PENCIL: return exa::FORM_PENCIL;
+    case EXA_CONTRACT_NODE: return exa::FORM_NODE;
+    case EXA_CONTRACT_PENCIL_PAIR: return exa::FORM_PENCIL_PAIR;
+    default: break;
+  }
   const int tc = g_tuned[dev & 63][N][c.precision & 3].contraction;
-  if (tc) return tc == EXA_CONTRACT_NODE;
+  if (tc) return tc == EXA_CONTRACT_NODE ? exa::FORM_NODE
+                                         : (tc == EXA_CONTRACT_PENCIL_PAIR ? exa::FORM_PENCIL_PAIR : exa::FORM_PENCIL);
   // measured at 81^3 cells (FP64 and mixed): pencil owners win at N=5
   // (1.9x), node owners at N=3 (1.3x) and N=7 (1.1x)
-  return N != 5;
+  return N == 5 ? exa::FORM_PENCIL : exa::FORM_NODE;
 }
 
 extern "C" int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out) {
@@ -175,7 +180,7 @@ extern "C" int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out)
   EXA_ARG(cfg->precision >= EXA_PREC_FP64 && cfg->precision <= EXA_PREC_BF16_MFMA, "precision");
   EXA_ARG(cfg->precision < EXA_PREC_FP32_MFMA || cfg->order <= 7, "MFMA sweeps need order <= 7");
   EXA_ARG(cfg->waves_per_simd >= 0 && cfg->waves_per_simd <= 2, "waves_per_simd must be 0, 1 or 2");
-  EXA_ARG(cfg->contraction >= 0 && cfg->contraction <= 2, "contraction must be 0, 1 or 2");
+  EXA_ARG(cfg->contraction >= 0 && cfg->contraction <= 3, "contraction must be 0..3");
   int dev = 0;
   EXA_CHECK(hipGetDevice(&dev));
   EXA_TRY(exa_init(dev));
@@ -249,8 +254,8 @@ extern "C" int exa_aderdg_space_time_predictor(exa_solver* s, double* lQhbnd, do
   const double gamma = s->cfg.gamma;
   EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
-    const bool node = predictor_node_contraction(s->cfg, NN, s->device);
-    PredFn kern = predictor_kernel<NN>(true, s->cfg.precision, 1, node);
+    const int form = predictor_form(s->cfg, NN, s->device);
+    PredFn kern = predictor_kernel<NN>(true, s->cfg.precision, 1, form);
     hipLaunchKernelGGL(kern, dim3(1), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), s->stream, s->luh.as<double>(),
                        s->lduh.as<double>(), s->qf.as<double>(), s->ff.as<double>(), s->iters.as<int>(),
                        s->dt.as<double>(), dx[0], dx[1], dx[2], gamma, s->cfg.fp64_sweeps, nullptr, dbg, exa::Ens{});
@@ -686,8 +691,8 @@ static int enqueue_step(exa_grid* g, bool timed) {
   EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
     const int wpe = predictor_waves_per_simd(g->s->cfg, NN, g->s->device);
-    const bool node = predictor_node_contraction(g->s->cfg, NN, g->s->device);
-    PredFn kern = predictor_kernel<NN>(false, g->s->cfg.precision, wpe, node);
+    const int form = predictor_form(g->s->cfg, NN, g->s->device);
+    PredFn kern = predictor_kernel<NN>(false, g->s->cfg.precision, wpe, form);
     hipLaunchKernelGGL(kern, dim3((unsigned)g->nc), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), g->stream,
                        g->luh.as<double>(), g->lduh.as<double>(), g->qf.as<double>(), g->ff.as<double>(),
                        g->iters.as<int>(), g->dt_cur(), g->dx[0], g->dx[1], g->dx[2], g->s->cfg.gamma,
@@ -1015,8 +1020,8 @@ static int enqueue_ensemble_step(exa_ensemble* e) {
     EXA_ORDER_SWITCH(gr->order, {
       using O = exa::Ord<NN>;
       const int wpe = predictor_waves_per_simd(cfg, NN, gr->s->device);
-      const bool node = predictor_node_contraction(cfg, NN, gr->s->device);
-      PredFn kern = predictor_kernel<NN>(false, cfg.precision, wpe, node);
+      const int form = predictor_form(cfg, NN, gr->s->device);
+      PredFn kern = predictor_kernel<NN>(false, cfg.precision, wpe, form);
       hipLaunchKernelGGL(kern, dim3((unsigned)gr->nc), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), e->stream,
                          gr->luh.as<double>(), gr->lduh.as<double>(), gr->qf.as<double>(), gr->ff.as<double>(),
                          gr->iters.as<int>(), nullptr, 1.0, 1.0, 1.0, 1.4, cfg.fp64_sweeps,
@@ -1115,7 +1120,7 @@ extern "C" int exa_ensemble_total_picard_iterations(exa_ensemble* e, long long* 
 // contraction form) pair of one order and precision on a synthetic vortex
 // grid and records the fastest for this device.
 extern "C" int exa_autotune(int order, int precision, int ncells, int* waves_per_simd, int* contraction,
-                            double ms[4]) {
+                            double ms[6]) {
   EXA_ARG(order >= 1 && order <= 7, "order must be 1..7");
   EXA_ARG(precision >= EXA_PREC_FP64 && precision <= EXA_PREC_BF16_MFMA, "precision");
   EXA_ARG(ncells >= 4 && ncells <= 128, "ncells must be 4..128");
@@ -1132,7 +1137,7 @@ extern "C" int exa_autotune(int order, int precision, int ncells, int* waves_per
   double best = 1e300;
   int bw = 0, bc = 0, k = 0;
   for (int w = 1; w <= 2; ++w)
-    for (int cform = EXA_CONTRACT_PENCIL; cform <= EXA_CONTRACT_NODE; ++cform, ++k) {
+    for (int cform = EXA_CONTRACT_PENCIL; cform <= EXA_CONTRACT_PENCIL_PAIR; ++cform, ++k) {
       exa_aderdg_config cfg{};
       cfg.order = order;
       cfg.nvar = 5;

@@ -9,22 +9,31 @@
 namespace {
 constexpr int NP = EXA_PRED_ORDER;
 
-template <bool DEBUG, int WPE, bool NODE>
+template <bool DEBUG, int WPE, int FORM>
 PredFn by_prec(int prec) {
   switch (prec) {
-    case EXA_PREC_FP32: return &exa::k_predictor<NP, DEBUG, exa::PRED_FP32, WPE, NODE>;
-    case EXA_PREC_FP32_MFMA: return &exa::k_predictor<NP, DEBUG, exa::PRED_FP32_MFMA, WPE, NODE>;
-    case EXA_PREC_BF16_MFMA: return &exa::k_predictor<NP, DEBUG, exa::PRED_BF16_MFMA, WPE, NODE>;
-    default: return &exa::k_predictor<NP, DEBUG, exa::PRED_FP64, WPE, NODE>;
+    case EXA_PREC_FP32: return &exa::k_predictor<NP, DEBUG, exa::PRED_FP32, WPE, FORM>;
+    case EXA_PREC_FP32_MFMA: return &exa::k_predictor<NP, DEBUG, exa::PRED_FP32_MFMA, WPE, FORM>;
+    case EXA_PREC_BF16_MFMA: return &exa::k_predictor<NP, DEBUG, exa::PRED_BF16_MFMA, WPE, FORM>;
+    default: return &exa::k_predictor<NP, DEBUG, exa::PRED_FP64, WPE, FORM>;
+  }
+}
+
+template <bool DEBUG, int WPE>
+PredFn by_form(int prec, int form) {
+  switch (form) {
+    case exa::FORM_NODE: return by_prec<DEBUG, WPE, exa::FORM_NODE>(prec);
+    case exa::FORM_PENCIL_PAIR: return by_prec<DEBUG, WPE, exa::FORM_PENCIL_PAIR>(prec);
+    default: return by_prec<DEBUG, WPE, exa::FORM_PENCIL>(prec);
   }
 }
 }  // namespace
 
-PredFn EXA_CAT(exa_pred_kernel_, EXA_PRED_ORDER)(bool debug, int prec, int wpe, bool node) {
-  if (debug) return node ? by_prec<true, 1, true>(prec) : by_prec<true, 1, false>(prec);
+PredFn EXA_CAT(exa_pred_kernel_, EXA_PRED_ORDER)(bool debug, int prec, int wpe, int form) {
+  if (debug) return by_form<true, 1>(prec, form);
   constexpr int W2 = NP <= 5 ? 2 : 1;
-  if (wpe == 2) return node ? by_prec<false, W2, true>(prec) : by_prec<false, W2, false>(prec);
-  return node ? by_prec<false, 1, true>(prec) : by_prec<false, 1, false>(prec);
+  if (wpe == 2) return by_form<false, W2>(prec, form);
+  return by_form<false, 1>(prec, form);
 }
 
 hipError_t EXA_CAT(exa_pred_init_, EXA_PRED_ORDER)(const DGTables* tab, const DGTablesF* tabf) {
@@ -36,10 +45,10 @@ hipError_t EXA_CAT(exa_pred_init_, EXA_PRED_ORDER)(const DGTables* tab, const DG
   const int precs[4] = {EXA_PREC_FP64, EXA_PREC_FP32, EXA_PREC_FP32_MFMA, EXA_PREC_BF16_MFMA};
   for (int dbg = 0; dbg < 2; ++dbg)
     for (int w = 1; w <= 2; ++w)
-      for (int node = 0; node < 2; ++node)
+      for (int form = 0; form < 3; ++form)
         for (int p : precs) {
           e = hipFuncSetAttribute(
-              reinterpret_cast<const void*>(EXA_CAT(exa_pred_kernel_, EXA_PRED_ORDER)(dbg, p, w, node)),
+              reinterpret_cast<const void*>(EXA_CAT(exa_pred_kernel_, EXA_PRED_ORDER)(dbg, p, w, form)),
               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
           if (e != hipSuccess) return e;
         }

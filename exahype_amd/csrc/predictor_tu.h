@@ -15,10 +15,11 @@ using PredFn = void (*)(const double*, double*, double*, double*, int*, const do
                         int, unsigned long long*, exa::PredictorDebug, exa::Ens);
 
 // Kernel of one variant: per-cell debug outputs (one wave per SIMD only),
-// precision EXA_PREC_*, waves per SIMD (2 is built for N <= 5), node-owner
-// contraction or pencil-owner contraction.
+// precision EXA_PREC_*, waves per SIMD (2 is built for N <= 5), contraction
+// form exa::FORM_* (pencil owners, node owners, pencil owners two slices at
+// a time).
 #define EXA_PRED_TU_DECL(N)                                                   \
-  PredFn exa_pred_kernel_##N(bool debug, int prec, int wpe, bool node);       \
+  PredFn exa_pred_kernel_##N(bool debug, int prec, int wpe, int form);        \
   hipError_t exa_pred_init_##N(const DGTables* tab, const DGTablesF* tabf);
 EXA_PRED_TU_DECL(1)
 EXA_PRED_TU_DECL(2)

@@ -103,16 +103,18 @@ typedef struct {
 #define EXA_CONTRACT_DEFAULT 0  /* tuned per order                                        */
 #define EXA_CONTRACT_PENCIL 1   /* pencil owners contract, two barriers per time slice    */
 #define EXA_CONTRACT_NODE 2     /* node owners contract their pencils, one barrier per slice */
+#define EXA_CONTRACT_PENCIL_PAIR 3 /* pencil owners, two slices per phase, contraction in place */
 
 int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out);
 
 /* Autotuner: times the predictor of one order and precision in every launch
- * variant (1 or 2 waves per SIMD x pencil or node contraction) on an
- * ncells^3 vortex grid and records the fastest for this device; solvers
+ * variant (1 or 2 waves per SIMD x EXA_CONTRACT_PENCIL, _NODE, _PENCIL_PAIR)
+ * on an ncells^3 vortex grid and records the fastest for this device; solvers
  * created later with waves_per_simd = 0 and contraction = 0 use it.  Reports
- * the choice and the four average launch times (ms, order w1-pencil,
- * w1-node, w2-pencil, w2-node).  No counterpart in the reference. */
-int exa_autotune(int order, int precision, int ncells, int* waves_per_simd, int* contraction, double ms[4]);
+ * the choice and the six average launch times (ms, waves-major: w1-pencil,
+ * w1-node, w1-pair, w2-pencil, w2-node, w2-pair).  No counterpart in the
+ * reference. */
+int exa_autotune(int order, int precision, int ncells, int* waves_per_simd, int* contraction, double ms[6]);
 int exa_aderdg_destroy(exa_solver* s);
 
 /* ADERDGSolver::spaceTimePredictor (E/solvers/ADERDGSolver.h:1049-1059;

@@ -106,7 +106,7 @@ def test_golden_volume_surface_riemann_update(s3p1, golden):
 
 
 # ---------------------------------------------------------------- per-cell vs oracle
-@pytest.mark.parametrize("contraction", [1, 2])
+@pytest.mark.parametrize("contraction", [1, 2, 3])
 @pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7])
 def test_predictor_vs_oracle(N, contraction):
     s = exa().ADERDGSolver(N, contraction=contraction)
@@ -170,7 +170,7 @@ def _ic_kw(seed):
                 phi=rng.uniform(0, 6.28, 8))
 
 
-@pytest.mark.parametrize("contraction", [1, 2])
+@pytest.mark.parametrize("contraction", [1, 2, 3])
 @pytest.mark.parametrize("N,n,steps", [(3, 6, 5), (5, 4, 5), (2, 5, 3), (7, 2, 2)])
 def test_grid_steps_vs_oracle(N, n, steps, contraction):
     E = exa()
@@ -236,7 +236,7 @@ def test_autotune_records_a_variant():
     w, c, ms = E.autotune(3, E.PREC_FP64, 12)
     assert w in (1, 2) and c in (1, 2)
     assert np.all(np.isfinite(ms)) and np.all(ms > 0)
-    assert ms[(w - 1) * 2 + (c - 1)] == ms.min()
+    assert ms[(w - 1) * 3 + (c - 1)] == ms.min()
     s = E.ADERDGSolver(3)
     c3 = O.cfg(3)
     luh = smooth_cell_state(3, 0)
