@@ -166,9 +166,9 @@ static int predictor_form(const exa_aderdg_config& c, int N, int dev) {
   const int tc = g_tuned[dev & 63][N][c.precision & 3].contraction;
   if (tc) return tc == EXA_CONTRACT_NODE ? exa::FORM_NODE
                                          : (tc == EXA_CONTRACT_PENCIL_PAIR ? exa::FORM_PENCIL_PAIR : exa::FORM_PENCIL);
-  // measured at 81^3 cells (FP64 and mixed): pencil owners win at N=5
-  // (1.9x), node owners at N=3 (1.3x) and N=7 (1.1x)
-  return N == 5 ? exa::FORM_PENCIL : exa::FORM_NODE;
+  // measured at 81^3 cells, FP64 predictor ms (pencil / node / pencil pairs):
+  // N=3 6.73 / 7.15 / 6.91, N=5 42.1 / 67.4 / 38.1, N=7 355 / 346 / 341
+  return N >= 5 ? exa::FORM_PENCIL_PAIR : exa::FORM_PENCIL;
 }
 
 extern "C" int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out) {
