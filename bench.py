@@ -194,7 +194,7 @@ def main():
     achieved = flops / (pred_ms * 1e-3) / 1e12
 
     out = {
-        "metric": METRIC,
+        "metric": METRIC if N == 5 else METRIC.replace("N=5", "N=%d" % N),
         "value": rate,
         "unit": "DoF-updates/s",
         "n_gpus": 1,
@@ -341,7 +341,8 @@ def run_ensemble(args):
     waves.append(cur)
     prec, sweeps = E.PREC_FP32, 1
     dof_steps, wall_mx, wall_64, l2s = 0, 0.0, 0.0, [None] * len(members)
-    for w in waves:
+    for wi, w in enumerate(waves):
+        print("ensemble wave %d/%d: %d members" % (wi + 1, len(waves), len(w)), file=sys.stderr, flush=True)
         ms = [members[i] for i in w]
         ics = [E.make_ic(0, **m["ic"]) for m in ms]
         runs = {}
@@ -359,6 +360,7 @@ def run_ensemble(args):
             ens.sync()
             wall = time.perf_counter() - t0
             runs[p] = (ens, wall)
+            print("  precision %d: %d steps in %.2f s" % (p, steps, wall), file=sys.stderr, flush=True)
         wall_mx += runs[prec][1]
         wall_64 += runs[E.PREC_FP64][1]
         for j, i in enumerate(w):
@@ -368,6 +370,8 @@ def run_ensemble(args):
             a = runs[prec][0].download(j)
             b = runs[E.PREC_FP64][0].download(j)
             l2s[i] = float(l2_per_variable(a, b, m["order"], nc).max())
+            if j % 32 == 31:
+                print("  compared %d members" % (j + 1), file=sys.stderr, flush=True)
         for ens, _ in runs.values():
             ens.close()
     l2a = np.array(l2s)

@@ -100,5 +100,5 @@ def test_mixed_50_steps_l2_vs_fp64(N, n, prec, sweeps, l2bar, cellbar):
     (a, ta), (b, tb) = out
     l2 = l2_per_variable(b, a, N, n ** 3, 1.0 / n ** 3)
     assert l2.max() <= l2bar, l2
-    # dt follows the state, so the simulated times differ slightly
-    assert abs(ta - tb) <= 1e-6 * ta
+    # dt follows the state, so the simulated times differ at the solution's level
+    assert abs(ta - tb) <= l2bar * ta
