@@ -84,6 +84,7 @@ EXPORTS = [
     "exa_ensemble_create", "exa_ensemble_destroy", "exa_ensemble_set_initial_conditions", "exa_ensemble_compute_dt",
     "exa_ensemble_use_graph", "exa_ensemble_step", "exa_ensemble_sync", "exa_ensemble_download",
     "exa_ensemble_total_picard_iterations", "exa_autotune",
+    "exa_grid_limiter_snapshot", "exa_grid_limiter_check", "exa_grid_limiter_project",
 ]
 
 
@@ -147,6 +148,9 @@ def load_library(path=LIB_PATH):
     L.exa_device_copy.argtypes = [C.c_void_p, C.c_void_p, LL, P(C.c_float)]
     L.exa_selftest_mfma.argtypes = [I, C.c_void_p, C.c_void_p, C.c_void_p]
     L.exa_autotune.argtypes = [I, I, I, P(I), P(I), _D]
+    L.exa_grid_limiter_snapshot.argtypes = [C.c_void_p, I]
+    L.exa_grid_limiter_check.argtypes = [C.c_void_p, C.c_double, C.c_double, P(LL), P(I)]
+    L.exa_grid_limiter_project.argtypes = [C.c_void_p, I, C.c_void_p, LL, P(I)]
     L.exa_ensemble_create.argtypes = [P(EnsembleMember), I, P(AderdgConfig), P(C.c_void_p)]
     L.exa_ensemble_destroy.argtypes = [C.c_void_p]
     L.exa_ensemble_set_initial_conditions.argtypes = [C.c_void_p, P(InitialCondition)]
@@ -330,6 +334,28 @@ class Grid:
 
     def step(self, nsteps=1):
         _check(_lib.exa_grid_step(self._h, int(nsteps)))
+
+    # limiter (troubled-cell detection and batched FV projection)
+    def limiter_snapshot(self, nobs=5):
+        _check(_lib.exa_grid_limiter_snapshot(self._h, nobs))
+
+    def limiter_check(self, relax=1e-4, scaling=1e-3):
+        n = C.c_longlong(0)
+        flags = np.zeros(self.ncells, np.int32)
+        _check(_lib.exa_grid_limiter_check(self._h, float(relax), float(scaling), C.byref(n),
+                                           flags.ctypes.data_as(C.POINTER(C.c_int))))
+        return n.value, flags
+
+    def limiter_project(self, ghost, ntroubled):
+        """Troubled cells' subcell patches: (cells, patches[n][V*S^3])."""
+        S = 2 * self.solver.order + 1 + 2 * ghost
+        per = 5 * S ** 3
+        d = DeviceArray(max(1, ntroubled) * per)
+        cells = np.zeros(max(1, ntroubled), np.int32)
+        _check(_lib.exa_grid_limiter_project(self._h, ghost, d.ptr, d.n, cells.ctypes.data_as(C.POINTER(C.c_int))))
+        out = d.numpy()[:ntroubled * per].reshape(ntroubled, per)
+        d.close()
+        return cells[:ntroubled], out
 
     def sync(self):
         t, dt = C.c_double(0.0), C.c_double(0.0)

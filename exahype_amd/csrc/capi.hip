@@ -12,6 +12,7 @@
 #include "aderdg_kernels.h"
 #include "grid_kernels.h"
 #include "predictor_tu.h"
+#include "grid_view.h"
 #include "host_util.h"
 #include "dg_tables_data.h"
 
@@ -556,10 +557,24 @@ static void grid_clear_events(exa_grid* g) {
   for (auto& v : g->ev) v.clear();
 }
 
+int exa_grid_view(exa_grid* g, ExaGridView* v) {
+  EXA_ARG(g && v, "null pointer");
+  v->luh = g->luh.as<double>();
+  v->nc = g->nc;
+  v->nx = g->nx;
+  v->ny = g->ny;
+  v->nz = g->nz;
+  v->order = g->s->cfg.order;
+  v->device = g->s->device;
+  v->stream = g->stream;
+  return EXA_OK;
+}
+
 extern "C" int exa_grid_destroy(exa_grid* g) {
   if (!g) return EXA_OK;
   (void)hipSetDevice(g->s->device);
   if (g->stream) (void)hipStreamSynchronize(g->stream);
+  exa_grid_limiter_release(g);
   grid_clear_events(g);
   if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
   if (g->graph) (void)hipGraphDestroy(g->graph);

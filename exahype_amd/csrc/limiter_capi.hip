@@ -183,3 +183,179 @@ extern "C" int exa_limiter_dmp(int order, const double* luh, int nobs, double re
   (void)L;
   return EXA_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Troubled-cell detection on a device grid and the batched FV projection of
+// the troubled cells (the per-step limiter work of LimitingADERDGSolver,
+// E/solvers/LimitingADERDGSolver.cpp:838-1048, on a single-level periodic
+// grid): one launch per stage over all cells, or over the compacted list of
+// troubled cells.
+#include <map>
+
+#include "grid_view.h"
+
+namespace {
+
+struct GridLimiter {
+  int nobs = 0;
+  bool have_snapshot = false;
+  DevBuf mm_prev, bmm, mm_new, ok, list, count;
+};
+std::mutex g_gl_mu;
+std::map<exa_grid*, GridLimiter*> g_gl;
+
+GridLimiter* grid_limiter(exa_grid* g) {
+  std::lock_guard<std::mutex> lk(g_gl_mu);
+  GridLimiter*& p = g_gl[g];
+  if (!p) p = new GridLimiter();
+  return p;
+}
+
+// bmm[cell] = min/max over the cell and its six face neighbours (periodic)
+// of the snapshot min/max -- the boundary extrema the reference merges over
+// the faces (mergeSolutionMinMaxOnFace) before its DMP.
+__global__ void k_merge_neighbour_minmax(const double* __restrict__ mm, double* __restrict__ bmm, int nx, int ny,
+                                         int nz, int nobs) {
+  const long long nc = (long long)nx * ny * nz;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const int ix = (int)(c % nx), iy = (int)((c / nx) % ny), iz = (int)(c / ((long long)nx * ny));
+  long long nb[7];
+  nb[0] = c;
+  nb[1] = ((long long)iz * ny + iy) * nx + (ix + 1) % nx;
+  nb[2] = ((long long)iz * ny + iy) * nx + (ix + nx - 1) % nx;
+  nb[3] = ((long long)iz * ny + (iy + 1) % ny) * nx + ix;
+  nb[4] = ((long long)iz * ny + (iy + ny - 1) % ny) * nx + ix;
+  nb[5] = ((long long)((iz + 1) % nz) * ny + iy) * nx + ix;
+  nb[6] = ((long long)((iz + nz - 1) % nz) * ny + iy) * nx + ix;
+  for (int v = 0; v < nobs; ++v) {
+    double lo = mm[(c * 2 + 0) * nobs + v], hi = mm[(c * 2 + 1) * nobs + v];
+    for (int k = 1; k < 7; ++k) {
+      const double a = mm[(nb[k] * 2 + 0) * nobs + v], b = mm[(nb[k] * 2 + 1) * nobs + v];
+      lo = a < lo ? a : lo;
+      hi = hi < b ? b : hi;
+    }
+    bmm[(c * 2 + 0) * nobs + v] = lo;
+    bmm[(c * 2 + 1) * nobs + v] = hi;
+  }
+}
+
+// Physical admissibility of the Euler state (density and energy minima
+// positive, as the Euler demonstrators' isPhysicallyAdmissible) folded into
+// the DMP flags; troubled cells appended to the list.
+__global__ void k_flag_and_compact(const double* __restrict__ mm_new, int* __restrict__ ok, int* __restrict__ list,
+                                   int* __restrict__ count, long long nc, int nobs) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  int good = ok[c];
+  if (nobs >= 1 && !(mm_new[(c * 2 + 0) * nobs + 0] > 0.0)) good = 0;
+  if (nobs >= 5 && !(mm_new[(c * 2 + 0) * nobs + 4] > 0.0)) good = 0;
+  ok[c] = good;
+  if (!good) list[atomicAdd(count, 1)] = (int)c;
+}
+
+int minmax_all(const ExaGridView& v, const LimTables* tab, double* out, int nobs) {
+  const int B = v.order + 1;
+  const size_t lds = proj_lds_bytes(B) + sizeof(double) * (3 * exa::lim::V * B * B * B + 2 * nobs * 4);
+  EXA_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::lim::k_cell_min_max),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(exa::lim::k_cell_min_max, dim3((unsigned)v.nc), dim3(256), lds, v.stream, v.luh, out, tab, B,
+                     nobs, (int)v.nc);
+  EXA_CHECK(hipGetLastError());
+  return EXA_OK;
+}
+
+}  // namespace
+
+void exa_grid_limiter_release(exa_grid* g) {
+  std::lock_guard<std::mutex> lk(g_gl_mu);
+  auto it = g_gl.find(g);
+  if (it != g_gl.end()) {
+    delete it->second;
+    g_gl.erase(it);
+  }
+}
+
+extern "C" int exa_grid_limiter_snapshot(exa_grid* g, int nobs) {
+  EXA_ARG(g && nobs >= 1 && nobs <= 5, "grid and 1..5 observables");
+  ExaGridView v;
+  EXA_TRY(exa_grid_view(g, &v));
+  EXA_TRY(check_order(v.order));
+  EXA_CHECK(hipSetDevice(v.device));
+  const LimTables* tab = nullptr;
+  EXA_TRY(lim_tables(v.device, v.order, &tab));
+  GridLimiter* L = grid_limiter(g);
+  EXA_TRY(L->mm_prev.ensure(sizeof(double) * 2 * nobs * v.nc));
+  EXA_TRY(L->bmm.ensure(sizeof(double) * 2 * nobs * v.nc));
+  EXA_TRY(minmax_all(v, tab, L->mm_prev.as<double>(), nobs));
+  hipLaunchKernelGGL(k_merge_neighbour_minmax, dim3((unsigned)((v.nc + 255) / 256)), dim3(256), 0, v.stream,
+                     L->mm_prev.as<double>(), L->bmm.as<double>(), v.nx, v.ny, v.nz, nobs);
+  EXA_CHECK(hipGetLastError());
+  EXA_CHECK(hipStreamSynchronize(v.stream));
+  L->nobs = nobs;
+  L->have_snapshot = true;
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_limiter_check(exa_grid* g, double relax, double scaling, long long* ntroubled,
+                                      int* flags) {
+  EXA_ARG(g && ntroubled, "null pointer");
+  GridLimiter* L = grid_limiter(g);
+  EXA_ARG(L->have_snapshot, "exa_grid_limiter_snapshot must precede exa_grid_limiter_check");
+  ExaGridView v;
+  EXA_TRY(exa_grid_view(g, &v));
+  EXA_CHECK(hipSetDevice(v.device));
+  const LimTables* tab = nullptr;
+  EXA_TRY(lim_tables(v.device, v.order, &tab));
+  const int nobs = L->nobs;
+  EXA_TRY(L->mm_new.ensure(sizeof(double) * 2 * nobs * v.nc));
+  EXA_TRY(L->ok.ensure(sizeof(int) * v.nc));
+  EXA_TRY(L->list.ensure(sizeof(int) * v.nc));
+  EXA_TRY(L->count.ensure(sizeof(int)));
+  EXA_TRY(minmax_all(v, tab, L->mm_new.as<double>(), nobs));
+  const unsigned blocks = (unsigned)((v.nc + 255) / 256);
+  hipLaunchKernelGGL(exa::lim::k_dmp, dim3(blocks), dim3(256), 0, v.stream, L->mm_new.as<double>(),
+                     L->bmm.as<double>(), L->ok.as<int>(), nobs, (int)v.nc, relax, scaling);
+  EXA_CHECK(hipMemsetAsync(L->count.p, 0, sizeof(int), v.stream));
+  hipLaunchKernelGGL(k_flag_and_compact, dim3(blocks), dim3(256), 0, v.stream, L->mm_new.as<double>(),
+                     L->ok.as<int>(), L->list.as<int>(), L->count.as<int>(), v.nc, nobs);
+  EXA_CHECK(hipGetLastError());
+  int n = 0;
+  EXA_CHECK(hipMemcpyAsync(&n, L->count.p, sizeof(int), hipMemcpyDeviceToHost, v.stream));
+  if (flags) {
+    std::vector<int> ok(v.nc);
+    EXA_CHECK(hipMemcpyAsync(ok.data(), L->ok.p, sizeof(int) * v.nc, hipMemcpyDeviceToHost, v.stream));
+    EXA_CHECK(hipStreamSynchronize(v.stream));
+    for (long long c = 0; c < v.nc; ++c) flags[c] = ok[c] ? 0 : 1;
+  }
+  EXA_CHECK(hipStreamSynchronize(v.stream));
+  *ntroubled = n;
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_limiter_project(exa_grid* g, int ghost, double* d_lim, long long capacity,
+                                        int* cells) {
+  EXA_ARG(g && d_lim && ghost >= 0 && ghost <= 4, "arguments");
+  GridLimiter* L = grid_limiter(g);
+  EXA_ARG(L->list.p != nullptr, "exa_grid_limiter_check must precede exa_grid_limiter_project");
+  ExaGridView v;
+  EXA_TRY(exa_grid_view(g, &v));
+  EXA_CHECK(hipSetDevice(v.device));
+  const LimTables* tab = nullptr;
+  EXA_TRY(lim_tables(v.device, v.order, &tab));
+  int n = 0;
+  EXA_CHECK(hipMemcpyAsync(&n, L->count.p, sizeof(int), hipMemcpyDeviceToHost, v.stream));
+  EXA_CHECK(hipStreamSynchronize(v.stream));
+  const int B = v.order + 1, S = 2 * v.order + 1 + 2 * ghost;
+  EXA_ARG((long long)n * exa::lim::V * S * S * S <= capacity, "d_lim holds fewer patches than troubled cells");
+  if (n > 0) {
+    EXA_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::lim::k_project_on_fv),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)proj_lds_bytes(B)));
+    hipLaunchKernelGGL(exa::lim::k_project_on_fv, dim3(n), dim3(256), proj_lds_bytes(B), v.stream, v.luh, d_lim, tab,
+                       B, ghost, L->list.as<int>());
+    EXA_CHECK(hipGetLastError());
+  }
+  if (cells && n > 0) EXA_CHECK(hipMemcpyAsync(cells, L->list.p, sizeof(int) * n, hipMemcpyDeviceToHost, v.stream));
+  EXA_CHECK(hipStreamSynchronize(v.stream));
+  return EXA_OK;
+}

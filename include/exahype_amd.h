@@ -325,6 +325,31 @@ int exa_limiter_project_on_dg(int order, const double* lim, int ghost, double* l
 int exa_limiter_dmp(int order, const double* luh, int nobs, double relax, double scaling, double* bmin,
                     double* bmax, int* admissible);
 
+/* ---- Limiter on a device grid (orders 1..5, Euler observables) ----------
+ * The per-step troubled-cell work of LimitingADERDGSolver on a single-level
+ * periodic grid (E/solvers/LimitingADERDGSolver.cpp:838-1048), one launch
+ * per stage over all cells:
+ *   snapshot: every cell's min/max of the first nobs variables (GL nodes,
+ *     Gauss-Lobatto nodes, subcell averages: Limiter.cpp:210-368) of the
+ *     current state, merged over the cell and its six face neighbours (the
+ *     boundary extrema the reference merges over faces);
+ *   check (after a step): the new min/max against the snapshot's merged
+ *     extrema with the relaxed DMP (Limiter.cpp:114-165), plus physical
+ *     admissibility (density and, for nobs = 5, energy minima > 0);
+ *     *ntroubled receives the count, flags (host, one int per cell, may be
+ *     NULL) 1 for troubled cells;
+ *   project: the troubled cells' DG solutions projected onto their
+ *     (2N+1)^3 subcells (projectOnFVLimiterSpace, Limiter.cpp:23-66), one
+ *     workgroup per troubled cell, into the device array d_lim of capacity
+ *     doubles: patch i = [v][z][y][x] of side 2N+1+2*ghost (interior
+ *     written) for the cell cells[i] (host, may be NULL; the order of the
+ *     list is not deterministic).
+ * The limiter status machine, rollback and FV recomputation are not part of
+ * this layer. */
+int exa_grid_limiter_snapshot(exa_grid* g, int nobs);
+int exa_grid_limiter_check(exa_grid* g, double relax, double scaling, long long* ntroubled, int* flags);
+int exa_grid_limiter_project(exa_grid* g, int ghost, double* d_lim, long long capacity, int* cells);
+
 /* ---- AMR level transfers (orders 0..9, 1..10 levels) ---------------------
  * Between a coarse cell/face and a descendant fineGridLevel - coarseGridLevel
  * levels finer; the sub-index of the descendant counts 0..3^levels-1 per axis
