@@ -346,16 +346,20 @@ class Grid:
                                            flags.ctypes.data_as(C.POINTER(C.c_int))))
         return n.value, flags
 
-    def limiter_project(self, ghost, ntroubled):
-        """Troubled cells' subcell patches: (cells, patches[n][V*S^3])."""
+    def limiter_project(self, ghost, ntroubled, out=None):
+        """Troubled cells' subcell patches.  With out (a DeviceArray of at
+        least ntroubled * V * S^3 doubles) the patches stay on the device and
+        the cell list is returned; without, returns (cells, patches[n][V*S^3])."""
         S = 2 * self.solver.order + 1 + 2 * ghost
         per = 5 * S ** 3
-        d = DeviceArray(max(1, ntroubled) * per)
+        d = out if out is not None else DeviceArray(max(1, ntroubled) * per)
         cells = np.zeros(max(1, ntroubled), np.int32)
         _check(_lib.exa_grid_limiter_project(self._h, ghost, d.ptr, d.n, cells.ctypes.data_as(C.POINTER(C.c_int))))
-        out = d.numpy()[:ntroubled * per].reshape(ntroubled, per)
+        if out is not None:
+            return cells[:ntroubled]
+        res = d.numpy()[:ntroubled * per].reshape(ntroubled, per)
         d.close()
-        return cells[:ntroubled], out
+        return cells[:ntroubled], res
 
     def sync(self):
         t, dt = C.c_double(0.0), C.c_double(0.0)

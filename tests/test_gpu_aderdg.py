@@ -234,7 +234,7 @@ def test_autotune_records_a_variant():
     solver left at the tuned defaults still reproduces the oracle."""
     E = exa()
     w, c, ms = E.autotune(3, E.PREC_FP64, 12)
-    assert w in (1, 2) and c in (1, 2)
+    assert w in (1, 2) and c in (1, 2, 3)
     assert np.all(np.isfinite(ms)) and np.all(ms > 0)
     assert ms[(w - 1) * 3 + (c - 1)] == ms.min()
     s = E.ADERDGSolver(3)
