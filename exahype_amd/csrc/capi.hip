@@ -487,6 +487,7 @@ struct exa_grid {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   bool timing = false;
+  long long version = 0;  // bumped whenever luh changes (steps, uploads, initial conditions)
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
   double times[3] = {0, 0, 0};
   long long counts[3] = {0, 0, 0};
@@ -567,6 +568,7 @@ int exa_grid_view(exa_grid* g, ExaGridView* v) {
   v->order = g->s->cfg.order;
   v->device = g->s->device;
   v->stream = g->stream;
+  v->version = g->version;
   return EXA_OK;
 }
 
@@ -613,6 +615,7 @@ static int grid_transfer(exa_grid* g, double* host, bool upload) {
 extern "C" int exa_grid_upload(exa_grid* g, const double* luh) {
   EXA_ARG(g && luh, "null pointer");
   EXA_CHECK(hipSetDevice(g->s->device));
+  ++g->version;
   return grid_transfer(g, const_cast<double*>(luh), true);
 }
 
@@ -626,6 +629,7 @@ extern "C" int exa_grid_download(exa_grid* g, double* luh) {
 extern "C" int exa_grid_set_initial_conditions(exa_grid* g, const exa_ic* ic) {
   EXA_ARG(g && ic, "null pointer");
   EXA_CHECK(hipSetDevice(g->s->device));
+  ++g->version;
   const int N = g->s->cfg.order;
   EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
@@ -747,6 +751,7 @@ extern "C" int exa_grid_enable_timing(exa_grid* g, int enable) {
 extern "C" int exa_grid_step(exa_grid* g, int nsteps) {
   EXA_ARG(g && nsteps >= 0, "nsteps");
   EXA_CHECK(hipSetDevice(g->s->device));
+  g->version += nsteps;
   if (g->use_graph && !g->timing) {
     if (!g->gexec) {
       EXA_CHECK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));

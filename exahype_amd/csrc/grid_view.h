@@ -12,6 +12,7 @@ struct ExaGridView {
   int order;
   int device;
   hipStream_t stream;
+  long long version;  // changes whenever luh changes
 };
 
 int exa_grid_view(exa_grid* g, ExaGridView* v);

@@ -199,6 +199,7 @@ namespace {
 struct GridLimiter {
   int nobs = 0;
   bool have_snapshot = false;
+  long long mm_new_version = -1;  // grid state the last check's extrema belong to
   DevBuf mm_prev, bmm, mm_new, ok, list, count;
 };
 std::mutex g_gl_mu;
@@ -255,12 +256,14 @@ __global__ void k_flag_and_compact(const double* __restrict__ mm_new, int* __res
 }
 
 int minmax_all(const ExaGridView& v, const LimTables* tab, double* out, int nobs) {
-  const int B = v.order + 1;
-  const size_t lds = proj_lds_bytes(B) + sizeof(double) * (3 * exa::lim::V * B * B * B + 2 * nobs * 4);
-  EXA_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::lim::k_cell_min_max),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(exa::lim::k_cell_min_max, dim3((unsigned)v.nc), dim3(256), lds, v.stream, v.luh, out, tab, B,
-                     nobs, (int)v.nc);
+  const dim3 grid((unsigned)v.nc), block(256);
+  switch (v.order) {
+    case 1: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<2>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
+    case 2: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<3>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
+    case 3: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<4>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
+    case 4: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<5>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
+    default: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<6>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
+  }
   EXA_CHECK(hipGetLastError());
   return EXA_OK;
 }
@@ -287,7 +290,13 @@ extern "C" int exa_grid_limiter_snapshot(exa_grid* g, int nobs) {
   GridLimiter* L = grid_limiter(g);
   EXA_TRY(L->mm_prev.ensure(sizeof(double) * 2 * nobs * v.nc));
   EXA_TRY(L->bmm.ensure(sizeof(double) * 2 * nobs * v.nc));
-  EXA_TRY(minmax_all(v, tab, L->mm_prev.as<double>(), nobs));
+  if (L->mm_new_version == v.version && L->nobs == nobs) {
+    // the last check computed the extrema of this very state
+    L->mm_prev.swap(L->mm_new);
+    L->mm_new_version = -1;
+  } else {
+    EXA_TRY(minmax_all(v, tab, L->mm_prev.as<double>(), nobs));
+  }
   hipLaunchKernelGGL(k_merge_neighbour_minmax, dim3((unsigned)((v.nc + 255) / 256)), dim3(256), 0, v.stream,
                      L->mm_prev.as<double>(), L->bmm.as<double>(), v.nx, v.ny, v.nz, nobs);
   EXA_CHECK(hipGetLastError());
@@ -330,6 +339,7 @@ extern "C" int exa_grid_limiter_check(exa_grid* g, double relax, double scaling,
   }
   EXA_CHECK(hipStreamSynchronize(v.stream));
   *ntroubled = n;
+  L->mm_new_version = v.version;
   return EXA_OK;
 }
 
