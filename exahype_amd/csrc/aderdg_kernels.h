@@ -160,6 +160,19 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
   // offsets of this node in the three direction layouts (flux and partials)
   const int ox = (z * B + y) * PS + x, oy = (z * B + x) * PS + y, oz = (y * B + x) * PS + z;
 
+  // per-thread item tables of the paired form (up to N=5; at N=7 the ten
+  // extra registers would push the sweeps into scratch)
+  constexpr bool PTAB = PAIR && B <= 6;
+  constexpr int PR = PTAB ? (2 * NITEMS + O::NT - 1) / O::NT : 1;
+  int po[PR], pc[PR];
+#pragma unroll
+  for (int r = 0; r < (PTAB ? PR : 0); ++r) {
+    const int i = s + r * O::NT;
+    const int kdv = i / B2, ab = i - kdv * B2;   // kdv = slice-in-pair * 3V + d*V + v
+    const int k = kdv / (3 * V), d = (kdv - k * 3 * V) / V;
+    po[r] = kdv * BLK + ab * PS;
+    pc[r] = (d * B + k) * B2 + ab;
+  }
   int it = 0;
   while (it < max_sweeps) {
     ++it;
@@ -218,8 +231,11 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
       }
     } else {
       // two slices per phase, contraction in place (an item reads its whole
-      // pencil before writing it): slice t0 in sF, slice t0+1 in sP; the
-      // last slice of an odd B goes alone.  Three barriers per pair.
+      // pencil before writing it): slice t0 in sF, slice t0+1 in sP (which
+      // follows sF directly); the last slice of an odd B goes alone.  Three
+      // barriers per pair.  Item i of a pair is pencil i/B2 (slice-major,
+      // then direction and variable) at i%B2; its LDS offset and pencil
+      // constant index are the same for every pair and sweep (po, pc).
 #pragma unroll
       for (int t0 = 0; t0 < B; t0 += 2) {
         asm volatile("" : "+s"(tz));
@@ -241,21 +257,41 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
           }
         }
         __syncthreads();
-        for (int i = s; i < ns * NITEMS; i += O::NT) {
-          const int k = i >= NITEMS ? 1 : 0;
-          const int ii = i - k * NITEMS;
-          const int dv = ii / B2, ab = ii - dv * B2, d = dv / V;
-          const T c = sc[(d * B + t0 + k) * B2 + ab];
-          T* pf = (k == 0 ? sF : sP) + dv * BLK + ab * PS;
-          T f[B];
+        if constexpr (PTAB) {
 #pragma unroll
-          for (int n = 0; n < B; ++n) f[n] = c * pf[n];
+          for (int r = 0; r < PR; ++r) {
+            if (s + r * O::NT < ns * NITEMS) {
+              const T c = sc[pc[r] + t0 * B2];
+              T* pf = sF + po[r];
+              T f[B];
 #pragma unroll
-          for (int l = 0; l < B; ++l) {
-            T acc = T(0);
+              for (int n = 0; n < B; ++n) f[n] = c * pf[n];
 #pragma unroll
-            for (int n = 0; n < B; ++n) acc += f[n] * KXI(n, l);
-            pf[l] = acc;
+              for (int l = 0; l < B; ++l) {
+                T acc = T(0);
+#pragma unroll
+                for (int n = 0; n < B; ++n) acc += f[n] * KXI(n, l);
+                pf[l] = acc;
+              }
+            }
+          }
+        } else {
+          for (int i = s; i < ns * NITEMS; i += O::NT) {
+            const int k = i >= NITEMS ? 1 : 0;
+            const int ii = i - k * NITEMS;
+            const int dv = ii / B2, ab = ii - dv * B2, d = dv / V;
+            const T c = sc[(d * B + t0 + k) * B2 + ab];
+            T* pf = (k == 0 ? sF : sP) + dv * BLK + ab * PS;
+            T f[B];
+#pragma unroll
+            for (int n = 0; n < B; ++n) f[n] = c * pf[n];
+#pragma unroll
+            for (int l = 0; l < B; ++l) {
+              T acc = T(0);
+#pragma unroll
+              for (int n = 0; n < B; ++n) acc += f[n] * KXI(n, l);
+              pf[l] = acc;
+            }
           }
         }
         __syncthreads();
