@@ -160,13 +160,16 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
   // offsets of this node in the three direction layouts (flux and partials)
   const int ox = (z * B + y) * PS + x, oy = (z * B + x) * PS + y, oz = (y * B + x) * PS + z;
 
-  // per-thread item tables of the paired form (up to N=5; at N=7 the ten
-  // extra registers would push the sweeps into scratch)
-  constexpr bool PTAB = PAIR && B <= 6;
-  constexpr int PR = PTAB ? (2 * NITEMS + O::NT - 1) / O::NT : 1;
+  // per-thread item tables: LDS offset and pencil-constant index of every
+  // item a thread owns, the same for every slice and sweep (paired form up
+  // to N=5 only; at N=7 the extra registers would push the sweeps into
+  // scratch)
+  constexpr bool TAB = B <= 6;
+  constexpr bool PTAB = PAIR && TAB;
+  constexpr int PR = TAB ? ((PAIR ? 2 : 1) * NITEMS + O::NT - 1) / O::NT : 1;
   int po[PR], pc[PR];
 #pragma unroll
-  for (int r = 0; r < (PTAB ? PR : 0); ++r) {
+  for (int r = 0; r < (TAB ? PR : 0); ++r) {
     const int i = s + r * O::NT;
     const int kdv = i / B2, ab = i - kdv * B2;   // kdv = slice-in-pair * 3V + d*V + v
     const int k = kdv / (3 * V), d = (kdv - k * 3 * V) / V;
@@ -200,20 +203,28 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
         // phase P: item = (d*V + v, pencil ab); pencils and partials are
         // contiguous runs of B in the direction layouts, so the item needs no
         // direction-dependent addressing
-        for (int i = s; i < NITEMS; i += O::NT) {
-          const int dv = i / B2, ab = i - dv * B2, d = dv / V;
-          const T c = sc[(d * B + t) * B2 + ab];
-          const T* pf = sF + dv * BLK + ab * PS;
+        auto item = [&](const T c, int off) {
+          const T* pf = sF + off;
           T f[B];
 #pragma unroll
           for (int n = 0; n < B; ++n) f[n] = c * pf[n];
-          T* pp = sP + dv * BLK + ab * PS;
+          T* pp = sP + off;
 #pragma unroll
           for (int l = 0; l < B; ++l) {
             T acc = T(0);
 #pragma unroll
             for (int n = 0; n < B; ++n) acc += f[n] * KXI(n, l);
             pp[l] = acc;
+          }
+        };
+        if constexpr (TAB) {
+#pragma unroll
+          for (int r = 0; r < PR; ++r)
+            if (s + r * O::NT < NITEMS) item(sc[pc[r] + t * B2], po[r]);
+        } else {
+          for (int i = s; i < NITEMS; i += O::NT) {
+            const int dv = i / B2, ab = i - dv * B2, d = dv / V;
+            item(sc[(d * B + t) * B2 + ab], dv * BLK + ab * PS);
           }
         }
         __syncthreads();
