@@ -65,7 +65,7 @@ def vortex_ic_kwargs(seed=0):
                 phi=rng.uniform(0, 2 * np.pi, 8))
 
 
-def run_gpu(order, n, steps, warmup, precision, waves_per_simd=0, contraction=0, fp64_sweeps=1):
+def run_gpu(order, n, steps, warmup, precision, waves_per_simd=0, contraction=0, fp64_sweeps=1, keep_state=True):
     import exahype_amd as E
     s = E.ADERDGSolver(order, precision=precision, fp64_sweeps=fp64_sweeps, waves_per_simd=waves_per_simd,
                        contraction=contraction)
@@ -85,7 +85,7 @@ def run_gpu(order, n, steps, warmup, precision, waves_per_simd=0, contraction=0,
     wall = time.perf_counter() - t0
     times, counts = g.kernel_times()
     sweeps = g.total_picard_iterations() / (g.ncells * steps)
-    state = g.download()
+    state = g.download() if keep_state else None
     g.close()
     s.close()
     return dict(wall=wall, times=times, counts=counts, iters=sweeps, state=state)
@@ -158,11 +158,13 @@ def main():
     ncells = n ** 3
     dof = ncells * NVAR * (N + 1) ** 3
 
-    r64 = run_gpu(N, n, args.steps, args.warmup, E.PREC_FP64, args.waves_per_simd, args.contraction)
+    # the FP64 state is the reference of the mixed runs (orders <= 7 only)
+    r64 = run_gpu(N, n, args.steps, args.warmup, E.PREC_FP64, args.waves_per_simd, args.contraction,
+                  keep_state=N <= 7)
     rate64 = dof * args.steps / r64["wall"]
     mixed = {}
     for name, prec, sweeps in MIXED_VARIANTS:
-        if prec >= E.PREC_FP32_MFMA and N > 7:
+        if N > 7:  # orders 8 and 9 run the FP64 predictor only
             continue
         r = run_gpu(N, n, args.steps, args.warmup, prec, args.waves_per_simd, args.contraction, sweeps)
         l2 = l2_per_variable(r["state"], r64["state"], N, ncells)
@@ -233,12 +235,16 @@ def main():
     }
     if not args.no_cpu_baseline:
         threads = min(len(os.sched_getaffinity(0)), 16)
-        v, wall = cpu_baseline(N, n, threads, sample_cells=27, steps=30)
-        v1, wall1 = cpu_baseline(N, n, 1, sample_cells=9, steps=20)
+        # bounded CPU sample: the work per cell-step grows ~B^5, so the step
+        # counts shrink with the order (30 and 20 steps at N=5)
+        scale = (6.0 / (N + 1)) ** 5
+        st, st1 = max(1, min(30, int(round(30 * scale)))), max(1, min(20, int(round(20 * scale))))
+        v, wall = cpu_baseline(N, n, threads, sample_cells=27, steps=st)
+        v1, wall1 = cpu_baseline(N, n, 1, sample_cells=9, steps=st1)
         out["cpu_baseline"] = {"value": v, "unit": "DoF-updates/s", "cores": threads, "kind": "port",
                                "sample": "oracle (C, OpenMP), order %d, 27^3 periodic cells of size 1/%d, same initial "
-                                         "data, 30 steps (%.1f s wall)" % (N, n, wall),
-                               "single_thread": {"value": v1, "sample": "9^3 cells, 20 steps (%.1f s)" % wall1}}
+                                         "data, %d steps (%.1f s wall)" % (N, n, st, wall),
+                               "single_thread": {"value": v1, "sample": "9^3 cells, %d steps (%.1f s)" % (st1, wall1)}}
     if args.copy_gib > 0:
         out["hbm_copy"] = copy_reference(args.copy_gib)
     print(json.dumps(out))

@@ -28,6 +28,10 @@ struct Ord {
   static constexpr int PS = (B % 2 == 0) ? B + 1 : B;
   static constexpr int BLK = B2 * PS;                // one (direction, variable) block of pencils
   static constexpr int LDS_DOUBLES = 6 * V * BLK + 3 * B3 + 16;  // flux slices + pencil constants + reduction scratch
+  // staged predictor (N >= 8): one flux slice + pencil constants during the
+  // sweeps, time averages sQh/sFh afterwards, reduction scratch at the end
+  static constexpr int LDS_STAGED =
+      (3 * V * BLK + 3 * B3 > 4 * V * B3 ? 3 * V * BLK + 3 * B3 : 4 * V * B3) + 16;
 };
 
 // ---------------------------------------------------------------------------
@@ -841,6 +845,231 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
   if (s == 0) {
     if (iters) iters[cell] = it;
     if (sweep_count) atomicAdd(sweep_count, (unsigned long long)it);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Space-time predictor for the orders whose space-time state does not fit the
+// registers of a thread per node (N >= 8: the iterate and its successor alone
+// would take 2 B V doubles = 200 VGPRs per thread at N = 9, while B^3 = 1000
+// threads per workgroup leave 128).  FP64 only; the same products and sums in
+// the same order as k_predictor (spaceTimePredictorNonlinear.cpph:43-577,
+// volumeIntegralNonlinear.cpph:26-125).
+//
+// Persistent grid: workgroup g owns staging slot g, two space-time arrays
+// [t][v][node] of B^4 V doubles in global memory (800 KB at N = 9, resident
+// in the last-level cache for a grid of one workgroup per CU), and walks the
+// cells g, g + gridDim.x, ...  Every slot word is only ever touched by the
+// thread of its node, so the slot needs no barriers.  One Picard sweep:
+//   for every slice t:  F  node owners load q[t] from the slot, evaluate the
+//                          flux and store it in the direction layouts;
+//                       P  pencil items contract in place with Kxi, scaled
+//                          by the pencil constant (LDS table);
+//                       A  node owners store rhs / w^3 into the second array;
+//   then every node owner applies iK1 to its own rhs history, takes the
+//   residual against q and overwrites the history with the new iterate; the
+//   two arrays swap roles between sweeps.
+template <int N, bool DEBUG>
+__global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
+    const double* __restrict__ luh, double* __restrict__ lduh, double* __restrict__ qf, double* __restrict__ ff,
+    int* __restrict__ iters, const double* __restrict__ dtp, double dx0, double dx1, double dx2, double gamma,
+    long long ncells, double* __restrict__ stage, unsigned long long* __restrict__ sweep_count,
+    PredictorDebug dbg) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2, B3 = O::B3, PS = O::PS, BLK = O::BLK;
+  constexpr int NITEMS = 3 * V * B2;
+  constexpr int SQ = B * V * B3;  // doubles of one staged space-time array
+  constexpr int ROUNDS = (NITEMS + O::NT - 1) / O::NT;
+  constexpr int MaxIterations = 2 * (N + 1);
+  extern __shared__ double sm[];
+  double* sF = sm;                     // flux slice, direction layouts, contracted in place
+  double* sc = sm + 3 * V * BLK;       // pencil constants [d][t][a*B+b]
+  double* red = sm + O::LDS_STAGED - 16;
+  double* sQh = sm;                    // after the sweeps: time averages
+  double* sFh = sm + V * B3;
+  const DGTables& T = c_tab[N];
+  const int s = threadIdx.x;
+  const bool act = s < B3;
+  const int z = act ? s / B2 : 0, y = act ? (s / B) % B : 0, x = act ? s % B : 0;
+  const int ox = (z * B + y) * PS + x, oy = (z * B + x) * PS + y, oz = (y * B + x) * PS + z;
+  const double w3 = T.w[z] * T.w[y] * T.w[x];
+  const double iw3 = 1.0 / w3;
+  const double dt = *dtp;
+  double* const slot = stage + (size_t)blockIdx.x * 2 * SQ;
+  // opaque zero: keeps the Kxi / iK1 reads scalar loads inside the loops
+  int tz = 0;
+
+  for (long long cell = blockIdx.x; cell < ncells; cell += gridDim.x) {
+    __syncthreads();  // the previous cell is done with the LDS
+    pencil_constants<N, double>(sc, dt, dx0, dx1, dx2);
+    double u[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) u[v] = act ? luh[cell * V * B3 + v * B3 + s] : 1.0;
+    double* q = slot;
+    double* r = slot + SQ;
+#pragma unroll 1
+    for (int t = 0; t < B; ++t) {
+      asm volatile("" : "+s"(tz));
+      const int sl = s + tz;
+      if (act)
+#pragma unroll
+        for (int v = 0; v < V; ++v) q[(t * V + v) * B3 + sl] = u[v];
+    }
+    __syncthreads();
+    int it = 0;
+    bool done = false;
+    while (!done) {
+      ++it;
+#pragma unroll 1
+      for (int t = 0; t < B; ++t) {
+        asm volatile("" : "+s"(tz));
+        const int sl = s + tz;
+        if (act) {
+          double Q[V], F[3][V];
+#pragma unroll
+          for (int v = 0; v < V; ++v) Q[v] = q[(t * V + v) * B3 + sl];
+          EulerPDE::flux<double>(Q, F, gamma);
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            sF[(0 * V + v) * BLK + ox] = F[0][v];
+            sF[(1 * V + v) * BLK + oy] = F[1][v];
+            sF[(2 * V + v) * BLK + oz] = F[2][v];
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < ROUNDS; ++k) {
+          asm volatile("" : "+s"(tz));
+          const int i = s + k * O::NT;
+          if (i < NITEMS) {
+            const int dv = i / B2, ab = i - dv * B2, d = dv / V;
+            const double c = sc[(d * B + t) * B2 + ab];
+            double* pf = sF + dv * BLK + ab * PS;
+            double f[B];
+#pragma unroll
+            for (int n = 0; n < B; ++n) f[n] = c * pf[n];
+#pragma unroll
+            for (int l = 0; l < B; ++l) {
+              double acc = 0.0;
+#pragma unroll
+              for (int n = 0; n < B; ++n) acc += f[n] * c_tab[N + tz].Kxi[n][l];
+              pf[l] = acc;
+            }
+          }
+        }
+        __syncthreads();
+        if (act) {
+          const double c0 = w3 * c_tab[N + tz].F0[t];
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            const double rr =
+                ((c0 * u[v] - sF[(0 * V + v) * BLK + ox]) - sF[(1 * V + v) * BLK + oy]) - sF[(2 * V + v) * BLK + oz];
+            r[(t * V + v) * B3 + sl] = iw3 * rr;
+          }
+        }
+        __syncthreads();
+      }
+      // (threads past the last node read node 0 and store nothing)
+      double res = 0.0;
+      const int sa = act ? s : 0;
+#pragma unroll 1
+      for (int v = 0; v < V; ++v) {
+        asm volatile("" : "+s"(tz));
+        const int sv = sa + tz;
+        double ri[B];
+#pragma unroll
+        for (int t = 0; t < B; ++t) ri[t] = r[(t * V + v) * B3 + sv];
+#pragma unroll
+        for (int t2 = 0; t2 < B; ++t2) {
+          asm volatile("" : "+s"(tz));
+          double acc = 0.0;
+#pragma unroll
+          for (int t = 0; t < B; ++t) acc += ri[t] * c_tab[N + tz].iK1[t2][t];
+          const double dd = q[(t2 * V + v) * B3 + sv] - acc;
+          if (act) {
+            res += dd * dd;
+            r[(t2 * V + v) * B3 + sv] = acc;
+          }
+        }
+      }
+      res = block_sum(res, red);
+      // block-uniform (every thread holds the same sum); readfirstlane makes
+      // that visible to the compiler, which keeps the loops scalar
+      done = __builtin_amdgcn_readfirstlane((res < 1e-7 * 1e-7 || it == MaxIterations) ? 1 : 0) != 0;
+      if (!done) {
+        double* tmp = q;
+        q = r;
+        r = tmp;
+      }
+    }
+    // q: the input iterate of the last sweep (its fluxes are the time-averaged
+    // fluxes, aderPredictorNonlinear :326-397); r: the final iterate
+    double fh[3][V];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int v = 0; v < V; ++v) fh[d][v] = 0.0;
+#pragma unroll 1
+    for (int t = 0; t < B; ++t) {
+      asm volatile("" : "+s"(tz));
+      const int sl = s + tz;
+      if (act) {
+        double Q[V], F[3][V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) Q[v] = q[(t * V + v) * B3 + sl];
+        EulerPDE::flux<double>(Q, F, gamma);
+        const double wt = T.w[t];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            fh[d][v] += F[d][v] * wt;
+            if (DEBUG) dbg.lFi[((t * B3 + s) * 4 + d) * V + v] = F[d][v];
+          }
+        if (DEBUG)
+#pragma unroll
+          for (int v = 0; v < V; ++v) dbg.lFi[((t * B3 + s) * 4 + 3) * V + v] = 0.0;
+      }
+    }
+    if (act) {
+#pragma unroll 1
+      for (int v = 0; v < V; ++v) {
+        double a = 0.0;
+#pragma unroll
+        for (int t = 0; t < B; ++t) {
+          const double qt = r[(t * V + v) * B3 + s];
+          a += qt * T.w[t];
+          if (DEBUG) dbg.lQi[(s * B + t) * V + v] = qt;
+        }
+        sQh[v * B3 + s] = a;
+        if (DEBUG) dbg.lQhi[s * V + v] = a;
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int v = 0; v < V; ++v) sFh[(d * V + v) * B3 + s] = fh[d][v];
+      if (DEBUG) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          dbg.lFhi[0 * B3 * V + ((z * B + y) * B + x) * V + v] = fh[0][v];
+          dbg.lFhi[1 * B3 * V + ((z * B + x) * B + y) * V + v] = fh[1][v];
+          dbg.lFhi[2 * B3 * V + ((y * B + x) * B + z) * V + v] = fh[2][v];
+          dbg.lFhi[3 * B3 * V + s * V + v] = 0.0;
+        }
+      }
+    }
+    __syncthreads();
+    if (act) {
+      double out[V];
+      volume_integral_node<N>(sFh, s, z, y, x, dx0, dx1, dx2, out);
+#pragma unroll
+      for (int v = 0; v < V; ++v) lduh[cell * V * B3 + v * B3 + s] = out[v];
+    }
+    extrapolate_faces<N>(sQh, sFh, s, qf + cell * 6 * V * B2, ff + cell * 6 * V * B2);
+    if (s == 0) {
+      if (iters) iters[cell] = it;
+      if (sweep_count) atomicAdd(sweep_count, (unsigned long long)it);
+    }
   }
 }
 

@@ -38,8 +38,10 @@ void exa_set_error(const char* fmt, ...) {
     case 5: { constexpr int NN = 5; BODY; } break; \
     case 6: { constexpr int NN = 6; BODY; } break; \
     case 7: { constexpr int NN = 7; BODY; } break; \
+    case 8: { constexpr int NN = 8; BODY; } break; \
+    case 9: { constexpr int NN = 9; BODY; } break; \
     default:                                      \
-      exa_set_error("order %d not supported (1..7)", N); \
+      exa_set_error("order %d not supported (1..9)", N); \
       return EXA_ERR_UNSUPPORTED;                 \
   }
 
@@ -61,13 +63,37 @@ static PredFn predictor_kernel(bool debug, int prec, int wpe, int form) {
     case 4: return exa_pred_kernel_4(debug, prec, wpe, form);
     case 5: return exa_pred_kernel_5(debug, prec, wpe, form);
     case 6: return exa_pred_kernel_6(debug, prec, wpe, form);
-    default: return exa_pred_kernel_7(debug, prec, wpe, form);
+    case 7: return exa_pred_kernel_7(debug, prec, wpe, form);
+    default: return nullptr;  // orders 8, 9: exa_pred_staged_kernel
   }
 }
 
+// Staging slots of the order 8/9 predictor: one per resident workgroup.
+static int staged_slots(int N, int device, long long ncells, int* slots) {
+  int ncu = 0, per_cu = 0;
+  EXA_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+  const int nt = N == 8 ? exa::Ord<8>::NT : exa::Ord<9>::NT;
+  EXA_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, reinterpret_cast<const void*>(exa_pred_staged_kernel(N, false)), nt,
+      (size_t)exa_pred_staged_lds_bytes(N)));
+  long long n = (long long)ncu * (per_cu > 0 ? per_cu : 1);
+  if (n > ncells) n = ncells;
+  *slots = (int)(n > 0 ? n : 1);
+  return EXA_OK;
+}
+static size_t staged_slot_bytes(int N) {
+  const size_t B = (size_t)N + 1;
+  return 2 * B * B * B * B * exa::V * sizeof(double);
+}
+
+// LDS of the stand-alone volume integral: sQh slot + sFh, 4 V B^3 doubles
+template <int N>
+constexpr int volume_integral_lds_bytes() {
+  return 4 * exa::V * exa::Ord<N>::B3 * (int)sizeof(double);
+}
 template <int N>
 static hipError_t set_lds_attrs() {
-  const int bytes = exa::Ord<N>::LDS_DOUBLES * (int)sizeof(double);
+  const int bytes = volume_integral_lds_bytes<N>();
   return hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::k_volume_integral_ref<N>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
@@ -120,6 +146,7 @@ extern "C" int exa_init(int device) {
   EXA_CHECK(exa_pred_init_5(host, hostf));
   EXA_CHECK(exa_pred_init_6(host, hostf));
   EXA_CHECK(exa_pred_init_7(host, hostf));
+  EXA_CHECK(exa_pred_staged_init(host, hostf));
   EXA_CHECK(set_lds_attrs<1>());
   EXA_CHECK(set_lds_attrs<2>());
   EXA_CHECK(set_lds_attrs<3>());
@@ -127,6 +154,8 @@ extern "C" int exa_init(int device) {
   EXA_CHECK(set_lds_attrs<5>());
   EXA_CHECK(set_lds_attrs<6>());
   EXA_CHECK(set_lds_attrs<7>());
+  EXA_CHECK(set_lds_attrs<8>());
+  EXA_CHECK(set_lds_attrs<9>());
   g_init_mask |= (1ull << device);
   return EXA_OK;
 }
@@ -144,7 +173,7 @@ struct exa_solver {
   hipStream_t stream = nullptr;
   std::mutex mu;
   // scratch for the per-cell entry points
-  DevBuf luh, lduh, qf, ff, iters, dt, dbg, misc;
+  DevBuf luh, lduh, qf, ff, iters, dt, dbg, misc, stage;
 };
 
 // Contraction form of the predictor's Picard sweeps: EXA_CONTRACT_PENCIL
@@ -174,7 +203,8 @@ static int predictor_form(const exa_aderdg_config& c, int N, int dev) {
 
 extern "C" int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out) {
   EXA_ARG(cfg && out, "null pointer");
-  EXA_ARG(cfg->order >= 1 && cfg->order <= 7, "order must be 1..7");
+  EXA_ARG(cfg->order >= 1 && cfg->order <= 9, "order must be 1..9");
+  EXA_ARG(cfg->order <= 7 || cfg->precision == EXA_PREC_FP64, "orders 8 and 9 run the FP64 predictor only");
   EXA_ARG(cfg->pde == EXA_PDE_EULER && cfg->nvar == 5, "only 3D Euler with 5 variables is compiled in");
   EXA_ARG(cfg->npar >= 0 && cfg->npar <= 3, "npar must be 0..3");
   EXA_ARG(cfg->cfl > 0.0 && cfg->gamma > 1.0, "cfl/gamma");
@@ -255,11 +285,20 @@ extern "C" int exa_aderdg_space_time_predictor(exa_solver* s, double* lQhbnd, do
   const double gamma = s->cfg.gamma;
   EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
-    const int form = predictor_form(s->cfg, NN, s->device);
-    PredFn kern = predictor_kernel<NN>(true, s->cfg.precision, 1, form);
-    hipLaunchKernelGGL(kern, dim3(1), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), s->stream, s->luh.as<double>(),
-                       s->lduh.as<double>(), s->qf.as<double>(), s->ff.as<double>(), s->iters.as<int>(),
-                       s->dt.as<double>(), dx[0], dx[1], dx[2], gamma, s->cfg.fp64_sweeps, nullptr, dbg, exa::Ens{});
+    if constexpr (NN >= 8) {
+      EXA_TRY(s->stage.ensure(staged_slot_bytes(NN)));
+      hipLaunchKernelGGL(exa_pred_staged_kernel(NN, true), dim3(1), dim3(O::NT),
+                         (size_t)exa_pred_staged_lds_bytes(NN), s->stream, s->luh.as<double>(), s->lduh.as<double>(),
+                         s->qf.as<double>(), s->ff.as<double>(), s->iters.as<int>(), s->dt.as<double>(), dx[0],
+                         dx[1], dx[2], gamma, 1LL, s->stage.as<double>(), nullptr, dbg);
+    } else {
+      const int form = predictor_form(s->cfg, NN, s->device);
+      PredFn kern = predictor_kernel<NN>(true, s->cfg.precision, 1, form);
+      hipLaunchKernelGGL(kern, dim3(1), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), s->stream, s->luh.as<double>(),
+                         s->lduh.as<double>(), s->qf.as<double>(), s->ff.as<double>(), s->iters.as<int>(),
+                         s->dt.as<double>(), dx[0], dx[1], dx[2], gamma, s->cfg.fp64_sweeps, nullptr, dbg,
+                         exa::Ens{});
+    }
   });
   EXA_CHECK(hipGetLastError());
   std::vector<double> hq(6 * V * B2), hf(6 * V * B2), hdbg(nQi + nFi + nQhi + nFhi);
@@ -332,7 +371,7 @@ extern "C" int exa_aderdg_volume_integral(exa_solver* s, double* lduh, const dou
   EXA_CHECK(hipMemcpyAsync(s->misc.p, lFhi, sizeof(double) * 4 * B3 * V, hipMemcpyHostToDevice, s->stream));
   EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
-    hipLaunchKernelGGL((exa::k_volume_integral_ref<NN>), dim3(1), dim3(O::NT), O::LDS_DOUBLES * sizeof(double),
+    hipLaunchKernelGGL((exa::k_volume_integral_ref<NN>), dim3(1), dim3(O::NT), (size_t)volume_integral_lds_bytes<NN>(),
                        s->stream, s->misc.as<double>(), s->lduh.as<double>(), dx[0], dx[1], dx[2]);
   });
   EXA_CHECK(hipGetLastError());
@@ -482,6 +521,8 @@ struct exa_grid {
   long long nc = 0;
   double off[3] = {0, 0, 0}, width[3] = {1, 1, 1}, dx[3] = {1, 1, 1};
   DevBuf luh, lduh, qf, ff, iters, nbr, state;
+  DevBuf stage;    // order 8/9 predictor: one staging slot per resident workgroup
+  int nslots = 0;
   hipStream_t stream = nullptr;
   bool use_graph = false;
   hipGraph_t graph = nullptr;
@@ -525,6 +566,13 @@ extern "C" int exa_grid_create(exa_solver* s, int nx, int ny, int nz, const doub
       (r = g->state.ensure(sizeof(double) * 8)) != EXA_OK) {
     delete g;
     return r;
+  }
+  if (s->cfg.order >= 8) {
+    if ((r = staged_slots(s->cfg.order, s->device, g->nc, &g->nslots)) != EXA_OK ||
+        (r = g->stage.ensure(staged_slot_bytes(s->cfg.order) * (size_t)g->nslots)) != EXA_OK) {
+      delete g;
+      return r;
+    }
   }
   std::vector<int> nbr(3 * g->nc);
   for (long long c = 0; c < g->nc; ++c) {
@@ -709,13 +757,20 @@ static int enqueue_step(exa_grid* g, bool timed) {
   nodbg.lQi = nodbg.lFi = nodbg.lQhi = nodbg.lFhi = nullptr;
   EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
-    const int wpe = predictor_waves_per_simd(g->s->cfg, NN, g->s->device);
-    const int form = predictor_form(g->s->cfg, NN, g->s->device);
-    PredFn kern = predictor_kernel<NN>(false, g->s->cfg.precision, wpe, form);
-    hipLaunchKernelGGL(kern, dim3((unsigned)g->nc), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), g->stream,
-                       g->luh.as<double>(), g->lduh.as<double>(), g->qf.as<double>(), g->ff.as<double>(),
-                       g->iters.as<int>(), g->dt_cur(), g->dx[0], g->dx[1], g->dx[2], g->s->cfg.gamma,
-                       g->s->cfg.fp64_sweeps, g->iter_sum(), nodbg, exa::Ens{});
+    if constexpr (NN >= 8) {
+      hipLaunchKernelGGL(exa_pred_staged_kernel(NN, false), dim3((unsigned)g->nslots), dim3(O::NT),
+                         (size_t)exa_pred_staged_lds_bytes(NN), g->stream, g->luh.as<double>(), g->lduh.as<double>(),
+                         g->qf.as<double>(), g->ff.as<double>(), g->iters.as<int>(), g->dt_cur(), g->dx[0],
+                         g->dx[1], g->dx[2], g->s->cfg.gamma, g->nc, g->stage.as<double>(), g->iter_sum(), nodbg);
+    } else {
+      const int wpe = predictor_waves_per_simd(g->s->cfg, NN, g->s->device);
+      const int form = predictor_form(g->s->cfg, NN, g->s->device);
+      PredFn kern = predictor_kernel<NN>(false, g->s->cfg.precision, wpe, form);
+      hipLaunchKernelGGL(kern, dim3((unsigned)g->nc), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), g->stream,
+                         g->luh.as<double>(), g->lduh.as<double>(), g->qf.as<double>(), g->ff.as<double>(),
+                         g->iters.as<int>(), g->dt_cur(), g->dx[0], g->dx[1], g->dx[2], g->s->cfg.gamma,
+                         g->s->cfg.fp64_sweeps, g->iter_sum(), nodbg, exa::Ens{});
+    }
   });
   EXA_CHECK(hipGetLastError());
   EXA_TRY(tstop(0));

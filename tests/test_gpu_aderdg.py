@@ -123,7 +123,29 @@ def test_predictor_vs_oracle(N, contraction):
             assert rel_err(g[k], o[k]) <= TOL, (N, k, rel_err(g[k], o[k]))
 
 
-@pytest.mark.parametrize("N", [2, 3, 5, 7])
+@pytest.mark.parametrize("N", [8, 9])
+def test_staged_predictor_vs_oracle(N):
+    """Orders 8 and 9 stage the space-time iterate in global memory; same bar."""
+    s = exa().ADERDGSolver(N)
+    c = O.cfg(N)
+    for seed in range(2):
+        luh = smooth_cell_state(N, seed)
+        dx = np.array([1 / 27, 1 / 27, 1 / 27])
+        dt = 0.9 * O.stable_time_step_size(c, luh, dx)
+        g = s.spaceTimePredictor(luh, dx, dt)
+        o = O.space_time_predictor(c, luh, dx, dt)
+        assert g["iters"] == o["iters"], (N, seed)
+        for k in ("lQi", "lQhi", "lQhbnd", "lFhbnd", "lFhi", "lFi"):
+            assert rel_err(g[k], o[k]) <= TOL, (N, k, rel_err(g[k], o[k]))
+
+
+def test_staged_orders_reject_mixed_precision():
+    E = exa()
+    with pytest.raises(RuntimeError, match="FP64"):
+        E.ADERDGSolver(9, precision=E.PREC_FP32)
+
+
+@pytest.mark.parametrize("N", [2, 3, 5, 7, 9])
 def test_face_and_cell_kernels_vs_oracle(N):
     s = exa().ADERDGSolver(N)
     c = O.cfg(N)
@@ -189,6 +211,47 @@ def test_grid_steps_vs_oracle(N, n, steps, contraction):
         t, dt_g = g.sync()
         assert abs(dt_g - dt_o) <= 1e-12 * dt_o, (k, dt_g, dt_o)
         assert np.array_equal(g.picard_iterations(), og.iters()), k
+    assert rel_err(g.download(), og.luh()) <= TOL
+
+
+@pytest.mark.parametrize("N,dims,steps", [(9, (2, 3, 1), 2), (8, (3, 2, 2), 2)])
+def test_staged_grid_steps_vs_oracle(N, dims, steps):
+    E = exa()
+    s = E.ADERDGSolver(N)
+    c = O.cfg(N)
+    og = O.OracleGrid(c, *dims)
+    og.init(O.make_ic(0, **_ic_kw(11)))
+    g = E.Grid(s, *dims)
+    g.upload(og.luh().copy())
+    dt_o = og.stable_dt()
+    assert abs(g.compute_dt() - dt_o) <= 1e-14 * dt_o
+    for k in range(steps):
+        g.step(1)
+        dt_o = og.step(dt_o)
+        t, dt_g = g.sync()
+        assert abs(dt_g - dt_o) <= 1e-12 * dt_o, (k, dt_g, dt_o)
+        assert np.array_equal(g.picard_iterations(), og.iters()), k
+    assert rel_err(g.download(), og.luh()) <= TOL
+
+
+def test_staged_grid_more_cells_than_slots():
+    """The order-8 grid walks more cells than it has staging slots (one per
+    resident workgroup): every cell must match the per-cell predictor path of
+    the oracle after one step."""
+    E = exa()
+    N, dims = 8, (9, 7, 5)  # 315 cells > 256 slots
+    s = E.ADERDGSolver(N)
+    c = O.cfg(N)
+    og = O.OracleGrid(c, *dims)
+    og.init(O.make_ic(0, **_ic_kw(3)))
+    g = E.Grid(s, *dims)
+    g.upload(og.luh().copy())
+    dt_o = og.stable_dt()
+    g.compute_dt()
+    g.step(1)
+    og.step(dt_o)
+    g.sync()
+    assert np.array_equal(g.picard_iterations(), og.iters())
     assert rel_err(g.download(), og.luh()) <= TOL
 
 

@@ -29,7 +29,7 @@ def _poly_ic(rng, N):
     return u0
 
 
-@pytest.mark.parametrize("N", [1, 2, 3, 5, 7])
+@pytest.mark.parametrize("N", [1, 2, 3, 5, 7, 8, 9])
 def test_predictor_exact_for_linear_advection(N):
     rng = np.random.default_rng(100 + N)
     a = np.array([0.3, -0.5, 0.7])
@@ -48,14 +48,17 @@ def test_predictor_exact_for_linear_advection(N):
     luh = (u0(pts)[..., None] * scale).ravel()
     out = O.space_time_predictor(c, luh, dx, dt)
     assert out["iters"] <= 2 * B
+    # Picard stops once the update is below 1e-7 in the l2 norm over the
+    # cell; at N >= 8 (values ~10 here) that stopping error exceeds 1e-9
+    tol = 1e-9 if N <= 7 else 1e-7
     lQi = out["lQi"].reshape(B, B, B, B, 5)
     for t in range(B):
         exact = u0(pts - a * xi[t] * dt)[..., None] * scale
-        np.testing.assert_allclose(lQi[:, :, :, t, :], exact, rtol=1e-9, atol=1e-9)
+        np.testing.assert_allclose(lQi[:, :, :, t, :], exact, rtol=tol, atol=tol)
     # time average and face values
     tq, tw = dg_math.gl_nodes_weights(N + 4)
     avg = sum(wt * u0(pts - a * tt * dt) for tt, wt in zip(tq, tw))[..., None] * scale
-    np.testing.assert_allclose(out["lQhi"].reshape(B, B, B, 5), avg, rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(out["lQhi"].reshape(B, B, B, 5), avg, rtol=tol, atol=tol)
     lQhbnd = out["lQhbnd"].reshape(6, B, B, 5)
     lFhbnd = out["lFhbnd"].reshape(6, B, B, 5)
     # x faces: (a,b)=(z,y); y faces: (z,x); z faces: (y,x)
@@ -70,8 +73,8 @@ def test_predictor_exact_for_linear_advection(N):
                 ref = np.einsum("zyxv,y->zxv", qh, FLR)
             else:
                 ref = np.einsum("zyxv,z->yxv", qh, FLR)
-            np.testing.assert_allclose(lQhbnd[face], ref, rtol=1e-9, atol=1e-9)
-            np.testing.assert_allclose(lFhbnd[face], a[d] * ref, rtol=1e-9, atol=1e-9)
+            np.testing.assert_allclose(lQhbnd[face], ref, rtol=tol, atol=tol)
+            np.testing.assert_allclose(lFhbnd[face], a[d] * ref, rtol=tol, atol=tol)
 
 
 def _euler_state(rng, B, nnodes):
@@ -82,7 +85,7 @@ def _euler_state(rng, B, nnodes):
     return np.column_stack([rho, rho * u[:, 0], rho * u[:, 1], rho * u[:, 2], E])
 
 
-@pytest.mark.parametrize("N", [2, 3, 5])
+@pytest.mark.parametrize("N", [2, 3, 5, 8, 9])
 def test_predictor_satisfies_discrete_fixed_point(N):
     """lQi = (1/w^3) iK1 . (w^3 F0 luh - sum_d c_d Kxi^T F_d(lQi)) up to the Picard tolerance."""
     rng = np.random.default_rng(7 + N)
