@@ -29,6 +29,7 @@ METRIC = "DoF-updates/s, 3D Euler ADER-DG N=5, mixed vs FP64 GPU at L2 err ≤1e
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_BF16_TFLOPS = 2500.0
+PEAK_HBM_GBPS = 8000.0
 # mixed-precision variants measured: (name, precision, FP64 correction sweeps)
 MIXED_VARIANTS = [("fp32", 1, 1), ("fp32_mfma", 2, 1), ("bf16_mfma", 3, 2), ("bf16_mfma_3", 3, 3)]
 PHI_FLUX = 40   # flops of one 3D Euler flux evaluation (1 division counted as 1)
@@ -129,12 +130,14 @@ def main():
     ap.add_argument("--waves-per-simd", type=int, default=0, help="predictor occupancy target (0 = tuned)")
     ap.add_argument("--contraction", type=int, default=0,
                     help="predictor sweep contraction: 0 tuned, 1 pencil owners, 2 node owners, "
-                         "3 pencil owners two slices at a time")
-    ap.add_argument("--workload", default="aderdg", choices=["aderdg", "fv2d", "ensemble", "limiter"],
+                         "3 pencil owners two slices at a time, 4 staged iterate (FP64, orders 5..9)")
+    ap.add_argument("--workload", default="aderdg", choices=["aderdg", "fv2d", "ensemble", "limiter", "muscl", "amr"],
                     help="aderdg: the headline ADER-DG step; fv2d: the EulerFV demonstrator "
                          "(2D Godunov, 81x81 patches of 10x10 volumes, 200 steps); ensemble: independent "
                          "grids of random order/size/gamma advanced together; limiter: troubled-cell detection and "
-                         "subcell projection on the Sod sphere (27^3 cells, N=5)")
+                         "subcell projection on the Sod sphere (27^3 cells, N=5); muscl: 3D MUSCL-Hancock FV on "
+                         "27^3 patches of 11^3 subcells (the limiter's FV scheme of the N=5 Sod sphere); amr: batched "
+                         "volume prolongation/restriction of 27^3 N=5 cells one level down and back")
     ap.add_argument("--instances", type=int, default=512, help="ensemble members (--workload ensemble)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also time a plain device copy of this many GiB (HBM reference, counter calibration)")
@@ -153,6 +156,12 @@ def main():
     if args.workload == "limiter":
         print(json.dumps(run_limiter(args)))
         return
+    if args.workload == "muscl":
+        print(json.dumps(run_muscl(args)))
+        return
+    if args.workload == "amr":
+        print(json.dumps(run_amr(args)))
+        return
     import exahype_amd as E
     N, n = args.order, args.cells
     ncells = n ** 3
@@ -166,7 +175,9 @@ def main():
     for name, prec, sweeps in MIXED_VARIANTS:
         if N > 7:  # orders 8 and 9 run the FP64 predictor only
             continue
-        r = run_gpu(N, n, args.steps, args.warmup, prec, args.waves_per_simd, args.contraction, sweeps)
+        # the staged form is FP64 only: the mixed runs take their tuned form
+        cm = 0 if args.contraction == 4 else args.contraction
+        r = run_gpu(N, n, args.steps, args.warmup, prec, args.waves_per_simd, cm, sweeps)
         l2 = l2_per_variable(r["state"], r64["state"], N, ncells)
         r.update(rate=dof * args.steps / r["wall"], l2=l2, ok=bool(l2.max() <= 1e-6), name=name, sweeps_fp64=sweeps,
                  prec=prec)
@@ -307,6 +318,139 @@ def run_fv2d(args):
                                "kind": "port", "sample": "oracle (C, OpenMP), the whole workload, %d steps (%.1f s)"
                                                          % (steps, cwall)}
         out["parity_vs_oracle_rel_inf"] = err
+    return out
+
+
+def sod_ic(mod):
+    return mod.make_ic(1, xc=(0.5, 0.5, 0.5), r0=0.25, rho_in=1.0, p_in=1.0, rho_out=0.125, p_out=0.1)
+
+
+def run_muscl(args):
+    """3D MUSCL-Hancock finite volumes (kernels/finitevolumes/musclhancock/c/3d/
+    musclhancock.cpph) on the limiter's subcell grid of the N=5 Sod sphere: 27^3
+    patches of 11^3 volumes with 2 ghost layers (297^3 volumes), periodic,
+    20 steps.  Roofline: HBM bytes of the MUSCL-Hancock update (slopes + half-
+    step states kernel, then the Riemann/update kernel), counted as the
+    compulsory read of every patch (ghosts included) and write of its interior,
+    over the two kernels' HIP-event time."""
+    import exahype_amd as E
+    from oracle import oracle as O
+    n, patch, steps = 27, 11, args.steps if args.steps != 10 else 20
+    S = patch + 4
+    s = E.FiniteVolumesSolver(3, patch, E.FV_MUSCLHANCOCK)
+    g = E.FVGrid(s, n, n, n)
+    g.set_initial_conditions(sod_ic(E))
+    g.compute_dt()
+    g.step(1)
+    g.sync()
+    g.kernel_times()
+    g.enable_timing(True)
+    E.device_synchronize()
+    t0 = time.perf_counter()
+    g.step(steps)
+    t_end, _, _ = g.sync()
+    wall = time.perf_counter() - t0
+    times, counts = g.kernel_times()
+    ms = times / np.maximum(counts, 1)
+    vols = (n * patch) ** 3
+    rate = vols * NVAR * steps / wall
+    bytes_upd = n ** 3 * NVAR * 8 * (S ** 3 + patch ** 3)
+    upd_ms = ms[1] + ms[2]
+    achieved = bytes_upd / (upd_ms * 1e-3) / 1e9
+    out = {"metric": "DoF-updates/s, 3D Euler FV MUSCL-Hancock (27^3 patches of 11^3 volumes)", "value": rate,
+           "unit": "DoF-updates/s", "n_gpus": 1, "steps": steps, "warmup": 1, "ms_per_step": wall / steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "3D Euler MUSCL-Hancock + Rusanov, minmod slopes, CFL 0.99, 27^3 patches of 11^3 "
+                                  "volumes (2 ghost layers), periodic, Sod sphere r0=0.25", "volumes": vols,
+                      "t_end": t_end},
+           "phase_ms_per_step": {"ghosts": ms[0], "states": ms[1], "update": ms[2], "dt": ms[3]},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                        "frac": achieved / PEAK_HBM_GBPS, "traffic": None,
+                        "kernel": "k_muscl_states + k_muscl_update", "avg_launch_ms": upd_ms,
+                        "bytes_per_launch": bytes_upd}}
+    if not args.no_cpu_baseline:
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        c = O.cfg(1)
+        ns, st = 9, 3
+        og = O.OracleFVGrid(c, 3, 1, patch, ns, ns, ns)
+        og.init(sod_ic(O))
+        dt = og.stable_dt(threads)
+        t0 = time.perf_counter()
+        for _ in range(st):
+            dt, _ = og.step(dt, threads)
+        cwall = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": (ns * patch) ** 3 * NVAR * st / cwall, "unit": "DoF-updates/s",
+                               "cores": threads, "kind": "port",
+                               "sample": "oracle (C, OpenMP), 9^3 patches of 11^3 volumes, Sod sphere, %d steps "
+                                         "(%.1f s)" % (st, cwall)}
+    return out
+
+
+def run_amr(args):
+    """Batched AMR level transfers of volume unknowns (amrRoutines.cpph:
+    volumeUnknownsProlongation / Restriction): every cell of a 27^3 N=5 grid
+    prolongated to its 27 children one level down (531441 transfers), then the
+    children restricted back (27 consecutive transfers per coarse cell).
+    Roofline: compulsory HBM bytes (read the inputs once, write the outputs
+    once) over the kernel's HIP-event time, median of 5 launches."""
+    import exahype_amd as E
+    from oracle import oracle as O
+    N, n = 5, 27
+    B = N + 1
+    blk = B ** 3 * NVAR
+    nc = n ** 3
+    nf = nc * 27
+    rng = np.random.default_rng(1)
+    coarse_h = 1.0 + rng.random(nc * blk)
+    coarse = E.DeviceArray(coarse_h)
+    fine = E.DeviceArray(nf * blk)
+    back = E.DeviceArray(nc * blk)
+    child = np.arange(nf)
+    par = child // 27
+    sub = child % 27
+    idx = np.stack([sub % 3, (sub // 3) % 3, sub // 9], axis=1)
+    tr = E.amr_transfer_array(child * blk, par * blk, 1, idx)
+    res = {}
+    for name, fn, args_ in (("prolongation", E.amrProlongationBatched, (3, N, NVAR, fine, coarse, tr)),
+                            ("restriction", E.amrRestrictionBatched, (3, N, NVAR, back, fine, tr))):
+        t = []
+        for _ in range(6):
+            if name == "restriction":
+                back.upload(np.zeros(nc * blk))
+            fn(*args_)
+            t.append(E.amr_last_kernel_ms())
+        ms = float(np.median(t[1:]))
+        byts = (nc + nf) * blk * 8
+        res[name] = {"kernel_ms": ms, "bytes": byts, "GBps": byts / (ms * 1e-3) / 1e9}
+    # identity check: restrict(prolong(u)) == u for every cell (L2 projection of an exact embedding)
+    err = float(np.abs(back.numpy() - coarse_h).max() / np.abs(coarse_h).max())
+    ms_tot = res["prolongation"]["kernel_ms"] + res["restriction"]["kernel_ms"]
+    rate = 2 * nf / (ms_tot * 1e-3)
+    dom = max(res, key=lambda k: res[k]["kernel_ms"])
+    achieved = res[dom]["GBps"]
+    out = {"metric": "AMR volume transfers/s (N=5, one level, prolongation + restriction)", "value": rate,
+           "unit": "transfers/s", "n_gpus": 1, "steps": 5, "warmup": 1, "ms_per_step": ms_tot,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "27^3 cells N=5 -> 27 children each (531441 prolongations), then 531441 "
+                                  "restrictions back", "transfers_each": nf},
+           "kernels": res, "restrict_prolong_identity_rel_inf": err,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                        "frac": achieved / PEAK_HBM_GBPS, "traffic": None,
+                        "kernel": "k_amr_transfer<3, %s>" % ("true" if dom == "prolongation" else "false"),
+                        "avg_launch_ms": res[dom]["kernel_ms"]}}
+    if not args.no_cpu_baseline:
+        # the oracle's per-cell transfers, one thread, on a sample of the list
+        cnt = 2000
+        ch = coarse_h.reshape(nc, blk)
+        t0 = time.perf_counter()
+        for k in range(cnt):
+            O.volume_unknowns_prolongation(N, ch[par[k]], 0, 1, tuple(idx[k]))
+        cwall = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cnt / cwall, "unit": "transfers/s", "cores": 1, "kind": "port",
+                               "sample": "oracle volume prolongation, %d transfers through the Python wrapper "
+                                         "(%.2f s)" % (cnt, cwall)}
+    for a in (coarse, fine, back):
+        a.close()
     return out
 
 

@@ -26,7 +26,7 @@ PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_FP32_MFMA = 2
 PREC_BF16_MFMA = 3
-CONTRACT_DEFAULT, CONTRACT_PENCIL, CONTRACT_NODE, CONTRACT_PENCIL_PAIR = 0, 1, 2, 3
+CONTRACT_DEFAULT, CONTRACT_PENCIL, CONTRACT_NODE, CONTRACT_PENCIL_PAIR, CONTRACT_STAGED = 0, 1, 2, 3, 4
 PDE_EULER = 0
 PDE_EULERFV2D = 1
 FV_GODUNOV = 0
@@ -75,10 +75,10 @@ EXPORTS = [
     "exa_fv_create", "exa_fv_destroy", "exa_fv_solution_update", "exa_fv_stable_time_step_size",
     "exa_fv_riemann_solver", "exa_fv_grid_create", "exa_fv_grid_destroy", "exa_fv_grid_upload",
     "exa_fv_grid_download", "exa_fv_grid_set_initial_conditions", "exa_fv_grid_compute_dt", "exa_fv_grid_step",
-    "exa_fv_grid_sync", "exa_limiter_project_on_fv", "exa_limiter_project_on_dg", "exa_limiter_dmp",
+    "exa_fv_grid_sync", "exa_fv_grid_enable_timing", "exa_fv_grid_kernel_times", "exa_limiter_project_on_fv", "exa_limiter_project_on_dg", "exa_limiter_dmp",
     "exa_amr_face_unknowns_prolongation", "exa_amr_face_unknowns_restriction",
     "exa_amr_volume_unknowns_prolongation", "exa_amr_volume_unknowns_restriction",
-    "exa_amr_prolongation_batched", "exa_amr_restriction_batched",
+    "exa_amr_prolongation_batched", "exa_amr_restriction_batched", "exa_amr_last_kernel_ms",
     "exa_device_malloc", "exa_device_free", "exa_memcpy_htod", "exa_memcpy_dtoh", "exa_device_copy",
     "exa_selftest_mfma",
     "exa_ensemble_create", "exa_ensemble_destroy", "exa_ensemble_set_initial_conditions", "exa_ensemble_compute_dt",
@@ -138,6 +138,9 @@ def load_library(path=LIB_PATH):
     L.exa_fv_grid_compute_dt.argtypes = [C.c_void_p, _D]
     L.exa_fv_grid_step.argtypes = [C.c_void_p, I]
     L.exa_fv_grid_sync.argtypes = [C.c_void_p, _D, _D, _D]
+    L.exa_fv_grid_enable_timing.argtypes = [C.c_void_p, I]
+    L.exa_fv_grid_kernel_times.argtypes = [C.c_void_p, _D, P(LL)]
+    L.exa_amr_last_kernel_ms.argtypes = [_D]
     L.exa_limiter_project_on_fv.argtypes = [I, _D, I, _D]
     L.exa_limiter_project_on_dg.argtypes = [I, _D, I, _D]
     L.exa_limiter_dmp.argtypes = [I, _D, I, C.c_double, C.c_double, _D, _D, P(I)]
@@ -487,6 +490,17 @@ class FVGrid:
         _check(_lib.exa_fv_grid_sync(self._h, C.byref(t), C.byref(dt), C.byref(adm)))
         return t.value, dt.value, adm.value
 
+    def enable_timing(self, on=True):
+        _check(_lib.exa_fv_grid_enable_timing(self._h, 1 if on else 0))
+
+    def kernel_times(self):
+        """(ms summed, launches) per phase since the last call: ghost layers,
+        MUSCL-Hancock states, update, time step."""
+        t = np.zeros(4)
+        c = (C.c_longlong * 4)()
+        _check(_lib.exa_fv_grid_kernel_times(self._h, _p(t), c))
+        return t, np.array(list(c))
+
 
 def projectOnFVLimiterSpace(luh, order, ghost=0, lim=None):
     """Subcell averages of the DG polynomial (2N+1 subcells per direction)."""
@@ -580,7 +594,26 @@ def volumeUnknownsRestriction(luhCoarse, luhFine, coarseGridLevel, fineGridLevel
     return out
 
 
+AMR_TRANSFER_DTYPE = np.dtype([("fine", np.int64), ("coarse", np.int64), ("levels", np.int32),
+                               ("index", np.int32, (3,))], align=True)
+
+
+def amr_transfer_array(fine, coarse, levels, index):
+    """Transfer list as a numpy structured array (columns: fine and coarse
+    offsets in doubles, level difference, sub-cell index per axis)."""
+    fine = np.asarray(fine)
+    a = np.zeros(fine.size, AMR_TRANSFER_DTYPE)
+    a["fine"], a["coarse"], a["levels"] = fine, coarse, levels
+    idx = np.asarray(index).reshape(fine.size, -1)
+    a["index"][:, :idx.shape[1]] = idx
+    return a
+
+
 def _transfers(lst):
+    if isinstance(lst, np.ndarray):
+        if lst.dtype != AMR_TRANSFER_DTYPE or not lst.flags.c_contiguous:
+            raise ValueError("transfer arrays come from amr_transfer_array()")
+        return lst.ctypes.data_as(C.POINTER(AmrTransfer))
     arr = (AmrTransfer * max(1, len(lst)))()
     for i, (fine, coarse, levels, idx) in enumerate(lst):
         arr[i].fine, arr[i].coarse, arr[i].levels = int(fine), int(coarse), int(levels)
@@ -660,6 +693,13 @@ def amrProlongationBatched(dim, order, ndata, fine, coarse, transfers):
     pf, nf = _dev(fine)
     pc, nc = _dev(coarse)
     _check(L.exa_amr_prolongation_batched(dim, order, ndata, pf, nf, pc, nc, _transfers(transfers), len(transfers)))
+
+
+def amr_last_kernel_ms():
+    """Kernel duration (HIP events) of the last batched AMR call on this device."""
+    ms = C.c_double(0.0)
+    _check(load_library().exa_amr_last_kernel_ms(C.byref(ms)))
+    return ms.value
 
 
 def amrRestrictionBatched(dim, order, ndata, coarse, fine, transfers):

@@ -28,10 +28,10 @@ struct Ord {
   static constexpr int PS = (B % 2 == 0) ? B + 1 : B;
   static constexpr int BLK = B2 * PS;                // one (direction, variable) block of pencils
   static constexpr int LDS_DOUBLES = 6 * V * BLK + 3 * B3 + 16;  // flux slices + pencil constants + reduction scratch
-  // staged predictor (N >= 8): one flux slice + pencil constants during the
-  // sweeps, time averages sQh/sFh afterwards, reduction scratch at the end
-  static constexpr int LDS_STAGED =
-      (3 * V * BLK + 3 * B3 > 4 * V * B3 ? 3 * V * BLK + 3 * B3 : 4 * V * B3) + 16;
+  // staged predictor: one flux slice + pencil constants during the sweeps,
+  // time averages sQh/sFh afterwards; the reduction scratch of the residual
+  // sum aliases the flux slice (80 KB at N = 7: two workgroups per CU)
+  static constexpr int LDS_STAGED = 3 * V * BLK + 3 * B3 > 4 * V * B3 ? 3 * V * BLK + 3 * B3 : 4 * V * B3;
 };
 
 // ---------------------------------------------------------------------------
@@ -884,7 +884,7 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
   extern __shared__ double sm[];
   double* sF = sm;                     // flux slice, direction layouts, contracted in place
   double* sc = sm + 3 * V * BLK;       // pencil constants [d][t][a*B+b]
-  double* red = sm + O::LDS_STAGED - 16;
+  double* red = sm;                    // residual sum scratch (between slice loops only)
   double* sQh = sm;                    // after the sweeps: time averages
   double* sFh = sm + V * B3;
   const DGTables& T = c_tab[N];
@@ -993,6 +993,7 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
         }
       }
       res = block_sum(res, red);
+      __syncthreads();  // every thread has read red before the flux slice reuses it
       // block-uniform (every thread holds the same sum); readfirstlane makes
       // that visible to the compiler, which keeps the loops scalar
       done = __builtin_amdgcn_readfirstlane((res < 1e-7 * 1e-7 || it == MaxIterations) ? 1 : 0) != 0;

@@ -22,6 +22,8 @@ struct AmrCache {
   DevBuf tab[EXA_MAX_ORDER + 1];
   bool ready[EXA_MAX_ORDER + 1] = {};
   DevBuf items, groups, a, b, c, d;  // staging of the per-call and batched forms
+  hipEvent_t e0 = nullptr, e1 = nullptr;  // around the last batched launch
+  float last_ms = 0.f;
 };
 AmrCache g_amr[64];
 
@@ -160,9 +162,18 @@ int batched(int dim, bool prolong, int order, int nd, double* d_fine, long long 
   EXA_TRY(c.groups.ensure(sizeof(int) * groups.size()));
   EXA_CHECK(hipMemcpy(c.items.p, items.data(), sizeof(Transfer) * items.size(), hipMemcpyHostToDevice));
   EXA_CHECK(hipMemcpy(c.groups.p, groups.data(), sizeof(int) * groups.size(), hipMemcpyHostToDevice));
+  if (!c.e0) {
+    EXA_CHECK(hipEventCreate(&c.e0));
+    EXA_CHECK(hipEventCreate(&c.e1));
+  }
+  EXA_CHECK(hipEventRecord(c.e0, nullptr));
   EXA_TRY(launch(dim, prolong, tab, c.items.as<Transfer>(), c.groups.as<int>(), (int)groups.size() - 1, d_fine,
                  d_coarse, B, nd));
+  EXA_CHECK(hipEventRecord(c.e1, nullptr));
   EXA_CHECK(hipDeviceSynchronize());
+  float ms = 0.f;
+  EXA_CHECK(hipEventElapsedTime(&ms, c.e0, c.e1));
+  c.last_ms = ms;
   return EXA_OK;
 }
 
@@ -220,4 +231,15 @@ extern "C" int exa_amr_restriction_batched(int dim, int order, int ndata, double
                                            const exa_amr_transfer* transfers, int count) {
   EXA_ARG(dim == 2 || dim == 3, "dim must be 2 (faces) or 3 (volumes)");
   return batched(dim, false, order, ndata, const_cast<double*>(d_fine), n_fine, d_coarse, n_coarse, transfers, count);
+}
+
+extern "C" int exa_amr_last_kernel_ms(double* ms) {
+  EXA_ARG(ms, "null pointer");
+  int dev = 0;
+  EXA_CHECK(hipGetDevice(&dev));
+  EXA_ARG(dev >= 0 && dev < 64, "device index");
+  AmrCache& c = g_amr[dev];
+  std::lock_guard<std::mutex> lk(c.mu);
+  *ms = c.last_ms;
+  return EXA_OK;
 }

@@ -68,11 +68,11 @@ static PredFn predictor_kernel(bool debug, int prec, int wpe, int form) {
   }
 }
 
-// Staging slots of the order 8/9 predictor: one per resident workgroup.
+// Staging slots of the staged predictor: one per resident workgroup.
 static int staged_slots(int N, int device, long long ncells, int* slots) {
   int ncu = 0, per_cu = 0;
   EXA_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
-  const int nt = N == 8 ? exa::Ord<8>::NT : exa::Ord<9>::NT;
+  const int nt = round_up64((N + 1) * (N + 1) * (N + 1));
   EXA_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
       &per_cu, reinterpret_cast<const void*>(exa_pred_staged_kernel(N, false)), nt,
       (size_t)exa_pred_staged_lds_bytes(N)));
@@ -186,6 +186,15 @@ struct TunedVariant {
 };
 static TunedVariant g_tuned[64][EXA_MAX_ORDER + 1][4];
 
+// The staged predictor: orders 8, 9, chosen explicitly, or the measured
+// default of the FP64 predictor at N = 7 (81^3 cells: 222 ms per launch
+// staged vs 339 ms register-resident, whose sweeps spill at that order).
+static bool predictor_staged(const exa_aderdg_config& c, int dev) {
+  if (c.order >= 8 || c.contraction == EXA_CONTRACT_STAGED) return true;
+  return c.order == 7 && c.precision == EXA_PREC_FP64 && c.contraction == EXA_CONTRACT_DEFAULT &&
+         g_tuned[dev & 63][7][EXA_PREC_FP64].contraction == 0;  // exa_autotune's choice wins
+}
+
 static int predictor_form(const exa_aderdg_config& c, int N, int dev) {
   switch (c.contraction) {
     case EXA_CONTRACT_PENCIL: return exa::FORM_PENCIL;
@@ -205,13 +214,16 @@ extern "C" int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out)
   EXA_ARG(cfg && out, "null pointer");
   EXA_ARG(cfg->order >= 1 && cfg->order <= 9, "order must be 1..9");
   EXA_ARG(cfg->order <= 7 || cfg->precision == EXA_PREC_FP64, "orders 8 and 9 run the FP64 predictor only");
+  EXA_ARG(cfg->contraction != EXA_CONTRACT_STAGED ||
+              (cfg->precision == EXA_PREC_FP64 && cfg->order >= EXA_STAGED_MIN_ORDER),
+          "the staged predictor is FP64 and built for orders 5..9");
   EXA_ARG(cfg->pde == EXA_PDE_EULER && cfg->nvar == 5, "only 3D Euler with 5 variables is compiled in");
   EXA_ARG(cfg->npar >= 0 && cfg->npar <= 3, "npar must be 0..3");
   EXA_ARG(cfg->cfl > 0.0 && cfg->gamma > 1.0, "cfl/gamma");
   EXA_ARG(cfg->precision >= EXA_PREC_FP64 && cfg->precision <= EXA_PREC_BF16_MFMA, "precision");
   EXA_ARG(cfg->precision < EXA_PREC_FP32_MFMA || cfg->order <= 7, "MFMA sweeps need order <= 7");
   EXA_ARG(cfg->waves_per_simd >= 0 && cfg->waves_per_simd <= 2, "waves_per_simd must be 0, 1 or 2");
-  EXA_ARG(cfg->contraction >= 0 && cfg->contraction <= 3, "contraction must be 0..3");
+  EXA_ARG(cfg->contraction >= 0 && cfg->contraction <= 4, "contraction must be 0..4");
   int dev = 0;
   EXA_CHECK(hipGetDevice(&dev));
   EXA_TRY(exa_init(dev));
@@ -285,7 +297,7 @@ extern "C" int exa_aderdg_space_time_predictor(exa_solver* s, double* lQhbnd, do
   const double gamma = s->cfg.gamma;
   EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
-    if constexpr (NN >= 8) {
+    if (predictor_staged(s->cfg, s->device)) {
       EXA_TRY(s->stage.ensure(staged_slot_bytes(NN)));
       hipLaunchKernelGGL(exa_pred_staged_kernel(NN, true), dim3(1), dim3(O::NT),
                          (size_t)exa_pred_staged_lds_bytes(NN), s->stream, s->luh.as<double>(), s->lduh.as<double>(),
@@ -521,8 +533,9 @@ struct exa_grid {
   long long nc = 0;
   double off[3] = {0, 0, 0}, width[3] = {1, 1, 1}, dx[3] = {1, 1, 1};
   DevBuf luh, lduh, qf, ff, iters, nbr, state;
-  DevBuf stage;    // order 8/9 predictor: one staging slot per resident workgroup
+  DevBuf stage;    // staged predictor: one staging slot per resident workgroup
   int nslots = 0;
+  bool staged = false;  // decided at creation (the slots are allocated then)
   hipStream_t stream = nullptr;
   bool use_graph = false;
   hipGraph_t graph = nullptr;
@@ -567,7 +580,8 @@ extern "C" int exa_grid_create(exa_solver* s, int nx, int ny, int nz, const doub
     delete g;
     return r;
   }
-  if (s->cfg.order >= 8) {
+  g->staged = predictor_staged(s->cfg, s->device);
+  if (g->staged) {
     if ((r = staged_slots(s->cfg.order, s->device, g->nc, &g->nslots)) != EXA_OK ||
         (r = g->stage.ensure(staged_slot_bytes(s->cfg.order) * (size_t)g->nslots)) != EXA_OK) {
       delete g;
@@ -757,7 +771,7 @@ static int enqueue_step(exa_grid* g, bool timed) {
   nodbg.lQi = nodbg.lFi = nodbg.lQhi = nodbg.lFhi = nullptr;
   EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
-    if constexpr (NN >= 8) {
+    if (g->staged) {
       hipLaunchKernelGGL(exa_pred_staged_kernel(NN, false), dim3((unsigned)g->nslots), dim3(O::NT),
                          (size_t)exa_pred_staged_lds_bytes(NN), g->stream, g->luh.as<double>(), g->lduh.as<double>(),
                          g->qf.as<double>(), g->ff.as<double>(), g->iters.as<int>(), g->dt_cur(), g->dx[0],
@@ -929,6 +943,7 @@ extern "C" int exa_ensemble_destroy(exa_ensemble* e) {
 extern "C" int exa_ensemble_create(const exa_ensemble_member* m, int n, const exa_aderdg_config* cfg,
                                    exa_ensemble** out) {
   EXA_ARG(m && n > 0 && cfg && out, "arguments");
+  EXA_ARG(cfg->contraction != EXA_CONTRACT_STAGED, "ensembles run the register-resident predictor forms");
   size_t need = 0;
   for (int i = 0; i < n; ++i) {
     EXA_ARG(m[i].order >= 1 && m[i].order <= 7, "member order must be 1..7");

@@ -74,14 +74,16 @@ static void patch_soa_to_ref(const double* soa, double* ref, int nbox) {
 }
 
 // Launches the update of `np` patches: in -> out (out interior overwritten).
+// mid: recorded between the MUSCL-Hancock states and update kernels (timing).
 static int launch_fv_update(const exa_fv_solver* s, hipStream_t st, int np, const double* in, double* out,
                             double* W, double* dt_patch, unsigned long long* dt_min, const double dx[3],
-                            const double* dt) {
+                            const double* dt, hipEvent_t mid = nullptr) {
   const PatchGeom g = s->geom();
   const double gam = s->cfg.gamma;
   if (s->cfg.scheme == EXA_FV_MUSCLHANCOCK) {
     hipLaunchKernelGGL((exa::fv::k_muscl_states<EulerPDE>), dim3(np), dim3(256), 0, st, in, W, g, dx[0], dx[1],
                        dx[2], dt, gam);
+    if (mid) EXA_CHECK(hipEventRecord(mid, st));
     // the kernel's own CFL (kernels/finitevolumes/musclhancock/c/musclhancock.h:29)
     hipLaunchKernelGGL((exa::fv::k_muscl_update<EulerPDE>), dim3(np), dim3(256), 0, st, in, W, out, dt_patch, dt_min,
                        g, dx[0], dx[1], dx[2], dt, gam, 0.99);
@@ -195,6 +197,11 @@ struct exa_fv_grid {
   double off[3] = {0, 0, 0}, width[3] = {1, 1, 1}, dx[3] = {1, 1, 1};
   DevBuf cur, nxt, w, nbr, state;
   hipStream_t stream = nullptr;
+  // per-phase kernel timing (0 ghost layers, 1 MUSCL-Hancock states, 2 update, 3 dt)
+  bool timing = false;
+  std::vector<std::vector<hipEvent_t>> marks;  // 5 events per timed step
+  double times[4] = {0, 0, 0, 0};
+  long long counts[4] = {0, 0, 0, 0};
   double* dt_cur() const { return state.as<double>(); }
   unsigned long long* dt_next() const { return reinterpret_cast<unsigned long long*>(state.as<double>() + 1); }
   double* time() const { return state.as<double>() + 2; }
@@ -277,6 +284,12 @@ extern "C" int exa_fv_grid_create(exa_fv_solver* s, int nx, int ny, int nz, cons
   return EXA_OK;
 }
 
+static void fv_clear_marks(exa_fv_grid* g) {
+  for (auto& m : g->marks)
+    for (hipEvent_t e : m) (void)hipEventDestroy(e);
+  g->marks.clear();
+}
+
 extern "C" int exa_fv_grid_destroy(exa_fv_grid* g) {
   if (!g) return EXA_OK;
   (void)hipSetDevice(g->s->device);
@@ -284,7 +297,35 @@ extern "C" int exa_fv_grid_destroy(exa_fv_grid* g) {
     (void)hipStreamSynchronize(g->stream);
     (void)hipStreamDestroy(g->stream);
   }
+  fv_clear_marks(g);
   delete g;
+  return EXA_OK;
+}
+
+extern "C" int exa_fv_grid_enable_timing(exa_fv_grid* g, int enable) {
+  EXA_ARG(g, "null pointer");
+  g->timing = enable != 0;
+  return EXA_OK;
+}
+
+extern "C" int exa_fv_grid_kernel_times(exa_fv_grid* g, double times[4], long long counts[4]) {
+  EXA_ARG(g && times && counts, "null pointer");
+  EXA_CHECK(hipSetDevice(g->s->device));
+  EXA_CHECK(hipStreamSynchronize(g->stream));
+  for (auto& m : g->marks)
+    for (int p = 0; p < 4; ++p) {
+      float ms = 0.f;
+      EXA_CHECK(hipEventElapsedTime(&ms, m[p], m[p + 1]));
+      g->times[p] += ms;
+      g->counts[p] += 1;
+    }
+  fv_clear_marks(g);
+  for (int p = 0; p < 4; ++p) {
+    times[p] = g->times[p];
+    counts[p] = g->counts[p];
+    g->times[p] = 0.0;
+    g->counts[p] = 0;
+  }
   return EXA_OK;
 }
 
@@ -351,6 +392,12 @@ extern "C" int exa_fv_grid_step(exa_fv_grid* g, int nsteps) {
   EXA_CHECK(hipSetDevice(g->s->device));
   const PatchGeom geo = g->s->geom();
   for (int i = 0; i < nsteps; ++i) {
+    hipEvent_t m[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (g->timing) {
+      for (auto& e : m) EXA_CHECK(hipEventCreate(&e));
+      g->marks.emplace_back(m, m + 5);
+      EXA_CHECK(hipEventRecord(m[0], g->stream));
+    }
     if (g->s->cfg.dim == 3)
       hipLaunchKernelGGL(exa::fv::k_fv_ghosts<3>, dim3(g->np), dim3(256), 0, g->stream, g->cur.as<double>(),
                          g->nbr.as<int>(), geo, g->np);
@@ -359,10 +406,15 @@ extern "C" int exa_fv_grid_step(exa_fv_grid* g, int nsteps) {
                          g->nbr.as<int>(), geo, g->np);
     EXA_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_fv_reset, dim3(1), dim3(1), 0, g->stream, g->adm());
+    const bool muscl = g->s->cfg.scheme == EXA_FV_MUSCLHANCOCK;
+    if (m[1]) EXA_CHECK(hipEventRecord(m[1], g->stream));
+    if (m[2] && !muscl) EXA_CHECK(hipEventRecord(m[2], g->stream));  // no states phase
     EXA_TRY(launch_fv_update(g->s, g->stream, g->np, g->cur.as<double>(), g->nxt.as<double>(), g->w.as<double>(),
-                             nullptr, g->adm(), g->dx, g->dt_cur()));
+                             nullptr, g->adm(), g->dx, g->dt_cur(), muscl ? m[2] : nullptr));
+    if (m[3]) EXA_CHECK(hipEventRecord(m[3], g->stream));
     g->cur.swap(g->nxt);
     EXA_TRY(launch_fv_dt(g->s, g->stream, g->np, g->cur.as<double>(), nullptr, g->dt_next(), g->dx));
+    if (m[4]) EXA_CHECK(hipEventRecord(m[4], g->stream));
     hipLaunchKernelGGL(k_fv_dt_advance, dim3(1), dim3(1), 0, g->stream, g->dt_cur(), g->dt_next(), g->time());
     EXA_CHECK(hipGetLastError());
   }

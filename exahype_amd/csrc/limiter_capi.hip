@@ -62,9 +62,10 @@ static int lim_tables(int device, int N, const LimTables** out) {
   return EXA_OK;
 }
 
+// one variable's contraction chain (the kernels loop over the variables)
 static size_t proj_lds_bytes(int B) {
   const int L = 2 * B - 1;
-  return sizeof(double) * exa::lim::V * (size_t)(B * B * L + B * L * L + L * L * L);
+  return sizeof(double) * (size_t)(B * B * L + B * L * L + L * L * L);
 }
 
 struct LimScratch {
@@ -74,8 +75,8 @@ struct LimScratch {
 static LimScratch g_ls[64];
 
 static int check_order(int N) {
-  if (N < 1 || N > 5) {
-    exa_set_error("the subcell limiter supports orders 1..5 (LDS budget of one workgroup)");
+  if (N < 1 || N > EXA_MAX_ORDER) {
+    exa_set_error("the subcell limiter supports orders 1..9");
     return EXA_ERR_UNSUPPORTED;
   }
   return EXA_OK;
@@ -167,7 +168,7 @@ extern "C" int exa_limiter_dmp(int order, const double* luh, int nobs, double re
   EXA_TRY(sc.d.ensure(sizeof(int)));
   EXA_CHECK(hipMemcpy(sc.a.p, hin.data(), sizeof(double) * hin.size(), hipMemcpyHostToDevice));
   EXA_CHECK(hipMemcpy(sc.c.p, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice));
-  const size_t lds = proj_lds_bytes(B) + sizeof(double) * (3 * V * B * B * B + 2 * nobs * 4);
+  const size_t lds = proj_lds_bytes(B) + sizeof(double) * (2 * B * B * B + 2 * 4);
   EXA_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::lim::k_cell_min_max),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(exa::lim::k_cell_min_max, dim3(1), dim3(256), lds, 0, sc.a.as<double>(), sc.b.as<double>(), tab,
@@ -262,7 +263,12 @@ int minmax_all(const ExaGridView& v, const LimTables* tab, double* out, int nobs
     case 2: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<3>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
     case 3: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<4>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
     case 4: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<5>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
-    default: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<6>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
+    case 5: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<6>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
+    case 6: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<7>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
+    case 7: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<8>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
+    case 8: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<9>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
+    case 9: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<10>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
+    default: exa_set_error("order %d", v.order); return EXA_ERR_UNSUPPORTED;
   }
   EXA_CHECK(hipGetLastError());
   return EXA_OK;

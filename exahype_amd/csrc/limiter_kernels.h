@@ -10,7 +10,9 @@
 // operations gives the same matrices bit for bit.
 //
 // Device kernels contract one axis at a time (sum factorisation): one
-// workgroup per cell, the partially contracted tensors in LDS.
+// workgroup per cell, one variable at a time, the partially contracted
+// tensors of that variable in LDS ((B^2 L + B L^2 + L^3) doubles, 99 KB at
+// N = 9).
 #pragma once
 #include <cmath>
 #include <cstring>
@@ -121,23 +123,23 @@ inline bool build_tables(int N, const double* gl_nodes, const double* gl_weights
   return true;
 }
 
-// Contract axis `axis` of a [v][n2][n1][n0] tensor (n0 fastest) with the
-// matrix M[a_in][a_out] (row stride ms): in has extent nin along the axis,
-// out has extent nout.  Threads stride over the output elements.
+// Contract axis `axis` of one variable's [n2][n1][n0] tensor (n0 fastest)
+// with the matrix M[a_in][a_out]: in has extent nin along the axis, out has
+// extent nout.  Threads stride over the output elements.
 template <typename MAT>
 __device__ inline void contract_axis(const double* in, double* out, int n0, int n1, int n2, int axis, int nin,
                                      int nout, MAT mat) {
   const int o0 = axis == 0 ? nout : n0, o1 = axis == 1 ? nout : n1, o2 = axis == 2 ? nout : n2;
-  const int total = V * o0 * o1 * o2;
+  const int total = o0 * o1 * o2;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    const int i0 = e % o0, i1 = (e / o0) % o1, i2 = (e / (o0 * o1)) % o2, v = e / (o0 * o1 * o2);
+    const int i0 = e % o0, i1 = (e / o0) % o1, i2 = e / (o0 * o1);
     double acc = 0.0;
     for (int k = 0; k < nin; ++k) {
       const int j0 = axis == 0 ? k : i0, j1 = axis == 1 ? k : i1, j2 = axis == 2 ? k : i2;
       const int oo = axis == 0 ? i0 : (axis == 1 ? i1 : i2);
-      acc += in[((v * n2 + j2) * n1 + j1) * n0 + j0] * mat(k, oo);
+      acc += in[(j2 * n1 + j1) * n0 + j0] * mat(k, oo);
     }
-    out[((v * o2 + i2) * o1 + i1) * o0 + i0] = acc;
+    out[(i2 * o1 + i1) * o0 + i0] = acc;
   }
 }
 
@@ -148,21 +150,24 @@ __global__ void k_project_on_fv(const double* __restrict__ luh, double* __restri
   extern __shared__ double sm[];
   const int L = 2 * B - 1, S = L + 2 * ghost;
   const int cell = cells ? cells[blockIdx.x] : blockIdx.x;
-  const double* u = luh + (size_t)cell * V * B * B * B;
-  double* a = sm;                      // [v][B][B][L]
-  double* b = sm + V * B * B * L;      // [v][B][L][L]
-  double* c = b + V * B * L * L;       // [v][L][L][L]
+  double* a = sm;              // [B][B][L]
+  double* b = sm + B * B * L;  // [B][L][L]
+  double* c = b + B * L * L;   // [L][L][L]
   auto m = [&](int k, int o) { return tab->u2l[k][o]; };
-  contract_axis(u, a, B, B, B, 0, B, L, m);
-  __syncthreads();
-  contract_axis(a, b, L, B, B, 1, B, L, m);
-  __syncthreads();
-  contract_axis(b, c, L, L, B, 2, B, L, m);
-  __syncthreads();
-  double* o = lim + (size_t)blockIdx.x * V * S * S * S;
-  for (int e = threadIdx.x; e < V * L * L * L; e += blockDim.x) {
-    const int x = e % L, y = (e / L) % L, z = (e / (L * L)) % L, v = e / (L * L * L);
-    o[((v * S + z + ghost) * S + y + ghost) * S + x + ghost] = c[e];
+  for (int v = 0; v < V; ++v) {
+    const double* u = luh + ((size_t)cell * V + v) * B * B * B;
+    contract_axis(u, a, B, B, B, 0, B, L, m);
+    __syncthreads();
+    contract_axis(a, b, L, B, B, 1, B, L, m);
+    __syncthreads();
+    contract_axis(b, c, L, L, B, 2, B, L, m);
+    __syncthreads();
+    double* o = lim + ((size_t)blockIdx.x * V + v) * S * S * S;
+    for (int e = threadIdx.x; e < L * L * L; e += blockDim.x) {
+      const int x = e % L, y = (e / L) % L, z = e / (L * L);
+      o[((z + ghost) * S + y + ghost) * S + x + ghost] = c[e];
+    }
+    __syncthreads();
   }
 }
 
@@ -172,21 +177,24 @@ __global__ void k_project_on_dg(const double* __restrict__ lim, double* __restri
   extern __shared__ double sm[];
   const int L = 2 * B - 1, S = L + 2 * ghost;
   const int cell = cells ? cells[blockIdx.x] : blockIdx.x;
-  const double* in = lim + (size_t)blockIdx.x * V * S * S * S;
-  double* c = sm;                      // [v][L][L][L]
-  double* a = sm + V * L * L * L;      // [v][L][L][B]
-  double* b = a + V * L * L * B;       // [v][L][B][B]
-  for (int e = threadIdx.x; e < V * L * L * L; e += blockDim.x) {
-    const int x = e % L, y = (e / L) % L, z = (e / (L * L)) % L, v = e / (L * L * L);
-    c[e] = in[((v * S + z + ghost) * S + y + ghost) * S + x + ghost];
-  }
-  __syncthreads();
+  double* c = sm;              // [L][L][L]
+  double* a = sm + L * L * L;  // [L][L][B]
+  double* b = a + L * L * B;   // [L][B][B]
   auto m = [&](int k, int o) { return tab->l2u[k][o]; };
-  contract_axis(c, a, L, L, L, 0, L, B, m);
-  __syncthreads();
-  contract_axis(a, b, B, L, L, 1, L, B, m);
-  __syncthreads();
-  contract_axis(b, luh + (size_t)cell * V * B * B * B, B, B, L, 2, L, B, m);
+  for (int v = 0; v < V; ++v) {
+    const double* in = lim + ((size_t)blockIdx.x * V + v) * S * S * S;
+    for (int e = threadIdx.x; e < L * L * L; e += blockDim.x) {
+      const int x = e % L, y = (e / L) % L, z = e / (L * L);
+      c[e] = in[((z + ghost) * S + y + ghost) * S + x + ghost];
+    }
+    __syncthreads();
+    contract_axis(c, a, L, L, L, 0, L, B, m);
+    __syncthreads();
+    contract_axis(a, b, B, L, L, 1, L, B, m);
+    __syncthreads();
+    contract_axis(b, luh + ((size_t)cell * V + v) * B * B * B, B, B, L, 2, L, B, m);
+    __syncthreads();
+  }
 }
 
 // Cell min/max of the observables (the first nobs variables) over the GL
@@ -196,57 +204,54 @@ __global__ void k_project_on_dg(const double* __restrict__ lim, double* __restri
 __global__ void k_cell_min_max(const double* __restrict__ luh, double* __restrict__ out, const LimTables* tab, int B,
                                int nobs, int ncells) {
   extern __shared__ double sm[];
-  const int L = 2 * B - 1;
+  const int L = 2 * B - 1, B3 = B * B * B;
   const int cell = blockIdx.x;
-  const double* u = luh + (size_t)cell * V * B * B * B;
-  double* a = sm;                  // scratch for the two contractions chains
-  double* b = a + V * B * B * L;
-  double* c = b + V * B * L * L;   // [v][L][L][L]
-  double* g1 = c + V * L * L * L;  // [v][B][B][B] lobatto chain
-  double* g2 = g1 + V * B * B * B;
-  double* g3 = g2 + V * B * B * B;
-  double* red = g3 + V * B * B * B;  // 2 * nobs * (blockDim/64)
+  double* a = sm;                // subcell chain: [B][B][L], [B][L][L], [L][L][L]
+  double* b = a + B * B * L;
+  double* c = b + B * L * L;
+  double* g1 = c + L * L * L;    // Lobatto chain: [B][B][B] x 2
+  double* g2 = g1 + B3;
+  double* red = g2 + B3;         // 2 * (blockDim / 64)
   auto ml = [&](int k, int o) { return tab->u2l[k][o]; };
   auto mg = [&](int k, int o) { return tab->u2lob[k][o]; };
-  contract_axis(u, a, B, B, B, 0, B, L, ml);
-  contract_axis(u, g1, B, B, B, 0, B, B, mg);
-  __syncthreads();
-  contract_axis(a, b, L, B, B, 1, B, L, ml);
-  contract_axis(g1, g2, B, B, B, 1, B, B, mg);
-  __syncthreads();
-  contract_axis(b, c, L, L, B, 2, B, L, ml);
-  contract_axis(g2, g3, B, B, B, 2, B, B, mg);
-  __syncthreads();
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
   for (int v = 0; v < nobs; ++v) {
+    const double* u = luh + ((size_t)cell * V + v) * B3;
+    contract_axis(u, a, B, B, B, 0, B, L, ml);
+    contract_axis(u, g1, B, B, B, 0, B, B, mg);
+    __syncthreads();
+    contract_axis(a, b, L, B, B, 1, B, L, ml);
+    contract_axis(g1, g2, B, B, B, 1, B, B, mg);
+    __syncthreads();
+    contract_axis(b, c, L, L, B, 2, B, L, ml);
+    contract_axis(g2, g1, B, B, B, 2, B, B, mg);
+    __syncthreads();
     double mn = 1.0e300, mx = -1.0e300;
-    for (int e = threadIdx.x; e < B * B * B; e += blockDim.x) {
-      const double p = u[v * B * B * B + e], q = g3[v * B * B * B + e];
+    for (int e = threadIdx.x; e < B3; e += blockDim.x) {
+      const double p = u[e], q = g1[e];
       mn = fmin(mn, fmin(p, q));
       mx = fmax(mx, fmax(p, q));
     }
     for (int e = threadIdx.x; e < L * L * L; e += blockDim.x) {
-      const double q = c[v * L * L * L + e];
-      mn = fmin(mn, q);
-      mx = fmax(mx, q);
+      mn = fmin(mn, c[e]);
+      mx = fmax(mx, c[e]);
     }
     mn = wave_min(mn);
     mx = -wave_min(-mx);
     if (lane == 0) {
-      red[(2 * v) * nw + wid] = mn;
-      red[(2 * v + 1) * nw + wid] = mx;
+      red[wid] = mn;
+      red[nw + wid] = mx;
     }
-  }
-  __syncthreads();
-  if (threadIdx.x < nobs) {
-    const int v = threadIdx.x;
-    double mn = red[(2 * v) * nw], mx = red[(2 * v + 1) * nw];
-    for (int w = 1; w < nw; ++w) {
-      mn = fmin(mn, red[(2 * v) * nw + w]);
-      mx = fmax(mx, red[(2 * v + 1) * nw + w]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < nw; ++w) {
+        mn = fmin(mn, red[w]);
+        mx = fmax(mx, red[nw + w]);
+      }
+      out[((size_t)cell * 2 + 0) * nobs + v] = mn;
+      out[((size_t)cell * 2 + 1) * nobs + v] = mx;
     }
-    out[((size_t)cell * 2 + 0) * nobs + v] = mn;
-    out[((size_t)cell * 2 + 1) * nobs + v] = mx;
+    __syncthreads();
   }
 }
 

@@ -1,17 +1,31 @@
-// Predictor kernels of orders 8 and 9 (iterate staged in global memory).
+// Predictor kernels with the space-time iterate staged in global memory:
+// the only form of orders 8 and 9, selectable (EXA_CONTRACT_STAGED) at 5..7.
 #include "aderdg_kernels.h"
 #include "predictor_tu.h"
 
+namespace {
+template <int N>
+StagedFn pick(bool debug) {
+  return debug ? &exa::k_predictor_staged<N, true> : &exa::k_predictor_staged<N, false>;
+}
+}  // namespace
+
 StagedFn exa_pred_staged_kernel(int order, bool debug) {
   switch (order) {
-    case 8: return debug ? &exa::k_predictor_staged<8, true> : &exa::k_predictor_staged<8, false>;
-    case 9: return debug ? &exa::k_predictor_staged<9, true> : &exa::k_predictor_staged<9, false>;
+    case 5: return pick<5>(debug);
+    case 6: return pick<6>(debug);
+    case 7: return pick<7>(debug);
+    case 8: return pick<8>(debug);
+    case 9: return pick<9>(debug);
     default: return nullptr;
   }
 }
 
 int exa_pred_staged_lds_bytes(int order) {
   switch (order) {
+    case 5: return exa::Ord<5>::LDS_STAGED * (int)sizeof(double);
+    case 6: return exa::Ord<6>::LDS_STAGED * (int)sizeof(double);
+    case 7: return exa::Ord<7>::LDS_STAGED * (int)sizeof(double);
     case 8: return exa::Ord<8>::LDS_STAGED * (int)sizeof(double);
     case 9: return exa::Ord<9>::LDS_STAGED * (int)sizeof(double);
     default: return 0;
@@ -23,7 +37,7 @@ hipError_t exa_pred_staged_init(const DGTables* tab, const DGTablesF* tabf) {
   if (e != hipSuccess) return e;
   e = hipMemcpyToSymbol(HIP_SYMBOL(c_tabf), tabf, sizeof(DGTablesF) * (EXA_MAX_ORDER + 1));
   if (e != hipSuccess) return e;
-  for (int order = 8; order <= 9; ++order)
+  for (int order = EXA_STAGED_MIN_ORDER; order <= EXA_MAX_ORDER; ++order)
     for (int dbg = 0; dbg < 2; ++dbg) {
       e = hipFuncSetAttribute(reinterpret_cast<const void*>(exa_pred_staged_kernel(order, dbg)),
                               hipFuncAttributeMaxDynamicSharedMemorySize, exa_pred_staged_lds_bytes(order));

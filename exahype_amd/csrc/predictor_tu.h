@@ -30,9 +30,11 @@ EXA_PRED_TU_DECL(6)
 EXA_PRED_TU_DECL(7)
 #undef EXA_PRED_TU_DECL
 
-// Orders 8 and 9 (pred_staged.hip): the space-time iterate lives in a
-// staging slot of global memory per workgroup (2 B^4 V doubles), the grid is
-// persistent (one workgroup per slot, cells strided by the grid size).
+// Staged predictor (pred_staged.hip; orders 5..9, the only form of 8 and 9):
+// the space-time iterate lives in a staging slot of global memory per
+// workgroup (2 B^4 V doubles), the grid is persistent (one workgroup per
+// slot, cells strided by the grid size).
+#define EXA_STAGED_MIN_ORDER 5
 using StagedFn = void (*)(const double*, double*, double*, double*, int*, const double*, double, double, double,
                           double, long long, double*, unsigned long long*, exa::PredictorDebug);
 StagedFn exa_pred_staged_kernel(int order, bool debug);

@@ -104,6 +104,8 @@ typedef struct {
 #define EXA_CONTRACT_PENCIL 1   /* pencil owners contract, two barriers per time slice    */
 #define EXA_CONTRACT_NODE 2     /* node owners contract their pencils, one barrier per slice */
 #define EXA_CONTRACT_PENCIL_PAIR 3 /* pencil owners, two slices per phase, contraction in place */
+#define EXA_CONTRACT_STAGED 4   /* space-time iterate staged in global memory (FP64 only,
+                                   orders >= 5; the only form of orders 8 and 9)           */
 
 int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out);
 
@@ -304,6 +306,13 @@ int exa_fv_grid_step(exa_fv_grid* g, int nsteps);
 /* simulated time, dt of the next step, min over the last step's patches of
  * the kernels' admissible dt */
 int exa_fv_grid_sync(exa_fv_grid* g, double* time, double* next_dt, double* admissible_dt);
+/* Per-phase kernel times of the steps taken with timing enabled (HIP events
+ * on the grid's stream), summed since the previous call, which resets them:
+ * k = 0 ghost layers, 1 MUSCL-Hancock slopes and half-step states (0 for
+ * Godunov), 2 update (Riemann + accumulation), 3 CFL time step.  No
+ * counterpart in the reference (measurement). */
+int exa_fv_grid_enable_timing(exa_fv_grid* g, int enable);
+int exa_fv_grid_kernel_times(exa_fv_grid* g, double times[4], long long counts[4]);
 
 /* ---------------------------------------------------------------------- */
 /* Subcell limiter (orders 1..5, 2N+1 subcells per direction, 5 variables).
@@ -404,6 +413,9 @@ int exa_amr_prolongation_batched(int dim, int order, int ndata, double* d_fine, 
 int exa_amr_restriction_batched(int dim, int order, int ndata, double* d_coarse, long long n_coarse,
                                 const double* d_fine, long long n_fine, const exa_amr_transfer* transfers,
                                 int count);
+/* Duration (HIP events) of the transfer kernel of the last batched call on
+ * the current device.  No counterpart in the reference (measurement). */
+int exa_amr_last_kernel_ms(double* ms);
 
 #ifdef __cplusplus
 }

@@ -139,6 +139,44 @@ def test_staged_predictor_vs_oracle(N):
             assert rel_err(g[k], o[k]) <= TOL, (N, k, rel_err(g[k], o[k]))
 
 
+@pytest.mark.parametrize("N", [5, 6, 7])
+def test_staged_form_at_lower_orders(N):
+    """The staged predictor selected explicitly (EXA_CONTRACT_STAGED) at orders
+    that also have register-resident forms: per-cell outputs and a grid step."""
+    E = exa()
+    s = E.ADERDGSolver(N, contraction=E.CONTRACT_STAGED)
+    c = O.cfg(N)
+    luh = smooth_cell_state(N, 3)
+    dx = np.array([1 / 27, 1 / 27, 1 / 27])
+    dt = 0.9 * O.stable_time_step_size(c, luh, dx)
+    g = s.spaceTimePredictor(luh, dx, dt)
+    o = O.space_time_predictor(c, luh, dx, dt)
+    assert g["iters"] == o["iters"]
+    for k in ("lQi", "lQhi", "lQhbnd", "lFhbnd", "lFhi", "lFi"):
+        assert rel_err(g[k], o[k]) <= TOL, (N, k, rel_err(g[k], o[k]))
+    dims = (4, 3, 3)
+    og = O.OracleGrid(c, *dims)
+    og.init(O.make_ic(0, **_ic_kw(5)))
+    gr = E.Grid(s, *dims)
+    gr.upload(og.luh().copy())
+    dt_o = og.stable_dt()
+    gr.compute_dt()
+    for k in range(2):
+        gr.step(1)
+        dt_o = og.step(dt_o)
+        gr.sync()
+        assert np.array_equal(gr.picard_iterations(), og.iters()), k
+    assert rel_err(gr.download(), og.luh()) <= TOL
+
+
+def test_staged_form_rejects_low_orders_and_ensembles():
+    E = exa()
+    with pytest.raises(RuntimeError, match="staged"):
+        E.ADERDGSolver(3, contraction=E.CONTRACT_STAGED)
+    with pytest.raises(RuntimeError, match="staged"):
+        E.ADERDGSolver(5, contraction=E.CONTRACT_STAGED, precision=E.PREC_FP32)
+
+
 def test_staged_orders_reject_mixed_precision():
     E = exa()
     with pytest.raises(RuntimeError, match="FP64"):

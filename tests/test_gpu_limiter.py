@@ -1,5 +1,5 @@
 """GPU parity of the subcell limiter through the C ABI: the reference's golden
-vectors (order 3) and the oracle (orders 1..5) for both projections, the cell
+vectors (order 3) and the oracle (orders 1..9) for both projections, the cell
 min/max and the DMP decision.  Bar: <= 1e-12 relative (sum-factorised
 contractions on the GPU vs the oracle's direct triple sums)."""
 import json
@@ -28,7 +28,7 @@ def test_golden_projections_gpu():
     assert rel(luh, g["testToLuhConversion/luh_out"]) <= 1e-12
 
 
-@pytest.mark.parametrize("N", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7, 8, 9])
 @pytest.mark.parametrize("ghost", [0, 2])
 def test_projections_vs_oracle(N, ghost):
     import exahype_amd as E
@@ -42,7 +42,7 @@ def test_projections_vs_oracle(N, ghost):
     assert rel(E.projectOnDGSpace(lim, N, ghost), O.project_on_dg(N, lim, ghost)) <= 1e-12
 
 
-@pytest.mark.parametrize("N", [2, 3, 5])
+@pytest.mark.parametrize("N", [2, 3, 5, 7, 9])
 def test_dmp_vs_oracle(N):
     import exahype_amd as E
     rng = np.random.default_rng(N)

@@ -42,7 +42,7 @@ def test_golden_lobatto_values(lg):
     assert np.abs(lob - exp).max() <= 1e-12 * np.abs(exp).max()
 
 
-@pytest.mark.parametrize("N", [1, 2, 3, 5, 7])
+@pytest.mark.parametrize("N", [1, 2, 3, 5, 7, 9])
 def test_projection_properties(N):
     B, L = N + 1, 2 * N + 1
     u2l, u2lob, l2u = O.limiter_matrices(N)
