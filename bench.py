@@ -329,10 +329,10 @@ def run_muscl(args):
     """3D MUSCL-Hancock finite volumes (kernels/finitevolumes/musclhancock/c/3d/
     musclhancock.cpph) on the limiter's subcell grid of the N=5 Sod sphere: 27^3
     patches of 11^3 volumes with 2 ghost layers (297^3 volumes), periodic,
-    20 steps.  Roofline: HBM bytes of the MUSCL-Hancock update (slopes + half-
-    step states kernel, then the Riemann/update kernel), counted as the
-    compulsory read of every patch (ghosts included) and write of its interior,
-    over the two kernels' HIP-event time."""
+    20 steps.  Roofline: HBM bytes of the MUSCL-Hancock kernel (slopes, half
+    step and Riemann fluxes in one launch), counted as the compulsory read of
+    every patch (ghosts included) and write of its interior, over the kernel's
+    HIP-event time."""
     import exahype_amd as E
     from oracle import oracle as O
     n, patch, steps = 27, 11, args.steps if args.steps != 10 else 20
@@ -363,10 +363,10 @@ def run_muscl(args):
            "config": {"workload": "3D Euler MUSCL-Hancock + Rusanov, minmod slopes, CFL 0.99, 27^3 patches of 11^3 "
                                   "volumes (2 ghost layers), periodic, Sod sphere r0=0.25", "volumes": vols,
                       "t_end": t_end},
-           "phase_ms_per_step": {"ghosts": ms[0], "states": ms[1], "update": ms[2], "dt": ms[3]},
+           "phase_ms_per_step": {"ghosts": ms[0], "update": ms[2], "dt": ms[3]},
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                         "frac": achieved / PEAK_HBM_GBPS, "traffic": None,
-                        "kernel": "k_muscl_states + k_muscl_update", "avg_launch_ms": upd_ms,
+                        "kernel": "k_muscl", "avg_launch_ms": upd_ms,
                         "bytes_per_launch": bytes_upd}}
     if not args.no_cpu_baseline:
         threads = min(len(os.sched_getaffinity(0)), 16)

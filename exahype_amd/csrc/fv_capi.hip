@@ -18,7 +18,7 @@ struct exa_fv_solver {
   int ghost = 1;
   hipStream_t stream = nullptr;
   std::mutex mu;
-  DevBuf a, b, w, st;
+  DevBuf a, b, st;
   PatchGeom geom() const {
     PatchGeom g;
     g.P = cfg.patch;
@@ -29,12 +29,17 @@ struct exa_fv_solver {
   }
 };
 
+// LDS of the MUSCL-Hancock kernel: the half-step term of every interior subcell.
+static size_t muscl_lds_bytes(const PatchGeom& g) { return sizeof(double) * FV_V * (size_t)g.P * g.P * g.P; }
+
 extern "C" int exa_fv_create(const exa_fv_config* cfg, exa_fv_solver** out) {
   EXA_ARG(cfg && out, "null pointer");
   EXA_ARG(cfg->dim == 2 || cfg->dim == 3, "dim must be 2 or 3");
   EXA_ARG(cfg->patch >= 2 && cfg->patch <= 32, "patch size 2..32");
   EXA_ARG(cfg->scheme == EXA_FV_GODUNOV || (cfg->scheme == EXA_FV_MUSCLHANCOCK && cfg->dim == 3),
           "scheme: Godunov (2D/3D) or MUSCL-Hancock (3D)");
+  EXA_ARG(cfg->scheme != EXA_FV_MUSCLHANCOCK || cfg->patch <= 15,
+          "MUSCL-Hancock patches up to 15^3 (the half-step terms of a patch are held in LDS)");
   EXA_ARG(cfg->nvar == 5 && cfg->npar == 0, "5 variables, no parameters");
   EXA_ARG((cfg->pde == EXA_PDE_EULER && cfg->dim == 3) || (cfg->pde == EXA_PDE_EULERFV2D && cfg->dim == 2),
           "pde: EXA_PDE_EULER in 3D, EXA_PDE_EULERFV2D in 2D");
@@ -46,7 +51,11 @@ extern "C" int exa_fv_create(const exa_fv_config* cfg, exa_fv_solver** out) {
   s->cfg = *cfg;
   s->device = dev;
   s->ghost = cfg->scheme == EXA_FV_MUSCLHANCOCK ? 2 : 1;
-  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  hipError_t e = hipSuccess;
+  if (cfg->scheme == EXA_FV_MUSCLHANCOCK)
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::fv::k_muscl<EulerPDE>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)muscl_lds_bytes(s->geom()));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete s;
     exa_set_error("hipStreamCreate: %s", hipGetErrorString(e));
@@ -74,19 +83,14 @@ static void patch_soa_to_ref(const double* soa, double* ref, int nbox) {
 }
 
 // Launches the update of `np` patches: in -> out (out interior overwritten).
-// mid: recorded between the MUSCL-Hancock states and update kernels (timing).
 static int launch_fv_update(const exa_fv_solver* s, hipStream_t st, int np, const double* in, double* out,
-                            double* W, double* dt_patch, unsigned long long* dt_min, const double dx[3],
-                            const double* dt, hipEvent_t mid = nullptr) {
+                            double* dt_patch, unsigned long long* dt_min, const double dx[3], const double* dt) {
   const PatchGeom g = s->geom();
   const double gam = s->cfg.gamma;
   if (s->cfg.scheme == EXA_FV_MUSCLHANCOCK) {
-    hipLaunchKernelGGL((exa::fv::k_muscl_states<EulerPDE>), dim3(np), dim3(256), 0, st, in, W, g, dx[0], dx[1],
-                       dx[2], dt, gam);
-    if (mid) EXA_CHECK(hipEventRecord(mid, st));
     // the kernel's own CFL (kernels/finitevolumes/musclhancock/c/musclhancock.h:29)
-    hipLaunchKernelGGL((exa::fv::k_muscl_update<EulerPDE>), dim3(np), dim3(256), 0, st, in, W, out, dt_patch, dt_min,
-                       g, dx[0], dx[1], dx[2], dt, gam, 0.99);
+    hipLaunchKernelGGL((exa::fv::k_muscl<EulerPDE>), dim3(np), dim3(256), muscl_lds_bytes(g), st, in, out, dt_patch,
+                       dt_min, g, dx[0], dx[1], dx[2], dt, gam, 0.99);
   } else if (s->cfg.dim == 3) {
     hipLaunchKernelGGL((exa::fv::k_godunov<3, EulerPDE>), dim3(np), dim3(256), 0, st, in, out, dt_patch, dt_min, g,
                        dx[0], dx[1], dx[2], dt, gam, s->cfg.cfl);
@@ -120,15 +124,14 @@ extern "C" int exa_fv_solution_update(exa_fv_solver* s, double* luh_new, const d
   std::vector<double> h(nbox * FV_V);
   EXA_TRY(s->a.ensure(sizeof(double) * nbox * FV_V));
   EXA_TRY(s->b.ensure(sizeof(double) * nbox * FV_V));
-  EXA_TRY(s->w.ensure(sizeof(double) * nbox * FV_V * 6));
   EXA_TRY(s->st.ensure(sizeof(double) * 2));
   patch_ref_to_soa(luh, h.data(), nbox);
   EXA_CHECK(hipMemcpy(s->a.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
   patch_ref_to_soa(luh_new, h.data(), nbox);
   EXA_CHECK(hipMemcpy(s->b.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
   EXA_CHECK(hipMemcpy(s->st.as<double>() + 1, &dt, sizeof(double), hipMemcpyHostToDevice));
-  EXA_TRY(launch_fv_update(s, s->stream, 1, s->a.as<double>(), s->b.as<double>(), s->w.as<double>(),
-                           s->st.as<double>(), nullptr, dx, s->st.as<double>() + 1));
+  EXA_TRY(launch_fv_update(s, s->stream, 1, s->a.as<double>(), s->b.as<double>(), s->st.as<double>(), nullptr, dx,
+                           s->st.as<double>() + 1));
   EXA_CHECK(hipStreamSynchronize(s->stream));
   EXA_CHECK(hipMemcpy(h.data(), s->b.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
   patch_soa_to_ref(h.data(), luh_new, nbox);
@@ -195,9 +198,9 @@ struct exa_fv_grid {
   exa_fv_solver* s = nullptr;
   int nx = 1, ny = 1, nz = 1, np = 0;
   double off[3] = {0, 0, 0}, width[3] = {1, 1, 1}, dx[3] = {1, 1, 1};
-  DevBuf cur, nxt, w, nbr, state;
+  DevBuf cur, nxt, nbr, state;
   hipStream_t stream = nullptr;
-  // per-phase kernel timing (0 ghost layers, 1 MUSCL-Hancock states, 2 update, 3 dt)
+  // per-phase kernel timing (0 ghost layers, 1 unused, 2 update, 3 dt)
   bool timing = false;
   std::vector<std::vector<hipEvent_t>> marks;  // 5 events per timed step
   double times[4] = {0, 0, 0, 0};
@@ -250,8 +253,7 @@ extern "C" int exa_fv_grid_create(exa_fv_solver* s, int nx, int ny, int nz, cons
   int r = EXA_OK;
   if ((r = g->cur.ensure(sizeof(double) * psz * g->np)) != EXA_OK ||
       (r = g->nxt.ensure(sizeof(double) * psz * g->np)) != EXA_OK ||
-      (r = g->nbr.ensure(sizeof(int) * 6 * g->np)) != EXA_OK || (r = g->state.ensure(sizeof(double) * 8)) != EXA_OK ||
-      (s->cfg.scheme == EXA_FV_MUSCLHANCOCK && (r = g->w.ensure(sizeof(double) * psz * 6 * g->np)) != EXA_OK)) {
+      (r = g->nbr.ensure(sizeof(int) * 6 * g->np)) != EXA_OK || (r = g->state.ensure(sizeof(double) * 8)) != EXA_OK) {
     delete g;
     return r;
   }
@@ -406,11 +408,10 @@ extern "C" int exa_fv_grid_step(exa_fv_grid* g, int nsteps) {
                          g->nbr.as<int>(), geo, g->np);
     EXA_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_fv_reset, dim3(1), dim3(1), 0, g->stream, g->adm());
-    const bool muscl = g->s->cfg.scheme == EXA_FV_MUSCLHANCOCK;
     if (m[1]) EXA_CHECK(hipEventRecord(m[1], g->stream));
-    if (m[2] && !muscl) EXA_CHECK(hipEventRecord(m[2], g->stream));  // no states phase
-    EXA_TRY(launch_fv_update(g->s, g->stream, g->np, g->cur.as<double>(), g->nxt.as<double>(), g->w.as<double>(),
-                             nullptr, g->adm(), g->dx, g->dt_cur(), muscl ? m[2] : nullptr));
+    if (m[2]) EXA_CHECK(hipEventRecord(m[2], g->stream));  // phase 1 is empty (one update kernel)
+    EXA_TRY(launch_fv_update(g->s, g->stream, g->np, g->cur.as<double>(), g->nxt.as<double>(), nullptr, g->adm(),
+                             g->dx, g->dt_cur()));
     if (m[3]) EXA_CHECK(hipEventRecord(m[3], g->stream));
     g->cur.swap(g->nxt);
     EXA_TRY(launch_fv_dt(g->s, g->stream, g->np, g->cur.as<double>(), nullptr, g->dt_next(), g->dx));

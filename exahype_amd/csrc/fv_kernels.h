@@ -148,96 +148,85 @@ __device__ inline double minmod(double a, double b) {
   return fabs(a) < fabs(b) ? a : b;
 }
 
-// MUSCL-Hancock, phase 1 (musclhancock.cpph:62-213): minmod slopes, boundary
-// extrapolated states w, and for interior cells the half-step evolution with
-// the fluxes of the cell's own states.  The first ghost layer gets its
-// extrapolated states in the normal direction only and is not evolved.
-// W: [patch][6][v][cell], states (L,R) per direction x,y,z.
-template <class PDE>
-__global__ __launch_bounds__(256) void k_muscl_states(const double* __restrict__ luh, double* __restrict__ W,
-                                                      PatchGeom g, double dx0, double dx1, double dx2,
-                                                      const double* __restrict__ dtp, double gamma) {
-  const double dt = *dtp;
-  const int nbox = g.ncells_box();
-  const double* u = luh + (size_t)blockIdx.x * V * nbox;
-  double* w = W + (size_t)blockIdx.x * 6 * V * nbox;
-  const double h[3] = {dx0 / g.P, dx1 / g.P, dx2 / g.P};
+// MUSCL-Hancock (kernels/finitevolumes/musclhancock/c/3d/musclhancock.cpph:
+// 27-309), one workgroup per patch, one launch:
+//   phase 1  every interior subcell evaluates its half-step evolution term
+//            Qt = sum_d (F_d(w_L,d) - F_d(w_R,d)) / h_d from its minmod slopes
+//            and boundary-extrapolated states (:62-213) into LDS;
+//   phase 2  every interior subcell rebuilds the evolved states of the faces it
+//            touches -- its own and its neighbours' w = q -+ h/2 slope, plus
+//            dt/2 Qt where the owner is interior -- and applies the Rusanov
+//            fluxes on (w_R[i], w_L[i+1]) (:215-281); the first ghost layer
+//            receives the patch-boundary face updates as in the reference.
+// The states are recomputed from the patch values instead of being stored
+// (six state arrays per patch would be 6x the patch's HBM traffic); they are
+// the same operations on the same inputs, hence the same values.
+__device__ inline void muscl_face_state(const double* u, int nbox, const PatchGeom& g, const double* sQt, int idx,
+                                        int d, int side, double h, double dt, double* w) {
+  double q[V], qm[V], qp[V];
+  load_q(u, nbox, idx, q);
+  load_q(u, nbox, idx - g.stride(d), qm);
+  load_q(u, nbox, idx + g.stride(d), qp);
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    const double slope = minmod(qp[v] - q[v], q[v] - qm[v]) / h;
+    w[v] = side ? q[v] + 0.5 * h * slope : q[v] - 0.5 * h * slope;
+  }
+  const int i = idx % g.S, j = (idx / g.S) % g.S, k = idx / (g.S * g.S);
   const int b = g.G, e = g.G + g.P;
-  for (int c = threadIdx.x; c < nbox; c += blockDim.x) {
-    const int i = c % g.S, j = (c / g.S) % g.S, k = c / (g.S * g.S);
-    const bool in[3] = {i >= b && i < e, j >= b && j < e, k >= b && k < e};
-    const bool ghost[3] = {i == b - 1 || i == e, j == b - 1 || j == e, k == b - 1 || k == e};
-    const bool interior = in[0] && in[1] && in[2];
-    bool want[3];
-    want[0] = interior || (ghost[0] && in[1] && in[2]);
-    want[1] = interior || (ghost[1] && in[0] && in[2]);
-    want[2] = interior || (ghost[2] && in[0] && in[1]);
-    if (!(want[0] || want[1] || want[2])) continue;
-    double q[V];
-    load_q(u, nbox, c, q);
-    double wL[3][V], wR[3][V];
+  if (i >= b && i < e && j >= b && j < e && k >= b && k < e) {
+    const int ci = ((k - b) * g.P + (j - b)) * g.P + (i - b), ni = g.P * g.P * g.P;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      if (!want[d]) continue;
-      double qm[V], qp[V];
-      load_q(u, nbox, c - g.stride(d), qm);
-      load_q(u, nbox, c + g.stride(d), qp);
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        const double slope = minmod(qp[v] - q[v], q[v] - qm[v]) / h[d];
-        wL[d][v] = q[v] - 0.5 * h[d] * slope;
-        wR[d][v] = q[v] + 0.5 * h[d] * slope;
-      }
-    }
-    if (interior) {
-      double Qt[V], F[V];
-#pragma unroll
-      for (int v = 0; v < V; ++v) Qt[v] = 0.0;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        PDE::flux_dir(wL[d], d, F, gamma);
-#pragma unroll
-        for (int v = 0; v < V; ++v) Qt[v] += F[v] / h[d];
-        PDE::flux_dir(wR[d], d, F, gamma);
-#pragma unroll
-        for (int v = 0; v < V; ++v) Qt[v] -= F[v] / h[d];
-      }
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-          wL[d][v] += 0.5 * dt * Qt[v];
-          wR[d][v] += 0.5 * dt * Qt[v];
-        }
-    }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      if (!want[d]) continue;
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        w[((2 * d) * V + v) * nbox + c] = wL[d][v];
-        w[((2 * d + 1) * V + v) * nbox + c] = wR[d][v];
-      }
-    }
+    for (int v = 0; v < V; ++v) w[v] += 0.5 * dt * sQt[v * ni + ci];
   }
 }
 
-// MUSCL-Hancock, phase 2 (musclhancock.cpph:215-281): Rusanov on
-// (w_R[i], w_L[i+1]) at every face, applied per interior cell.
 template <class PDE>
-__global__ __launch_bounds__(256) void k_muscl_update(const double* __restrict__ luh, const double* __restrict__ W,
-                                                      double* __restrict__ luh_new, double* __restrict__ dt_patch,
-                                                      unsigned long long* dt_min, PatchGeom g, double dx0, double dx1,
-                                                      double dx2, const double* __restrict__ dtp, double gamma,
-                                                      double cfl) {
+__global__ __launch_bounds__(256) void k_muscl(const double* __restrict__ luh, double* __restrict__ luh_new,
+                                               double* __restrict__ dt_patch, unsigned long long* dt_min,
+                                               PatchGeom g, double dx0, double dx1, double dx2,
+                                               const double* __restrict__ dtp, double gamma, double cfl) {
+  extern __shared__ double sQt[];  // [v][interior subcell]
   __shared__ double red[8];
   const double dt = *dtp;
   const int nbox = g.ncells_box();
   const double* u = luh + (size_t)blockIdx.x * V * nbox;
-  const double* w = W + (size_t)blockIdx.x * 6 * V * nbox;
   double* un = luh_new + (size_t)blockIdx.x * V * nbox;
   const double h[3] = {dx0 / g.P, dx1 / g.P, dx2 / g.P};
   const int ninterior = g.P * g.P * g.P;
+  // phase 1: half-step evolution term of every interior subcell
+  for (int c = threadIdx.x; c < ninterior; c += blockDim.x) {
+    int i, j, k;
+    interior_coords(g, c, i, j, k);
+    const int idx = box_index(g, i, j, k);
+    double q[V];
+    load_q(u, nbox, idx, q);
+    double Qt[V], F[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) Qt[v] = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double qm[V], qp[V], wL[V], wR[V];
+      load_q(u, nbox, idx - g.stride(d), qm);
+      load_q(u, nbox, idx + g.stride(d), qp);
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const double slope = minmod(qp[v] - q[v], q[v] - qm[v]) / h[d];
+        wL[v] = q[v] - 0.5 * h[d] * slope;
+        wR[v] = q[v] + 0.5 * h[d] * slope;
+      }
+      PDE::flux_dir(wL, d, F, gamma);
+#pragma unroll
+      for (int v = 0; v < V; ++v) Qt[v] += F[v] / h[d];
+      PDE::flux_dir(wR, d, F, gamma);
+#pragma unroll
+      for (int v = 0; v < V; ++v) Qt[v] -= F[v] / h[d];
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) sQt[v * ninterior + c] = Qt[v];
+  }
+  __syncthreads();
+  // phase 2: Rusanov fluxes of the evolved face states
   double dtm = 1.0e300;
   for (int c = threadIdx.x; c < ninterior; c += blockDim.x) {
     int i, j, k;
@@ -248,17 +237,15 @@ __global__ __launch_bounds__(256) void k_muscl_update(const double* __restrict__
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const double fac = dt / h[d];
-      const double* wLd = w + (2 * d) * V * nbox;
-      const double* wRd = w + (2 * d + 1) * V * nbox;
-      load_q(wRd, nbox, idx - g.stride(d), a);
-      load_q(wLd, nbox, idx, b);
+      muscl_face_state(u, nbox, g, sQt, idx - g.stride(d), d, 1, h[d], dt, a);
+      muscl_face_state(u, nbox, g, sQt, idx, d, 0, h[d], dt, b);
       double s = rusanov<PDE>(a, b, d, gamma, f);
       double cand = cfl / 3 * h[d] / s;
       dtm = cand < dtm ? cand : dtm;
 #pragma unroll
       for (int v = 0; v < V; ++v) acc[v] += fac * f[v];
-      load_q(wRd, nbox, idx, a);
-      load_q(wLd, nbox, idx + g.stride(d), b);
+      muscl_face_state(u, nbox, g, sQt, idx, d, 1, h[d], dt, a);
+      muscl_face_state(u, nbox, g, sQt, idx + g.stride(d), d, 0, h[d], dt, b);
       s = rusanov<PDE>(a, b, d, gamma, f);
       cand = cfl / 3 * h[d] / s;
       dtm = cand < dtm ? cand : dtm;
@@ -273,10 +260,8 @@ __global__ __launch_bounds__(256) void k_muscl_update(const double* __restrict__
     int d, side, idx, inner;
     ghost_layer_cell(g, it, d, side, idx, inner);
     double a[V], b[V], f[V];
-    const double* wLd = w + (2 * d) * V * nbox;
-    const double* wRd = w + (2 * d + 1) * V * nbox;
-    load_q(wRd, nbox, side ? inner : idx, a);
-    load_q(wLd, nbox, side ? idx : inner, b);
+    muscl_face_state(u, nbox, g, sQt, side ? inner : idx, d, 1, h[d], dt, a);
+    muscl_face_state(u, nbox, g, sQt, side ? idx : inner, d, 0, h[d], dt, b);
     rusanov<PDE>(a, b, d, gamma, f);
     const double fac = dt / h[d];
 #pragma unroll

@@ -308,8 +308,8 @@ int exa_fv_grid_step(exa_fv_grid* g, int nsteps);
 int exa_fv_grid_sync(exa_fv_grid* g, double* time, double* next_dt, double* admissible_dt);
 /* Per-phase kernel times of the steps taken with timing enabled (HIP events
  * on the grid's stream), summed since the previous call, which resets them:
- * k = 0 ghost layers, 1 MUSCL-Hancock slopes and half-step states (0 for
- * Godunov), 2 update (Riemann + accumulation), 3 CFL time step.  No
+ * k = 0 ghost layers, 1 (reserved, 0), 2 update (Godunov, or MUSCL-Hancock
+ * slopes + half step + Riemann in one kernel), 3 CFL time step.  No
  * counterpart in the reference (measurement). */
 int exa_fv_grid_enable_timing(exa_fv_grid* g, int enable);
 int exa_fv_grid_kernel_times(exa_fv_grid* g, double times[4], long long counts[4]);
