@@ -371,7 +371,7 @@ def run_muscl(args):
     if not args.no_cpu_baseline:
         threads = min(len(os.sched_getaffinity(0)), 16)
         c = O.cfg(1)
-        ns, st = 9, 3
+        ns, st = 27, 8
         og = O.OracleFVGrid(c, 3, 1, patch, ns, ns, ns)
         og.init(sod_ic(O))
         dt = og.stable_dt(threads)
@@ -381,8 +381,8 @@ def run_muscl(args):
         cwall = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": (ns * patch) ** 3 * NVAR * st / cwall, "unit": "DoF-updates/s",
                                "cores": threads, "kind": "port",
-                               "sample": "oracle (C, OpenMP), 9^3 patches of 11^3 volumes, Sod sphere, %d steps "
-                                         "(%.1f s)" % (st, cwall)}
+                               "sample": "oracle (C, OpenMP), the same 27^3 patches of 11^3 volumes, Sod sphere, "
+                                         "%d steps (%.1f s)" % (st, cwall)}
     return out
 
 
@@ -440,7 +440,7 @@ def run_amr(args):
                         "avg_launch_ms": res[dom]["kernel_ms"]}}
     if not args.no_cpu_baseline:
         # the oracle's per-cell transfers, one thread, on a sample of the list
-        cnt = 2000
+        cnt = 50000
         ch = coarse_h.reshape(nc, blk)
         t0 = time.perf_counter()
         for k in range(cnt):

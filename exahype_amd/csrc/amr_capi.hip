@@ -23,12 +23,25 @@ struct AmrCache {
   bool ready[EXA_MAX_ORDER + 1] = {};
   DevBuf items, groups, a, b, c, d;  // staging of the per-call and batched forms
   hipEvent_t e0 = nullptr, e1 = nullptr;  // around the last batched launch
+  bool lds_set = false;
   float last_ms = 0.f;
 };
 AmrCache g_amr[64];
 
 int amr_tables(int dev, int N, const AmrTables** out) {
   AmrCache& c = g_amr[dev];
+  if (!c.lds_set) {
+    const int bytes = 2 * exa::amr::AMR_LDS_DOUBLES * (int)sizeof(double);
+    EXA_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::amr::k_amr_transfer<3, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    EXA_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::amr::k_amr_transfer<3, false>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    EXA_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::amr::k_amr_transfer<2, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    EXA_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::amr::k_amr_transfer<2, false>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    c.lds_set = true;
+  }
   if (!c.ready[N]) {
     AmrTables t{};
     for (int s = 0; s < 3; ++s)
@@ -58,17 +71,22 @@ int launch(int dim, bool prolong, const AmrTables* tab, const Transfer* items, c
            double* fine, double* coarse, int B, int nd) {
   if (ngroups == 0) return EXA_OK;
   const dim3 grid(ngroups), block(exa::amr::THREADS);
+  // LDS: two buffers of one value chunk (as many values per node as fit)
+  const int n = dim == 3 ? B * B * B : B * B;
+  const size_t lds = 2 * sizeof(double) * (size_t)n * exa::amr::amr_chunk(B, dim, nd);
   if (dim == 3) {
     if (prolong)
-      hipLaunchKernelGGL((exa::amr::k_amr_transfer<3, true>), grid, block, 0, 0, tab, items, groups, fine, coarse, B, nd);
+      hipLaunchKernelGGL((exa::amr::k_amr_transfer<3, true>), grid, block, lds, 0, tab, items, groups, fine, coarse, B,
+                         nd);
     else
-      hipLaunchKernelGGL((exa::amr::k_amr_transfer<3, false>), grid, block, 0, 0, tab, items, groups, fine, coarse, B,
+      hipLaunchKernelGGL((exa::amr::k_amr_transfer<3, false>), grid, block, lds, 0, tab, items, groups, fine, coarse, B,
                          nd);
   } else {
     if (prolong)
-      hipLaunchKernelGGL((exa::amr::k_amr_transfer<2, true>), grid, block, 0, 0, tab, items, groups, fine, coarse, B, nd);
+      hipLaunchKernelGGL((exa::amr::k_amr_transfer<2, true>), grid, block, lds, 0, tab, items, groups, fine, coarse, B,
+                         nd);
     else
-      hipLaunchKernelGGL((exa::amr::k_amr_transfer<2, false>), grid, block, 0, 0, tab, items, groups, fine, coarse, B,
+      hipLaunchKernelGGL((exa::amr::k_amr_transfer<2, false>), grid, block, lds, 0, tab, items, groups, fine, coarse, B,
                          nd);
   }
   EXA_CHECK(hipGetLastError());

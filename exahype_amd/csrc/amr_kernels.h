@@ -45,6 +45,15 @@ __device__ inline double level_op(const AmrTables& t, bool prolong, int sub, int
 __device__ inline void build_axis_ops(const AmrTables& t, bool prolong, const Transfer& tr, int dim, int B,
                                       double* A, double* tmp) {
   const int tid = threadIdx.x;
+  if (tr.levels == 1) {  // one level: the operators are table rows, all axes in one pass
+    __syncthreads();     // the previous transfer of the group is done with A
+    for (int e = tid; e < dim * B * B; e += blockDim.x) {
+      const int a = e / (B * B), oi = e - a * B * B, o = oi / B, i = oi - o * B;
+      A[a * MAXB * MAXB + o * MAXB + i] = level_op(t, prolong, tr.index[a] % 3, o, i);
+    }
+    __syncthreads();
+    return;
+  }
   for (int a = 0; a < dim; ++a) {
     double* Aa = A + a * MAXB * MAXB;
     int pw = 1;
@@ -74,26 +83,39 @@ __device__ inline void build_axis_ops(const AmrTables& t, bool prolong, const Tr
   __syncthreads();
 }
 
-// out[...o on axis...] = sum_i A[o][i] * in[...i on axis...] over a B^DIM block;
-// axis 0 is the fastest index.
+// out[...o on axis...] = sum_i A[o][i] * in[...i on axis...] over a B^DIM block
+// of nodes (axis 0 fastest) holding vc values per node (value fastest).
 template <int DIM>
-__device__ inline void axis_pass(const double* in, double* out, const double* A, int axis, int B) {
+__device__ inline void axis_pass(const double* in, double* out, const double* A, int axis, int B, int vc) {
   int n = B * B;
   if (DIM == 3) n *= B;
   int stride = 1;
   for (int a = 0; a < axis; ++a) stride *= B;
-  for (int e = threadIdx.x; e < n; e += blockDim.x) {
-    const int o = (e / stride) % B;
-    const int base = e - o * stride;
+  for (int e = threadIdx.x; e < n * vc; e += blockDim.x) {
+    const int node = e / vc, v = e - node * vc;
+    const int o = (node / stride) % B;
+    const int base = node - o * stride;
     double s = 0.0;
-    for (int i = 0; i < B; ++i) s += A[o * MAXB + i] * in[base + i * stride];
+    for (int i = 0; i < B; ++i) s += A[o * MAXB + i] * in[(base + i * stride) * vc + v];
     out[e] = s;
   }
 }
 
+// Values per node processed together: the two LDS buffers hold B^DIM * vc
+// doubles each (AMR_LDS_DOUBLES per buffer).
+constexpr int AMR_LDS_DOUBLES = 8192;
+__host__ __device__ inline int amr_chunk(int B, int DIM, int nd) {
+  const int n = DIM == 3 ? B * B * B : B * B;
+  int vc = AMR_LDS_DOUBLES / n;
+  if (vc > nd) vc = nd;
+  return vc < 1 ? 1 : vc;
+}
+
 // groups[g]..groups[g+1]-1 are the transfers of workgroup g.  Prolongation
 // overwrites the fine block (one transfer per group); restriction adds every
-// transfer of the group into their common coarse block.
+// transfer of the group into their common coarse block.  Blocks are
+// [node][nd] (the reference layout); a chunk of vc values per node is loaded
+// into LDS in one pass and contracted axis by axis for all its values.
 template <int DIM, bool PROLONG>
 __global__ void __launch_bounds__(THREADS) k_amr_transfer(const AmrTables* __restrict__ tab,
                                                           const Transfer* __restrict__ tr, const int* __restrict__ groups,
@@ -101,30 +123,37 @@ __global__ void __launch_bounds__(THREADS) k_amr_transfer(const AmrTables* __res
                                                           int nd) {
   __shared__ double A[3 * MAXB * MAXB];
   __shared__ double tmp[MAXB * MAXB];
-  __shared__ double buf0[MAXB * MAXB * MAXB];
-  __shared__ double buf1[MAXB * MAXB * MAXB];
+  extern __shared__ double sbuf[];  // 2 * n * amr_chunk(B, DIM, nd), set by the launch
   const int g = blockIdx.x;
   const int n = DIM == 3 ? B * B * B : B * B;
+  const int vcmax = amr_chunk(B, DIM, nd);
+  double* buf0 = sbuf;
+  double* buf1 = sbuf + n * vcmax;
   for (int it = groups[g]; it < groups[g + 1]; ++it) {
     const Transfer t = tr[it];
     build_axis_ops(*tab, PROLONG, t, DIM, B, A, tmp);
     const double* src = PROLONG ? coarse + t.coarse : fine + t.fine;
     double* dst = PROLONG ? fine + t.fine : coarse + t.coarse;
-    for (int v = 0; v < nd; ++v) {
-      for (int e = threadIdx.x; e < n; e += blockDim.x) buf0[e] = src[(long long)e * nd + v];
+    for (int v0 = 0; v0 < nd; v0 += vcmax) {
+      const int vc = nd - v0 < vcmax ? nd - v0 : vcmax;
+      for (int e = threadIdx.x; e < n * vc; e += blockDim.x) {
+        const int node = e / vc;
+        buf0[e] = src[(long long)node * nd + v0 + (e - node * vc)];
+      }
       __syncthreads();
-      axis_pass<DIM>(buf0, buf1, A, 0, B);
+      axis_pass<DIM>(buf0, buf1, A, 0, B, vc);
       __syncthreads();
-      axis_pass<DIM>(buf1, buf0, A + MAXB * MAXB, 1, B);
+      axis_pass<DIM>(buf1, buf0, A + MAXB * MAXB, 1, B, vc);
       __syncthreads();
       const double* res = buf0;
       if (DIM == 3) {
-        axis_pass<DIM>(buf0, buf1, A + 2 * MAXB * MAXB, 2, B);
+        axis_pass<DIM>(buf0, buf1, A + 2 * MAXB * MAXB, 2, B, vc);
         __syncthreads();
         res = buf1;
       }
-      for (int e = threadIdx.x; e < n; e += blockDim.x) {
-        double* o = dst + (long long)e * nd + v;
+      for (int e = threadIdx.x; e < n * vc; e += blockDim.x) {
+        const int node = e / vc;
+        double* o = dst + (long long)node * nd + v0 + (e - node * vc);
         if (PROLONG)
           *o = res[e];
         else
