@@ -88,7 +88,7 @@ int exa_selftest_mfma(int kind, const float* A, const float* B, float* D);
 typedef struct exa_solver exa_solver;
 
 typedef struct {
-  int order;        /* N, 1..7                                     */
+  int order;        /* N, 1..9 (8, 9: FP64 only)                   */
   int nvar;         /* must be 5 for EXA_PDE_EULER                 */
   int npar;         /* material parameters (passed through)        */
   double cfl;       /* CFL factor (0.9 in the reference)           */
@@ -100,7 +100,7 @@ typedef struct {
   int contraction;  /* EXA_CONTRACT_*: form of the sweeps' spatial contraction */
 } exa_aderdg_config;
 
-#define EXA_CONTRACT_DEFAULT 0  /* tuned per order                                        */
+#define EXA_CONTRACT_DEFAULT 0  /* tuned per order (the staged form for FP64 at N = 7)    */
 #define EXA_CONTRACT_PENCIL 1   /* pencil owners contract, two barriers per time slice    */
 #define EXA_CONTRACT_NODE 2     /* node owners contract their pencils, one barrier per slice */
 #define EXA_CONTRACT_PENCIL_PAIR 3 /* pencil owners, two slices per phase, contraction in place */
