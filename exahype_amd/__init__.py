@@ -67,7 +67,8 @@ EXPORTS = [
     "exa_last_error", "exa_version", "exa_init", "exa_device_synchronize", "exa_aderdg_create", "exa_aderdg_destroy",
     "exa_aderdg_space_time_predictor", "exa_aderdg_volume_integral", "exa_aderdg_surface_integral",
     "exa_aderdg_riemann_solver", "exa_aderdg_solution_update", "exa_aderdg_stable_time_step_size",
-    "exa_aderdg_adjust_solution", "exa_grid_create", "exa_grid_destroy", "exa_grid_num_cells",
+    "exa_aderdg_adjust_solution", "exa_aderdg_batch_prediction", "exa_aderdg_batch_riemann",
+    "exa_aderdg_batch_update", "exa_aderdg_batch_pending", "exa_aderdg_batch_flush", "exa_grid_create", "exa_grid_destroy", "exa_grid_num_cells",
     "exa_grid_upload", "exa_grid_download", "exa_grid_set_initial_conditions", "exa_grid_compute_dt",
     "exa_grid_set_dt", "exa_grid_step", "exa_grid_sync", "exa_grid_picard_iterations",
     "exa_grid_use_graph", "exa_grid_enable_timing", "exa_grid_kernel_times",
@@ -107,6 +108,11 @@ def load_library(path=LIB_PATH):
     L.exa_aderdg_volume_integral.argtypes = [C.c_void_p, _D, _D, _D]
     L.exa_aderdg_surface_integral.argtypes = [C.c_void_p, _D, _D, _D]
     L.exa_aderdg_riemann_solver.argtypes = [C.c_void_p, _D, _D, _D, _D, C.c_double, I]
+    L.exa_aderdg_batch_prediction.argtypes = [C.c_void_p, _D, _D, _D, _D, _D, C.c_double]
+    L.exa_aderdg_batch_riemann.argtypes = [C.c_void_p, _D, _D, _D, _D, C.c_double, I]
+    L.exa_aderdg_batch_update.argtypes = [C.c_void_p, _D, _D, _D, _D, C.c_double]
+    L.exa_aderdg_batch_pending.argtypes = [C.c_void_p, P(LL)]
+    L.exa_aderdg_batch_flush.argtypes = [C.c_void_p]
     L.exa_aderdg_solution_update.argtypes = [C.c_void_p, _D, _D, C.c_double]
     L.exa_aderdg_stable_time_step_size.argtypes = [C.c_void_p, _D, _D, _D]
     L.exa_aderdg_adjust_solution.argtypes = [C.c_void_p, _D, _D, _D, C.c_double, P(InitialCondition)]
@@ -226,6 +232,7 @@ class ADERDGSolver:
         self._h = h
         self.order, self.nvar, self.npar = order, nvar, npar
         self.B = order + 1
+        self._pending = []
         self.ndata = nvar + npar
 
     def close(self):
@@ -271,6 +278,35 @@ class ADERDGSolver:
                                               _p(np.ascontiguousarray(QR, np.float64)), float(dt),
                                               int(normalNonZero)))
         return FL, FR
+
+    # deferred forms: outputs are written into the given (float64, contiguous,
+    # writable) arrays by batch_flush(), which the caller invokes at the end of
+    # a sweep; the arrays are kept alive here until then
+    def batch_prediction(self, lQhbnd, lFhbnd, lduh, luh, dx, dt):
+        self._pending.extend((lQhbnd, lFhbnd, lduh))
+        _check(_lib.exa_aderdg_batch_prediction(self._h, _p(lQhbnd), _p(lFhbnd), _p(lduh),
+                                                _p(np.ascontiguousarray(luh, np.float64)), _p(_vec3(dx)), float(dt)))
+
+    def batch_riemann(self, FL, FR, QL, QR, dt, normalNonZero):
+        self._pending.extend((FL, FR))
+        _check(_lib.exa_aderdg_batch_riemann(self._h, _p(FL), _p(FR), _p(np.ascontiguousarray(QL, np.float64)),
+                                             _p(np.ascontiguousarray(QR, np.float64)), float(dt), int(normalNonZero)))
+
+    def batch_update(self, luh, lduh, lFhbnd, dx, dt):
+        self._pending.extend((luh, lduh))
+        _check(_lib.exa_aderdg_batch_update(self._h, _p(luh), _p(lduh), _p(np.ascontiguousarray(lFhbnd, np.float64)),
+                                            _p(_vec3(dx)), float(dt)))
+
+    def batch_pending(self):
+        c = (C.c_longlong * 3)()
+        _check(_lib.exa_aderdg_batch_pending(self._h, c))
+        return tuple(c)
+
+    def batch_flush(self):
+        try:
+            _check(_lib.exa_aderdg_batch_flush(self._h))
+        finally:
+            self._pending = []
 
     def solutionUpdate(self, luh, lduh, dt):
         luh = np.array(luh, dtype=np.float64, copy=True)

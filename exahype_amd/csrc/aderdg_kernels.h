@@ -874,7 +874,7 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
     const double* __restrict__ luh, double* __restrict__ lduh, double* __restrict__ qf, double* __restrict__ ff,
     int* __restrict__ iters, const double* __restrict__ dtp, double dx0, double dx1, double dx2, double gamma,
     long long ncells, double* __restrict__ stage, unsigned long long* __restrict__ sweep_count,
-    PredictorDebug dbg) {
+    PredictorDebug dbg, Ens ens) {
   using O = Ord<N>;
   constexpr int B = O::B, B2 = O::B2, B3 = O::B3, PS = O::PS, BLK = O::BLK;
   constexpr int NITEMS = 3 * V * B2;
@@ -894,12 +894,20 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
   const int ox = (z * B + y) * PS + x, oy = (z * B + x) * PS + y, oz = (y * B + x) * PS + z;
   const double w3 = T.w[z] * T.w[y] * T.w[x];
   const double iw3 = 1.0 / w3;
-  const double dt = *dtp;
+  double dt = ens.cell_inst ? 0.0 : *dtp;
   double* const slot = stage + (size_t)blockIdx.x * 2 * SQ;
   // opaque zero: keeps the Kxi / iK1 reads scalar loads inside the loops
   int tz = 0;
 
   for (long long cell = blockIdx.x; cell < ncells; cell += gridDim.x) {
+    if (ens.cell_inst) {  // per-cell mesh width, time step and gamma
+      const InstParams& P = ens.inst[ens.cell_inst[cell]];
+      dt = P.dt_cur;
+      dx0 = P.dx[0];
+      dx1 = P.dx[1];
+      dx2 = P.dx[2];
+      gamma = P.gamma;
+    }
     __syncthreads();  // the previous cell is done with the LDS
     pencil_constants<N, double>(sc, dt, dx0, dx1, dx2);
     double u[V];
@@ -1104,31 +1112,16 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_volume_integral_ref(const double
 }
 
 // ---------------------------------------------------------------------------
-// Rusanov flux on every face: one wavefront per face.
-// riemannSolverNonlinear.cpph:38-120; caller solveRiemannProblemAtInterface
-// (E/solvers/ADERDGSolver.cpp:2744-2825): left cell face 2d+1, right cell face 2d.
-// nbr[d*ncells + c] = neighbour of cell c in +d direction.
+// One face, one wavefront (lane = threadIdx.x % 64): Rusanov flux from the
+// face-averaged wave speeds; FL and FR enter holding the extrapolated normal
+// fluxes and both leave holding the numerical flux.
 template <int N>
-__global__ __launch_bounds__(256) void k_riemann(const double* __restrict__ qf, double* __restrict__ ff,
-                                                 const int* __restrict__ nbr, int ncells, double gamma,
-                                                 long long wave0, long long nwaves, Ens ens) {
+__device__ inline void riemann_face(const double* __restrict__ QL, const double* __restrict__ QR,
+                                    double* __restrict__ FL, double* __restrict__ FR, int d, double gamma, int lane) {
   using O = Ord<N>;
   constexpr int B = O::B, B2 = O::B2;
   constexpr int NPL = (B2 + 63) / 64;
   const DGTables& T = c_tab[N];
-  const long long local = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (local >= nwaves) return;
-  const long long wave = wave0 + local;
-  const int d = (int)(wave / ncells);
-  const int cl = (int)(wave % ncells);
-  const int cr = nbr[(size_t)d * ncells + cl];
-  if (ens.cell_inst) gamma = ens.inst[ens.cell_inst[cl]].gamma;
-  const double* QL = qf + ((size_t)cl * 6 + 2 * d + 1) * V * B2;
-  const double* QR = qf + ((size_t)cr * 6 + 2 * d) * V * B2;
-  double* FL = ff + ((size_t)cl * 6 + 2 * d + 1) * V * B2;
-  double* FR = ff + ((size_t)cr * 6 + 2 * d) * V * B2;
-
   double aL[V], aR[V];
 #pragma unroll
   for (int v = 0; v < V; ++v) aL[v] = aR[v] = 0.0;
@@ -1165,6 +1158,40 @@ __global__ __launch_bounds__(256) void k_riemann(const double* __restrict__ qf, 
       }
     }
   }
+}
+
+// Rusanov flux on every face of a grid: one wavefront per face.
+// riemannSolverNonlinear.cpph:38-120; caller solveRiemannProblemAtInterface
+// (E/solvers/ADERDGSolver.cpp:2744-2825): left cell face 2d+1, right cell face 2d.
+// nbr[d*ncells + c] = neighbour of cell c in +d direction.
+template <int N>
+__global__ __launch_bounds__(256) void k_riemann(const double* __restrict__ qf, double* __restrict__ ff,
+                                                 const int* __restrict__ nbr, int ncells, double gamma,
+                                                 long long wave0, long long nwaves, Ens ens) {
+  constexpr int B2 = Ord<N>::B2;
+  const long long local = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (local >= nwaves) return;
+  const long long wave = wave0 + local;
+  const int d = (int)(wave / ncells);
+  const int cl = (int)(wave % ncells);
+  const int cr = nbr[(size_t)d * ncells + cl];
+  if (ens.cell_inst) gamma = ens.inst[ens.cell_inst[cl]].gamma;
+  riemann_face<N>(qf + ((size_t)cl * 6 + 2 * d + 1) * V * B2, qf + ((size_t)cr * 6 + 2 * d) * V * B2,
+                  ff + ((size_t)cl * 6 + 2 * d + 1) * V * B2, ff + ((size_t)cr * 6 + 2 * d) * V * B2, d, gamma, lane);
+}
+
+// Rusanov flux of a list of independent faces (the batched per-face entry):
+// face i reads QL = q[2i], QR = q[2i+1] and updates FL = f[2i], FR = f[2i+1]
+// ([v][a*B+b] blocks of V B^2 doubles), normal direction dirs[i].
+template <int N>
+__global__ __launch_bounds__(256) void k_riemann_list(const double* __restrict__ q, double* __restrict__ f,
+                                                      const int* __restrict__ dirs, long long nfaces, double gamma) {
+  constexpr int B2 = Ord<N>::B2;
+  const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (i >= nfaces) return;
+  riemann_face<N>(q + (size_t)(2 * i) * V * B2, q + (size_t)(2 * i + 1) * V * B2, f + (size_t)(2 * i) * V * B2,
+                  f + (size_t)(2 * i + 1) * V * B2, dirs[i], gamma, threadIdx.x & 63);
 }
 
 // ---------------------------------------------------------------------------

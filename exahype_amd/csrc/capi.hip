@@ -174,6 +174,21 @@ struct exa_solver {
   std::mutex mu;
   // scratch for the per-cell entry points
   DevBuf luh, lduh, qf, ff, iters, dt, dbg, misc, stage;
+  // deferred per-cell work (exa_aderdg_batch_*): host staging until the flush
+  struct PredItem { double *lQhbnd, *lFhbnd, *lduh; };
+  struct RiemItem { double *FL, *FR; };
+  struct UpdItem { double *luh, *lduh; };
+  std::vector<PredItem> bp;
+  std::vector<double> bp_luh, bp_par;  // SoA state per cell; reference-layout copy when npar > 0
+  std::vector<exa::InstParams> bp_inst;
+  std::vector<RiemItem> br;
+  std::vector<double> br_q, br_f;
+  std::vector<int> br_dir;
+  std::vector<double> br_dt;
+  std::vector<UpdItem> bu;
+  std::vector<double> bu_luh, bu_lduh, bu_ff;
+  std::vector<exa::InstParams> bu_inst;
+  DevBuf b_luh, b_lduh, b_qf, b_ff, b_inst, b_cinst, b_iters, b_stage, b_dirs;
 };
 
 // Contraction form of the predictor's Picard sweeps: EXA_CONTRACT_PENCIL
@@ -272,6 +287,37 @@ static int sync_stream(exa_solver* s) {
   return EXA_OK;
 }
 
+// Material parameters are constant in time and carried as the reference does:
+// the trivial guess copies them, time averaging and face extrapolation weight
+// them.  Writes their entries of lQhbnd; parh = their time averages [node][P].
+static void param_predictor(int N, int nd, const double* luh, double* lQhbnd, std::vector<double>& parh) {
+  const int B = N + 1, B3 = B * B * B, V = exa::V, P = nd - V;
+  parh.assign((size_t)B3 * P, 0.0);
+  const double* w = EXA_TAB_gl_weights[N];
+  for (int n = 0; n < B3; ++n)
+    for (int p = 0; p < P; ++p) {
+      double a = 0.0;
+      for (int t = 0; t < B; ++t) a += luh[n * nd + V + p] * w[t];
+      parh[n * P + p] = a;
+    }
+  const double* FLc = EXA_TAB_FLCoeff[N];
+  const double* FRc = EXA_TAB_FRCoeff[N];
+  for (int d = 0; d < 3; ++d)
+    for (int a = 0; a < B; ++a)
+      for (int b = 0; b < B; ++b)
+        for (int p = 0; p < P; ++p) {
+          double qL = 0.0, qR = 0.0;
+          for (int l = 0; l < B; ++l) {
+            const int z = d == 2 ? l : a, y = d == 0 ? b : (d == 1 ? l : a), x = d == 0 ? l : b;
+            const double v = parh[((z * B + y) * B + x) * P + p];
+            qL += v * FLc[l];
+            qR += v * FRc[l];
+          }
+          lQhbnd[(((2 * d) * B + a) * B + b) * nd + V + p] = qL;
+          lQhbnd[(((2 * d + 1) * B + a) * B + b) * nd + V + p] = qR;
+        }
+}
+
 extern "C" int exa_aderdg_space_time_predictor(exa_solver* s, double* lQhbnd, double* lFhbnd, double* lQi,
                                                double* lFi, double* lQhi, double* lFhi, const double* luh,
                                                const double dx[3], double dt, int* picard_iterations) {
@@ -302,7 +348,7 @@ extern "C" int exa_aderdg_space_time_predictor(exa_solver* s, double* lQhbnd, do
       hipLaunchKernelGGL(exa_pred_staged_kernel(NN, true), dim3(1), dim3(O::NT),
                          (size_t)exa_pred_staged_lds_bytes(NN), s->stream, s->luh.as<double>(), s->lduh.as<double>(),
                          s->qf.as<double>(), s->ff.as<double>(), s->iters.as<int>(), s->dt.as<double>(), dx[0],
-                         dx[1], dx[2], gamma, 1LL, s->stage.as<double>(), nullptr, dbg);
+                         dx[1], dx[2], gamma, 1LL, s->stage.as<double>(), nullptr, dbg, exa::Ens{});
     } else {
       const int form = predictor_form(s->cfg, NN, s->device);
       PredFn kern = predictor_kernel<NN>(true, s->cfg.precision, 1, form);
@@ -326,34 +372,8 @@ extern "C" int exa_aderdg_space_time_predictor(exa_solver* s, double* lQhbnd, do
   const double* gFi = gQi + nQi;
   const double* gQhi = gFi + nFi;
   const double* gFhi = gQhi + nQhi;
-  // material parameters are constant in time: carried as the reference does
-  // (trivial guess copies them, time averaging and extrapolation weight them)
-  std::vector<double> parh((size_t)B3 * (P > 0 ? P : 1), 0.0);
-  if (P > 0) {
-    const double* w = EXA_TAB_gl_weights[N];
-    for (int n = 0; n < B3; ++n)
-      for (int p = 0; p < P; ++p) {
-        double a = 0.0;
-        for (int t = 0; t < B; ++t) a += luh[n * nd + V + p] * w[t];
-        parh[n * P + p] = a;
-      }
-    const double* FLc = EXA_TAB_FLCoeff[N];
-    const double* FRc = EXA_TAB_FRCoeff[N];
-    for (int d = 0; d < 3; ++d)
-      for (int a = 0; a < B; ++a)
-        for (int b = 0; b < B; ++b)
-          for (int p = 0; p < P; ++p) {
-            double qL = 0.0, qR = 0.0;
-            for (int l = 0; l < B; ++l) {
-              const int z = d == 2 ? l : a, y = d == 0 ? b : (d == 1 ? l : a), x = d == 0 ? l : b;
-              const double v = parh[((z * B + y) * B + x) * P + p];
-              qL += v * FLc[l];
-              qR += v * FRc[l];
-            }
-            lQhbnd[(((2 * d) * B + a) * B + b) * nd + V + p] = qL;
-            lQhbnd[(((2 * d + 1) * B + a) * B + b) * nd + V + p] = qR;
-          }
-  }
+  std::vector<double> parh;
+  if (P > 0) param_predictor(N, nd, luh, lQhbnd, parh);
   if (lQi) {
     for (int n = 0; n < B3; ++n)
       for (int t = 0; t < B; ++t) {
@@ -523,6 +543,246 @@ extern "C" int exa_aderdg_adjust_solution(exa_solver* s, double* luh, const doub
   EXA_CHECK(hipMemcpy(h.data(), s->luh.p, sizeof(double) * B3 * V, hipMemcpyDeviceToHost));
   soa_to_ref_cell(h.data(), luh, B3, nd);
   return EXA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Deferred per-cell entry points.  The reference calls its kernels cell by
+// cell (or face by face) inside one mesh traversal and consumes the results
+// only in the next traversal: spaceTimePredictor + volumeIntegral in the
+// prediction sweep (performPredictionAndVolumeIntegral,
+// E/solvers/ADERDGSolver.cpp:1818-1946), riemannSolver in the merging sweep
+// (solveRiemannProblemAtInterface, :2744-2825), surfaceIntegral +
+// solutionUpdate in the update sweep (updateSolution, :2100-2179).  The
+// batch_* calls copy their inputs and queue the work; exa_aderdg_batch_flush
+// (at the traversal's end) runs every queued item of a kind in one launch and
+// writes the outputs to the callers' arrays.
+
+static int predictor_waves_per_simd(const exa_aderdg_config& c, int N, int dev);
+
+static exa::InstParams cell_params(const exa_solver* s, const double dx[3], double dt) {
+  exa::InstParams P{};
+  P.dx[0] = dx[0];
+  P.dx[1] = dx[1];
+  P.dx[2] = dx[2];
+  P.gamma = s->cfg.gamma;
+  P.dt_cur = dt;
+  const double big = 1.0e300;
+  std::memcpy(&P.dt_next, &big, sizeof(big));
+  return P;
+}
+
+extern "C" int exa_aderdg_batch_prediction(exa_solver* s, double* lQhbnd, double* lFhbnd, double* lduh,
+                                           const double* luh, const double dx[3], double dt) {
+  EXA_ARG(s && lQhbnd && lFhbnd && lduh && luh && dx, "null pointer");
+  EXA_ARG(dt > 0.0, "dt must be positive");
+  std::lock_guard<std::mutex> lk(s->mu);
+  const int B = s->cfg.order + 1, B3 = B * B * B, nd = s->cfg.nvar + s->cfg.npar;
+  const size_t off = s->bp_luh.size();
+  s->bp_luh.resize(off + (size_t)exa::V * B3);
+  ref_to_soa_cell(luh, s->bp_luh.data() + off, B3, nd);
+  if (s->cfg.npar > 0) s->bp_par.insert(s->bp_par.end(), luh, luh + (size_t)B3 * nd);
+  s->bp_inst.push_back(cell_params(s, dx, dt));
+  s->bp.push_back({lQhbnd, lFhbnd, lduh});
+  return EXA_OK;
+}
+
+extern "C" int exa_aderdg_batch_riemann(exa_solver* s, double* FL, double* FR, const double* QL, const double* QR,
+                                        double dt, int normalNonZero) {
+  EXA_ARG(s && FL && FR && QL && QR, "null pointer");
+  EXA_ARG(normalNonZero >= 0 && normalNonZero < 3, "normalNonZero must be 0, 1 or 2");
+  std::lock_guard<std::mutex> lk(s->mu);
+  const int B = s->cfg.order + 1, B2 = B * B, V = exa::V, nd = s->cfg.nvar + s->cfg.npar;
+  const size_t off = s->br_q.size();
+  s->br_q.resize(off + 2 * (size_t)V * B2);
+  s->br_f.resize(off + 2 * (size_t)V * B2);
+  ref_to_soa_cell(QL, s->br_q.data() + off, B2, nd);
+  ref_to_soa_cell(QR, s->br_q.data() + off + (size_t)V * B2, B2, nd);
+  ref_to_soa_cell(FL, s->br_f.data() + off, B2, V);
+  ref_to_soa_cell(FR, s->br_f.data() + off + (size_t)V * B2, B2, V);
+  s->br_dir.push_back(normalNonZero);
+  s->br_dt.push_back(dt);
+  s->br.push_back({FL, FR});
+  return EXA_OK;
+}
+
+extern "C" int exa_aderdg_batch_update(exa_solver* s, double* luh, double* lduh, const double* lFhbnd,
+                                       const double dx[3], double dt) {
+  EXA_ARG(s && luh && lduh && lFhbnd && dx, "null pointer");
+  std::lock_guard<std::mutex> lk(s->mu);
+  const int B = s->cfg.order + 1, B2 = B * B, B3 = B2 * B, V = exa::V, nd = s->cfg.nvar + s->cfg.npar;
+  const size_t off = s->bu_luh.size(), offf = s->bu_ff.size();
+  s->bu_luh.resize(off + (size_t)V * B3);
+  s->bu_lduh.resize(off + (size_t)V * B3);
+  s->bu_ff.resize(offf + (size_t)6 * V * B2);
+  ref_to_soa_cell(luh, s->bu_luh.data() + off, B3, nd);
+  ref_to_soa_cell(lduh, s->bu_lduh.data() + off, B3, V);
+  ref_to_soa_faces(lFhbnd, s->bu_ff.data() + offf, 6, B2, V);
+  s->bu_inst.push_back(cell_params(s, dx, dt));
+  s->bu.push_back({luh, lduh});
+  return EXA_OK;
+}
+
+extern "C" int exa_aderdg_batch_pending(exa_solver* s, long long counts[3]) {
+  EXA_ARG(s && counts, "null pointer");
+  std::lock_guard<std::mutex> lk(s->mu);
+  counts[0] = (long long)s->bp.size();
+  counts[1] = (long long)s->br.size();
+  counts[2] = (long long)s->bu.size();
+  return EXA_OK;
+}
+
+static void batch_clear(exa_solver* s) {
+  s->bp.clear();
+  s->bp_luh.clear();
+  s->bp_par.clear();
+  s->bp_inst.clear();
+  s->br.clear();
+  s->br_q.clear();
+  s->br_f.clear();
+  s->br_dir.clear();
+  s->br_dt.clear();
+  s->bu.clear();
+  s->bu_luh.clear();
+  s->bu_lduh.clear();
+  s->bu_ff.clear();
+  s->bu_inst.clear();
+}
+
+static int batch_run_predictions(exa_solver* s) {
+  const int N = s->cfg.order, B = N + 1, B2 = B * B, B3 = B2 * B, V = exa::V, nd = s->cfg.nvar + s->cfg.npar;
+  const long long n = (long long)s->bp.size();
+  if (n == 0) return EXA_OK;
+  std::vector<int> ci(n);
+  for (long long c = 0; c < n; ++c) ci[c] = (int)c;
+  EXA_TRY(s->b_luh.ensure(sizeof(double) * s->bp_luh.size()));
+  EXA_TRY(s->b_lduh.ensure(sizeof(double) * n * V * B3));
+  EXA_TRY(s->b_qf.ensure(sizeof(double) * n * 6 * V * B2));
+  EXA_TRY(s->b_ff.ensure(sizeof(double) * n * 6 * V * B2));
+  EXA_TRY(s->b_inst.ensure(sizeof(exa::InstParams) * n));
+  EXA_TRY(s->b_cinst.ensure(sizeof(int) * n));
+  EXA_TRY(s->b_iters.ensure(sizeof(int) * n));
+  EXA_CHECK(hipMemcpyAsync(s->b_luh.p, s->bp_luh.data(), sizeof(double) * s->bp_luh.size(), hipMemcpyHostToDevice,
+                           s->stream));
+  EXA_CHECK(hipMemcpyAsync(s->b_inst.p, s->bp_inst.data(), sizeof(exa::InstParams) * n, hipMemcpyHostToDevice,
+                           s->stream));
+  EXA_CHECK(hipMemcpyAsync(s->b_cinst.p, ci.data(), sizeof(int) * n, hipMemcpyHostToDevice, s->stream));
+  exa::Ens ens{s->b_cinst.as<int>(), s->b_inst.as<exa::InstParams>(), nullptr};
+  exa::PredictorDebug nodbg;
+  nodbg.lQi = nodbg.lFi = nodbg.lQhi = nodbg.lFhi = nullptr;
+  EXA_ORDER_SWITCH(N, {
+    using O = exa::Ord<NN>;
+    if (predictor_staged(s->cfg, s->device)) {
+      int slots = 0;
+      EXA_TRY(staged_slots(NN, s->device, n, &slots));
+      EXA_TRY(s->b_stage.ensure(staged_slot_bytes(NN) * (size_t)slots));
+      hipLaunchKernelGGL(exa_pred_staged_kernel(NN, false), dim3((unsigned)slots), dim3(O::NT),
+                         (size_t)exa_pred_staged_lds_bytes(NN), s->stream, s->b_luh.as<double>(),
+                         s->b_lduh.as<double>(), s->b_qf.as<double>(), s->b_ff.as<double>(), s->b_iters.as<int>(),
+                         nullptr, 0.0, 0.0, 0.0, s->cfg.gamma, n, s->b_stage.as<double>(), nullptr, nodbg, ens);
+    } else {
+      const int wpe = predictor_waves_per_simd(s->cfg, NN, s->device);
+      const int form = predictor_form(s->cfg, NN, s->device);
+      PredFn kern = predictor_kernel<NN>(false, s->cfg.precision, wpe, form);
+      hipLaunchKernelGGL(kern, dim3((unsigned)n), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), s->stream,
+                         s->b_luh.as<double>(), s->b_lduh.as<double>(), s->b_qf.as<double>(), s->b_ff.as<double>(),
+                         s->b_iters.as<int>(), nullptr, 0.0, 0.0, 0.0, s->cfg.gamma, s->cfg.fp64_sweeps, nullptr,
+                         nodbg, ens);
+    }
+  });
+  EXA_CHECK(hipGetLastError());
+  std::vector<double> hq((size_t)n * 6 * V * B2), hf((size_t)n * 6 * V * B2), hd((size_t)n * V * B3);
+  EXA_CHECK(hipMemcpyAsync(hq.data(), s->b_qf.p, sizeof(double) * hq.size(), hipMemcpyDeviceToHost, s->stream));
+  EXA_CHECK(hipMemcpyAsync(hf.data(), s->b_ff.p, sizeof(double) * hf.size(), hipMemcpyDeviceToHost, s->stream));
+  EXA_CHECK(hipMemcpyAsync(hd.data(), s->b_lduh.p, sizeof(double) * hd.size(), hipMemcpyDeviceToHost, s->stream));
+  EXA_TRY(sync_stream(s));
+  std::vector<double> parh;
+  for (long long c = 0; c < n; ++c) {
+    const auto& it = s->bp[c];
+    soa_to_ref_faces(hq.data() + (size_t)c * 6 * V * B2, it.lQhbnd, 6, B2, nd);
+    soa_to_ref_faces(hf.data() + (size_t)c * 6 * V * B2, it.lFhbnd, 6, B2, V);
+    soa_to_ref_cell(hd.data() + (size_t)c * V * B3, it.lduh, B3, V);
+    if (s->cfg.npar > 0) param_predictor(N, nd, s->bp_par.data() + (size_t)c * B3 * nd, it.lQhbnd, parh);
+  }
+  return EXA_OK;
+}
+
+static int batch_run_riemann(exa_solver* s) {
+  const int N = s->cfg.order, B = N + 1, B2 = B * B, V = exa::V;
+  const long long m = (long long)s->br.size();
+  if (m == 0) return EXA_OK;
+  EXA_TRY(s->b_qf.ensure(sizeof(double) * s->br_q.size()));
+  EXA_TRY(s->b_ff.ensure(sizeof(double) * s->br_f.size()));
+  EXA_TRY(s->b_dirs.ensure(sizeof(int) * m));
+  EXA_CHECK(hipMemcpyAsync(s->b_qf.p, s->br_q.data(), sizeof(double) * s->br_q.size(), hipMemcpyHostToDevice,
+                           s->stream));
+  EXA_CHECK(hipMemcpyAsync(s->b_ff.p, s->br_f.data(), sizeof(double) * s->br_f.size(), hipMemcpyHostToDevice,
+                           s->stream));
+  EXA_CHECK(hipMemcpyAsync(s->b_dirs.p, s->br_dir.data(), sizeof(int) * m, hipMemcpyHostToDevice, s->stream));
+  const unsigned blocks = (unsigned)((m * 64 + 255) / 256);
+  EXA_ORDER_SWITCH(N, {
+    hipLaunchKernelGGL((exa::k_riemann_list<NN>), dim3(blocks), dim3(256), 0, s->stream, s->b_qf.as<double>(),
+                       s->b_ff.as<double>(), s->b_dirs.as<int>(), m, s->cfg.gamma);
+  });
+  EXA_CHECK(hipGetLastError());
+  EXA_CHECK(hipMemcpyAsync(s->br_f.data(), s->b_ff.p, sizeof(double) * s->br_f.size(), hipMemcpyDeviceToHost,
+                           s->stream));
+  EXA_TRY(sync_stream(s));
+  for (long long i = 0; i < m; ++i) {
+    soa_to_ref_cell(s->br_f.data() + (size_t)(2 * i) * V * B2, s->br[i].FL, B2, V);
+    soa_to_ref_cell(s->br_f.data() + (size_t)(2 * i + 1) * V * B2, s->br[i].FR, B2, V);
+  }
+  return EXA_OK;
+}
+
+static int batch_run_updates(exa_solver* s) {
+  const int N = s->cfg.order, B = N + 1, B2 = B * B, B3 = B2 * B, V = exa::V, nd = s->cfg.nvar + s->cfg.npar;
+  const long long n = (long long)s->bu.size();
+  if (n == 0) return EXA_OK;
+  std::vector<int> ci(n);
+  for (long long c = 0; c < n; ++c) ci[c] = (int)c;
+  EXA_TRY(s->b_luh.ensure(sizeof(double) * s->bu_luh.size()));
+  EXA_TRY(s->b_lduh.ensure(sizeof(double) * s->bu_lduh.size()));
+  EXA_TRY(s->b_ff.ensure(sizeof(double) * s->bu_ff.size()));
+  EXA_TRY(s->b_inst.ensure(sizeof(exa::InstParams) * n));
+  EXA_TRY(s->b_cinst.ensure(sizeof(int) * n));
+  EXA_CHECK(hipMemcpyAsync(s->b_luh.p, s->bu_luh.data(), sizeof(double) * s->bu_luh.size(), hipMemcpyHostToDevice,
+                           s->stream));
+  EXA_CHECK(hipMemcpyAsync(s->b_lduh.p, s->bu_lduh.data(), sizeof(double) * s->bu_lduh.size(),
+                           hipMemcpyHostToDevice, s->stream));
+  EXA_CHECK(hipMemcpyAsync(s->b_ff.p, s->bu_ff.data(), sizeof(double) * s->bu_ff.size(), hipMemcpyHostToDevice,
+                           s->stream));
+  EXA_CHECK(hipMemcpyAsync(s->b_inst.p, s->bu_inst.data(), sizeof(exa::InstParams) * n, hipMemcpyHostToDevice,
+                           s->stream));
+  EXA_CHECK(hipMemcpyAsync(s->b_cinst.p, ci.data(), sizeof(int) * n, hipMemcpyHostToDevice, s->stream));
+  exa::Ens ens{s->b_cinst.as<int>(), s->b_inst.as<exa::InstParams>(), nullptr};
+  EXA_ORDER_SWITCH(N, {
+    using O = exa::Ord<NN>;
+    hipLaunchKernelGGL((exa::k_update<NN, exa::UPD_SURFACE | exa::UPD_UPDATE | exa::UPD_WRITE_LDUH>), dim3((unsigned)n),
+                       dim3(O::NT), 0, s->stream, s->b_luh.as<double>(), s->b_lduh.as<double>(), s->b_ff.as<double>(),
+                       nullptr, nullptr, 0.0, 0.0, 0.0, s->cfg.gamma, s->cfg.cfl, ens);
+  });
+  EXA_CHECK(hipGetLastError());
+  EXA_CHECK(hipMemcpyAsync(s->bu_luh.data(), s->b_luh.p, sizeof(double) * s->bu_luh.size(), hipMemcpyDeviceToHost,
+                           s->stream));
+  EXA_CHECK(hipMemcpyAsync(s->bu_lduh.data(), s->b_lduh.p, sizeof(double) * s->bu_lduh.size(),
+                           hipMemcpyDeviceToHost, s->stream));
+  EXA_TRY(sync_stream(s));
+  for (long long c = 0; c < n; ++c) {
+    soa_to_ref_cell(s->bu_luh.data() + (size_t)c * V * B3, s->bu[c].luh, B3, nd);
+    soa_to_ref_cell(s->bu_lduh.data() + (size_t)c * V * B3, s->bu[c].lduh, B3, V);
+  }
+  return EXA_OK;
+}
+
+extern "C" int exa_aderdg_batch_flush(exa_solver* s) {
+  EXA_ARG(s, "null pointer");
+  std::lock_guard<std::mutex> lk(s->mu);
+  EXA_CHECK(hipSetDevice(s->device));
+  int r = batch_run_predictions(s);
+  if (r == EXA_OK) r = batch_run_riemann(s);
+  if (r == EXA_OK) r = batch_run_updates(s);
+  batch_clear(s);
+  return r;
 }
 
 // ---------------------------------------------------------------------------
@@ -775,7 +1035,8 @@ static int enqueue_step(exa_grid* g, bool timed) {
       hipLaunchKernelGGL(exa_pred_staged_kernel(NN, false), dim3((unsigned)g->nslots), dim3(O::NT),
                          (size_t)exa_pred_staged_lds_bytes(NN), g->stream, g->luh.as<double>(), g->lduh.as<double>(),
                          g->qf.as<double>(), g->ff.as<double>(), g->iters.as<int>(), g->dt_cur(), g->dx[0],
-                         g->dx[1], g->dx[2], g->s->cfg.gamma, g->nc, g->stage.as<double>(), g->iter_sum(), nodbg);
+                         g->dx[1], g->dx[2], g->s->cfg.gamma, g->nc, g->stage.as<double>(), g->iter_sum(), nodbg,
+                         exa::Ens{});
     } else {
       const int wpe = predictor_waves_per_simd(g->s->cfg, NN, g->s->device);
       const int form = predictor_form(g->s->cfg, NN, g->s->device);

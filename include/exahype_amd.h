@@ -174,6 +174,33 @@ typedef struct {
 int exa_aderdg_adjust_solution(exa_solver* s, double* luh, const double centre[3],
                                const double dx[3], double t, const exa_ic* ic);
 
+/* Deferred per-cell entry points.  The reference calls its kernels cell by
+ * cell inside one mesh traversal and reads the results only in the next one;
+ * these calls copy their inputs, queue the work and return, and
+ * exa_aderdg_batch_flush -- called where the traversal ends (the mappings'
+ * endIteration, e.g. E/mappings/Prediction.cpp:105-108) -- runs every queued
+ * item of a kind in one launch and writes the outputs to the arrays given at
+ * queue time (which must stay valid until then).  Kinds run in the order
+ * prediction, Riemann, update; queued items must not depend on each other.
+ *   batch_prediction: spaceTimePredictor + volumeIntegral of one cell
+ *     (performPredictionAndVolumeIntegral, E/solvers/ADERDGSolver.cpp:1818-1946):
+ *     lQhbnd, lFhbnd and lduh of luh over dt (the predictor time step).
+ *   batch_riemann: riemannSolver of one face (solveRiemannProblemAtInterface,
+ *     E/solvers/ADERDGSolver.cpp:2744-2825; arguments as exa_aderdg_riemann_solver).
+ *   batch_update: surfaceIntegral + solutionUpdate of one cell (updateSolution,
+ *     E/solvers/ADERDGSolver.cpp:2100-2179): luh += dt/w * (lduh - surface
+ *     fluxes); lduh receives the surface-integrated update.
+ * Same layouts as the per-cell calls above. */
+int exa_aderdg_batch_prediction(exa_solver* s, double* lQhbnd, double* lFhbnd, double* lduh, const double* luh,
+                                const double dx[3], double dt);
+int exa_aderdg_batch_riemann(exa_solver* s, double* FL, double* FR, const double* QL, const double* QR, double dt,
+                             int normalNonZero);
+int exa_aderdg_batch_update(exa_solver* s, double* luh, double* lduh, const double* lFhbnd, const double dx[3],
+                            double dt);
+/* queued items per kind: counts[0] predictions, [1] Riemann, [2] updates */
+int exa_aderdg_batch_pending(exa_solver* s, long long counts[3]);
+int exa_aderdg_batch_flush(exa_solver* s);
+
 /* ---------------------------------------------------------------------- */
 /* Device-resident periodic grid. */
 typedef struct exa_grid exa_grid;
