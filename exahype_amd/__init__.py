@@ -28,6 +28,7 @@ PREC_FP32_MFMA = 2
 PREC_BF16_MFMA = 3
 CONTRACT_DEFAULT, CONTRACT_PENCIL, CONTRACT_NODE, CONTRACT_PENCIL_PAIR, CONTRACT_STAGED = 0, 1, 2, 3, 4
 PDE_EULER = 0
+PDE_EULER_GRAVITY = 2
 PDE_EULERFV2D = 1
 FV_GODUNOV = 0
 FV_MUSCLHANCOCK = 1
@@ -39,7 +40,7 @@ _lib = None
 class AderdgConfig(C.Structure):
     _fields_ = [("order", C.c_int), ("nvar", C.c_int), ("npar", C.c_int), ("cfl", C.c_double),
                 ("gamma", C.c_double), ("pde", C.c_int), ("precision", C.c_int), ("fp64_sweeps", C.c_int),
-                ("waves_per_simd", C.c_int), ("contraction", C.c_int)]
+                ("waves_per_simd", C.c_int), ("contraction", C.c_int), ("gravity", C.c_double)]
 
 
 class EnsembleMember(C.Structure):
@@ -222,11 +223,14 @@ class ADERDGSolver:
     """Per-cell kernel surface of exahype::solvers::ADERDGSolver on the GPU."""
 
     def __init__(self, order, nvar=5, npar=0, cfl=0.9, gamma=1.4, precision=PREC_FP64, fp64_sweeps=1,
-                 waves_per_simd=0, contraction=0, device=0):
+                 waves_per_simd=0, contraction=0, device=0, gravity=None):
+        """gravity: None for the Euler equations, g for Euler with constant
+        gravity along -z (EXA_PDE_EULER_GRAVITY, an algebraic source)."""
         L = load_library()
         _check(L.exa_init(device))
-        self.cfg = AderdgConfig(order, nvar, npar, cfl, gamma, PDE_EULER, precision, fp64_sweeps, waves_per_simd,
-                                contraction)
+        pde = PDE_EULER if gravity is None else PDE_EULER_GRAVITY
+        self.cfg = AderdgConfig(order, nvar, npar, cfl, gamma, pde, precision, fp64_sweeps, waves_per_simd,
+                                contraction, 0.0 if gravity is None else float(gravity))
         h = C.c_void_p()
         _check(L.exa_aderdg_create(C.byref(self.cfg), C.byref(h)))
         self._h = h

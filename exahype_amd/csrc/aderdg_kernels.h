@@ -113,7 +113,16 @@ struct Ens {
   const int* cell_inst;  // instance of every cell; nullptr: a single grid
   InstParams* inst;
   const exa_ic* ics;  // per-instance initial conditions
+  double grav;        // gravity g along -z of the Euler-with-gravity PDE; 0: no source
 };
+
+// Algebraic source of Euler under gravity g along -z (the useSource arm of
+// spaceTimePredictorNonlinear.cpph:227-252): S = (0, 0, 0, -rho g, -m_z g).
+// Component v of S at state q (only v = 3, 4 are nonzero).
+template <typename T>
+__device__ inline T gravity_source(const T* q, int v, T g) {
+  return v == 3 ? -q[0] * g : (v == 4 ? -q[3] * g : T(0));
+}
 
 // ---------------------------------------------------------------------------
 // Picard sweeps of the space-time predictor for one cell held by one
@@ -138,7 +147,8 @@ struct Ens {
 template <int N, typename T, bool ACC_FH, bool DEBUG, bool PAIR = false>
 __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], T* sF, T* sP, const T* sc,
                                     double* red, bool act, int s, int z, int y, int x, double gamma, int max_sweeps,
-                                    double tol2, double (&fh)[3][V], PredictorDebug dbg) {
+                                    double tol2, double (&fh)[3][V], PredictorDebug dbg, double grav = 0.0,
+                                    double dt = 0.0, double* sh = nullptr) {
   using O = Ord<N>;
   constexpr int B = O::B, B2 = O::B2, B3 = O::B3, PS = O::PS, BLK = O::BLK;
   constexpr int NITEMS = 3 * V * B2;
@@ -235,9 +245,11 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
         // phase A
         if (act) {
           const T c0 = w3 * F0(t);
+          const T cs = grav != 0.0 ? (((W(t) * W(z)) * W(y)) * W(x)) * (T)dt : T(0);  // source weight
 #pragma unroll
           for (int v = 0; v < V; ++v) {
-            const T r = ((c0 * uu[v] - sP[(0 * V + v) * BLK + ox]) - sP[(1 * V + v) * BLK + oy]) - sP[(2 * V + v) * BLK + oz];
+            T r = ((c0 * uu[v] - sP[(0 * V + v) * BLK + ox]) - sP[(1 * V + v) * BLK + oy]) - sP[(2 * V + v) * BLK + oz];
+            if (v >= 3 && grav != 0.0) r += cs * gravity_source<T>(q[t], v, (T)grav);
             const T ri = iw3 * r;
 #pragma unroll
             for (int t2 = 0; t2 < B; ++t2) qn[t2][v] += ri * IK1(t2, t);
@@ -316,10 +328,12 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
             if (k < ns) {
               const T* buf = k == 0 ? sF : sP;
               const T c0 = w3 * F0(t0 + k);
+              const T cs = grav != 0.0 ? (((W(t0 + k) * W(z)) * W(y)) * W(x)) * (T)dt : T(0);
 #pragma unroll
               for (int v = 0; v < V; ++v) {
-                const T r =
+                T r =
                     ((c0 * uu[v] - buf[(0 * V + v) * BLK + ox]) - buf[(1 * V + v) * BLK + oy]) - buf[(2 * V + v) * BLK + oz];
+                if (v >= 3 && grav != 0.0) r += cs * gravity_source<T>(q[t0 + k], v, (T)grav);
                 const T ri = iw3 * r;
 #pragma unroll
                 for (int t2 = 0; t2 < B; ++t2) qn[t2][v] += ri * IK1(t2, t0 + k);
@@ -351,6 +365,7 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
       for (int d = 0; d < 3; ++d)
 #pragma unroll
         for (int v = 0; v < V; ++v) fh[d][v] = 0.0;
+      if (sh) sh[0] = sh[1] = 0.0;
 #pragma unroll
       for (int t = 0; t < B; ++t) {
         T F[3][V];
@@ -362,9 +377,15 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
             fh[d][v] += (double)F[d][v] * Td.w[t];
             if (DEBUG) dbg.lFi[((t * B3 + s) * 4 + d) * V + v] = (double)F[d][v];
           }
+        // the time-averaged source of the same iterate (lShi, components 3, 4)
+        if (sh && grav != 0.0) {
+          sh[0] += (double)gravity_source<T>(q[t], 3, (T)grav) * Td.w[t];
+          sh[1] += (double)gravity_source<T>(q[t], 4, (T)grav) * Td.w[t];
+        }
         if (DEBUG)
 #pragma unroll
-          for (int v = 0; v < V; ++v) dbg.lFi[((t * B3 + s) * 4 + 3) * V + v] = 0.0;
+          for (int v = 0; v < V; ++v)
+            dbg.lFi[((t * B3 + s) * 4 + 3) * V + v] = grav != 0.0 ? (double)gravity_source<T>(q[t], v, (T)grav) : 0.0;
       }
     }
 #pragma unroll
@@ -393,7 +414,8 @@ __device__ inline int picard_sweeps(T (&q)[Ord<N>::B][V], const double (&u)[V], 
 template <int N, typename T, bool ACC_FH, bool DEBUG>
 __device__ inline int picard_sweeps_node(T (&q)[Ord<N>::B][V], const double (&u)[V], T* sF, const T* sc,
                                          double* red, bool act, int s, int z, int y, int x, double gamma,
-                                         int max_sweeps, double tol2, double (&fh)[3][V], PredictorDebug dbg) {
+                                         int max_sweeps, double tol2, double (&fh)[3][V], PredictorDebug dbg,
+                                         double grav = 0.0, double dt = 0.0, double* sh = nullptr) {
   using O = Ord<N>;
   constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
   constexpr bool DP = std::is_same<T, double>::value;
@@ -444,6 +466,7 @@ __device__ inline int picard_sweeps_node(T (&q)[Ord<N>::B][V], const double (&u)
       __syncthreads();
       if (act) {
         const T c0 = w3 * F0(t);
+        const T cs = grav != 0.0 ? (((W(t) * W(z)) * W(y)) * W(x)) * (T)dt : T(0);
 #pragma unroll
         for (int v = 0; v < V; ++v) {
           const T* bx = buf + (0 * V + v) * B3 + px;
@@ -456,7 +479,8 @@ __device__ inline int picard_sweeps_node(T (&q)[Ord<N>::B][V], const double (&u)
           for (int n = 0; n < B; ++n) ay += by[n * B] * KXI(n, y);
 #pragma unroll
           for (int n = 0; n < B; ++n) az += bz[n * B2] * KXI(n, z);
-          const T r = ((c0 * uu[v] - ax) - ay) - az;
+          T r = ((c0 * uu[v] - ax) - ay) - az;
+          if (v >= 3 && grav != 0.0) r += cs * gravity_source<T>(q[t], v, (T)grav);
           const T ri = iw3 * r;
 #pragma unroll
           for (int t2 = 0; t2 < B; ++t2) qn[t2][v] += ri * IK1(t2, t);
@@ -482,6 +506,7 @@ __device__ inline int picard_sweeps_node(T (&q)[Ord<N>::B][V], const double (&u)
       for (int d = 0; d < 3; ++d)
 #pragma unroll
         for (int v = 0; v < V; ++v) fh[d][v] = 0.0;
+      if (sh) sh[0] = sh[1] = 0.0;
 #pragma unroll
       for (int t = 0; t < B; ++t) {
         T F[3][V];
@@ -493,9 +518,15 @@ __device__ inline int picard_sweeps_node(T (&q)[Ord<N>::B][V], const double (&u)
             fh[d][v] += (double)F[d][v] * Td.w[t];
             if (DEBUG) dbg.lFi[((t * B3 + s) * 4 + d) * V + v] = (double)F[d][v];
           }
+        // the time-averaged source of the same iterate (lShi, components 3, 4)
+        if (sh && grav != 0.0) {
+          sh[0] += (double)gravity_source<T>(q[t], 3, (T)grav) * Td.w[t];
+          sh[1] += (double)gravity_source<T>(q[t], 4, (T)grav) * Td.w[t];
+        }
         if (DEBUG)
 #pragma unroll
-          for (int v = 0; v < V; ++v) dbg.lFi[((t * B3 + s) * 4 + 3) * V + v] = 0.0;
+          for (int v = 0; v < V; ++v)
+            dbg.lFi[((t * B3 + s) * 4 + 3) * V + v] = grav != 0.0 ? (double)gravity_source<T>(q[t], v, (T)grav) : 0.0;
       }
     }
 #pragma unroll
@@ -709,7 +740,9 @@ enum : int { PRED_FP64 = 0, PRED_FP32 = 1, PRED_FP32_MFMA = 2, PRED_BF16_MFMA = 
 // Contraction form of the sweeps (kernel template argument).
 enum : int { FORM_PENCIL = 0, FORM_NODE = 1, FORM_PENCIL_PAIR = 2 };
 
-template <int N, bool DEBUG, int PREC, int WPE, int FORM>
+// SRC: the PDE has the gravity source (Euler with gravity, FP64 only); the
+// source code is compiled out of the plain Euler kernels.
+template <int N, bool DEBUG, int PREC, int WPE, int FORM, bool SRC = false>
 __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_predictor(
     const double* __restrict__ luh, double* __restrict__ lduh, double* __restrict__ qf,
     double* __restrict__ ff, int* __restrict__ iters, const double* __restrict__ dtp, double dx0,
@@ -750,6 +783,8 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
 
   double q[B][V];
   double fh[3][V];
+  double sh[2] = {0.0, 0.0};  // time-averaged source, components 3 and 4 (Euler with gravity)
+  const double grav = SRC ? ens.grav : 0.0;
   constexpr int MaxIterations = 2 * (N + 1);
   int it = 0;
   if constexpr (MIXED) {
@@ -785,11 +820,11 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
       for (int v = 0; v < V; ++v) q[t][v] = (double)qs[t][v];
     if constexpr (FORM == FORM_NODE)
       it += picard_sweeps_node<N, double, true, DEBUG>(q, u, sm, scd, red, act, s, z, y, x, gamma, fp64_sweeps,
-                                                       1e-7 * 1e-7, fh, dbg);
+                                                       1e-7 * 1e-7, fh, dbg, grav, dt, sh);
     else
       it += picard_sweeps<N, double, true, DEBUG, FORM == FORM_PENCIL_PAIR>(q, u, sm, sm + 3 * V * BLK, scd, red, act,
                                                                             s, z, y, x, gamma, fp64_sweeps,
-                                                                            1e-7 * 1e-7, fh, dbg);
+                                                                            1e-7 * 1e-7, fh, dbg, grav, dt, sh);
   } else {
 #pragma unroll
     for (int t = 0; t < B; ++t)
@@ -797,11 +832,11 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
       for (int v = 0; v < V; ++v) q[t][v] = u[v];
     if constexpr (FORM == FORM_NODE)
       it = picard_sweeps_node<N, double, true, DEBUG>(q, u, sm, scd, red, act, s, z, y, x, gamma, MaxIterations,
-                                                      1e-7 * 1e-7, fh, dbg);
+                                                      1e-7 * 1e-7, fh, dbg, grav, dt, sh);
     else
       it = picard_sweeps<N, double, true, DEBUG, FORM == FORM_PENCIL_PAIR>(q, u, sm, sm + 3 * V * BLK, scd, red, act, s,
                                                                            z, y, x, gamma, MaxIterations, 1e-7 * 1e-7,
-                                                                           fh, dbg);
+                                                                           fh, dbg, grav, dt, sh);
   }
 
   // time averages (aderPredictorNonlinear :326-397)
@@ -830,7 +865,7 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
         dbg.lFhi[0 * B3 * V + ((z * B + y) * B + x) * V + v] = fh[0][v];
         dbg.lFhi[1 * B3 * V + ((z * B + x) * B + y) * V + v] = fh[1][v];
         dbg.lFhi[2 * B3 * V + ((y * B + x) * B + z) * V + v] = fh[2][v];
-        dbg.lFhi[3 * B3 * V + s * V + v] = 0.0;
+        dbg.lFhi[3 * B3 * V + s * V + v] = v >= 3 ? sh[v - 3] : 0.0;
       }
     }
   }
@@ -838,6 +873,11 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
   if (act) {
     double out[V];
     volume_integral_node<N>(sFh, s, z, y, x, dx0, dx1, dx2, out);
+    if (SRC && grav != 0.0) {  // source term of the volume integral: += w^3 lShi
+      const double weight = T.w[z] * T.w[y] * T.w[x];
+      out[3] += weight * sh[0];
+      out[4] += weight * sh[1];
+    }
 #pragma unroll
     for (int v = 0; v < V; ++v) lduh[cell * V * B3 + v * B3 + s] = out[v];
   }
@@ -869,7 +909,7 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
 //   then every node owner applies iK1 to its own rhs history, takes the
 //   residual against q and overwrites the history with the new iterate; the
 //   two arrays swap roles between sweeps.
-template <int N, bool DEBUG>
+template <int N, bool DEBUG, bool SRC = false>
 __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
     const double* __restrict__ luh, double* __restrict__ lduh, double* __restrict__ qf, double* __restrict__ ff,
     int* __restrict__ iters, const double* __restrict__ dtp, double dx0, double dx1, double dx2, double gamma,
@@ -898,6 +938,7 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
   double* const slot = stage + (size_t)blockIdx.x * 2 * SQ;
   // opaque zero: keeps the Kxi / iK1 reads scalar loads inside the loops
   int tz = 0;
+  const double grav = SRC ? ens.grav : 0.0;  // Euler with gravity: algebraic source (0: none)
 
   for (long long cell = blockIdx.x; cell < ncells; cell += gridDim.x) {
     if (ens.cell_inst) {  // per-cell mesh width, time step and gamma
@@ -968,10 +1009,18 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
         __syncthreads();
         if (act) {
           const double c0 = w3 * c_tab[N + tz].F0[t];
+          double cs = 0.0, src[V] = {0.0, 0.0, 0.0, 0.0, 0.0};
+          if (grav != 0.0) {  // rhs += ((((w_t w_z) w_y) w_x) dt) S(q[t])
+            double Qs[V] = {q[(t * V + 0) * B3 + sl], 0.0, 0.0, q[(t * V + 3) * B3 + sl], 0.0};
+            cs = (((T.w[t] * T.w[z]) * T.w[y]) * T.w[x]) * dt;
+            src[3] = gravity_source<double>(Qs, 3, grav);
+            src[4] = gravity_source<double>(Qs, 4, grav);
+          }
 #pragma unroll
           for (int v = 0; v < V; ++v) {
-            const double rr =
+            double rr =
                 ((c0 * u[v] - sF[(0 * V + v) * BLK + ox]) - sF[(1 * V + v) * BLK + oy]) - sF[(2 * V + v) * BLK + oz];
+            if (v >= 3 && grav != 0.0) rr += cs * src[v];
             r[(t * V + v) * B3 + sl] = iw3 * rr;
           }
         }
@@ -1014,6 +1063,7 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
     // q: the input iterate of the last sweep (its fluxes are the time-averaged
     // fluxes, aderPredictorNonlinear :326-397); r: the final iterate
     double fh[3][V];
+    double sh[2] = {0.0, 0.0};  // time-averaged source, components 3 and 4
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -1035,9 +1085,14 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
             fh[d][v] += F[d][v] * wt;
             if (DEBUG) dbg.lFi[((t * B3 + s) * 4 + d) * V + v] = F[d][v];
           }
+        if (grav != 0.0) {
+          sh[0] += gravity_source<double>(Q, 3, grav) * wt;
+          sh[1] += gravity_source<double>(Q, 4, grav) * wt;
+        }
         if (DEBUG)
 #pragma unroll
-          for (int v = 0; v < V; ++v) dbg.lFi[((t * B3 + s) * 4 + 3) * V + v] = 0.0;
+          for (int v = 0; v < V; ++v)
+            dbg.lFi[((t * B3 + s) * 4 + 3) * V + v] = grav != 0.0 ? gravity_source<double>(Q, v, grav) : 0.0;
       }
     }
     if (act) {
@@ -1063,7 +1118,7 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
           dbg.lFhi[0 * B3 * V + ((z * B + y) * B + x) * V + v] = fh[0][v];
           dbg.lFhi[1 * B3 * V + ((z * B + x) * B + y) * V + v] = fh[1][v];
           dbg.lFhi[2 * B3 * V + ((y * B + x) * B + z) * V + v] = fh[2][v];
-          dbg.lFhi[3 * B3 * V + s * V + v] = 0.0;
+          dbg.lFhi[3 * B3 * V + s * V + v] = v >= 3 ? sh[v - 3] : 0.0;
         }
       }
     }
@@ -1071,6 +1126,11 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
     if (act) {
       double out[V];
       volume_integral_node<N>(sFh, s, z, y, x, dx0, dx1, dx2, out);
+      if (grav != 0.0) {  // source term of the volume integral: += w^3 lShi
+        const double weight = T.w[z] * T.w[y] * T.w[x];
+        out[3] += weight * sh[0];
+        out[4] += weight * sh[1];
+      }
 #pragma unroll
       for (int v = 0; v < V; ++v) lduh[cell * V * B3 + v * B3 + s] = out[v];
     }
@@ -1086,7 +1146,7 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
 template <int N>
 __global__ __launch_bounds__(Ord<N>::NT) void k_volume_integral_ref(const double* __restrict__ lFhi,
                                                                     double* __restrict__ lduh, double dx0,
-                                                                    double dx1, double dx2) {
+                                                                    double dx1, double dx2, int src) {
   using O = Ord<N>;
   constexpr int B = O::B, B3 = O::B3;
   extern __shared__ double sm[];
@@ -1106,6 +1166,11 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_volume_integral_ref(const double
   if (act) {
     double out[V];
     volume_integral_node<N>(sFh, s, z, y, x, dx0, dx1, dx2, out);
+    if (src) {  // source term: += w^3 lShi (volumeIntegralNonlinear.cpph, useSourceOrNCP branch)
+      const double weight = c_tab[N].w[z] * c_tab[N].w[y] * c_tab[N].w[x];
+#pragma unroll
+      for (int v = 0; v < V; ++v) out[v] += weight * lFhi[3 * B3 * V + s * V + v];
+    }
 #pragma unroll
     for (int v = 0; v < V; ++v) lduh[v * B3 + s] = out[v];
   }

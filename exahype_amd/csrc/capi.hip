@@ -55,6 +55,20 @@ static unsigned long long g_init_mask = 0;
 
 // Predictor kernels live in the per-order units pred_n<N>.hip.
 template <int N>
+static PredFn predictor_kernel_src(bool debug) {
+  switch (N) {
+    case 1: return exa_pred_kernel_src_1(debug);
+    case 2: return exa_pred_kernel_src_2(debug);
+    case 3: return exa_pred_kernel_src_3(debug);
+    case 4: return exa_pred_kernel_src_4(debug);
+    case 5: return exa_pred_kernel_src_5(debug);
+    case 6: return exa_pred_kernel_src_6(debug);
+    case 7: return exa_pred_kernel_src_7(debug);
+    default: return nullptr;
+  }
+}
+
+template <int N>
 static PredFn predictor_kernel(bool debug, int prec, int wpe, int form) {
   switch (N) {
     case 1: return exa_pred_kernel_1(debug, prec, wpe, form);
@@ -210,6 +224,14 @@ static bool predictor_staged(const exa_aderdg_config& c, int dev) {
          g_tuned[dev & 63][7][EXA_PREC_FP64].contraction == 0;  // exa_autotune's choice wins
 }
 
+// Launch extras of a single-grid predictor launch: the gravity source of
+// EXA_PDE_EULER_GRAVITY (0 for the plain Euler equations).
+static exa::Ens launch_extras(const exa_aderdg_config& c) {
+  exa::Ens e{};
+  e.grav = c.pde == EXA_PDE_EULER_GRAVITY ? c.gravity : 0.0;
+  return e;
+}
+
 static int predictor_form(const exa_aderdg_config& c, int N, int dev) {
   switch (c.contraction) {
     case EXA_CONTRACT_PENCIL: return exa::FORM_PENCIL;
@@ -232,7 +254,10 @@ extern "C" int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out)
   EXA_ARG(cfg->contraction != EXA_CONTRACT_STAGED ||
               (cfg->precision == EXA_PREC_FP64 && cfg->order >= EXA_STAGED_MIN_ORDER),
           "the staged predictor is FP64 and built for orders 5..9");
-  EXA_ARG(cfg->pde == EXA_PDE_EULER && cfg->nvar == 5, "only 3D Euler with 5 variables is compiled in");
+  EXA_ARG((cfg->pde == EXA_PDE_EULER || cfg->pde == EXA_PDE_EULER_GRAVITY) && cfg->nvar == 5,
+          "only 3D Euler (optionally with gravity) with 5 variables is compiled in");
+  EXA_ARG(cfg->pde != EXA_PDE_EULER_GRAVITY || cfg->precision == EXA_PREC_FP64,
+          "the gravity source runs in the FP64 predictor only");
   EXA_ARG(cfg->npar >= 0 && cfg->npar <= 3, "npar must be 0..3");
   EXA_ARG(cfg->cfl > 0.0 && cfg->gamma > 1.0, "cfl/gamma");
   EXA_ARG(cfg->precision >= EXA_PREC_FP64 && cfg->precision <= EXA_PREC_BF16_MFMA, "precision");
@@ -345,17 +370,18 @@ extern "C" int exa_aderdg_space_time_predictor(exa_solver* s, double* lQhbnd, do
     using O = exa::Ord<NN>;
     if (predictor_staged(s->cfg, s->device)) {
       EXA_TRY(s->stage.ensure(staged_slot_bytes(NN)));
-      hipLaunchKernelGGL(exa_pred_staged_kernel(NN, true), dim3(1), dim3(O::NT),
+      hipLaunchKernelGGL(exa_pred_staged_kernel(NN, true, s->cfg.pde == EXA_PDE_EULER_GRAVITY), dim3(1), dim3(O::NT),
                          (size_t)exa_pred_staged_lds_bytes(NN), s->stream, s->luh.as<double>(), s->lduh.as<double>(),
                          s->qf.as<double>(), s->ff.as<double>(), s->iters.as<int>(), s->dt.as<double>(), dx[0],
-                         dx[1], dx[2], gamma, 1LL, s->stage.as<double>(), nullptr, dbg, exa::Ens{});
+                         dx[1], dx[2], gamma, 1LL, s->stage.as<double>(), nullptr, dbg, launch_extras(s->cfg));
     } else {
       const int form = predictor_form(s->cfg, NN, s->device);
-      PredFn kern = predictor_kernel<NN>(true, s->cfg.precision, 1, form);
+      PredFn kern = s->cfg.pde == EXA_PDE_EULER_GRAVITY ? predictor_kernel_src<NN>(true)
+                                                          : predictor_kernel<NN>(true, s->cfg.precision, 1, form);
       hipLaunchKernelGGL(kern, dim3(1), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), s->stream, s->luh.as<double>(),
                          s->lduh.as<double>(), s->qf.as<double>(), s->ff.as<double>(), s->iters.as<int>(),
                          s->dt.as<double>(), dx[0], dx[1], dx[2], gamma, s->cfg.fp64_sweeps, nullptr, dbg,
-                         exa::Ens{});
+                         launch_extras(s->cfg));
     }
   });
   EXA_CHECK(hipGetLastError());
@@ -404,7 +430,8 @@ extern "C" int exa_aderdg_volume_integral(exa_solver* s, double* lduh, const dou
   EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
     hipLaunchKernelGGL((exa::k_volume_integral_ref<NN>), dim3(1), dim3(O::NT), (size_t)volume_integral_lds_bytes<NN>(),
-                       s->stream, s->misc.as<double>(), s->lduh.as<double>(), dx[0], dx[1], dx[2]);
+                       s->stream, s->misc.as<double>(), s->lduh.as<double>(), dx[0], dx[1], dx[2],
+                       s->cfg.pde == EXA_PDE_EULER_GRAVITY ? 1 : 0);
   });
   EXA_CHECK(hipGetLastError());
   std::vector<double> h(B3 * V);
@@ -666,7 +693,9 @@ static int batch_run_predictions(exa_solver* s) {
   EXA_CHECK(hipMemcpyAsync(s->b_inst.p, s->bp_inst.data(), sizeof(exa::InstParams) * n, hipMemcpyHostToDevice,
                            s->stream));
   EXA_CHECK(hipMemcpyAsync(s->b_cinst.p, ci.data(), sizeof(int) * n, hipMemcpyHostToDevice, s->stream));
-  exa::Ens ens{s->b_cinst.as<int>(), s->b_inst.as<exa::InstParams>(), nullptr};
+  exa::Ens ens = launch_extras(s->cfg);
+  ens.cell_inst = s->b_cinst.as<int>();
+  ens.inst = s->b_inst.as<exa::InstParams>();
   exa::PredictorDebug nodbg;
   nodbg.lQi = nodbg.lFi = nodbg.lQhi = nodbg.lFhi = nullptr;
   EXA_ORDER_SWITCH(N, {
@@ -675,14 +704,15 @@ static int batch_run_predictions(exa_solver* s) {
       int slots = 0;
       EXA_TRY(staged_slots(NN, s->device, n, &slots));
       EXA_TRY(s->b_stage.ensure(staged_slot_bytes(NN) * (size_t)slots));
-      hipLaunchKernelGGL(exa_pred_staged_kernel(NN, false), dim3((unsigned)slots), dim3(O::NT),
+      hipLaunchKernelGGL(exa_pred_staged_kernel(NN, false, s->cfg.pde == EXA_PDE_EULER_GRAVITY), dim3((unsigned)slots), dim3(O::NT),
                          (size_t)exa_pred_staged_lds_bytes(NN), s->stream, s->b_luh.as<double>(),
                          s->b_lduh.as<double>(), s->b_qf.as<double>(), s->b_ff.as<double>(), s->b_iters.as<int>(),
                          nullptr, 0.0, 0.0, 0.0, s->cfg.gamma, n, s->b_stage.as<double>(), nullptr, nodbg, ens);
     } else {
       const int wpe = predictor_waves_per_simd(s->cfg, NN, s->device);
       const int form = predictor_form(s->cfg, NN, s->device);
-      PredFn kern = predictor_kernel<NN>(false, s->cfg.precision, wpe, form);
+      PredFn kern = s->cfg.pde == EXA_PDE_EULER_GRAVITY ? predictor_kernel_src<NN>(false)
+                                                          : predictor_kernel<NN>(false, s->cfg.precision, wpe, form);
       hipLaunchKernelGGL(kern, dim3((unsigned)n), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), s->stream,
                          s->b_luh.as<double>(), s->b_lduh.as<double>(), s->b_qf.as<double>(), s->b_ff.as<double>(),
                          s->b_iters.as<int>(), nullptr, 0.0, 0.0, 0.0, s->cfg.gamma, s->cfg.fp64_sweeps, nullptr,
@@ -1032,19 +1062,20 @@ static int enqueue_step(exa_grid* g, bool timed) {
   EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
     if (g->staged) {
-      hipLaunchKernelGGL(exa_pred_staged_kernel(NN, false), dim3((unsigned)g->nslots), dim3(O::NT),
+      hipLaunchKernelGGL(exa_pred_staged_kernel(NN, false, g->s->cfg.pde == EXA_PDE_EULER_GRAVITY), dim3((unsigned)g->nslots), dim3(O::NT),
                          (size_t)exa_pred_staged_lds_bytes(NN), g->stream, g->luh.as<double>(), g->lduh.as<double>(),
                          g->qf.as<double>(), g->ff.as<double>(), g->iters.as<int>(), g->dt_cur(), g->dx[0],
                          g->dx[1], g->dx[2], g->s->cfg.gamma, g->nc, g->stage.as<double>(), g->iter_sum(), nodbg,
-                         exa::Ens{});
+                         launch_extras(g->s->cfg));
     } else {
       const int wpe = predictor_waves_per_simd(g->s->cfg, NN, g->s->device);
       const int form = predictor_form(g->s->cfg, NN, g->s->device);
-      PredFn kern = predictor_kernel<NN>(false, g->s->cfg.precision, wpe, form);
+      PredFn kern = g->s->cfg.pde == EXA_PDE_EULER_GRAVITY ? predictor_kernel_src<NN>(false)
+                                                             : predictor_kernel<NN>(false, g->s->cfg.precision, wpe, form);
       hipLaunchKernelGGL(kern, dim3((unsigned)g->nc), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), g->stream,
                          g->luh.as<double>(), g->lduh.as<double>(), g->qf.as<double>(), g->ff.as<double>(),
                          g->iters.as<int>(), g->dt_cur(), g->dx[0], g->dx[1], g->dx[2], g->s->cfg.gamma,
-                         g->s->cfg.fp64_sweeps, g->iter_sum(), nodbg, exa::Ens{});
+                         g->s->cfg.fp64_sweeps, g->iter_sum(), nodbg, launch_extras(g->s->cfg));
     }
   });
   EXA_CHECK(hipGetLastError());
@@ -1205,6 +1236,7 @@ extern "C" int exa_ensemble_create(const exa_ensemble_member* m, int n, const ex
                                    exa_ensemble** out) {
   EXA_ARG(m && n > 0 && cfg && out, "arguments");
   EXA_ARG(cfg->contraction != EXA_CONTRACT_STAGED, "ensembles run the register-resident predictor forms");
+  EXA_ARG(cfg->pde != EXA_PDE_EULER_GRAVITY, "ensemble members solve the Euler equations without sources");
   size_t need = 0;
   for (int i = 0; i < n; ++i) {
     EXA_ARG(m[i].order >= 1 && m[i].order <= 7, "member order must be 1..7");

@@ -36,6 +36,12 @@ PredFn EXA_CAT(exa_pred_kernel_, EXA_PRED_ORDER)(bool debug, int prec, int wpe, 
   return by_form<false, 1>(prec, form);
 }
 
+// The gravity-source variant: FP64, pencil owners, one wave per SIMD.
+PredFn EXA_CAT(exa_pred_kernel_src_, EXA_PRED_ORDER)(bool debug) {
+  if (debug) return &exa::k_predictor<NP, true, exa::PRED_FP64, 1, exa::FORM_PENCIL, true>;
+  return &exa::k_predictor<NP, false, exa::PRED_FP64, 1, exa::FORM_PENCIL, true>;
+}
+
 hipError_t EXA_CAT(exa_pred_init_, EXA_PRED_ORDER)(const DGTables* tab, const DGTablesF* tabf) {
   hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab), tab, sizeof(DGTables) * (EXA_MAX_ORDER + 1));
   if (e != hipSuccess) return e;
@@ -52,5 +58,10 @@ hipError_t EXA_CAT(exa_pred_init_, EXA_PRED_ORDER)(const DGTables* tab, const DG
               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
           if (e != hipSuccess) return e;
         }
+  for (int dbg = 0; dbg < 2; ++dbg) {
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(EXA_CAT(exa_pred_kernel_src_, EXA_PRED_ORDER)(dbg)),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+  }
   return hipSuccess;
 }

@@ -5,18 +5,19 @@
 
 namespace {
 template <int N>
-StagedFn pick(bool debug) {
+StagedFn pick(bool debug, bool src) {
+  if (src) return debug ? &exa::k_predictor_staged<N, true, true> : &exa::k_predictor_staged<N, false, true>;
   return debug ? &exa::k_predictor_staged<N, true> : &exa::k_predictor_staged<N, false>;
 }
 }  // namespace
 
-StagedFn exa_pred_staged_kernel(int order, bool debug) {
+StagedFn exa_pred_staged_kernel(int order, bool debug, bool src) {
   switch (order) {
-    case 5: return pick<5>(debug);
-    case 6: return pick<6>(debug);
-    case 7: return pick<7>(debug);
-    case 8: return pick<8>(debug);
-    case 9: return pick<9>(debug);
+    case 5: return pick<5>(debug, src);
+    case 6: return pick<6>(debug, src);
+    case 7: return pick<7>(debug, src);
+    case 8: return pick<8>(debug, src);
+    case 9: return pick<9>(debug, src);
     default: return nullptr;
   }
 }
@@ -38,10 +39,11 @@ hipError_t exa_pred_staged_init(const DGTables* tab, const DGTablesF* tabf) {
   e = hipMemcpyToSymbol(HIP_SYMBOL(c_tabf), tabf, sizeof(DGTablesF) * (EXA_MAX_ORDER + 1));
   if (e != hipSuccess) return e;
   for (int order = EXA_STAGED_MIN_ORDER; order <= EXA_MAX_ORDER; ++order)
-    for (int dbg = 0; dbg < 2; ++dbg) {
-      e = hipFuncSetAttribute(reinterpret_cast<const void*>(exa_pred_staged_kernel(order, dbg)),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, exa_pred_staged_lds_bytes(order));
-      if (e != hipSuccess) return e;
-    }
+    for (int dbg = 0; dbg < 2; ++dbg)
+      for (int src = 0; src < 2; ++src) {
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(exa_pred_staged_kernel(order, dbg, src)),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, exa_pred_staged_lds_bytes(order));
+        if (e != hipSuccess) return e;
+      }
   return hipSuccess;
 }

@@ -17,9 +17,11 @@ using PredFn = void (*)(const double*, double*, double*, double*, int*, const do
 // Kernel of one variant: per-cell debug outputs (one wave per SIMD only),
 // precision EXA_PREC_*, waves per SIMD (2 is built for N <= 5), contraction
 // form exa::FORM_* (pencil owners, node owners, pencil owners two slices at
-// a time).
+// a time).  exa_pred_kernel_src_N: the gravity-source variant (FP64, pencil
+// owners, one wave per SIMD).
 #define EXA_PRED_TU_DECL(N)                                                   \
   PredFn exa_pred_kernel_##N(bool debug, int prec, int wpe, int form);        \
+  PredFn exa_pred_kernel_src_##N(bool debug);                                 \
   hipError_t exa_pred_init_##N(const DGTables* tab, const DGTablesF* tabf);
 EXA_PRED_TU_DECL(1)
 EXA_PRED_TU_DECL(2)
@@ -37,6 +39,6 @@ EXA_PRED_TU_DECL(7)
 #define EXA_STAGED_MIN_ORDER 5
 using StagedFn = void (*)(const double*, double*, double*, double*, int*, const double*, double, double, double,
                           double, long long, double*, unsigned long long*, exa::PredictorDebug, exa::Ens);
-StagedFn exa_pred_staged_kernel(int order, bool debug);
+StagedFn exa_pred_staged_kernel(int order, bool debug, bool src = false);  // src: gravity source
 int exa_pred_staged_lds_bytes(int order);
 hipError_t exa_pred_staged_init(const DGTables* tab, const DGTablesF* tabf);

@@ -51,6 +51,10 @@ extern "C" {
 
 /* PDEs compiled into the kernels */
 #define EXA_PDE_EULER 0   /* 3D compressible Euler, 5 variables (gamma from config) */
+#define EXA_PDE_EULER_GRAVITY 2  /* 3D Euler with constant gravity g along -z: algebraic
+                                    source S = (0, 0, 0, -rho g, -m_z g), the useSource arm of
+                                    spaceTimePredictorNonlinear.cpph:227-252 and of
+                                    volumeIntegralNonlinear.cpph (FP64 predictor only) */
 
 const char* exa_last_error(void);
 
@@ -98,6 +102,7 @@ typedef struct {
   int fp64_sweeps;  /* mixed precisions: FP64 Picard sweeps after the low-precision phase (>=1) */
   int waves_per_simd; /* predictor occupancy target: 0 = tuned default per order, 1 or 2 */
   int contraction;  /* EXA_CONTRACT_*: form of the sweeps' spatial contraction */
+  double gravity;   /* g of EXA_PDE_EULER_GRAVITY (ignored otherwise)          */
 } exa_aderdg_config;
 
 #define EXA_CONTRACT_DEFAULT 0  /* tuned per order (the staged form for FP64 at N = 7)    */

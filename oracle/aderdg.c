@@ -66,6 +66,19 @@ void orc_eigenvalues(const orc_cfg* c, const double* Q, int d, double* lambda) {
   lambda[4] = u_n + cs;
 }
 
+/* Algebraic source of the user PDE (UserSolverInterface::algebraicSource,
+ * reached through fusedSource in spaceTimePredictorNonlinear.cpph:227-252).
+ * pde 3: Euler under constant gravity g along -z, S = (0, 0, 0, -rho g, -m_z g). */
+int orc_has_source(const orc_cfg* c) { return c->pde == 3; }
+
+void orc_source(const orc_cfg* c, const double* Q, double* S) {
+  for (int v = 0; v < c->nvar; ++v) S[v] = 0.0;
+  if (c->pde == 3) {
+    S[3] = -Q[0] * c->grav;
+    S[4] = -Q[3] * c->grav;
+  }
+}
+
 /* per-thread scratch of the predictor, sized for N=9 and nData<=8 */
 static _Thread_local double* tl_lQi;
 static _Thread_local double* tl_lQi_old;
@@ -137,6 +150,8 @@ int orc_space_time_predictor(const orc_cfg* c, double* lQhbnd, double* lFhbnd,
             orc_flux(c, &LQI(lQi, j, k, l, i, 0), F);
             for (int d = 0; d < 3; ++d)
               for (int m = 0; m < nv; ++m) LFI(i, j, k, l, d, m) = F[d * nv + m];
+            /* the source slot is written below when the PDE has one; zero otherwise */
+            for (int m = 0; m < nv; ++m) LFI(i, j, k, l, 3, m) = 0.0;
           }
       /* rhs0 = w^3 F0 luh (:118-133) */
       for (int j = 0; j < B; j++)
@@ -176,6 +191,21 @@ int orc_space_time_predictor(const orc_cfg* c, double* lQhbnd, double* lFhbnd,
               for (int n = 0; n < B; n++)
                 RHS(i, l, j, k, m) -= updateSize * LFI(i, n, j, k, 2, m) * KXI(n, l);
         }
+      /* algebraic source (:227-252): lFi[..][3] = S, rhs += w_t w^3 dt S */
+      if (orc_has_source(c)) {
+        for (int j = 0; j < B; j++)
+          for (int k = 0; k < B; k++)
+            for (int l = 0; l < B; l++) {
+              const double weight = W(N, i) * W(N, j) * W(N, k) * W(N, l);
+              const double updateSize = weight * dt;
+              double S[8];
+              orc_source(c, &LQI(lQi, j, k, l, i, 0), S);
+              for (int m = 0; m < nv; m++) {
+                LFI(i, j, k, l, 3, m) = S[m];
+                RHS(i, j, k, l, m) += updateSize * S[m];
+              }
+            }
+      }
     }
     /* zero the variables of lQi (:255-268) */
     for (int i = 0; i < B; i++)
@@ -208,7 +238,6 @@ int orc_space_time_predictor(const orc_cfg* c, double* lQhbnd, double* lFhbnd,
   double* lShi = lFhi + 3 * B3 * nv;
   memset(lQhi, 0, (size_t)B3 * nd * sizeof(double));
   memset(lFhi, 0, (size_t)4 * B3 * nv * sizeof(double));
-  (void)lShi;
 #define LFHI(A, a, b, cc, m) A[(((a) * B + (b)) * B + (cc)) * nv + (m)]
 #define LQHI(z, y, x, m) lQhi[(((z) * B + (y)) * B + (x)) * nd + (m)]
   for (int i = 0; i < B; i++)
@@ -219,6 +248,7 @@ int orc_space_time_predictor(const orc_cfg* c, double* lQhbnd, double* lFhbnd,
             LFHI(lFhi_x, i, j, k, l) += LFI(m, i, j, k, 0, l) * W(N, m);
             LFHI(lFhi_y, i, k, j, l) += LFI(m, i, j, k, 1, l) * W(N, m);
             LFHI(lFhi_z, j, k, i, l) += LFI(m, i, j, k, 2, l) * W(N, m);
+            if (orc_has_source(c)) LFHI(lShi, i, j, k, l) += LFI(m, i, j, k, 3, l) * W(N, m);
           }
         for (int l = 0; l < nd; l++)
           for (int m = 0; m < B; m++) LQHI(i, j, k, l) += LQI(lQi, i, j, k, m, l) * W(N, m);
@@ -315,6 +345,16 @@ void orc_volume_integral(const orc_cfg* c, double* lduh, const double* lFhi, con
           for (int m = 0; m < B; m++)
             lduh[IDX(k, i, j, l)] += KXI(k, m) * lFhi[z_off + IDX(i, j, m, l)] * updateSize;
     }
+  /* source term (volumeIntegralNonlinear.cpph, useSourceOrNCP branch): += w^3 lShi */
+  if (orc_has_source(c)) {
+    const int s_off = 3 * B3 * nv;
+    for (int i = 0; i < B; i++)
+      for (int j = 0; j < B; j++)
+        for (int k = 0; k < B; k++) {
+          const double weight = W(N, i) * W(N, j) * W(N, k);
+          for (int l = 0; l < nv; l++) lduh[IDX(i, j, k, l)] += weight * lFhi[s_off + IDX(i, j, k, l)];
+        }
+  }
 #undef IDX
 }
 

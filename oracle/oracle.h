@@ -35,13 +35,19 @@ typedef struct {
   int npar;       /* material parameters carried in luh (0 for Euler) */
   double gamma;   /* ratio of specific heats                         */
   double cfl;     /* CFL factor (0.9 for ADER-DG)                    */
-  int pde;        /* 0: 3D Euler; 1: linear advection test PDE; 2: EulerFV demonstrator (FV only) */
+  int pde;        /* 0: 3D Euler; 1: linear advection test PDE; 2: EulerFV demonstrator (FV only);
+                     3: 3D Euler with constant gravity (algebraic source, the useSource arm) */
   double adv[3];  /* advection velocity for pde==1                   */
+  double grav;    /* gravity along -z for pde==3                      */
 } orc_cfg;
 
 /* user PDE (E/tests/kernels/c/GenericEulerKernelTest3D.cpp:26-82) */
 void orc_flux(const orc_cfg* c, const double* Q, double* F /* [3][nvar] */);
 void orc_eigenvalues(const orc_cfg* c, const double* Q, int d, double* lambda);
+/* algebraic source S(Q) (the reference's fusedSource without NCP); zero
+ * except for pde==3: S = (0, 0, 0, -rho g, -m_z g) */
+void orc_source(const orc_cfg* c, const double* Q, double* S);
+int orc_has_source(const orc_cfg* c);
 
 /* ADER-DG kernels, kernels/aderdg/generic/c/3d/ */
 int orc_space_time_predictor(const orc_cfg* c, double* lQhbnd, double* lFhbnd,

@@ -20,7 +20,7 @@ def build():
 class Cfg(C.Structure):
     _fields_ = [("order", C.c_int), ("nvar", C.c_int), ("npar", C.c_int),
                 ("gamma", C.c_double), ("cfl", C.c_double), ("pde", C.c_int),
-                ("adv", C.c_double * 3)]
+                ("adv", C.c_double * 3), ("grav", C.c_double)]
 
 
 class IC(C.Structure):
@@ -60,6 +60,7 @@ def lib():
         P = C.POINTER
         L.orc_flux.argtypes = [P(Cfg), _D, _D]
         L.orc_eigenvalues.argtypes = [P(Cfg), _D, C.c_int, _D]
+        L.orc_source.argtypes = [P(Cfg), _D, _D]
         L.orc_space_time_predictor.argtypes = [P(Cfg), _D, _D, _D, _D, _D, _D, _D, _D, C.c_double]
         L.orc_space_time_predictor.restype = C.c_int
         L.orc_volume_integral.argtypes = [P(Cfg), _D, _D, _D]
@@ -118,11 +119,20 @@ def ptr(a):
     return a.ctypes.data_as(_D)
 
 
-def cfg(order, nvar=5, npar=0, gamma=1.4, cfl=0.9, pde=0, adv=(0.0, 0.0, 0.0)):
+def cfg(order, nvar=5, npar=0, gamma=1.4, cfl=0.9, pde=0, adv=(0.0, 0.0, 0.0), grav=0.0):
+    """pde 0 Euler, 1 linear advection (velocity adv), 2 EulerFV 2D, 3 Euler
+    with gravity grav along -z (algebraic source)."""
     c = Cfg(order, nvar, npar, gamma, cfl, pde)
     for i in range(3):
         c.adv[i] = adv[i]
+    c.grav = grav
     return c
+
+
+def source(c, Q):
+    S = np.zeros(c.nvar)
+    lib().orc_source(C.byref(c), ptr(dvec(Q)), ptr(S))
+    return S
 
 
 def dvec(x):

@@ -218,3 +218,50 @@ def test_vortex_convergence_order():
     e2 = _l2_error(3, 32, 1, 0.02, _vortex_ic())
     order = np.log2(e1 / e2)
     assert order > 3.5, (e1, e2, order)
+
+
+@pytest.mark.parametrize("N", [2, 3, 5])
+def test_source_arm_fixed_point_and_volume_term(N):
+    """Euler with gravity (algebraic source, the useSource arm of
+    spaceTimePredictorNonlinear.cpph:227-252): the converged iterate satisfies
+    lQi = (1/w^3) iK1 . (w^3 F0 luh - sum_d c_d Kxi^T F_d + w_t w^3 dt S(lQi)),
+    the time-averaged source is the GL-weighted sum of the source slices, and
+    the volume integral adds w^3 lShi to the flux terms."""
+    rng = np.random.default_rng(70 + N)
+    B = N + 1
+    g = 3.0
+    c = O.cfg(N, 5, 0, pde=3, grav=g)
+    luh = _euler_state(rng, B, B ** 3)
+    dx = np.array([0.05, 0.04, 0.06])
+    dt = 0.1 * O.stable_time_step_size(c, luh.ravel(), dx)
+    out = O.space_time_predictor(c, luh.ravel(), dx, dt)
+    assert out["iters"] < 2 * B
+    M = dg_math.operators(N)
+    w, K, iK1, F0 = M["gl_weights"], M["Kxi"], M["iK1"], M["F0"]
+    q = out["lQi"].reshape(B, B, B, B, 5)  # z y x t v
+    F = np.zeros((B, B, B, B, 3, 5))
+    S = np.zeros((B, B, B, B, 5))
+    for idx in np.ndindex(B, B, B, B):
+        F[idx] = O.flux(c, q[idx])
+        S[idx] = O.source(c, q[idx])
+    assert np.all(S[..., :3] == 0) and np.allclose(S[..., 3], -g * q[..., 0]) and np.allclose(S[..., 4], -g * q[..., 3])
+    w3 = np.einsum("z,y,x->zyx", w, w, w)
+    rhs = np.einsum("zyx,t,zyxv->tzyxv", w3, F0, luh.reshape(B, B, B, 5))
+    rhs -= np.einsum("t,z,y,zyntv,nl->tzylv", w * dt / dx[0], w, w, F[..., 0, :], K)
+    rhs -= np.einsum("t,z,x,zlxtv,ly->tzyxv", w * dt / dx[1], w, w, F[..., 1, :], K)
+    rhs -= np.einsum("t,y,x,lyxtv,lz->tzyxv", w * dt / dx[2], w, w, F[..., 2, :], K)
+    rhs += np.einsum("t,zyx,zyxtv->tzyxv", w * dt, w3, S)
+    q_next = np.einsum("lt,tzyxv->zyxlv", iK1, rhs) / w3[..., None, None]
+    assert np.sqrt(np.sum((q_next - q) ** 2)) < 1e-7 * 1.01
+    # without the source term the same iterate is far from a fixed point
+    rhs_nosrc = rhs - np.einsum("t,zyx,zyxtv->tzyxv", w * dt, w3, S)
+    assert np.sqrt(np.sum((np.einsum("lt,tzyxv->zyxlv", iK1, rhs_nosrc) / w3[..., None, None] - q) ** 2)) > 1e-5
+    # time-averaged source block of lFhi
+    lFi = out["lFi"].reshape(B, B, B, B, 4, 5)  # t z y x d v
+    lShi = out["lFhi"].reshape(4, B, B, B, 5)[3]
+    np.testing.assert_allclose(lShi, np.einsum("t,tzyxv->zyxv", w, lFi[..., 3, :]), rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(lFi[..., 3, :].transpose(1, 2, 3, 0, 4), S, rtol=1e-6, atol=1e-6)
+    # volume integral: flux terms + w^3 lShi
+    c0 = O.cfg(N, 5, 0)
+    d_src = O.volume_integral(c, out["lFhi"], dx) - O.volume_integral(c0, out["lFhi"], dx)
+    np.testing.assert_allclose(d_src.reshape(B, B, B, 5), w3[..., None] * lShi, rtol=1e-10, atol=1e-13)
