@@ -32,7 +32,8 @@ def euler_state(rng, n, amp=0.2):
     return np.column_stack([rho, rho * u[:, 0], rho * u[:, 1], rho * u[:, 2], E]).ravel()
 
 
-CASES = [(2, 10, 0, 2), (3, 6, 0, 0), (3, 11, 1, 0), (3, 5, 1, 0)]  # dim, patch, scheme, oracle pde
+# dim, patch, scheme, oracle pde (MUSCL-Hancock up to the 15^3 patch limit)
+CASES = [(2, 10, 0, 2), (3, 6, 0, 0), (3, 11, 1, 0), (3, 5, 1, 0), (3, 13, 1, 0)]
 
 
 @pytest.mark.parametrize("dim,patch,scheme,opde", CASES)
