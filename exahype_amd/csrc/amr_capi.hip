@@ -32,14 +32,19 @@ int amr_tables(int dev, int N, const AmrTables** out) {
   AmrCache& c = g_amr[dev];
   if (!c.lds_set) {
     const int bytes = 2 * exa::amr::AMR_LDS_DOUBLES * (int)sizeof(double);
-    EXA_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::amr::k_amr_transfer<3, true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-    EXA_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::amr::k_amr_transfer<3, false>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-    EXA_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::amr::k_amr_transfer<2, true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-    EXA_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::amr::k_amr_transfer<2, false>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    hipError_t e = hipSuccess;
+    auto setattr = [&](const void* f) {
+      if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    };
+#define EXA_AMR_ATTR(BB)                                                              \
+    setattr(reinterpret_cast<const void*>(&exa::amr::k_amr_transfer<3, true, BB>));  \
+    setattr(reinterpret_cast<const void*>(&exa::amr::k_amr_transfer<3, false, BB>)); \
+    setattr(reinterpret_cast<const void*>(&exa::amr::k_amr_transfer<2, true, BB>));  \
+    setattr(reinterpret_cast<const void*>(&exa::amr::k_amr_transfer<2, false, BB>));
+    EXA_AMR_ATTR(1) EXA_AMR_ATTR(2) EXA_AMR_ATTR(3) EXA_AMR_ATTR(4) EXA_AMR_ATTR(5)
+    EXA_AMR_ATTR(6) EXA_AMR_ATTR(7) EXA_AMR_ATTR(8) EXA_AMR_ATTR(9) EXA_AMR_ATTR(10)
+#undef EXA_AMR_ATTR
+    EXA_CHECK(e);
     c.lds_set = true;
   }
   if (!c.ready[N]) {
@@ -74,20 +79,27 @@ int launch(int dim, bool prolong, const AmrTables* tab, const Transfer* items, c
   // LDS: two buffers of one value chunk (as many values per node as fit)
   const int n = dim == 3 ? B * B * B : B * B;
   const size_t lds = 2 * sizeof(double) * (size_t)n * exa::amr::amr_chunk(B, dim, nd);
-  if (dim == 3) {
-    if (prolong)
-      hipLaunchKernelGGL((exa::amr::k_amr_transfer<3, true>), grid, block, lds, 0, tab, items, groups, fine, coarse, B,
-                         nd);
-    else
-      hipLaunchKernelGGL((exa::amr::k_amr_transfer<3, false>), grid, block, lds, 0, tab, items, groups, fine, coarse, B,
-                         nd);
-  } else {
-    if (prolong)
-      hipLaunchKernelGGL((exa::amr::k_amr_transfer<2, true>), grid, block, lds, 0, tab, items, groups, fine, coarse, B,
-                         nd);
-    else
-      hipLaunchKernelGGL((exa::amr::k_amr_transfer<2, false>), grid, block, lds, 0, tab, items, groups, fine, coarse, B,
-                         nd);
+  bool ok = false;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, grid, block, lds, 0, tab, items, groups, fine, coarse, nd);
+    ok = true;
+  };
+  switch (B) {
+#define EXA_AMR_CASE(BB)                                                                           \
+  case BB:                                                                                         \
+    if (dim == 3)                                                                                  \
+      prolong ? go(exa::amr::k_amr_transfer<3, true, BB>) : go(exa::amr::k_amr_transfer<3, false, BB>); \
+    else                                                                                           \
+      prolong ? go(exa::amr::k_amr_transfer<2, true, BB>) : go(exa::amr::k_amr_transfer<2, false, BB>); \
+    break;
+    EXA_AMR_CASE(1) EXA_AMR_CASE(2) EXA_AMR_CASE(3) EXA_AMR_CASE(4) EXA_AMR_CASE(5)
+    EXA_AMR_CASE(6) EXA_AMR_CASE(7) EXA_AMR_CASE(8) EXA_AMR_CASE(9) EXA_AMR_CASE(10)
+#undef EXA_AMR_CASE
+    default: break;
+  }
+  if (!ok) {
+    exa_set_error("AMR transfer: order %d", B - 1);
+    return EXA_ERR_UNSUPPORTED;
   }
   EXA_CHECK(hipGetLastError());
   return EXA_OK;

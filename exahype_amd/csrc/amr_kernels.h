@@ -19,7 +19,7 @@ namespace exa {
 namespace amr {
 
 constexpr int MAXB = 10;
-constexpr int THREADS = 256;
+constexpr int THREADS = 256;  // >= B^2 = 100: build_axis_ops gives every (o, i) its own thread
 
 // Per-order tables: P[sub][coarse][fine] = phi_coarse((sub + xi_fine)/3), w = GL weights.
 struct AmrTables {
@@ -85,19 +85,19 @@ __device__ inline void build_axis_ops(const AmrTables& t, bool prolong, const Tr
 
 // out[...o on axis...] = sum_i A[o][i] * in[...i on axis...] over a B^DIM block
 // of nodes (axis 0 fastest) holding vc values per node (value fastest).
-template <int DIM>
-__device__ inline void axis_pass(const double* in, double* out, const double* A, int axis, int B, int vc) {
-  int n = B * B;
-  if (DIM == 3) n *= B;
-  int stride = 1;
-  for (int a = 0; a < axis; ++a) stride *= B;
-  for (int e = threadIdx.x; e < n * vc; e += blockDim.x) {
-    const int node = e / vc, v = e - node * vc;
+template <int DIM, int B>
+__device__ inline void axis_pass(const double* in, double* out, const double* A, int axis, int vc) {
+  constexpr int n = DIM == 3 ? B * B * B : B * B;
+  const int stride = axis == 0 ? 1 : (axis == 1 ? B : B * B);
+  for (int node = threadIdx.x; node < n; node += blockDim.x) {
     const int o = (node / stride) % B;
     const int base = node - o * stride;
-    double s = 0.0;
-    for (int i = 0; i < B; ++i) s += A[o * MAXB + i] * in[(base + i * stride) * vc + v];
-    out[e] = s;
+    for (int v = 0; v < vc; ++v) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < B; ++i) s += A[o * MAXB + i] * in[(base + i * stride) * vc + v];
+      out[node * vc + v] = s;
+    }
   }
 }
 
@@ -115,17 +115,18 @@ __host__ __device__ inline int amr_chunk(int B, int DIM, int nd) {
 // overwrites the fine block (one transfer per group); restriction adds every
 // transfer of the group into their common coarse block.  Blocks are
 // [node][nd] (the reference layout); a chunk of vc values per node is loaded
-// into LDS in one pass and contracted axis by axis for all its values.
-template <int DIM, bool PROLONG>
+// into LDS in one pass and contracted axis by axis for all its values.  The
+// order B is a template argument (constant index arithmetic).
+template <int DIM, bool PROLONG, int B>
 __global__ void __launch_bounds__(THREADS) k_amr_transfer(const AmrTables* __restrict__ tab,
                                                           const Transfer* __restrict__ tr, const int* __restrict__ groups,
-                                                          double* __restrict__ fine, double* __restrict__ coarse, int B,
+                                                          double* __restrict__ fine, double* __restrict__ coarse,
                                                           int nd) {
   __shared__ double A[3 * MAXB * MAXB];
   __shared__ double tmp[MAXB * MAXB];
   extern __shared__ double sbuf[];  // 2 * n * amr_chunk(B, DIM, nd), set by the launch
   const int g = blockIdx.x;
-  const int n = DIM == 3 ? B * B * B : B * B;
+  constexpr int n = DIM == 3 ? B * B * B : B * B;
   const int vcmax = amr_chunk(B, DIM, nd);
   double* buf0 = sbuf;
   double* buf1 = sbuf + n * vcmax;
@@ -136,28 +137,39 @@ __global__ void __launch_bounds__(THREADS) k_amr_transfer(const AmrTables* __res
     double* dst = PROLONG ? fine + t.fine : coarse + t.coarse;
     for (int v0 = 0; v0 < nd; v0 += vcmax) {
       const int vc = nd - v0 < vcmax ? nd - v0 : vcmax;
-      for (int e = threadIdx.x; e < n * vc; e += blockDim.x) {
-        const int node = e / vc;
-        buf0[e] = src[(long long)node * nd + v0 + (e - node * vc)];
+      if (vc == nd) {  // whole blocks: one contiguous stream
+        for (int e = threadIdx.x; e < n * nd; e += blockDim.x) buf0[e] = src[e];
+      } else {
+        for (int node = threadIdx.x; node < n; node += blockDim.x)
+          for (int v = 0; v < vc; ++v) buf0[node * vc + v] = src[(long long)node * nd + v0 + v];
       }
       __syncthreads();
-      axis_pass<DIM>(buf0, buf1, A, 0, B, vc);
+      axis_pass<DIM, B>(buf0, buf1, A, 0, vc);
       __syncthreads();
-      axis_pass<DIM>(buf1, buf0, A + MAXB * MAXB, 1, B, vc);
+      axis_pass<DIM, B>(buf1, buf0, A + MAXB * MAXB, 1, vc);
       __syncthreads();
       const double* res = buf0;
       if (DIM == 3) {
-        axis_pass<DIM>(buf0, buf1, A + 2 * MAXB * MAXB, 2, B, vc);
+        axis_pass<DIM, B>(buf0, buf1, A + 2 * MAXB * MAXB, 2, vc);
         __syncthreads();
         res = buf1;
       }
-      for (int e = threadIdx.x; e < n * vc; e += blockDim.x) {
-        const int node = e / vc;
-        double* o = dst + (long long)node * nd + v0 + (e - node * vc);
-        if (PROLONG)
-          *o = res[e];
-        else
-          *o += res[e];
+      if (vc == nd) {
+        for (int e = threadIdx.x; e < n * nd; e += blockDim.x) {
+          if (PROLONG)
+            dst[e] = res[e];
+          else
+            dst[e] += res[e];
+        }
+      } else {
+        for (int node = threadIdx.x; node < n; node += blockDim.x)
+          for (int v = 0; v < vc; ++v) {
+            double* o = dst + (long long)node * nd + v0 + v;
+            if (PROLONG)
+              *o = res[node * vc + v];
+            else
+              *o += res[node * vc + v];
+          }
       }
       __syncthreads();
     }
