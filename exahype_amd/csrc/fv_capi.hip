@@ -29,6 +29,22 @@ struct exa_fv_solver {
   }
 };
 
+// MUSCL-Hancock kernel of a patch size (2..15 subcells per direction).
+using MusclFn = void (*)(const double*, double*, double*, unsigned long long*, PatchGeom, double, double, double,
+                         const double*, double, double);
+static MusclFn muscl_kernel(int P) {
+  switch (P) {
+#define EXA_MUSCL_CASE(PP) \
+  case PP:                 \
+    return &exa::fv::k_muscl<EulerPDE, PP>;
+    EXA_MUSCL_CASE(2) EXA_MUSCL_CASE(3) EXA_MUSCL_CASE(4) EXA_MUSCL_CASE(5) EXA_MUSCL_CASE(6) EXA_MUSCL_CASE(7)
+    EXA_MUSCL_CASE(8) EXA_MUSCL_CASE(9) EXA_MUSCL_CASE(10) EXA_MUSCL_CASE(11) EXA_MUSCL_CASE(12)
+    EXA_MUSCL_CASE(13) EXA_MUSCL_CASE(14) EXA_MUSCL_CASE(15)
+#undef EXA_MUSCL_CASE
+    default: return nullptr;
+  }
+}
+
 // LDS of the MUSCL-Hancock kernel: the half-step term of every interior subcell.
 static size_t muscl_lds_bytes(const PatchGeom& g) { return sizeof(double) * FV_V * (size_t)g.P * g.P * g.P; }
 
@@ -53,7 +69,7 @@ extern "C" int exa_fv_create(const exa_fv_config* cfg, exa_fv_solver** out) {
   s->ghost = cfg->scheme == EXA_FV_MUSCLHANCOCK ? 2 : 1;
   hipError_t e = hipSuccess;
   if (cfg->scheme == EXA_FV_MUSCLHANCOCK)
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::fv::k_muscl<EulerPDE>),
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(muscl_kernel(cfg->patch)),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)muscl_lds_bytes(s->geom()));
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -89,8 +105,8 @@ static int launch_fv_update(const exa_fv_solver* s, hipStream_t st, int np, cons
   const double gam = s->cfg.gamma;
   if (s->cfg.scheme == EXA_FV_MUSCLHANCOCK) {
     // the kernel's own CFL (kernels/finitevolumes/musclhancock/c/musclhancock.h:29)
-    hipLaunchKernelGGL((exa::fv::k_muscl<EulerPDE>), dim3(np), dim3(256), muscl_lds_bytes(g), st, in, out, dt_patch,
-                       dt_min, g, dx[0], dx[1], dx[2], dt, gam, 0.99);
+    hipLaunchKernelGGL(muscl_kernel(g.P), dim3(np), dim3(256), muscl_lds_bytes(g), st, in, out, dt_patch, dt_min, g,
+                       dx[0], dx[1], dx[2], dt, gam, 0.99);
   } else if (s->cfg.dim == 3) {
     hipLaunchKernelGGL((exa::fv::k_godunov<3, EulerPDE>), dim3(np), dim3(256), 0, st, in, out, dt_patch, dt_min, g,
                        dx[0], dx[1], dx[2], dt, gam, s->cfg.cfl);

@@ -181,11 +181,14 @@ __device__ inline void muscl_face_state(const double* u, int nbox, const PatchGe
   }
 }
 
-template <class PDE>
+// PC: subcells per direction (compile time: the index arithmetic of the
+// recomputed face states is then constant; the launch's geometry must match).
+template <class PDE, int PC>
 __global__ __launch_bounds__(256) void k_muscl(const double* __restrict__ luh, double* __restrict__ luh_new,
                                                double* __restrict__ dt_patch, unsigned long long* dt_min,
-                                               PatchGeom g, double dx0, double dx1, double dx2,
+                                               PatchGeom, double dx0, double dx1, double dx2,
                                                const double* __restrict__ dtp, double gamma, double cfl) {
+  const PatchGeom g{PC, 2, PC + 4, 3};
   extern __shared__ double sQt[];  // [v][interior subcell]
   __shared__ double red[8];
   const double dt = *dtp;
