@@ -29,6 +29,9 @@ PREC_BF16_MFMA = 3
 CONTRACT_DEFAULT, CONTRACT_PENCIL, CONTRACT_NODE, CONTRACT_PENCIL_PAIR, CONTRACT_STAGED = 0, 1, 2, 3, 4
 PDE_EULER = 0
 PDE_EULER_GRAVITY = 2
+PDE_ADVECTION = 3          # linear advection of 5 variables, flux form
+PDE_ADVECTION_NCP = 4      # the same in non-conservative form (no flux)
+PDE_EULER_NCP_TEST = 5     # Euler with the reference kernel test's non-conservative product
 PDE_EULERFV2D = 1
 FV_GODUNOV = 0
 FV_MUSCLHANCOCK = 1
@@ -40,7 +43,8 @@ _lib = None
 class AderdgConfig(C.Structure):
     _fields_ = [("order", C.c_int), ("nvar", C.c_int), ("npar", C.c_int), ("cfl", C.c_double),
                 ("gamma", C.c_double), ("pde", C.c_int), ("precision", C.c_int), ("fp64_sweeps", C.c_int),
-                ("waves_per_simd", C.c_int), ("contraction", C.c_int), ("gravity", C.c_double)]
+                ("waves_per_simd", C.c_int), ("contraction", C.c_int), ("gravity", C.c_double),
+                ("velocity", C.c_double * 3)]
 
 
 class EnsembleMember(C.Structure):
@@ -223,14 +227,18 @@ class ADERDGSolver:
     """Per-cell kernel surface of exahype::solvers::ADERDGSolver on the GPU."""
 
     def __init__(self, order, nvar=5, npar=0, cfl=0.9, gamma=1.4, precision=PREC_FP64, fp64_sweeps=1,
-                 waves_per_simd=0, contraction=0, device=0, gravity=None):
+                 waves_per_simd=0, contraction=0, device=0, gravity=None, pde=None, velocity=(0.0, 0.0, 0.0)):
         """gravity: None for the Euler equations, g for Euler with constant
-        gravity along -z (EXA_PDE_EULER_GRAVITY, an algebraic source)."""
+        gravity along -z (EXA_PDE_EULER_GRAVITY, an algebraic source).
+        pde: one of the PDE_* ids to select another compiled PDE (the
+        advection PDEs take their velocity from `velocity`)."""
         L = load_library()
         _check(L.exa_init(device))
-        pde = PDE_EULER if gravity is None else PDE_EULER_GRAVITY
+        if pde is None:
+            pde = PDE_EULER if gravity is None else PDE_EULER_GRAVITY
         self.cfg = AderdgConfig(order, nvar, npar, cfl, gamma, pde, precision, fp64_sweeps, waves_per_simd,
-                                contraction, 0.0 if gravity is None else float(gravity))
+                                contraction, 0.0 if gravity is None else float(gravity),
+                                (C.c_double * 3)(*[float(a) for a in velocity]))
         h = C.c_void_p()
         _check(L.exa_aderdg_create(C.byref(self.cfg), C.byref(h)))
         self._h = h

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libexahype_amd.so")
-SOURCES = ["capi.hip"] + ["pred_n%d.hip" % n for n in range(1, 8)] + ["pred_staged.hip"] + [
+SOURCES = ["capi.hip"] + ["pred_n%d.hip" % n for n in range(1, 8)] + ["pred_staged.hip", "pde_tu.hip"] + [
     "fv_capi.hip", "limiter_capi.hip", "amr_capi.hip", "device_capi.hip"]
 
 HIPCC_FLAGS = [

@@ -11,6 +11,7 @@
 #pragma once
 #include "common.h"
 #include "pde_euler.h"
+#include "pde_generic.h"
 #include "mfma_util.h"
 #include <type_traits>
 
@@ -114,6 +115,7 @@ struct Ens {
   InstParams* inst;
   const exa_ic* ics;  // per-instance initial conditions
   double grav;        // gravity g along -z of the Euler-with-gravity PDE; 0: no source
+  double adv[3];      // advection velocity of the linear-advection PDEs
 };
 
 // Algebraic source of Euler under gravity g along -z (the useSource arm of
@@ -1146,7 +1148,8 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
 template <int N>
 __global__ __launch_bounds__(Ord<N>::NT) void k_volume_integral_ref(const double* __restrict__ lFhi,
                                                                     double* __restrict__ lduh, double dx0,
-                                                                    double dx1, double dx2, int src) {
+                                                                    double dx1, double dx2, int src,
+                                                                    int flux) {
   using O = Ord<N>;
   constexpr int B = O::B, B3 = O::B3;
   extern __shared__ double sm[];
@@ -1165,7 +1168,12 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_volume_integral_ref(const double
   __syncthreads();
   if (act) {
     double out[V];
-    volume_integral_node<N>(sFh, s, z, y, x, dx0, dx1, dx2, out);
+    if (flux) {  // without a flux only the source part is integrated (useFlux = false)
+      volume_integral_node<N>(sFh, s, z, y, x, dx0, dx1, dx2, out);
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; ++v) out[v] = 0.0;
+    }
     if (src) {  // source term: += w^3 lShi (volumeIntegralNonlinear.cpph, useSourceOrNCP branch)
       const double weight = c_tab[N].w[z] * c_tab[N].w[y] * c_tab[N].w[x];
 #pragma unroll
@@ -1265,7 +1273,7 @@ __global__ __launch_bounds__(256) void k_riemann_list(const double* __restrict__
 // solutionUpdate.cpph:23-47, stableTimeStepSize.cpph:21-61.
 enum : int { UPD_SURFACE = 1, UPD_UPDATE = 2, UPD_DT = 4, UPD_WRITE_LDUH = 8 };
 
-template <int N, int MODE>
+template <int N, int MODE, class PDE = PdeEuler>
 __global__ __launch_bounds__(Ord<N>::NT) void k_update(double* __restrict__ luh, double* __restrict__ lduh,
                                                        const double* __restrict__ ff,
                                                        const double* __restrict__ dtp,
@@ -1330,7 +1338,7 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_update(double* __restrict__ luh,
     }
     if (MODE & UPD_DT) {
       double m[3];
-      EulerPDE::max_abs_eigenvalues3(u, gamma, m);
+      PDE::max_abs_eigenvalues3(u, PdeParams{gamma, {ens.adv[0], ens.adv[1], ens.adv[2]}}, m);
       double den = 0.0;
       den += m[0] / dx0;
       den += m[1] / dx1;

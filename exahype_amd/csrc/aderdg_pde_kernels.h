@@ -1,0 +1,335 @@
+// ADER-DG kernels of the generic PDE arms (flux / algebraic source /
+// non-conservative product), FP64, orders 1..7.
+//
+// Same device layout and summation orders as the Euler kernels
+// (aderdg_kernels.h).  The predictor keeps one thread per spatial node with
+// the current iterate and the next one in registers and passes one time
+// slice at a time through LDS: the flux slice for the Kxi contraction and,
+// for a non-conservative product, the iterate slice for the dudx gradients.
+// spaceTimePredictorNonlinear.cpph:43-577 (all three arms),
+// riemannSolverNonlinear.cpph:38-120 (useNCP), volumeIntegralNonlinear.cpph:26-125.
+#pragma once
+#include "aderdg_kernels.h"
+#include "pde_generic.h"
+
+namespace exa {
+
+constexpr int PDE_MAX_ORDER = 7;
+
+template <int N>
+struct PdeOrd {
+  // iterate slice [v][node] + flux slice [d][v][node] + reduction scratch;
+  // after the sweeps the same words hold sQh [v][node] and sFh [d][v][node]
+  static constexpr int LDS_DOUBLES = 4 * V * Ord<N>::B3 + 16;
+};
+
+template <int N, class PDE, bool DEBUG>
+__global__ __launch_bounds__(Ord<N>::NT) void k_predictor_pde(
+    const double* __restrict__ luh, double* __restrict__ lduh, double* __restrict__ qf, double* __restrict__ ff,
+    int* __restrict__ iters, const double* __restrict__ dtp, double dx0, double dx1, double dx2, double gamma,
+    unsigned long long* __restrict__ sweep_count, PredictorDebug dbg, Ens ens) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
+  constexpr int MaxIterations = 2 * (N + 1);
+  extern __shared__ double sm[];
+  double* sQ = sm;                  // iterate slice [v][node]
+  double* sF = sm + V * B3;         // flux slice [d][v][node]
+  double* red = sm + 4 * V * B3;    // residual sum scratch
+  double* sQh = sm;                 // after the sweeps: time averages
+  double* sFh = sm + V * B3;
+  const DGTables& T = c_tab[N];
+  const PdeParams P{gamma, {ens.adv[0], ens.adv[1], ens.adv[2]}};
+  const size_t cell = blockIdx.x;
+  const int s = threadIdx.x;
+  const bool act = s < B3;
+  const int z = act ? s / B2 : 0, y = act ? (s / B) % B : 0, x = act ? s % B : 0;
+  const double dt = *dtp;
+  const double w3 = (T.w[z] * T.w[y]) * T.w[x];
+  const double iweight = 1.0 / w3;
+
+  double u0[V], q[B][V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) u0[v] = act ? luh[cell * V * B3 + v * B3 + s] : 1.0;
+  // trivial initial guess (:68-78)
+#pragma unroll
+  for (int t = 0; t < B; ++t)
+#pragma unroll
+    for (int v = 0; v < V; ++v) q[t][v] = u0[v];
+
+  double fh[3][V], sh[V];
+  int it = 0;
+  for (int iter = 0; iter < MaxIterations; ++iter) {
+    ++it;
+    double qn[B][V];
+#pragma unroll
+    for (int t = 0; t < B; ++t)
+#pragma unroll
+      for (int v = 0; v < V; ++v) qn[t][v] = 0.0;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      sh[v] = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) fh[d][v] = 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < B; ++t) {
+      __syncthreads();  // the previous slice is consumed
+      if (act) {
+        if (PDE::NCP) {
+#pragma unroll
+          for (int v = 0; v < V; ++v) sQ[v * B3 + s] = q[t][v];
+        }
+        double F[3][V];
+        if (PDE::FLUX) {
+          PDE::flux(q[t], F, P);
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+              sF[(d * V + v) * B3 + s] = F[d][v];
+              fh[d][v] += F[d][v] * T.w[t];  // time average of the sweep (:326-397)
+            }
+        }
+        if (DEBUG) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int v = 0; v < V; ++v) dbg.lFi[((t * B3 + s) * 4 + d) * V + v] = PDE::FLUX ? F[d][v] : 0.0;
+        }
+      }
+      __syncthreads();
+      if (act) {
+        // rhs0 = w^3 F0 luh (:118-133)
+        double r[V];
+        const double c0 = w3 * T.F0[t];
+#pragma unroll
+        for (int v = 0; v < V; ++v) r[v] = c0 * u0[v];
+        if (PDE::FLUX) {  // rhs -= (w dt/dx) lFi Kxi per direction (:143-222)
+          const double usx = ((T.w[t] * T.w[z]) * T.w[y]) * dt / dx0;
+          const double usy = ((T.w[t] * T.w[z]) * T.w[x]) * dt / dx1;
+          const double usz = ((T.w[t] * T.w[y]) * T.w[x]) * dt / dx2;
+#pragma unroll
+          for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int n = 0; n < B; ++n) r[v] -= usx * sF[(0 * V + v) * B3 + z * B2 + y * B + n] * T.Kxi[n][x];
+#pragma unroll
+          for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int n = 0; n < B; ++n) r[v] -= usy * sF[(1 * V + v) * B3 + z * B2 + n * B + x] * T.Kxi[n][y];
+#pragma unroll
+          for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int n = 0; n < B; ++n) r[v] -= usz * sF[(2 * V + v) * B3 + n * B2 + y * B + x] * T.Kxi[n][z];
+        }
+        if (PDE::SRC || PDE::NCP) {
+          // gradients of the iterate (:136-222), fused source (:227-252)
+          double G[3][V];
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            G[0][v] = G[1][v] = G[2][v] = 0.0;
+            if (PDE::NCP) {
+#pragma unroll
+              for (int n = 0; n < B; ++n) G[0][v] += 1.0 / dx0 * sQ[v * B3 + z * B2 + y * B + n] * T.dudx[x][n];
+#pragma unroll
+              for (int n = 0; n < B; ++n) G[1][v] += 1.0 / dx1 * sQ[v * B3 + z * B2 + n * B + x] * T.dudx[y][n];
+#pragma unroll
+              for (int n = 0; n < B; ++n) G[2][v] += 1.0 / dx2 * sQ[v * B3 + n * B2 + y * B + x] * T.dudx[z][n];
+            }
+          }
+          double S[V];
+          fused_source<PDE>(q[t], G, S, P);
+          const double us = (((T.w[t] * T.w[z]) * T.w[y]) * T.w[x]) * dt;
+#pragma unroll
+          for (int v = 0; v < V; ++v) {
+            r[v] += us * S[v];
+            sh[v] += S[v] * T.w[t];
+          }
+          if (DEBUG) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) dbg.lFi[((t * B3 + s) * 4 + 3) * V + v] = S[v];
+          }
+        } else if (DEBUG) {
+#pragma unroll
+          for (int v = 0; v < V; ++v) dbg.lFi[((t * B3 + s) * 4 + 3) * V + v] = 0.0;
+        }
+        // this time node's share of the new iterate: += (w^-3 rhs) iK1 (:270-296)
+#pragma unroll
+        for (int l = 0; l < B; ++l)
+#pragma unroll
+          for (int v = 0; v < V; ++v) qn[l][v] += iweight * r[v] * T.iK1[l][t];
+      }
+    }
+    // exit condition (:299-306)
+    double res = 0.0;
+    if (act) {
+#pragma unroll
+      for (int t = 0; t < B; ++t)
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const double d = q[t][v] - qn[t][v];
+          res += d * d;
+          q[t][v] = qn[t][v];
+        }
+    }
+    res = block_sum(res, red);
+    if (res < 1e-14) break;
+  }
+
+  // time averages (:326-397) into LDS, then volume integral and extrapolation
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      double a = 0.0;
+#pragma unroll
+      for (int t = 0; t < B; ++t) a += q[t][v] * T.w[t];
+      sQh[v * B3 + s] = a;
+      if (DEBUG) {
+        dbg.lQhi[s * V + v] = a;
+#pragma unroll
+        for (int t = 0; t < B; ++t) dbg.lQi[(s * B + t) * V + v] = q[t][v];
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int v = 0; v < V; ++v) sFh[(d * V + v) * B3 + s] = PDE::FLUX ? fh[d][v] : 0.0;
+    if (DEBUG) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        dbg.lFhi[0 * B3 * V + ((z * B + y) * B + x) * V + v] = PDE::FLUX ? fh[0][v] : 0.0;
+        dbg.lFhi[1 * B3 * V + ((z * B + x) * B + y) * V + v] = PDE::FLUX ? fh[1][v] : 0.0;
+        dbg.lFhi[2 * B3 * V + ((y * B + x) * B + z) * V + v] = PDE::FLUX ? fh[2][v] : 0.0;
+        dbg.lFhi[3 * B3 * V + s * V + v] = (PDE::SRC || PDE::NCP) ? sh[v] : 0.0;
+      }
+    }
+  }
+  __syncthreads();
+  if (act) {
+    double out[V];
+    if (PDE::FLUX) {
+      volume_integral_node<N>(sFh, s, z, y, x, dx0, dx1, dx2, out);
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; ++v) out[v] = 0.0;
+    }
+    if (PDE::SRC || PDE::NCP) {  // += w^3 lShi (volumeIntegralNonlinear.cpph:106-124)
+#pragma unroll
+      for (int v = 0; v < V; ++v) out[v] += w3 * sh[v];
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) lduh[cell * V * B3 + v * B3 + s] = out[v];
+  }
+  // without a flux sFh is zero and so is lFhbnd, as the reference writes it (:418)
+  extrapolate_faces<N>(sQh, sFh, s, qf + cell * 6 * V * B2, ff + cell * 6 * V * B2);
+  if (s == 0) {
+    if (iters) iters[cell] = it;
+    if (sweep_count) atomicAdd(sweep_count, (unsigned long long)it);
+  }
+}
+
+// One face, one wavefront: Rusanov flux from the face-averaged wave speeds
+// plus, for a non-conservative product, the fluctuation ncp(Qavg, gradQ)
+// with gradQ zero except the normal row QR - QL: FL = F + ncp/2,
+// FR = F - ncp/2 (riemannSolverNonlinear.cpph:83-116).
+template <int N, class PDE>
+__device__ inline void riemann_face_pde(const double* __restrict__ QL, const double* __restrict__ QR,
+                                        double* __restrict__ FL, double* __restrict__ FR, int d,
+                                        const PdeParams& P, int lane) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2;
+  constexpr int NPL = (B2 + 63) / 64;
+  const DGTables& T = c_tab[N];
+  double aL[V], aR[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) aL[v] = aR[v] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NPL; ++k) {
+    const int node = lane + 64 * k;
+    if (node < B2) {
+      const double w = T.w[node / B] * T.w[node % B];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        aL[v] += w * QL[v * B2 + node];
+        aR[v] += w * QR[v * B2 + node];
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    aL[v] = wave_sum(aL[v]);
+    aR[v] = wave_sum(aR[v]);
+  }
+  const double sL = PDE::max_abs_eigenvalue(aL, d, P);
+  const double sR = PDE::max_abs_eigenvalue(aR, d, P);
+  const double smax = sL < sR ? sR : sL;
+#pragma unroll
+  for (int k = 0; k < NPL; ++k) {
+    const int node = lane + 64 * k;
+    if (node < B2) {
+      double ncp[V];
+      if (PDE::NCP) {
+        double G[3][V], Qa[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const int i = v * B2 + node;
+          const double jump = QR[i] - QL[i];
+          G[0][v] = d == 0 ? jump : 0.0;
+          G[1][v] = d == 1 ? jump : 0.0;
+          G[2][v] = d == 2 ? jump : 0.0;
+          Qa[v] = 0.5 * (QR[i] + QL[i]);
+        }
+        PDE::ncp(Qa, G, ncp, P);
+      }
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const int i = v * B2 + node;
+        const double F = 0.5 * (FR[i] + FL[i]) - 0.5 * smax * (QR[i] - QL[i]);
+        if (PDE::NCP) {
+          FR[i] = F - 0.5 * ncp[v];
+          FL[i] = F + 0.5 * ncp[v];
+        } else {
+          FL[i] = F;
+          FR[i] = F;
+        }
+      }
+    }
+  }
+}
+
+// Every face of a grid, one wavefront per face (left cell face 2d+1, right
+// cell face 2d; E/solvers/ADERDGSolver.cpp:2744-2825).
+template <int N, class PDE>
+__global__ __launch_bounds__(256) void k_riemann_pde(const double* __restrict__ qf, double* __restrict__ ff,
+                                                     const int* __restrict__ nbr, int ncells, double gamma,
+                                                     long long wave0, long long nwaves, Ens ens) {
+  constexpr int B2 = Ord<N>::B2;
+  const long long local = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (local >= nwaves) return;
+  const long long wave = wave0 + local;
+  const int d = (int)(wave / ncells);
+  const int cl = (int)(wave % ncells);
+  const int cr = nbr[(size_t)d * ncells + cl];
+  const PdeParams P{gamma, {ens.adv[0], ens.adv[1], ens.adv[2]}};
+  riemann_face_pde<N, PDE>(qf + ((size_t)cl * 6 + 2 * d + 1) * V * B2, qf + ((size_t)cr * 6 + 2 * d) * V * B2,
+                           ff + ((size_t)cl * 6 + 2 * d + 1) * V * B2, ff + ((size_t)cr * 6 + 2 * d) * V * B2, d, P,
+                           lane);
+}
+
+// A list of independent faces: face i reads QL = q[2i], QR = q[2i+1] and
+// updates FL = f[2i], FR = f[2i+1], normal direction dirs[i].
+template <int N, class PDE>
+__global__ __launch_bounds__(256) void k_riemann_list_pde(const double* __restrict__ q, double* __restrict__ f,
+                                                          const int* __restrict__ dirs, long long nfaces,
+                                                          double gamma, Ens ens) {
+  constexpr int B2 = Ord<N>::B2;
+  const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (i >= nfaces) return;
+  const PdeParams P{gamma, {ens.adv[0], ens.adv[1], ens.adv[2]}};
+  riemann_face_pde<N, PDE>(q + (size_t)(2 * i) * V * B2, q + (size_t)(2 * i + 1) * V * B2,
+                           f + (size_t)(2 * i) * V * B2, f + (size_t)(2 * i + 1) * V * B2, dirs[i], P,
+                           threadIdx.x & 63);
+}
+
+}  // namespace exa

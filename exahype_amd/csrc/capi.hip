@@ -137,6 +137,7 @@ extern "C" int exa_init(int device) {
       for (int j = 0; j < EXA_NB; ++j) {
         host[N].Kxi[i][j] = EXA_TAB_Kxi[N][i][j];
         host[N].iK1[i][j] = EXA_TAB_iK1[N][i][j];
+        host[N].dudx[i][j] = EXA_TAB_dudx[N][i][j];
       }
     }
   }
@@ -161,6 +162,7 @@ extern "C" int exa_init(int device) {
   EXA_CHECK(exa_pred_init_6(host, hostf));
   EXA_CHECK(exa_pred_init_7(host, hostf));
   EXA_CHECK(exa_pred_staged_init(host, hostf));
+  EXA_CHECK(exa_pde_init(host));
   EXA_CHECK(set_lds_attrs<1>());
   EXA_CHECK(set_lds_attrs<2>());
   EXA_CHECK(set_lds_attrs<3>());
@@ -219,6 +221,7 @@ static TunedVariant g_tuned[64][EXA_MAX_ORDER + 1][4];
 // default of the FP64 predictor at N = 7 (81^3 cells: 222 ms per launch
 // staged vs 339 ms register-resident, whose sweeps spill at that order).
 static bool predictor_staged(const exa_aderdg_config& c, int dev) {
+  if (exa_pde_generic(c.pde)) return false;
   if (c.order >= 8 || c.contraction == EXA_CONTRACT_STAGED) return true;
   return c.order == 7 && c.precision == EXA_PREC_FP64 && c.contraction == EXA_CONTRACT_DEFAULT &&
          g_tuned[dev & 63][7][EXA_PREC_FP64].contraction == 0;  // exa_autotune's choice wins
@@ -226,9 +229,14 @@ static bool predictor_staged(const exa_aderdg_config& c, int dev) {
 
 // Launch extras of a single-grid predictor launch: the gravity source of
 // EXA_PDE_EULER_GRAVITY (0 for the plain Euler equations).
+static bool pde_has_source_or_ncp(int pde) {
+  return pde == EXA_PDE_EULER_GRAVITY || pde == EXA_PDE_ADVECTION_NCP || pde == EXA_PDE_EULER_NCP_TEST;
+}
+
 static exa::Ens launch_extras(const exa_aderdg_config& c) {
   exa::Ens e{};
   e.grav = c.pde == EXA_PDE_EULER_GRAVITY ? c.gravity : 0.0;
+  for (int d = 0; d < 3; ++d) e.adv[d] = c.velocity[d];
   return e;
 }
 
@@ -254,8 +262,14 @@ extern "C" int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out)
   EXA_ARG(cfg->contraction != EXA_CONTRACT_STAGED ||
               (cfg->precision == EXA_PREC_FP64 && cfg->order >= EXA_STAGED_MIN_ORDER),
           "the staged predictor is FP64 and built for orders 5..9");
-  EXA_ARG((cfg->pde == EXA_PDE_EULER || cfg->pde == EXA_PDE_EULER_GRAVITY) && cfg->nvar == 5,
-          "only 3D Euler (optionally with gravity) with 5 variables is compiled in");
+  EXA_ARG((cfg->pde == EXA_PDE_EULER || cfg->pde == EXA_PDE_EULER_GRAVITY || exa_pde_generic(cfg->pde)) &&
+              cfg->nvar == 5,
+          "the compiled PDEs are 3D Euler (optionally with gravity or the test non-conservative product) "
+          "and linear advection (flux or non-conservative form), all with 5 variables");
+  EXA_ARG(!exa_pde_generic(cfg->pde) || (cfg->precision == EXA_PREC_FP64 && cfg->order <= 7 &&
+                                         cfg->contraction == EXA_CONTRACT_DEFAULT && cfg->npar == 0),
+          "the advection and non-conservative-product PDEs run the FP64 predictor at orders 1..7 "
+          "(default contraction, no parameters)");
   EXA_ARG(cfg->pde != EXA_PDE_EULER_GRAVITY || cfg->precision == EXA_PREC_FP64,
           "the gravity source runs in the FP64 predictor only");
   EXA_ARG(cfg->npar >= 0 && cfg->npar <= 3, "npar must be 0..3");
@@ -366,7 +380,12 @@ extern "C" int exa_aderdg_space_time_predictor(exa_solver* s, double* lQhbnd, do
   EXA_CHECK(hipMemcpyAsync(s->luh.p, h.data(), sizeof(double) * V * B3, hipMemcpyHostToDevice, s->stream));
   EXA_CHECK(hipMemcpyAsync(s->dt.p, &dt, sizeof(double), hipMemcpyHostToDevice, s->stream));
   const double gamma = s->cfg.gamma;
-  EXA_ORDER_SWITCH(N, {
+  if (exa_pde_generic(s->cfg.pde)) {
+    hipLaunchKernelGGL(exa_pde_predictor(s->cfg.pde, N, true), dim3(1), dim3(round_up64(B3)),
+                       (size_t)exa_pde_predictor_lds_bytes(N), s->stream, s->luh.as<double>(), s->lduh.as<double>(),
+                       s->qf.as<double>(), s->ff.as<double>(), s->iters.as<int>(), s->dt.as<double>(), dx[0], dx[1],
+                       dx[2], gamma, nullptr, dbg, launch_extras(s->cfg));
+  } else EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
     if (predictor_staged(s->cfg, s->device)) {
       EXA_TRY(s->stage.ensure(staged_slot_bytes(NN)));
@@ -431,7 +450,7 @@ extern "C" int exa_aderdg_volume_integral(exa_solver* s, double* lduh, const dou
     using O = exa::Ord<NN>;
     hipLaunchKernelGGL((exa::k_volume_integral_ref<NN>), dim3(1), dim3(O::NT), (size_t)volume_integral_lds_bytes<NN>(),
                        s->stream, s->misc.as<double>(), s->lduh.as<double>(), dx[0], dx[1], dx[2],
-                       s->cfg.pde == EXA_PDE_EULER_GRAVITY ? 1 : 0);
+                       pde_has_source_or_ncp(s->cfg.pde) ? 1 : 0, s->cfg.pde == EXA_PDE_ADVECTION_NCP ? 0 : 1);
   });
   EXA_CHECK(hipGetLastError());
   std::vector<double> h(B3 * V);
@@ -471,7 +490,11 @@ static int run_update_cell(exa_solver* s, double* luh_ref, double* lduh_ref, con
   EXA_CHECK(hipMemcpy(s->dt.p, hdt, sizeof(hdt), hipMemcpyHostToDevice));
   double* dtp = s->dt.as<double>();
   auto* dtn = reinterpret_cast<unsigned long long*>(dtp + 1);
-  EXA_ORDER_SWITCH(N, {
+  if ((MODE & exa::UPD_DT) && exa_pde_generic(s->cfg.pde)) {
+    hipLaunchKernelGGL(exa_pde_update(s->cfg.pde, N, MODE), dim3(1), dim3(round_up64(B3)), 0, s->stream,
+                       s->luh.as<double>(), s->lduh.as<double>(), s->ff.as<double>(), dtp, dtn, dx[0], dx[1], dx[2],
+                       s->cfg.gamma, s->cfg.cfl, launch_extras(s->cfg));
+  } else EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
     hipLaunchKernelGGL((exa::k_update<NN, MODE>), dim3(1), dim3(O::NT), 0, s->stream, s->luh.as<double>(),
                        s->lduh.as<double>(), s->ff.as<double>(), dtp, dtn, dx[0], dx[1], dx[2], s->cfg.gamma,
@@ -538,7 +561,11 @@ extern "C" int exa_aderdg_riemann_solver(exa_solver* s, double* FL, double* FR, 
   EXA_CHECK(hipMemcpy(s->qf.p, hq.data(), sizeof(double) * hq.size(), hipMemcpyHostToDevice));
   EXA_CHECK(hipMemcpy(s->ff.p, hf.data(), sizeof(double) * hf.size(), hipMemcpyHostToDevice));
   EXA_CHECK(hipMemcpy(s->misc.p, nbr, sizeof(nbr), hipMemcpyHostToDevice));
-  EXA_ORDER_SWITCH(N, {
+  if (exa_pde_generic(s->cfg.pde)) {
+    hipLaunchKernelGGL(exa_pde_riemann(s->cfg.pde, N), dim3(1), dim3(64), 0, s->stream, s->qf.as<double>(),
+                       s->ff.as<double>(), s->misc.as<int>(), 2, s->cfg.gamma, (long long)(2 * d), 1LL,
+                       launch_extras(s->cfg));
+  } else EXA_ORDER_SWITCH(N, {
     hipLaunchKernelGGL((exa::k_riemann<NN>), dim3(1), dim3(64), 0, s->stream, s->qf.as<double>(), s->ff.as<double>(),
                        s->misc.as<int>(), 2, s->cfg.gamma, (long long)(2 * d), 1LL, exa::Ens{});
   });
@@ -602,6 +629,7 @@ extern "C" int exa_aderdg_batch_prediction(exa_solver* s, double* lQhbnd, double
                                            const double* luh, const double dx[3], double dt) {
   EXA_ARG(s && lQhbnd && lFhbnd && lduh && luh && dx, "null pointer");
   EXA_ARG(dt > 0.0, "dt must be positive");
+  EXA_ARG(!exa_pde_generic(s->cfg.pde), "the batched entry points run the Euler PDEs");
   std::lock_guard<std::mutex> lk(s->mu);
   const int B = s->cfg.order + 1, B3 = B * B * B, nd = s->cfg.nvar + s->cfg.npar;
   const size_t off = s->bp_luh.size();
@@ -617,6 +645,7 @@ extern "C" int exa_aderdg_batch_riemann(exa_solver* s, double* FL, double* FR, c
                                         double dt, int normalNonZero) {
   EXA_ARG(s && FL && FR && QL && QR, "null pointer");
   EXA_ARG(normalNonZero >= 0 && normalNonZero < 3, "normalNonZero must be 0, 1 or 2");
+  EXA_ARG(!exa_pde_generic(s->cfg.pde), "the batched entry points run the Euler PDEs");
   std::lock_guard<std::mutex> lk(s->mu);
   const int B = s->cfg.order + 1, B2 = B * B, V = exa::V, nd = s->cfg.nvar + s->cfg.npar;
   const size_t off = s->br_q.size();
@@ -635,6 +664,7 @@ extern "C" int exa_aderdg_batch_riemann(exa_solver* s, double* FL, double* FR, c
 extern "C" int exa_aderdg_batch_update(exa_solver* s, double* luh, double* lduh, const double* lFhbnd,
                                        const double dx[3], double dt) {
   EXA_ARG(s && luh && lduh && lFhbnd && dx, "null pointer");
+  EXA_ARG(!exa_pde_generic(s->cfg.pde), "the batched entry points run the Euler PDEs");
   std::lock_guard<std::mutex> lk(s->mu);
   const int B = s->cfg.order + 1, B2 = B * B, B3 = B2 * B, V = exa::V, nd = s->cfg.nvar + s->cfg.npar;
   const size_t off = s->bu_luh.size(), offf = s->bu_ff.size();
@@ -997,6 +1027,14 @@ extern "C" int exa_grid_set_initial_conditions(exa_grid* g, const exa_ic* ic) {
 template <int MODE>
 static int launch_update(exa_grid* g) {
   const int N = g->s->cfg.order;
+  if (exa_pde_generic(g->s->cfg.pde)) {
+    hipLaunchKernelGGL(exa_pde_update(g->s->cfg.pde, N, MODE), dim3((unsigned)g->nc), dim3(round_up64((N + 1) * (N + 1) * (N + 1))),
+                       0, g->stream, g->luh.as<double>(), g->lduh.as<double>(), g->ff.as<double>(), g->dt_cur(),
+                       g->dt_next(), g->dx[0], g->dx[1], g->dx[2], g->s->cfg.gamma, g->s->cfg.cfl,
+                       launch_extras(g->s->cfg));
+    EXA_CHECK(hipGetLastError());
+    return EXA_OK;
+  }
   EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
     hipLaunchKernelGGL((exa::k_update<NN, MODE>), dim3((unsigned)g->nc), dim3(O::NT), 0, g->stream,
@@ -1059,7 +1097,14 @@ static int enqueue_step(exa_grid* g, bool timed) {
   EXA_TRY(tstart(0));
   exa::PredictorDebug nodbg;
   nodbg.lQi = nodbg.lFi = nodbg.lQhi = nodbg.lFhi = nullptr;
-  EXA_ORDER_SWITCH(N, {
+  const bool generic = exa_pde_generic(g->s->cfg.pde);
+  if (generic) {
+    hipLaunchKernelGGL(exa_pde_predictor(g->s->cfg.pde, N, false), dim3((unsigned)g->nc),
+                       dim3(round_up64((N + 1) * (N + 1) * (N + 1))), (size_t)exa_pde_predictor_lds_bytes(N),
+                       g->stream, g->luh.as<double>(), g->lduh.as<double>(), g->qf.as<double>(), g->ff.as<double>(),
+                       g->iters.as<int>(), g->dt_cur(), g->dx[0], g->dx[1], g->dx[2], g->s->cfg.gamma,
+                       g->iter_sum(), nodbg, launch_extras(g->s->cfg));
+  } else EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
     if (g->staged) {
       hipLaunchKernelGGL(exa_pred_staged_kernel(NN, false, g->s->cfg.pde == EXA_PDE_EULER_GRAVITY), dim3((unsigned)g->nslots), dim3(O::NT),
@@ -1083,7 +1128,11 @@ static int enqueue_step(exa_grid* g, bool timed) {
   EXA_TRY(tstart(1));
   const long long waves = 3LL * g->nc;
   const unsigned blocks = (unsigned)((waves * 64 + 255) / 256);
-  EXA_ORDER_SWITCH(N, {
+  if (generic) {
+    hipLaunchKernelGGL(exa_pde_riemann(g->s->cfg.pde, N), dim3(blocks), dim3(256), 0, g->stream, g->qf.as<double>(),
+                       g->ff.as<double>(), g->nbr.as<int>(), (int)g->nc, g->s->cfg.gamma, 0LL, waves,
+                       launch_extras(g->s->cfg));
+  } else EXA_ORDER_SWITCH(N, {
     hipLaunchKernelGGL((exa::k_riemann<NN>), dim3(blocks), dim3(256), 0, g->stream, g->qf.as<double>(),
                        g->ff.as<double>(), g->nbr.as<int>(), (int)g->nc, g->s->cfg.gamma, 0LL, waves, exa::Ens{});
   });
@@ -1236,7 +1285,7 @@ extern "C" int exa_ensemble_create(const exa_ensemble_member* m, int n, const ex
                                    exa_ensemble** out) {
   EXA_ARG(m && n > 0 && cfg && out, "arguments");
   EXA_ARG(cfg->contraction != EXA_CONTRACT_STAGED, "ensembles run the register-resident predictor forms");
-  EXA_ARG(cfg->pde != EXA_PDE_EULER_GRAVITY, "ensemble members solve the Euler equations without sources");
+  EXA_ARG(cfg->pde == EXA_PDE_EULER, "ensemble members solve the Euler equations without sources");
   size_t need = 0;
   for (int i = 0; i < n; ++i) {
     EXA_ARG(m[i].order >= 1 && m[i].order <= 7, "member order must be 1..7");

@@ -105,6 +105,14 @@ __host__ __device__ inline void exa_ic_point(const exa_ic& ic, double gamma, con
     for (int v = 0; v < 5; ++v) Q[v] = ic.q0[v];
     return;
   }
+  if (ic.kind == 3) {  // sine modes of the linear-advection PDEs, advected with uinf
+    double s = 0.0;
+    for (int m = 0; m < ic.nmodes; ++m)
+      s += ic.a[m] * sin(2.0 * pi * (ic.k[m][0] * (x[0] - ic.uinf[0] * t) + ic.k[m][1] * (x[1] - ic.uinf[1] * t) +
+                                     ic.k[m][2] * (x[2] - ic.uinf[2] * t)) + ic.phi[m]);
+    for (int v = 0; v < 5; ++v) Q[v] = 1.0 + (v + 1) * s;
+    return;
+  }
   if (ic.kind == 5) {
     const double d0 = x[0] - ic.xc[0], d1 = x[1] - ic.xc[1], d2 = x[2] - ic.xc[2];
     rho = ic.rho_inf + ic.amp * exp(-(d0 * d0 + d1 * d1 + d2 * d2) / ic.L);

@@ -1,0 +1,103 @@
+// Kernels of the generic PDE arms: linear advection in flux and in
+// non-conservative form, Euler with the test non-conservative product.
+#include "aderdg_pde_kernels.h"
+#include "predictor_tu.h"
+
+namespace {
+template <class PDE, int N>
+PdePredFn pred(bool debug) {
+  return debug ? &exa::k_predictor_pde<N, PDE, true> : &exa::k_predictor_pde<N, PDE, false>;
+}
+template <class PDE, int N>
+PdeUpdateFn upd(int mode) {
+  constexpr int FULL = exa::UPD_SURFACE | exa::UPD_UPDATE | exa::UPD_DT;
+  if (mode == exa::UPD_DT) return &exa::k_update<N, exa::UPD_DT, PDE>;
+  if (mode == FULL) return &exa::k_update<N, FULL, PDE>;
+  return nullptr;
+}
+
+// F(PDE, N) for the PDE of an id and an order 1..7
+#define EXA_PDE_DISPATCH(pde, order, F)                                     \
+  do {                                                                      \
+    switch (pde) {                                                          \
+      case EXA_PDE_ADVECTION: EXA_PDE_ORDERS(exa::PdeAdvection, order, F); break; \
+      case EXA_PDE_ADVECTION_NCP: EXA_PDE_ORDERS(exa::PdeAdvectionNcp, order, F); break; \
+      case EXA_PDE_EULER_NCP_TEST: EXA_PDE_ORDERS(exa::PdeEulerNcpTest, order, F); break; \
+      default: break;                                                       \
+    }                                                                       \
+  } while (0)
+#define EXA_PDE_ORDERS(P, order, F)          \
+  switch (order) {                           \
+    case 1: F(P, 1); break;                  \
+    case 2: F(P, 2); break;                  \
+    case 3: F(P, 3); break;                  \
+    case 4: F(P, 4); break;                  \
+    case 5: F(P, 5); break;                  \
+    case 6: F(P, 6); break;                  \
+    case 7: F(P, 7); break;                  \
+    default: break;                          \
+  }
+}  // namespace
+
+bool exa_pde_generic(int pde) {
+  return pde == EXA_PDE_ADVECTION || pde == EXA_PDE_ADVECTION_NCP || pde == EXA_PDE_EULER_NCP_TEST;
+}
+
+PdePredFn exa_pde_predictor(int pde, int order, bool debug) {
+  PdePredFn f = nullptr;
+#define F_(P, N) f = pred<P, N>(debug)
+  EXA_PDE_DISPATCH(pde, order, F_);
+#undef F_
+  return f;
+}
+
+int exa_pde_predictor_lds_bytes(int order) {
+  switch (order) {
+    case 1: return exa::PdeOrd<1>::LDS_DOUBLES * (int)sizeof(double);
+    case 2: return exa::PdeOrd<2>::LDS_DOUBLES * (int)sizeof(double);
+    case 3: return exa::PdeOrd<3>::LDS_DOUBLES * (int)sizeof(double);
+    case 4: return exa::PdeOrd<4>::LDS_DOUBLES * (int)sizeof(double);
+    case 5: return exa::PdeOrd<5>::LDS_DOUBLES * (int)sizeof(double);
+    case 6: return exa::PdeOrd<6>::LDS_DOUBLES * (int)sizeof(double);
+    case 7: return exa::PdeOrd<7>::LDS_DOUBLES * (int)sizeof(double);
+    default: return 0;
+  }
+}
+
+PdeRiemannFn exa_pde_riemann(int pde, int order) {
+  PdeRiemannFn f = nullptr;
+#define F_(P, N) f = &exa::k_riemann_pde<N, P>
+  EXA_PDE_DISPATCH(pde, order, F_);
+#undef F_
+  return f;
+}
+
+PdeRiemannListFn exa_pde_riemann_list(int pde, int order) {
+  PdeRiemannListFn f = nullptr;
+#define F_(P, N) f = &exa::k_riemann_list_pde<N, P>
+  EXA_PDE_DISPATCH(pde, order, F_);
+#undef F_
+  return f;
+}
+
+PdeUpdateFn exa_pde_update(int pde, int order, int mode) {
+  PdeUpdateFn f = nullptr;
+#define F_(P, N) f = upd<P, N>(mode)
+  EXA_PDE_DISPATCH(pde, order, F_);
+#undef F_
+  return f;
+}
+
+hipError_t exa_pde_init(const DGTables* tab) {
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab), tab, sizeof(DGTables) * (EXA_MAX_ORDER + 1));
+  if (e != hipSuccess) return e;
+  const int pdes[3] = {EXA_PDE_ADVECTION, EXA_PDE_ADVECTION_NCP, EXA_PDE_EULER_NCP_TEST};
+  for (int pde : pdes)
+    for (int order = 1; order <= exa::PDE_MAX_ORDER; ++order)
+      for (int dbg = 0; dbg < 2; ++dbg) {
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(exa_pde_predictor(pde, order, dbg)),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, exa_pde_predictor_lds_bytes(order));
+        if (e != hipSuccess) return e;
+      }
+  return hipSuccess;
+}

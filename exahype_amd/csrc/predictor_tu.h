@@ -42,3 +42,22 @@ using StagedFn = void (*)(const double*, double*, double*, double*, int*, const 
 StagedFn exa_pred_staged_kernel(int order, bool debug, bool src = false);  // src: gravity source
 int exa_pred_staged_lds_bytes(int order);
 hipError_t exa_pred_staged_init(const DGTables* tab, const DGTablesF* tabf);
+
+// Kernels of the generic PDE arms (pde_tu.hip): EXA_PDE_ADVECTION,
+// EXA_PDE_ADVECTION_NCP, EXA_PDE_EULER_NCP_TEST at orders 1..7; nullptr for
+// any other PDE or order.
+using PdePredFn = void (*)(const double*, double*, double*, double*, int*, const double*, double, double, double,
+                           double, unsigned long long*, exa::PredictorDebug, exa::Ens);
+using PdeRiemannFn = void (*)(const double*, double*, const int*, int, double, long long, long long, exa::Ens);
+using PdeRiemannListFn = void (*)(const double*, double*, const int*, long long, double, exa::Ens);
+using PdeUpdateFn = void (*)(double*, double*, const double*, const double*, unsigned long long*, double, double,
+                             double, double, double, exa::Ens);
+bool exa_pde_generic(int pde);
+PdePredFn exa_pde_predictor(int pde, int order, bool debug);
+int exa_pde_predictor_lds_bytes(int order);
+PdeRiemannFn exa_pde_riemann(int pde, int order);
+PdeRiemannListFn exa_pde_riemann_list(int pde, int order);
+// modes with UPD_DT (the only ones that depend on the PDE): UPD_DT and
+// UPD_SURFACE | UPD_UPDATE | UPD_DT
+PdeUpdateFn exa_pde_update(int pde, int order, int mode);
+hipError_t exa_pde_init(const DGTables* tab);

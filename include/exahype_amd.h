@@ -55,6 +55,18 @@ extern "C" {
                                     source S = (0, 0, 0, -rho g, -m_z g), the useSource arm of
                                     spaceTimePredictorNonlinear.cpph:227-252 and of
                                     volumeIntegralNonlinear.cpph (FP64 predictor only) */
+/* Linear advection of 5 variables with the velocity of the config, in the
+ * flux form F_d = a_d Q and in the non-conservative form (no flux,
+ * nonConservativeProduct = a . grad Q): the useFlux = false / useNCP = true
+ * arms of spaceTimePredictorNonlinear.cpph:42-252 and
+ * riemannSolverNonlinear.cpph:95-113.  FP64, orders 1..7. */
+#define EXA_PDE_ADVECTION 3
+#define EXA_PDE_ADVECTION_NCP 4
+/* 3D Euler with the test non-conservative product of the reference's Euler
+ * kernel test, BgradQ = (Q0, Q3, 3, dQ0/dx, 0.7)
+ * (E/tests/kernels/c/GenericEulerKernelTest3D.cpp:84-106): the useNCP arm on
+ * a nonlinear flux.  FP64, orders 1..7. */
+#define EXA_PDE_EULER_NCP_TEST 5
 
 const char* exa_last_error(void);
 
@@ -103,6 +115,7 @@ typedef struct {
   int waves_per_simd; /* predictor occupancy target: 0 = tuned default per order, 1 or 2 */
   int contraction;  /* EXA_CONTRACT_*: form of the sweeps' spatial contraction */
   double gravity;   /* g of EXA_PDE_EULER_GRAVITY (ignored otherwise)          */
+  double velocity[3]; /* advection velocity of EXA_PDE_ADVECTION(_NCP) (ignored otherwise) */
 } exa_aderdg_config;
 
 #define EXA_CONTRACT_DEFAULT 0  /* tuned per order (the staged form for FP64 at N = 7)    */
@@ -162,7 +175,9 @@ int exa_aderdg_stable_time_step_size(exa_solver* s, const double* luh, const dou
  * kernels/aderdg/generic/c/3d/solutionAdjustment.cpph:17-52). */
 typedef struct {
   int kind;              /* 0 isentropic vortex + density modes, 1 Sod sphere,
-                            2 constant state, 4 entropy wave,
+                            2 constant state, 3 sine modes Q_v = 1 + (v+1) sum_m a_m
+                            sin(2 pi k_m.(x - uinf t) + phi_m) (linear-advection PDEs),
+                            4 entropy wave,
                             5 Gaussian density pulse rho_inf + amp exp(-|x-xc|^2/L) */
   double xc[3];          /* vortex / sphere centre                       */
   double L, eps;         /* vortex length scale and strength             */

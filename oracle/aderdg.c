@@ -20,6 +20,10 @@ void orc_flux(const orc_cfg* c, const double* Q, double* F) {
   double* f = F;
   double* g = F + nv;
   double* h = F + 2 * nv;
+  if (c->pde == 4) { /* advection in non-conservative form: no flux */
+    for (int v = 0; v < 3 * nv; ++v) F[v] = 0.0;
+    return;
+  }
   if (c->pde == 1) { /* linear advection test PDE: F_d = a_d Q */
     for (int v = 0; v < nv; ++v) {
       f[v] = c->adv[0] * Q[v];
@@ -50,7 +54,7 @@ void orc_flux(const orc_cfg* c, const double* Q, double* F) {
 
 /* E/tests/kernels/c/GenericEulerKernelTest3D.cpp:65-82 */
 void orc_eigenvalues(const orc_cfg* c, const double* Q, int d, double* lambda) {
-  if (c->pde == 1) {
+  if (c->pde == 1 || c->pde == 4) {
     for (int v = 0; v < c->nvar; ++v) lambda[v] = c->adv[d];
     return;
   }
@@ -79,6 +83,44 @@ void orc_source(const orc_cfg* c, const double* Q, double* S) {
   }
 }
 
+/* The feature arms of the generic kernels (the useFlux / useNCP template
+ * arguments of spaceTimePredictorNonlinear.cpph:42, riemannSolverNonlinear.cpph:37
+ * and volumeIntegralNonlinear.cpph:26): pde 4 has no flux, pdes 4 and 5 a
+ * non-conservative product. */
+int orc_has_flux(const orc_cfg* c) { return c->pde != 4; }
+int orc_has_ncp(const orc_cfg* c) { return c->pde == 4 || c->pde == 5; }
+
+/* nonConservativeProduct(Q, gradQ, BgradQ), gradQ[d][v] (d = x, y, z).
+ * pde 4: B_d = a_d I, BgradQ = a_x dQ/dx + a_y dQ/dy + a_z dQ/dz.
+ * pde 5: the test product of the reference's Euler kernel test,
+ * E/tests/kernels/c/GenericEulerKernelTest3D.cpp:84-106 (the
+ * _setNcpAndMatrixBToZero == false branch): (Q0, Q3, 3, gradQ[0][0], 0.7). */
+void orc_ncp(const orc_cfg* c, const double* Q, const double* gradQ, double* BgradQ) {
+  const int nv = c->nvar;
+  for (int v = 0; v < nv; ++v) BgradQ[v] = 0.0;
+  if (c->pde == 4) {
+    for (int v = 0; v < nv; ++v)
+      BgradQ[v] = c->adv[0] * gradQ[0 * nv + v] + c->adv[1] * gradQ[1 * nv + v] + c->adv[2] * gradQ[2 * nv + v];
+  } else if (c->pde == 5) {
+    BgradQ[0] = Q[0];
+    BgradQ[1] = Q[3];
+    BgradQ[2] = 3.0;
+    BgradQ[3] = gradQ[0];
+    BgradQ[4] = 0.7;
+  }
+}
+
+/* kernels::fusedSource (kernels/fusedMethods.cpph:17-32): S = algebraicSource(Q)
+ * (or 0) minus nonConservativeProduct(Q, gradQ). */
+void orc_fused_source(const orc_cfg* c, const double* Q, const double* gradQ, double* S) {
+  orc_source(c, Q, S);
+  if (orc_has_ncp(c)) {
+    double ncp[16];
+    orc_ncp(c, Q, gradQ, ncp);
+    for (int l = 0; l < c->nvar; l++) S[l] -= ncp[l];
+  }
+}
+
 /* per-thread scratch of the predictor, sized for N=9 and nData<=8 */
 static _Thread_local double* tl_lQi;
 static _Thread_local double* tl_lQi_old;
@@ -86,6 +128,7 @@ static _Thread_local double* tl_rhs;
 static _Thread_local double* tl_lFi;
 static _Thread_local double* tl_lQhi;
 static _Thread_local double* tl_lFhi;
+static _Thread_local double* tl_gradQ;
 
 static void ensure_scratch(void) {
   if (tl_lQi) return;
@@ -96,10 +139,11 @@ static void ensure_scratch(void) {
   tl_lFi = (double*)calloc(b4 * 4 * 8, sizeof(double));
   tl_lQhi = (double*)calloc(MAXB * MAXB * MAXB * 8, sizeof(double));
   tl_lFhi = (double*)calloc(MAXB * MAXB * MAXB * 4 * 8, sizeof(double));
+  tl_gradQ = (double*)calloc(MAXB * MAXB * MAXB * 3 * 8, sizeof(double));
 }
 
 /*
- * spaceTimePredictorNonlinear<useSource=false,useFlux=true,useNCP=false>
+ * spaceTimePredictorNonlinear<useSource,useFlux,useNCP> (the arms the PDE uses)
  * kernels/aderdg/generic/c/3d/spaceTimePredictorNonlinear.cpph:524-577
  * with aderPicardLoopNonlinear (:43-317), aderPredictorNonlinear (:326-397)
  * and aderExtrapolatorNonlinear (:400-519).  lQi/lFi may be NULL (then
@@ -124,8 +168,13 @@ int orc_space_time_predictor(const orc_cfg* c, double* lQhbnd, double* lFhbnd,
 #define LFI(t, z, y, x, d, m) lFi[(((((t) * B + (z)) * B + (y)) * B + (x)) * 4 + (d)) * nv + (m)]
   const double* Kxi = &EXA_TAB_Kxi[N][0][0]; /* Kxi[a][b] = Kxi[a*10+b] */
   const double* iK1 = &EXA_TAB_iK1[N][0][0];
+  const double* dudx = &EXA_TAB_dudx[N][0][0];
+  const int has_flux = orc_has_flux(c), has_ncp = orc_has_ncp(c);
+  double* gradQ = tl_gradQ;
 #define KXI(a, b) Kxi[(a) * MAXB + (b)]
 #define IK1(a, b) iK1[(a) * MAXB + (b)]
+#define DUDX(a, b) dudx[(a) * MAXB + (b)]
+#define GQ(z, y, x, d, m) gradQ[((((z) * B + (y)) * B + (x)) * 3 + (d)) * nv + (m)]
 
   /* 1. trivial initial guess (:68-78) */
   for (int i = 0; i < B; i++)
@@ -142,12 +191,13 @@ int orc_space_time_predictor(const orc_cfg* c, double* lQhbnd, double* lFhbnd,
     executed++;
     memcpy(lQi_old, lQi, (size_t)B4 * nd * sizeof(double));
     for (int i = 0; i < B; i++) { /* time DOF */
-      /* fluxes (:96-115) */
+      /* fluxes (:96-115); a PDE without flux leaves the flux slots at zero
+       * (the reference leaves them unwritten) */
       for (int j = 0; j < B; j++)
         for (int k = 0; k < B; k++)
           for (int l = 0; l < B; l++) {
-            double F[3 * 8];
-            orc_flux(c, &LQI(lQi, j, k, l, i, 0), F);
+            double F[3 * 8] = {0.0};
+            if (has_flux) orc_flux(c, &LQI(lQi, j, k, l, i, 0), F);
             for (int d = 0; d < 3; ++d)
               for (int m = 0; m < nv; ++m) LFI(i, j, k, l, d, m) = F[d * nv + m];
             /* the source slot is written below when the PDE has one; zero otherwise */
@@ -161,15 +211,19 @@ int orc_space_time_predictor(const orc_cfg* c, double* lQhbnd, double* lFhbnd,
             for (int m = 0; m < nv; m++)
               RHS(i, j, k, l, m) = weight * EXA_TAB_F0[N][i] * LUH(j, k, l, m);
           }
-      /* x direction (:143-166) */
+      /* gradients of the iterate for the non-conservative product (:136-139) */
+      if (has_ncp) memset(gradQ, 0, (size_t)B3 * 3 * nv * sizeof(double));
+      /* x direction (:143-166); gradQ[z][y][x][d][m] of this time node */
       for (int j = 0; j < B; j++)
         for (int k = 0; k < B; k++) {
           const double weight = W(N, i) * W(N, j) * W(N, k);
           const double updateSize = weight * dt / dx[0];
           for (int l = 0; l < B; l++)
             for (int m = 0; m < nv; m++)
-              for (int n = 0; n < B; n++)
-                RHS(i, j, k, l, m) -= updateSize * LFI(i, j, k, n, 0, m) * KXI(n, l);
+              for (int n = 0; n < B; n++) {
+                if (has_flux) RHS(i, j, k, l, m) -= updateSize * LFI(i, j, k, n, 0, m) * KXI(n, l);
+                if (has_ncp) GQ(j, k, l, 0, m) += 1.0 / dx[0] * LQI(lQi, j, k, n, i, m) * DUDX(l, n);
+              }
         }
       /* y direction (:169-194) */
       for (int j = 0; j < B; j++)
@@ -178,8 +232,10 @@ int orc_space_time_predictor(const orc_cfg* c, double* lQhbnd, double* lFhbnd,
           const double updateSize = weight * dt / dx[1];
           for (int l = 0; l < B; l++)
             for (int m = 0; m < nv; m++)
-              for (int n = 0; n < B; n++)
-                RHS(i, j, l, k, m) -= updateSize * LFI(i, j, n, k, 1, m) * KXI(n, l);
+              for (int n = 0; n < B; n++) {
+                if (has_flux) RHS(i, j, l, k, m) -= updateSize * LFI(i, j, n, k, 1, m) * KXI(n, l);
+                if (has_ncp) GQ(j, l, k, 1, m) += 1.0 / dx[1] * LQI(lQi, j, n, k, i, m) * DUDX(l, n);
+              }
         }
       /* z direction (:197-222) */
       for (int j = 0; j < B; j++)
@@ -188,18 +244,20 @@ int orc_space_time_predictor(const orc_cfg* c, double* lQhbnd, double* lFhbnd,
           const double updateSize = weight * dt / dx[2];
           for (int l = 0; l < B; l++)
             for (int m = 0; m < nv; m++)
-              for (int n = 0; n < B; n++)
-                RHS(i, l, j, k, m) -= updateSize * LFI(i, n, j, k, 2, m) * KXI(n, l);
+              for (int n = 0; n < B; n++) {
+                if (has_flux) RHS(i, l, j, k, m) -= updateSize * LFI(i, n, j, k, 2, m) * KXI(n, l);
+                if (has_ncp) GQ(l, j, k, 2, m) += 1.0 / dx[2] * LQI(lQi, n, j, k, i, m) * DUDX(l, n);
+              }
         }
-      /* algebraic source (:227-252): lFi[..][3] = S, rhs += w_t w^3 dt S */
-      if (orc_has_source(c)) {
+      /* fused source (:227-252): lFi[..][3] = S = source - ncp, rhs += w_t w^3 dt S */
+      if (orc_has_source(c) || has_ncp) {
         for (int j = 0; j < B; j++)
           for (int k = 0; k < B; k++)
             for (int l = 0; l < B; l++) {
               const double weight = W(N, i) * W(N, j) * W(N, k) * W(N, l);
               const double updateSize = weight * dt;
               double S[8];
-              orc_source(c, &LQI(lQi, j, k, l, i, 0), S);
+              orc_fused_source(c, &LQI(lQi, j, k, l, i, 0), &GQ(j, k, l, 0, 0), S);
               for (int m = 0; m < nv; m++) {
                 LFI(i, j, k, l, 3, m) = S[m];
                 RHS(i, j, k, l, m) += updateSize * S[m];
@@ -245,10 +303,12 @@ int orc_space_time_predictor(const orc_cfg* c, double* lQhbnd, double* lFhbnd,
       for (int k = 0; k < B; k++) {
         for (int l = 0; l < nv; l++)
           for (int m = 0; m < B; m++) {
-            LFHI(lFhi_x, i, j, k, l) += LFI(m, i, j, k, 0, l) * W(N, m);
-            LFHI(lFhi_y, i, k, j, l) += LFI(m, i, j, k, 1, l) * W(N, m);
-            LFHI(lFhi_z, j, k, i, l) += LFI(m, i, j, k, 2, l) * W(N, m);
-            if (orc_has_source(c)) LFHI(lShi, i, j, k, l) += LFI(m, i, j, k, 3, l) * W(N, m);
+            if (has_flux) {
+              LFHI(lFhi_x, i, j, k, l) += LFI(m, i, j, k, 0, l) * W(N, m);
+              LFHI(lFhi_y, i, k, j, l) += LFI(m, i, j, k, 1, l) * W(N, m);
+              LFHI(lFhi_z, j, k, i, l) += LFI(m, i, j, k, 2, l) * W(N, m);
+            }
+            if (orc_has_source(c) || has_ncp) LFHI(lShi, i, j, k, l) += LFI(m, i, j, k, 3, l) * W(N, m);
           }
         for (int l = 0; l < nd; l++)
           for (int m = 0; m < B; m++) LQHI(i, j, k, l) += LQI(lQi, i, j, k, m, l) * W(N, m);
@@ -261,9 +321,10 @@ int orc_space_time_predictor(const orc_cfg* c, double* lQhbnd, double* lFhbnd,
   const double* FR = EXA_TAB_FRCoeff[N];
 #define QB(f, a, b, m) lQhbnd[(((f) * B + (a)) * B + (b)) * nd + (m)]
 #define FB(f, a, b, m) lFhbnd[(((f) * B + (a)) * B + (b)) * nv + (m)]
+  /* without a flux the reference zeroes lFhbnd (:418) */
   for (int i = 0; i < B; i++)
     for (int j = 0; j < B; j++) {
-      for (int k = 0; k < nv; k++)
+      for (int k = 0; k < nv && has_flux; k++)
         for (int l = 0; l < B; l++) {
           FB(0, i, j, k) += LFHI(lFhi_x, i, j, l, k) * FL[l];
           FB(1, i, j, k) += LFHI(lFhi_x, i, j, l, k) * FR[l];
@@ -276,7 +337,7 @@ int orc_space_time_predictor(const orc_cfg* c, double* lQhbnd, double* lFhbnd,
     }
   for (int i = 0; i < B; i++)
     for (int j = 0; j < B; j++) {
-      for (int k = 0; k < nv; k++)
+      for (int k = 0; k < nv && has_flux; k++)
         for (int l = 0; l < B; l++) {
           FB(2, i, j, k) += LFHI(lFhi_y, i, j, l, k) * FL[l];
           FB(3, i, j, k) += LFHI(lFhi_y, i, j, l, k) * FR[l];
@@ -289,7 +350,7 @@ int orc_space_time_predictor(const orc_cfg* c, double* lQhbnd, double* lFhbnd,
     }
   for (int i = 0; i < B; i++)
     for (int j = 0; j < B; j++) {
-      for (int k = 0; k < nv; k++)
+      for (int k = 0; k < nv && has_flux; k++)
         for (int l = 0; l < B; l++) {
           FB(4, i, j, k) += LFHI(lFhi_z, i, j, l, k) * FL[l];
           FB(5, i, j, k) += LFHI(lFhi_z, i, j, l, k) * FR[l];
@@ -307,9 +368,11 @@ int orc_space_time_predictor(const orc_cfg* c, double* lQhbnd, double* lFhbnd,
 #undef LFI
 #undef QB
 #undef FB
+#undef GQ
+#undef DUDX
 }
 
-/* volumeIntegralNonlinear<useSourceOrNCP=false,useFlux=true>,
+/* volumeIntegralNonlinear<useSourceOrNCP,useFlux>,
  * kernels/aderdg/generic/c/3d/volumeIntegralNonlinear.cpph:26-125 */
 void orc_volume_integral(const orc_cfg* c, double* lduh, const double* lFhi, const double dx[3]) {
   const int N = c->order, B = N + 1, nv = c->nvar;
@@ -318,7 +381,7 @@ void orc_volume_integral(const orc_cfg* c, double* lduh, const double* lFhi, con
 #define IDX(a, b, cc, m) ((((a) * B + (b)) * B + (cc)) * nv + (m))
   memset(lduh, 0, (size_t)B3 * nv * sizeof(double));
   const int x_off = 0, y_off = B3 * nv, z_off = 2 * B3 * nv;
-  for (int i = 0; i < B; i++)
+  for (int i = 0; i < B && orc_has_flux(c); i++)
     for (int j = 0; j < B; j++) {
       const double weight = W(N, i) * W(N, j);
       const double updateSize = weight / dx[0];
@@ -327,7 +390,7 @@ void orc_volume_integral(const orc_cfg* c, double* lduh, const double* lFhi, con
           for (int m = 0; m < B; m++)
             lduh[IDX(i, j, k, l)] += KXI(k, m) * lFhi[x_off + IDX(i, j, m, l)] * updateSize;
     }
-  for (int i = 0; i < B; i++)
+  for (int i = 0; i < B && orc_has_flux(c); i++)
     for (int j = 0; j < B; j++) {
       const double weight = W(N, i) * W(N, j);
       const double updateSize = weight / dx[1];
@@ -336,7 +399,7 @@ void orc_volume_integral(const orc_cfg* c, double* lduh, const double* lFhi, con
           for (int m = 0; m < B; m++)
             lduh[IDX(i, k, j, l)] += KXI(k, m) * lFhi[y_off + IDX(i, j, m, l)] * updateSize;
     }
-  for (int i = 0; i < B; i++)
+  for (int i = 0; i < B && orc_has_flux(c); i++)
     for (int j = 0; j < B; j++) {
       const double weight = W(N, i) * W(N, j);
       const double updateSize = weight / dx[2];
@@ -345,8 +408,8 @@ void orc_volume_integral(const orc_cfg* c, double* lduh, const double* lFhi, con
           for (int m = 0; m < B; m++)
             lduh[IDX(k, i, j, l)] += KXI(k, m) * lFhi[z_off + IDX(i, j, m, l)] * updateSize;
     }
-  /* source term (volumeIntegralNonlinear.cpph, useSourceOrNCP branch): += w^3 lShi */
-  if (orc_has_source(c)) {
+  /* source term (volumeIntegralNonlinear.cpph:106-124, useSourceOrNCP branch): += w^3 lShi */
+  if (orc_has_source(c) || orc_has_ncp(c)) {
     const int s_off = 3 * B3 * nv;
     for (int i = 0; i < B; i++)
       for (int j = 0; j < B; j++)
@@ -393,7 +456,7 @@ void orc_surface_integral(const orc_cfg* c, double* lduh, const double* lFbnd, c
 #undef IF
 }
 
-/* riemannSolverNonlinear<useNCP=false>, kernels/aderdg/generic/c/3d/riemannSolverNonlinear.cpph:38-120 */
+/* riemannSolverNonlinear<useNCP>, kernels/aderdg/generic/c/3d/riemannSolverNonlinear.cpph:38-120 */
 void orc_riemann_solver(const orc_cfg* c, double* FL, double* FR, const double* QL,
                         const double* QR, double dt, int direction) {
   (void)dt;
@@ -416,13 +479,30 @@ void orc_riemann_solver(const orc_cfg* c, double* FL, double* FR, const double* 
     if (fabs(LR[v]) > smax_R) smax_R = fabs(LR[v]);
   }
   const double smax = (smax_L < smax_R) ? smax_R : smax_L; /* std::max(a,b) */
+  /* non-conservative fluctuation (:95-113): ncp(Qavg, gradQ) with gradQ zero
+   * except row `direction` = QR - QL; FR = F - ncp/2, FL = F + ncp/2 */
+  const int has_ncp = orc_has_ncp(c);
+  double gradQ[3 * 16] = {0.0}, Qavg[16] = {0.0}, ncp[16] = {0.0};
   for (int i = 0; i < B; i++)
-    for (int j = 0; j < B; j++)
+    for (int j = 0; j < B; j++) {
+      if (has_ncp) {
+        for (int l = 0; l < nv; l++) {
+          gradQ[direction * nv + l] = QR[(i * B + j) * nd + l] - QL[(i * B + j) * nd + l];
+          Qavg[l] = 0.5 * (QR[(i * B + j) * nd + l] + QL[(i * B + j) * nd + l]);
+        }
+        orc_ncp(c, Qavg, gradQ, ncp);
+      }
       for (int k = 0; k < nv; k++) {
         const int f = (i * B + j) * nv + k, q = (i * B + j) * nd + k;
         FL[f] = 0.5 * (FR[f] + FL[f]) - 0.5 * smax * (QR[q] - QL[q]);
-        FR[f] = FL[f];
+        if (has_ncp) {
+          FR[f] = FL[f] - 0.5 * ncp[k];
+          FL[f] = FL[f] + 0.5 * ncp[k];
+        } else {
+          FR[f] = FL[f];
+        }
       }
+    }
 }
 
 /* solutionUpdate, kernels/aderdg/generic/c/3d/solutionUpdate.cpph:23-47 */

@@ -36,8 +36,12 @@ typedef struct {
   double gamma;   /* ratio of specific heats                         */
   double cfl;     /* CFL factor (0.9 for ADER-DG)                    */
   int pde;        /* 0: 3D Euler; 1: linear advection test PDE; 2: EulerFV demonstrator (FV only);
-                     3: 3D Euler with constant gravity (algebraic source, the useSource arm) */
-  double adv[3];  /* advection velocity for pde==1                   */
+                     3: 3D Euler with constant gravity (algebraic source, the useSource arm);
+                     4: linear advection in non-conservative form (no flux, B_d = a_d I: the
+                        useFlux = false, useNCP = true arms);
+                     5: 3D Euler with the test non-conservative product of the reference's
+                        Euler kernel test (the useNCP arm on a nonlinear flux) */
+  double adv[3];  /* advection velocity for pdes 1 and 4             */
   double grav;    /* gravity along -z for pde==3                      */
 } orc_cfg;
 
@@ -48,6 +52,13 @@ void orc_eigenvalues(const orc_cfg* c, const double* Q, int d, double* lambda);
  * except for pde==3: S = (0, 0, 0, -rho g, -m_z g) */
 void orc_source(const orc_cfg* c, const double* Q, double* S);
 int orc_has_source(const orc_cfg* c);
+/* the PDE's feature arms: flux (all but pde 4), non-conservative product (pdes 4, 5) */
+int orc_has_flux(const orc_cfg* c);
+int orc_has_ncp(const orc_cfg* c);
+/* nonConservativeProduct (UserSolverInterface.h), gradQ [3][nvar] */
+void orc_ncp(const orc_cfg* c, const double* Q, const double* gradQ, double* BgradQ);
+/* kernels::fusedSource (kernels/fusedMethods.cpph:17-32): source - ncp */
+void orc_fused_source(const orc_cfg* c, const double* Q, const double* gradQ, double* S);
 
 /* ADER-DG kernels, kernels/aderdg/generic/c/3d/ */
 int orc_space_time_predictor(const orc_cfg* c, double* lQhbnd, double* lFhbnd,

@@ -61,6 +61,8 @@ def lib():
         L.orc_flux.argtypes = [P(Cfg), _D, _D]
         L.orc_eigenvalues.argtypes = [P(Cfg), _D, C.c_int, _D]
         L.orc_source.argtypes = [P(Cfg), _D, _D]
+        L.orc_ncp.argtypes = [P(Cfg), _D, _D, _D]
+        L.orc_fused_source.argtypes = [P(Cfg), _D, _D, _D]
         L.orc_space_time_predictor.argtypes = [P(Cfg), _D, _D, _D, _D, _D, _D, _D, _D, C.c_double]
         L.orc_space_time_predictor.restype = C.c_int
         L.orc_volume_integral.argtypes = [P(Cfg), _D, _D, _D]
@@ -121,7 +123,9 @@ def ptr(a):
 
 def cfg(order, nvar=5, npar=0, gamma=1.4, cfl=0.9, pde=0, adv=(0.0, 0.0, 0.0), grav=0.0):
     """pde 0 Euler, 1 linear advection (velocity adv), 2 EulerFV 2D, 3 Euler
-    with gravity grav along -z (algebraic source)."""
+    with gravity grav along -z (algebraic source), 4 linear advection in
+    non-conservative form (velocity adv, no flux), 5 Euler with the test
+    non-conservative product of the reference's Euler kernel test."""
     c = Cfg(order, nvar, npar, gamma, cfl, pde)
     for i in range(3):
         c.adv[i] = adv[i]
@@ -133,6 +137,12 @@ def source(c, Q):
     S = np.zeros(c.nvar)
     lib().orc_source(C.byref(c), ptr(dvec(Q)), ptr(S))
     return S
+
+
+def ncp(c, Q, gradQ):
+    out = np.zeros(c.nvar)
+    lib().orc_ncp(C.byref(c), ptr(dvec(Q)), ptr(dvec(gradQ)), ptr(out))
+    return out
 
 
 def dvec(x):
