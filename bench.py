@@ -131,7 +131,8 @@ def main():
     ap.add_argument("--contraction", type=int, default=0,
                     help="predictor sweep contraction: 0 tuned, 1 pencil owners, 2 node owners, "
                          "3 pencil owners two slices at a time, 4 staged iterate (FP64, orders 5..9)")
-    ap.add_argument("--workload", default="aderdg", choices=["aderdg", "fv2d", "ensemble", "limiter", "muscl", "amr"],
+    ap.add_argument("--workload", default="aderdg", choices=["aderdg", "fv2d", "ensemble", "limiter", "muscl", "amr",
+                                                          "ncp"],
                     help="aderdg: the headline ADER-DG step; fv2d: the EulerFV demonstrator "
                          "(2D Godunov, 81x81 patches of 10x10 volumes, 200 steps); ensemble: independent "
                          "grids of random order/size/gamma advanced together; limiter: troubled-cell detection and "
@@ -161,6 +162,9 @@ def main():
         return
     if args.workload == "amr":
         print(json.dumps(run_amr(args)))
+        return
+    if args.workload == "ncp":
+        print(json.dumps(run_ncp(args)))
         return
     import exahype_amd as E
     N, n = args.order, args.cells
@@ -259,6 +263,103 @@ def main():
     if args.copy_gib > 0:
         out["hbm_copy"] = copy_reference(args.copy_gib)
     print(json.dumps(out))
+
+
+NCP_VELOCITY = (0.3, -0.7, 0.5)
+PHI_NCP = 6 * NVAR  # a . grad Q: three multiply-adds per variable
+
+
+def advection_ic_kwargs():
+    return dict(nmodes=2, k=[(1, 0, 1), (0, 1, 1)], a=[0.2, 0.1], phi=[0.3, 1.1], uinf=NCP_VELOCITY)
+
+
+def run_advection(pde, N, n, steps, warmup):
+    import exahype_amd as E
+    s = E.ADERDGSolver(N, pde=pde, velocity=NCP_VELOCITY)
+    g = E.Grid(s, n, n, n)
+    g.set_initial_conditions(E.make_ic(3, **advection_ic_kwargs()))
+    g.compute_dt()
+    g.step(warmup)
+    g.sync()
+    g.kernel_times()
+    g.total_picard_iterations()
+    g.enable_timing(True)
+    E.device_synchronize()
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.sync()
+    E.device_synchronize()
+    wall = time.perf_counter() - t0
+    times, counts = g.kernel_times()
+    sweeps = g.total_picard_iterations() / (g.ncells * steps)
+    g.close()
+    s.close()
+    return dict(wall=wall, times=times, counts=counts, iters=sweeps)
+
+
+def run_ncp(args):
+    """Linear advection of 5 variables in non-conservative form (no flux,
+    nonConservativeProduct = a . grad Q: the useFlux = false / useNCP = true
+    arms) on n^3 periodic cells at order N, with the flux form of the same
+    PDE timed beside it; the predictor is the dominant kernel."""
+    import exahype_amd as E
+    N, n = args.order, args.cells
+    ncells = n ** 3
+    dof = ncells * NVAR * (N + 1) ** 3
+    r = run_advection(E.PDE_ADVECTION_NCP, N, n, args.steps, args.warmup)
+    rf = run_advection(E.PDE_ADVECTION, N, n, args.steps, args.warmup)
+    B = N + 1
+    pred_ms = r["times"][0] / max(r["counts"][0], 1)
+    # per sweep: the dudx gradients (6 V B^5) and the iK1 contraction (2 V B^5),
+    # B^4 products a . grad Q and fused-source updates; then time averages,
+    # volume integral (source part) and face extrapolation
+    flops = (r["iters"] * (8 * NVAR * B ** 5 + B ** 4 * (PHI_NCP + 5 * NVAR)) + 14 * NVAR * B ** 4
+             + 24 * NVAR * B ** 3) * ncells
+    achieved = flops / (pred_ms * 1e-3) / 1e12
+    out = {
+        "metric": "DoF-updates/s, 3D linear advection in non-conservative form, ADER-DG N=%d" % N,
+        "value": dof * args.steps / r["wall"],
+        "unit": "DoF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["wall"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "3D linear advection of 5 variables, non-conservative form (no flux, B_d = a_d I), "
+                               "ADER-DG order %d, %d^3 periodic cells, 2 sine modes, CFL 0.9" % (N, n),
+                   "velocity": list(NCP_VELOCITY), "cells": ncells, "order": N, "dof_per_step": dof,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "k_predictor_pde<%d, PdeAdvectionNcp>" % N, "avg_launch_ms": pred_ms,
+                     "picard_sweeps_per_cell_step": r["iters"], "flop_per_launch": flops},
+        "phase_ms_per_step": {"predictor": pred_ms, "riemann": r["times"][1] / max(r["counts"][1], 1),
+                              "update_surface_dt": r["times"][2] / max(r["counts"][2], 1)},
+        "flux_form": {"dof_updates_per_s": dof * args.steps / rf["wall"], "ms_per_step": rf["wall"] / args.steps * 1e3,
+                      "predictor_ms": rf["times"][0] / max(rf["counts"][0], 1), "sweeps_per_cell": rf["iters"]},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        steps = max(1, min(8, int(round(8 * (6.0 / (N + 1)) ** 5))))
+        og = O.OracleGrid(O.cfg(N, pde=4, adv=NCP_VELOCITY), 27, 27, 27)
+        for d in range(3):
+            og.g.width[d] = 27 / n
+        og.init(O.make_ic(kind=3, **advection_ic_kwargs()))
+        dt = og.stable_dt(threads)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            dt = og.step(dt, threads)
+        wall = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": og.ncells * NVAR * B ** 3 * steps / wall, "unit": "DoF-updates/s",
+                               "cores": threads, "kind": "port",
+                               "sample": "oracle (C, OpenMP), order %d, 27^3 periodic cells of size 1/%d, same "
+                                         "initial data, %d steps (%.1f s wall)" % (N, n, steps, wall)}
+    return out
 
 
 def fv2d_ic(mod, seed=0):
