@@ -689,6 +689,13 @@ void orc_grid_predict_cells(const orc_cfg* c, orc_grid* g, double dt, int first,
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 16)
   for (int q = first; q < first + count; ++q) {
     ensure_scratch();
+    if (c->linear) { /* isLinear branch of the generated glue */
+      orc_space_time_predictor_linear(c, g->lQhbnd + (size_t)q * 6 * B2 * nd, g->lFhbnd + (size_t)q * 6 * B2 * nv,
+                                      NULL, NULL, tl_lQhi, tl_lFhi, g->luh + (size_t)q * B3 * nd, dxv, dt);
+      orc_volume_integral_linear(c, g->lduh + (size_t)q * B3 * nv, tl_lFhi, dxv);
+      g->iters[q] = 0;
+      continue;
+    }
     g->iters[q] = orc_space_time_predictor(c, g->lQhbnd + (size_t)q * 6 * B2 * nd,
                                            g->lFhbnd + (size_t)q * 6 * B2 * nv, NULL, NULL, NULL, tl_lFhi,
                                            g->luh + (size_t)q * B3 * nd, dxv, dt);
@@ -714,16 +721,16 @@ double orc_grid_step(const orc_cfg* c, orc_grid* g, double dt, int nthreads) {
       if (d == 1) jy = (iy + 1) % ny;
       if (d == 2) jz = (iz + 1) % nz;
       const int r = (jz * ny + jy) * nx + jx;
-      orc_riemann_solver(c, g->lFhbnd + ((size_t)q * 6 + 2 * d + 1) * B2 * nv,
-                         g->lFhbnd + ((size_t)r * 6 + 2 * d) * B2 * nv,
-                         g->lQhbnd + ((size_t)q * 6 + 2 * d + 1) * B2 * nd,
-                         g->lQhbnd + ((size_t)r * 6 + 2 * d) * B2 * nd, dt, d);
+      (c->linear ? orc_riemann_solver_linear : orc_riemann_solver)(
+          c, g->lFhbnd + ((size_t)q * 6 + 2 * d + 1) * B2 * nv, g->lFhbnd + ((size_t)r * 6 + 2 * d) * B2 * nv,
+          g->lQhbnd + ((size_t)q * 6 + 2 * d + 1) * B2 * nd, g->lQhbnd + ((size_t)r * 6 + 2 * d) * B2 * nd, dt, d);
     }
   }
   /* SolutionUpdate (E/solvers/ADERDGSolver.cpp:2100-2179) */
 #pragma omp parallel for num_threads(nthreads) schedule(static)
   for (int q = 0; q < nc; ++q) {
-    orc_surface_integral(c, g->lduh + (size_t)q * B3 * nv, g->lFhbnd + (size_t)q * 6 * B2 * nv, dxv);
+    (c->linear ? orc_surface_integral_linear : orc_surface_integral)(c, g->lduh + (size_t)q * B3 * nv,
+                                                                    g->lFhbnd + (size_t)q * 6 * B2 * nv, dxv);
     orc_solution_update(c, g->luh + (size_t)q * B3 * nd, g->lduh + (size_t)q * B3 * nv, dt);
   }
   /* TimeStepSizeComputation (E/solvers/ADERDGSolver.cpp:1948-2001) */

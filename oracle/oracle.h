@@ -43,6 +43,8 @@ typedef struct {
                         Euler kernel test (the useNCP arm on a nonlinear flux) */
   double adv[3];  /* advection velocity for pdes 1 and 4             */
   double grav;    /* gravity along -z for pde==3                      */
+  int linear;     /* 1: the linear kernel family (pdes 4, 5): Cauchy-Kowalevski
+                     predictor, fluctuation-form integrals, coefficient-matrix Riemann */
 } orc_cfg;
 
 /* user PDE (E/tests/kernels/c/GenericEulerKernelTest3D.cpp:26-82) */
@@ -70,6 +72,16 @@ void orc_riemann_solver(const orc_cfg* c, double* FL, double* FR, const double* 
                         const double* QR, double dt, int direction);
 void orc_solution_update(const orc_cfg* c, double* luh, const double* lduh, double dt);
 double orc_stable_time_step_size(const orc_cfg* c, const double* luh, const double dx[3]);
+
+/* the linear kernels of kernels/aderdg/generic/c/3d (oracle/linear.c) */
+void orc_ncp_linear(const orc_cfg* c, const double* Q, const double* gradQ, double* T /* [3][nvar] */);
+void orc_coefficient_matrix(const orc_cfg* c, const double* Q, int d, double* Bn /* [nvar][nvar] */);
+void orc_space_time_predictor_linear(const orc_cfg* c, double* lQbnd, double* lFbnd, double* lQi, double* lFi,
+                                     double* lQh, double* lFh, const double* luh, const double dx[3], double dt);
+void orc_volume_integral_linear(const orc_cfg* c, double* lduh, const double* lFhi, const double dx[3]);
+void orc_surface_integral_linear(const orc_cfg* c, double* lduh, const double* lFbnd, const double dx[3]);
+void orc_riemann_solver_linear(const orc_cfg* c, double* FL, double* FR, const double* QL, const double* QR,
+                               double dt, int direction);
 
 /* initial conditions at GL nodes (kernels/aderdg/generic/c/3d/solutionAdjustment.cpph:17-52) */
 typedef struct {

@@ -20,7 +20,7 @@ def build():
 class Cfg(C.Structure):
     _fields_ = [("order", C.c_int), ("nvar", C.c_int), ("npar", C.c_int),
                 ("gamma", C.c_double), ("cfl", C.c_double), ("pde", C.c_int),
-                ("adv", C.c_double * 3), ("grav", C.c_double)]
+                ("adv", C.c_double * 3), ("grav", C.c_double), ("linear", C.c_int)]
 
 
 class IC(C.Structure):
@@ -62,6 +62,12 @@ def lib():
         L.orc_eigenvalues.argtypes = [P(Cfg), _D, C.c_int, _D]
         L.orc_source.argtypes = [P(Cfg), _D, _D]
         L.orc_ncp.argtypes = [P(Cfg), _D, _D, _D]
+        L.orc_ncp_linear.argtypes = [P(Cfg), _D, _D, _D]
+        L.orc_coefficient_matrix.argtypes = [P(Cfg), _D, C.c_int, _D]
+        L.orc_space_time_predictor_linear.argtypes = [P(Cfg), _D, _D, _D, _D, _D, _D, _D, _D, C.c_double]
+        L.orc_volume_integral_linear.argtypes = [P(Cfg), _D, _D, _D]
+        L.orc_surface_integral_linear.argtypes = [P(Cfg), _D, _D, _D]
+        L.orc_riemann_solver_linear.argtypes = [P(Cfg), _D, _D, _D, _D, C.c_double, C.c_int]
         L.orc_fused_source.argtypes = [P(Cfg), _D, _D, _D]
         L.orc_space_time_predictor.argtypes = [P(Cfg), _D, _D, _D, _D, _D, _D, _D, _D, C.c_double]
         L.orc_space_time_predictor.restype = C.c_int
@@ -121,7 +127,7 @@ def ptr(a):
     return a.ctypes.data_as(_D)
 
 
-def cfg(order, nvar=5, npar=0, gamma=1.4, cfl=0.9, pde=0, adv=(0.0, 0.0, 0.0), grav=0.0):
+def cfg(order, nvar=5, npar=0, gamma=1.4, cfl=0.9, pde=0, adv=(0.0, 0.0, 0.0), grav=0.0, linear=0):
     """pde 0 Euler, 1 linear advection (velocity adv), 2 EulerFV 2D, 3 Euler
     with gravity grav along -z (algebraic source), 4 linear advection in
     non-conservative form (velocity adv, no flux), 5 Euler with the test
@@ -130,6 +136,7 @@ def cfg(order, nvar=5, npar=0, gamma=1.4, cfl=0.9, pde=0, adv=(0.0, 0.0, 0.0), g
     for i in range(3):
         c.adv[i] = adv[i]
     c.grav = grav
+    c.linear = int(linear)
     return c
 
 
@@ -173,6 +180,46 @@ def space_time_predictor(c, luh, dx, dt):
                                         ptr(dvec(dx)), dt)
     out["iters"] = it
     return out
+
+
+def space_time_predictor_linear(c, luh, dx, dt):
+    """Cauchy-Kowalevski predictor: lQi [B+1][z][y][x][nData] (time
+    derivatives), lFi [B][z][y][x][3][nVar], lQhi, lFhi [4][z][y][x][nVar]
+    (block 3 zero), lQhbnd, lFhbnd."""
+    B = c.order + 1
+    nd, nv = c.nvar + c.npar, c.nvar
+    out = dict(lQhbnd=np.zeros(6 * B * B * nd), lFhbnd=np.zeros(6 * B * B * nv),
+               lQi=np.zeros((B + 1) * B ** 3 * nd), lFi=np.zeros(B ** 4 * 3 * nv),
+               lQhi=np.zeros(B ** 3 * nd), lFhi=np.zeros(4 * B ** 3 * nv))
+    lib().orc_space_time_predictor_linear(C.byref(c), ptr(out["lQhbnd"]), ptr(out["lFhbnd"]), ptr(out["lQi"]),
+                                          ptr(out["lFi"]), ptr(out["lQhi"]), ptr(out["lFhi"]), ptr(dvec(luh)),
+                                          ptr(dvec(dx)), dt)
+    return out
+
+
+def volume_integral_linear(c, lFhi, dx):
+    lduh = np.zeros((c.order + 1) ** 3 * c.nvar)
+    lib().orc_volume_integral_linear(C.byref(c), ptr(lduh), ptr(dvec(lFhi)), ptr(dvec(dx)))
+    return lduh
+
+
+def surface_integral_linear(c, lduh, lFhbnd, dx):
+    lduh = dvec(lduh).copy()
+    lib().orc_surface_integral_linear(C.byref(c), ptr(lduh), ptr(dvec(lFhbnd)), ptr(dvec(dx)))
+    return lduh
+
+
+def riemann_solver_linear(c, QL, QR, dt, direction):
+    B = c.order + 1
+    FL, FR = np.zeros(B * B * c.nvar), np.zeros(B * B * c.nvar)
+    lib().orc_riemann_solver_linear(C.byref(c), ptr(FL), ptr(FR), ptr(dvec(QL)), ptr(dvec(QR)), dt, direction)
+    return FL, FR
+
+
+def coefficient_matrix(c, Q, d):
+    Bn = np.zeros(c.nvar * c.nvar)
+    lib().orc_coefficient_matrix(C.byref(c), ptr(dvec(Q)), d, ptr(Bn))
+    return Bn.reshape(c.nvar, c.nvar)
 
 
 def volume_integral(c, lFhi, dx):

@@ -1,0 +1,174 @@
+"""Pins of the oracle's linear kernel family (oracle/linear.c).
+
+* volumeIntegralLinear, surfaceIntegralLinear, riemannSolverLinear: the
+  reference's golden vectors (testVolumeIntegralLinear,
+  testSurfaceIntegralLinear, testRiemannSolverLinear in
+  E/tests/kernels/c/GenericEulerKernelTest3D.cpp; data from
+  generic_euler_testdata.cpp), at the reference's 1e-10.  The Riemann test
+  runs the reference's Euler test class: Euler eigenvalues and a zero
+  coefficient matrix (pde 5).
+* spaceTimePredictorLinear: the reference's own test is disabled
+  ("Test data might not be correct anymore", GenericEulerKernelTest.cpp:50-51),
+  so the Cauchy-Kowalevski predictor is pinned by exactness: for
+  constant-coefficient advection and spatial data of total degree <= N the
+  Taylor series in time terminates, so the time average at every node, its
+  face extrapolation and the averaged products a_d dq/dx_d equal the exact
+  averages of q(x - a t) over [0, dt].  A wrong Taylor weight, a transposed
+  dudx or a lost 1/dx fails this.
+* the whole linear scheme: convergence order N+1 on an advected sine.
+"""
+import numpy as np
+import pytest
+
+from exahype_amd.tables import dg_math
+from oracle import oracle as O
+
+VEL = (0.3, -0.7, 0.5)
+PNPM = (1.0, 0.33, 0.17, 0.1, 0.069, 0.045, 0.038, 0.03, 0.02, 0.015)
+
+
+def test_golden_volume_integral_linear(golden):
+    c = O.cfg(3, pde=5, linear=1)
+    lFhi = np.zeros(960)
+    for i in range(0, 320, 5):
+        lFhi[i + 1] = 1.0
+        lFhi[320 + i + 2] = 2.0
+        lFhi[640 + i + 3] = 3.0
+    out = O.volume_integral_linear(c, lFhi, [0.5] * 3)
+    assert np.abs(out - golden["testVolumeIntegralLinear/lduh"]).max() <= 1e-10
+
+
+def test_golden_surface_integral_linear(golden):
+    c = O.cfg(3, pde=5, linear=1)
+    lFhbnd = np.zeros(480)
+    for i in range(0, 80, 5):
+        lFhbnd[i + 1] = lFhbnd[80 + i + 1] = 1.0
+        lFhbnd[160 + i + 2] = lFhbnd[240 + i + 2] = 1.0
+        lFhbnd[320 + i + 3] = lFhbnd[400 + i + 3] = 1.0
+    out = O.surface_integral_linear(c, np.arange(320) / 10.0, lFhbnd, [0.1] * 3)
+    assert np.abs(out - golden["testSurfaceIntegralLinear/lduh"]).max() <= 1e-10
+
+
+def test_golden_riemann_solver_linear(golden):
+    c = O.cfg(3, pde=5, linear=1)
+    FL, FR = O.riemann_solver_linear(c, golden["testRiemannSolver/QL"], golden["testRiemannSolver/QR"],
+                                     1.40831757919882352703e-03, 1)
+    assert np.abs(FL - golden["testRiemannSolverLinear/FL"]).max() <= 1e-10
+    assert np.abs(FR - golden["testRiemannSolverLinear/FR"]).max() <= 1e-10
+
+
+def test_coefficient_matrix_riemann_closed_form():
+    """Advection: Bn = a_d I, so FL = (a_d - s)(QR - QL)/2 and FR = (a_d + s)(QR - QL)/2."""
+    c = O.cfg(2, pde=4, adv=VEL, linear=1)
+    rng = np.random.default_rng(1)
+    QL, QR = rng.standard_normal(45), rng.standard_normal(45)
+    for d in range(3):
+        FL, FR = O.riemann_solver_linear(c, QL, QR, 0.0, d)
+        s = abs(VEL[d])
+        assert np.allclose(FL, 0.5 * (VEL[d] - s) * (QR - QL), rtol=0, atol=1e-15)
+        assert np.allclose(FR, 0.5 * (VEL[d] + s) * (QR - QL), rtol=0, atol=1e-15)
+
+
+def _poly(N, seed):
+    """Random polynomials of total degree <= N in (x, y, z), one per variable."""
+    rng = np.random.default_rng(seed)
+    terms = [(i, j, k) for i in range(N + 1) for j in range(N + 1) for k in range(N + 1) if i + j + k <= N]
+    coef = rng.uniform(-1, 1, (5, len(terms)))
+    coef[:, 0] += 2.0
+
+    def ev(x, y, z):
+        return np.stack([sum(cf[t] * x ** a * y ** b * z ** g for t, (a, b, g) in enumerate(terms)) for cf in coef],
+                        axis=-1)
+
+    def grad(x, y, z):
+        gx = np.stack([sum(cf[t] * a * x ** max(a - 1, 0) * y ** b * z ** g for t, (a, b, g) in enumerate(terms))
+                       for cf in coef], axis=-1)
+        gy = np.stack([sum(cf[t] * b * x ** a * y ** max(b - 1, 0) * z ** g for t, (a, b, g) in enumerate(terms))
+                       for cf in coef], axis=-1)
+        gz = np.stack([sum(cf[t] * g * x ** a * y ** b * z ** max(g - 1, 0) for t, (a, b, g) in enumerate(terms))
+                       for cf in coef], axis=-1)
+        return gx, gy, gz
+
+    return ev, grad
+
+
+TOL_Q, TOL_G = 1e-11, 1e-8
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5, 7, 9])
+def test_predictor_linear_exact_for_polynomials(N):
+    c = O.cfg(N, pde=4, adv=VEL, linear=1)
+    B = N + 1
+    ev, grad = _poly(N, N)
+    x0 = np.array([0.3, -0.2, 0.1])
+    dx = np.array([0.05, 0.08, 0.03])
+    # the CFL step of the scheme (stableTimeStepSize.cpph:34-56)
+    dt = 0.9 * PNPM[N] / sum(abs(VEL[d]) / dx[d] for d in range(3))
+    xi = np.asarray(dg_math.gl_nodes_weights(N)[0])
+    Z, Y, X = np.meshgrid(xi, xi, xi, indexing="ij")  # [z][y][x]
+    px, py, pz = x0[0] + dx[0] * X, x0[1] + dx[1] * Y, x0[2] + dx[2] * Z
+    luh = ev(px, py, pz).ravel()
+    o = O.space_time_predictor_linear(c, luh, dx, dt)
+    # exact time average of q(x - a t) over [0, dt] (Gauss-Legendre in t, exact)
+    tq, tw = np.polynomial.legendre.leggauss(N + 2)
+    ts, ws = 0.5 * dt * (tq + 1), 0.5 * tw
+    avg = sum(w * ev(px - VEL[0] * t, py - VEL[1] * t, pz - VEL[2] * t) for t, w in zip(ts, ws))
+    scale = np.abs(avg).max()
+    # every Taylor term applies the nodal derivative once more, which at the
+    # CFL step damps the 13-digit dudx literals' error (|a| dt |dudx| / dx < 1);
+    # a wrong Taylor weight (dt^i / i! for dt^i / (i+1)!) is off by ~1e-3
+    assert np.abs(o["lQhi"] - avg.ravel()).max() <= TOL_Q * scale
+    # averaged products a_d dq/dx_d, block d of lFhi ([d][z][y][x][v])
+    gavg = [sum(w * grad(px - VEL[0] * t, py - VEL[1] * t, pz - VEL[2] * t)[d] for t, w in zip(ts, ws))
+            for d in range(3)]
+    # the products are nodal derivatives: the dudx literals' ~5e-12 error times
+    # sum_n |q_n| / dx (~1e3 here) relative to |a . grad q|
+    gscale = max(np.abs(VEL[d] * gavg[d]).max() for d in range(3))
+    for d in range(3):
+        blk = o["lFhi"][d * B ** 3 * 5:(d + 1) * B ** 3 * 5]
+        assert np.abs(blk - (VEL[d] * gavg[d]).ravel()).max() <= TOL_G * gscale, d
+    assert not np.any(o["lFhi"][3 * B ** 3 * 5:])
+    # the x- face: q averaged at x = x0
+    fz, fy = np.meshgrid(xi, xi, indexing="ij")
+    qx0 = sum(w * ev(x0[0] - VEL[0] * t + 0 * fz, x0[1] + dx[1] * fy - VEL[1] * t, x0[2] + dx[2] * fz - VEL[2] * t)
+              for t, w in zip(ts, ws))
+    assert np.abs(o["lQhbnd"][:B * B * 5] - qx0.ravel()).max() <= TOL_Q * scale
+
+
+def _sine_exact(x, y, z, t):
+    s = 0.2 * np.sin(2 * np.pi * ((x - VEL[0] * t) + (z - VEL[2] * t)) + 0.3) + \
+        0.1 * np.sin(2 * np.pi * ((y - VEL[1] * t) + (z - VEL[2] * t)) + 1.1)
+    return np.stack([1.0 + (v + 1) * s for v in range(5)], axis=-1)
+
+
+def _linear_error(N, n, t_end):
+    c = O.cfg(N, pde=4, adv=VEL, linear=1)
+    g = O.OracleGrid(c, n, n, n)
+    g.init(O.make_ic(kind=3, nmodes=2, k=[(1, 0, 1), (0, 1, 1)], a=[0.2, 0.1], phi=[0.3, 1.1]))
+    dt = g.stable_dt()
+    steps = int(np.ceil(t_end / dt))
+    dt = t_end / steps
+    for _ in range(steps):
+        g.step(dt)
+    B = N + 1
+    xi = np.asarray(dg_math.gl_nodes_weights(N)[0])
+    h = 1.0 / n
+    err = 0.0
+    u = g.luh().reshape(n, n, n, B, B, B, 5)
+    iz, iy, ix = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    for a in range(B):
+        for b in range(B):
+            for cc in range(B):
+                ex = _sine_exact((ix + xi[cc]) * h, (iy + xi[b]) * h, (iz + xi[a]) * h, t_end)
+                err = max(err, np.abs(u[:, :, :, a, b, cc, :] - ex).max())
+    return err
+
+
+@pytest.mark.parametrize("N,n", [(2, 8), (3, 4)])
+def test_linear_scheme_converges_at_order_n_plus_1(N, n):
+    """Max nodal error at t = 0.05 on n^3 and (2n)^3 cells; measured EOC 2.72
+    (N = 2, 8 -> 16 cells) and 3.60 (N = 3, 4 -> 8 cells)."""
+    e1 = _linear_error(N, n, 0.05)
+    e2 = _linear_error(N, 2 * n, 0.05)
+    eoc = np.log2(e1 / e2)
+    assert eoc >= N + 0.4, (e1, e2, eoc)
