@@ -44,7 +44,7 @@ class AderdgConfig(C.Structure):
     _fields_ = [("order", C.c_int), ("nvar", C.c_int), ("npar", C.c_int), ("cfl", C.c_double),
                 ("gamma", C.c_double), ("pde", C.c_int), ("precision", C.c_int), ("fp64_sweeps", C.c_int),
                 ("waves_per_simd", C.c_int), ("contraction", C.c_int), ("gravity", C.c_double),
-                ("velocity", C.c_double * 3)]
+                ("velocity", C.c_double * 3), ("linear", C.c_int)]
 
 
 class EnsembleMember(C.Structure):
@@ -227,22 +227,27 @@ class ADERDGSolver:
     """Per-cell kernel surface of exahype::solvers::ADERDGSolver on the GPU."""
 
     def __init__(self, order, nvar=5, npar=0, cfl=0.9, gamma=1.4, precision=PREC_FP64, fp64_sweeps=1,
-                 waves_per_simd=0, contraction=0, device=0, gravity=None, pde=None, velocity=(0.0, 0.0, 0.0)):
+                 waves_per_simd=0, contraction=0, device=0, gravity=None, pde=None, velocity=(0.0, 0.0, 0.0),
+                 linear=False):
         """gravity: None for the Euler equations, g for Euler with constant
         gravity along -z (EXA_PDE_EULER_GRAVITY, an algebraic source).
         pde: one of the PDE_* ids to select another compiled PDE (the
-        advection PDEs take their velocity from `velocity`)."""
+        advection PDEs take their velocity from `velocity`).
+        linear: the linear kernel family (Cauchy-Kowalevski predictor,
+        coefficient-matrix Riemann solver), for PDE_ADVECTION_NCP and
+        PDE_EULER_NCP_TEST."""
         L = load_library()
         _check(L.exa_init(device))
         if pde is None:
             pde = PDE_EULER if gravity is None else PDE_EULER_GRAVITY
         self.cfg = AderdgConfig(order, nvar, npar, cfl, gamma, pde, precision, fp64_sweeps, waves_per_simd,
                                 contraction, 0.0 if gravity is None else float(gravity),
-                                (C.c_double * 3)(*[float(a) for a in velocity]))
+                                (C.c_double * 3)(*[float(a) for a in velocity]), 1 if linear else 0)
         h = C.c_void_p()
         _check(L.exa_aderdg_create(C.byref(self.cfg), C.byref(h)))
         self._h = h
         self.order, self.nvar, self.npar = order, nvar, npar
+        self.linear = bool(linear)
         self.B = order + 1
         self._pending = []
         self.ndata = nvar + npar
@@ -262,8 +267,11 @@ class ADERDGSolver:
     def spaceTimePredictor(self, luh, dx, dt):
         B, nd, nv = self.B, self.ndata, self.nvar
         luh = np.ascontiguousarray(luh, dtype=np.float64)
-        out = dict(lQhbnd=np.zeros(6 * B * B * nd), lFhbnd=np.zeros(6 * B * B * nv), lQi=np.zeros(B ** 4 * nd),
-                   lFi=np.zeros(B ** 4 * 4 * nv), lQhi=np.zeros(B ** 3 * nd), lFhi=np.zeros(4 * B ** 3 * nv))
+        # a linear solver returns the B+1 time derivatives and the 3 products per time level
+        out = dict(lQhbnd=np.zeros(6 * B * B * nd), lFhbnd=np.zeros(6 * B * B * nv),
+                   lQi=np.zeros((B + 1) * B ** 3 * nd if self.linear else B ** 4 * nd),
+                   lFi=np.zeros(B ** 4 * (3 if self.linear else 4) * nv), lQhi=np.zeros(B ** 3 * nd),
+                   lFhi=np.zeros(4 * B ** 3 * nv))
         it = C.c_int(0)
         _check(_lib.exa_aderdg_space_time_predictor(self._h, _p(out["lQhbnd"]), _p(out["lFhbnd"]), _p(out["lQi"]),
                                                     _p(out["lFi"]), _p(out["lQhi"]), _p(out["lFhi"]), _p(luh),

@@ -1271,7 +1271,10 @@ __global__ __launch_bounds__(256) void k_riemann_list(const double* __restrict__
 // Surface integral + solution update + per-node CFL dt with a block minimum
 // and one atomic per cell.  surfaceIntegralNonlinear.cpph:21-86,
 // solutionUpdate.cpph:23-47, stableTimeStepSize.cpph:21-61.
-enum : int { UPD_SURFACE = 1, UPD_UPDATE = 2, UPD_DT = 4, UPD_WRITE_LDUH = 8 };
+// UPD_LINEAR: the surface integral of the linear scheme, whose Riemann
+// solver returns fluctuations: both faces enter with a plus sign
+// (surfaceIntegralLinear.cpph:24-90).
+enum : int { UPD_SURFACE = 1, UPD_UPDATE = 2, UPD_DT = 4, UPD_WRITE_LDUH = 8, UPD_LINEAR = 16 };
 
 template <int N, int MODE, class PDE = PdeEuler>
 __global__ __launch_bounds__(Ord<N>::NT) void k_update(double* __restrict__ luh, double* __restrict__ lduh,
@@ -1313,15 +1316,21 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_update(double* __restrict__ luh,
       const double usx = (T.w[z] * T.w[y]) / dx0;
 #pragma unroll
       for (int v = 0; v < V; ++v)
-        du[v] -= (F[(1 * V + v) * B2 + z * B + y] * T.FR[x] - F[(0 * V + v) * B2 + z * B + y] * T.FL[x]) * usx;
+        du[v] -= (MODE & UPD_LINEAR)
+                     ? (F[(1 * V + v) * B2 + z * B + y] * T.FR[x] + F[(0 * V + v) * B2 + z * B + y] * T.FL[x]) * usx
+                     : (F[(1 * V + v) * B2 + z * B + y] * T.FR[x] - F[(0 * V + v) * B2 + z * B + y] * T.FL[x]) * usx;
       const double usy = (T.w[z] * T.w[x]) / dx1;
 #pragma unroll
       for (int v = 0; v < V; ++v)
-        du[v] -= (F[(3 * V + v) * B2 + z * B + x] * T.FR[y] - F[(2 * V + v) * B2 + z * B + x] * T.FL[y]) * usy;
+        du[v] -= (MODE & UPD_LINEAR)
+                     ? (F[(3 * V + v) * B2 + z * B + x] * T.FR[y] + F[(2 * V + v) * B2 + z * B + x] * T.FL[y]) * usy
+                     : (F[(3 * V + v) * B2 + z * B + x] * T.FR[y] - F[(2 * V + v) * B2 + z * B + x] * T.FL[y]) * usy;
       const double usz = (T.w[y] * T.w[x]) / dx2;
 #pragma unroll
       for (int v = 0; v < V; ++v)
-        du[v] -= (F[(5 * V + v) * B2 + y * B + x] * T.FR[z] - F[(4 * V + v) * B2 + y * B + x] * T.FL[z]) * usz;
+        du[v] -= (MODE & UPD_LINEAR)
+                     ? (F[(5 * V + v) * B2 + y * B + x] * T.FR[z] + F[(4 * V + v) * B2 + y * B + x] * T.FL[z]) * usz
+                     : (F[(5 * V + v) * B2 + y * B + x] * T.FR[z] - F[(4 * V + v) * B2 + y * B + x] * T.FL[z]) * usz;
     }
     if (MODE & UPD_WRITE_LDUH) {
 #pragma unroll

@@ -332,4 +332,224 @@ __global__ __launch_bounds__(256) void k_riemann_list_pde(const double* __restri
                            threadIdx.x & 63);
 }
 
+// ---------------------------------------------------------------------------
+// The linear kernel family (the isLinear branch of the generated glue),
+// orders 1..9.
+constexpr int LINEAR_MAX_ORDER = 9;
+
+// Cauchy-Kowalevski predictor + linear volume integral
+// (spaceTimePredictorLinear.cpph:33-443, volumeIntegralLinear.cpph:21-55).
+// Thread s owns node s and walks the time derivatives lQi[0..B]: the level
+// passes through LDS for its dudx gradients, the product tensor
+// T = B_d dQ/dx_d gives the next level -sum_d T_d, and the Taylor time
+// averages dt^i/(i+1)! accumulate in registers.
+template <int N, class PDE, bool DEBUG>
+__global__ __launch_bounds__(Ord<N>::NT) void k_predictor_linear(
+    const double* __restrict__ luh, double* __restrict__ lduh, double* __restrict__ qf, double* __restrict__ ff,
+    int* __restrict__ iters, const double* __restrict__ dtp, double dx0, double dx1, double dx2, double gamma,
+    unsigned long long* __restrict__ sweep_count, PredictorDebug dbg, Ens ens) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
+  extern __shared__ double sm[];
+  double* sQ = sm;           // one derivative level [v][node]
+  double* sQh = sm;          // afterwards: time averages [v][node]
+  double* sFh = sm + V * B3; // and the averaged products [d][v][node]
+  const DGTables& T = c_tab[N];
+  const PdeParams P{gamma, {ens.adv[0], ens.adv[1], ens.adv[2]}};
+  const size_t cell = blockIdx.x;
+  const int s = threadIdx.x;
+  const bool act = s < B3;
+  const int z = act ? s / B2 : 0, y = act ? (s / B) % B : 0, x = act ? s % B : 0;
+  const double dt = *dtp;
+  double q[V], qh[V], fh[3][V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) q[v] = qh[v] = act ? luh[cell * V * B3 + v * B3 + s] : 1.0;
+  double fac = 0.5 * dt;  // dtavFac of the level (:236-248)
+  for (int i = 0; i < B; ++i) {
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) sQ[v * B3 + s] = q[v];
+    }
+    __syncthreads();
+    if (act) {
+      double G[3][V], Tn[3][V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        G[0][v] = G[1][v] = G[2][v] = 0.0;
+#pragma unroll
+        for (int n = 0; n < B; ++n) G[0][v] += 1.0 / dx0 * sQ[v * B3 + z * B2 + y * B + n] * T.dudx[x][n];
+#pragma unroll
+        for (int n = 0; n < B; ++n) G[1][v] += 1.0 / dx1 * sQ[v * B3 + z * B2 + n * B + x] * T.dudx[y][n];
+#pragma unroll
+        for (int n = 0; n < B; ++n) G[2][v] += 1.0 / dx2 * sQ[v * B3 + n * B2 + y * B + x] * T.dudx[z][n];
+      }
+      PDE::ncp_linear(q, G, Tn, P);
+      if (DEBUG) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          dbg.lQi[(i * B3 + s) * V + v] = q[v];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) dbg.lFi[((i * B3 + s) * 3 + d) * V + v] = Tn[d][v];
+        }
+      }
+      // time averages: level 0 initialises, level i >= 1 adds dtavFac_i times it
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        if (i > 0) qh[v] += fac * q[v];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) fh[d][v] = i == 0 ? Tn[d][v] : fh[d][v] + fac * Tn[d][v];
+      }
+      if (i > 0) fac *= dt / (i + 2);
+      // next level: lQi[i+1] = 0 - T_x - T_y - T_z (:196-207)
+#pragma unroll
+      for (int v = 0; v < V; ++v) q[v] = ((0.0 - Tn[0][v]) - Tn[1][v]) - Tn[2][v];
+    }
+  }
+  if (DEBUG && act) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) dbg.lQi[(B * B3 + s) * V + v] = q[v];
+  }
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      sQh[v * B3 + s] = qh[v];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) sFh[(d * V + v) * B3 + s] = fh[d][v];
+    }
+    if (DEBUG) {
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        dbg.lQhi[s * V + v] = qh[v];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) dbg.lFhi[(d * B3 + s) * V + v] = fh[d][v];
+        dbg.lFhi[(3 * B3 + s) * V + v] = 0.0;
+      }
+    }
+    // volume integral: lduh = -w^3 sum_d lFh[d]
+    const double w3 = (T.w[z] * T.w[y]) * T.w[x];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      double o = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) o -= w3 * fh[d][v];
+      lduh[cell * V * B3 + v * B3 + s] = o;
+    }
+  }
+  __syncthreads();
+  extrapolate_faces<N>(sQh, sFh, s, qf + cell * 6 * V * B2, ff + cell * 6 * V * B2);
+  if (s == 0 && iters) iters[cell] = 0;  // no Picard iterations
+  (void)sweep_count;
+}
+
+// Linear Riemann solver on one face, one wavefront: face averages, smax from
+// the eigenvalues, Bn = coefficientMatrix(mean of the averages); per node
+// FL = (Bn - smax I)^T (QR - QL)/2, FR = (Bn + smax I)^T (QR - QL)/2
+// (riemannSolverLinear.cpph:24-140).
+template <int N, class PDE>
+__device__ inline void riemann_face_linear(const double* __restrict__ QL, const double* __restrict__ QR,
+                                           double* __restrict__ FL, double* __restrict__ FR, int d,
+                                           const PdeParams& P, int lane) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2;
+  constexpr int NPL = (B2 + 63) / 64;
+  const DGTables& T = c_tab[N];
+  double aL[V], aR[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) aL[v] = aR[v] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NPL; ++k) {
+    const int node = lane + 64 * k;
+    if (node < B2) {
+      const double w = T.w[node / B] * T.w[node % B];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        aL[v] += w * QL[v * B2 + node];
+        aR[v] += w * QR[v * B2 + node];
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    aL[v] = wave_sum(aL[v]);
+    aR[v] = wave_sum(aR[v]);
+  }
+  const double sL = PDE::max_abs_eigenvalue(aL, d, P);
+  const double sR = PDE::max_abs_eigenvalue(aR, d, P);
+  const double smax = sL < sR ? sR : sL;
+  double Qa[V], Bm[V][V], Bp[V][V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) Qa[v] = 0.5 * (aL[v] + aR[v]);
+  PDE::coefficient_matrix(Qa, d, Bm, P);
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+#pragma unroll
+    for (int j = 0; j < V; ++j) Bp[i][j] = Bm[i][j];
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    Bm[i][i] -= smax;
+    Bp[i][i] += smax;
+  }
+#pragma unroll
+  for (int k = 0; k < NPL; ++k) {
+    const int node = lane + 64 * k;
+    if (node < B2) {
+      double Qd[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) Qd[v] = 0.5 * (QR[v * B2 + node] - QL[v * B2 + node]);
+#pragma unroll
+      for (int kk = 0; kk < V; ++kk) {
+        double l = 0.0, r = 0.0;
+#pragma unroll
+        for (int ll = 0; ll < V; ++ll) {
+          l += Bm[ll][kk] * Qd[ll];
+          r += Bp[ll][kk] * Qd[ll];
+        }
+        FL[kk * B2 + node] = l;
+        FR[kk * B2 + node] = r;
+      }
+    }
+  }
+}
+
+template <int N, class PDE>
+__global__ __launch_bounds__(256) void k_riemann_linear(const double* __restrict__ qf, double* __restrict__ ff,
+                                                        const int* __restrict__ nbr, int ncells, double gamma,
+                                                        long long wave0, long long nwaves, Ens ens) {
+  constexpr int B2 = Ord<N>::B2;
+  const long long local = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (local >= nwaves) return;
+  const long long wave = wave0 + local;
+  const int d = (int)(wave / ncells);
+  const int cl = (int)(wave % ncells);
+  const int cr = nbr[(size_t)d * ncells + cl];
+  const PdeParams P{gamma, {ens.adv[0], ens.adv[1], ens.adv[2]}};
+  riemann_face_linear<N, PDE>(qf + ((size_t)cl * 6 + 2 * d + 1) * V * B2, qf + ((size_t)cr * 6 + 2 * d) * V * B2,
+                              ff + ((size_t)cl * 6 + 2 * d + 1) * V * B2, ff + ((size_t)cr * 6 + 2 * d) * V * B2, d,
+                              P, lane);
+}
+
+// volumeIntegralLinear of the per-cell entry: lFhi [d][node][V] (reference
+// layout, d < 3) -> lduh [v][node].
+template <int N>
+__global__ __launch_bounds__(Ord<N>::NT) void k_volume_integral_linear(const double* __restrict__ lFhi,
+                                                                       double* __restrict__ lduh) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
+  const int s = threadIdx.x;
+  if (s >= B3) return;
+  const DGTables& T = c_tab[N];
+  const int z = s / B2, y = (s / B) % B, x = s % B;
+  const double w3 = (T.w[z] * T.w[y]) * T.w[x];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    double o = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) o -= w3 * lFhi[(d * B3 + s) * V + v];
+    lduh[v * B3 + s] = o;
+  }
+}
+
 }  // namespace exa

@@ -266,10 +266,13 @@ extern "C" int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out)
               cfg->nvar == 5,
           "the compiled PDEs are 3D Euler (optionally with gravity or the test non-conservative product) "
           "and linear advection (flux or non-conservative form), all with 5 variables");
-  EXA_ARG(!exa_pde_generic(cfg->pde) || (cfg->precision == EXA_PREC_FP64 && cfg->order <= 7 &&
+  EXA_ARG(!exa_pde_generic(cfg->pde) || (cfg->precision == EXA_PREC_FP64 && (cfg->linear || cfg->order <= 7) &&
                                          cfg->contraction == EXA_CONTRACT_DEFAULT && cfg->npar == 0),
           "the advection and non-conservative-product PDEs run the FP64 predictor at orders 1..7 "
-          "(default contraction, no parameters)");
+          "(1..9 with the linear kernels; default contraction, no parameters)");
+  EXA_ARG(!cfg->linear || exa_pde_linear_supported(cfg->pde),
+          "the linear kernels need a PDE with a coefficient matrix: EXA_PDE_ADVECTION_NCP or "
+          "EXA_PDE_EULER_NCP_TEST");
   EXA_ARG(cfg->pde != EXA_PDE_EULER_GRAVITY || cfg->precision == EXA_PREC_FP64,
           "the gravity source runs in the FP64 predictor only");
   EXA_ARG(cfg->npar >= 0 && cfg->npar <= 3, "npar must be 0..3");
@@ -357,11 +360,55 @@ static void param_predictor(int N, int nd, const double* luh, double* lQhbnd, st
         }
 }
 
+// spaceTimePredictorLinear of one cell (+ its volume integral, not returned).
+static int linear_predictor_cell(exa_solver* s, double* lQhbnd, double* lFhbnd, double* lQi, double* lFi,
+                                 double* lQhi, double* lFhi, const double* luh, const double dx[3], double dt,
+                                 int* picard_iterations) {
+  const int N = s->cfg.order, B = N + 1, B2 = B * B, B3 = B2 * B, B4 = B2 * B2, V = exa::V;
+  std::vector<double> h(std::max<size_t>((size_t)V * B3, (size_t)6 * V * B2));
+  ref_to_soa_cell(luh, h.data(), B3, V);
+  EXA_TRY(s->luh.ensure(sizeof(double) * V * B3));
+  EXA_TRY(s->lduh.ensure(sizeof(double) * V * B3));
+  EXA_TRY(s->qf.ensure(sizeof(double) * 6 * V * B2));
+  EXA_TRY(s->ff.ensure(sizeof(double) * 6 * V * B2));
+  EXA_TRY(s->iters.ensure(sizeof(int) * 4));
+  EXA_TRY(s->dt.ensure(sizeof(double) * 4));
+  const size_t nQi = (size_t)(B + 1) * B3 * V, nFi = (size_t)B4 * 3 * V, nQhi = (size_t)B3 * V,
+               nFhi = (size_t)4 * B3 * V;
+  EXA_TRY(s->dbg.ensure(sizeof(double) * (nQi + nFi + nQhi + nFhi)));
+  exa::PredictorDebug dbg{s->dbg.as<double>(), s->dbg.as<double>() + nQi, s->dbg.as<double>() + nQi + nFi,
+                          s->dbg.as<double>() + nQi + nFi + nQhi};
+  EXA_CHECK(hipMemcpyAsync(s->luh.p, h.data(), sizeof(double) * V * B3, hipMemcpyHostToDevice, s->stream));
+  EXA_CHECK(hipMemcpyAsync(s->dt.p, &dt, sizeof(double), hipMemcpyHostToDevice, s->stream));
+  hipLaunchKernelGGL(exa_pde_linear_predictor(s->cfg.pde, N, true), dim3(1), dim3(round_up64(B3)),
+                     (size_t)exa_pde_linear_lds_bytes(N), s->stream, s->luh.as<double>(), s->lduh.as<double>(),
+                     s->qf.as<double>(), s->ff.as<double>(), s->iters.as<int>(), s->dt.as<double>(), dx[0], dx[1],
+                     dx[2], s->cfg.gamma, nullptr, dbg, launch_extras(s->cfg));
+  EXA_CHECK(hipGetLastError());
+  std::vector<double> hq(6 * V * B2), hf(6 * V * B2), hdbg(nQi + nFi + nQhi + nFhi);
+  EXA_CHECK(hipMemcpyAsync(hq.data(), s->qf.p, sizeof(double) * 6 * V * B2, hipMemcpyDeviceToHost, s->stream));
+  EXA_CHECK(hipMemcpyAsync(hf.data(), s->ff.p, sizeof(double) * 6 * V * B2, hipMemcpyDeviceToHost, s->stream));
+  EXA_CHECK(hipMemcpyAsync(hdbg.data(), s->dbg.p, sizeof(double) * hdbg.size(), hipMemcpyDeviceToHost, s->stream));
+  EXA_TRY(sync_stream(s));
+  soa_to_ref_faces(hq.data(), lQhbnd, 6, B2, V);
+  soa_to_ref_faces(hf.data(), lFhbnd, 6, B2, V);
+  if (lQi) std::memcpy(lQi, hdbg.data(), sizeof(double) * nQi);
+  if (lFi) std::memcpy(lFi, hdbg.data() + nQi, sizeof(double) * nFi);
+  if (lQhi) std::memcpy(lQhi, hdbg.data() + nQi + nFi, sizeof(double) * nQhi);
+  if (lFhi) std::memcpy(lFhi, hdbg.data() + nQi + nFi + nQhi, sizeof(double) * nFhi);
+  if (picard_iterations) *picard_iterations = 0;
+  return EXA_OK;
+}
+
 extern "C" int exa_aderdg_space_time_predictor(exa_solver* s, double* lQhbnd, double* lFhbnd, double* lQi,
                                                double* lFi, double* lQhi, double* lFhi, const double* luh,
                                                const double dx[3], double dt, int* picard_iterations) {
   EXA_ARG(s && lQhbnd && lFhbnd && luh && dx, "null pointer");
   std::lock_guard<std::mutex> lk(s->mu);
+  if (s->cfg.linear) {
+    EXA_CHECK(hipSetDevice(s->device));
+    return linear_predictor_cell(s, lQhbnd, lFhbnd, lQi, lFi, lQhi, lFhi, luh, dx, dt, picard_iterations);
+  }
   EXA_CHECK(hipSetDevice(s->device));
   const int N = s->cfg.order, B = N + 1, B2 = B * B, B3 = B2 * B, B4 = B2 * B2;
   const int nd = s->cfg.nvar + s->cfg.npar, V = exa::V, P = s->cfg.npar;
@@ -445,8 +492,12 @@ extern "C" int exa_aderdg_volume_integral(exa_solver* s, double* lduh, const dou
   const int N = s->cfg.order, B = N + 1, B3 = B * B * B, V = exa::V;
   EXA_TRY(s->misc.ensure(sizeof(double) * 4 * B3 * V));
   EXA_TRY(s->lduh.ensure(sizeof(double) * B3 * V));
-  EXA_CHECK(hipMemcpyAsync(s->misc.p, lFhi, sizeof(double) * 4 * B3 * V, hipMemcpyHostToDevice, s->stream));
-  EXA_ORDER_SWITCH(N, {
+  EXA_CHECK(hipMemcpyAsync(s->misc.p, lFhi, sizeof(double) * (s->cfg.linear ? 3 : 4) * B3 * V,
+                           hipMemcpyHostToDevice, s->stream));
+  if (s->cfg.linear) {
+    hipLaunchKernelGGL(exa_pde_linear_volume(N), dim3(1), dim3(round_up64(B3)), 0, s->stream, s->misc.as<double>(),
+                       s->lduh.as<double>());
+  } else EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
     hipLaunchKernelGGL((exa::k_volume_integral_ref<NN>), dim3(1), dim3(O::NT), (size_t)volume_integral_lds_bytes<NN>(),
                        s->stream, s->misc.as<double>(), s->lduh.as<double>(), dx[0], dx[1], dx[2],
@@ -491,7 +542,8 @@ static int run_update_cell(exa_solver* s, double* luh_ref, double* lduh_ref, con
   double* dtp = s->dt.as<double>();
   auto* dtn = reinterpret_cast<unsigned long long*>(dtp + 1);
   if ((MODE & exa::UPD_DT) && exa_pde_generic(s->cfg.pde)) {
-    hipLaunchKernelGGL(exa_pde_update(s->cfg.pde, N, MODE), dim3(1), dim3(round_up64(B3)), 0, s->stream,
+    hipLaunchKernelGGL(s->cfg.linear ? exa_pde_linear_update(s->cfg.pde, N, MODE) : exa_pde_update(s->cfg.pde, N, MODE),
+                       dim3(1), dim3(round_up64(B3)), 0, s->stream,
                        s->luh.as<double>(), s->lduh.as<double>(), s->ff.as<double>(), dtp, dtn, dx[0], dx[1], dx[2],
                        s->cfg.gamma, s->cfg.cfl, launch_extras(s->cfg));
   } else EXA_ORDER_SWITCH(N, {
@@ -521,6 +573,9 @@ extern "C" int exa_aderdg_surface_integral(exa_solver* s, double* lduh, const do
   EXA_ARG(s && lduh && lFhbnd && dx, "null pointer");
   std::lock_guard<std::mutex> lk(s->mu);
   EXA_CHECK(hipSetDevice(s->device));
+  if (s->cfg.linear)
+    return run_update_cell<exa::UPD_SURFACE | exa::UPD_WRITE_LDUH | exa::UPD_LINEAR>(s, nullptr, lduh, lFhbnd, dx,
+                                                                                      0.0, nullptr);
   return run_update_cell<exa::UPD_SURFACE | exa::UPD_WRITE_LDUH>(s, nullptr, lduh, lFhbnd, dx, 0.0, nullptr);
 }
 
@@ -562,7 +617,8 @@ extern "C" int exa_aderdg_riemann_solver(exa_solver* s, double* FL, double* FR, 
   EXA_CHECK(hipMemcpy(s->ff.p, hf.data(), sizeof(double) * hf.size(), hipMemcpyHostToDevice));
   EXA_CHECK(hipMemcpy(s->misc.p, nbr, sizeof(nbr), hipMemcpyHostToDevice));
   if (exa_pde_generic(s->cfg.pde)) {
-    hipLaunchKernelGGL(exa_pde_riemann(s->cfg.pde, N), dim3(1), dim3(64), 0, s->stream, s->qf.as<double>(),
+    hipLaunchKernelGGL(s->cfg.linear ? exa_pde_linear_riemann(s->cfg.pde, N) : exa_pde_riemann(s->cfg.pde, N),
+                       dim3(1), dim3(64), 0, s->stream, s->qf.as<double>(),
                        s->ff.as<double>(), s->misc.as<int>(), 2, s->cfg.gamma, (long long)(2 * d), 1LL,
                        launch_extras(s->cfg));
   } else EXA_ORDER_SWITCH(N, {
@@ -1028,7 +1084,10 @@ template <int MODE>
 static int launch_update(exa_grid* g) {
   const int N = g->s->cfg.order;
   if (exa_pde_generic(g->s->cfg.pde)) {
-    hipLaunchKernelGGL(exa_pde_update(g->s->cfg.pde, N, MODE), dim3((unsigned)g->nc), dim3(round_up64((N + 1) * (N + 1) * (N + 1))),
+    PdeUpdateFn fn = g->s->cfg.linear
+                         ? exa_pde_linear_update(g->s->cfg.pde, N, (MODE & exa::UPD_SURFACE) ? MODE | exa::UPD_LINEAR : MODE)
+                         : exa_pde_update(g->s->cfg.pde, N, MODE);
+    hipLaunchKernelGGL(fn, dim3((unsigned)g->nc), dim3(round_up64((N + 1) * (N + 1) * (N + 1))),
                        0, g->stream, g->luh.as<double>(), g->lduh.as<double>(), g->ff.as<double>(), g->dt_cur(),
                        g->dt_next(), g->dx[0], g->dx[1], g->dx[2], g->s->cfg.gamma, g->s->cfg.cfl,
                        launch_extras(g->s->cfg));
@@ -1098,9 +1157,12 @@ static int enqueue_step(exa_grid* g, bool timed) {
   exa::PredictorDebug nodbg;
   nodbg.lQi = nodbg.lFi = nodbg.lQhi = nodbg.lFhi = nullptr;
   const bool generic = exa_pde_generic(g->s->cfg.pde);
+  const bool linear = g->s->cfg.linear != 0;
   if (generic) {
-    hipLaunchKernelGGL(exa_pde_predictor(g->s->cfg.pde, N, false), dim3((unsigned)g->nc),
-                       dim3(round_up64((N + 1) * (N + 1) * (N + 1))), (size_t)exa_pde_predictor_lds_bytes(N),
+    hipLaunchKernelGGL(linear ? exa_pde_linear_predictor(g->s->cfg.pde, N, false)
+                              : exa_pde_predictor(g->s->cfg.pde, N, false),
+                       dim3((unsigned)g->nc), dim3(round_up64((N + 1) * (N + 1) * (N + 1))),
+                       (size_t)(linear ? exa_pde_linear_lds_bytes(N) : exa_pde_predictor_lds_bytes(N)),
                        g->stream, g->luh.as<double>(), g->lduh.as<double>(), g->qf.as<double>(), g->ff.as<double>(),
                        g->iters.as<int>(), g->dt_cur(), g->dx[0], g->dx[1], g->dx[2], g->s->cfg.gamma,
                        g->iter_sum(), nodbg, launch_extras(g->s->cfg));
@@ -1129,7 +1191,8 @@ static int enqueue_step(exa_grid* g, bool timed) {
   const long long waves = 3LL * g->nc;
   const unsigned blocks = (unsigned)((waves * 64 + 255) / 256);
   if (generic) {
-    hipLaunchKernelGGL(exa_pde_riemann(g->s->cfg.pde, N), dim3(blocks), dim3(256), 0, g->stream, g->qf.as<double>(),
+    hipLaunchKernelGGL(linear ? exa_pde_linear_riemann(g->s->cfg.pde, N) : exa_pde_riemann(g->s->cfg.pde, N),
+                       dim3(blocks), dim3(256), 0, g->stream, g->qf.as<double>(),
                        g->ff.as<double>(), g->nbr.as<int>(), (int)g->nc, g->s->cfg.gamma, 0LL, waves,
                        launch_extras(g->s->cfg));
   } else EXA_ORDER_SWITCH(N, {

@@ -91,6 +91,16 @@ struct PdeAdvectionNcp {
   __device__ static inline void ncp(const double*, const double G[3][5], double out[5], const PdeParams& p) {
     for (int v = 0; v < 5; ++v) out[v] = p.adv[0] * G[0][v] + p.adv[1] * G[1][v] + p.adv[2] * G[2][v];
   }
+  // the linear scheme's product: a dim x nVar tensor B_d dQ/dx_d
+  // (spaceTimePredictorLinear.cpph:28-30), and coefficientMatrix B_d = a_d I
+  __device__ static inline void ncp_linear(const double*, const double G[3][5], double T[3][5], const PdeParams& p) {
+    for (int d = 0; d < 3; ++d)
+      for (int v = 0; v < 5; ++v) T[d][v] = p.adv[d] * G[d][v];
+  }
+  __device__ static inline void coefficient_matrix(const double*, int d, double Bn[5][5], const PdeParams& p) {
+    for (int i = 0; i < 5; ++i)
+      for (int j = 0; j < 5; ++j) Bn[i][j] = i == j ? p.adv[d] : 0.0;
+  }
 };
 
 // 3D Euler with the test non-conservative product of the reference's Euler
@@ -116,6 +126,18 @@ struct PdeEulerNcpTest {
     out[2] = 3.0;
     out[3] = G[0][0];
     out[4] = 0.7;
+  }
+  // in the linear scheme the test product fills the first row of the tensor
+  // (it writes nVar values); the coefficient matrix of the test is zero
+  // (GenericEulerKernelTest3D.cpp:108-131)
+  __device__ static inline void ncp_linear(const double* Q, const double G[3][5], double T[3][5],
+                                           const PdeParams& p) {
+    ncp(Q, G, T[0], p);
+    for (int v = 0; v < 5; ++v) T[1][v] = T[2][v] = 0.0;
+  }
+  __device__ static inline void coefficient_matrix(const double*, int, double Bn[5][5], const PdeParams&) {
+    for (int i = 0; i < 5; ++i)
+      for (int j = 0; j < 5; ++j) Bn[i][j] = 0.0;
   }
 };
 

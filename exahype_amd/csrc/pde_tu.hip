@@ -13,8 +13,36 @@ PdeUpdateFn upd(int mode) {
   constexpr int FULL = exa::UPD_SURFACE | exa::UPD_UPDATE | exa::UPD_DT;
   if (mode == exa::UPD_DT) return &exa::k_update<N, exa::UPD_DT, PDE>;
   if (mode == FULL) return &exa::k_update<N, FULL, PDE>;
+  if (mode == (FULL | exa::UPD_LINEAR)) return &exa::k_update<N, FULL | exa::UPD_LINEAR, PDE>;
   return nullptr;
 }
+template <class PDE, int N>
+PdePredFn lin_pred(bool debug) {
+  return debug ? &exa::k_predictor_linear<N, PDE, true> : &exa::k_predictor_linear<N, PDE, false>;
+}
+
+// F(PDE, N) for the linear family: the PDEs with a coefficient matrix, orders 1..9
+#define EXA_LINEAR_DISPATCH(pde, order, F)                                              \
+  do {                                                                                  \
+    switch (pde) {                                                                      \
+      case EXA_PDE_ADVECTION_NCP: EXA_LINEAR_ORDERS(exa::PdeAdvectionNcp, order, F); break; \
+      case EXA_PDE_EULER_NCP_TEST: EXA_LINEAR_ORDERS(exa::PdeEulerNcpTest, order, F); break; \
+      default: break;                                                                   \
+    }                                                                                   \
+  } while (0)
+#define EXA_LINEAR_ORDERS(P, order, F) \
+  switch (order) {                     \
+    case 1: F(P, 1); break;            \
+    case 2: F(P, 2); break;            \
+    case 3: F(P, 3); break;            \
+    case 4: F(P, 4); break;            \
+    case 5: F(P, 5); break;            \
+    case 6: F(P, 6); break;            \
+    case 7: F(P, 7); break;            \
+    case 8: F(P, 8); break;            \
+    case 9: F(P, 9); break;            \
+    default: break;                    \
+  }
 
 // F(PDE, N) for the PDE of an id and an order 1..7
 #define EXA_PDE_DISPATCH(pde, order, F)                                     \
@@ -88,6 +116,52 @@ PdeUpdateFn exa_pde_update(int pde, int order, int mode) {
   return f;
 }
 
+bool exa_pde_linear_supported(int pde) { return pde == EXA_PDE_ADVECTION_NCP || pde == EXA_PDE_EULER_NCP_TEST; }
+
+PdePredFn exa_pde_linear_predictor(int pde, int order, bool debug) {
+  PdePredFn f = nullptr;
+#define F_(P, N) f = lin_pred<P, N>(debug)
+  EXA_LINEAR_DISPATCH(pde, order, F_);
+#undef F_
+  return f;
+}
+
+int exa_pde_linear_lds_bytes(int order) {
+  const int B = order + 1;
+  return 4 * exa::V * B * B * B * (int)sizeof(double);
+}
+
+PdeRiemannFn exa_pde_linear_riemann(int pde, int order) {
+  PdeRiemannFn f = nullptr;
+#define F_(P, N) f = &exa::k_riemann_linear<N, P>
+  EXA_LINEAR_DISPATCH(pde, order, F_);
+#undef F_
+  return f;
+}
+
+PdeUpdateFn exa_pde_linear_update(int pde, int order, int mode) {
+  PdeUpdateFn f = nullptr;
+#define F_(P, N) f = upd<P, N>(mode)
+  EXA_LINEAR_DISPATCH(pde, order, F_);
+#undef F_
+  return f;
+}
+
+LinearVolumeFn exa_pde_linear_volume(int order) {
+  switch (order) {
+    case 1: return &exa::k_volume_integral_linear<1>;
+    case 2: return &exa::k_volume_integral_linear<2>;
+    case 3: return &exa::k_volume_integral_linear<3>;
+    case 4: return &exa::k_volume_integral_linear<4>;
+    case 5: return &exa::k_volume_integral_linear<5>;
+    case 6: return &exa::k_volume_integral_linear<6>;
+    case 7: return &exa::k_volume_integral_linear<7>;
+    case 8: return &exa::k_volume_integral_linear<8>;
+    case 9: return &exa::k_volume_integral_linear<9>;
+    default: return nullptr;
+  }
+}
+
 hipError_t exa_pde_init(const DGTables* tab) {
   hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab), tab, sizeof(DGTables) * (EXA_MAX_ORDER + 1));
   if (e != hipSuccess) return e;
@@ -97,6 +171,14 @@ hipError_t exa_pde_init(const DGTables* tab) {
       for (int dbg = 0; dbg < 2; ++dbg) {
         e = hipFuncSetAttribute(reinterpret_cast<const void*>(exa_pde_predictor(pde, order, dbg)),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, exa_pde_predictor_lds_bytes(order));
+        if (e != hipSuccess) return e;
+      }
+  const int lin[2] = {EXA_PDE_ADVECTION_NCP, EXA_PDE_EULER_NCP_TEST};
+  for (int pde : lin)
+    for (int order = 1; order <= exa::LINEAR_MAX_ORDER; ++order)
+      for (int dbg = 0; dbg < 2; ++dbg) {
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(exa_pde_linear_predictor(pde, order, dbg)),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, exa_pde_linear_lds_bytes(order));
         if (e != hipSuccess) return e;
       }
   return hipSuccess;

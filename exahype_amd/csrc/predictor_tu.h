@@ -60,4 +60,16 @@ PdeRiemannListFn exa_pde_riemann_list(int pde, int order);
 // modes with UPD_DT (the only ones that depend on the PDE): UPD_DT and
 // UPD_SURFACE | UPD_UPDATE | UPD_DT
 PdeUpdateFn exa_pde_update(int pde, int order, int mode);
+// The linear kernel family (Cauchy-Kowalevski predictor with the linear
+// volume integral, coefficient-matrix Riemann, fluctuation-form surface
+// integral via UPD_LINEAR): EXA_PDE_ADVECTION_NCP and EXA_PDE_EULER_NCP_TEST
+// at orders 1..9; nullptr otherwise.
+using LinearVolumeFn = void (*)(const double*, double*);
+bool exa_pde_linear_supported(int pde);
+PdePredFn exa_pde_linear_predictor(int pde, int order, bool debug);
+int exa_pde_linear_lds_bytes(int order);
+PdeRiemannFn exa_pde_linear_riemann(int pde, int order);
+// modes UPD_DT and UPD_SURFACE | UPD_UPDATE | UPD_DT | UPD_LINEAR
+PdeUpdateFn exa_pde_linear_update(int pde, int order, int mode);
+LinearVolumeFn exa_pde_linear_volume(int order);
 hipError_t exa_pde_init(const DGTables* tab);

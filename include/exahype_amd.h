@@ -116,6 +116,16 @@ typedef struct {
   int contraction;  /* EXA_CONTRACT_*: form of the sweeps' spatial contraction */
   double gravity;   /* g of EXA_PDE_EULER_GRAVITY (ignored otherwise)          */
   double velocity[3]; /* advection velocity of EXA_PDE_ADVECTION(_NCP) (ignored otherwise) */
+  int linear;       /* 1: the linear kernel family, the generated glue's isLinear branch
+                       (TT/AbstractGenericADERDGSolverImplementation.template:55-192):
+                       Cauchy-Kowalevski predictor (spaceTimePredictorLinear.cpph),
+                       volumeIntegralLinear, surfaceIntegralLinear (fluctuations),
+                       riemannSolverLinear (coefficient matrix).  For
+                       EXA_PDE_ADVECTION_NCP and EXA_PDE_EULER_NCP_TEST, FP64,
+                       orders 1..9.  For a linear solver, spaceTimePredictor's lQi
+                       receives the B+1 time derivatives [t][z][y][x][nData] and lFi
+                       the products [t][z][y][x][3][nVar]; lFhi is [4][z][y][x][nVar]
+                       (block 3 zero) and volumeIntegral reads its first 3 blocks. */
 } exa_aderdg_config;
 
 #define EXA_CONTRACT_DEFAULT 0  /* tuned per order (the staged form for FP64 at N = 7)    */
