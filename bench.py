@@ -132,7 +132,7 @@ def main():
                     help="predictor sweep contraction: 0 tuned, 1 pencil owners, 2 node owners, "
                          "3 pencil owners two slices at a time, 4 staged iterate (FP64, orders 5..9)")
     ap.add_argument("--workload", default="aderdg", choices=["aderdg", "fv2d", "ensemble", "limiter", "muscl", "amr",
-                                                          "ncp"],
+                                                          "ncp", "linear"],
                     help="aderdg: the headline ADER-DG step; fv2d: the EulerFV demonstrator "
                          "(2D Godunov, 81x81 patches of 10x10 volumes, 200 steps); ensemble: independent "
                          "grids of random order/size/gamma advanced together; limiter: troubled-cell detection and "
@@ -165,6 +165,9 @@ def main():
         return
     if args.workload == "ncp":
         print(json.dumps(run_ncp(args)))
+        return
+    if args.workload == "linear":
+        print(json.dumps(run_linear(args)))
         return
     import exahype_amd as E
     N, n = args.order, args.cells
@@ -273,9 +276,9 @@ def advection_ic_kwargs():
     return dict(nmodes=2, k=[(1, 0, 1), (0, 1, 1)], a=[0.2, 0.1], phi=[0.3, 1.1], uinf=NCP_VELOCITY)
 
 
-def run_advection(pde, N, n, steps, warmup):
+def run_advection(pde, N, n, steps, warmup, linear=False):
     import exahype_amd as E
-    s = E.ADERDGSolver(N, pde=pde, velocity=NCP_VELOCITY)
+    s = E.ADERDGSolver(N, pde=pde, velocity=NCP_VELOCITY, linear=linear)
     g = E.Grid(s, n, n, n)
     g.set_initial_conditions(E.make_ic(3, **advection_ic_kwargs()))
     g.compute_dt()
@@ -347,6 +350,74 @@ def run_ncp(args):
         threads = min(len(os.sched_getaffinity(0)), 16)
         steps = max(1, min(8, int(round(8 * (6.0 / (N + 1)) ** 5))))
         og = O.OracleGrid(O.cfg(N, pde=4, adv=NCP_VELOCITY), 27, 27, 27)
+        for d in range(3):
+            og.g.width[d] = 27 / n
+        og.init(O.make_ic(kind=3, **advection_ic_kwargs()))
+        dt = og.stable_dt(threads)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            dt = og.step(dt, threads)
+        wall = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": og.ncells * NVAR * B ** 3 * steps / wall, "unit": "DoF-updates/s",
+                               "cores": threads, "kind": "port",
+                               "sample": "oracle (C, OpenMP), order %d, 27^3 periodic cells of size 1/%d, same "
+                                         "initial data, %d steps (%.1f s wall)" % (N, n, steps, wall)}
+    return out
+
+
+def run_linear(args):
+    """The linear kernel family (Cauchy-Kowalevski predictor, coefficient-
+    matrix Riemann solver, fluctuation-form surface integral) on linear
+    advection in non-conservative form, n^3 periodic cells at order N."""
+    import exahype_amd as E
+    N, n = args.order, args.cells
+    ncells = n ** 3
+    B = N + 1
+    dof = ncells * NVAR * B ** 3
+    r = run_advection(E.PDE_ADVECTION_NCP, N, n, args.steps, args.warmup, linear=True)
+    pred_ms = r["times"][0] / max(r["counts"][0], 1)
+    # per time level: dudx gradients 6 V B^4, products and the next level
+    # 6 V B^3, two Taylor accumulations 8 V B^3; then the volume integral
+    # (6 V B^3) and the face extrapolation (24 V B^3)
+    flops = (B * (6 * NVAR * B ** 4 + 14 * NVAR * B ** 3) + 30 * NVAR * B ** 3) * ncells
+    # compulsory bytes: luh read, lduh written, both extrapolated face arrays written
+    nbytes = 8 * (2 * NVAR * B ** 3 + 12 * NVAR * B ** 2) * ncells
+    t = pred_ms * 1e-3
+    ai = flops / nbytes
+    if ai >= PEAK_FP64_TFLOPS * 1e12 / (PEAK_HBM_GBPS * 1e9):
+        roof = {"bound": "mfma", "achieved": flops / t / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}
+    else:
+        roof = {"bound": "hbm", "achieved": nbytes / t / 1e9, "peak": PEAK_HBM_GBPS, "unit": "GB/s"}
+    roof.update(frac=roof["achieved"] / roof["peak"], traffic=None,
+                kernel="k_predictor_linear<%d, PdeAdvectionNcp>" % N, avg_launch_ms=pred_ms, flop_per_launch=flops,
+                bytes_per_launch=nbytes, flop_per_byte=ai)
+    out = {
+        "metric": "DoF-updates/s, 3D linear advection, linear ADER-DG kernels N=%d" % N,
+        "value": dof * args.steps / r["wall"],
+        "unit": "DoF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["wall"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "3D linear advection of 5 variables, linear ADER-DG kernels (Cauchy-Kowalevski "
+                               "predictor, coefficient-matrix Rusanov), order %d, %d^3 periodic cells, 2 sine modes, "
+                               "CFL 0.9" % (N, n),
+                   "velocity": list(NCP_VELOCITY), "cells": ncells, "order": N, "dof_per_step": dof,
+                   "parallelism": "single GPU"},
+        "roofline": roof,
+        "phase_ms_per_step": {"predictor": pred_ms, "riemann": r["times"][1] / max(r["counts"][1], 1),
+                              "update_surface_dt": r["times"][2] / max(r["counts"][2], 1)},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        steps = 20
+        og = O.OracleGrid(O.cfg(N, pde=4, adv=NCP_VELOCITY, linear=1), 27, 27, 27)
         for d in range(3):
             og.g.width[d] = 27 / n
         og.init(O.make_ic(kind=3, **advection_ic_kwargs()))
