@@ -206,7 +206,8 @@ def device_synchronize():
 
 
 def make_ic(kind=0, **kw):
-    """Build an InitialCondition: kind 0 vortex+modes, 1 Sod sphere, 2 constant, 4 entropy wave."""
+    """Build an InitialCondition: kind 0 vortex+modes, 1 Sod sphere, 2 constant, 3 sine modes
+    (linear-advection PDEs), 4 entropy wave, 5 Gaussian density pulse."""
     ic = InitialCondition()
     ic.kind = kind
     for k, v in kw.items():
