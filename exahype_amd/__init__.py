@@ -74,7 +74,8 @@ EXPORTS = [
     "exa_aderdg_riemann_solver", "exa_aderdg_solution_update", "exa_aderdg_stable_time_step_size",
     "exa_aderdg_adjust_solution", "exa_aderdg_batch_prediction", "exa_aderdg_batch_riemann",
     "exa_aderdg_batch_update", "exa_aderdg_batch_pending", "exa_aderdg_batch_flush", "exa_grid_create", "exa_grid_destroy", "exa_grid_num_cells",
-    "exa_grid_upload", "exa_grid_download", "exa_grid_set_initial_conditions", "exa_grid_compute_dt",
+    "exa_grid_upload", "exa_grid_download", "exa_grid_set_initial_conditions", "exa_grid_sample_cartesian",
+    "exa_grid_compute_dt",
     "exa_grid_set_dt", "exa_grid_step", "exa_grid_sync", "exa_grid_picard_iterations",
     "exa_grid_use_graph", "exa_grid_enable_timing", "exa_grid_kernel_times",
     "exa_grid_total_picard_iterations",
@@ -126,6 +127,7 @@ def load_library(path=LIB_PATH):
     L.exa_grid_num_cells.argtypes = [C.c_void_p, P(LL)]
     L.exa_grid_upload.argtypes = [C.c_void_p, _D]
     L.exa_grid_download.argtypes = [C.c_void_p, _D]
+    L.exa_grid_sample_cartesian.argtypes = [C.c_void_p, _D]
     L.exa_grid_set_initial_conditions.argtypes = [C.c_void_p, P(InitialCondition)]
     L.exa_grid_compute_dt.argtypes = [C.c_void_p, _D]
     L.exa_grid_set_dt.argtypes = [C.c_void_p, C.c_double]
@@ -357,6 +359,7 @@ class Grid:
         _check(_lib.exa_grid_create(solver._h, nx, ny, nz, _p(_vec3(offset)), _p(_vec3(width)), C.byref(h)))
         self._h = h
         self.nx, self.ny, self.nz = nx, ny, nz
+        self.offset, self.width = tuple(offset), tuple(width)
         self.ncells = nx * ny * nz
         self.cell_size = solver.B ** 3 * solver.nvar
 
@@ -380,6 +383,13 @@ class Grid:
         out = np.zeros(self.ncells * self.cell_size)
         _check(_lib.exa_grid_download(self._h, _p(out)))
         return out
+
+    def sample_cartesian(self):
+        """The solution at every cell's (N+1)^3 equidistant vertices (the
+        values of a Cartesian-vertices plot): [cell][vertex, x fastest][nVar]."""
+        out = np.zeros(self.ncells * self.solver.B ** 3 * self.solver.nvar)
+        _check(_lib.exa_grid_sample_cartesian(self._h, _p(out)))
+        return out.reshape(self.ncells, self.solver.B ** 3, self.solver.nvar)
 
     def set_initial_conditions(self, ic):
         _check(_lib.exa_grid_set_initial_conditions(self._h, C.byref(ic)))
@@ -508,6 +518,8 @@ class FVGrid:
         h = C.c_void_p()
         _check(_lib.exa_fv_grid_create(solver._h, nx, ny, nz, _p(_vec3(offset)), _p(_vec3(width)), C.byref(h)))
         self._h = h
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.offset, self.width = tuple(offset), tuple(width)
         self.npatch = nx * ny * nz
 
     def close(self):

@@ -1361,6 +1361,33 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_update(double* __restrict__ luh,
   }
 }
 
+// The solution at the (N+1)^3 equidistant vertices of every cell, as a
+// Cartesian-vertices plotter samples it (E/plotters/VTK/
+// ADERDG2CartesianVTK.cpp:288-336, same products and summation order):
+// out [cell][vertex][V], vertex (z*B + y)*B + x at offset + (x, y, z) dx / N.
+template <int N>
+__global__ __launch_bounds__(Ord<N>::NT) void k_sample_cartesian(const double* __restrict__ luh,
+                                                                 double* __restrict__ out) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
+  const DGTables& T = c_tab[N];
+  const size_t cell = blockIdx.x;
+  const int s = threadIdx.x;
+  if (s >= B3) return;
+  const int i2 = s / B2, i1 = (s / B) % B, i0 = s % B;
+  const double* u = luh + cell * V * B3;
+#pragma unroll 1
+  for (int v = 0; v < V; ++v) {
+    double acc = 0.0;
+    for (int j2 = 0; j2 < B; ++j2)
+      for (int j1 = 0; j1 < B; ++j1)
+#pragma unroll
+        for (int j0 = 0; j0 < B; ++j0)
+          acc += T.eq[j1][i1] * T.eq[j0][i0] * T.eq[j2][i2] * u[v * B3 + (j2 * B + j1) * B + j0];
+    out[(cell * B3 + s) * V + v] = acc;
+  }
+}
+
 // Initial conditions at the GL nodes (solutionAdjustment.cpph:17-52).
 template <int N>
 __global__ __launch_bounds__(Ord<N>::NT) void k_initial_conditions(double* __restrict__ luh, exa_ic ic,

@@ -138,6 +138,7 @@ extern "C" int exa_init(int device) {
         host[N].Kxi[i][j] = EXA_TAB_Kxi[N][i][j];
         host[N].iK1[i][j] = EXA_TAB_iK1[N][i][j];
         host[N].dudx[i][j] = EXA_TAB_dudx[N][i][j];
+        host[N].eq[i][j] = EXA_TAB_equidistantGridProjector1d[N][i][j];
       }
     }
   }
@@ -910,6 +911,7 @@ struct exa_grid {
   double off[3] = {0, 0, 0}, width[3] = {1, 1, 1}, dx[3] = {1, 1, 1};
   DevBuf luh, lduh, qf, ff, iters, nbr, state;
   DevBuf stage;    // staged predictor: one staging slot per resident workgroup
+  DevBuf sample;   // Cartesian-vertex samples (output)
   int nslots = 0;
   bool staged = false;  // decided at creation (the slots are allocated then)
   hipStream_t stream = nullptr;
@@ -1062,6 +1064,23 @@ extern "C" int exa_grid_download(exa_grid* g, double* luh) {
   EXA_CHECK(hipSetDevice(g->s->device));
   EXA_CHECK(hipStreamSynchronize(g->stream));
   return grid_transfer(g, luh, false);
+}
+
+extern "C" int exa_grid_sample_cartesian(exa_grid* g, double* out) {
+  EXA_ARG(g && out, "null pointer");
+  EXA_CHECK(hipSetDevice(g->s->device));
+  const int N = g->s->cfg.order, B = N + 1, B3 = B * B * B;
+  const size_t n = (size_t)g->nc * B3 * exa::V;
+  EXA_TRY(g->sample.ensure(sizeof(double) * n));
+  EXA_ORDER_SWITCH(N, {
+    using O = exa::Ord<NN>;
+    hipLaunchKernelGGL((exa::k_sample_cartesian<NN>), dim3((unsigned)g->nc), dim3(O::NT), 0, g->stream,
+                       g->luh.as<double>(), g->sample.as<double>());
+  });
+  EXA_CHECK(hipGetLastError());
+  EXA_CHECK(hipMemcpyAsync(out, g->sample.p, sizeof(double) * n, hipMemcpyDeviceToHost, g->stream));
+  EXA_CHECK(hipStreamSynchronize(g->stream));
+  return EXA_OK;
 }
 
 extern "C" int exa_grid_set_initial_conditions(exa_grid* g, const exa_ic* ic) {

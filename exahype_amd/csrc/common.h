@@ -33,6 +33,7 @@ struct DGTables {
   double Kxi[EXA_NB][EXA_NB];  // stiffness matrix Kxi[k][m]
   double iK1[EXA_NB][EXA_NB];  // inverse time matrix iK1[t'][t]
   double dudx[EXA_NB][EXA_NB]; // derivative of the basis dudx[l][n] = phi_n'(xi_l) (NCP gradients)
+  double eq[EXA_NB][EXA_NB];   // equidistantGridProjector1d[node][j] = phi_node(j/N) (plotting)
 };
 
 // one copy per translation unit; every unit that launches kernels reading the

@@ -244,6 +244,12 @@ int exa_grid_num_cells(const exa_grid* g, long long* n);
 int exa_grid_upload(exa_grid* g, const double* luh);
 int exa_grid_download(exa_grid* g, double* luh);
 int exa_grid_set_initial_conditions(exa_grid* g, const exa_ic* ic);
+/* The values a Cartesian-vertices plotter writes
+ * (exahype::plotters::ADERDG2CartesianVTK::plotVertexData,
+ * E/plotters/VTK/ADERDG2CartesianVTK.cpp:288-336): the solution of every
+ * cell at its (N+1)^3 equidistant vertices offset + (i, j, k) dx / N,
+ * computed on the device; out (host) [cell][vertex, x fastest][nVar]. */
+int exa_grid_sample_cartesian(exa_grid* g, double* out);
 /* Stable dt of the current state (global minimum over cells); also
  * becomes the dt of the next step.  Synchronous. */
 int exa_grid_compute_dt(exa_grid* g, double* dt);

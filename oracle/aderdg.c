@@ -736,3 +736,28 @@ double orc_grid_step(const orc_cfg* c, orc_grid* g, double dt, int nthreads) {
   /* TimeStepSizeComputation (E/solvers/ADERDGSolver.cpp:1948-2001) */
   return orc_grid_stable_dt(c, g, nthreads);
 }
+
+/* The values a Cartesian-vertices plotter writes for one cell: the solution
+ * at the (N+1)^3 equidistant vertices offset + i dx / N (x fastest), as
+ * ADERDG2CartesianVTK::plotVertexData sums them
+ * (E/plotters/VTK/ADERDG2CartesianVTK.cpp:288-336):
+ * out[i][m] = sum_ii P[ii_y][i_y] P[ii_x][i_x] P[ii_z][i_z] luh[ii][m] with
+ * P = equidistantGridProjector1d. */
+void orc_sample_cartesian_vertices(const orc_cfg* c, const double* luh, double* out) {
+  const int N = c->order, B = N + 1, nv = c->nvar, nd = c->nvar + c->npar;
+  const double* P = &EXA_TAB_equidistantGridProjector1d[N][0][0];
+#define EQ(g, e) P[(g) * MAXB + (e)]
+  for (int i2 = 0; i2 < B; i2++)
+    for (int i1 = 0; i1 < B; i1++)
+      for (int i0 = 0; i0 < B; i0++)
+        for (int m = 0; m < nv; m++) {
+          double acc = 0.0;
+          for (int j2 = 0; j2 < B; j2++)
+            for (int j1 = 0; j1 < B; j1++)
+              for (int j0 = 0; j0 < B; j0++)
+                acc += EQ(j1, i1) * EQ(j0, i0) * EQ(j2, i2) * luh[((j2 * B + j1) * B + j0) * nd + m];
+          out[((i2 * B + i1) * B + i0) * nv + m] = acc;
+        }
+#undef EQ
+}
+

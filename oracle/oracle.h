@@ -73,6 +73,10 @@ void orc_riemann_solver(const orc_cfg* c, double* FL, double* FR, const double* 
 void orc_solution_update(const orc_cfg* c, double* luh, const double* lduh, double dt);
 double orc_stable_time_step_size(const orc_cfg* c, const double* luh, const double dx[3]);
 
+/* Cartesian-vertex sampling of one cell (ADERDG2CartesianVTK::plotVertexData):
+ * out [(N+1)^3 vertices, x fastest][nvar] */
+void orc_sample_cartesian_vertices(const orc_cfg* c, const double* luh, double* out);
+
 /* the linear kernels of kernels/aderdg/generic/c/3d (oracle/linear.c) */
 void orc_ncp_linear(const orc_cfg* c, const double* Q, const double* gradQ, double* T /* [3][nvar] */);
 void orc_coefficient_matrix(const orc_cfg* c, const double* Q, int d, double* Bn /* [nvar][nvar] */);

@@ -63,6 +63,7 @@ def lib():
         L.orc_source.argtypes = [P(Cfg), _D, _D]
         L.orc_ncp.argtypes = [P(Cfg), _D, _D, _D]
         L.orc_ncp_linear.argtypes = [P(Cfg), _D, _D, _D]
+        L.orc_sample_cartesian_vertices.argtypes = [P(Cfg), _D, _D]
         L.orc_coefficient_matrix.argtypes = [P(Cfg), _D, C.c_int, _D]
         L.orc_space_time_predictor_linear.argtypes = [P(Cfg), _D, _D, _D, _D, _D, _D, _D, _D, C.c_double]
         L.orc_volume_integral_linear.argtypes = [P(Cfg), _D, _D, _D]
@@ -214,6 +215,12 @@ def riemann_solver_linear(c, QL, QR, dt, direction):
     FL, FR = np.zeros(B * B * c.nvar), np.zeros(B * B * c.nvar)
     lib().orc_riemann_solver_linear(C.byref(c), ptr(FL), ptr(FR), ptr(dvec(QL)), ptr(dvec(QR)), dt, direction)
     return FL, FR
+
+
+def sample_cartesian_vertices(c, luh):
+    out = np.zeros((c.order + 1) ** 3 * c.nvar)
+    lib().orc_sample_cartesian_vertices(C.byref(c), ptr(dvec(luh)), ptr(out))
+    return out
 
 
 def coefficient_matrix(c, Q, d):
