@@ -378,7 +378,7 @@ int exa_fv_grid_enable_timing(exa_fv_grid* g, int enable);
 int exa_fv_grid_kernel_times(exa_fv_grid* g, double times[4], long long counts[4]);
 
 /* ---------------------------------------------------------------------- */
-/* Subcell limiter (orders 1..5, 2N+1 subcells per direction, 5 variables).
+/* Subcell limiter (orders 1..9, 2N+1 subcells per direction, 5 variables).
  * luh [z][y][x][nVar]; lim [z][y][x][nVar] of side 2N+1+2*ghost, of which the
  * interior is read or written. */
 
@@ -397,7 +397,7 @@ int exa_limiter_project_on_dg(int order, const double* lim, int ghost, double* l
 int exa_limiter_dmp(int order, const double* luh, int nobs, double relax, double scaling, double* bmin,
                     double* bmax, int* admissible);
 
-/* ---- Limiter on a device grid (orders 1..5, Euler observables) ----------
+/* ---- Limiter on a device grid (orders 1..9, Euler observables) ----------
  * The per-step troubled-cell work of LimitingADERDGSolver on a single-level
  * periodic grid (E/solvers/LimitingADERDGSolver.cpp:838-1048), one launch
  * per stage over all cells:
