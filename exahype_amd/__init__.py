@@ -44,7 +44,9 @@ class AderdgConfig(C.Structure):
     _fields_ = [("order", C.c_int), ("nvar", C.c_int), ("npar", C.c_int), ("cfl", C.c_double),
                 ("gamma", C.c_double), ("pde", C.c_int), ("precision", C.c_int), ("fp64_sweeps", C.c_int),
                 ("waves_per_simd", C.c_int), ("contraction", C.c_int), ("gravity", C.c_double),
-                ("velocity", C.c_double * 3), ("linear", C.c_int)]
+                ("velocity", C.c_double * 3), ("linear", C.c_int), ("point_source", C.c_int),
+                ("source_x0", C.c_double * 3), ("source_amp", C.c_double * 5), ("source_f0", C.c_double),
+                ("source_t0", C.c_double)]
 
 
 class EnsembleMember(C.Structure):
@@ -70,7 +72,7 @@ class InitialCondition(C.Structure):
 # every symbol declared in include/exahype_amd.h
 EXPORTS = [
     "exa_last_error", "exa_version", "exa_init", "exa_device_synchronize", "exa_aderdg_create", "exa_aderdg_destroy",
-    "exa_aderdg_space_time_predictor", "exa_aderdg_volume_integral", "exa_aderdg_surface_integral",
+    "exa_aderdg_space_time_predictor", "exa_aderdg_space_time_predictor_ps", "exa_aderdg_point_source", "exa_aderdg_volume_integral", "exa_aderdg_surface_integral",
     "exa_aderdg_riemann_solver", "exa_aderdg_solution_update", "exa_aderdg_stable_time_step_size",
     "exa_aderdg_adjust_solution", "exa_aderdg_batch_prediction", "exa_aderdg_batch_riemann",
     "exa_aderdg_batch_update", "exa_aderdg_batch_pending", "exa_aderdg_batch_flush", "exa_grid_create", "exa_grid_destroy", "exa_grid_num_cells",
@@ -111,6 +113,9 @@ def load_library(path=LIB_PATH):
     L.exa_aderdg_create.argtypes = [P(AderdgConfig), P(C.c_void_p)]
     L.exa_aderdg_destroy.argtypes = [C.c_void_p]
     L.exa_aderdg_space_time_predictor.argtypes = [C.c_void_p, _D, _D, _D, _D, _D, _D, _D, _D, C.c_double, P(I)]
+    L.exa_aderdg_space_time_predictor_ps.argtypes = [C.c_void_p, _D, _D, _D, _D, _D, _D, _D, _D, C.c_double, _D,
+                                                     P(I)]
+    L.exa_aderdg_point_source.argtypes = [C.c_void_p, C.c_double, C.c_double, _D, _D, _D]
     L.exa_aderdg_volume_integral.argtypes = [C.c_void_p, _D, _D, _D]
     L.exa_aderdg_surface_integral.argtypes = [C.c_void_p, _D, _D, _D]
     L.exa_aderdg_riemann_solver.argtypes = [C.c_void_p, _D, _D, _D, _D, C.c_double, I]
@@ -231,14 +236,16 @@ class ADERDGSolver:
 
     def __init__(self, order, nvar=5, npar=0, cfl=0.9, gamma=1.4, precision=PREC_FP64, fp64_sweeps=1,
                  waves_per_simd=0, contraction=0, device=0, gravity=None, pde=None, velocity=(0.0, 0.0, 0.0),
-                 linear=False):
+                 linear=False, point_source=None):
         """gravity: None for the Euler equations, g for Euler with constant
         gravity along -z (EXA_PDE_EULER_GRAVITY, an algebraic source).
         pde: one of the PDE_* ids to select another compiled PDE (the
         advection PDEs take their velocity from `velocity`).
         linear: the linear kernel family (Cauchy-Kowalevski predictor,
         coefficient-matrix Riemann solver), for PDE_ADVECTION_NCP and
-        PDE_EULER_NCP_TEST."""
+        PDE_EULER_NCP_TEST.
+        point_source (linear solvers): None, or dict(x0=(3,), amp=(5,), f0=,
+        t0=) -- a Ricker-wavelet point force at x0 (usePointSource)."""
         L = load_library()
         _check(L.exa_init(device))
         if pde is None:
@@ -246,6 +253,13 @@ class ADERDGSolver:
         self.cfg = AderdgConfig(order, nvar, npar, cfl, gamma, pde, precision, fp64_sweeps, waves_per_simd,
                                 contraction, 0.0 if gravity is None else float(gravity),
                                 (C.c_double * 3)(*[float(a) for a in velocity]), 1 if linear else 0)
+        if point_source is not None:
+            self.cfg.point_source = 1
+            for d in range(3):
+                self.cfg.source_x0[d] = float(point_source["x0"][d])
+            for v in range(5):
+                self.cfg.source_amp[v] = float(point_source["amp"][v])
+            self.cfg.source_f0, self.cfg.source_t0 = float(point_source["f0"]), float(point_source["t0"])
         h = C.c_void_p()
         _check(L.exa_aderdg_create(C.byref(self.cfg), C.byref(h)))
         self._h = h
@@ -267,7 +281,14 @@ class ADERDGSolver:
             pass
 
     # --- virtual kernel entry points (reference names) ---
-    def spaceTimePredictor(self, luh, dx, dt):
+    def pointSource(self, t, dt, centre, dx):
+        """ADERDGSolver::pointSource: the cell's point forces [B+1][z][y][x][nVar]."""
+        psi = np.zeros((self.B + 1) * self.B ** 3 * self.nvar)
+        _check(_lib.exa_aderdg_point_source(self._h, float(t), float(dt), _p(_vec3(centre)), _p(_vec3(dx)),
+                                            _p(psi)))
+        return psi
+
+    def spaceTimePredictor(self, luh, dx, dt, pointForceSources=None):
         B, nd, nv = self.B, self.ndata, self.nvar
         luh = np.ascontiguousarray(luh, dtype=np.float64)
         # a linear solver returns the B+1 time derivatives and the 3 products per time level
@@ -276,9 +297,11 @@ class ADERDGSolver:
                    lFi=np.zeros(B ** 4 * (3 if self.linear else 4) * nv), lQhi=np.zeros(B ** 3 * nd),
                    lFhi=np.zeros(4 * B ** 3 * nv))
         it = C.c_int(0)
-        _check(_lib.exa_aderdg_space_time_predictor(self._h, _p(out["lQhbnd"]), _p(out["lFhbnd"]), _p(out["lQi"]),
-                                                    _p(out["lFi"]), _p(out["lQhi"]), _p(out["lFhi"]), _p(luh),
-                                                    _p(_vec3(dx)), float(dt), C.byref(it)))
+        psi = None if pointForceSources is None else np.ascontiguousarray(pointForceSources, dtype=np.float64)
+        _check(_lib.exa_aderdg_space_time_predictor_ps(self._h, _p(out["lQhbnd"]), _p(out["lFhbnd"]),
+                                                       _p(out["lQi"]), _p(out["lFi"]), _p(out["lQhi"]),
+                                                       _p(out["lFhi"]), _p(luh), _p(_vec3(dx)), float(dt),
+                                                       _p(psi), C.byref(it)))
         out["iters"] = it.value
         return out
 

@@ -116,6 +116,7 @@ struct Ens {
   const exa_ic* ics;  // per-instance initial conditions
   double grav;        // gravity g along -z of the Euler-with-gravity PDE; 0: no source
   double adv[3];      // advection velocity of the linear-advection PDEs
+  PointSrc ps;        // point source of the linear predictor (mode 0: none)
 };
 
 // Algebraic source of Euler under gravity g along -z (the useSource arm of

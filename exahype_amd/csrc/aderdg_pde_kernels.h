@@ -337,6 +337,82 @@ __global__ __launch_bounds__(256) void k_riemann_list_pde(const double* __restri
 // orders 1..9.
 constexpr int LINEAR_MAX_ORDER = 9;
 
+// ---------------------------------------------------------------------------
+// Point sources (kernels/aderdg/generic/c/3d/pointSource.cpph:28-145).
+// Multiplies and adds stay separately rounded here, as in the reference's
+// host build: the source position in reference coordinates decides whether a
+// cell holds the source.
+
+// Lagrange basis polynomial of GL node i at xi from its definition,
+// prod_{m != i} (xi - xi_m) / (xi_i - xi_m) (the reference evaluates
+// generated literals of the same polynomials, kernels/DGBasisFunctions.cpp).
+template <int N>
+__device__ inline double lagrange(int i, double xi) {
+#pragma clang fp contract(off)
+  const DGTables& T = c_tab[N];
+  double num = 1.0, den = 1.0;
+#pragma unroll
+  for (int m = 0; m <= N; ++m) {
+    if (m == i) continue;
+    num *= xi - T.xi[m];
+    den *= T.xi[i] - T.xi[m];
+  }
+  return num / den;
+}
+
+// The user pointSource(x, t, dt, forceVector, x0) (E/solvers/
+// UserSolverInterface.h:91-95): the Ricker wavelet force of PointSrc.
+__device__ inline void user_point_source(const double*, double t, double, const PointSrc& ps, double f[5],
+                                         double x0[3]) {
+#pragma clang fp contract(off)
+  const double pa = M_PI * ps.f0 * (t - ps.t0);
+  const double a = pa * pa;
+  const double r = (1.0 - 2.0 * a) * exp(-a);
+  for (int v = 0; v < 5; ++v) f[v] = ps.amp[v] * r;
+  for (int d = 0; d < 3; ++d) x0[d] = ps.x0[d];
+}
+
+// PSi[n][z][y][x][v] of one node (pointSource.cpph:58-125): level n = 0 at t,
+// n >= 1 at the GL time node n-1 of [t, t+dt]; the force at the node,
+// projected onto the node's basis function at the source position, zero
+// when the source lies outside the cell.
+template <int N>
+__device__ inline void point_source_node(int n, int z, int y, int x, double t, double dt, const double c[3],
+                                         const double h[3], const PointSrc& ps, double out[5]) {
+#pragma clang fp contract(off)
+  const DGTables& T = c_tab[N];
+  const double tn = n == 0 ? t : dt * T.xi[n - 1] + t;
+  const double X[3] = {c[0] + h[0] * (T.xi[x] - 0.5), c[1] + h[1] * (T.xi[y] - 0.5), c[2] + h[2] * (T.xi[z] - 0.5)};
+  double f[5], x0[3], r[3];
+  user_point_source(X, tn, dt, ps, f, x0);
+  for (int d = 0; d < 3; ++d) r[d] = (x0[d] - (c[d] - h[d] / 2.0)) / h[d];
+  if (r[0] >= 0.0 && r[1] >= 0.0 && r[2] >= 0.0 && r[0] <= 1.0 && r[1] <= 1.0 && r[2] <= 1.0) {
+    const double b = lagrange<N>(z, r[2]) * lagrange<N>(y, r[1]) * lagrange<N>(x, r[0]);
+    const double den = T.w[z] * h[2] * T.w[y] * h[1] * T.w[x] * h[0];
+    for (int v = 0; v < 5; ++v) out[v] = b * f[v] / den;
+  } else {
+    for (int v = 0; v < 5; ++v) out[v] = 0.0;
+  }
+}
+
+// pointSource of one cell: thread per node, PSi [B+1][z][y][x][v].
+template <int N>
+__global__ __launch_bounds__(Ord<N>::NT) void k_point_source(double t, double dt, double c0, double c1, double c2,
+                                                             double h0, double h1, double h2, double* __restrict__ psi,
+                                                             Ens ens) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
+  const int s = threadIdx.x;
+  if (s >= B3) return;
+  const int z = s / B2, y = (s / B) % B, x = s % B;
+  const double c[3] = {c0, c1, c2}, h[3] = {h0, h1, h2};
+  for (int n = 0; n <= B; ++n) {
+    double o[5];
+    point_source_node<N>(n, z, y, x, t, dt, c, h, ens.ps, o);
+    for (int v = 0; v < V; ++v) psi[(n * B3 + s) * V + v] = o[v];
+  }
+}
+
 // Cauchy-Kowalevski predictor + linear volume integral
 // (spaceTimePredictorLinear.cpph:33-443, volumeIntegralLinear.cpph:21-55).
 // Thread s owns node s and walks the time derivatives lQi[0..B]: the level
@@ -364,6 +440,39 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_linear(
   double q[V], qh[V], fh[3][V];
 #pragma unroll
   for (int v = 0; v < V; ++v) q[v] = qh[v] = act ? luh[cell * V * B3 + v * B3 + s] : 1.0;
+  // point source (:78-102): the force at t enters the first time derivative,
+  // its time integral sum_n w_n PSi[n+1] the zeroth
+  double ps0[V], psint[V];
+  if (ens.ps.mode && act) {
+    double c[3], h[3] = {dx0, dx1, dx2};
+    if (ens.ps.mode == 2) {
+      const long long ix = (long long)(cell % ens.ps.nx), iy = (long long)((cell / ens.ps.nx) % ens.ps.ny),
+                      iz = (long long)(cell / ((size_t)ens.ps.nx * ens.ps.ny));
+      c[0] = __dadd_rn(ens.ps.off[0], __dmul_rn((double)ix + 0.5, dx0));
+      c[1] = __dadd_rn(ens.ps.off[1], __dmul_rn((double)iy + 0.5, dx1));
+      c[2] = __dadd_rn(ens.ps.off[2], __dmul_rn((double)iz + 0.5, dx2));
+    }
+    for (int n = 0; n <= B; ++n) {
+      double o[V];
+      if (ens.ps.mode == 1) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) o[v] = ens.ps.psi[((cell * (B + 1) + n) * B3 + s) * V + v];
+      } else {
+        point_source_node<N>(n, z, y, x, dtp[2], dt, c, h, ens.ps, o);
+      }
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        if (n == 0) {
+          ps0[v] = o[v];
+          psint[v] = 0.0;
+        } else {
+          psint[v] = __dadd_rn(psint[v], __dmul_rn(T.w[n - 1], o[v]));
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) q[v] = qh[v] = __dadd_rn(q[v], __dmul_rn(dt, psint[v]));
+  }
   double fac = 0.5 * dt;  // dtavFac of the level (:236-248)
   for (int i = 0; i < B; ++i) {
     __syncthreads();
@@ -401,14 +510,22 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_linear(
         for (int d = 0; d < 3; ++d) fh[d][v] = i == 0 ? Tn[d][v] : fh[d][v] + fac * Tn[d][v];
       }
       if (i > 0) fac *= dt / (i + 2);
-      // next level: lQi[i+1] = 0 - T_x - T_y - T_z (:196-207)
+      // next level: lQi[i+1] = 0 - T_x - T_y - T_z (:196-207), the first
+      // level starting from the force at t when there is a point source
 #pragma unroll
-      for (int v = 0; v < V; ++v) q[v] = ((0.0 - Tn[0][v]) - Tn[1][v]) - Tn[2][v];
+      for (int v = 0; v < V; ++v) q[v] = (((i == 0 && ens.ps.mode ? ps0[v] : 0.0) - Tn[0][v]) - Tn[1][v]) - Tn[2][v];
     }
   }
   if (DEBUG && act) {
 #pragma unroll
     for (int v = 0; v < V; ++v) dbg.lQi[(B * B3 + s) * V + v] = q[v];
+  }
+  // a third of the force's time integral in every direction's block (:287-302)
+  if (ens.ps.mode && act) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int v = 0; v < V; ++v) fh[d][v] += psint[v] / 3.0;
   }
   __syncthreads();
   if (act) {

@@ -191,6 +191,7 @@ struct exa_solver {
   std::mutex mu;
   // scratch for the per-cell entry points
   DevBuf luh, lduh, qf, ff, iters, dt, dbg, misc, stage;
+  DevBuf psi;  // point forces of one cell (linear predictor with a point source)
   // deferred per-cell work (exa_aderdg_batch_*): host staging until the flush
   struct PredItem { double *lQhbnd, *lFhbnd, *lduh; };
   struct RiemItem { double *FL, *FR; };
@@ -238,6 +239,12 @@ static exa::Ens launch_extras(const exa_aderdg_config& c) {
   exa::Ens e{};
   e.grav = c.pde == EXA_PDE_EULER_GRAVITY ? c.gravity : 0.0;
   for (int d = 0; d < 3; ++d) e.adv[d] = c.velocity[d];
+  if (c.point_source) {  // the source's parameters; callers choose how the predictor gets it
+    for (int d = 0; d < 3; ++d) e.ps.x0[d] = c.source_x0[d];
+    for (int v = 0; v < 5; ++v) e.ps.amp[v] = c.source_amp[v];
+    e.ps.f0 = c.source_f0;
+    e.ps.t0 = c.source_t0;
+  }
   return e;
 }
 
@@ -274,6 +281,8 @@ extern "C" int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out)
   EXA_ARG(!cfg->linear || exa_pde_linear_supported(cfg->pde),
           "the linear kernels need a PDE with a coefficient matrix: EXA_PDE_ADVECTION_NCP or "
           "EXA_PDE_EULER_NCP_TEST");
+  EXA_ARG(!cfg->point_source || cfg->linear,
+          "point sources belong to the linear kernels (the usePointSource arm of spaceTimePredictorLinear)");
   EXA_ARG(cfg->pde != EXA_PDE_EULER_GRAVITY || cfg->precision == EXA_PREC_FP64,
           "the gravity source runs in the FP64 predictor only");
   EXA_ARG(cfg->npar >= 0 && cfg->npar <= 3, "npar must be 0..3");
@@ -364,7 +373,7 @@ static void param_predictor(int N, int nd, const double* luh, double* lQhbnd, st
 // spaceTimePredictorLinear of one cell (+ its volume integral, not returned).
 static int linear_predictor_cell(exa_solver* s, double* lQhbnd, double* lFhbnd, double* lQi, double* lFi,
                                  double* lQhi, double* lFhi, const double* luh, const double dx[3], double dt,
-                                 int* picard_iterations) {
+                                 const double* point_forces, int* picard_iterations) {
   const int N = s->cfg.order, B = N + 1, B2 = B * B, B3 = B2 * B, B4 = B2 * B2, V = exa::V;
   std::vector<double> h(std::max<size_t>((size_t)V * B3, (size_t)6 * V * B2));
   ref_to_soa_cell(luh, h.data(), B3, V);
@@ -381,10 +390,18 @@ static int linear_predictor_cell(exa_solver* s, double* lQhbnd, double* lFhbnd, 
                           s->dbg.as<double>() + nQi + nFi + nQhi};
   EXA_CHECK(hipMemcpyAsync(s->luh.p, h.data(), sizeof(double) * V * B3, hipMemcpyHostToDevice, s->stream));
   EXA_CHECK(hipMemcpyAsync(s->dt.p, &dt, sizeof(double), hipMemcpyHostToDevice, s->stream));
+  exa::Ens ens = launch_extras(s->cfg);
+  if (point_forces) {
+    const size_t np = (size_t)(B + 1) * B3 * V;
+    EXA_TRY(s->psi.ensure(sizeof(double) * np));
+    EXA_CHECK(hipMemcpyAsync(s->psi.p, point_forces, sizeof(double) * np, hipMemcpyHostToDevice, s->stream));
+    ens.ps.mode = 1;
+    ens.ps.psi = s->psi.as<double>();
+  }
   hipLaunchKernelGGL(exa_pde_linear_predictor(s->cfg.pde, N, true), dim3(1), dim3(round_up64(B3)),
                      (size_t)exa_pde_linear_lds_bytes(N), s->stream, s->luh.as<double>(), s->lduh.as<double>(),
                      s->qf.as<double>(), s->ff.as<double>(), s->iters.as<int>(), s->dt.as<double>(), dx[0], dx[1],
-                     dx[2], s->cfg.gamma, nullptr, dbg, launch_extras(s->cfg));
+                     dx[2], s->cfg.gamma, nullptr, dbg, ens);
   EXA_CHECK(hipGetLastError());
   std::vector<double> hq(6 * V * B2), hf(6 * V * B2), hdbg(nQi + nFi + nQhi + nFhi);
   EXA_CHECK(hipMemcpyAsync(hq.data(), s->qf.p, sizeof(double) * 6 * V * B2, hipMemcpyDeviceToHost, s->stream));
@@ -401,14 +418,52 @@ static int linear_predictor_cell(exa_solver* s, double* lQhbnd, double* lFhbnd, 
   return EXA_OK;
 }
 
+extern "C" int exa_aderdg_point_source(exa_solver* s, double t, double dt, const double centre[3],
+                                       const double dx[3], double* point_forces) {
+  EXA_ARG(s && centre && dx && point_forces, "null pointer");
+  EXA_ARG(s->cfg.point_source, "the solver has no point source (config point_source = 0)");
+  std::lock_guard<std::mutex> lk(s->mu);
+  EXA_CHECK(hipSetDevice(s->device));
+  const int N = s->cfg.order, B = N + 1, B3 = B * B * B;
+  const size_t np = (size_t)(B + 1) * B3 * exa::V;
+  EXA_TRY(s->psi.ensure(sizeof(double) * np));
+  hipLaunchKernelGGL(exa_pde_point_source(N), dim3(1), dim3(round_up64(B3)), 0, s->stream, t, dt, centre[0],
+                     centre[1], centre[2], dx[0], dx[1], dx[2], s->psi.as<double>(), launch_extras(s->cfg));
+  EXA_CHECK(hipGetLastError());
+  EXA_CHECK(hipMemcpyAsync(point_forces, s->psi.p, sizeof(double) * np, hipMemcpyDeviceToHost, s->stream));
+  return sync_stream(s);
+}
+
+static int space_time_predictor(exa_solver* s, double* lQhbnd, double* lFhbnd, double* lQi, double* lFi,
+                                double* lQhi, double* lFhi, const double* luh, const double dx[3], double dt,
+                                const double* point_forces, int* picard_iterations);
+
 extern "C" int exa_aderdg_space_time_predictor(exa_solver* s, double* lQhbnd, double* lFhbnd, double* lQi,
                                                double* lFi, double* lQhi, double* lFhi, const double* luh,
                                                const double dx[3], double dt, int* picard_iterations) {
+  return space_time_predictor(s, lQhbnd, lFhbnd, lQi, lFi, lQhi, lFhi, luh, dx, dt, nullptr, picard_iterations);
+}
+
+extern "C" int exa_aderdg_space_time_predictor_ps(exa_solver* s, double* lQhbnd, double* lFhbnd, double* lQi,
+                                                  double* lFi, double* lQhi, double* lFhi, const double* luh,
+                                                  const double dx[3], double dt, const double* point_forces,
+                                                  int* picard_iterations) {
+  EXA_ARG(s, "null pointer");
+  EXA_ARG(!point_forces || s->cfg.linear,
+          "point forces need a linear solver (the nonlinear predictor has no point-source arm)");
+  return space_time_predictor(s, lQhbnd, lFhbnd, lQi, lFi, lQhi, lFhi, luh, dx, dt, point_forces,
+                              picard_iterations);
+}
+
+static int space_time_predictor(exa_solver* s, double* lQhbnd, double* lFhbnd, double* lQi, double* lFi,
+                                double* lQhi, double* lFhi, const double* luh, const double dx[3], double dt,
+                                const double* point_forces, int* picard_iterations) {
   EXA_ARG(s && lQhbnd && lFhbnd && luh && dx, "null pointer");
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->cfg.linear) {
     EXA_CHECK(hipSetDevice(s->device));
-    return linear_predictor_cell(s, lQhbnd, lFhbnd, lQi, lFi, lQhi, lFhi, luh, dx, dt, picard_iterations);
+    return linear_predictor_cell(s, lQhbnd, lFhbnd, lQi, lFi, lQhi, lFhi, luh, dx, dt, point_forces,
+                                 picard_iterations);
   }
   EXA_CHECK(hipSetDevice(s->device));
   const int N = s->cfg.order, B = N + 1, B2 = B * B, B3 = B2 * B, B4 = B2 * B2;
@@ -1156,6 +1211,19 @@ static int predictor_waves_per_simd(const exa_aderdg_config& c, int N, int dev) 
   return N <= 5 ? 2 : 1;
 }
 
+// Extras of a grid's launches: a point source is evaluated per node from the
+// grid's geometry and the device-held time of the step.
+static exa::Ens grid_extras(const exa_grid* g) {
+  exa::Ens e = launch_extras(g->s->cfg);
+  if (g->s->cfg.point_source) {
+    e.ps.mode = 2;
+    for (int d = 0; d < 3; ++d) e.ps.off[d] = g->off[d];
+    e.ps.nx = g->nx;
+    e.ps.ny = g->ny;
+  }
+  return e;
+}
+
 static int enqueue_step(exa_grid* g, bool timed) {
   const int N = g->s->cfg.order;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1184,7 +1252,7 @@ static int enqueue_step(exa_grid* g, bool timed) {
                        (size_t)(linear ? exa_pde_linear_lds_bytes(N) : exa_pde_predictor_lds_bytes(N)),
                        g->stream, g->luh.as<double>(), g->lduh.as<double>(), g->qf.as<double>(), g->ff.as<double>(),
                        g->iters.as<int>(), g->dt_cur(), g->dx[0], g->dx[1], g->dx[2], g->s->cfg.gamma,
-                       g->iter_sum(), nodbg, launch_extras(g->s->cfg));
+                       g->iter_sum(), nodbg, grid_extras(g));
   } else EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
     if (g->staged) {

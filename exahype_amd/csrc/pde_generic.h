@@ -20,6 +20,21 @@ struct PdeParams {
   double adv[3];  // advection velocity (linear advection)
 };
 
+// A point source of the linear scheme (the usePointSource arm of
+// spaceTimePredictorLinear.cpph:78-102, 287-302) as the linear predictor
+// receives it.  mode 0: none; 1: the forces PSi [cell][B+1][z][y][x][v] that
+// pointSource computed, read from `psi`; 2: evaluated per node on a regular
+// grid, cell (ix, iy, iz) centred at off + (i + 1/2) h with x fastest, at the
+// time stored after the step size (dtp[2]).  The user pointSource is a Ricker
+// wavelet force amp[v] (1 - 2a) exp(-a), a = (pi f0 (t - t0))^2, at x0.
+struct PointSrc {
+  int mode;
+  const double* psi;
+  double off[3];
+  int nx, ny;
+  double x0[3], amp[5], f0, t0;
+};
+
 // 3D Euler (flux only): the existing Euler functions behind this interface.
 struct PdeEuler {
   static constexpr bool FLUX = true, SRC = false, NCP = false;

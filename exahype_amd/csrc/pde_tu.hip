@@ -147,6 +147,21 @@ PdeUpdateFn exa_pde_linear_update(int pde, int order, int mode) {
   return f;
 }
 
+PointSourceFn exa_pde_point_source(int order) {
+  switch (order) {
+    case 1: return &exa::k_point_source<1>;
+    case 2: return &exa::k_point_source<2>;
+    case 3: return &exa::k_point_source<3>;
+    case 4: return &exa::k_point_source<4>;
+    case 5: return &exa::k_point_source<5>;
+    case 6: return &exa::k_point_source<6>;
+    case 7: return &exa::k_point_source<7>;
+    case 8: return &exa::k_point_source<8>;
+    case 9: return &exa::k_point_source<9>;
+    default: return nullptr;
+  }
+}
+
 LinearVolumeFn exa_pde_linear_volume(int order) {
   switch (order) {
     case 1: return &exa::k_volume_integral_linear<1>;

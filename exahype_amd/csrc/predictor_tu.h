@@ -52,6 +52,9 @@ using PdeRiemannFn = void (*)(const double*, double*, const int*, int, double, l
 using PdeRiemannListFn = void (*)(const double*, double*, const int*, long long, double, exa::Ens);
 using PdeUpdateFn = void (*)(double*, double*, const double*, const double*, unsigned long long*, double, double,
                              double, double, double, exa::Ens);
+// pointSource of one cell (k_point_source) at orders 1..9
+using PointSourceFn = void (*)(double, double, double, double, double, double, double, double, double*, exa::Ens);
+PointSourceFn exa_pde_point_source(int order);
 bool exa_pde_generic(int pde);
 PdePredFn exa_pde_predictor(int pde, int order, bool debug);
 int exa_pde_predictor_lds_bytes(int order);

@@ -126,6 +126,17 @@ typedef struct {
                        receives the B+1 time derivatives [t][z][y][x][nData] and lFi
                        the products [t][z][y][x][3][nVar]; lFhi is [4][z][y][x][nVar]
                        (block 3 zero) and volumeIntegral reads its first 3 blocks. */
+  int point_source; /* 1 (linear solvers only): usePointSource -- the point-source arm of
+                       spaceTimePredictorLinear.cpph:78-102, 287-302, with the user
+                       pointSource (E/solvers/UserSolverInterface.h:91-95) compiled as a
+                       Ricker wavelet force source_amp[v] (1 - 2a) exp(-a),
+                       a = (pi source_f0 (t - source_t0))^2, located at source_x0.  Grids
+                       evaluate it per node at the time of each step;
+                       exa_aderdg_point_source gives the forces of one cell. */
+  double source_x0[3];
+  double source_amp[5];
+  double source_f0;
+  double source_t0;
 } exa_aderdg_config;
 
 #define EXA_CONTRACT_DEFAULT 0  /* tuned per order (the staged form for FP64 at N = 7)    */
@@ -155,6 +166,25 @@ int exa_aderdg_space_time_predictor(exa_solver* s, double* lQhbnd, double* lFhbn
                                     double* lQi, double* lFi, double* lQhi, double* lFhi,
                                     const double* luh, const double dx[3], double dt,
                                     int* picard_iterations);
+
+/* ADERDGSolver::spaceTimePredictor with its pointForceSources argument
+ * (E/solvers/ADERDGSolver.h:1092-1103): point_forces, the forces of
+ * exa_aderdg_point_source [B+1][z][y][x][nVar], enter the linear predictor's
+ * point-source arm (spaceTimePredictorLinear.cpph:78-102, 287-302); NULL
+ * behaves as exa_aderdg_space_time_predictor.  Non-NULL point_forces need a
+ * linear solver. */
+int exa_aderdg_space_time_predictor_ps(exa_solver* s, double* lQhbnd, double* lFhbnd,
+                                       double* lQi, double* lFi, double* lQhi, double* lFhi,
+                                       const double* luh, const double dx[3], double dt,
+                                       const double* point_forces, int* picard_iterations);
+
+/* ADERDGSolver::pointSource (E/solvers/ADERDGSolver.h:1195-1200; kernel
+ * kernels/aderdg/generic/c/3d/pointSource.cpph:28-145): the point forces of
+ * the cell with the given centre and size at time t, step dt, projected onto
+ * the nodal basis: point_forces [B+1][z][y][x][nVar], level 0 at t and level
+ * n >= 1 at the GL time node n-1 of [t, t+dt].  Needs config point_source. */
+int exa_aderdg_point_source(exa_solver* s, double t, double dt, const double centre[3], const double dx[3],
+                            double* point_forces);
 
 /* ADERDGSolver::volumeIntegral (E/solvers/ADERDGSolver.h:1015-1017;
  * kernels/aderdg/generic/c/3d/volumeIntegralNonlinear.cpph:26-125) */

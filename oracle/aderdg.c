@@ -690,8 +690,17 @@ void orc_grid_predict_cells(const orc_cfg* c, orc_grid* g, double dt, int first,
   for (int q = first; q < first + count; ++q) {
     ensure_scratch();
     if (c->linear) { /* isLinear branch of the generated glue */
+      double* psi = NULL;
+      if (c->point_source) { /* ADERDGSolver::performPredictionAndVolumeIntegral (E/solvers/ADERDGSolver.cpp:1863-1866) */
+        static _Thread_local double* tl_psi;
+        if (!tl_psi) tl_psi = (double*)malloc(sizeof(double) * (size_t)(MAXB + 1) * MAXB * MAXB * MAXB * 8);
+        double centre[3], h[3];
+        cell_geom(g, q, centre, h);
+        orc_point_source(c, g->t, dt, centre, dxv, tl_psi);
+        psi = tl_psi;
+      }
       orc_space_time_predictor_linear(c, g->lQhbnd + (size_t)q * 6 * B2 * nd, g->lFhbnd + (size_t)q * 6 * B2 * nv,
-                                      NULL, NULL, tl_lQhi, tl_lFhi, g->luh + (size_t)q * B3 * nd, dxv, dt);
+                                      NULL, NULL, tl_lQhi, tl_lFhi, g->luh + (size_t)q * B3 * nd, dxv, dt, psi);
       orc_volume_integral_linear(c, g->lduh + (size_t)q * B3 * nv, tl_lFhi, dxv);
       g->iters[q] = 0;
       continue;
@@ -733,6 +742,7 @@ double orc_grid_step(const orc_cfg* c, orc_grid* g, double dt, int nthreads) {
                                                                     g->lFhbnd + (size_t)q * 6 * B2 * nv, dxv);
     orc_solution_update(c, g->luh + (size_t)q * B3 * nd, g->lduh + (size_t)q * B3 * nv, dt);
   }
+  g->t += dt;
   /* TimeStepSizeComputation (E/solvers/ADERDGSolver.cpp:1948-2001) */
   return orc_grid_stable_dt(c, g, nthreads);
 }

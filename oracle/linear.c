@@ -45,13 +45,82 @@ void orc_coefficient_matrix(const orc_cfg* c, const double* Q, int d, double* Bn
     for (int v = 0; v < nv; ++v) Bn[v * nv + v] = c->adv[d];
 }
 
-/* spaceTimePredictorLinear<usePointSource=false, useSource=false,
- * useFlux=false, useNCP=true> (kernels/aderdg/generic/c/3d/
- * spaceTimePredictorLinear.cpph:33-443).  lQi [B+1][z][y][x][nData] (time
- * derivatives) and lFi [B][z][y][x][3][nVar] may be NULL; lQh [z][y][x][nData];
- * lFh [4][z][y][x][nVar] (block 3, the source, zero). */
+/* The Lagrange basis polynomial of GL node i of the order's nodes on [0,1],
+ * from its definition l_i(xi) = prod_{m != i} (xi - xi_m) / (xi_i - xi_m).
+ * The reference evaluates generated polynomial literals
+ * (kernels/DGBasisFunctions.cpp:327-...), which agree to rounding. */
+double orc_lagrange(int order, int i, double xi) {
+  const double* n = EXA_TAB_gl_nodes[order];
+  double num = 1.0, den = 1.0;
+  for (int m = 0; m <= order; ++m) {
+    if (m == i) continue;
+    num *= xi - n[m];
+    den *= n[i] - n[m];
+  }
+  return num / den;
+}
+
+/* The user pointSource(x, t, dt, forceVector, x0) of the plugin surface
+ * (E/solvers/UserSolverInterface.h:91-95) used by the point-source arm here:
+ * a Ricker wavelet force amp[v] (1 - 2a) exp(-a), a = (pi f0 (t - t0))^2,
+ * located at src_x0 (independent of the node x and of dt). */
+void orc_user_point_source(const orc_cfg* c, const double x[3], double t, double dt, double* force, double x0[3]) {
+  (void)x;
+  (void)dt;
+  const double pa = M_PI * c->src_f0 * (t - c->src_t0);
+  const double a = pa * pa;
+  const double r = (1.0 - 2.0 * a) * exp(-a);
+  for (int v = 0; v < c->nvar; ++v) force[v] = c->src_amp[v] * r;
+  for (int d = 0; d < 3; ++d) x0[d] = c->src_x0[d];
+}
+
+/* kernels::aderdg::generic::c::pointSource (kernels/aderdg/generic/c/3d/
+ * pointSource.cpph:28-145): for time level n (n = 0 at t, n >= 1 at the GL
+ * time node n-1 of [t, t+dt]) and every spatial node, the user force at the
+ * node, projected onto the node's basis function at the source position:
+ * PSi = ((l_z(r_z) l_y(r_y)) l_x(r_x)) force / (((((w_z dz) w_y) dy) w_x) dx),
+ * r = (x0 - (centre - dx/2)) / dx the source in reference coordinates; zero
+ * when the source lies outside the cell. */
+void orc_point_source(const orc_cfg* c, double t, double dt, const double centre[3], const double dx[3],
+                      double* PSi) {
+  const int N = c->order, B = N + 1, nv = c->nvar;
+  const double* xi = EXA_TAB_gl_nodes[N];
+  const double* w = EXA_TAB_gl_weights[N];
+  double x[3], x0[3], f[8];
+  for (int n = 0; n < B + 1; n++) {
+    const double tn = n == 0 ? t : dt * xi[n - 1] + t;
+    for (int i = 0; i < B; i++) {
+      x[2] = centre[2] + dx[2] * (xi[i] - 0.5);
+      for (int j = 0; j < B; j++) {
+        x[1] = centre[1] + dx[1] * (xi[j] - 0.5);
+        for (int k = 0; k < B; k++) {
+          x[0] = centre[0] + dx[0] * (xi[k] - 0.5);
+          orc_user_point_source(c, x, tn, dt, f, x0);
+          double r[3];
+          for (int d = 0; d < 3; d++) r[d] = (x0[d] - (centre[d] - dx[d] / 2.0)) / dx[d];
+          double* out = PSi + ((((size_t)n * B + i) * B + j) * B + k) * nv;
+          if (r[0] >= 0.0 && r[1] >= 0.0 && r[2] >= 0.0 && r[0] <= 1.0 && r[1] <= 1.0 && r[2] <= 1.0) {
+            const double b = orc_lagrange(N, i, r[2]) * orc_lagrange(N, j, r[1]) * orc_lagrange(N, k, r[0]);
+            const double den = w[i] * dx[2] * w[j] * dx[1] * w[k] * dx[0];
+            for (int v = 0; v < nv; v++) out[v] = b * f[v] / den;
+          } else {
+            for (int v = 0; v < nv; v++) out[v] = 0.0;
+          }
+        }
+      }
+    }
+  }
+}
+
+/* spaceTimePredictorLinear<usePointSource, useSource=false, useFlux=false,
+ * useNCP=true> (kernels/aderdg/generic/c/3d/spaceTimePredictorLinear.cpph:
+ * 33-443).  lQi [B+1][z][y][x][nData] (time derivatives) and lFi
+ * [B][z][y][x][3][nVar] may be NULL; lQh [z][y][x][nData]; lFh
+ * [4][z][y][x][nVar] (block 3, the source, zero).  PSi: the point forces of
+ * orc_point_source, or NULL (usePointSource = false). */
 void orc_space_time_predictor_linear(const orc_cfg* c, double* lQbnd, double* lFbnd, double* lQi, double* lFi,
-                                     double* lQh, double* lFh, const double* luh, const double dx[3], double dt) {
+                                     double* lQh, double* lFh, const double* luh, const double dx[3], double dt,
+                                     const double* PSi) {
   /* per-thread scratch, allocated on first use */
   static _Thread_local double *sQi, *sFi, *gradQ;
   if (!sQi) {
@@ -75,6 +144,23 @@ void orc_space_time_predictor_linear(const orc_cfg* c, double* lQbnd, double* lF
   for (int i = 0; i < B; i++)
     for (int j = 0; j < B; j++)
       for (int k = 0; k < B; k++) memcpy(&QI(0, i, j, k, 0), luh + ((i * B + j) * B + k) * nd, sizeof(double) * nd);
+#define PS(n, z, y, x, m) PSi[((((n) * B + (z)) * B + (y)) * B + (x)) * nv + (m)]
+  if (PSi) {
+    /* the time integral of the force enters the zeroth derivative (:78-91),
+     * the force at t the first (:93-102) */
+    for (int j = 0; j < B; j++)
+      for (int k = 0; k < B; k++)
+        for (int l = 0; l < B; l++)
+          for (int m = 0; m < nv; m++) {
+            double sum = 0.0;
+            for (int n = 1; n < B + 1; n++) sum = sum + W(N, n - 1) * PS(n, j, k, l, m);
+            QI(0, j, k, l, m) += dt * sum;
+          }
+    for (int j = 0; j < B; j++)
+      for (int k = 0; k < B; k++)
+        for (int l = 0; l < B; l++)
+          for (int n = 0; n < nv; n++) QI(1, j, k, l, n) += PS(0, j, k, l, n);
+  }
   memset(gradQ, 0, sizeof(double) * (size_t)B3 * B * 3 * nv);
   /* Cauchy-Kowalevski (:110-222) */
   for (int i = 0; i < B; i++) {
@@ -134,6 +220,19 @@ void orc_space_time_predictor_linear(const orc_cfg* c, double* lQbnd, double* lF
             for (int n = 0; n < nv; n++) FH(j, k, l, m, n) += dtavFac * FI(i, k, l, m, j, n);
     dtavFac *= dt / (i + 2);
   }
+  if (PSi) {
+    /* a third of the force's time integral in every direction's block (:287-302) */
+    for (int j = 0; j < 3; j++)
+      for (int k = 0; k < B; k++)
+        for (int l = 0; l < B; l++)
+          for (int m = 0; m < B; m++)
+            for (int n = 0; n < nv; n++) {
+              double sum = 0.0;
+              for (int n0 = 1; n0 < B + 1; n0++) sum = sum + W(N, n0 - 1) * PS(n0, k, l, m, n);
+              FH(j, k, l, m, n) += sum / 3;
+            }
+  }
+#undef PS
   /* boundary extrapolation (:305-440) */
   memset(lQbnd, 0, sizeof(double) * 6 * B2 * nd);
   memset(lFbnd, 0, sizeof(double) * 6 * B2 * nv);

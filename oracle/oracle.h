@@ -45,6 +45,11 @@ typedef struct {
   double grav;    /* gravity along -z for pde==3                      */
   int linear;     /* 1: the linear kernel family (pdes 4, 5): Cauchy-Kowalevski
                      predictor, fluctuation-form integrals, coefficient-matrix Riemann */
+  int point_source;      /* 1 (linear only): the usePointSource arm, with the user
+                            pointSource of orc_user_point_source */
+  double src_x0[3];      /* position of the point source                     */
+  double src_amp[5];     /* force amplitude per variable                     */
+  double src_f0, src_t0; /* Ricker wavelet: peak frequency and delay         */
 } orc_cfg;
 
 /* user PDE (E/tests/kernels/c/GenericEulerKernelTest3D.cpp:26-82) */
@@ -81,7 +86,13 @@ void orc_sample_cartesian_vertices(const orc_cfg* c, const double* luh, double* 
 void orc_ncp_linear(const orc_cfg* c, const double* Q, const double* gradQ, double* T /* [3][nvar] */);
 void orc_coefficient_matrix(const orc_cfg* c, const double* Q, int d, double* Bn /* [nvar][nvar] */);
 void orc_space_time_predictor_linear(const orc_cfg* c, double* lQbnd, double* lFbnd, double* lQi, double* lFi,
-                                     double* lQh, double* lFh, const double* luh, const double dx[3], double dt);
+                                     double* lQh, double* lFh, const double* luh, const double dx[3], double dt,
+                                     const double* PSi /* NULL: no point source */);
+/* point sources (kernels/aderdg/generic/c/3d/pointSource.cpph) */
+double orc_lagrange(int order, int i, double xi);
+void orc_user_point_source(const orc_cfg* c, const double x[3], double t, double dt, double* force, double x0[3]);
+void orc_point_source(const orc_cfg* c, double t, double dt, const double centre[3], const double dx[3],
+                      double* PSi /* [B+1][z][y][x][nVar] */);
 void orc_volume_integral_linear(const orc_cfg* c, double* lduh, const double* lFhi, const double dx[3]);
 void orc_surface_integral_linear(const orc_cfg* c, double* lduh, const double* lFbnd, const double dx[3]);
 void orc_riemann_solver_linear(const orc_cfg* c, double* FL, double* FR, const double* QL, const double* QR,
@@ -114,6 +125,7 @@ typedef struct {
   double* lQhbnd;   /* [cell][6][B^2][nData]*/
   double* lFhbnd;   /* [cell][6][B^2][nvar] */
   int* iters;       /* Picard iterations of the last predictor per cell */
+  double t;         /* time at the start of the next step (advanced by orc_grid_step) */
 } orc_grid;
 int orc_grid_alloc(const orc_cfg* c, orc_grid* g, int nx, int ny, int nz);
 void orc_grid_free(orc_grid* g);

@@ -20,7 +20,9 @@ def build():
 class Cfg(C.Structure):
     _fields_ = [("order", C.c_int), ("nvar", C.c_int), ("npar", C.c_int),
                 ("gamma", C.c_double), ("cfl", C.c_double), ("pde", C.c_int),
-                ("adv", C.c_double * 3), ("grav", C.c_double), ("linear", C.c_int)]
+                ("adv", C.c_double * 3), ("grav", C.c_double), ("linear", C.c_int),
+                ("point_source", C.c_int), ("src_x0", C.c_double * 3), ("src_amp", C.c_double * 5),
+                ("src_f0", C.c_double), ("src_t0", C.c_double)]
 
 
 class IC(C.Structure):
@@ -37,7 +39,7 @@ class Grid(C.Structure):
                 ("offset", C.c_double * 3), ("width", C.c_double * 3),
                 ("luh", C.POINTER(C.c_double)), ("lduh", C.POINTER(C.c_double)),
                 ("lQhbnd", C.POINTER(C.c_double)), ("lFhbnd", C.POINTER(C.c_double)),
-                ("iters", C.POINTER(C.c_int))]
+                ("iters", C.POINTER(C.c_int)), ("t", C.c_double)]
 
 
 class FVGridS(C.Structure):
@@ -65,7 +67,11 @@ def lib():
         L.orc_ncp_linear.argtypes = [P(Cfg), _D, _D, _D]
         L.orc_sample_cartesian_vertices.argtypes = [P(Cfg), _D, _D]
         L.orc_coefficient_matrix.argtypes = [P(Cfg), _D, C.c_int, _D]
-        L.orc_space_time_predictor_linear.argtypes = [P(Cfg), _D, _D, _D, _D, _D, _D, _D, _D, C.c_double]
+        L.orc_space_time_predictor_linear.argtypes = [P(Cfg), _D, _D, _D, _D, _D, _D, _D, _D, C.c_double, _D]
+        L.orc_lagrange.argtypes = [C.c_int, C.c_int, C.c_double]
+        L.orc_lagrange.restype = C.c_double
+        L.orc_user_point_source.argtypes = [P(Cfg), _D, C.c_double, C.c_double, _D, _D]
+        L.orc_point_source.argtypes = [P(Cfg), C.c_double, C.c_double, _D, _D, _D]
         L.orc_volume_integral_linear.argtypes = [P(Cfg), _D, _D, _D]
         L.orc_surface_integral_linear.argtypes = [P(Cfg), _D, _D, _D]
         L.orc_riemann_solver_linear.argtypes = [P(Cfg), _D, _D, _D, _D, C.c_double, C.c_int]
@@ -128,17 +134,47 @@ def ptr(a):
     return a.ctypes.data_as(_D)
 
 
-def cfg(order, nvar=5, npar=0, gamma=1.4, cfl=0.9, pde=0, adv=(0.0, 0.0, 0.0), grav=0.0, linear=0):
+def cfg(order, nvar=5, npar=0, gamma=1.4, cfl=0.9, pde=0, adv=(0.0, 0.0, 0.0), grav=0.0, linear=0,
+        point_source=None):
     """pde 0 Euler, 1 linear advection (velocity adv), 2 EulerFV 2D, 3 Euler
     with gravity grav along -z (algebraic source), 4 linear advection in
     non-conservative form (velocity adv, no flux), 5 Euler with the test
-    non-conservative product of the reference's Euler kernel test."""
+    non-conservative product of the reference's Euler kernel test.
+    point_source: None, or dict(x0=(3,), amp=(5,), f0=, t0=) -- the Ricker
+    force of the linear predictor's usePointSource arm."""
     c = Cfg(order, nvar, npar, gamma, cfl, pde)
     for i in range(3):
         c.adv[i] = adv[i]
     c.grav = grav
     c.linear = int(linear)
+    if point_source is not None:
+        c.point_source = 1
+        for i in range(3):
+            c.src_x0[i] = point_source["x0"][i]
+        for i in range(5):
+            c.src_amp[i] = point_source["amp"][i]
+        c.src_f0, c.src_t0 = point_source["f0"], point_source["t0"]
     return c
+
+
+def lagrange(order, i, xi):
+    """Lagrange basis polynomial of GL node i of the order's nodes at xi."""
+    return lib().orc_lagrange(order, i, xi)
+
+
+def user_point_source(c, x, t, dt):
+    """(force[nvar], x0[3]) of the compiled user pointSource."""
+    f, x0 = np.zeros(c.nvar), np.zeros(3)
+    lib().orc_user_point_source(C.byref(c), ptr(dvec(x)), t, dt, ptr(f), ptr(x0))
+    return f, x0
+
+
+def point_source(c, t, dt, centre, dx):
+    """pointSource.cpph: PSi [B+1][z][y][x][nVar]."""
+    B = c.order + 1
+    psi = np.zeros((B + 1) * B ** 3 * c.nvar)
+    lib().orc_point_source(C.byref(c), t, dt, ptr(dvec(centre)), ptr(dvec(dx)), ptr(psi))
+    return psi
 
 
 def source(c, Q):
@@ -183,10 +219,11 @@ def space_time_predictor(c, luh, dx, dt):
     return out
 
 
-def space_time_predictor_linear(c, luh, dx, dt):
+def space_time_predictor_linear(c, luh, dx, dt, psi=None):
     """Cauchy-Kowalevski predictor: lQi [B+1][z][y][x][nData] (time
     derivatives), lFi [B][z][y][x][3][nVar], lQhi, lFhi [4][z][y][x][nVar]
-    (block 3 zero), lQhbnd, lFhbnd."""
+    (block 3 zero), lQhbnd, lFhbnd.  psi: point forces (point_source) or
+    None."""
     B = c.order + 1
     nd, nv = c.nvar + c.npar, c.nvar
     out = dict(lQhbnd=np.zeros(6 * B * B * nd), lFhbnd=np.zeros(6 * B * B * nv),
@@ -194,7 +231,7 @@ def space_time_predictor_linear(c, luh, dx, dt):
                lQhi=np.zeros(B ** 3 * nd), lFhi=np.zeros(4 * B ** 3 * nv))
     lib().orc_space_time_predictor_linear(C.byref(c), ptr(out["lQhbnd"]), ptr(out["lFhbnd"]), ptr(out["lQi"]),
                                           ptr(out["lFi"]), ptr(out["lQhi"]), ptr(out["lFhi"]), ptr(dvec(luh)),
-                                          ptr(dvec(dx)), dt)
+                                          ptr(dvec(dx)), dt, None if psi is None else ptr(dvec(psi)))
     return out
 
 
