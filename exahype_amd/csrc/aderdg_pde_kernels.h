@@ -419,7 +419,7 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_point_source(double t, double dt
 // passes through LDS for its dudx gradients, the product tensor
 // T = B_d dQ/dx_d gives the next level -sum_d T_d, and the Taylor time
 // averages dt^i/(i+1)! accumulate in registers.
-template <int N, class PDE, bool DEBUG>
+template <int N, class PDE, bool DEBUG, bool PS = false>  // PS: the point-source arm is compiled in
 __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_linear(
     const double* __restrict__ luh, double* __restrict__ lduh, double* __restrict__ qf, double* __restrict__ ff,
     int* __restrict__ iters, const double* __restrict__ dtp, double dx0, double dx1, double dx2, double gamma,
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_linear(
   // point source (:78-102): the force at t enters the first time derivative,
   // its time integral sum_n w_n PSi[n+1] the zeroth
   double ps0[V], psint[V];
-  if (ens.ps.mode && act) {
+  if (PS && ens.ps.mode && act) {
     double c[3], h[3] = {dx0, dx1, dx2};
     if (ens.ps.mode == 2) {
       const long long ix = (long long)(cell % ens.ps.nx), iy = (long long)((cell / ens.ps.nx) % ens.ps.ny),
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_linear(
       // next level: lQi[i+1] = 0 - T_x - T_y - T_z (:196-207), the first
       // level starting from the force at t when there is a point source
 #pragma unroll
-      for (int v = 0; v < V; ++v) q[v] = (((i == 0 && ens.ps.mode ? ps0[v] : 0.0) - Tn[0][v]) - Tn[1][v]) - Tn[2][v];
+      for (int v = 0; v < V; ++v) q[v] = (((PS && i == 0 && ens.ps.mode ? ps0[v] : 0.0) - Tn[0][v]) - Tn[1][v]) - Tn[2][v];
     }
   }
   if (DEBUG && act) {
@@ -521,7 +521,7 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_linear(
     for (int v = 0; v < V; ++v) dbg.lQi[(B * B3 + s) * V + v] = q[v];
   }
   // a third of the force's time integral in every direction's block (:287-302)
-  if (ens.ps.mode && act) {
+  if (PS && ens.ps.mode && act) {
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll

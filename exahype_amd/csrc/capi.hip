@@ -398,7 +398,8 @@ static int linear_predictor_cell(exa_solver* s, double* lQhbnd, double* lFhbnd, 
     ens.ps.mode = 1;
     ens.ps.psi = s->psi.as<double>();
   }
-  hipLaunchKernelGGL(exa_pde_linear_predictor(s->cfg.pde, N, true), dim3(1), dim3(round_up64(B3)),
+  hipLaunchKernelGGL(exa_pde_linear_predictor(s->cfg.pde, N, true, point_forces != nullptr), dim3(1),
+                     dim3(round_up64(B3)),
                      (size_t)exa_pde_linear_lds_bytes(N), s->stream, s->luh.as<double>(), s->lduh.as<double>(),
                      s->qf.as<double>(), s->ff.as<double>(), s->iters.as<int>(), s->dt.as<double>(), dx[0], dx[1],
                      dx[2], s->cfg.gamma, nullptr, dbg, ens);
@@ -1246,7 +1247,7 @@ static int enqueue_step(exa_grid* g, bool timed) {
   const bool generic = exa_pde_generic(g->s->cfg.pde);
   const bool linear = g->s->cfg.linear != 0;
   if (generic) {
-    hipLaunchKernelGGL(linear ? exa_pde_linear_predictor(g->s->cfg.pde, N, false)
+    hipLaunchKernelGGL(linear ? exa_pde_linear_predictor(g->s->cfg.pde, N, false, g->s->cfg.point_source != 0)
                               : exa_pde_predictor(g->s->cfg.pde, N, false),
                        dim3((unsigned)g->nc), dim3(round_up64((N + 1) * (N + 1) * (N + 1))),
                        (size_t)(linear ? exa_pde_linear_lds_bytes(N) : exa_pde_predictor_lds_bytes(N)),

@@ -17,7 +17,8 @@ PdeUpdateFn upd(int mode) {
   return nullptr;
 }
 template <class PDE, int N>
-PdePredFn lin_pred(bool debug) {
+PdePredFn lin_pred(bool debug, bool ps) {
+  if (ps) return debug ? &exa::k_predictor_linear<N, PDE, true, true> : &exa::k_predictor_linear<N, PDE, false, true>;
   return debug ? &exa::k_predictor_linear<N, PDE, true> : &exa::k_predictor_linear<N, PDE, false>;
 }
 
@@ -118,9 +119,9 @@ PdeUpdateFn exa_pde_update(int pde, int order, int mode) {
 
 bool exa_pde_linear_supported(int pde) { return pde == EXA_PDE_ADVECTION_NCP || pde == EXA_PDE_EULER_NCP_TEST; }
 
-PdePredFn exa_pde_linear_predictor(int pde, int order, bool debug) {
+PdePredFn exa_pde_linear_predictor(int pde, int order, bool debug, bool point_source) {
   PdePredFn f = nullptr;
-#define F_(P, N) f = lin_pred<P, N>(debug)
+#define F_(P, N) f = lin_pred<P, N>(debug, point_source)
   EXA_LINEAR_DISPATCH(pde, order, F_);
 #undef F_
   return f;
@@ -191,10 +192,11 @@ hipError_t exa_pde_init(const DGTables* tab) {
   const int lin[2] = {EXA_PDE_ADVECTION_NCP, EXA_PDE_EULER_NCP_TEST};
   for (int pde : lin)
     for (int order = 1; order <= exa::LINEAR_MAX_ORDER; ++order)
-      for (int dbg = 0; dbg < 2; ++dbg) {
-        e = hipFuncSetAttribute(reinterpret_cast<const void*>(exa_pde_linear_predictor(pde, order, dbg)),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, exa_pde_linear_lds_bytes(order));
-        if (e != hipSuccess) return e;
-      }
+      for (int dbg = 0; dbg < 2; ++dbg)
+        for (int ps = 0; ps < 2; ++ps) {
+          e = hipFuncSetAttribute(reinterpret_cast<const void*>(exa_pde_linear_predictor(pde, order, dbg, ps)),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, exa_pde_linear_lds_bytes(order));
+          if (e != hipSuccess) return e;
+        }
   return hipSuccess;
 }

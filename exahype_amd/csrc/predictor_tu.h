@@ -69,7 +69,7 @@ PdeUpdateFn exa_pde_update(int pde, int order, int mode);
 // at orders 1..9; nullptr otherwise.
 using LinearVolumeFn = void (*)(const double*, double*);
 bool exa_pde_linear_supported(int pde);
-PdePredFn exa_pde_linear_predictor(int pde, int order, bool debug);
+PdePredFn exa_pde_linear_predictor(int pde, int order, bool debug, bool point_source = false);
 int exa_pde_linear_lds_bytes(int order);
 PdeRiemannFn exa_pde_linear_riemann(int pde, int order);
 // modes UPD_DT and UPD_SURFACE | UPD_UPDATE | UPD_DT | UPD_LINEAR
