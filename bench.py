@@ -140,6 +140,9 @@ def main():
                          "27^3 patches of 11^3 subcells (the limiter's FV scheme of the N=5 Sod sphere); amr: batched "
                          "volume prolongation/restriction of 27^3 N=5 cells one level down and back")
     ap.add_argument("--instances", type=int, default=512, help="ensemble members (--workload ensemble)")
+    ap.add_argument("--point-source", action="store_true",
+                    help="--workload linear: with a Ricker point source at the domain centre (evaluated per node "
+                         "in every cell's predictor)")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also time a plain device copy of this many GiB (HBM reference, counter calibration)")
     args = ap.parse_args()
@@ -276,9 +279,14 @@ def advection_ic_kwargs():
     return dict(nmodes=2, k=[(1, 0, 1), (0, 1, 1)], a=[0.2, 0.1], phi=[0.3, 1.1], uinf=NCP_VELOCITY)
 
 
-def run_advection(pde, N, n, steps, warmup, linear=False):
+# A Ricker-wavelet point force at the domain centre (inside cell 40 of 81 per
+# axis), peaking during the first steps (--workload linear --point-source)
+POINT_SOURCE = dict(x0=(0.5, 0.5, 0.5), amp=(1.0, 0.5, -0.5, 0.25, 2.0), f0=10.0, t0=0.005)
+
+
+def run_advection(pde, N, n, steps, warmup, linear=False, point_source=None):
     import exahype_amd as E
-    s = E.ADERDGSolver(N, pde=pde, velocity=NCP_VELOCITY, linear=linear)
+    s = E.ADERDGSolver(N, pde=pde, velocity=NCP_VELOCITY, linear=linear, point_source=point_source)
     g = E.Grid(s, n, n, n)
     g.set_initial_conditions(E.make_ic(3, **advection_ic_kwargs()))
     g.compute_dt()
@@ -374,7 +382,8 @@ def run_linear(args):
     ncells = n ** 3
     B = N + 1
     dof = ncells * NVAR * B ** 3
-    r = run_advection(E.PDE_ADVECTION_NCP, N, n, args.steps, args.warmup, linear=True)
+    ps = POINT_SOURCE if args.point_source else None
+    r = run_advection(E.PDE_ADVECTION_NCP, N, n, args.steps, args.warmup, linear=True, point_source=ps)
     pred_ms = r["times"][0] / max(r["counts"][0], 1)
     # per time level: dudx gradients 6 V B^4, products and the next level
     # 6 V B^3, two Taylor accumulations 8 V B^3; then the volume integral
@@ -389,10 +398,12 @@ def run_linear(args):
     else:
         roof = {"bound": "hbm", "achieved": nbytes / t / 1e9, "peak": PEAK_HBM_GBPS, "unit": "GB/s"}
     roof.update(frac=roof["achieved"] / roof["peak"], traffic=None,
-                kernel="k_predictor_linear<%d, PdeAdvectionNcp>" % N, avg_launch_ms=pred_ms, flop_per_launch=flops,
+                kernel="k_predictor_linear<%d, PdeAdvectionNcp%s>" % (N, ", point source" if ps else ""),
+                avg_launch_ms=pred_ms, flop_per_launch=flops,
                 bytes_per_launch=nbytes, flop_per_byte=ai)
     out = {
-        "metric": "DoF-updates/s, 3D linear advection, linear ADER-DG kernels N=%d" % N,
+        "metric": "DoF-updates/s, 3D linear advection%s, linear ADER-DG kernels N=%d" % (
+            " with a point source" if ps else "", N),
         "value": dof * args.steps / r["wall"],
         "unit": "DoF-updates/s",
         "n_gpus": 1,
@@ -408,7 +419,7 @@ def run_linear(args):
                                "predictor, coefficient-matrix Rusanov), order %d, %d^3 periodic cells, 2 sine modes, "
                                "CFL 0.9" % (N, n),
                    "velocity": list(NCP_VELOCITY), "cells": ncells, "order": N, "dof_per_step": dof,
-                   "parallelism": "single GPU"},
+                   "point_source": ps, "parallelism": "single GPU"},
         "roofline": roof,
         "phase_ms_per_step": {"predictor": pred_ms, "riemann": r["times"][1] / max(r["counts"][1], 1),
                               "update_surface_dt": r["times"][2] / max(r["counts"][2], 1)},
@@ -417,7 +428,10 @@ def run_linear(args):
         from oracle import oracle as O
         threads = min(len(os.sched_getaffinity(0)), 16)
         steps = 20
-        og = O.OracleGrid(O.cfg(N, pde=4, adv=NCP_VELOCITY, linear=1), 27, 27, 27)
+        # the point source of the sample sits at the same place of the
+        # (smaller) domain: the centre of its middle cell
+        pso = None if ps is None else dict(ps, x0=(13.5 / n,) * 3)
+        og = O.OracleGrid(O.cfg(N, pde=4, adv=NCP_VELOCITY, linear=1, point_source=pso), 27, 27, 27)
         for d in range(3):
             og.g.width[d] = 27 / n
         og.init(O.make_ic(kind=3, **advection_ic_kwargs()))
