@@ -21,6 +21,12 @@
  *   stable dt             closed form
  *   whole scheme          convergence order on the isentropic vortex
  *   FV kernels            closed forms, conservation, symmetry
+ *   NCP arm               flux form == non-conservative form (advection)
+ *   linear family         goldens testVolumeIntegralLinear, testSurfaceIntegralLinear,
+ *                         testRiemannSolverLinear; predictor exact for polynomial data
+ *   point sources         Lagrange basis by interpolation uniqueness; projection by
+ *                         node sources and quadrature of the force; predictor arm by
+ *                         closed forms at zero velocity
  */
 #ifndef EXA_ORACLE_H
 #define EXA_ORACLE_H
