@@ -416,7 +416,9 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_point_source(double t, double dt
 // Cauchy-Kowalevski predictor + linear volume integral
 // (spaceTimePredictorLinear.cpph:33-443, volumeIntegralLinear.cpph:21-55).
 // Thread s owns node s and walks the time derivatives lQi[0..B]: the level
-// passes through LDS for its dudx gradients, the product tensor
+// passes through LDS, pencil owners contract each pencil with dudx once per
+// direction (LDS bandwidth binds this kernel: one read per value and
+// direction instead of B), node owners read their gradients back, the product tensor
 // T = B_d dQ/dx_d gives the next level -sum_d T_d, and the Taylor time
 // averages dt^i/(i+1)! accumulate in registers.
 template <int N, class PDE, bool DEBUG, bool PS = false>  // PS: the point-source arm is compiled in
@@ -428,6 +430,7 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_linear(
   constexpr int B = O::B, B2 = O::B2, B3 = O::B3;
   extern __shared__ double sm[];
   double* sQ = sm;           // one derivative level [v][node]
+  double* sG = sm + V * B3;  // its gradients [d][v][node]
   double* sQh = sm;          // afterwards: time averages [v][node]
   double* sFh = sm + V * B3; // and the averaged products [d][v][node]
   const DGTables& T = c_tab[N];
@@ -481,18 +484,31 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_linear(
       for (int v = 0; v < V; ++v) sQ[v * B3 + s] = q[v];
     }
     __syncthreads();
+    // pencil owners: one (direction, variable, pencil) item reads its B
+    // values once and writes the B nodal derivatives sum_n (q_n / dx_d) dudx[l][n]
+    for (int k = s; k < 3 * V * B2; k += O::NT) {
+      const int d = k / (V * B2), v = (k / B2) % V, ab = k % B2, a = ab / B, b = ab % B;
+      const int base = d == 0 ? a * B2 + b * B : (d == 1 ? a * B2 + b : a * B + b);
+      const int step = d == 0 ? 1 : (d == 1 ? B : B2);
+      const double idx = d == 0 ? 1.0 / dx0 : (d == 1 ? 1.0 / dx1 : 1.0 / dx2);
+      double pq[B];
+#pragma unroll
+      for (int n = 0; n < B; ++n) pq[n] = sQ[v * B3 + base + n * step];
+#pragma unroll
+      for (int l = 0; l < B; ++l) {
+        double g = 0.0;
+#pragma unroll
+        for (int n = 0; n < B; ++n) g += idx * pq[n] * T.dudx[l][n];
+        sG[(d * V + v) * B3 + base + l * step] = g;
+      }
+    }
+    __syncthreads();
     if (act) {
       double G[3][V], Tn[3][V];
 #pragma unroll
-      for (int v = 0; v < V; ++v) {
-        G[0][v] = G[1][v] = G[2][v] = 0.0;
+      for (int v = 0; v < V; ++v)
 #pragma unroll
-        for (int n = 0; n < B; ++n) G[0][v] += 1.0 / dx0 * sQ[v * B3 + z * B2 + y * B + n] * T.dudx[x][n];
-#pragma unroll
-        for (int n = 0; n < B; ++n) G[1][v] += 1.0 / dx1 * sQ[v * B3 + z * B2 + n * B + x] * T.dudx[y][n];
-#pragma unroll
-        for (int n = 0; n < B; ++n) G[2][v] += 1.0 / dx2 * sQ[v * B3 + n * B2 + y * B + x] * T.dudx[z][n];
-      }
+        for (int d = 0; d < 3; ++d) G[d][v] = sG[(d * V + v) * B3 + s];
       PDE::ncp_linear(q, G, Tn, P);
       if (DEBUG) {
 #pragma unroll
