@@ -20,7 +20,7 @@ template <int N>
 struct PdeOrd {
   // iterate slice [v][node] + flux slice [d][v][node] + reduction scratch;
   // after the sweeps the same words hold sQh [v][node] and sFh [d][v][node]
-  static constexpr int LDS_DOUBLES = 4 * V * Ord<N>::B3 + 16;
+  static constexpr int LDS_DOUBLES = 7 * V * Ord<N>::B3 + 16;  // + the NCP gradients
 };
 
 template <int N, class PDE, bool DEBUG>
@@ -35,6 +35,7 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_pde(
   double* sQ = sm;                  // iterate slice [v][node]
   double* sF = sm + V * B3;         // flux slice [d][v][node]
   double* red = sm + 4 * V * B3;    // residual sum scratch
+  double* sG = sm + 4 * V * B3 + 16; // gradients of the slice [d][v][node] (NCP)
   double* sQh = sm;                 // after the sweeps: time averages
   double* sFh = sm + V * B3;
   const DGTables& T = c_tab[N];
@@ -98,6 +99,27 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_pde(
         }
       }
       __syncthreads();
+      if (PDE::NCP) {
+        // pencil owners: the B nodal derivatives of one (direction, variable,
+        // pencil) from one read of its values
+        for (int k = s; k < 3 * V * B2; k += O::NT) {
+          const int d = k / (V * B2), v = (k / B2) % V, ab = k % B2, a = ab / B, b = ab % B;
+          const int base = d == 0 ? a * B2 + b * B : (d == 1 ? a * B2 + b : a * B + b);
+          const int step = d == 0 ? 1 : (d == 1 ? B : B2);
+          const double idx = d == 0 ? 1.0 / dx0 : (d == 1 ? 1.0 / dx1 : 1.0 / dx2);
+          double pq[B];
+#pragma unroll
+          for (int n = 0; n < B; ++n) pq[n] = sQ[v * B3 + base + n * step];
+#pragma unroll
+          for (int l = 0; l < B; ++l) {
+            double g = 0.0;
+#pragma unroll
+            for (int n = 0; n < B; ++n) g += idx * pq[n] * T.dudx[l][n];
+            sG[(d * V + v) * B3 + base + l * step] = g;
+          }
+        }
+        __syncthreads();
+      }
       if (act) {
         // rhs0 = w^3 F0 luh (:118-133)
         double r[V];
@@ -129,11 +151,7 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_pde(
             G[0][v] = G[1][v] = G[2][v] = 0.0;
             if (PDE::NCP) {
 #pragma unroll
-              for (int n = 0; n < B; ++n) G[0][v] += 1.0 / dx0 * sQ[v * B3 + z * B2 + y * B + n] * T.dudx[x][n];
-#pragma unroll
-              for (int n = 0; n < B; ++n) G[1][v] += 1.0 / dx1 * sQ[v * B3 + z * B2 + n * B + x] * T.dudx[y][n];
-#pragma unroll
-              for (int n = 0; n < B; ++n) G[2][v] += 1.0 / dx2 * sQ[v * B3 + n * B2 + y * B + x] * T.dudx[z][n];
+              for (int d = 0; d < 3; ++d) G[d][v] = sG[(d * V + v) * B3 + s];
             }
           }
           double S[V];
