@@ -108,3 +108,23 @@ def test_point_source_rejections():
     s = E.ADERDGSolver(3, pde=E.PDE_ADVECTION_NCP, velocity=VEL, linear=True)
     with pytest.raises(RuntimeError, match="no point source"):
         s.pointSource(0.0, 0.01, (0.5, 0.5, 0.5), (0.1, 0.1, 0.1))
+
+
+def test_grid_source_on_a_shared_face_vs_oracle():
+    """A source on the face between two cells (x = 0.5 on 4 cells): the
+    device's inside test admits it in both cells, as the oracle's does."""
+    E = exa()
+    N, dims = 3, (4, 3, 3)
+    src = dict(SRC, x0=(0.5, 0.3, 0.7), t0=0.01, f0=20.0)
+    s = solver(4, N, src)
+    og = O.OracleGrid(O.cfg(N, pde=4, adv=VEL, linear=1, point_source=src), *dims)
+    og.init(O.make_ic(kind=3, nmodes=2, k=[(1, 0, 1), (0, 1, 1)], a=[0.2, 0.1], phi=[0.3, 1.1]))
+    g = E.Grid(s, *dims)
+    g.upload(og.luh().copy())
+    dt_o = og.stable_dt()
+    g.compute_dt()
+    for _ in range(3):
+        g.step(1)
+        dt_o = og.step(dt_o)
+        g.sync()
+    assert rel_err(g.download(), og.luh()) <= TOL

@@ -133,3 +133,19 @@ def test_oracle_grid_injects_the_source_in_its_cell_only():
     diff = np.abs(g.luh() - before).reshape(27, -1).max(1)
     # cell (x, y, z) = (1, 0, 2)
     assert diff[(2 * 3 + 0) * 3 + 1] > 0 and np.count_nonzero(diff) == 1
+
+
+def test_source_on_a_shared_face_counts_in_both_cells():
+    """The inside test includes the cell boundary (0 <= r <= 1), so a source
+    on the face between two cells is injected by both, as in the reference;
+    x = 0.5 on 4 cells of width 0.25 gives r = 1 and r = 0 exactly."""
+    N = 3
+    c = O.cfg(N, pde=4, adv=(0.0, 0.0, 0.0), linear=1, point_source=dict(SRC, x0=(0.5, 0.3, 0.7)))
+    g = O.OracleGrid(c, 4, 3, 3)
+    g.init(O.make_ic(kind=2, q0=[1.0, 0.0, 0.0, 0.0, 2.5]))
+    before = g.luh().copy()
+    g.step(0.01)
+    diff = np.abs(g.luh() - before).reshape(36, -1).max(1)
+    # cells (x, y, z) = (1, 0, 2) and (2, 0, 2)
+    changed = sorted(np.nonzero(diff)[0].tolist())
+    assert changed == [(2 * 3 + 0) * 4 + 1, (2 * 3 + 0) * 4 + 2], changed
