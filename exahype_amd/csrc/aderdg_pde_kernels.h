@@ -99,6 +99,26 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_pde(
         }
       }
       __syncthreads();
+      if (PDE::FLUX) {
+        // pencil owners contract one (direction, variable, pencil) of the
+        // flux slice with Kxi in place: sum_n F_n Kxi[n][l] at node l
+        for (int k = s; k < 3 * V * B2; k += O::NT) {
+          const int d = k / (V * B2), v = (k / B2) % V, ab = k % B2, a = ab / B, b = ab % B;
+          const int base = d == 0 ? a * B2 + b * B : (d == 1 ? a * B2 + b : a * B + b);
+          const int step = d == 0 ? 1 : (d == 1 ? B : B2);
+          double* pf = sF + (d * V + v) * B3 + base;
+          double pv[B];
+#pragma unroll
+          for (int n = 0; n < B; ++n) pv[n] = pf[n * step];
+#pragma unroll
+          for (int l = 0; l < B; ++l) {
+            double c = 0.0;
+#pragma unroll
+            for (int n = 0; n < B; ++n) c += pv[n] * T.Kxi[n][l];
+            pf[l * step] = c;
+          }
+        }
+      }
       if (PDE::NCP) {
         // pencil owners: the B nodal derivatives of one (direction, variable,
         // pencil) from one read of its values
@@ -118,8 +138,8 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_pde(
             sG[(d * V + v) * B3 + base + l * step] = g;
           }
         }
-        __syncthreads();
       }
+      if (PDE::FLUX || PDE::NCP) __syncthreads();
       if (act) {
         // rhs0 = w^3 F0 luh (:118-133)
         double r[V];
@@ -131,17 +151,11 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_pde(
           const double usy = ((T.w[t] * T.w[z]) * T.w[x]) * dt / dx1;
           const double usz = ((T.w[t] * T.w[y]) * T.w[x]) * dt / dx2;
 #pragma unroll
-          for (int v = 0; v < V; ++v)
-#pragma unroll
-            for (int n = 0; n < B; ++n) r[v] -= usx * sF[(0 * V + v) * B3 + z * B2 + y * B + n] * T.Kxi[n][x];
-#pragma unroll
-          for (int v = 0; v < V; ++v)
-#pragma unroll
-            for (int n = 0; n < B; ++n) r[v] -= usy * sF[(1 * V + v) * B3 + z * B2 + n * B + x] * T.Kxi[n][y];
-#pragma unroll
-          for (int v = 0; v < V; ++v)
-#pragma unroll
-            for (int n = 0; n < B; ++n) r[v] -= usz * sF[(2 * V + v) * B3 + n * B2 + y * B + x] * T.Kxi[n][z];
+          for (int v = 0; v < V; ++v) {
+            r[v] -= usx * sF[(0 * V + v) * B3 + s];
+            r[v] -= usy * sF[(1 * V + v) * B3 + s];
+            r[v] -= usz * sF[(2 * V + v) * B3 + s];
+          }
         }
         if (PDE::SRC || PDE::NCP) {
           // gradients of the iterate (:136-222), fused source (:227-252)
