@@ -397,8 +397,8 @@ def run_linear(args):
         roof = {"bound": "mfma", "achieved": flops / t / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}
     else:
         roof = {"bound": "hbm", "achieved": nbytes / t / 1e9, "peak": PEAK_HBM_GBPS, "unit": "GB/s"}
-    roof.update(frac=roof["achieved"] / roof["peak"], traffic=None,
-                kernel="k_predictor_linear<%d, PdeAdvectionNcp%s>" % (N, ", point source" if ps else ""),
+    kname = "k_predictor_linear<%d, PdeAdvectionNcp%s>" % (N, ", point source" if ps else "")
+    roof.update(frac=roof["achieved"] / roof["peak"], traffic=measured_traffic(kname, n, N), kernel=kname,
                 avg_launch_ms=pred_ms, flop_per_launch=flops,
                 bytes_per_launch=nbytes, flop_per_byte=ai)
     out = {
