@@ -66,39 +66,100 @@ def vortex_ic_kwargs(seed=0):
                 phi=rng.uniform(0, 2 * np.pi, 8))
 
 
-def run_gpu(order, n, steps, warmup, precision, waves_per_simd=0, contraction=0, fp64_sweeps=1, keep_state=True):
+def run_gpu(order, n, steps, warmup, precision, waves_per_simd=0, contraction=0, fp64_sweeps=1, keep_state=True,
+            timing_steps=5):
+    """Times `steps` steps of the device grid as the product runs them: one
+    captured hipGraph per step, no per-kernel events (the wall clock of the
+    steady-state loop).  Then runs `timing_steps` more steps with HIP events
+    around every launch (graph off) for the per-phase kernel times."""
     import exahype_amd as E
     s = E.ADERDGSolver(order, precision=precision, fp64_sweeps=fp64_sweeps, waves_per_simd=waves_per_simd,
                        contraction=contraction)
     g = E.Grid(s, n, n, n)
     g.set_initial_conditions(E.make_ic(0, **vortex_ic_kwargs()))
     g.compute_dt()
+    g.use_graph(True)
     g.step(warmup)
     g.sync()
-    g.kernel_times()  # discard warm-up timings
     g.total_picard_iterations()  # reset the device sweep counter
-    g.enable_timing(True)
     E.device_synchronize()
     t0 = time.perf_counter()
     g.step(steps)
     g.sync()
     E.device_synchronize()
     wall = time.perf_counter() - t0
-    times, counts = g.kernel_times()
     sweeps = g.total_picard_iterations() / (g.ncells * steps)
     state = g.download() if keep_state else None
+    g.kernel_times()  # discard
+    g.enable_timing(True)
+    g.step(timing_steps)
+    g.sync()
+    times, counts = g.kernel_times()
     g.close()
     s.close()
     return dict(wall=wall, times=times, counts=counts, iters=sweeps, state=state)
 
 
-def l2_per_variable(a, b, N, ncells):
-    """Domain L2 norm (GL quadrature per cell, unit cube) of a-b per variable."""
+def error_norms(a, b, N, ncells):
+    """Domain L1, L2 (GL quadrature per cell, unit cube) and max norms of a-b
+    per variable."""
     from exahype_amd.tables import dg_math
     w = dg_math.gl_nodes_weights(N)[1]
     w3 = np.einsum("z,y,x->zyx", w, w, w).ravel()
     d = (a - b).reshape(ncells, (N + 1) ** 3, NVAR)
-    return np.sqrt(np.einsum("n,cnv->v", w3, d ** 2) / ncells)
+    return dict(l1=np.einsum("n,cnv->v", w3, np.abs(d)) / ncells,
+                l2=np.sqrt(np.einsum("n,cnv->v", w3, d ** 2) / ncells),
+                linf=np.abs(d).max(axis=(0, 1)))
+
+
+def l2_per_variable(a, b, N, ncells):
+    return error_norms(a, b, N, ncells)["l2"]
+
+
+def host_info():
+    """CPU model, logical CPUs usable by this process and the oracle's compiler."""
+    import platform
+    import subprocess
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cc = subprocess.check_output(["gcc", "--version"], text=True).splitlines()[0]
+    except (OSError, subprocess.CalledProcessError):
+        cc = "unknown"
+    return dict(cpu_model=model, affinity_cpus=len(os.sched_getaffinity(0)), os_cpu_count=os.cpu_count(),
+                cgroup_cpu_quota=cgroup_cpus(), omp_num_threads=os.environ.get("OMP_NUM_THREADS"), compiler=cc,
+                oracle_flags="-O3 -march=x86-64-v3 -ffp-contract=off -fopenmp (built in the build container; native would target its CPU, not this host)")
+
+
+def cgroup_cpus():
+    """CPUs of this process's cgroup quota (cpu.max), or None without one."""
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        return None if quota == "max" else float(quota) / float(period)
+    except (OSError, ValueError):
+        return None
+
+
+def host_threads():
+    """The oracle's OpenMP team: all the CPUs this process may use -- the
+    affinity mask, capped by the cgroup CPU quota and by OMP_NUM_THREADS
+    where the host sets them (threads beyond the quota only time-slice it)."""
+    n = len(os.sched_getaffinity(0))
+    q = cgroup_cpus()
+    if q:
+        n = min(n, max(1, int(q)))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
 
 
 def cpu_baseline(order, n_full, threads, sample_cells=27, steps=1):
@@ -188,9 +249,10 @@ def main():
         # the staged form is FP64 only: the mixed runs take their tuned form
         cm = 0 if args.contraction == 4 else args.contraction
         r = run_gpu(N, n, args.steps, args.warmup, prec, args.waves_per_simd, cm, sweeps)
-        l2 = l2_per_variable(r["state"], r64["state"], N, ncells)
-        r.update(rate=dof * args.steps / r["wall"], l2=l2, ok=bool(l2.max() <= 1e-6), name=name, sweeps_fp64=sweeps,
-                 prec=prec)
+        err = error_norms(r["state"], r64["state"], N, ncells)
+        l2 = err["l2"]
+        r.update(rate=dof * args.steps / r["wall"], l2=l2, err=err, ok=bool(l2.max() <= 1e-6), name=name,
+                 sweeps_fp64=sweeps, prec=prec)
         del r["state"]
         mixed[name] = r
     passing = [r for r in mixed.values() if r["ok"]]
@@ -214,10 +276,16 @@ def main():
         peak_lp = PEAK_BF16_TFLOPS if best["prec"] == E.PREC_BF16_MFMA else PEAK_FP32_TFLOPS
         peak = flops / (f_lp / peak_lp + f64 / PEAK_FP64_TFLOPS)
         kname = "k_predictor<%d, %s + %d FP64 sweep(s)>" % (N, best["name"], best["sweeps_fp64"])
+        # what bounds each part: FP32 sweeps on the vector ALU (no packed
+        # FP32), MFMA forms on the matrix cores, FP64 sweeps and tail on the
+        # FP64 vector ALU
+        bound = {E.PREC_FP32: "fp32_valu+fp64_valu", E.PREC_FP32_MFMA: "mfma_fp32+fp64_valu",
+                 E.PREC_BF16_MFMA: "mfma_bf16+fp64_valu"}[best["prec"]]
     else:
         flops = predictor_flops_per_cell(N, sweeps) * ncells
         peak = PEAK_FP64_TFLOPS
         kname = "k_predictor<%d, FP64>" % N
+        bound = "fp64_valu"
     achieved = flops / (pred_ms * 1e-3) / 1e12
 
     out = {
@@ -239,7 +307,7 @@ def main():
                                  % (best["name"], best["sweeps_fp64"]) if mixed_ok else "fp64"),
                    "waves_per_simd": args.waves_per_simd or "tuned", "contraction": args.contraction or "tuned",
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+        "roofline": {"bound": bound, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": measured_traffic(kname, n, N),
                      "kernel": kname, "avg_launch_ms": pred_ms,
                      "picard_sweeps_per_cell_step": sweeps, "flop_per_launch": flops},
@@ -252,10 +320,15 @@ def main():
             **{k: {"dof_updates_per_s": r["rate"], "ms_per_step": r["wall"] / args.steps * 1e3,
                    "predictor_ms": r["times"][0] / max(r["counts"][0], 1), "sweeps_per_cell": r["iters"],
                    "fp64_correction_sweeps": r["sweeps_fp64"], "l2_vs_fp64_per_variable": r["l2"].tolist(),
+                   "l1_vs_fp64_per_variable": r["err"]["l1"].tolist(),
+                   "linf_vs_fp64_per_variable": r["err"]["linf"].tolist(),
                    "speedup_vs_fp64": r["rate"] / rate64, "l2_ok": r["ok"]} for k, r in mixed.items()}),
+        "timing": "value and ms_per_step: wall clock of the steps replayed as one hipGraph per step, no events; "
+                  "phase_ms_per_step and roofline: HIP events around every launch of %d further steps "
+                  "(graph off)" % 5,
     }
     if not args.no_cpu_baseline:
-        threads = min(len(os.sched_getaffinity(0)), 16)
+        threads = host_threads()
         # bounded CPU sample: the work per cell-step grows ~B^5, so the step
         # counts shrink with the order (30 and 20 steps at N=5)
         scale = (6.0 / (N + 1)) ** 5
@@ -263,6 +336,7 @@ def main():
         v, wall = cpu_baseline(N, n, threads, sample_cells=27, steps=st)
         v1, wall1 = cpu_baseline(N, n, 1, sample_cells=9, steps=st1)
         out["cpu_baseline"] = {"value": v, "unit": "DoF-updates/s", "cores": threads, "kind": "port",
+                               "host": host_info(),
                                "sample": "oracle (C, OpenMP), order %d, 27^3 periodic cells of size 1/%d, same initial "
                                          "data, %d steps (%.1f s wall)" % (N, n, st, wall),
                                "single_thread": {"value": v1, "sample": "9^3 cells, %d steps (%.1f s)" % (st1, wall1)}}
@@ -344,7 +418,7 @@ def run_ncp(args):
                                "ADER-DG order %d, %d^3 periodic cells, 2 sine modes, CFL 0.9" % (N, n),
                    "velocity": list(NCP_VELOCITY), "cells": ncells, "order": N, "dof_per_step": dof,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
                      "kernel": "k_predictor_pde<%d, PdeAdvectionNcp>" % N, "avg_launch_ms": pred_ms,
                      "picard_sweeps_per_cell_step": r["iters"], "flop_per_launch": flops},
@@ -355,7 +429,7 @@ def run_ncp(args):
     }
     if not args.no_cpu_baseline:
         from oracle import oracle as O
-        threads = min(len(os.sched_getaffinity(0)), 16)
+        threads = host_threads()
         steps = max(1, min(8, int(round(8 * (6.0 / (N + 1)) ** 5))))
         og = O.OracleGrid(O.cfg(N, pde=4, adv=NCP_VELOCITY), 27, 27, 27)
         for d in range(3):
@@ -394,7 +468,7 @@ def run_linear(args):
     t = pred_ms * 1e-3
     ai = flops / nbytes
     if ai >= PEAK_FP64_TFLOPS * 1e12 / (PEAK_HBM_GBPS * 1e9):
-        roof = {"bound": "mfma", "achieved": flops / t / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}
+        roof = {"bound": "fp64_valu", "achieved": flops / t / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}
     else:
         roof = {"bound": "hbm", "achieved": nbytes / t / 1e9, "peak": PEAK_HBM_GBPS, "unit": "GB/s"}
     kname = "k_predictor_linear<%d, PdeAdvectionNcp%s>" % (N, ", point source" if ps else "")
@@ -426,7 +500,7 @@ def run_linear(args):
     }
     if not args.no_cpu_baseline:
         from oracle import oracle as O
-        threads = min(len(os.sched_getaffinity(0)), 16)
+        threads = host_threads()
         steps = 20
         # the point source of the sample sits at the same place of the
         # (smaller) domain: the centre of its middle cell
@@ -485,7 +559,7 @@ def run_fv2d(args):
            "config": {"workload": "EulerFV demonstrator: 2D Euler, Godunov + Rusanov, 81x81 patches of 10x10, "
                                   "periodic, CFL 0.9", "volumes": vols, "first_dt": dt0, "t_end": t_end}}
     if not args.no_cpu_baseline:
-        threads = min(len(os.sched_getaffinity(0)), 16)
+        threads = host_threads()
         c = O.cfg(1, pde=2)
         og = O.OracleFVGrid(c, 2, 0, patch, n, n, 1)
         og.init(fv2d_ic(O))
@@ -555,7 +629,7 @@ def run_muscl(args):
                         "kernel": "k_muscl", "avg_launch_ms": upd_ms,
                         "bytes_per_launch": bytes_upd}}
     if not args.no_cpu_baseline:
-        threads = min(len(os.sched_getaffinity(0)), 16)
+        threads = host_threads()
         c = O.cfg(1)
         ns, st = 27, 8
         og = O.OracleFVGrid(c, 3, 1, patch, ns, ns, ns)

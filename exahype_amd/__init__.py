@@ -191,9 +191,18 @@ def load_library(path=LIB_PATH):
     return L
 
 
+EXA_ERR_NUMERICS = -5
+
+
+class NumericsError(RuntimeError):
+    """The state holds non-finite values or a negative pressure: its stable
+    time step is not a positive number (EXA_ERR_NUMERICS)."""
+
+
 def _check(rc):
     if rc != EXA_OK:
-        raise RuntimeError("exahype_amd error %d: %s" % (rc, _lib.exa_last_error().decode()))
+        msg = "exahype_amd error %d: %s" % (rc, _lib.exa_last_error().decode())
+        raise (NumericsError if rc == EXA_ERR_NUMERICS else RuntimeError)(msg)
 
 
 def _p(a):

@@ -1353,7 +1353,7 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_update(double* __restrict__ luh,
       den += m[0] / dx0;
       den += m[1] / dx1;
       den += m[2] / dx2;
-      dtn = cfl * exa_pnpm(N) / den;
+      dtn = exa_dt_sane(cfl * exa_pnpm(N) / den);
     }
   }
   if (MODE & UPD_DT) {

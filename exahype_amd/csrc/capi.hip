@@ -622,6 +622,7 @@ static int run_update_cell(exa_solver* s, double* luh_ref, double* lduh_ref, con
   if (MODE & exa::UPD_DT) {
     EXA_CHECK(hipMemcpy(hdt, s->dt.p, sizeof(hdt), hipMemcpyDeviceToHost));
     *dt_out = hdt[1];
+    EXA_TRY(exa_check_dt(hdt[1], "stableTimeStepSize"));
   }
   return EXA_OK;
 }
@@ -975,6 +976,7 @@ struct exa_grid {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   bool timing = false;
+  bool dt_valid = false;  // a dt has been computed or set (sync checks it from then on)
   long long version = 0;  // bumped whenever luh changes (steps, uploads, initial conditions)
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
   double times[3] = {0, 0, 0};
@@ -1191,7 +1193,8 @@ extern "C" int exa_grid_compute_dt(exa_grid* g, double* dt) {
   EXA_CHECK(hipMemcpyAsync(&h, g->dt_cur(), sizeof(double), hipMemcpyDeviceToHost, g->stream));
   EXA_CHECK(hipStreamSynchronize(g->stream));
   if (dt) *dt = h;
-  return EXA_OK;
+  g->dt_valid = true;
+  return exa_check_dt(h, "exa_grid_compute_dt");
 }
 
 extern "C" int exa_grid_set_dt(exa_grid* g, double dt) {
@@ -1199,6 +1202,7 @@ extern "C" int exa_grid_set_dt(exa_grid* g, double dt) {
   EXA_CHECK(hipSetDevice(g->s->device));
   EXA_CHECK(hipMemcpyAsync(g->dt_cur(), &dt, sizeof(double), hipMemcpyHostToDevice, g->stream));
   EXA_CHECK(hipStreamSynchronize(g->stream));
+  g->dt_valid = true;
   return EXA_OK;
 }
 
@@ -1340,7 +1344,7 @@ extern "C" int exa_grid_sync(exa_grid* g, double* time, double* next_dt) {
   EXA_CHECK(hipGetLastError());
   if (time) *time = st[2];
   if (next_dt) *next_dt = st[0];
-  return EXA_OK;
+  return g->dt_valid ? exa_check_dt(st[0], "exa_grid_sync") : EXA_OK;
 }
 
 extern "C" int exa_grid_picard_iterations(exa_grid* g, int* iters) {
@@ -1405,6 +1409,7 @@ struct exa_ensemble {
   std::vector<std::pair<int, int>> where;  // instance -> (group, index in group)
   hipStream_t stream = nullptr;
   bool use_graph = true;
+  bool dt_valid = false;  // exa_ensemble_compute_dt has run (sync checks the dts from then on)
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
 };
@@ -1591,7 +1596,8 @@ extern "C" int exa_ensemble_compute_dt(exa_ensemble* e) {
     EXA_CHECK(hipGetLastError());
   }
   EXA_CHECK(hipStreamSynchronize(e->stream));
-  return EXA_OK;
+  e->dt_valid = true;
+  return exa_ensemble_sync(e, nullptr, nullptr);
 }
 
 static int enqueue_ensemble_step(exa_ensemble* e) {
@@ -1663,6 +1669,14 @@ extern "C" int exa_ensemble_sync(exa_ensemble* e, double* times, double* dts) {
     for (size_t j = 0; j < p.size(); ++j) {
       if (times) times[gr->members[j]] = p[j].time;
       if (dts) dts[gr->members[j]] = p[j].dt_cur;
+    }
+    for (size_t j = 0; e->dt_valid && j < p.size(); ++j) {
+      const int r = exa_check_dt(p[j].dt_cur, "exa_ensemble_sync");
+      if (r != EXA_OK) {
+        exa_set_error("exa_ensemble_sync: member %d has stable time step %g -- its state holds non-finite values "
+                      "or a negative pressure", gr->members[j], p[j].dt_cur);
+        return r;
+      }
     }
   }
   return EXA_OK;

@@ -56,11 +56,20 @@ __host__ __device__ inline double exa_pnpm(int N) {
   return t[N];
 }
 
+// A time-step candidate that is not a finite positive number -- from a state
+// with NaN entries or negative pressure (sqrt of a negative sound speed
+// squared) -- becomes 0.0.  Zero then wins every minimum (comparisons with
+// NaN are false, so a NaN would be dropped or kept depending on the order of
+// the reduction), and the host reads a next time step of 0 as a blown-up
+// state instead of stepping on with a finite-looking dt.
+__device__ inline double exa_dt_sane(double v) { return (v > 0.0 && v <= 1.7976931348623157e308) ? v : 0.0; }
+
 // positive doubles order like their bit patterns: min via integer atomics
 // The minimum only decreases: a value not below the current one skips the
 // atomic, so the hundreds of thousands of per-cell candidates of a step cost
 // a plain load each and only the few improving ones contend for the address.
 __device__ inline void atomic_min_pos_double(unsigned long long* addr, double v) {
+  v = exa_dt_sane(v);
   const unsigned long long b = (unsigned long long)__double_as_longlong(v);
   if (b < __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(addr, b);
 }

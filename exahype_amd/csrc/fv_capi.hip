@@ -170,7 +170,7 @@ extern "C" int exa_fv_stable_time_step_size(exa_fv_solver* s, const double* luh,
   EXA_TRY(launch_fv_dt(s, s->stream, 1, s->a.as<double>(), s->st.as<double>(), nullptr, dx));
   EXA_CHECK(hipStreamSynchronize(s->stream));
   EXA_CHECK(hipMemcpy(dt, s->st.p, sizeof(double), hipMemcpyDeviceToHost));
-  return EXA_OK;
+  return exa_check_dt(*dt, "FV stableTimeStepSize");
 }
 
 template <class PDE>
@@ -216,6 +216,7 @@ struct exa_fv_grid {
   double off[3] = {0, 0, 0}, width[3] = {1, 1, 1}, dx[3] = {1, 1, 1};
   DevBuf cur, nxt, nbr, state;
   hipStream_t stream = nullptr;
+  bool dt_valid = false;  // a stable dt has been computed (sync checks it from then on)
   // per-phase kernel timing (0 ghost layers, 1 unused, 2 update, 3 dt)
   bool timing = false;
   std::vector<std::vector<hipEvent_t>> marks;  // 5 events per timed step
@@ -402,7 +403,8 @@ extern "C" int exa_fv_grid_compute_dt(exa_fv_grid* g, double* dt) {
   EXA_CHECK(hipMemcpyAsync(&h, g->dt_cur(), sizeof(double), hipMemcpyDeviceToHost, g->stream));
   EXA_CHECK(hipStreamSynchronize(g->stream));
   if (dt) *dt = h;
-  return EXA_OK;
+  g->dt_valid = true;
+  return exa_check_dt(h, "exa_fv_grid_compute_dt");
 }
 
 extern "C" int exa_fv_grid_step(exa_fv_grid* g, int nsteps) {
@@ -448,5 +450,5 @@ extern "C" int exa_fv_grid_sync(exa_fv_grid* g, double* time, double* next_dt, d
   if (time) *time = st[2];
   if (next_dt) *next_dt = st[0];
   if (admissible_dt) *admissible_dt = st[3];
-  return EXA_OK;
+  return g->dt_valid ? exa_check_dt(st[0], "exa_fv_grid_sync") : EXA_OK;
 }

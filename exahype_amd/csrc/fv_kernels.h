@@ -107,13 +107,13 @@ __global__ __launch_bounds__(256) void k_godunov(const double* __restrict__ luh,
       const double fac = dt / h[d];
       load_q(u, nbox, idx - g.stride(d), qn);
       double s = rusanov<PDE>(qn, q, d, gamma, f);
-      double cand = cfl / DIM * h[d] / s;
+      double cand = exa_dt_sane(cfl / DIM * h[d] / s);
       dtm = cand < dtm ? cand : dtm;
 #pragma unroll
       for (int v = 0; v < V; ++v) acc[v] += fac * f[v];
       load_q(u, nbox, idx + g.stride(d), qn);
       s = rusanov<PDE>(q, qn, d, gamma, f);
-      cand = cfl / DIM * h[d] / s;
+      cand = exa_dt_sane(cfl / DIM * h[d] / s);
       dtm = cand < dtm ? cand : dtm;
 #pragma unroll
       for (int v = 0; v < V; ++v) acc[v] -= fac * f[v];
@@ -243,14 +243,14 @@ __global__ __launch_bounds__(256) void k_muscl(const double* __restrict__ luh, d
       muscl_face_state(u, nbox, g, sQt, idx - g.stride(d), d, 1, h[d], dt, a);
       muscl_face_state(u, nbox, g, sQt, idx, d, 0, h[d], dt, b);
       double s = rusanov<PDE>(a, b, d, gamma, f);
-      double cand = cfl / 3 * h[d] / s;
+      double cand = exa_dt_sane(cfl / 3 * h[d] / s);
       dtm = cand < dtm ? cand : dtm;
 #pragma unroll
       for (int v = 0; v < V; ++v) acc[v] += fac * f[v];
       muscl_face_state(u, nbox, g, sQt, idx, d, 1, h[d], dt, a);
       muscl_face_state(u, nbox, g, sQt, idx + g.stride(d), d, 0, h[d], dt, b);
       s = rusanov<PDE>(a, b, d, gamma, f);
-      cand = cfl / 3 * h[d] / s;
+      cand = exa_dt_sane(cfl / 3 * h[d] / s);
       dtm = cand < dtm ? cand : dtm;
 #pragma unroll
       for (int v = 0; v < V; ++v) acc[v] -= fac * f[v];
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(256) void k_fv_stable_dt(const double* __restrict__
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
       const double s = PDE::max_abs_eigenvalue(q, d, gamma);
-      const double cand = cfl / DIM * (dxs[d] / g.P) / s;
+      const double cand = exa_dt_sane(cfl / DIM * (dxs[d] / g.P) / s);
       dtm = cand < dtm ? cand : dtm;
     }
   }

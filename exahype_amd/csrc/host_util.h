@@ -70,3 +70,13 @@ static inline unsigned long long exa_big_bits() {
   __builtin_memcpy(&b, &big, sizeof(b));
   return b;
 }
+
+// A stable time step that is 0 (or not a finite positive number) means a cell
+// held NaN entries or a negative pressure: the kernels' dt reduction maps
+// every such candidate to 0 so that it wins the global minimum.
+static inline int exa_check_dt(double dt, const char* what) {
+  if (dt > 0.0 && dt <= 1.7976931348623157e308) return EXA_OK;
+  exa_set_error("%s: the stable time step is %g -- the state holds non-finite values or a negative pressure", what,
+                dt);
+  return EXA_ERR_NUMERICS;
+}

@@ -258,7 +258,9 @@ class Spec:
         box = max(self.domain_size)
         level = compute_mesh_level(self.maximum_mesh_size(k), box)
         h = box / 3 ** (level - 1)
-        cells = tuple(int(math.ceil(w / h - 1e-9)) for w in self.domain_size)
+        # Runner::determineScaledDomainSize (E/runners/Runner.cpp:389-398):
+        # std::ceil(domain / meshSize), no tolerance
+        cells = tuple(int(math.ceil(w / h)) for w in self.domain_size)
         return cells, h, tuple(n * h for n in cells)
 
     def solver_config(self, k):

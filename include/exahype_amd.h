@@ -40,6 +40,8 @@ extern "C" {
 #define EXA_ERR_HIP (-2)
 #define EXA_ERR_UNSUPPORTED (-3)
 #define EXA_ERR_NOMEM (-4)
+#define EXA_ERR_NUMERICS (-5) /* the state is not finite or has a negative pressure: its stable dt
+                                  is not a positive number (reported by the dt and sync calls)  */
 
 /* predictor precision */
 #define EXA_PREC_FP64 0       /* plain method: every operation in FP64                        */

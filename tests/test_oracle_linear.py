@@ -95,7 +95,7 @@ def _poly(N, seed):
 TOL_Q, TOL_G = 1e-11, 1e-8
 
 
-@pytest.mark.parametrize("N", [1, 2, 3, 5, 7, 9])
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7, 8, 9])
 def test_predictor_linear_exact_for_polynomials(N):
     c = O.cfg(N, pde=4, adv=VEL, linear=1)
     B = N + 1
