@@ -70,7 +70,7 @@ struct EulerPDE {
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       const double a = fabs(lam[v]);
-      m = (m < a) ? a : m;
+      m = (a < m) ? m : a;  // std::max(a, m): a NaN wave speed propagates
     }
     return m;
   }
@@ -89,7 +89,7 @@ struct EulerPDE {
 #pragma unroll
       for (int v = 0; v < 5; ++v) {
         const double a = fabs(lam[v]);
-        mm = (mm < a) ? a : mm;
+        mm = (a < mm) ? mm : a;
       }
       m[d] = mm;
     }

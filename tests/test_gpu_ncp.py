@@ -127,6 +127,11 @@ def _vortex_ic():
 @pytest.mark.parametrize("pde", [3, 4, 5])
 @pytest.mark.parametrize("N,dims,steps", [(3, (4, 3, 3), 3), (5, (3, 2, 3), 2), (7, (2, 2, 2), 2)])
 def test_grid_steps_vs_oracle(pde, N, dims, steps):
+    if pde == 5 and N == 7:
+        # the reference test's artificial product drives the pressure of a
+        # node negative in the second step (-8.9e-3 by the oracle); that
+        # state is checked to be reported in the next test
+        steps = 1
     print("grid pde", pde, "N", N, dims, flush=True)
     g, og = _grid_pair(pde, N, dims, _vortex_ic() if pde == 5 else _sine_ic())
     dt_o = og.stable_dt()
@@ -139,6 +144,27 @@ def test_grid_steps_vs_oracle(pde, N, dims, steps):
         assert abs(dt_g - dt_o) <= 1e-12 * dt_o, (k, dt_g, dt_o)
         assert np.array_equal(g.picard_iterations(), og.iters()), k
     assert rel_err(g.download(), og.luh()) <= TOL
+
+
+def test_negative_pressure_of_the_test_product_is_reported():
+    """Euler with the reference test's product at N=7 on 2^3 cells: after two
+    steps a node holds a negative pressure (the oracle's state, below), where
+    the Euler wave speed sqrt(gamma p / rho) is not a number.  The device dt
+    keeps that NaN (std::max(|lambda|, m) as stableTimeStepSize.cpph:45-47
+    accumulates it) and the sync call reports the state instead of stepping on."""
+    E = exa()
+    N, dims = 7, (2, 2, 2)
+    g, og = _grid_pair(5, N, dims, _vortex_ic())
+    dt_o = og.stable_dt()
+    g.compute_dt()
+    for _ in range(2):
+        dt_o = og.step(dt_o)
+    u = og.luh().reshape(-1, 5)
+    p = 0.4 * (u[:, 4] - 0.5 * (u[:, 1] ** 2 + u[:, 2] ** 2 + u[:, 3] ** 2) / u[:, 0])
+    assert p.min() < 0.0
+    g.step(2)
+    with pytest.raises(E.NumericsError):
+        g.sync()
 
 
 @pytest.mark.parametrize("N", [2, 3, 5])
