@@ -88,6 +88,7 @@ EXPORTS = [
     "exa_amr_face_unknowns_prolongation", "exa_amr_face_unknowns_restriction",
     "exa_amr_volume_unknowns_prolongation", "exa_amr_volume_unknowns_restriction",
     "exa_amr_prolongation_batched", "exa_amr_restriction_batched", "exa_amr_last_kernel_ms",
+    "exa_amr_restriction_batched_noalloc",
     "exa_device_malloc", "exa_device_free", "exa_memcpy_htod", "exa_memcpy_dtoh", "exa_device_copy",
     "exa_selftest_mfma",
     "exa_ensemble_create", "exa_ensemble_destroy", "exa_ensemble_set_initial_conditions", "exa_ensemble_compute_dt",
@@ -185,7 +186,7 @@ def load_library(path=LIB_PATH):
         getattr(L, nm).argtypes = [I, I, I, _D, _D, _D, _D, I, I, P(I)]
     for nm in ("exa_amr_volume_unknowns_prolongation", "exa_amr_volume_unknowns_restriction"):
         getattr(L, nm).argtypes = [I, I, I, _D, _D, I, I, P(I)]
-    for nm in ("exa_amr_prolongation_batched", "exa_amr_restriction_batched"):
+    for nm in ("exa_amr_prolongation_batched", "exa_amr_restriction_batched", "exa_amr_restriction_batched_noalloc"):
         getattr(L, nm).argtypes = [I, I, I, C.c_void_p, LL, C.c_void_p, LL, P(AmrTransfer), I]
     _lib = L
     return L
@@ -810,6 +811,20 @@ def amrRestrictionBatched(dim, order, ndata, coarse, fine, transfers):
     pf, nf = _dev(fine)
     pc, nc = _dev(coarse)
     _check(L.exa_amr_restriction_batched(dim, order, ndata, pc, nc, pf, nf, _transfers(transfers), len(transfers)))
+
+
+def amrRestrictionBatchedNoAlloc(dim, order, ndata, coarse, fine, transfers):
+    """amrRestrictionBatched through the allocation-free entry point (at most
+    EXA_AMR_NOALLOC_MAX_TRANSFERS transfers; exa_init must have run)."""
+    L = load_library()
+    _check(L.exa_init(0))
+    pf, nf = _dev(fine)
+    pc, nc = _dev(coarse)
+    _check(L.exa_amr_restriction_batched_noalloc(dim, order, ndata, pc, nc, pf, nf, _transfers(transfers),
+                                                 len(transfers)))
+
+
+AMR_NOALLOC_MAX_TRANSFERS = 16384
 
 
 # ---------------------------------------------------------------------------

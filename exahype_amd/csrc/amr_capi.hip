@@ -28,8 +28,15 @@ struct AmrCache {
 };
 AmrCache g_amr[64];
 
-int amr_tables(int dev, int N, const AmrTables** out) {
-  AmrCache& c = g_amr[dev];
+void fill_tables(int N, AmrTables& t) {
+  for (int s = 0; s < 3; ++s)
+    for (int i = 0; i <= N; ++i)
+      for (int j = 0; j <= N; ++j) t.P[s][i][j] = EXA_TAB_fineGridProjector1d[N][s][i][j];
+  for (int i = 0; i <= N; ++i) t.w[i] = EXA_TAB_gl_weights[N][i];
+}
+
+// Dynamic-LDS limit of every transfer kernel instance, once per device.
+int amr_lds_attrs(AmrCache& c) {
   if (!c.lds_set) {
     const int bytes = 2 * exa::amr::AMR_LDS_DOUBLES * (int)sizeof(double);
     hipError_t e = hipSuccess;
@@ -47,12 +54,15 @@ int amr_tables(int dev, int N, const AmrTables** out) {
     EXA_CHECK(e);
     c.lds_set = true;
   }
+  return EXA_OK;
+}
+
+int amr_tables(int dev, int N, const AmrTables** out) {
+  AmrCache& c = g_amr[dev];
+  EXA_TRY(amr_lds_attrs(c));
   if (!c.ready[N]) {
     AmrTables t{};
-    for (int s = 0; s < 3; ++s)
-      for (int i = 0; i <= N; ++i)
-        for (int j = 0; j <= N; ++j) t.P[s][i][j] = EXA_TAB_fineGridProjector1d[N][s][i][j];
-    for (int i = 0; i <= N; ++i) t.w[i] = EXA_TAB_gl_weights[N][i];
+    fill_tables(N, t);
     EXA_TRY(c.tab[N].ensure(sizeof(AmrTables)));
     EXA_CHECK(hipMemcpy(c.tab[N].p, &t, sizeof(t), hipMemcpyHostToDevice));
     c.ready[N] = true;
@@ -148,6 +158,33 @@ int one_transfer(int dim, bool prolong, int order, int nd, double* fine, double*
   return EXA_OK;
 }
 
+// Validates a transfer list and fills items[count] and groups[*ngroups + 1]
+// (group g = items groups[g] .. groups[g+1]-1); outs needs count entries.
+// Prolongation: one group per transfer; restriction: runs of equal coarse
+// offsets.  Output blocks of different groups must not overlap.
+int build_list(int dim, bool prolong, long long blk, long long n_fine, long long n_coarse,
+               const exa_amr_transfer* transfers, int count, Transfer* items, int* groups, long long* outs,
+               int* ngroups) {
+  int ng = 0;
+  for (int i = 0; i < count; ++i) {
+    EXA_TRY(check_transfer(transfers[i], dim));
+    const exa_amr_transfer& t = transfers[i];
+    EXA_ARG(t.fine + blk <= n_fine && t.coarse + blk <= n_coarse, "transfer block beyond the end of its array");
+    items[i] = Transfer{t.fine, t.coarse, t.levels, {t.index[0], t.index[1], dim == 3 ? t.index[2] : 0}};
+    if (prolong || i == 0 || t.coarse != transfers[i - 1].coarse) groups[ng++] = i;
+  }
+  groups[ng] = count;
+  for (int g = 0; g < ng; ++g) outs[g] = prolong ? items[groups[g]].fine : items[groups[g]].coarse;
+  std::sort(outs, outs + ng);
+  for (int i = 1; i < ng; ++i)
+    EXA_ARG(outs[i] >= outs[i - 1] + blk,
+            prolong ? "two prolongations write overlapping fine blocks"
+                    : "restrictions into one coarse block must be consecutive in the list, and coarse blocks "
+                      "must not overlap");
+  *ngroups = ng;
+  return EXA_OK;
+}
+
 int batched(int dim, bool prolong, int order, int nd, double* d_fine, long long n_fine, double* d_coarse,
             long long n_coarse, const exa_amr_transfer* transfers, int count) {
   EXA_ARG(order >= 0 && order <= EXA_MAX_ORDER, "order must be in 0..9");
@@ -157,30 +194,11 @@ int batched(int dim, bool prolong, int order, int nd, double* d_fine, long long 
   const int B = order + 1;
   const long long blk = (long long)(dim == 3 ? B * B * B : B * B) * nd;
   std::vector<Transfer> items(count);
-  std::vector<int> groups;
-  groups.reserve(count + 1);
-  for (int i = 0; i < count; ++i) {
-    EXA_TRY(check_transfer(transfers[i], dim));
-    const exa_amr_transfer& t = transfers[i];
-    EXA_ARG(t.fine + blk <= n_fine && t.coarse + blk <= n_coarse, "transfer block beyond the end of its array");
-    items[i] = Transfer{t.fine, t.coarse, t.levels, {t.index[0], t.index[1], dim == 3 ? t.index[2] : 0}};
-    // prolongation: one group per transfer; restriction: runs of equal coarse offsets
-    if (prolong || i == 0 || t.coarse != transfers[i - 1].coarse) groups.push_back(i);
-  }
-  groups.push_back(count);
-  // Output blocks of different workgroups must not overlap: each fine block
-  // is written by one prolongation, each coarse block by one run of restrictions.
-  {
-    std::vector<long long> outs;
-    for (size_t g = 0; g + 1 < groups.size(); ++g)
-      outs.push_back(prolong ? items[groups[g]].fine : items[groups[g]].coarse);
-    std::sort(outs.begin(), outs.end());
-    for (size_t i = 1; i < outs.size(); ++i)
-      EXA_ARG(outs[i] >= outs[i - 1] + blk,
-              prolong ? "two prolongations write overlapping fine blocks"
-                      : "restrictions into one coarse block must be consecutive in the list, and coarse blocks "
-                        "must not overlap");
-  }
+  std::vector<int> groups(count + 1);
+  std::vector<long long> outs(count);
+  int ng = 0;
+  EXA_TRY(build_list(dim, prolong, blk, n_fine, n_coarse, transfers, count, items.data(), groups.data(),
+                     outs.data(), &ng));
   int dev = 0;
   EXA_CHECK(hipGetDevice(&dev));
   EXA_TRY(exa_init(dev));
@@ -188,17 +206,16 @@ int batched(int dim, bool prolong, int order, int nd, double* d_fine, long long 
   std::lock_guard<std::mutex> lk(c.mu);
   const AmrTables* tab = nullptr;
   EXA_TRY(amr_tables(dev, order, &tab));
-  EXA_TRY(c.items.ensure(sizeof(Transfer) * items.size()));
-  EXA_TRY(c.groups.ensure(sizeof(int) * groups.size()));
-  EXA_CHECK(hipMemcpy(c.items.p, items.data(), sizeof(Transfer) * items.size(), hipMemcpyHostToDevice));
-  EXA_CHECK(hipMemcpy(c.groups.p, groups.data(), sizeof(int) * groups.size(), hipMemcpyHostToDevice));
+  EXA_TRY(c.items.ensure(sizeof(Transfer) * count));
+  EXA_TRY(c.groups.ensure(sizeof(int) * (ng + 1)));
+  EXA_CHECK(hipMemcpy(c.items.p, items.data(), sizeof(Transfer) * count, hipMemcpyHostToDevice));
+  EXA_CHECK(hipMemcpy(c.groups.p, groups.data(), sizeof(int) * (ng + 1), hipMemcpyHostToDevice));
   if (!c.e0) {
     EXA_CHECK(hipEventCreate(&c.e0));
     EXA_CHECK(hipEventCreate(&c.e1));
   }
   EXA_CHECK(hipEventRecord(c.e0, nullptr));
-  EXA_TRY(launch(dim, prolong, tab, c.items.as<Transfer>(), c.groups.as<int>(), (int)groups.size() - 1, d_fine,
-                 d_coarse, B, nd));
+  EXA_TRY(launch(dim, prolong, tab, c.items.as<Transfer>(), c.groups.as<int>(), ng, d_fine, d_coarse, B, nd));
   EXA_CHECK(hipEventRecord(c.e1, nullptr));
   EXA_CHECK(hipDeviceSynchronize());
   float ms = 0.f;
@@ -271,5 +288,84 @@ extern "C" int exa_amr_last_kernel_ms(double* ms) {
   AmrCache& c = g_amr[dev];
   std::lock_guard<std::mutex> lk(c.mu);
   *ms = c.last_ms;
+  return EXA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free batched restriction.  For callers that cannot allocate
+// device memory (code under HIP graph capture, tight serving loops, harnesses
+// that forbid hipMalloc).  The operator tables and the transfer list are
+// staged in `scratch`, a fixed file-scope device array of
+// EXA_AMR_NOALLOC_MAX_TRANSFERS transfers (about 590 KB) that every GPU the
+// library runs on holds until the process ends.  exa_init takes that array's
+// address on each device once (exa_amr_noalloc_setup); host code only ever
+// uses that address, never the array's name.  Calls are serialised by a
+// process-wide lock; the array keeps the last call's tables and list, and two
+// processes calling at once on one GPU would race on it.
+namespace {
+struct NoAllocScratch {
+  AmrTables tab;
+  Transfer items[EXA_AMR_NOALLOC_MAX_TRANSFERS];
+  int groups[EXA_AMR_NOALLOC_MAX_TRANSFERS + 1];
+};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[sizeof(NoAllocScratch)];
+
+namespace {
+NoAllocScratch* g_scratch[64];  // per device: that device's copy of `scratch`
+std::mutex g_noalloc_mu;
+// host staging of the list (static: the call allocates no host memory either)
+Transfer h_items[EXA_AMR_NOALLOC_MAX_TRANSFERS];
+int h_groups[EXA_AMR_NOALLOC_MAX_TRANSFERS + 1];
+long long h_outs[EXA_AMR_NOALLOC_MAX_TRANSFERS];
+}  // namespace
+
+// Called by exa_init once per device, with that device current.
+int exa_amr_noalloc_setup(int dev) {
+  if (dev < 0 || dev >= 64) {
+    exa_set_error("device index %d", dev);
+    return EXA_ERR_ARG;
+  }
+  NoAllocScratch* ptr = nullptr;
+  EXA_CHECK(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+  AmrCache& c = g_amr[dev];
+  std::lock_guard<std::mutex> lk(c.mu);
+  EXA_TRY(amr_lds_attrs(c));
+  std::lock_guard<std::mutex> lk2(g_noalloc_mu);
+  g_scratch[dev] = ptr;
+  return EXA_OK;
+}
+
+extern "C" int exa_amr_restriction_batched_noalloc(int dim, int order, int ndata, double* d_coarse,
+                                                   long long n_coarse, const double* d_fine, long long n_fine,
+                                                   const exa_amr_transfer* transfers, int count) {
+  EXA_ARG(dim == 2 || dim == 3, "dim must be 2 (faces) or 3 (volumes)");
+  EXA_ARG(order >= 0 && order <= EXA_MAX_ORDER, "order must be in 0..9");
+  EXA_ARG(ndata >= 1 && ndata <= 64, "number of data per node must be in 1..64");
+  EXA_ARG(count >= 0 && (count == 0 || (transfers && d_fine && d_coarse)), "arguments");
+  if (count > EXA_AMR_NOALLOC_MAX_TRANSFERS) {
+    exa_set_error("invalid argument: %d transfers exceed the %d of the allocation-free staging area", count,
+                  EXA_AMR_NOALLOC_MAX_TRANSFERS);
+    return EXA_ERR_ARG;
+  }
+  if (count == 0) return EXA_OK;
+  int dev = 0;
+  EXA_CHECK(hipGetDevice(&dev));
+  EXA_ARG(dev >= 0 && dev < 64, "device index");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  NoAllocScratch* sp = g_scratch[dev];
+  EXA_ARG(sp, "exa_init has not run on the current device (it sets up the staging area)");
+  const int B = order + 1;
+  const long long blk = (long long)(dim == 3 ? B * B * B : B * B) * ndata;
+  int ng = 0;
+  EXA_TRY(build_list(dim, false, blk, n_fine, n_coarse, transfers, count, h_items, h_groups, h_outs, &ng));
+  AmrTables t{};
+  fill_tables(order, t);
+  EXA_CHECK(hipMemcpy(&sp->tab, &t, sizeof(t), hipMemcpyHostToDevice));
+  EXA_CHECK(hipMemcpy(sp->items, h_items, sizeof(Transfer) * count, hipMemcpyHostToDevice));
+  EXA_CHECK(hipMemcpy(sp->groups, h_groups, sizeof(int) * (ng + 1), hipMemcpyHostToDevice));
+  EXA_TRY(launch(dim, false, &sp->tab, sp->items, sp->groups, ng, const_cast<double*>(d_fine), d_coarse, B, ndata));
+  EXA_CHECK(hipDeviceSynchronize());
   return EXA_OK;
 }

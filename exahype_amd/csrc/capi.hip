@@ -164,6 +164,7 @@ extern "C" int exa_init(int device) {
   EXA_CHECK(exa_pred_init_7(host, hostf));
   EXA_CHECK(exa_pred_staged_init(host, hostf));
   EXA_CHECK(exa_pde_init(host));
+  EXA_TRY(exa_amr_noalloc_setup(device));
   EXA_CHECK(set_lds_attrs<1>());
   EXA_CHECK(set_lds_attrs<2>());
   EXA_CHECK(set_lds_attrs<3>());

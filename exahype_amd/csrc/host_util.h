@@ -80,3 +80,7 @@ static inline int exa_check_dt(double dt, const char* what) {
                 dt);
   return EXA_ERR_NUMERICS;
 }
+
+// Takes the device address of the allocation-free AMR staging area on the
+// current device (amr_capi.hip); called by exa_init once per device.
+int exa_amr_noalloc_setup(int dev);

@@ -508,6 +508,20 @@ int exa_amr_prolongation_batched(int dim, int order, int ndata, double* d_fine, 
 int exa_amr_restriction_batched(int dim, int order, int ndata, double* d_coarse, long long n_coarse,
                                 const double* d_fine, long long n_fine, const exa_amr_transfer* transfers,
                                 int count);
+/* exa_amr_restriction_batched without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, checks and results as
+ * exa_amr_restriction_batched; the tables and the transfer list are staged in
+ * a fixed device array set up by exa_init (call exa_init on the device
+ * first), so at most EXA_AMR_NOALLOC_MAX_TRANSFERS transfers per call (more:
+ * EXA_ERR_ARG).  Calls are serialised within the process; the staging area
+ * keeps the last call's contents, and calls from two processes on one GPU at
+ * once would race on it.  Synchronous; records no kernel time. */
+#define EXA_AMR_NOALLOC_MAX_TRANSFERS 16384
+int exa_amr_restriction_batched_noalloc(int dim, int order, int ndata, double* d_coarse, long long n_coarse,
+                                        const double* d_fine, long long n_fine, const exa_amr_transfer* transfers,
+                                        int count);
 /* Duration (HIP events) of the transfer kernel of the last batched call on
  * the current device.  No counterpart in the reference (measurement). */
 int exa_amr_last_kernel_ms(double* ms);
