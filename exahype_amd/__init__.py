@@ -89,6 +89,10 @@ EXPORTS = [
     "exa_amr_volume_unknowns_prolongation", "exa_amr_volume_unknowns_restriction",
     "exa_amr_prolongation_batched", "exa_amr_restriction_batched", "exa_amr_last_kernel_ms",
     "exa_amr_restriction_batched_noalloc",
+    "exa_amr_mesh_cells", "exa_amr_grid_create", "exa_amr_grid_destroy", "exa_amr_grid_num_cells", "exa_amr_grid_cells",
+    "exa_amr_grid_upload", "exa_amr_grid_download", "exa_amr_grid_set_initial_conditions", "exa_amr_grid_compute_dt",
+    "exa_amr_grid_use_graph", "exa_amr_grid_enable_timing", "exa_amr_grid_step", "exa_amr_grid_sync",
+    "exa_amr_grid_limiter_status", "exa_amr_grid_patches", "exa_amr_grid_counters", "exa_amr_grid_phase_times",
     "exa_device_malloc", "exa_device_free", "exa_memcpy_htod", "exa_memcpy_dtoh", "exa_device_copy",
     "exa_selftest_mfma",
     "exa_ensemble_create", "exa_ensemble_destroy", "exa_ensemble_set_initial_conditions", "exa_ensemble_compute_dt",
@@ -188,6 +192,23 @@ def load_library(path=LIB_PATH):
         getattr(L, nm).argtypes = [I, I, I, _D, _D, I, I, P(I)]
     for nm in ("exa_amr_prolongation_batched", "exa_amr_restriction_batched", "exa_amr_restriction_batched_noalloc"):
         getattr(L, nm).argtypes = [I, I, I, C.c_void_p, LL, C.c_void_p, LL, P(AmrTransfer), I]
+    L.exa_amr_mesh_cells.argtypes = [C.POINTER(AMRConfig), P(I), LL, P(LL)]
+    L.exa_amr_grid_create.argtypes = [C.c_void_p, C.POINTER(AMRConfig), P(C.c_void_p)]
+    L.exa_amr_grid_destroy.argtypes = [C.c_void_p]
+    L.exa_amr_grid_num_cells.argtypes = [C.c_void_p, P(LL)]
+    L.exa_amr_grid_cells.argtypes = [C.c_void_p, P(I)]
+    L.exa_amr_grid_upload.argtypes = [C.c_void_p, _D]
+    L.exa_amr_grid_download.argtypes = [C.c_void_p, _D]
+    L.exa_amr_grid_set_initial_conditions.argtypes = [C.c_void_p, P(InitialCondition)]
+    L.exa_amr_grid_compute_dt.argtypes = [C.c_void_p, _D]
+    L.exa_amr_grid_use_graph.argtypes = [C.c_void_p, I]
+    L.exa_amr_grid_enable_timing.argtypes = [C.c_void_p, I]
+    L.exa_amr_grid_step.argtypes = [C.c_void_p, I]
+    L.exa_amr_grid_sync.argtypes = [C.c_void_p, _D, _D]
+    L.exa_amr_grid_limiter_status.argtypes = [C.c_void_p, P(I)]
+    L.exa_amr_grid_patches.argtypes = [C.c_void_p, _D, P(I), LL, P(LL)]
+    L.exa_amr_grid_counters.argtypes = [C.c_void_p, P(LL)]
+    L.exa_amr_grid_phase_times.argtypes = [C.c_void_p, _D, P(LL)]
     _lib = L
     return L
 
@@ -829,6 +850,132 @@ AMR_NOALLOC_MAX_TRANSFERS = 16384
 
 # ---------------------------------------------------------------------------
 # Ensembles of independent grids
+
+class AMRConfig(C.Structure):
+    _fields_ = [("base_cells", C.c_int), ("levels", C.c_int), ("centre", C.c_double * 3), ("radius", C.c_double),
+                ("width", C.c_double * 2), ("dmp_relaxation", C.c_double), ("dmp_scaling", C.c_double),
+                ("steps_till_cured", C.c_int), ("patch_capacity", C.c_longlong)]
+
+
+def amr_config(base_cells, levels, centre, radius, widths, relax=1e-4, scaling=1e-3, steps_till_cured=0,
+               patch_capacity=0):
+    c = AMRConfig()
+    c.base_cells, c.levels, c.radius = base_cells, levels, radius
+    for a in range(3):
+        c.centre[a] = centre[a]
+    for l in range(2):
+        c.width[l] = widths[l] if l < len(widths) else 0.0
+    c.dmp_relaxation, c.dmp_scaling, c.steps_till_cured = relax, scaling, steps_till_cured
+    c.patch_capacity = patch_capacity
+    return c
+
+
+def amr_mesh_cells(cfg):
+    """Leaves [n][4] = (level, ix, iy, iz) of the mesh a configuration builds (host only)."""
+    L = load_library()
+    n = C.c_longlong(0)
+    _check(L.exa_amr_mesh_cells(C.byref(cfg), None, 0, C.byref(n)))
+    out = np.zeros((max(n.value, 1), 4), np.int32)
+    _check(L.exa_amr_mesh_cells(C.byref(cfg), out.ctypes.data_as(C.POINTER(C.c_int)), n.value, C.byref(n)))
+    return out[:n.value]
+
+
+AMR_STATUS_NAMES = ("Ok", "NeighbourOfTroubled4", "NeighbourOfTroubled3", "NeighbourOfTroubled2",
+                    "NeighbourOfTroubled1", "Troubled")
+AMR_COUNTER_NAMES = ("irregular_steps", "refinement_requests", "coarse_fine_limiter", "mirrored_ghost_faces",
+                     "cannot_communicate", "patch_overflow", "troubled", "fv_updates")
+
+
+class AMRGrid:
+    """Limiting ADER-DG on a static adaptive mesh resident in HBM (leaves of
+    up to three levels, MUSCL-Hancock subcell limiter on the finest level)."""
+
+    def __init__(self, solver, cfg):
+        self.solver = solver
+        self.cfg = cfg
+        h = C.c_void_p()
+        _check(_lib.exa_amr_grid_create(solver._h, C.byref(cfg), C.byref(h)))
+        self._h = h
+        n = C.c_longlong(0)
+        _check(_lib.exa_amr_grid_num_cells(self._h, C.byref(n)))
+        self.ncells = n.value
+        self.cell_size = solver.B ** 3 * solver.nvar
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.exa_amr_grid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def cells(self):
+        out = np.zeros((self.ncells, 4), np.int32)
+        _check(_lib.exa_amr_grid_cells(self._h, out.ctypes.data_as(C.POINTER(C.c_int))))
+        return out
+
+    def upload(self, luh):
+        luh = np.ascontiguousarray(luh, dtype=np.float64).ravel()
+        assert luh.size == self.ncells * self.cell_size
+        _check(_lib.exa_amr_grid_upload(self._h, _p(luh)))
+
+    def download(self):
+        out = np.zeros(self.ncells * self.cell_size)
+        _check(_lib.exa_amr_grid_download(self._h, _p(out)))
+        return out
+
+    def set_initial_conditions(self, ic):
+        _check(_lib.exa_amr_grid_set_initial_conditions(self._h, C.byref(ic)))
+
+    def compute_dt(self):
+        dt = C.c_double(0.0)
+        _check(_lib.exa_amr_grid_compute_dt(self._h, C.byref(dt)))
+        return dt.value
+
+    def use_graph(self, enable):
+        _check(_lib.exa_amr_grid_use_graph(self._h, int(bool(enable))))
+
+    def enable_timing(self, enable):
+        _check(_lib.exa_amr_grid_enable_timing(self._h, int(bool(enable))))
+
+    def step(self, nsteps=1):
+        _check(_lib.exa_amr_grid_step(self._h, int(nsteps)))
+
+    def sync(self):
+        t, dt = C.c_double(0.0), C.c_double(0.0)
+        _check(_lib.exa_amr_grid_sync(self._h, C.byref(t), C.byref(dt)))
+        return t.value, dt.value
+
+    def limiter_status(self):
+        out = np.zeros(self.ncells, np.int32)
+        _check(_lib.exa_amr_grid_limiter_status(self._h, out.ctypes.data_as(C.POINTER(C.c_int))))
+        return out
+
+    def patches(self):
+        """(cells, interiors [k][z][y][x][5]) of the limiter patches."""
+        n = C.c_longlong(0)
+        _check(_lib.exa_amr_grid_patches(self._h, None, None, 0, C.byref(n)))
+        P = 2 * self.solver.order + 1
+        out = np.zeros(max(n.value, 1) * 5 * P ** 3)
+        cells = np.zeros(max(n.value, 1), np.int32)
+        _check(_lib.exa_amr_grid_patches(self._h, _p(out), cells.ctypes.data_as(C.POINTER(C.c_int)), n.value,
+                                         C.byref(n)))
+        return cells[:n.value], out[:n.value * 5 * P ** 3].reshape(n.value, 5 * P ** 3)
+
+    def counters(self):
+        c = (C.c_longlong * 8)()
+        _check(_lib.exa_amr_grid_counters(self._h, c))
+        return dict(zip(AMR_COUNTER_NAMES, list(c)))
+
+    def phase_times(self):
+        ms = np.zeros(6)
+        n = C.c_longlong(0)
+        _check(_lib.exa_amr_grid_phase_times(self._h, _p(ms), C.byref(n)))
+        return ms, n.value
+
 
 class Ensemble:
     """Independent periodic grids (order, cells per direction, gamma per

@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libexahype_amd.so")
 SOURCES = ["capi.hip"] + ["pred_n%d.hip" % n for n in range(1, 8)] + ["pred_staged.hip", "pde_tu.hip"] + [
-    "fv_capi.hip", "limiter_capi.hip", "amr_capi.hip", "device_capi.hip"]
+    "fv_capi.hip", "limiter_capi.hip", "amr_capi.hip", "device_capi.hip", "amr_grid.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

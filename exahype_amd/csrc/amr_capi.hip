@@ -83,7 +83,7 @@ int check_transfer(const exa_amr_transfer& t, int dim) {
 // Launches one group per workgroup on the null stream; items/groups already on
 // the device.
 int launch(int dim, bool prolong, const AmrTables* tab, const Transfer* items, const int* groups, int ngroups,
-           double* fine, double* coarse, int B, int nd) {
+           double* fine, double* coarse, int B, int nd, hipStream_t st = nullptr) {
   if (ngroups == 0) return EXA_OK;
   const dim3 grid(ngroups), block(exa::amr::THREADS);
   // LDS: two buffers of one value chunk (as many values per node as fit)
@@ -91,7 +91,7 @@ int launch(int dim, bool prolong, const AmrTables* tab, const Transfer* items, c
   const size_t lds = 2 * sizeof(double) * (size_t)n * exa::amr::amr_chunk(B, dim, nd);
   bool ok = false;
   auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, grid, block, lds, 0, tab, items, groups, fine, coarse, nd);
+    hipLaunchKernelGGL(kern, grid, block, lds, st, tab, items, groups, fine, coarse, nd);
     ok = true;
   };
   switch (B) {
@@ -289,6 +289,23 @@ extern "C" int exa_amr_last_kernel_ms(double* ms) {
   std::lock_guard<std::mutex> lk(c.mu);
   *ms = c.last_ms;
   return EXA_OK;
+}
+
+// Device tables of one order and a transfer launch on a stream, for units that
+// keep their own transfer lists on the device (the adaptive-mesh step).
+int exa_amr_tables_dev(int dev, int order, const void** tab) {
+  AmrCache& c = g_amr[dev];
+  std::lock_guard<std::mutex> lk(c.mu);
+  const AmrTables* t = nullptr;
+  EXA_TRY(amr_tables(dev, order, &t));
+  *tab = t;
+  return EXA_OK;
+}
+
+int exa_amr_enqueue(int dim, bool prolong, int order, const void* tab, const void* items, const int* groups,
+                    int ngroups, double* fine, double* coarse, int nd, hipStream_t st) {
+  return launch(dim, prolong, static_cast<const AmrTables*>(tab), static_cast<const Transfer*>(items), groups,
+                ngroups, fine, coarse, order + 1, nd, st);
 }
 
 // ---------------------------------------------------------------------------

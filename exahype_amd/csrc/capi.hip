@@ -14,6 +14,7 @@
 #include "predictor_tu.h"
 #include "grid_view.h"
 #include "host_util.h"
+#include "solver_internal.h"
 #include "dg_tables_data.h"
 
 
@@ -165,6 +166,7 @@ extern "C" int exa_init(int device) {
   EXA_CHECK(exa_pred_staged_init(host, hostf));
   EXA_CHECK(exa_pde_init(host));
   EXA_TRY(exa_amr_noalloc_setup(device));
+  EXA_CHECK(exa_amr_grid_init(host));
   EXA_CHECK(set_lds_attrs<1>());
   EXA_CHECK(set_lds_attrs<2>());
   EXA_CHECK(set_lds_attrs<3>());
@@ -1216,6 +1218,25 @@ static int predictor_waves_per_simd(const exa_aderdg_config& c, int N, int dev) 
   // measured at 81^3 cells: two waves per SIMD is 1.6x faster at N=5
   return N <= 5 ? 2 : 1;
 }
+
+// The register-form predictor a solver's grids launch (Euler, orders 1..7,
+// the configured precision, occupancy and contraction form), for other units.
+int exa_solver_predictor(const exa_solver* s, PredFn* fn, int* threads, size_t* lds) {
+  const int N = s->cfg.order;
+  EXA_ARG(s->cfg.pde == EXA_PDE_EULER && !s->cfg.linear, "the Euler PDE (nonlinear kernels)");
+  EXA_ARG(N >= 1 && N <= 7, "orders 1..7 (the register-form predictor)");
+  EXA_ORDER_SWITCH(N, {
+    using O = exa::Ord<NN>;
+    const int wpe = predictor_waves_per_simd(s->cfg, NN, s->device);
+    const int form = predictor_form(s->cfg, NN, s->device);
+    *fn = predictor_kernel<NN>(false, s->cfg.precision, wpe, form);
+    *threads = O::NT;
+    *lds = O::LDS_DOUBLES * sizeof(double);
+  });
+  return *fn ? EXA_OK : EXA_ERR_UNSUPPORTED;
+}
+const exa_aderdg_config* exa_solver_cfg(const exa_solver* s) { return &s->cfg; }
+int exa_solver_device(const exa_solver* s) { return s->device; }
 
 // Extras of a grid's launches: a point source is evaluated per node from the
 // grid's geometry and the device-held time of the step.

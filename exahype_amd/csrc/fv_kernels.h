@@ -183,18 +183,14 @@ __device__ inline void muscl_face_state(const double* u, int nbox, const PatchGe
 
 // PC: subcells per direction (compile time: the index arithmetic of the
 // recomputed face states is then constant; the launch's geometry must match).
+// One patch per workgroup: u (with its ghost layers) -> un (interior and the
+// first ghost layer's face updates); sQt: V P^3 doubles of LDS.  Returns the
+// thread's part of the admissible time step.
 template <class PDE, int PC>
-__global__ __launch_bounds__(256) void k_muscl(const double* __restrict__ luh, double* __restrict__ luh_new,
-                                               double* __restrict__ dt_patch, unsigned long long* dt_min,
-                                               PatchGeom, double dx0, double dx1, double dx2,
-                                               const double* __restrict__ dtp, double gamma, double cfl) {
+__device__ inline double muscl_patch(const double* __restrict__ u, double* __restrict__ un, double* sQt,
+                                     double dx0, double dx1, double dx2, double dt, double gamma, double cfl) {
   const PatchGeom g{PC, 2, PC + 4, 3};
-  extern __shared__ double sQt[];  // [v][interior subcell]
-  __shared__ double red[8];
-  const double dt = *dtp;
   const int nbox = g.ncells_box();
-  const double* u = luh + (size_t)blockIdx.x * V * nbox;
-  double* un = luh_new + (size_t)blockIdx.x * V * nbox;
   const double h[3] = {dx0 / g.P, dx1 / g.P, dx2 / g.P};
   const int ninterior = g.P * g.P * g.P;
   // phase 1: half-step evolution term of every interior subcell
@@ -273,6 +269,19 @@ __global__ __launch_bounds__(256) void k_muscl(const double* __restrict__ luh, d
       un[v * nbox + idx] = side ? o + fac * f[v] : o - fac * f[v];
     }
   }
+  return dtm;
+}
+
+template <class PDE, int PC>
+__global__ __launch_bounds__(256) void k_muscl(const double* __restrict__ luh, double* __restrict__ luh_new,
+                                               double* __restrict__ dt_patch, unsigned long long* dt_min,
+                                               PatchGeom, double dx0, double dx1, double dx2,
+                                               const double* __restrict__ dtp, double gamma, double cfl) {
+  extern __shared__ double sQt[];  // [v][interior subcell]
+  __shared__ double red[8];
+  constexpr int nbox = (PC + 4) * (PC + 4) * (PC + 4);
+  double dtm = muscl_patch<PDE, PC>(luh + (size_t)blockIdx.x * V * nbox, luh_new + (size_t)blockIdx.x * V * nbox,
+                                    sQt, dx0, dx1, dx2, *dtp, gamma, cfl);
   dtm = block_min(dtm, red);
   if (threadIdx.x == 0) {
     if (dt_patch) dt_patch[blockIdx.x] = dtm;

@@ -62,6 +62,13 @@ static int lim_tables(int device, int N, const LimTables** out) {
   return EXA_OK;
 }
 
+int exa_lim_tables_dev(int device, int N, const void** out) {
+  const LimTables* t = nullptr;
+  EXA_TRY(lim_tables(device, N, &t));
+  *out = t;
+  return EXA_OK;
+}
+
 // one variable's contraction chain (the kernels loop over the variables)
 static size_t proj_lds_bytes(int B) {
   const int L = 2 * B - 1;

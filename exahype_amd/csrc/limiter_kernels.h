@@ -145,24 +145,23 @@ __device__ inline void contract_axis(const double* in, double* out, int n0, int 
 
 // luh [cell][v][B^3] -> lim [cell][v][S^3] interior (S = L + 2 ghost);
 // projectOnFVLimiterSpace, kernels/limiter/generic/c/Limiter.cpp:23-66.
-__global__ void k_project_on_fv(const double* __restrict__ luh, double* __restrict__ lim, const LimTables* tab,
-                                int B, int ghost, const int* __restrict__ cells) {
-  extern __shared__ double sm[];
+// One cell per workgroup: cell block u ([v][B^3]) -> patch block p ([v][S^3]).
+__device__ inline void project_on_fv_block(const double* __restrict__ cellu, double* __restrict__ patch,
+                                           const LimTables* tab, int B, int ghost, double* sm) {
   const int L = 2 * B - 1, S = L + 2 * ghost;
-  const int cell = cells ? cells[blockIdx.x] : blockIdx.x;
   double* a = sm;              // [B][B][L]
   double* b = sm + B * B * L;  // [B][L][L]
   double* c = b + B * L * L;   // [L][L][L]
   auto m = [&](int k, int o) { return tab->u2l[k][o]; };
   for (int v = 0; v < V; ++v) {
-    const double* u = luh + ((size_t)cell * V + v) * B * B * B;
+    const double* u = cellu + (size_t)v * B * B * B;
     contract_axis(u, a, B, B, B, 0, B, L, m);
     __syncthreads();
     contract_axis(a, b, L, B, B, 1, B, L, m);
     __syncthreads();
     contract_axis(b, c, L, L, B, 2, B, L, m);
     __syncthreads();
-    double* o = lim + ((size_t)blockIdx.x * V + v) * S * S * S;
+    double* o = patch + (size_t)v * S * S * S;
     for (int e = threadIdx.x; e < L * L * L; e += blockDim.x) {
       const int x = e % L, y = (e / L) % L, z = e / (L * L);
       o[((z + ghost) * S + y + ghost) * S + x + ghost] = c[e];
@@ -171,18 +170,25 @@ __global__ void k_project_on_fv(const double* __restrict__ luh, double* __restri
   }
 }
 
-// lim interior -> luh; projectOnDGSpace, kernels/limiter/generic/c/Limiter.cpp:69-112.
-__global__ void k_project_on_dg(const double* __restrict__ lim, double* __restrict__ luh, const LimTables* tab,
+static __global__ void k_project_on_fv(const double* __restrict__ luh, double* __restrict__ lim, const LimTables* tab,
                                 int B, int ghost, const int* __restrict__ cells) {
   extern __shared__ double sm[];
   const int L = 2 * B - 1, S = L + 2 * ghost;
   const int cell = cells ? cells[blockIdx.x] : blockIdx.x;
+  project_on_fv_block(luh + (size_t)cell * V * B * B * B, lim + (size_t)blockIdx.x * V * S * S * S, tab, B, ghost,
+                      sm);
+}
+
+// lim interior -> luh; projectOnDGSpace, kernels/limiter/generic/c/Limiter.cpp:69-112.
+__device__ inline void project_on_dg_block(const double* __restrict__ patch, double* __restrict__ cellu,
+                                           const LimTables* tab, int B, int ghost, double* sm) {
+  const int L = 2 * B - 1, S = L + 2 * ghost;
   double* c = sm;              // [L][L][L]
   double* a = sm + L * L * L;  // [L][L][B]
   double* b = a + L * L * B;   // [L][B][B]
   auto m = [&](int k, int o) { return tab->l2u[k][o]; };
   for (int v = 0; v < V; ++v) {
-    const double* in = lim + ((size_t)blockIdx.x * V + v) * S * S * S;
+    const double* in = patch + (size_t)v * S * S * S;
     for (int e = threadIdx.x; e < L * L * L; e += blockDim.x) {
       const int x = e % L, y = (e / L) % L, z = e / (L * L);
       c[e] = in[((z + ghost) * S + y + ghost) * S + x + ghost];
@@ -192,16 +198,25 @@ __global__ void k_project_on_dg(const double* __restrict__ lim, double* __restri
     __syncthreads();
     contract_axis(a, b, B, L, L, 1, L, B, m);
     __syncthreads();
-    contract_axis(b, luh + ((size_t)cell * V + v) * B * B * B, B, B, L, 2, L, B, m);
+    contract_axis(b, cellu + (size_t)v * B * B * B, B, B, L, 2, L, B, m);
     __syncthreads();
   }
+}
+
+static __global__ void k_project_on_dg(const double* __restrict__ lim, double* __restrict__ luh, const LimTables* tab,
+                                int B, int ghost, const int* __restrict__ cells) {
+  extern __shared__ double sm[];
+  const int L = 2 * B - 1, S = L + 2 * ghost;
+  const int cell = cells ? cells[blockIdx.x] : blockIdx.x;
+  project_on_dg_block(lim + (size_t)blockIdx.x * V * S * S * S, luh + (size_t)cell * V * B * B * B, tab, B, ghost,
+                      sm);
 }
 
 // Cell min/max of the observables (the first nobs variables) over the GL
 // nodes, the Gauss-Lobatto nodes and the subcell averages
 // (findCellLocalMinAndMax, kernels/limiter/generic/c/Limiter.cpp:210-368).
 // out: [cell][2][nobs] (min then max).
-__global__ void k_cell_min_max(const double* __restrict__ luh, double* __restrict__ out, const LimTables* tab, int B,
+static __global__ void k_cell_min_max(const double* __restrict__ luh, double* __restrict__ out, const LimTables* tab, int B,
                                int nobs, int ncells) {
   extern __shared__ double sm[];
   const int L = 2 * B - 1, B3 = B * B * B;
@@ -332,7 +347,7 @@ __global__ void __launch_bounds__(256) k_cell_min_max_t(const double* __restrict
 // cell are the minima/maxima of its neighbours' cell values merged over the
 // faces, E/solvers/LimitingADERDGSolver.cpp:993-1026).  ok[cell] = 1 if
 // admissible.  minmax: [cell][2][nobs]; nbr: [6][ncells] neighbour ids.
-__global__ void k_dmp(const double* __restrict__ minmax, const double* __restrict__ bminmax,
+static __global__ void k_dmp(const double* __restrict__ minmax, const double* __restrict__ bminmax,
                       int* __restrict__ ok, int nobs, int ncells, double relax, double scaling) {
   const int cell = blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= ncells) return;

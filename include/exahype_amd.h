@@ -526,6 +526,81 @@ int exa_amr_restriction_batched_noalloc(int dim, int order, int ndata, double* d
  * the current device.  No counterpart in the reference (measurement). */
 int exa_amr_last_kernel_ms(double* ms);
 
+/* ---------------------------------------------------------------------------
+ * Limiting ADER-DG on a static adaptive mesh: the step of
+ * exahype::solvers::LimitingADERDGSolver (E/solvers/LimitingADERDGSolver.cpp:
+ * 721-1490) over all leaves of a spacetree of up to three levels, the
+ * limiter (MUSCL-Hancock on 2N+1 subcells per direction) on the finest
+ * level, one hipGraph per step.  It replaces the engine's traversals of a
+ * non-fused step (E/runners/Runner.cpp:1123-1140) with the limiter's local
+ * recomputation (Runner.cpp:886-955):
+ *   prediction on every leaf; face data of virtual cells by face
+ *   prolongation / restriction (E/solvers/ADERDGSolver.cpp:2240-2470);
+ *   merging of limiter status, DMP extrema and Riemann problems per the
+ *   limiter status (LimitingADERDGSolver.cpp:1641-1850); the update switch,
+ *   DMP + admissibility and the new status (:721-1048); on a newly troubled
+ *   cell, status spreading (five rounds), rollback and FV recomputation
+ *   (:369-379, :1283-1441); the global ADER-DG time step.
+ * The mesh is built once: base_cells^3 cells over the unit cube (periodic),
+ * a cell of level l < levels-1 refined by 3 where its box meets the shell
+ * radius - width[l] <= |x - centre| <= radius + width[l], 2:1 balanced.  It
+ * never changes: a troubled cell below the finest level and a limiter
+ * status above Ok next to a coarser cell are counted (the reference would
+ * refine there) and those cells keep their ADER-DG update.  Euler, orders
+ * 1..7, the solver's predictor precision; FP64 update and time step. */
+typedef struct exa_amr_grid exa_amr_grid;
+typedef struct {
+  int base_cells;            /* cells per direction of level 0                       */
+  int levels;                /* 1..3                                                 */
+  double centre[3];          /* refinement shell                                     */
+  double radius;
+  double width[2];           /* half width of the shell refining level 0 and level 1 */
+  double dmp_relaxation;     /* the spec's dmp-relaxation-parameter (Parser.cpp:780) */
+  double dmp_scaling;        /* dmp-difference-scaling (Parser.cpp:797)              */
+  int steps_till_cured;      /* steps-till-cured (Parser.cpp:831)                    */
+  long long patch_capacity;  /* limiter patches kept on the device; 0: a quarter of
+                                the finest-level cells (at least 64)                 */
+} exa_amr_config;
+/* counters of exa_amr_grid_counters, summed over the steps */
+#define EXA_AMR_CNT_IRREGULAR_STEPS 0  /* steps with a newly troubled cell (recomputed)        */
+#define EXA_AMR_CNT_REFINE_REQUESTS 1  /* troubled cells below the finest level                */
+#define EXA_AMR_CNT_COARSE_FINE 2      /* finest cells in the limiter domain next to a coarser cell */
+#define EXA_AMR_CNT_MIRRORED 3         /* FV ghost faces filled from the patch itself          */
+#define EXA_AMR_CNT_CANNOT_COMMUNICATE 4 /* face pairs on which the reference aborts          */
+#define EXA_AMR_CNT_PATCH_OVERFLOW 5   /* cells that found no free limiter patch (an error)    */
+#define EXA_AMR_CNT_TROUBLED 6         /* troubled finest cells                                */
+#define EXA_AMR_CNT_FV_UPDATES 7       /* FV updates of finest cells                           */
+/* The leaves of the mesh a configuration builds, [n][4] = level, ix, iy, iz
+ * in storage order (host only: no device needed); cells may be NULL. */
+int exa_amr_mesh_cells(const exa_amr_config* cfg, int* cells, long long capacity, long long* n);
+int exa_amr_grid_create(exa_solver* s, const exa_amr_config* cfg, exa_amr_grid** out);
+int exa_amr_grid_destroy(exa_amr_grid* g);
+int exa_amr_grid_num_cells(const exa_amr_grid* g, long long* n);
+/* leaves in storage order: [n][4] = level, ix, iy, iz (index at that level) */
+int exa_amr_grid_cells(const exa_amr_grid* g, int* cells);
+/* solution of every leaf, [leaf][z][y][x][5]; setting it (or the initial
+ * conditions) starts the limiter as setInitialConditions does
+ * (LimitingADERDGSolver.cpp:692-715, :906-930): admissibility only, then
+ * spreading; the limiter patches then hold the FV initial conditions at the
+ * subcell centres (exa_amr_grid_set_initial_conditions) or the projection
+ * of the uploaded solution (exa_amr_grid_upload) */
+int exa_amr_grid_upload(exa_amr_grid* g, const double* luh);
+int exa_amr_grid_download(exa_amr_grid* g, double* luh);
+int exa_amr_grid_set_initial_conditions(exa_amr_grid* g, const exa_ic* ic);
+int exa_amr_grid_compute_dt(exa_amr_grid* g, double* dt);
+int exa_amr_grid_use_graph(exa_amr_grid* g, int enable);
+int exa_amr_grid_enable_timing(exa_amr_grid* g, int enable);
+int exa_amr_grid_step(exa_amr_grid* g, int nsteps);
+int exa_amr_grid_sync(exa_amr_grid* g, double* time, double* next_dt);
+int exa_amr_grid_limiter_status(exa_amr_grid* g, int* status);
+/* the interior subcells of the limiter patches, [k][z][y][x][5] (2N+1 per
+ * direction) of the k-th cell listed in cells (up to capacity patches) */
+int exa_amr_grid_patches(exa_amr_grid* g, double* patches, int* cells, long long capacity, long long* n);
+int exa_amr_grid_counters(exa_amr_grid* g, long long counts[8]);
+/* per step (timing on, graph off): predictor, face transfers + merges,
+ * Riemann, ADER-DG update, limiter (FV, status, recomputation), time step (ms) */
+int exa_amr_grid_phase_times(exa_amr_grid* g, double ms[6], long long* steps);
+
 #ifdef __cplusplus
 }
 #endif

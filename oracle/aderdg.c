@@ -533,7 +533,7 @@ double orc_stable_time_step_size(const orc_cfg* c, const double* luh, const doub
           double maxEigenvalue = 0.0;
           for (int ivar = 0; ivar < nv; ivar++) {
             const double a = fabs(lambda[ivar]);
-            maxEigenvalue = (maxEigenvalue < a) ? a : maxEigenvalue; /* std::max(a, m) */
+            maxEigenvalue = (a < maxEigenvalue) ? maxEigenvalue : a; /* std::max(a, m): keeps a NaN */
           }
           denominator += maxEigenvalue / dx[d];
         }

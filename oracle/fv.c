@@ -392,6 +392,25 @@ void orc_fv_grid_init(const orc_cfg* c, orc_fv_grid* g, const orc_ic* ic) {
       }
 }
 
+/* solutionAdjustment of one 3D patch at its subcell centres
+ * (kernels/finitevolumes/commons/c/commons.cpph:466-500): patch of P^3
+ * subcells with ghost width G, cell centre and size */
+void orc_fv_patch_ic(const orc_cfg* c, const orc_ic* ic, int P, int G, const double centre[3], const double dx[3],
+                     double* luh) {
+  const int nd = c->nvar + c->npar, S = P + 2 * G;
+  for (int k = G; k < G + P; k++) {
+    const double qk = (2.0 * (k - G) + 1.0) / (2.0 * P);
+    for (int j = G; j < G + P; j++) {
+      const double qj = (2.0 * (j - G) + 1.0) / (2.0 * P);
+      for (int i = G; i < G + P; i++) {
+        const double qi = (2.0 * (i - G) + 1.0) / (2.0 * P);
+        double x[3] = {centre[0] + dx[0] * (qi - 0.5), centre[1] + dx[1] * (qj - 0.5), centre[2] + dx[2] * (qk - 0.5)};
+        orc_ic_point(c, ic, x, 0.0, luh + (((size_t)k * S + j) * S + i) * nd);
+      }
+    }
+  }
+}
+
 double orc_fv_grid_stable_dt(const orc_cfg* c, const orc_fv_grid* g, int nthreads) {
   const int nd = c->nvar + c->npar, S = g->patch + 2 * g->ghost;
   const size_t psz = (size_t)(g->dim == 3 ? S * S * S : S * S) * nd;

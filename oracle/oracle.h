@@ -160,6 +160,8 @@ typedef struct {
   double* luh;      /* [patch][(z)][y][x][nData] including ghosts */
   double* luh_new;
 } orc_fv_grid;
+void orc_fv_patch_ic(const orc_cfg* c, const orc_ic* ic, int P, int G, const double centre[3], const double dx[3],
+                     double* luh);
 int orc_fv_grid_alloc(const orc_cfg* c, orc_fv_grid* g, int dim, int scheme, int patch, int nx, int ny, int nz);
 void orc_fv_grid_free(orc_fv_grid* g);
 void orc_fv_grid_init(const orc_cfg* c, orc_fv_grid* g, const orc_ic* ic);

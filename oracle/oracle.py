@@ -102,6 +102,7 @@ def lib():
         L.orc_godunov_update.argtypes = [P(Cfg), C.c_int, C.c_int, _D, _D, _D, C.c_double, C.c_double]
         L.orc_godunov_update.restype = C.c_double
         L.orc_musclhancock_update.argtypes = [P(Cfg), C.c_int, _D, _D, _D, C.c_double, C.c_double]
+        L.orc_fv_patch_ic.argtypes = [P(Cfg), P(IC), C.c_int, C.c_int, _D, _D, _D]
         L.orc_musclhancock_update.restype = C.c_double
         L.orc_fv_grid_alloc.argtypes = [P(Cfg), P(FVGridS), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_fv_grid_alloc.restype = C.c_int
@@ -375,6 +376,14 @@ def godunov_update(c, dim, patch, luh_new, luh, dx, dt, cfl):
     luh_new = dvec(luh_new).copy()
     a = lib().orc_godunov_update(C.byref(c), dim, patch, ptr(luh_new), ptr(dvec(luh)), ptr(dvec(dx)), dt, cfl)
     return luh_new, a
+
+
+def fv_patch_ic(c, ic, patch, ghost, centre, dx):
+    """Initial conditions of one 3D FV patch at its subcell centres (ghosts zero)."""
+    S = patch + 2 * ghost
+    out = np.zeros(S ** 3 * (c.nvar + c.npar))
+    lib().orc_fv_patch_ic(C.byref(c), C.byref(ic), patch, ghost, ptr(dvec(centre)), ptr(dvec(dx)), ptr(out))
+    return out
 
 
 def musclhancock_update(c, patch, luh_new, luh, dx, dt, cfl=0.99):

@@ -1,0 +1,166 @@
+"""Limiting ADER-DG on a static adaptive mesh (exa_amr_grid_*) against the
+oracle's restatement of the reference's step (oracle/amr_limiting.py):
+the same leaves, the same limiter statuses, patches and counters, and the
+solution and time step to 1e-12 after several steps -- on a single level
+(where the oracle's step is also the plain grid step), across two and three
+levels with smooth data (hanging faces, no limiter activity) and with a Sod
+sphere inside the finest level (troubled cells, status spreading, rollback
+and FV recomputation)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from oracle import amr_limiting as A
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def rel(a, b):
+    a, b = np.asarray(a).reshape(-1, 5), np.asarray(b).reshape(-1, 5)
+    scale = np.abs(b).max(axis=0)
+    return (np.abs(a - b).max(axis=0) / np.maximum(scale, 1e-3 * scale.max())).max()
+
+
+def _vortex_ic():
+    rng = np.random.default_rng(5)
+    return dict(kind=0, xc=(0.5, 0.5, 0.5), L=0.1, eps=5.0, uinf=(1.0, 1.0, 0.0), rho_inf=1.0, p_inf=1.0,
+                amp=1e-2, nmodes=8, k=rng.integers(1, 3, size=(8, 3)).astype(float), a=rng.uniform(-1, 1, 8),
+                phi=rng.uniform(0, 6.28, 8))
+
+
+def _sod_ic(r0, xc=(0.5, 0.5, 0.5)):
+    return dict(kind=1, xc=xc, r0=r0, rho_in=1.0, p_in=1.0, rho_out=0.125, p_out=0.1)
+
+
+NO_LIMITER = dict(relax=1e3)  # a DMP relaxation that no state violates
+
+
+def _pair(N, mesh_args, ic, upload=False, **kw):
+    """The oracle and the device grid from the same initial conditions: the
+    device evaluates them (DG nodes and, where a limiter patch exists from
+    the start, FV subcell centres), or the oracle's DG solution is uploaded
+    (patches are then its projections)."""
+    import exahype_amd as E
+    m = A.Mesh(*mesh_args)
+    og = A.LimitingGrid(m, N, **kw)
+    og.init(O.make_ic(**ic))
+    s = E.ADERDGSolver(N)
+    cfg = E.amr_config(*mesh_args, relax=og.relax, scaling=og.scaling, steps_till_cured=og.steps)
+    g = E.AMRGrid(s, cfg)
+    cells = g.cells()
+    assert np.array_equal(cells, np.array([(l,) + c for (l, c) in m.leaves], np.int32))
+    if upload:
+        og.init_from_luh(og.luh.copy())
+        g.upload(og.luh.ravel())
+    else:
+        g.set_initial_conditions(E.make_ic(**ic))
+        assert rel(g.download(), og.luh) <= 1e-14
+    assert np.array_equal(g.limiter_status(), og.status)
+    return g, og
+
+
+def _run(g, og, steps, graph):
+    g.use_graph(graph)
+    dt_o = og.stable_dt()
+    dt_g = g.compute_dt()
+    assert abs(dt_g - dt_o) <= 1e-14 * dt_o
+    for k in range(steps):
+        print("step", k, flush=True)
+        g.step(1)
+        dt_o = og.step(dt_o)
+        t, dt_g = g.sync()
+        assert np.array_equal(g.limiter_status(), og.status), k
+        assert abs(dt_g - dt_o) <= TOL * dt_o, (k, dt_g, dt_o)
+        assert rel(g.download(), og.luh) <= TOL, (k, rel(g.download(), og.luh))
+    cells, patches = g.patches()
+    assert sorted(cells.tolist()) == sorted(og.patch.keys())
+    S, P, G = og.S, og.P, A.GHOST
+    for i, q in enumerate(cells):
+        ref = og.patch[int(q)].reshape(S, S, S, 5)[G:G + P, G:G + P, G:G + P].ravel()
+        assert rel(patches[i], ref) <= TOL, q
+    c = g.counters()
+    for k in ("irregular_steps", "refinement_requests", "coarse_fine_limiter", "mirrored_ghost_faces",
+              "cannot_communicate"):
+        assert c[k] == og.counts[k], (k, c[k], og.counts[k])
+    assert c["patch_overflow"] == 0
+    return c
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_single_level_no_limiter(graph):
+    print("amr single level, graph", graph, flush=True)
+    g, og = _pair(3, (4, 1, (0.5, 0.5, 0.5), 0.0, []), _vortex_ic(), **NO_LIMITER)
+    c = _run(g, og, 3, graph)
+    assert c["fv_updates"] == 0
+
+
+def test_single_level_equals_plain_grid_step():
+    """On one level without troubled cells the adaptive-mesh step is the
+    plain grid step: the GPU adaptive mesh against the GPU grid."""
+    import exahype_amd as E
+    N, n = 3, 4
+    g, og = _pair(N, (n, 1, (0.5, 0.5, 0.5), 0.0, []), _vortex_ic(), **NO_LIMITER)
+    s = E.ADERDGSolver(N)
+    grid = E.Grid(s, n, n, n)
+    grid.upload(og.luh.ravel())
+    assert abs(grid.compute_dt() - g.compute_dt()) <= 1e-15 * grid.compute_dt()
+    g.step(2)
+    grid.step(2)
+    grid.sync()
+    g.sync()
+    assert rel(g.download(), grid.download()) <= TOL
+
+
+@pytest.mark.parametrize("levels,widths,N", [(2, [0.01], 3), (3, [0.01, 0.01], 3), (3, [0.01, 0.01], 5)])
+def test_hanging_faces_no_limiter(levels, widths, N):
+    print("amr smooth levels", levels, "N", N, flush=True)
+    g, og = _pair(N, (3, levels, (0.5, 0.5, 0.5), 0.0, widths), _vortex_ic(), **NO_LIMITER)
+    c = _run(g, og, 3, True)
+    assert og.counts["irregular_steps"] == 0 and c["fv_updates"] == 0
+
+
+@pytest.mark.parametrize("n0,levels,widths", [(4, 1, []), (3, 2, [0.01])])
+def test_dmp_troubles_smooth_data(n0, levels, widths):
+    """The vortex under-resolved on these meshes violates the DMP: cells become
+    troubled during the steps, so the irregular path (status spreading,
+    rollback, FV recomputation) runs."""
+    print("amr dmp levels", levels, flush=True)
+    g, og = _pair(3, (n0, levels, (0.5, 0.5, 0.5), 0.0, widths), _vortex_ic())
+    c = _run(g, og, 4, True)
+    assert c["irregular_steps"] > 0 and c["fv_updates"] > 0
+
+
+@pytest.mark.parametrize("levels,widths,r0,N", [(1, [], 0.3, 3), (2, [0.01], 0.06, 3), (3, [0.01, 0.01], 0.02, 3),
+                                                (2, [0.01], 0.06, 5)])
+def test_sod_sphere_limiter(levels, widths, r0, N):
+    print("amr sod levels", levels, "N", N, flush=True)
+    n0 = 5 if levels == 1 else 3
+    g, og = _pair(N, (n0, levels, (0.5, 0.5, 0.5), 0.0, widths), _sod_ic(r0))
+    c = _run(g, og, 4, True)
+    assert c["fv_updates"] > 0 and (og.status > 0).any()
+
+
+def test_uploaded_solution_projects_the_patches():
+    print("amr upload", flush=True)
+    g, og = _pair(3, (4, 1, (0.5, 0.5, 0.5), 0.0, []), _vortex_ic(), upload=True)
+    c = _run(g, og, 3, True)
+    assert c["irregular_steps"] > 0
+
+
+def test_constant_state_stays_constant_across_levels():
+    """A constant state is a fixed point of the scheme on the adaptive mesh
+    (face prolongation and restriction reproduce constants)."""
+    import exahype_amd as E
+    N = 3
+    q0 = (1.0, 0.1, -0.2, 0.3, 2.5)
+    s = E.ADERDGSolver(N)
+    g = E.AMRGrid(s, E.amr_config(3, 3, (0.5, 0.5, 0.5), 0.0, [0.01, 0.01]))
+    u = np.tile(np.array(q0), g.ncells * (N + 1) ** 3)
+    g.upload(u)
+    g.compute_dt()
+    g.step(3)
+    g.sync()
+    # to the 13-digit Kxi literals, which annihilate constants to ~1e-13 per 1/dx
+    assert np.abs(g.download() - u).max() <= 1e-11
