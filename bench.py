@@ -193,13 +193,15 @@ def main():
                     help="predictor sweep contraction: 0 tuned, 1 pencil owners, 2 node owners, "
                          "3 pencil owners two slices at a time, 4 staged iterate (FP64, orders 5..9)")
     ap.add_argument("--workload", default="aderdg", choices=["aderdg", "fv2d", "ensemble", "limiter", "muscl", "amr",
-                                                          "ncp", "linear"],
+                                                          "ncp", "linear", "amr_limiter"],
                     help="aderdg: the headline ADER-DG step; fv2d: the EulerFV demonstrator "
                          "(2D Godunov, 81x81 patches of 10x10 volumes, 200 steps); ensemble: independent "
                          "grids of random order/size/gamma advanced together; limiter: troubled-cell detection and "
                          "subcell projection on the Sod sphere (27^3 cells, N=5); muscl: 3D MUSCL-Hancock FV on "
                          "27^3 patches of 11^3 subcells (the limiter's FV scheme of the N=5 Sod sphere); amr: batched "
-                         "volume prolongation/restriction of 27^3 N=5 cells one level down and back")
+                         "volume prolongation/restriction of 27^3 N=5 cells one level down and back; amr_limiter: "
+                         "limiting ADER-DG (N=5) on a 3-level adaptive mesh refined in a spherical shell, Sod sphere, "
+                         "100 steps")
     ap.add_argument("--instances", type=int, default=512, help="ensemble members (--workload ensemble)")
     ap.add_argument("--point-source", action="store_true",
                     help="--workload linear: with a Ricker point source at the domain centre (evaluated per node "
@@ -232,6 +234,9 @@ def main():
         return
     if args.workload == "linear":
         print(json.dumps(run_linear(args)))
+        return
+    if args.workload == "amr_limiter":
+        print(json.dumps(run_amr_limiter(args)))
         return
     import exahype_amd as E
     N, n = args.order, args.cells
@@ -852,6 +857,120 @@ def run_limiter(args):
                                   "check": t_check / steps * 1e3, "project": t_proj / steps * 1e3},
             "note": "host-timed stages incl. synchronisation and the troubled count / cell list read back; the "
                     "limiter status machine and FV recomputation are not built"}
+
+
+AMR_BASE, AMR_LEVELS, AMR_RADIUS, AMR_WIDTHS = 27, 3, 0.25, (0.06, 0.031)
+
+
+def amr_centre(seed=4):
+    """The seeded centre of the refinement shell and of the Sod sphere."""
+    rng = np.random.default_rng(seed)
+    return tuple(0.5 + rng.uniform(-0.01, 0.01, 3))
+
+
+def run_amr_limiter(args):
+    """Limiting ADER-DG on a 3-level adaptive mesh: 3D Euler N=5, base 27^3
+    cells refined by 3 twice where a cell meets a spherical shell around the
+    Sod discontinuity (seeded centre, radius 0.25), 2:1 balanced; Sod sphere
+    (rho, p = 1, 1 inside r < 0.25; 0.125, 0.1 outside); MUSCL-Hancock on
+    11^3 subcells for the troubled cells and their neighbours on the finest
+    level; 100 steps, one hipGraph per step.  The predictor runs in FP64 and in
+    FP32 with one FP64 correction sweep (update, Riemann fluxes and the time
+    step stay FP64); value = the mixed run.  Roofline: the predictor's
+    algorithmic FLOP (Picard sweeps measured) over its HIP-event time."""
+    import exahype_amd as E
+    N, steps = 5, args.steps if args.steps != 10 else 100
+    centre = amr_centre()
+    cfg = E.amr_config(AMR_BASE, AMR_LEVELS, centre, AMR_RADIUS, AMR_WIDTHS, relax=1e-4, scaling=1e-3,
+                       steps_till_cured=0)
+    ic = E.make_ic(1, xc=centre, r0=AMR_RADIUS, rho_in=1.0, p_in=1.0, rho_out=0.125, p_out=0.1)
+    runs, states = {}, {}
+    for name, prec in (("fp64", E.PREC_FP64), ("fp32", E.PREC_FP32)):
+        s = E.ADERDGSolver(N, precision=prec, fp64_sweeps=1)
+        g = E.AMRGrid(s, cfg)
+        cells = g.cells()
+        g.set_initial_conditions(ic)
+        g.compute_dt()
+        print("amr_limiter %s: %d leaves, first step" % (name, g.ncells), file=sys.stderr, flush=True)
+        g.step(1)  # warm-up (graph capture)
+        g.sync()
+        it0 = g.total_picard_iterations()
+        E.device_synchronize()
+        t0 = time.perf_counter()
+        chunk = 20
+        for k in range(0, steps, chunk):
+            g.step(min(chunk, steps - k))
+            g.sync()
+            print("amr_limiter %s: %d steps" % (name, min(k + chunk, steps)), file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+        t_end, dt = g.sync()
+        sweeps = (g.total_picard_iterations() - it0) / (steps * g.ncells)
+        status = g.limiter_status()
+        cnt = g.counters()
+        # phase times of a few further steps, graph off
+        g.enable_timing(True)
+        g.step(3)
+        ms, nst = g.phase_times()
+        g.enable_timing(False)
+        fin = cells[:, 0] == AMR_LEVELS - 1
+        runs[name] = {"dof_updates_per_s": g.ncells * NVAR * (N + 1) ** 3 * steps / wall,
+                      "ms_per_step": wall / steps * 1e3, "sweeps_per_cell": sweeps, "t_end": t_end,
+                      "phase_ms_per_step": dict(zip(("predictor", "transfers_merge", "riemann", "dg_update",
+                                                     "limiter", "time_step"), (ms / max(nst, 1)).tolist())),
+                      "counters": cnt,
+                      "troubled_fraction_per_step": cnt["troubled"] / (steps + 1) / g.ncells,
+                      "final_status": {E.AMR_STATUS_NAMES[i]: int((status[fin] == i).sum()) for i in range(6)}}
+        states[name] = g.download()
+        levels = np.bincount(cells[:, 0], minlength=AMR_LEVELS)
+        ncells = g.ncells
+        g.close()
+        s.close() if hasattr(s, "close") else None
+    norms = error_norms(states["fp32"], states["fp64"], N, ncells)
+    m = runs["fp32"]
+    m["l1_vs_fp64_per_variable"] = norms["l1"].tolist()
+    m["l2_vs_fp64_per_variable"] = norms["l2"].tolist()
+    m["linf_vs_fp64_per_variable"] = norms["linf"].tolist()
+    m["speedup_vs_fp64"] = m["dof_updates_per_s"] / runs["fp64"]["dof_updates_per_s"]
+    pred_ms = m["phase_ms_per_step"]["predictor"]
+    flop = predictor_flops_per_cell(N, m["sweeps_per_cell"]) * ncells
+    achieved = flop / (pred_ms * 1e-3) / 1e12
+    # flop-weighted peak: the FP64 correction sweep and the tail in FP64, the rest in FP32
+    sw = m["sweeps_per_cell"]
+    f64 = predictor_flops_per_cell(N, 1.0) * ncells
+    frac64 = min(1.0, f64 / flop)
+    peak = 1.0 / (frac64 / PEAK_FP64_TFLOPS + (1 - frac64) / PEAK_FP32_TFLOPS)
+    out = {"metric": "DoF-updates/s, limiting ADER-DG N=5 on a 3-level adaptive mesh (Sod sphere), mixed predictor",
+           "value": m["dof_updates_per_s"], "unit": "DoF-updates/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+           "ms_per_step": m["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32+f64", "data": "synthetic",
+           "config": {"workload": "3D Euler limiting ADER-DG N=5, base 27^3 cells + 2 levels refined by 3 in a "
+                                  "spherical shell (radius 0.25, half widths 0.06 / 0.031, seeded centre), 2:1 "
+                                  "balanced, Sod sphere, MUSCL-Hancock 11^3 subcells on the finest level, DMP relax "
+                                  "1e-4 / scaling 1e-3, global time step, 100 steps",
+                      "cells": int(ncells), "cells_per_level": levels.tolist(), "centre": list(centre),
+                      "order": N, "predictor": "fp32 Picard sweeps + 1 FP64 correction sweep; FP64 update",
+                      "parallelism": "single GPU"},
+           "precisions": runs,
+           "roofline": {"bound": "fp32_valu+fp64", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                        "frac": achieved / peak, "traffic": None, "kernel": "k_predictor<5, fp32 + 1 FP64 sweep>",
+                        "avg_launch_ms": pred_ms, "picard_sweeps_per_cell_step": sw, "flop_per_launch": flop},
+           "timing": "value: wall clock of the steps replayed as one hipGraph per step; phase_ms_per_step: HIP "
+                     "events around the stages of 3 further steps (graph off)"}
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        from oracle import amr_limiting as A
+        og = A.LimitingGrid(A.Mesh(3, AMR_LEVELS, centre, AMR_RADIUS, list(AMR_WIDTHS)), N)
+        og.init(O.make_ic(1, xc=centre, r0=AMR_RADIUS, rho_in=1.0, p_in=1.0, rho_out=0.125, p_out=0.1))
+        dt = og.stable_dt()
+        t0 = time.perf_counter()
+        og.step(dt)
+        cwall = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": og.nc * NVAR * (N + 1) ** 3 / cwall, "unit": "DoF-updates/s", "cores": 1,
+                               "kind": "port",
+                               "sample": "oracle (per-cell C kernels driven from Python, one thread), the same "
+                                         "shell and Sod sphere on a base-3^3 mesh (%d leaves), 1 step (%.1f s)"
+                                         % (og.nc, cwall)}
+    return out
 
 
 def copy_reference(gib):

@@ -93,6 +93,7 @@ EXPORTS = [
     "exa_amr_grid_upload", "exa_amr_grid_download", "exa_amr_grid_set_initial_conditions", "exa_amr_grid_compute_dt",
     "exa_amr_grid_use_graph", "exa_amr_grid_enable_timing", "exa_amr_grid_step", "exa_amr_grid_sync",
     "exa_amr_grid_limiter_status", "exa_amr_grid_patches", "exa_amr_grid_counters", "exa_amr_grid_phase_times",
+    "exa_amr_grid_total_picard_iterations",
     "exa_device_malloc", "exa_device_free", "exa_memcpy_htod", "exa_memcpy_dtoh", "exa_device_copy",
     "exa_selftest_mfma",
     "exa_ensemble_create", "exa_ensemble_destroy", "exa_ensemble_set_initial_conditions", "exa_ensemble_compute_dt",
@@ -209,6 +210,7 @@ def load_library(path=LIB_PATH):
     L.exa_amr_grid_patches.argtypes = [C.c_void_p, _D, P(I), LL, P(LL)]
     L.exa_amr_grid_counters.argtypes = [C.c_void_p, P(LL)]
     L.exa_amr_grid_phase_times.argtypes = [C.c_void_p, _D, P(LL)]
+    L.exa_amr_grid_total_picard_iterations.argtypes = [C.c_void_p, P(LL)]
     _lib = L
     return L
 
@@ -969,6 +971,11 @@ class AMRGrid:
         c = (C.c_longlong * 8)()
         _check(_lib.exa_amr_grid_counters(self._h, c))
         return dict(zip(AMR_COUNTER_NAMES, list(c)))
+
+    def total_picard_iterations(self):
+        n = C.c_longlong(0)
+        _check(_lib.exa_amr_grid_total_picard_iterations(self._h, C.byref(n)))
+        return n.value
 
     def phase_times(self):
         ms = np.zeros(6)

@@ -1015,3 +1015,13 @@ extern "C" int exa_amr_grid_phase_times(exa_amr_grid* g, double ms[6], long long
   if (steps) *steps = g->timed_steps;
   return EXA_OK;
 }
+
+extern "C" int exa_amr_grid_total_picard_iterations(exa_amr_grid* g, long long* total) {
+  EXA_ARG(g && total, "null pointer");
+  EXA_CHECK(hipSetDevice(g->device));
+  EXA_CHECK(hipStreamSynchronize(g->stream));
+  unsigned long long t = 0;
+  EXA_CHECK(hipMemcpy(&t, g->iter_sum.p, sizeof(t), hipMemcpyDeviceToHost));
+  *total = (long long)t;
+  return EXA_OK;
+}

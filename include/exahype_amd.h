@@ -600,6 +600,9 @@ int exa_amr_grid_counters(exa_amr_grid* g, long long counts[8]);
 /* per step (timing on, graph off): predictor, face transfers + merges,
  * Riemann, ADER-DG update, limiter (FV, status, recomputation), time step (ms) */
 int exa_amr_grid_phase_times(exa_amr_grid* g, double ms[6], long long* steps);
+/* Picard sweeps of all predictor launches so far, summed over the leaves
+ * (measurement; no counterpart in the reference) */
+int exa_amr_grid_total_picard_iterations(exa_amr_grid* g, long long* total);
 
 #ifdef __cplusplus
 }

@@ -632,6 +632,57 @@ void orc_solution_adjustment(const orc_cfg* c, const orc_ic* ic, double* luh,
 }
 
 /* ---- periodic grid driver ---- */
+void orc_boundary_values(const orc_cfg* c, const orc_bc* bc, const double x[3], double t, double dt, int faceIndex,
+                         int d, const double* Fin, const double* Qin, double* Fout, double* Qout) {
+  const int nv = c->nvar, nd = c->nvar + c->npar;
+  double F[3 * 16];
+  (void)faceIndex;
+  if (bc->kind == 2) {
+    for (int v = 0; v < nd; ++v) Qout[v] = Qin[v];
+    Qout[1 + d] = -Qin[1 + d];
+    orc_flux(c, Qout, F);
+    for (int v = 0; v < nv; ++v) Fout[v] = F[d * nv + v];
+  } else if (bc->kind == 3) {
+    const int N = c->order;
+    double Q[16];
+    for (int v = 0; v < nd; ++v) Qout[v] = 0.0;
+    for (int v = 0; v < nv; ++v) Fout[v] = 0.0;
+    for (int i = 0; i <= N; ++i) {
+      const double w = EXA_TAB_gl_weights[N][i];
+      orc_ic_point(c, &bc->ic, x, t + EXA_TAB_gl_nodes[N][i] * dt, Q);
+      orc_flux(c, Q, F);
+      for (int v = 0; v < nd; ++v) Qout[v] += w * Q[v];
+      for (int v = 0; v < nv; ++v) Fout[v] += w * F[d * nv + v];
+    }
+  } else {
+    for (int v = 0; v < nd; ++v) Qout[v] = Qin[v];
+    for (int v = 0; v < nv; ++v) Fout[v] = Fin[v];
+  }
+}
+
+void orc_boundary_conditions(const orc_cfg* c, const orc_bc* bc, double* FOut, double* QOut, const double* FIn,
+                             const double* QIn, const double centre[3], const double size[3], double t, double dt,
+                             int faceIndex, int normalNonZero) {
+  const int N = c->order, B = N + 1, nv = c->nvar, nd = c->nvar + c->npar;
+  const int f = faceIndex - 2 * normalNonZero;
+  int d1 = 0, d2 = 0;
+  switch (normalNonZero) {
+    case 0: d1 = 1; d2 = 2; break;
+    case 1: d1 = 0; d2 = 2; break;
+    default: d1 = 0; d2 = 1; break;
+  }
+  double x[3];
+  x[normalNonZero] = centre[normalNonZero] + (-0.5 + f) * size[normalNonZero];
+  for (int i = 0; i < B; i++) {
+    x[d2] = centre[d2] + size[d2] * (EXA_TAB_gl_nodes[N][i] - 0.5);
+    for (int j = 0; j < B; j++) {
+      x[d1] = centre[d1] + size[d1] * (EXA_TAB_gl_nodes[N][j] - 0.5);
+      orc_boundary_values(c, bc, x, t, dt, faceIndex, normalNonZero, FIn + (i * B + j) * nv, QIn + (i * B + j) * nd,
+                          FOut + (i * B + j) * nv, QOut + (i * B + j) * nd);
+    }
+  }
+}
+
 int orc_grid_alloc(const orc_cfg* c, orc_grid* g, int nx, int ny, int nz) {
   const int B = c->order + 1, nd = c->nvar + c->npar, nv = c->nvar;
   const size_t nc = (size_t)nx * ny * nz;
@@ -643,6 +694,7 @@ int orc_grid_alloc(const orc_cfg* c, orc_grid* g, int nx, int ny, int nz) {
   g->lQhbnd = (double*)calloc(nc * 6 * B * B * nd, sizeof(double));
   g->lFhbnd = (double*)calloc(nc * 6 * B * B * nv, sizeof(double));
   g->iters = (int*)calloc(nc, sizeof(int));
+  g->periodic[0] = g->periodic[1] = g->periodic[2] = 1;
   return (g->luh && g->lduh && g->lQhbnd && g->lFhbnd && g->iters) ? 0 : -1;
 }
 
@@ -720,11 +772,30 @@ double orc_grid_step(const orc_cfg* c, orc_grid* g, double dt, int nthreads) {
   /* Prediction sweep (E/solvers/ADERDGSolver.cpp:1818-1946) */
   orc_grid_predict_cells(c, g, dt, 0, nc, nthreads);
   /* NeighbourDataMerging: Riemann problem on every face, periodic wrap
-   * (E/solvers/ADERDGSolver.cpp:2744-2825; left cell face 2d+1, right cell face 2d) */
+   * (E/solvers/ADERDGSolver.cpp:2744-2825; left cell face 2d+1, right cell face 2d);
+   * a domain face of a non-periodic axis: boundaryConditions, then Riemann
+   * with the outside state on the outer side (:2854-2931) */
   for (int d = 0; d < 3; ++d) {
 #pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int q = 0; q < nc; ++q) {
       const int ix = q % nx, iy = (q / nx) % ny, iz = q / (nx * ny);
+      const int id = d == 0 ? ix : (d == 1 ? iy : iz), nd_ = d == 0 ? nx : (d == 1 ? ny : nz);
+      if (!g->periodic[d]) {
+        double QOut[16 * 16 * 16], FOut[16 * 16 * 16], centre[3], dxc[3];
+        for (int side = 0; side < 2; ++side) {
+          if ((side == 0 && id != 0) || (side == 1 && id != nd_ - 1)) continue;
+          const int f = 2 * d + side;
+          double* QIn = g->lQhbnd + ((size_t)q * 6 + f) * B2 * nd;
+          double* FIn = g->lFhbnd + ((size_t)q * 6 + f) * B2 * nv;
+          cell_geom(g, q, centre, dxc);
+          orc_boundary_conditions(c, &g->bc, FOut, QOut, FIn, QIn, centre, dxc, g->t, dt, f, d);
+          if (f % 2 == 0)
+            orc_riemann_solver(c, FOut, FIn, QOut, QIn, dt, d);
+          else
+            orc_riemann_solver(c, FIn, FOut, QIn, QOut, dt, d);
+        }
+        if (id == nd_ - 1) continue;
+      }
       int jx = ix, jy = iy, jz = iz;
       if (d == 0) jx = (ix + 1) % nx;
       if (d == 1) jy = (iy + 1) % ny;

@@ -122,7 +122,26 @@ void orc_ic_point(const orc_cfg* c, const orc_ic* ic, const double x[3], double 
 void orc_solution_adjustment(const orc_cfg* c, const orc_ic* ic, double* luh,
                              const double centre[3], const double dx[3], double t);
 
-/* periodic regular grid driver (non-fused stepping, E/runners/Runner.cpp:1125-1190) */
+/* The compiled user boundaryValues of the Euler solvers (the body of the
+ * generated GenericADERDGSolverInCUserCode.template:44-52): kind 1 outflow
+ * (copy, as the EulerFV demonstrator, Demonstrators/EulerFV/MyEulerSolver.cpp:87-98),
+ * 2 reflecting wall (normal momentum negated, flux of that state), 3 the
+ * exact solution of ic averaged over the step at the GL time nodes. */
+typedef struct {
+  int kind;
+  orc_ic ic;
+} orc_bc;
+void orc_boundary_values(const orc_cfg* c, const orc_bc* bc, const double x[3], double t, double dt, int faceIndex,
+                         int d, const double* Fin, const double* Qin, double* Fout, double* Qout);
+/* boundaryConditions, kernels/aderdg/generic/c/3d/boundaryConditions.cpph:24-85 */
+void orc_boundary_conditions(const orc_cfg* c, const orc_bc* bc, double* FOut, double* QOut, const double* FIn,
+                             const double* QIn, const double centre[3], const double size[3], double t, double dt,
+                             int faceIndex, int normalNonZero);
+
+/* regular grid driver (non-fused stepping, E/runners/Runner.cpp:1125-1190);
+ * periodic along an axis unless periodic[d] == 0, where the domain faces take
+ * boundaryConditions + Riemann (applyBoundaryConditions,
+ * E/solvers/ADERDGSolver.cpp:2854-2931) */
 typedef struct {
   int nx, ny, nz;
   double offset[3], width[3];
@@ -132,6 +151,8 @@ typedef struct {
   double* lFhbnd;   /* [cell][6][B^2][nvar] */
   int* iters;       /* Picard iterations of the last predictor per cell */
   double t;         /* time at the start of the next step (advanced by orc_grid_step) */
+  int periodic[3];
+  orc_bc bc;
 } orc_grid;
 int orc_grid_alloc(const orc_cfg* c, orc_grid* g, int nx, int ny, int nz);
 void orc_grid_free(orc_grid* g);

@@ -34,12 +34,16 @@ class IC(C.Structure):
                 ("p_out", C.c_double), ("q0", C.c_double * 8)]
 
 
+class BC(C.Structure):
+    _fields_ = [("kind", C.c_int), ("ic", IC)]
+
+
 class Grid(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
                 ("offset", C.c_double * 3), ("width", C.c_double * 3),
                 ("luh", C.POINTER(C.c_double)), ("lduh", C.POINTER(C.c_double)),
                 ("lQhbnd", C.POINTER(C.c_double)), ("lFhbnd", C.POINTER(C.c_double)),
-                ("iters", C.POINTER(C.c_int)), ("t", C.c_double)]
+                ("iters", C.POINTER(C.c_int)), ("t", C.c_double), ("periodic", C.c_int * 3), ("bc", BC)]
 
 
 class FVGridS(C.Structure):
@@ -103,6 +107,8 @@ def lib():
         L.orc_godunov_update.restype = C.c_double
         L.orc_musclhancock_update.argtypes = [P(Cfg), C.c_int, _D, _D, _D, C.c_double, C.c_double]
         L.orc_fv_patch_ic.argtypes = [P(Cfg), P(IC), C.c_int, C.c_int, _D, _D, _D]
+        L.orc_boundary_conditions.argtypes = [P(Cfg), P(BC), _D, _D, _D, _D, _D, _D, C.c_double, C.c_double, C.c_int,
+                                              C.c_int]
         L.orc_musclhancock_update.restype = C.c_double
         L.orc_fv_grid_alloc.argtypes = [P(Cfg), P(FVGridS), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_fv_grid_alloc.restype = C.c_int
@@ -296,6 +302,26 @@ def stable_time_step_size(c, luh, dx):
     return lib().orc_stable_time_step_size(C.byref(c), ptr(dvec(luh)), ptr(dvec(dx)))
 
 
+BC_OUTFLOW, BC_REFLECTING, BC_EXACT = 1, 2, 3
+
+
+def make_bc(kind, ic=None):
+    b = BC()
+    b.kind = kind
+    if ic is not None:
+        b.ic = ic
+    return b
+
+
+def boundary_conditions(c, bc, FIn, QIn, centre, size, t, dt, face, normal):
+    """(FOut, QOut) of one face: the user boundaryValues at its GL nodes."""
+    B = c.order + 1
+    FOut, QOut = np.zeros(B * B * c.nvar), np.zeros(B * B * (c.nvar + c.npar))
+    lib().orc_boundary_conditions(C.byref(c), C.byref(bc), ptr(FOut), ptr(QOut), ptr(dvec(FIn)), ptr(dvec(QIn)),
+                                  ptr(dvec(centre)), ptr(dvec(size)), t, dt, face, normal)
+    return FOut, QOut
+
+
 def make_ic(kind=0, **kw):
     ic = IC()
     ic.kind = kind
@@ -340,6 +366,12 @@ class OracleGrid:
 
     def init(self, ic):
         lib().orc_grid_init(C.byref(self.c), C.byref(self.g), C.byref(ic))
+
+    def set_boundary(self, periodic, bc):
+        """periodic[d] = 0: the domain faces of axis d take the boundary conditions bc."""
+        for d in range(3):
+            self.g.periodic[d] = int(periodic[d])
+        self.g.bc = bc
 
     def luh(self):
         n = self.ncells * self.cell_size
