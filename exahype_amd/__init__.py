@@ -69,12 +69,31 @@ class InitialCondition(C.Structure):
                 ("p_out", C.c_double), ("q0", C.c_double * 8)]
 
 
+class BoundaryCondition(C.Structure):
+    """exa_bc: the boundary values of a domain face (EXA_BC_*)."""
+    _fields_ = [("kind", C.c_int), ("exact", InitialCondition)]
+
+
+BC_OUTFLOW, BC_REFLECTING, BC_EXACT = 1, 2, 3
+
+
+def make_bc(kind, exact=None):
+    """BoundaryCondition of a kind BC_*; exact: the InitialCondition whose
+    state BC_EXACT imposes."""
+    b = BoundaryCondition()
+    b.kind = kind
+    if exact is not None:
+        b.exact = exact
+    return b
+
+
 # every symbol declared in include/exahype_amd.h
 EXPORTS = [
     "exa_last_error", "exa_version", "exa_init", "exa_device_synchronize", "exa_aderdg_create", "exa_aderdg_destroy",
     "exa_aderdg_space_time_predictor", "exa_aderdg_space_time_predictor_ps", "exa_aderdg_point_source", "exa_aderdg_volume_integral", "exa_aderdg_surface_integral",
     "exa_aderdg_riemann_solver", "exa_aderdg_solution_update", "exa_aderdg_stable_time_step_size",
-    "exa_aderdg_adjust_solution", "exa_aderdg_batch_prediction", "exa_aderdg_batch_riemann",
+    "exa_aderdg_adjust_solution", "exa_aderdg_boundary_conditions", "exa_grid_set_boundary",
+    "exa_aderdg_batch_prediction", "exa_aderdg_batch_riemann",
     "exa_aderdg_batch_update", "exa_aderdg_batch_pending", "exa_aderdg_batch_flush", "exa_grid_create", "exa_grid_destroy", "exa_grid_num_cells",
     "exa_grid_upload", "exa_grid_download", "exa_grid_set_initial_conditions", "exa_grid_sample_cartesian",
     "exa_grid_compute_dt",
@@ -133,6 +152,9 @@ def load_library(path=LIB_PATH):
     L.exa_aderdg_solution_update.argtypes = [C.c_void_p, _D, _D, C.c_double]
     L.exa_aderdg_stable_time_step_size.argtypes = [C.c_void_p, _D, _D, _D]
     L.exa_aderdg_adjust_solution.argtypes = [C.c_void_p, _D, _D, _D, C.c_double, P(InitialCondition)]
+    L.exa_aderdg_boundary_conditions.argtypes = [C.c_void_p, P(BoundaryCondition), _D, _D, _D, _D, _D, _D,
+                                                 C.c_double, C.c_double, I, I]
+    L.exa_grid_set_boundary.argtypes = [C.c_void_p, P(I), P(BoundaryCondition)]
     L.exa_grid_create.argtypes = [C.c_void_p, I, I, I, _D, _D, P(C.c_void_p)]
     L.exa_grid_destroy.argtypes = [C.c_void_p]
     L.exa_grid_num_cells.argtypes = [C.c_void_p, P(LL)]
@@ -405,9 +427,22 @@ class ADERDGSolver:
                                                C.byref(ic)))
         return luh
 
+    def boundaryConditions(self, bc, FIn, QIn, centre, size, t, dt, faceIndex, normalNonZero):
+        """ADERDGSolver::boundaryConditions of one domain face: (FOut, QOut)
+        in the reference layout [B^2][nVar] / [B^2][nData]."""
+        B2 = self.B * self.B
+        FOut, QOut = np.zeros(B2 * self.nvar), np.zeros(B2 * self.ndata)
+        _check(_lib.exa_aderdg_boundary_conditions(self._h, C.byref(bc), _p(FOut), _p(QOut),
+                                                   _p(np.ascontiguousarray(FIn, np.float64)),
+                                                   _p(np.ascontiguousarray(QIn, np.float64)), _p(_vec3(centre)),
+                                                   _p(_vec3(size)), float(t), float(dt), int(faceIndex),
+                                                   int(normalNonZero)))
+        return FOut, QOut
+
 
 class Grid:
-    """Regular periodic mesh of ADER-DG cells resident in HBM."""
+    """Regular mesh of ADER-DG cells resident in HBM (periodic unless
+    set_boundary closes an axis)."""
 
     def __init__(self, solver, nx, ny, nz, offset=(0.0, 0.0, 0.0), width=(1.0, 1.0, 1.0)):
         self.solver = solver
@@ -449,6 +484,12 @@ class Grid:
 
     def set_initial_conditions(self, ic):
         _check(_lib.exa_grid_set_initial_conditions(self._h, C.byref(ic)))
+
+    def set_boundary(self, periodic, bc=None):
+        """periodic: three flags; the faces of a non-periodic axis take bc
+        (a BoundaryCondition)."""
+        per = (C.c_int * 3)(*[1 if p else 0 for p in periodic])
+        _check(_lib.exa_grid_set_boundary(self._h, per, None if bc is None else C.byref(bc)))
 
     def compute_dt(self):
         dt = C.c_double(0.0)

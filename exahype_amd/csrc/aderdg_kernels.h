@@ -1250,6 +1250,7 @@ __global__ __launch_bounds__(256) void k_riemann(const double* __restrict__ qf, 
   const int d = (int)(wave / ncells);
   const int cl = (int)(wave % ncells);
   const int cr = nbr[(size_t)d * ncells + cl];
+  if (cr < 0) return;  // a domain face of a non-periodic axis (k_boundary_faces)
   if (ens.cell_inst) gamma = ens.inst[ens.cell_inst[cl]].gamma;
   riemann_face<N>(qf + ((size_t)cl * 6 + 2 * d + 1) * V * B2, qf + ((size_t)cr * 6 + 2 * d) * V * B2,
                   ff + ((size_t)cl * 6 + 2 * d + 1) * V * B2, ff + ((size_t)cr * 6 + 2 * d) * V * B2, d, gamma, lane);
@@ -1266,6 +1267,151 @@ __global__ __launch_bounds__(256) void k_riemann_list(const double* __restrict__
   if (i >= nfaces) return;
   riemann_face<N>(q + (size_t)(2 * i) * V * B2, q + (size_t)(2 * i + 1) * V * B2, f + (size_t)(2 * i) * V * B2,
                   f + (size_t)(2 * i + 1) * V * B2, dirs[i], gamma, threadIdx.x & 63);
+}
+
+// ---------------------------------------------------------------------------
+// Boundary values of the Euler solvers at one face node (the user
+// boundaryValues, kinds EXA_BC_*): outflow copies the inside state and flux;
+// a reflecting wall negates the normal momentum and takes that state's
+// normal flux; exact data averages the state of bc.exact and its normal flux
+// over [t, t+dt] at the order's Gauss-Legendre time nodes.
+template <int N>
+__device__ inline void euler_boundary_values(const exa_bc& bc, const double x[3], double t, double dt, int d,
+                                             double gamma, const double (&Qin)[V], const double (&Fin)[V],
+                                             double (&Qout)[V], double (&Fout)[V]) {
+  if (bc.kind == EXA_BC_REFLECTING) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) Qout[v] = Qin[v];
+    Qout[1 + d] = -Qin[1 + d];
+    EulerPDE::flux_dir(Qout, d, Fout, gamma);
+  } else if (bc.kind == EXA_BC_EXACT) {
+    const DGTables& T = c_tab[N];
+#pragma unroll
+    for (int v = 0; v < V; ++v) Qout[v] = Fout[v] = 0.0;
+    for (int i = 0; i <= N; ++i) {
+      double Q[V], F[V];
+      exa_ic_point(bc.exact, gamma, x, t + T.xi[i] * dt, Q);
+      EulerPDE::flux_dir(Q, d, F, gamma);
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        Qout[v] += T.w[i] * Q[v];
+        Fout[v] += T.w[i] * F[v];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      Qout[v] = Qin[v];
+      Fout[v] = Fin[v];
+    }
+  }
+}
+
+// The two tangential axes of a face with normal d, faster one first (the
+// (a, b) = (slower, faster) node order of the face blocks).
+__device__ inline void face_axes(int d, int& d1, int& d2) {
+  d1 = d == 0 ? 1 : 0;
+  d2 = d == 2 ? 1 : 2;
+}
+
+// Domain faces of a grid with non-periodic axes, one wavefront per face:
+// boundaryConditions at every face node (boundaryConditions.cpph:24-85),
+// then the Riemann problem with the outside state on the outer side
+// (applyBoundaryConditions, E/solvers/ADERDGSolver.cpp:2854-2931).
+// bfaces[i] = cell * 6 + face; oq, of hold face i's outside state and flux
+// ([i][v][B^2]); the time and dt of the step are read from the device.
+template <int N>
+__global__ __launch_bounds__(256) void k_boundary_faces(double* __restrict__ qf, double* __restrict__ ff,
+                                                        const int* __restrict__ bfaces, int nb,
+                                                        double* __restrict__ oq, double* __restrict__ of,
+                                                        exa_bc bc, double gamma, const double* __restrict__ dt_cur,
+                                                        const double* __restrict__ time, double off0, double off1,
+                                                        double off2, double dx0, double dx1, double dx2, int nx,
+                                                        int ny) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2;
+  constexpr int NPL = (B2 + 63) / 64;
+  const int i = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= nb) return;
+  const DGTables& T = c_tab[N];
+  const int code = bfaces[i];
+  const long long cell = code / 6;
+  const int f = code % 6, d = f >> 1, side = f & 1;
+  const int ix = (int)(cell % nx), iy = (int)((cell / nx) % ny), iz = (int)(cell / ((long long)nx * ny));
+  const double h[3] = {dx0, dx1, dx2};
+  const double centre[3] = {off0 + (ix + 0.5) * dx0, off1 + (iy + 0.5) * dx1, off2 + (iz + 0.5) * dx2};
+  int d1, d2;
+  face_axes(d, d1, d2);
+  double* QIn = qf + ((size_t)cell * 6 + f) * V * B2;
+  double* FIn = ff + ((size_t)cell * 6 + f) * V * B2;
+  double* QO = oq + (size_t)i * V * B2;
+  double* FO = of + (size_t)i * V * B2;
+  const double t = *time, dt = *dt_cur;
+  double x[3];
+  x[d] = centre[d] + (-0.5 + side) * h[d];
+#pragma unroll
+  for (int k = 0; k < NPL; ++k) {
+    const int node = lane + 64 * k;
+    if (node < B2) {
+      x[d2] = centre[d2] + h[d2] * (T.xi[node / B] - 0.5);
+      x[d1] = centre[d1] + h[d1] * (T.xi[node % B] - 0.5);
+      double qi[V], fi[V], qo[V], fo[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        qi[v] = QIn[v * B2 + node];
+        fi[v] = FIn[v * B2 + node];
+      }
+      euler_boundary_values<N>(bc, x, t, dt, d, gamma, qi, fi, qo, fo);
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        QO[v * B2 + node] = qo[v];
+        FO[v * B2 + node] = fo[v];
+      }
+    }
+  }
+  // each lane reads back only the nodes it wrote
+  if (side == 0)
+    riemann_face<N>(QO, QIn, FO, FIn, d, gamma, lane);
+  else
+    riemann_face<N>(QIn, QO, FIn, FO, d, gamma, lane);
+}
+
+// boundaryConditions of one face in the reference layout (the per-face
+// entry): node-major QIn [B^2][nd], FIn [B^2][V] in, QOut, FOut out.
+template <int N>
+__global__ __launch_bounds__(64) void k_boundary_conditions_ref(double* __restrict__ FOut, double* __restrict__ QOut,
+                                                                const double* __restrict__ FIn,
+                                                                const double* __restrict__ QIn, exa_bc bc,
+                                                                double gamma, double c0, double c1, double c2,
+                                                                double s0, double s1, double s2, double t,
+                                                                double dt, int faceIndex, int d, int nd) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2;
+  const DGTables& T = c_tab[N];
+  const double centre[3] = {c0, c1, c2}, size[3] = {s0, s1, s2};
+  const int side = faceIndex - 2 * d;
+  int d1, d2;
+  face_axes(d, d1, d2);
+  for (int node = threadIdx.x; node < B2; node += blockDim.x) {
+    double x[3];
+    x[d] = centre[d] + (-0.5 + side) * size[d];
+    x[d2] = centre[d2] + size[d2] * (T.xi[node / B] - 0.5);
+    x[d1] = centre[d1] + size[d1] * (T.xi[node % B] - 0.5);
+    double qi[V], fi[V], qo[V], fo[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      qi[v] = QIn[(size_t)node * nd + v];
+      fi[v] = FIn[(size_t)node * V + v];
+    }
+    euler_boundary_values<N>(bc, x, t, dt, d, gamma, qi, fi, qo, fo);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      QOut[(size_t)node * nd + v] = qo[v];
+      FOut[(size_t)node * V + v] = fo[v];
+    }
+    for (int v = V; v < nd; ++v) QOut[(size_t)node * nd + v] = QIn[(size_t)node * nd + v];
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -694,6 +694,40 @@ extern "C" int exa_aderdg_riemann_solver(exa_solver* s, double* FL, double* FR, 
   return EXA_OK;
 }
 
+static bool bc_needs_euler(const exa_bc* bc) { return bc->kind == EXA_BC_REFLECTING || bc->kind == EXA_BC_EXACT; }
+static bool pde_is_euler(int pde) { return pde == EXA_PDE_EULER || pde == EXA_PDE_EULER_GRAVITY; }
+
+extern "C" int exa_aderdg_boundary_conditions(exa_solver* s, const exa_bc* bc, double* FOut, double* QOut,
+                                              const double* FIn, const double* QIn, const double centre[3],
+                                              const double size[3], double t, double dt, int faceIndex,
+                                              int normalNonZero) {
+  EXA_ARG(s && bc && FOut && QOut && FIn && QIn && centre && size, "null pointer");
+  EXA_ARG(normalNonZero >= 0 && normalNonZero < 3, "normalNonZero must be 0, 1 or 2");
+  EXA_ARG(faceIndex == 2 * normalNonZero || faceIndex == 2 * normalNonZero + 1, "faceIndex must be 2 normalNonZero + 0 or 1");
+  EXA_ARG(bc->kind >= EXA_BC_OUTFLOW && bc->kind <= EXA_BC_EXACT, "boundary condition kind");
+  EXA_ARG(!bc_needs_euler(bc) || pde_is_euler(s->cfg.pde), "reflecting and exact boundaries need an Euler PDE");
+  EXA_ARG(s->cfg.nvar == exa::V, "boundary values of 5 variables");
+  std::lock_guard<std::mutex> lk(s->mu);
+  EXA_CHECK(hipSetDevice(s->device));
+  const int N = s->cfg.order, B2 = (N + 1) * (N + 1), V = exa::V, nd = s->cfg.nvar + s->cfg.npar;
+  const size_t nq = (size_t)B2 * nd, nf = (size_t)B2 * V;
+  EXA_TRY(s->qf.ensure(sizeof(double) * 2 * nq));
+  EXA_TRY(s->ff.ensure(sizeof(double) * 2 * nf));
+  double* dq = s->qf.as<double>();
+  double* df = s->ff.as<double>();
+  EXA_CHECK(hipMemcpyAsync(dq, QIn, sizeof(double) * nq, hipMemcpyHostToDevice, s->stream));
+  EXA_CHECK(hipMemcpyAsync(df, FIn, sizeof(double) * nf, hipMemcpyHostToDevice, s->stream));
+  EXA_ORDER_SWITCH(N, {
+    hipLaunchKernelGGL((exa::k_boundary_conditions_ref<NN>), dim3(1), dim3(64), 0, s->stream, df + nf, dq + nq, df,
+                       dq, *bc, s->cfg.gamma, centre[0], centre[1], centre[2], size[0], size[1], size[2], t, dt,
+                       faceIndex, normalNonZero, nd);
+  });
+  EXA_CHECK(hipGetLastError());
+  EXA_CHECK(hipMemcpyAsync(QOut, dq + nq, sizeof(double) * nq, hipMemcpyDeviceToHost, s->stream));
+  EXA_CHECK(hipMemcpyAsync(FOut, df + nf, sizeof(double) * nf, hipMemcpyDeviceToHost, s->stream));
+  return sync_stream(s);
+}
+
 extern "C" int exa_aderdg_adjust_solution(exa_solver* s, double* luh, const double centre[3], const double dx[3],
                                           double t, const exa_ic* ic) {
   EXA_ARG(s && luh && centre && dx && ic, "null pointer");
@@ -971,6 +1005,12 @@ struct exa_grid {
   double off[3] = {0, 0, 0}, width[3] = {1, 1, 1}, dx[3] = {1, 1, 1};
   DevBuf luh, lduh, qf, ff, iters, nbr, state;
   DevBuf stage;    // staged predictor: one staging slot per resident workgroup
+  // domain boundaries (exa_grid_set_boundary): the faces of the non-periodic
+  // axes, cell * 6 + face, and their outside states and fluxes
+  int periodic[3] = {1, 1, 1};
+  exa_bc bc{};
+  int nb = 0;
+  DevBuf bfaces, bq, bf;
   DevBuf sample;   // Cartesian-vertex samples (output)
   int nslots = 0;
   bool staged = false;  // decided at creation (the slots are allocated then)
@@ -989,6 +1029,39 @@ struct exa_grid {
   double* time() const { return state.as<double>() + 2; }
   unsigned long long* iter_sum() const { return reinterpret_cast<unsigned long long*>(state.as<double>() + 3); }
 };
+
+// +d neighbours (nbr[d * nc + c]), wrapped on periodic axes and -1 past the
+// last cell of the others, and the list of domain faces of the non-periodic
+// axes (cell * 6 + face).
+static hipError_t grid_neighbours(exa_grid* g) {
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const int n[3] = {nx, ny, nz};
+  std::vector<int> nbr(3 * g->nc), bfaces;
+  for (long long c = 0; c < g->nc; ++c) {
+    const int ix = (int)(c % nx), iy = (int)((c / nx) % ny), iz = (int)(c / ((long long)nx * ny));
+    const int id[3] = {ix, iy, iz};
+    for (int d = 0; d < 3; ++d) {
+      int j[3] = {ix, iy, iz};
+      j[d] = (id[d] + 1) % n[d];
+      const bool edge = id[d] == n[d] - 1 && !g->periodic[d];
+      nbr[d * g->nc + c] = edge ? -1 : (int)(((long long)j[2] * ny + j[1]) * nx + j[0]);
+      if (!g->periodic[d]) {
+        if (id[d] == 0) bfaces.push_back((int)(c * 6 + 2 * d));
+        if (id[d] == n[d] - 1) bfaces.push_back((int)(c * 6 + 2 * d + 1));
+      }
+    }
+  }
+  hipError_t e = hipMemcpy(g->nbr.p, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice);
+  g->nb = (int)bfaces.size();
+  if (e == hipSuccess && g->nb > 0) {
+    const int B2 = (g->s->cfg.order + 1) * (g->s->cfg.order + 1);
+    if (g->bfaces.ensure(sizeof(int) * g->nb) != EXA_OK || g->bq.ensure(sizeof(double) * g->nb * exa::V * B2) != EXA_OK ||
+        g->bf.ensure(sizeof(double) * g->nb * exa::V * B2) != EXA_OK)
+      return hipErrorOutOfMemory;
+    e = hipMemcpy(g->bfaces.p, bfaces.data(), sizeof(int) * g->nb, hipMemcpyHostToDevice);
+  }
+  return e;
+}
 
 extern "C" int exa_grid_create(exa_solver* s, int nx, int ny, int nz, const double offset[3], const double width[3],
                                exa_grid** out) {
@@ -1027,14 +1100,7 @@ extern "C" int exa_grid_create(exa_solver* s, int nx, int ny, int nz, const doub
       return r;
     }
   }
-  std::vector<int> nbr(3 * g->nc);
-  for (long long c = 0; c < g->nc; ++c) {
-    const int ix = (int)(c % nx), iy = (int)((c / nx) % ny), iz = (int)(c / ((long long)nx * ny));
-    nbr[0 * g->nc + c] = (int)(((long long)iz * ny + iy) * nx + (ix + 1) % nx);
-    nbr[1 * g->nc + c] = (int)(((long long)iz * ny + (iy + 1) % ny) * nx + ix);
-    nbr[2 * g->nc + c] = (int)(((long long)((iz + 1) % nz) * ny + iy) * nx + ix);
-  }
-  hipError_t e = hipMemcpy(g->nbr.p, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice);
+  hipError_t e = grid_neighbours(g);
   double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const double big = 1.0e300;
   std::memcpy(&st[1], &big, sizeof(big));
@@ -1070,6 +1136,7 @@ int exa_grid_view(exa_grid* g, ExaGridView* v) {
   v->device = g->s->device;
   v->stream = g->stream;
   v->version = g->version;
+  v->periodic = g->periodic[0] && g->periodic[1] && g->periodic[2];
   return EXA_OK;
 }
 
@@ -1312,6 +1379,11 @@ static int enqueue_step(exa_grid* g, bool timed) {
   } else EXA_ORDER_SWITCH(N, {
     hipLaunchKernelGGL((exa::k_riemann<NN>), dim3(blocks), dim3(256), 0, g->stream, g->qf.as<double>(),
                        g->ff.as<double>(), g->nbr.as<int>(), (int)g->nc, g->s->cfg.gamma, 0LL, waves, exa::Ens{});
+    if (g->nb > 0)
+      hipLaunchKernelGGL((exa::k_boundary_faces<NN>), dim3((unsigned)((g->nb * 64LL + 255) / 256)), dim3(256), 0,
+                         g->stream, g->qf.as<double>(), g->ff.as<double>(), g->bfaces.as<int>(), g->nb,
+                         g->bq.as<double>(), g->bf.as<double>(), g->bc, g->s->cfg.gamma, g->dt_cur(), g->time(),
+                         g->off[0], g->off[1], g->off[2], g->dx[0], g->dx[1], g->dx[2], g->nx, g->ny);
   });
   EXA_CHECK(hipGetLastError());
   EXA_TRY(tstop(1));
@@ -1320,6 +1392,32 @@ static int enqueue_step(exa_grid* g, bool timed) {
   EXA_TRY(tstop(2));
   hipLaunchKernelGGL(exa::k_dt_advance, dim3(1), dim3(1), 0, g->stream, g->dt_cur(), g->dt_next(), g->time());
   EXA_CHECK(hipGetLastError());
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_set_boundary(exa_grid* g, const int periodic[3], const exa_bc* bc) {
+  EXA_ARG(g && periodic, "null pointer");
+  const bool all = periodic[0] && periodic[1] && periodic[2];
+  if (!all) {
+    EXA_ARG(bc, "a grid with a non-periodic axis needs boundary conditions");
+    EXA_ARG(bc->kind >= EXA_BC_OUTFLOW && bc->kind <= EXA_BC_EXACT, "boundary condition kind");
+    EXA_ARG(pde_is_euler(g->s->cfg.pde) && !g->s->cfg.linear,
+            "domain boundaries of the device grid are implemented for the Euler PDEs");
+  }
+  EXA_CHECK(hipSetDevice(g->s->device));
+  EXA_CHECK(hipStreamSynchronize(g->stream));
+  for (int d = 0; d < 3; ++d) g->periodic[d] = periodic[d] ? 1 : 0;
+  if (bc) g->bc = *bc;
+  const hipError_t e = grid_neighbours(g);
+  if (e != hipSuccess) {
+    exa_set_error("grid boundary setup: %s", hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? EXA_ERR_NOMEM : EXA_ERR_HIP;
+  }
+  // the captured step holds the old neighbour list and boundary kernel
+  if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  g->gexec = nullptr;
+  g->graph = nullptr;
   return EXA_OK;
 }
 

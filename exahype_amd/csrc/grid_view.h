@@ -13,6 +13,7 @@ struct ExaGridView {
   int device;
   hipStream_t stream;
   long long version;  // changes whenever luh changes
+  bool periodic;      // every axis periodic (exa_grid_set_boundary not used)
 };
 
 int exa_grid_view(exa_grid* g, ExaGridView* v);

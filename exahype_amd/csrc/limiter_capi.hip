@@ -297,6 +297,7 @@ extern "C" int exa_grid_limiter_snapshot(exa_grid* g, int nobs) {
   ExaGridView v;
   EXA_TRY(exa_grid_view(g, &v));
   EXA_TRY(check_order(v.order));
+  EXA_ARG(v.periodic, "the grid limiter merges neighbour extrema periodically: every axis must be periodic");
   EXA_CHECK(hipSetDevice(v.device));
   const LimTables* tab = nullptr;
   EXA_TRY(lim_tables(v.device, v.order, &tab));

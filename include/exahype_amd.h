@@ -236,6 +236,34 @@ typedef struct {
 int exa_aderdg_adjust_solution(exa_solver* s, double* luh, const double centre[3],
                                const double dx[3], double t, const exa_ic* ic);
 
+/* Boundary values of the Euler solvers, the user boundaryValues that the
+ * generated solver calls per face node (GenericADERDGSolverInCUserCode.template:44-52):
+ *   EXA_BC_OUTFLOW    outside state and normal flux = the inside ones (the
+ *                     EulerFV demonstrator's choice, Demonstrators/EulerFV/MyEulerSolver.cpp:87-98);
+ *   EXA_BC_REFLECTING the inside state with its normal momentum negated, and
+ *                     that state's normal flux (a slip wall);
+ *   EXA_BC_EXACT      the state of `exact` at the face node, state and normal
+ *                     flux averaged over [t, t+dt] with the order's
+ *                     Gauss-Legendre rule.
+ * REFLECTING and EXACT need the Euler PDEs (EXA_PDE_EULER, EXA_PDE_EULER_GRAVITY). */
+#define EXA_BC_OUTFLOW 1
+#define EXA_BC_REFLECTING 2
+#define EXA_BC_EXACT 3
+typedef struct {
+  int kind;
+  exa_ic exact; /* EXA_BC_EXACT only */
+} exa_bc;
+
+/* ADERDGSolver::boundaryConditions (E/solvers/ADERDGSolver.h:1061-1070;
+ * kernels/aderdg/generic/c/3d/boundaryConditions.cpph:24-85): the outside
+ * state QOut [B^2][nData] and normal flux FOut [B^2][nVar] of one domain
+ * face (faceIndex 0..5 = 2 normalNonZero + side) of the cell at centre with
+ * extent size, from the inside QIn, FIn; node (i, j) sits at the face's
+ * GL point with i along the slower, j along the faster tangential axis. */
+int exa_aderdg_boundary_conditions(exa_solver* s, const exa_bc* bc, double* FOut, double* QOut, const double* FIn,
+                                   const double* QIn, const double centre[3], const double size[3], double t,
+                                   double dt, int faceIndex, int normalNonZero);
+
 /* Deferred per-cell entry points.  The reference calls its kernels cell by
  * cell inside one mesh traversal and reads the results only in the next one;
  * these calls copy their inputs, queue the work and return, and
@@ -276,6 +304,13 @@ int exa_grid_num_cells(const exa_grid* g, long long* n);
 int exa_grid_upload(exa_grid* g, const double* luh);
 int exa_grid_download(exa_grid* g, double* luh);
 int exa_grid_set_initial_conditions(exa_grid* g, const exa_ic* ic);
+/* Domain boundaries: axis d stays periodic where periodic[d] != 0 (the
+ * default); on the two domain faces of every other axis each step calls
+ * boundaryConditions with bc and solves the Riemann problem with the
+ * outside state on the outer side (applyBoundaryConditions,
+ * E/solvers/ADERDGSolver.cpp:2854-2931).  Euler PDEs only (EXA_ERR_ARG
+ * otherwise, unless every axis is periodic).  Rebuilds the step's graph. */
+int exa_grid_set_boundary(exa_grid* g, const int periodic[3], const exa_bc* bc);
 /* The values a Cartesian-vertices plotter writes
  * (exahype::plotters::ADERDG2CartesianVTK::plotVertexData,
  * E/plotters/VTK/ADERDG2CartesianVTK.cpp:288-336): the solution of every
