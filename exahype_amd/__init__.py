@@ -101,7 +101,9 @@ EXPORTS = [
     "exa_grid_use_graph", "exa_grid_enable_timing", "exa_grid_kernel_times",
     "exa_grid_total_picard_iterations",
     "exa_fv_create", "exa_fv_destroy", "exa_fv_solution_update", "exa_fv_stable_time_step_size",
-    "exa_fv_riemann_solver", "exa_fv_grid_create", "exa_fv_grid_destroy", "exa_fv_grid_upload",
+    "exa_fv_riemann_solver", "exa_fv_ghost_layer_filling", "exa_fv_ghost_layer_filling_at_boundary",
+    "exa_fv_boundary_layer_extraction", "exa_fv_boundary_conditions", "exa_fv_grid_set_boundary",
+    "exa_fv_grid_create", "exa_fv_grid_destroy", "exa_fv_grid_upload",
     "exa_fv_grid_download", "exa_fv_grid_set_initial_conditions", "exa_fv_grid_compute_dt", "exa_fv_grid_step",
     "exa_fv_grid_sync", "exa_fv_grid_enable_timing", "exa_fv_grid_kernel_times", "exa_limiter_project_on_fv", "exa_limiter_project_on_dg", "exa_limiter_dmp",
     "exa_amr_face_unknowns_prolongation", "exa_amr_face_unknowns_restriction",
@@ -176,6 +178,12 @@ def load_library(path=LIB_PATH):
     L.exa_fv_solution_update.argtypes = [C.c_void_p, _D, _D, _D, C.c_double, _D]
     L.exa_fv_stable_time_step_size.argtypes = [C.c_void_p, _D, _D, _D]
     L.exa_fv_riemann_solver.argtypes = [C.c_void_p, _D, _D, _D, _D, I, _D]
+    for fn in ("exa_fv_ghost_layer_filling", "exa_fv_ghost_layer_filling_at_boundary",
+               "exa_fv_boundary_layer_extraction"):
+        getattr(L, fn).argtypes = [C.c_void_p, _D, _D, P(I)]
+    L.exa_fv_boundary_conditions.argtypes = [C.c_void_p, P(BoundaryCondition), _D, _D, _D, _D, C.c_double,
+                                             C.c_double, I, I]
+    L.exa_fv_grid_set_boundary.argtypes = [C.c_void_p, P(I), P(BoundaryCondition)]
     L.exa_fv_grid_create.argtypes = [C.c_void_p, I, I, I, _D, _D, P(C.c_void_p)]
     L.exa_fv_grid_destroy.argtypes = [C.c_void_p]
     L.exa_fv_grid_upload.argtypes = [C.c_void_p, _D]
@@ -606,9 +614,41 @@ class FiniteVolumesSolver:
                                           C.byref(smax)))
         return fL, fR, smax.value
 
+    # boundary hooks (positions: a unit vector towards the face)
+    def _block_size(self):
+        return self.patch ** (self.dim - 1) * self.ghost * 5
+
+    def ghostLayerFilling(self, luh, luhNeighbour, neighbourPosition):
+        luh = np.array(luh, dtype=np.float64, copy=True)
+        _check(_lib.exa_fv_ghost_layer_filling(self._h, _p(luh), _p(np.ascontiguousarray(luhNeighbour, np.float64)),
+                                               (C.c_int * 3)(*neighbourPosition)))
+        return luh
+
+    def ghostLayerFillingAtBoundary(self, luh, luhbnd, boundaryPosition):
+        luh = np.array(luh, dtype=np.float64, copy=True)
+        _check(_lib.exa_fv_ghost_layer_filling_at_boundary(self._h, _p(luh),
+                                                           _p(np.ascontiguousarray(luhbnd, np.float64)),
+                                                           (C.c_int * 3)(*boundaryPosition)))
+        return luh
+
+    def boundaryLayerExtraction(self, luh, boundaryPosition):
+        out = np.zeros(self._block_size())
+        _check(_lib.exa_fv_boundary_layer_extraction(self._h, _p(out), _p(np.ascontiguousarray(luh, np.float64)),
+                                                     (C.c_int * 3)(*boundaryPosition)))
+        return out
+
+    def boundaryConditions(self, bc, stateIn, centre, size, t, dt, faceIndex, normalNonZero):
+        out = np.zeros(self._block_size())
+        _check(_lib.exa_fv_boundary_conditions(self._h, C.byref(bc), _p(out),
+                                               _p(np.ascontiguousarray(stateIn, np.float64)), _p(_vec3(centre)),
+                                               _p(_vec3(size)), float(t), float(dt), int(faceIndex),
+                                               int(normalNonZero)))
+        return out
+
 
 class FVGrid:
-    """Periodic grid of finite-volume patches resident in HBM."""
+    """Grid of finite-volume patches resident in HBM (periodic unless
+    set_boundary closes an axis)."""
 
     def __init__(self, solver, nx, ny, nz=1, offset=(0.0, 0.0, 0.0), width=(1.0, 1.0, 1.0)):
         self.solver = solver
@@ -642,6 +682,10 @@ class FVGrid:
 
     def set_initial_conditions(self, ic):
         _check(_lib.exa_fv_grid_set_initial_conditions(self._h, C.byref(ic)))
+
+    def set_boundary(self, periodic, bc=None):
+        per = (C.c_int * 3)(*[1 if p else 0 for p in periodic])
+        _check(_lib.exa_fv_grid_set_boundary(self._h, per, None if bc is None else C.byref(bc)))
 
     def compute_dt(self):
         dt = C.c_double(0.0)

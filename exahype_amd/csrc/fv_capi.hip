@@ -215,6 +215,8 @@ struct exa_fv_grid {
   int nx = 1, ny = 1, nz = 1, np = 0;
   double off[3] = {0, 0, 0}, width[3] = {1, 1, 1}, dx[3] = {1, 1, 1};
   DevBuf cur, nxt, nbr, state;
+  int periodic[3] = {1, 1, 1};  // closed axes take the boundary values bc
+  exa_bc bc{};
   hipStream_t stream = nullptr;
   bool dt_valid = false;  // a stable dt has been computed (sync checks it from then on)
   // per-phase kernel timing (0 ghost layers, 1 unused, 2 update, 3 dt)
@@ -247,6 +249,119 @@ __global__ void k_fv_take(double* dt_cur, unsigned long long* dt_next) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// The boundary hooks of one patch (reference layout, nData per subcell).
+static bool fv_bc_ok(const exa_bc* bc) { return bc->kind >= EXA_BC_OUTFLOW && bc->kind <= EXA_BC_EXACT; }
+
+static int fv_hook(exa_fv_solver* s, exa::fv::FvHookArgs& A, double* dst, const double* src, bool dst_is_patch,
+                   bool src_is_patch) {
+  const exa::fv::PatchGeom geo = s->geom();
+  const int nd = s->cfg.nvar + s->cfg.npar;
+  const size_t patch = (size_t)geo.ncells_box() * nd;
+  const size_t block = (size_t)(geo.dim == 3 ? geo.P * geo.P : geo.P) * geo.G * nd;
+  const size_t nsrc = src_is_patch ? patch : block, ndst = dst_is_patch ? patch : block;
+  A.dim = geo.dim;
+  A.P = geo.P;
+  A.G = geo.G;
+  A.nd = nd;
+  A.gamma = s->cfg.gamma;
+  std::lock_guard<std::mutex> lk(s->mu);
+  EXA_CHECK(hipSetDevice(s->device));
+  EXA_TRY(s->a.ensure(sizeof(double) * nsrc));
+  EXA_TRY(s->b.ensure(sizeof(double) * ndst));
+  EXA_CHECK(hipMemcpyAsync(s->a.p, src, sizeof(double) * nsrc, hipMemcpyHostToDevice, s->stream));
+  // the untouched entries of the destination keep the caller's values
+  EXA_CHECK(hipMemcpyAsync(s->b.p, dst, sizeof(double) * ndst, hipMemcpyHostToDevice, s->stream));
+  hipLaunchKernelGGL(exa::fv::k_fv_patch_hook, dim3(1), dim3(256), 0, s->stream, s->b.as<double>(),
+                     s->a.as<double>(), A);
+  EXA_CHECK(hipGetLastError());
+  EXA_CHECK(hipMemcpyAsync(dst, s->b.p, sizeof(double) * ndst, hipMemcpyDeviceToHost, s->stream));
+  EXA_CHECK(hipStreamSynchronize(s->stream));
+  return EXA_OK;
+}
+
+static int fv_position(const exa_fv_solver* s, const int pos[3], int* d, int* side) {
+  int n = 0;
+  for (int k = 0; k < 3; ++k)
+    if (pos[k] != 0) {
+      EXA_ARG(pos[k] == 1 || pos[k] == -1, "positions are -1, 0 or +1");
+      EXA_ARG(k < s->cfg.dim, "position outside the patch's dimensions");
+      *d = k;
+      *side = pos[k] > 0;
+      ++n;
+    }
+  EXA_ARG(n == 1, "exactly one non-zero position entry (a face)");
+  return EXA_OK;
+}
+
+extern "C" int exa_fv_boundary_layer_extraction(exa_fv_solver* s, double* luhbnd, const double* luh,
+                                                const int boundaryPosition[3]) {
+  EXA_ARG(s && luhbnd && luh && boundaryPosition, "null pointer");
+  exa::fv::FvHookArgs A{};
+  A.op = 0;
+  EXA_TRY(fv_position(s, boundaryPosition, &A.d, &A.side));
+  return fv_hook(s, A, luhbnd, luh, false, true);
+}
+
+extern "C" int exa_fv_ghost_layer_filling_at_boundary(exa_fv_solver* s, double* luh, const double* luhbnd,
+                                                      const int boundaryPosition[3]) {
+  EXA_ARG(s && luhbnd && luh && boundaryPosition, "null pointer");
+  exa::fv::FvHookArgs A{};
+  A.op = 1;
+  EXA_TRY(fv_position(s, boundaryPosition, &A.d, &A.side));
+  return fv_hook(s, A, luh, luhbnd, true, false);
+}
+
+extern "C" int exa_fv_ghost_layer_filling(exa_fv_solver* s, double* luh, const double* luhNeighbour,
+                                          const int neighbourPosition[3]) {
+  EXA_ARG(s && luhNeighbour && luh && neighbourPosition, "null pointer");
+  exa::fv::FvHookArgs A{};
+  A.op = 2;
+  EXA_TRY(fv_position(s, neighbourPosition, &A.d, &A.side));
+  return fv_hook(s, A, luh, luhNeighbour, true, true);
+}
+
+extern "C" int exa_fv_boundary_conditions(exa_fv_solver* s, const exa_bc* bc, double* stateOut,
+                                          const double* stateIn, const double centre[3], const double size[3],
+                                          double t, double dt, int faceIndex, int normalNonZero) {
+  (void)dt;
+  EXA_ARG(s && bc && stateOut && stateIn && centre && size, "null pointer");
+  EXA_ARG(fv_bc_ok(bc), "boundary condition kind");
+  EXA_ARG(normalNonZero >= 0 && normalNonZero < s->cfg.dim && faceIndex / 2 == normalNonZero && faceIndex >= 0,
+          "faceIndex must be 2 normalNonZero + 0 or 1");
+  exa::fv::FvHookArgs A{};
+  A.op = 3;
+  A.d = normalNonZero;
+  A.side = faceIndex % 2;
+  A.bc = *bc;
+  for (int k = 0; k < 3; ++k) {
+    A.c[k] = k < s->cfg.dim ? centre[k] : 0.0;
+    A.h[k] = k < s->cfg.dim ? size[k] : 0.0;
+  }
+  A.t = t;
+  return fv_hook(s, A, stateOut, stateIn, false, false);
+}
+
+// Face neighbours nbr[(2d + side) np + p], wrapped on periodic axes and -1
+// across the domain faces of the closed ones.
+static hipError_t fv_grid_neighbours(exa_fv_grid* g) {
+  const int nx = g->nx, ny = g->ny, nz = g->nz;
+  const int n[3] = {nx, ny, nz};
+  std::vector<int> nbr(6 * g->np);
+  for (int p = 0; p < g->np; ++p) {
+    const int c[3] = {p % nx, (p / nx) % ny, p / (nx * ny)};
+    for (int d = 0; d < 3; ++d)
+      for (int side = 0; side < 2; ++side) {
+        int q[3] = {c[0], c[1], c[2]};
+        q[d] += side ? 1 : -1;
+        const bool outside = q[d] < 0 || q[d] >= n[d];
+        q[d] = (q[d] + n[d]) % n[d];
+        nbr[(2 * d + side) * g->np + p] = outside && !g->periodic[d] ? -1 : (q[2] * ny + q[1]) * nx + q[0];
+      }
+  }
+  return hipMemcpy(g->nbr.p, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice);
+}
+
 extern "C" int exa_fv_grid_create(exa_fv_solver* s, int nx, int ny, int nz, const double offset[3],
                                   const double width[3], exa_fv_grid** out) {
   EXA_ARG(s && out && nx > 0 && ny > 0 && nz > 0, "grid dimensions");
@@ -274,18 +389,7 @@ extern "C" int exa_fv_grid_create(exa_fv_solver* s, int nx, int ny, int nz, cons
     delete g;
     return r;
   }
-  std::vector<int> nbr(6 * g->np);
-  for (int p = 0; p < g->np; ++p) {
-    const int px = p % nx, py = (p / nx) % ny, pz = p / (nx * ny);
-    auto id = [&](int x, int y, int z) { return ((z + nz) % nz * ny + (y + ny) % ny) * nx + (x + nx) % nx; };
-    nbr[0 * g->np + p] = id(px - 1, py, pz);
-    nbr[1 * g->np + p] = id(px + 1, py, pz);
-    nbr[2 * g->np + p] = id(px, py - 1, pz);
-    nbr[3 * g->np + p] = id(px, py + 1, pz);
-    nbr[4 * g->np + p] = id(px, py, pz - 1);
-    nbr[5 * g->np + p] = id(px, py, pz + 1);
-  }
-  hipError_t e = hipMemcpy(g->nbr.p, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice);
+  hipError_t e = fv_grid_neighbours(g);
   double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const unsigned long long bb = exa_big_bits();
   std::memcpy(&st[1], &bb, 8);
@@ -307,6 +411,18 @@ static void fv_clear_marks(exa_fv_grid* g) {
   for (auto& m : g->marks)
     for (hipEvent_t e : m) (void)hipEventDestroy(e);
   g->marks.clear();
+}
+
+extern "C" int exa_fv_grid_set_boundary(exa_fv_grid* g, const int periodic[3], const exa_bc* bc) {
+  EXA_ARG(g && periodic, "null pointer");
+  const bool all = periodic[0] && periodic[1] && (periodic[2] || g->s->cfg.dim == 2);
+  if (!all) EXA_ARG(bc && fv_bc_ok(bc), "a grid with a closed axis needs boundary values of a known kind");
+  EXA_CHECK(hipSetDevice(g->s->device));
+  EXA_CHECK(hipStreamSynchronize(g->stream));
+  for (int d = 0; d < 3; ++d) g->periodic[d] = (periodic[d] || d >= g->s->cfg.dim) ? 1 : 0;
+  if (bc) g->bc = *bc;
+  EXA_CHECK(fv_grid_neighbours(g));
+  return EXA_OK;
 }
 
 extern "C" int exa_fv_grid_destroy(exa_fv_grid* g) {
@@ -411,6 +527,16 @@ extern "C" int exa_fv_grid_step(exa_fv_grid* g, int nsteps) {
   EXA_ARG(g && nsteps >= 0, "nsteps");
   EXA_CHECK(hipSetDevice(g->s->device));
   const PatchGeom geo = g->s->geom();
+  exa::fv::FvBoundary fb;
+  fb.bc = g->bc;
+  for (int d = 0; d < 3; ++d) {
+    fb.off[d] = g->off[d];
+    fb.h[d] = g->dx[d];
+  }
+  fb.nx = g->nx;
+  fb.ny = g->ny;
+  fb.time = g->time();
+  fb.gamma = g->s->cfg.gamma;
   for (int i = 0; i < nsteps; ++i) {
     hipEvent_t m[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     if (g->timing) {
@@ -420,10 +546,10 @@ extern "C" int exa_fv_grid_step(exa_fv_grid* g, int nsteps) {
     }
     if (g->s->cfg.dim == 3)
       hipLaunchKernelGGL(exa::fv::k_fv_ghosts<3>, dim3(g->np), dim3(256), 0, g->stream, g->cur.as<double>(),
-                         g->nbr.as<int>(), geo, g->np);
+                         g->nbr.as<int>(), geo, g->np, fb);
     else
       hipLaunchKernelGGL(exa::fv::k_fv_ghosts<2>, dim3(g->np), dim3(256), 0, g->stream, g->cur.as<double>(),
-                         g->nbr.as<int>(), geo, g->np);
+                         g->nbr.as<int>(), geo, g->np, fb);
     EXA_CHECK(hipGetLastError());
     hipLaunchKernelGGL(k_fv_reset, dim3(1), dim3(1), 0, g->stream, g->adm());
     if (m[1]) EXA_CHECK(hipEventRecord(m[1], g->stream));

@@ -319,12 +319,79 @@ __global__ __launch_bounds__(256) void k_fv_stable_dt(const double* __restrict__
   }
 }
 
-// Face ghost layers from the neighbours' interiors (commons.cpph:337-402,
-// :876-982).  nbr[(2*d + side)*npatch + p]: neighbour of patch p on side
-// (0: -, 1: +) of direction d.
+// ---------------------------------------------------------------------------
+// Boundary hooks of a patch (commons.cpph 2D :99-403, 3D :503-982).  Face
+// blocks hold (layer k, slower transverse a, faster transverse b): 3D
+// idx_bnd_x(P,P,G), idx_bnd_y(P,G,P), idx_bnd_z(G,P,P); 2D idx_bnd_x(P,G),
+// idx_bnd_y(G,P).
+__host__ __device__ inline int fv_bnd(int dim, int d, int P, int G, int k, int a, int b) {
+  if (dim == 2) return d == 0 ? b * G + k : k * P + b;
+  if (d == 0) return (a * P + b) * G + k;
+  if (d == 1) return (a * G + k) * P + b;
+  return (k * P + a) * P + b;
+}
+
+// the transverse axes of direction d: o1 faster, o2 slower (2 = none in 2D)
+__host__ __device__ inline void fv_axes(int dim, int d, int& o1, int& o2) {
+  o1 = dim == 2 ? 1 - d : (d == 0 ? 1 : 0);
+  o2 = dim == 2 ? 2 : (d == 2 ? 1 : 2);
+}
+
+// Layer of the patch that boundaryLayerExtraction puts at layer k of the
+// face block (side 1: patchEnd-G+k; side 0: patchBegin+k, but the 3D y and
+// z faces read patchBegin for every k, commons.cpph:561, :591).
+__host__ __device__ inline int fv_extract_layer(int dim, int d, int side, int P, int G, int k) {
+  return side ? P + k : (dim == 3 && d > 0 ? G : G + k);
+}
+
+// boundaryValues of the Euler FV solvers at an outside subcell centre:
+// outflow copies, a reflecting wall negates the normal momentum, exact data
+// give the state of bc.exact at (x, t).
+__device__ inline void fv_boundary_values(const exa_bc& bc, const double x[3], double t, int d, double gamma,
+                                          const double (&qin)[V], double (&qout)[V]) {
+  if (bc.kind == EXA_BC_EXACT) {
+    exa_ic_point(bc.exact, gamma, x, t, qout);
+    return;
+  }
+#pragma unroll
+  for (int v = 0; v < V; ++v) qout[v] = qin[v];
+  if (bc.kind == EXA_BC_REFLECTING) qout[1 + d] = -qin[1 + d];
+}
+
+// The outside subcell centre of boundaryConditions for layer k (0 the
+// outermost) at transverse (a, b) of face (d, side) of a cell.
+__device__ inline void fv_outside_centre(int dim, int d, int side, int P, int G, int k, int a, int b,
+                                         const double c[3], const double h[3], double x[3]) {
+  int o1, o2;
+  fv_axes(dim, d, o1, o2);
+  const int K = G - k - 1;  // mirror
+  x[0] = x[1] = x[2] = 0.0;
+  x[d] = side ? c[d] + 0.5 * h[d] * (1.0 + k * 1.0 / (2.0 * P)) : c[d] - 0.5 * h[d] * (1.0 + K * 1.0 / (2.0 * P));
+  const double qa = (2.0 * a + 1.0) / (2.0 * P), qb = (2.0 * b + 1.0) / (2.0 * P);
+  if (dim == 3) x[o2] = c[o2] + h[o2] * (qa - 0.5);
+  // the 3D x+ face scales its y coordinate by size[2] (commons.cpph:780)
+  x[o1] = c[o1] + h[(dim == 3 && d == 0 && side) ? 2 : o1] * (qb - 0.5);
+}
+
+// Domain boundaries of a grid: the boundary values, the cell geometry of
+// every patch and the device-held time stamp of the step.
+struct FvBoundary {
+  exa_bc bc;
+  double off[3], h[3];
+  int nx, ny;
+  const double* time;
+  double gamma;
+};
+
+// Face ghost layers: from the neighbours' interiors (ghostLayerFilling,
+// commons.cpph:337-402, :876-982), or, where a closed axis leaves no
+// neighbour (nbr < 0), as mergeWithBoundaryData fills them
+// (E/solvers/FiniteVolumesSolver.cpp:689-728): the mirrored layer of the
+// boundary-layer extraction through boundaryValues.  nbr[(2*d + side)*npatch
+// + p]: neighbour of patch p on side (0: -, 1: +) of direction d.
 template <int DIM>
 __global__ __launch_bounds__(256) void k_fv_ghosts(double* __restrict__ luh, const int* __restrict__ nbr,
-                                                   PatchGeom g, int npatch) {
+                                                   PatchGeom g, int npatch, FvBoundary fb) {
   const int p = blockIdx.x;
   const int nbox = g.ncells_box();
   const int trans = DIM == 3 ? g.P * g.P : g.P;
@@ -335,21 +402,88 @@ __global__ __launch_bounds__(256) void k_fv_ghosts(double* __restrict__ luh, con
     const int ds = it / per_side, r = it % per_side;
     const int d = ds / 2, side = ds % 2;
     const int kk = r / trans, t = r % trans;
-    const int a = DIM == 3 ? t / g.P + g.G : 0, b = t % g.P + g.G;
-    const int mine = side ? g.G + g.P + kk : kk;
-    const int theirs = side ? g.G + kk : g.P + kk;
+    const int ta = DIM == 3 ? t / g.P : 0, tb = t % g.P;  // transverse interior indices
+    int o1, o2;
+    fv_axes(DIM, d, o1, o2);
     int ci[3], ni[3];
-    // transverse coordinates: (b, a) fill the two directions other than d
-    const int o1 = d == 0 ? 1 : 0, o2 = d == 2 ? 1 : 2;
-    ci[d] = mine;
-    ni[d] = theirs;
-    ci[o1] = ni[o1] = b;
-    ci[o2] = ni[o2] = a;
-    const double* nb = luh + (size_t)nbr[(2 * d + side) * npatch + p] * V * nbox;
-    const int cidx = box_index(g, ci[0], ci[1], ci[2]);
-    const int nidx = box_index(g, ni[0], ni[1], ni[2]);
+    ci[o1] = ni[o1] = tb + g.G;
+    ci[o2] = ni[o2] = DIM == 3 ? ta + g.G : 0;
+    ci[d] = side ? g.G + g.P + kk : kk;
+    const int q = nbr[(2 * d + side) * npatch + p];
+    if (q >= 0) {
+      ni[d] = side ? g.G + kk : g.P + kk;
+      const double* nb = luh + (size_t)q * V * nbox;
+      const int cidx = box_index(g, ci[0], ci[1], ci[2]);
+      const int nidx = box_index(g, ni[0], ni[1], ni[2]);
 #pragma unroll
-    for (int v = 0; v < V; ++v) me[v * nbox + cidx] = nb[v * nbox + nidx];
+      for (int v = 0; v < V; ++v) me[v * nbox + cidx] = nb[v * nbox + nidx];
+    } else {
+      const int K = g.G - kk - 1;
+      ni[d] = fv_extract_layer(DIM, d, side, g.P, g.G, K);
+      const int px = p % fb.nx, py = (p / fb.nx) % fb.ny, pz = p / (fb.nx * fb.ny);
+      const double c[3] = {fb.off[0] + (px + 0.5) * fb.h[0], fb.off[1] + (py + 0.5) * fb.h[1],
+                           DIM == 3 ? fb.off[2] + (pz + 0.5) * fb.h[2] : 0.0};
+      double x[3], qin[V], qout[V];
+      fv_outside_centre(DIM, d, side, g.P, g.G, kk, ta, tb, c, fb.h, x);
+      load_q(me, nbox, box_index(g, ni[0], ni[1], ni[2]), qin);
+      fv_boundary_values(fb.bc, x, *fb.time, d, fb.gamma, qin, qout);
+      const int cidx = box_index(g, ci[0], ci[1], ci[2]);
+#pragma unroll
+      for (int v = 0; v < V; ++v) me[v * nbox + cidx] = qout[v];
+    }
+  }
+}
+
+// The per-patch hooks on one patch in the reference layout (node-major,
+// nd values per subcell), one thread per (layer, transverse) entry:
+// op 0 boundaryLayerExtraction (dst = luhbnd, src = luh), 1
+// ghostLayerFillingAtBoundary (dst = luh, src = luhbnd), 2 ghostLayerFilling
+// (dst = luh, src = the neighbour's luh), 3 boundaryConditions (dst =
+// stateOut, src = stateIn; face d, side given by faceIndex).
+struct FvHookArgs {
+  int op, dim, d, side, P, G, nd;
+  exa_bc bc;
+  double c[3], h[3], t, gamma;
+};
+
+static __global__ __launch_bounds__(256) void k_fv_patch_hook(double* __restrict__ dst, const double* __restrict__ src,
+                                                       FvHookArgs A) {
+  const int dim = A.dim, P = A.P, G = A.G, S = P + 2 * G, d = A.d, side = A.side, nd = A.nd;
+  const int na = dim == 3 ? P : 1;
+  const int total = na * P * G;
+  int o1, o2;
+  fv_axes(dim, d, o1, o2);
+  for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < total; it += gridDim.x * blockDim.x) {
+    const int k = it % G, b = (it / G) % P, a = it / (G * P);
+    int x[3] = {0, 0, 0};
+    x[o1] = G + b;
+    if (dim == 3) x[o2] = G + a;
+    auto cell = [&](int layer) {
+      int y[3] = {x[0], x[1], x[2]};
+      y[d] = layer;
+      return dim == 3 ? ((size_t)y[2] * S + y[1]) * S + y[0] : (size_t)y[1] * S + y[0];
+    };
+    const size_t bk = (size_t)fv_bnd(dim, d, P, G, k, a, b);
+    if (A.op == 0) {
+      const size_t s = cell(fv_extract_layer(dim, d, side, P, G, k));
+      for (int v = 0; v < nd; ++v) dst[bk * nd + v] = src[s * nd + v];
+    } else if (A.op == 1) {
+      const size_t o = cell(side ? G + P + k : k);
+      for (int v = 0; v < nd; ++v) dst[o * nd + v] = src[bk * nd + v];
+    } else if (A.op == 2) {
+      const size_t o = cell(side ? G + P + k : k), s = cell(side ? G + k : P + k);
+      for (int v = 0; v < nd; ++v) dst[o * nd + v] = src[s * nd + v];
+    } else {
+      const size_t bK = (size_t)fv_bnd(dim, d, P, G, G - k - 1, a, b);
+      double xo[3], qin[V], qout[V];
+      fv_outside_centre(dim, d, side, P, G, k, a, b, A.c, A.h, xo);
+#pragma unroll
+      for (int v = 0; v < V; ++v) qin[v] = src[bK * nd + v];
+      fv_boundary_values(A.bc, xo, A.t, d, A.gamma, qin, qout);
+#pragma unroll
+      for (int v = 0; v < V; ++v) dst[bk * nd + v] = qout[v];
+      for (int v = V; v < nd; ++v) dst[bk * nd + v] = src[bK * nd + v];
+    }
   }
 }
 

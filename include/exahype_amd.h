@@ -413,6 +413,33 @@ int exa_fv_solution_update(exa_fv_solver* s, double* luh_new, const double* luh,
  * kernels/finitevolumes/commons/c/commons.cpph:26-67, :410-463) */
 int exa_fv_stable_time_step_size(exa_fv_solver* s, const double* luh, const double dx[3], double* dt);
 
+/* The boundary hooks of one patch (luh with ghost layers, reference layout
+ * [(z)][y][x][nData]); positions are unit vectors towards the face, faces
+ * blocks hold G layers: 3D x faces [z][y][G], y faces [z][G][x], z faces
+ * [G][y][x]; 2D x faces [y][G], y faces [G][x] (each of nData).
+ * FiniteVolumesSolver::ghostLayerFilling (E/solvers/FiniteVolumesSolver.h:264-267;
+ * kernels/finitevolumes/commons/c/commons.cpph:338-403, :877-982): my ghost
+ * layers from the neighbour's interior layers next to the shared face. */
+int exa_fv_ghost_layer_filling(exa_fv_solver* s, double* luh, const double* luhNeighbour,
+                               const int neighbourPosition[3]);
+/* ::ghostLayerFillingAtBoundary (FiniteVolumesSolver.h:277-280; commons.cpph:267-336, :615-725) */
+int exa_fv_ghost_layer_filling_at_boundary(exa_fv_solver* s, double* luh, const double* luhbnd,
+                                           const int boundaryPosition[3]);
+/* ::boundaryLayerExtraction (FiniteVolumesSolver.h:289-292; commons.cpph:100-173, :503-613),
+ * including the reference's 3D y- and z- faces, which copy layer
+ * patchBegin into every layer of the block (commons.cpph:561, :591). */
+int exa_fv_boundary_layer_extraction(exa_fv_solver* s, double* luhbnd, const double* luh,
+                                     const int boundaryPosition[3]);
+/* ::boundaryConditions (FiniteVolumesSolver.h:306-312; commons.cpph:175-265,
+ * :727-875): stateOut layer k = boundaryValues of stateIn layer G-1-k at the
+ * outside subcell centre the reference computes (the 3D x+ face scales its
+ * y coordinate by size[2], commons.cpph:780).  FV boundary values: outflow
+ * copies, reflecting negates the normal momentum, exact gives the state of
+ * bc->exact at (x, t). */
+int exa_fv_boundary_conditions(exa_fv_solver* s, const exa_bc* bc, double* stateOut, const double* stateIn,
+                               const double centre[3], const double size[3], double t, double dt, int faceIndex,
+                               int normalNonZero);
+
 /* FiniteVolumesSolver::riemannSolver (E/solvers/FiniteVolumesSolver.h:333-336;
  * kernels/finitevolumes/riemannsolvers/c/rusanov.cpph:24-156); returns s_max. */
 int exa_fv_riemann_solver(exa_fv_solver* s, double* fL, double* fR, const double* qL, const double* qR,
@@ -431,6 +458,11 @@ int exa_fv_grid_destroy(exa_fv_grid* g);
 int exa_fv_grid_upload(exa_fv_grid* g, const double* luh);
 int exa_fv_grid_download(exa_fv_grid* g, double* luh);
 int exa_fv_grid_set_initial_conditions(exa_fv_grid* g, const exa_ic* ic);
+/* Closes the axes with periodic[d] == 0: their domain faces are filled as
+ * mergeWithBoundaryData does (E/solvers/FiniteVolumesSolver.cpp:689-728),
+ * boundaryLayerExtraction, boundaryConditions with bc at the patch's time
+ * stamp, ghostLayerFillingAtBoundary. */
+int exa_fv_grid_set_boundary(exa_fv_grid* g, const int periodic[3], const exa_bc* bc);
 int exa_fv_grid_compute_dt(exa_fv_grid* g, double* dt);
 int exa_fv_grid_step(exa_fv_grid* g, int nsteps);
 /* simulated time, dt of the next step, min over the last step's patches of

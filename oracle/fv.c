@@ -302,7 +302,9 @@ double orc_musclhancock_update(const orc_cfg* c, int patch, double* luh_new,
 
 /* ghostLayerFilling, kernels/finitevolumes/commons/c/commons.cpph:337-402 (2D), :876-982 (3D):
  * face ghost layers of width G copied from the neighbour's interior. */
-static void fv_fill_ghosts(const orc_cfg* c, const orc_fv_grid* g) {
+static void fv_fill_boundaries(const orc_cfg* c, const orc_fv_grid* g, double dt);
+
+static void fv_fill_ghosts(const orc_cfg* c, const orc_fv_grid* g, double dt) {
   const int nd = c->nvar + c->npar, P = g->patch, G = g->ghost, S = P + 2 * G;
   const int dim = g->dim;
   const size_t psz = (size_t)(dim == 3 ? S * S * S : S * S) * nd;
@@ -315,6 +317,8 @@ static void fv_fill_ghosts(const orc_cfg* c, const orc_fv_grid* g) {
           for (int side = 0; side < 2; ++side) {
             int qx = px, qy = py, qz = pz;
             const int step = side ? 1 : -1;
+            const int id = d == 0 ? px : (d == 1 ? py : pz), n = d == 0 ? g->nx : (d == 1 ? g->ny : nz);
+            if (!g->periodic[d] && id == (side ? n - 1 : 0)) continue; /* a domain face: fv_fill_boundaries */
             if (d == 0) qx = (px + step + g->nx) % g->nx;
             if (d == 1) qy = (py + step + g->ny) % g->ny;
             if (d == 2) qz = (pz + step + nz) % nz;
@@ -345,6 +349,166 @@ static void fv_fill_ghosts(const orc_cfg* c, const orc_fv_grid* g) {
       }
 }
 
+/* ---- boundary hooks of one patch (commons.cpph 2D :99-335, 3D :503-982) ---- */
+
+/* subcell (x, y, z) of a patch of S = P + 2G subcells per axis */
+static size_t fv_cell(int dim, int S, const int x[3]) {
+  return dim == 3 ? ((size_t)x[2] * S + x[1]) * S + x[0] : (size_t)x[1] * S + x[0];
+}
+
+/* entry (layer k, slower transverse a, faster transverse b) of the face block
+ * of direction d: 3D idx_bnd_x(P,P,G), idx_bnd_y(P,G,P), idx_bnd_z(G,P,P);
+ * 2D idx_bnd_x(P,G), idx_bnd_y(G,P) */
+static size_t fv_bnd(int dim, int d, int P, int G, int k, int a, int b) {
+  if (dim == 2) return d == 0 ? (size_t)b * G + k : (size_t)k * P + b;
+  if (d == 0) return ((size_t)a * P + b) * G + k;
+  if (d == 1) return ((size_t)a * G + k) * P + b;
+  return ((size_t)k * P + a) * P + b;
+}
+
+/* the transverse axes of direction d: o1 faster, o2 slower (3D) */
+static void fv_axes(int dim, int d, int* o1, int* o2) {
+  if (dim == 2) { *o1 = 1 - d; *o2 = 2; return; }
+  *o1 = d == 0 ? 1 : 0;
+  *o2 = d == 2 ? 1 : 2;
+}
+
+static int fv_face_dir(int dim, const int pos[3]) {
+  for (int d = 0; d < dim; ++d)
+    if (pos[d] != 0) return d;
+  return -1;
+}
+
+void orc_fv_boundary_layer_extraction(const orc_cfg* c, int dim, int P, int G, double* luhbnd, const double* luh,
+                                      const int pos[3]) {
+  const int nd = c->nvar + c->npar, S = P + 2 * G, d = fv_face_dir(dim, pos);
+  int o1, o2;
+  if (d < 0) return;
+  fv_axes(dim, d, &o1, &o2);
+  const int na = dim == 3 ? P : 1;
+  for (int a = 0; a < na; ++a)
+    for (int b = 0; b < P; ++b)
+      for (int k = 0; k < G; ++k) {
+        int x[3] = {0, 0, 0};
+        x[o1] = G + b;
+        if (dim == 3) x[o2] = G + a;
+        /* + side: patchEnd-G+k; - side: patchBegin+k, except that the 3D y
+         * and z faces read layer patchBegin for every k (commons.cpph:561, :591) */
+        x[d] = pos[d] > 0 ? P + k : (dim == 3 && d > 0 ? G : G + k);
+        memcpy(luhbnd + fv_bnd(dim, d, P, G, k, a, b) * nd, luh + fv_cell(dim, S, x) * nd, nd * sizeof(double));
+      }
+}
+
+void orc_fv_ghost_layer_filling_at_boundary(const orc_cfg* c, int dim, int P, int G, double* luh, const double* luhbnd,
+                                            const int pos[3]) {
+  const int nd = c->nvar + c->npar, S = P + 2 * G, d = fv_face_dir(dim, pos);
+  int o1, o2;
+  if (d < 0) return;
+  fv_axes(dim, d, &o1, &o2);
+  const int na = dim == 3 ? P : 1;
+  for (int a = 0; a < na; ++a)
+    for (int b = 0; b < P; ++b)
+      for (int k = 0; k < G; ++k) {
+        int x[3] = {0, 0, 0};
+        x[o1] = G + b;
+        if (dim == 3) x[o2] = G + a;
+        x[d] = pos[d] > 0 ? G + P + k : k;
+        memcpy(luh + fv_cell(dim, S, x) * nd, luhbnd + fv_bnd(dim, d, P, G, k, a, b) * nd, nd * sizeof(double));
+      }
+}
+
+void orc_fv_ghost_layer_filling(const orc_cfg* c, int dim, int P, int G, double* luh, const double* luhNb,
+                                const int pos[3]) {
+  const int nd = c->nvar + c->npar, S = P + 2 * G, d = fv_face_dir(dim, pos);
+  int o1, o2;
+  if (d < 0) return;
+  fv_axes(dim, d, &o1, &o2);
+  const int na = dim == 3 ? P : 1;
+  for (int a = 0; a < na; ++a)
+    for (int b = 0; b < P; ++b)
+      for (int k = 0; k < G; ++k) {
+        int x[3] = {0, 0, 0}, y[3];
+        x[o1] = G + b;
+        if (dim == 3) x[o2] = G + a;
+        memcpy(y, x, sizeof(x));
+        x[d] = pos[d] > 0 ? G + P + k : k; /* my ghost */
+        y[d] = pos[d] > 0 ? G + k : P + k; /* the neighbour's patchBegin+k | patchEnd-G+k */
+        memcpy(luh + fv_cell(dim, S, x) * nd, luhNb + fv_cell(dim, S, y) * nd, nd * sizeof(double));
+      }
+}
+
+void orc_fv_boundary_values(const orc_cfg* c, const orc_bc* bc, const double x[3], double t, double dt,
+                            int faceIndex, int d, const double* Qin, double* Qout) {
+  const int nd = c->nvar + c->npar;
+  (void)dt;
+  (void)faceIndex;
+  if (bc->kind == 3) {
+    orc_ic_point(c, &bc->ic, x, t, Qout);
+    for (int v = c->nvar; v < nd; ++v) Qout[v] = Qin[v];
+    return;
+  }
+  for (int v = 0; v < nd; ++v) Qout[v] = Qin[v];
+  if (bc->kind == 2) Qout[1 + d] = -Qin[1 + d];
+}
+
+void orc_fv_boundary_conditions(const orc_cfg* c, const orc_bc* bc, int dim, int P, int G, double* stateOut,
+                                const double* stateIn, const double centre[3], const double size[3], double t,
+                                double dt, int faceIndex, int normalNonZero) {
+  const int nd = c->nvar + c->npar, side = faceIndex % 2, d = (faceIndex - side) / 2;
+  int o1, o2;
+  fv_axes(dim, d, &o1, &o2);
+  const int na = dim == 3 ? P : 1;
+  for (int k = 0; k < G; ++k) {
+    const int K = G - k - 1; /* mirror */
+    double x[3] = {0.0, 0.0, 0.0};
+    /* the outside subcell centre, from K on the lower and k on the upper side */
+    x[d] = side ? centre[d] + 0.5 * size[d] * (1.0 + k * 1.0 / (2.0 * P))
+                : centre[d] - 0.5 * size[d] * (1.0 + K * 1.0 / (2.0 * P));
+    for (int a = 0; a < na; ++a)
+      for (int b = 0; b < P; ++b) {
+        const double qa = (2.0 * a + 1.0) / (2.0 * P), qb = (2.0 * b + 1.0) / (2.0 * P);
+        if (dim == 3) x[o2] = centre[o2] + size[o2] * (qa - 0.5);
+        /* the 3D x+ face scales its y coordinate by size[2] (commons.cpph:780) */
+        x[o1] = centre[o1] + size[(dim == 3 && d == 0 && side) ? 2 : o1] * (qb - 0.5);
+        orc_fv_boundary_values(c, bc, x, t, dt, faceIndex, normalNonZero,
+                               stateIn + fv_bnd(dim, d, P, G, K, a, b) * nd,
+                               stateOut + fv_bnd(dim, d, P, G, k, a, b) * nd);
+      }
+  }
+}
+
+/* the domain faces of the closed axes, as mergeWithBoundaryData fills them
+ * (E/solvers/FiniteVolumesSolver.cpp:689-728): boundaryLayerExtraction,
+ * boundaryConditions at the patch's time stamp and dt, then
+ * ghostLayerFillingAtBoundary */
+static void fv_fill_boundaries(const orc_cfg* c, const orc_fv_grid* g, double dt) {
+  const int nd = c->nvar + c->npar, P = g->patch, G = g->ghost, S = P + 2 * G, dim = g->dim;
+  const size_t psz = (size_t)(dim == 3 ? S * S * S : S * S) * nd;
+  const int nz = dim == 3 ? g->nz : 1;
+  const double h[3] = {g->width[0] / g->nx, g->width[1] / g->ny, g->width[2] / g->nz};
+  double* bin = (double*)malloc((size_t)P * P * G * nd * sizeof(double));
+  double* bout = (double*)malloc((size_t)P * P * G * nd * sizeof(double));
+  for (int pz = 0; pz < nz; ++pz)
+    for (int py = 0; py < g->ny; ++py)
+      for (int px = 0; px < g->nx; ++px) {
+        double* me = g->luh + (((size_t)pz * g->ny + py) * g->nx + px) * psz;
+        const int id[3] = {px, py, pz}, n[3] = {g->nx, g->ny, nz};
+        const double centre[3] = {g->offset[0] + (px + 0.5) * h[0], g->offset[1] + (py + 0.5) * h[1],
+                                  dim == 3 ? g->offset[2] + (pz + 0.5) * h[2] : 0.0};
+        for (int d = 0; d < dim; ++d)
+          for (int side = 0; side < 2; ++side) {
+            if (g->periodic[d] || id[d] != (side ? n[d] - 1 : 0)) continue;
+            int pos[3] = {0, 0, 0};
+            pos[d] = side ? 1 : -1;
+            orc_fv_boundary_layer_extraction(c, dim, P, G, bin, me, pos);
+            orc_fv_boundary_conditions(c, &g->bc, dim, P, G, bout, bin, centre, h, g->t, dt, 2 * d + side, d);
+            orc_fv_ghost_layer_filling_at_boundary(c, dim, P, G, me, bout, pos);
+          }
+      }
+  free(bin);
+  free(bout);
+}
+
 int orc_fv_grid_alloc(const orc_cfg* c, orc_fv_grid* g, int dim, int scheme, int patch, int nx, int ny, int nz) {
   memset(g, 0, sizeof(*g));
   g->dim = dim;
@@ -352,7 +516,10 @@ int orc_fv_grid_alloc(const orc_cfg* c, orc_fv_grid* g, int dim, int scheme, int
   g->patch = patch;
   g->ghost = scheme == 1 ? 2 : 1;
   g->nx = nx; g->ny = ny; g->nz = dim == 3 ? nz : 1;
-  for (int d = 0; d < 3; ++d) g->width[d] = 1.0;
+  for (int d = 0; d < 3; ++d) {
+    g->width[d] = 1.0;
+    g->periodic[d] = 1;
+  }
   const int S = patch + 2 * g->ghost, nd = c->nvar + c->npar;
   const size_t psz = (size_t)(dim == 3 ? S * S * S : S * S) * nd;
   const size_t np = (size_t)nx * ny * g->nz;
@@ -433,7 +600,8 @@ double orc_fv_grid_step(const orc_cfg* c, orc_fv_grid* g, double dt, double* dt_
   const size_t psz = (size_t)(g->dim == 3 ? S * S * S : S * S) * nd;
   const int np = g->nx * g->ny * g->nz;
   const double dx[3] = {g->width[0] / g->nx, g->width[1] / g->ny, g->width[2] / g->nz};
-  fv_fill_ghosts(c, g);
+  fv_fill_ghosts(c, g, dt);
+  fv_fill_boundaries(c, g, dt);
   memcpy(g->luh_new, g->luh, np * psz * sizeof(double));
   double adm = DBL_MAX;
 #pragma omp parallel for num_threads(nthreads) reduction(min : adm) schedule(dynamic, 4)
@@ -450,5 +618,6 @@ double orc_fv_grid_step(const orc_cfg* c, orc_fv_grid* g, double dt, double* dt_
   g->luh = g->luh_new;
   g->luh_new = t;
   if (dt_admissible) *dt_admissible = adm;
+  g->t += dt;
   return orc_fv_grid_stable_dt(c, g, nthreads);
 }

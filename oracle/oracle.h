@@ -180,9 +180,30 @@ typedef struct {
   double offset[3], width[3];
   double* luh;      /* [patch][(z)][y][x][nData] including ghosts */
   double* luh_new;
+  int periodic[3];  /* 0: the domain faces of that axis take boundary values */
+  orc_bc bc;
+  double t;         /* time stamp of the patches (advanced by each step)    */
 } orc_fv_grid;
 void orc_fv_patch_ic(const orc_cfg* c, const orc_ic* ic, int P, int G, const double centre[3], const double dx[3],
                      double* luh);
+/* The FV boundary hooks of one patch (P^dim subcells, ghost width G; layouts
+ * of kernels/finitevolumes/commons/c/commons.cpph, 2D :99-335, 3D :503-982):
+ * pos / faceIndex as FiniteVolumesSolver::mergeWithBoundaryData passes them
+ * (E/solvers/FiniteVolumesSolver.cpp:689-728). luhbnd blocks: x faces
+ * [(z)][y][G], y faces [(z)][G][x], z faces [G][y][x], each of nData. */
+void orc_fv_boundary_layer_extraction(const orc_cfg* c, int dim, int P, int G, double* luhbnd, const double* luh,
+                                      const int pos[3]);
+void orc_fv_ghost_layer_filling_at_boundary(const orc_cfg* c, int dim, int P, int G, double* luh, const double* luhbnd,
+                                            const int pos[3]);
+void orc_fv_ghost_layer_filling(const orc_cfg* c, int dim, int P, int G, double* luh, const double* luhNb,
+                                const int pos[3]);
+/* boundaryValues of the Euler FV solvers at an outside subcell centre: kind 1
+ * copy, 2 normal momentum negated, 3 the state of bc->ic at (x, t). */
+void orc_fv_boundary_values(const orc_cfg* c, const orc_bc* bc, const double x[3], double t, double dt,
+                            int faceIndex, int d, const double* Qin, double* Qout);
+void orc_fv_boundary_conditions(const orc_cfg* c, const orc_bc* bc, int dim, int P, int G, double* stateOut,
+                                const double* stateIn, const double centre[3], const double size[3], double t,
+                                double dt, int faceIndex, int normalNonZero);
 int orc_fv_grid_alloc(const orc_cfg* c, orc_fv_grid* g, int dim, int scheme, int patch, int nx, int ny, int nz);
 void orc_fv_grid_free(orc_fv_grid* g);
 void orc_fv_grid_init(const orc_cfg* c, orc_fv_grid* g, const orc_ic* ic);

@@ -50,7 +50,8 @@ class FVGridS(C.Structure):
     _fields_ = [("dim", C.c_int), ("scheme", C.c_int), ("patch", C.c_int), ("ghost", C.c_int),
                 ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
                 ("offset", C.c_double * 3), ("width", C.c_double * 3),
-                ("luh", C.POINTER(C.c_double)), ("luh_new", C.POINTER(C.c_double))]
+                ("luh", C.POINTER(C.c_double)), ("luh_new", C.POINTER(C.c_double)),
+                ("periodic", C.c_int * 3), ("bc", BC), ("t", C.c_double)]
 
 
 _lib = None
@@ -110,6 +111,11 @@ def lib():
         L.orc_boundary_conditions.argtypes = [P(Cfg), P(BC), _D, _D, _D, _D, _D, _D, C.c_double, C.c_double, C.c_int,
                                               C.c_int]
         L.orc_musclhancock_update.restype = C.c_double
+        for fn in ("orc_fv_boundary_layer_extraction", "orc_fv_ghost_layer_filling_at_boundary",
+                   "orc_fv_ghost_layer_filling"):
+            getattr(L, fn).argtypes = [P(Cfg), C.c_int, C.c_int, C.c_int, _D, _D, P(C.c_int)]
+        L.orc_fv_boundary_conditions.argtypes = [P(Cfg), P(BC), C.c_int, C.c_int, C.c_int, _D, _D, _D, _D,
+                                                 C.c_double, C.c_double, C.c_int, C.c_int]
         L.orc_fv_grid_alloc.argtypes = [P(Cfg), P(FVGridS), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_fv_grid_alloc.restype = C.c_int
         L.orc_fv_grid_free.argtypes = [P(FVGridS)]
@@ -424,6 +430,36 @@ def musclhancock_update(c, patch, luh_new, luh, dx, dt, cfl=0.99):
     return luh_new, a
 
 
+def _pos(p):
+    return (C.c_int * 3)(*[int(x) for x in p])
+
+
+def fv_boundary_layer_extraction(c, dim, P, G, luh, pos):
+    """boundaryLayerExtraction: the G layers of luh next to face pos, as a face block."""
+    out = np.zeros(P ** (dim - 1) * G * (c.nvar + c.npar))
+    lib().orc_fv_boundary_layer_extraction(C.byref(c), dim, P, G, ptr(out), ptr(dvec(luh)), _pos(pos))
+    return out
+
+
+def fv_ghost_layer_filling_at_boundary(c, dim, P, G, luh, luhbnd, pos):
+    out = dvec(luh).copy()
+    lib().orc_fv_ghost_layer_filling_at_boundary(C.byref(c), dim, P, G, ptr(out), ptr(dvec(luhbnd)), _pos(pos))
+    return out
+
+
+def fv_ghost_layer_filling(c, dim, P, G, luh, luh_nb, pos):
+    out = dvec(luh).copy()
+    lib().orc_fv_ghost_layer_filling(C.byref(c), dim, P, G, ptr(out), ptr(dvec(luh_nb)), _pos(pos))
+    return out
+
+
+def fv_boundary_conditions(c, bc, dim, P, G, state_in, centre, size, t, dt, face, normal):
+    out = np.zeros(P ** (dim - 1) * G * (c.nvar + c.npar))
+    lib().orc_fv_boundary_conditions(C.byref(c), C.byref(bc), dim, P, G, ptr(out), ptr(dvec(state_in)),
+                                     ptr(dvec(centre)), ptr(dvec(size)), t, dt, face, normal)
+    return out
+
+
 class OracleFVGrid:
     """Periodic grid of FV patches driven by the oracle (scheme 0 Godunov, 1 MUSCL-Hancock)."""
 
@@ -444,6 +480,12 @@ class OracleFVGrid:
 
     def init(self, ic):
         lib().orc_fv_grid_init(C.byref(self.c), C.byref(self.g), C.byref(ic))
+
+    def set_boundary(self, periodic, bc):
+        """periodic[d] = 0: the domain faces of axis d take the boundary values bc."""
+        for d in range(3):
+            self.g.periodic[d] = int(periodic[d])
+        self.g.bc = bc
 
     def luh(self):
         return np.ctypeslib.as_array(self.g.luh, shape=(self.npatch * self.patch_size,))
