@@ -285,12 +285,13 @@ size_t muscl_lds(int P) { return sizeof(double) * V * (size_t)P * P * P; }
 
 // ---- the limiter's starting state (setInitialConditions, then the status
 // after them: admissibility only, LimitingADERDGSolver.cpp:906-930)
-int enqueue_minmax_all(exa_amr_grid* g, double* out) {
+// only_if: a device flag that must be set for any work to be done (null: always)
+int enqueue_minmax_all(exa_amr_grid* g, double* out, const int* only_if = nullptr) {
   const auto* tab = static_cast<const lim::LimTables*>(g->lim_tab);
   return with_order(g->N, [&](auto n) {
     constexpr int NN = decltype(n)::value;
-    hipLaunchKernelGGL(lim::k_cell_min_max_t<NN + 1>, dim3((unsigned)g->nc), dim3(256), 0, g->stream,
-                       g->luh.as<double>(), out, tab, NOBS);
+    hipLaunchKernelGGL(lim::k_cell_min_max_w<NN + 1>, dim3((unsigned)(g->nc * NOBS)), dim3(64), 0, g->stream,
+                       g->luh.as<double>(), out, tab, NOBS, only_if);
   });
 }
 
@@ -542,7 +543,7 @@ int enqueue_step(exa_amr_grid* g, int par, bool timed) {
   EXA_TRY(enqueue_build(g, LIST_DG_PATCH, LIST_DG_PATCH, g->status_old.as<int>(), nullptr));
   EXA_TRY(enqueue_project_fv(g, g->luh.as<double>(), pool_new, LIST_DG_PATCH));
   // after a recomputation the extrema of every cell start afresh
-  EXA_TRY(enqueue_minmax_all(g, g->mm.as<double>()));
+  EXA_TRY(enqueue_minmax_all(g, g->mm.as<double>(), irr));
   EXA_TRY(patch_minmax(LIST_FV_NEW));
   hipLaunchKernelGGL(k_minmax_reset, dim3(blocks_for(nc)), dim3(256), 0, g->stream, g->mm.as<double>(),
                      g->lmm.as<double>(), g->fmin.as<double>(), g->fmax.as<double>(), g->status.as<int>(),

@@ -264,17 +264,19 @@ __global__ void k_flag_and_compact(const double* __restrict__ mm_new, int* __res
 }
 
 int minmax_all(const ExaGridView& v, const LimTables* tab, double* out, int nobs) {
-  const dim3 grid((unsigned)v.nc), block(256);
+  // one 64-lane workgroup per (cell, observable)
+  EXA_ARG(v.nc * nobs < (1LL << 31), "too many cells for one launch");
+  const dim3 grid((unsigned)(v.nc * nobs)), block(64);
   switch (v.order) {
-    case 1: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<2>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
-    case 2: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<3>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
-    case 3: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<4>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
-    case 4: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<5>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
-    case 5: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<6>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
-    case 6: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<7>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
-    case 7: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<8>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
-    case 8: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<9>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
-    case 9: hipLaunchKernelGGL(exa::lim::k_cell_min_max_t<10>, grid, block, 0, v.stream, v.luh, out, tab, nobs); break;
+    case 1: hipLaunchKernelGGL(exa::lim::k_cell_min_max_w<2>, grid, block, 0, v.stream, v.luh, out, tab, nobs, nullptr); break;
+    case 2: hipLaunchKernelGGL(exa::lim::k_cell_min_max_w<3>, grid, block, 0, v.stream, v.luh, out, tab, nobs, nullptr); break;
+    case 3: hipLaunchKernelGGL(exa::lim::k_cell_min_max_w<4>, grid, block, 0, v.stream, v.luh, out, tab, nobs, nullptr); break;
+    case 4: hipLaunchKernelGGL(exa::lim::k_cell_min_max_w<5>, grid, block, 0, v.stream, v.luh, out, tab, nobs, nullptr); break;
+    case 5: hipLaunchKernelGGL(exa::lim::k_cell_min_max_w<6>, grid, block, 0, v.stream, v.luh, out, tab, nobs, nullptr); break;
+    case 6: hipLaunchKernelGGL(exa::lim::k_cell_min_max_w<7>, grid, block, 0, v.stream, v.luh, out, tab, nobs, nullptr); break;
+    case 7: hipLaunchKernelGGL(exa::lim::k_cell_min_max_w<8>, grid, block, 0, v.stream, v.luh, out, tab, nobs, nullptr); break;
+    case 8: hipLaunchKernelGGL(exa::lim::k_cell_min_max_w<9>, grid, block, 0, v.stream, v.luh, out, tab, nobs, nullptr); break;
+    case 9: hipLaunchKernelGGL(exa::lim::k_cell_min_max_w<10>, grid, block, 0, v.stream, v.luh, out, tab, nobs, nullptr); break;
     default: exa_set_error("order %d", v.order); return EXA_ERR_UNSUPPORTED;
   }
   EXA_CHECK(hipGetLastError());

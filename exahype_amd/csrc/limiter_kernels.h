@@ -270,74 +270,110 @@ static __global__ void k_cell_min_max(const double* __restrict__ luh, double* __
   }
 }
 
-// One variable's axis contraction with compile-time extents: in is
-// [N2][N1][N0] (N0 fastest) with extent NIN along AXIS, out has NOUT there.
-template <int N0, int N1, int N2, int AXIS, int NIN, int NOUT, typename MAT>
-__device__ inline void contract1(const double* in, double* out, MAT mat) {
-  constexpr int O0 = AXIS == 0 ? NOUT : N0, O1 = AXIS == 1 ? NOUT : N1, O2 = AXIS == 2 ? NOUT : N2;
-  for (int e = threadIdx.x; e < O0 * O1 * O2; e += blockDim.x) {
-    const int i0 = e % O0, i1 = (e / O0) % O1, i2 = e / (O0 * O1);
-    const int oo = AXIS == 0 ? i0 : (AXIS == 1 ? i1 : i2);
-    double acc = 0.0;
-#pragma unroll
-    for (int k = 0; k < NIN; ++k) {
-      const int j0 = AXIS == 0 ? k : i0, j1 = AXIS == 1 ? k : i1, j2 = AXIS == 2 ? k : i2;
-      acc += in[(j2 * N1 + j1) * N0 + j0] * mat(k, oo);
-    }
-    out[(i2 * O1 + i1) * O0 + i0] = acc;
-  }
-}
-
-// k_cell_min_max with the order as a template argument, one observable at a
-// time (LDS (B^2 L + B L^2 + L^3 + 2 B^3) doubles instead of V times that,
-// so several cells share a CU).  Same values, same out layout.
+// k_cell_min_max with the order as a template argument and one wavefront per
+// (cell, observable): lanes own the x rows, then the y columns, then the z
+// pencils of the three axis contractions, the last of which feeds the
+// extrema without an LDS store; the tables are read through the scalar
+// cache.  The same products accumulated in the same order as k_cell_min_max,
+// the same out layout.  only_if: when not null, nothing is done unless *only_if != 0.
 template <int B>
-__global__ void __launch_bounds__(256) k_cell_min_max_t(const double* __restrict__ luh, double* __restrict__ out,
-                                                        const LimTables* tab, int nobs) {
-  constexpr int L = 2 * B - 1, B3 = B * B * B;
-  __shared__ double a[B * B * L], b[B * L * L], c[L * L * L], g1[B3], g2[B3], red[2 * 4];
-  const int cell = blockIdx.x;
-  const double* u0 = luh + (size_t)cell * V * B3;
-  auto ml = [&](int k, int o) { return tab->u2l[k][o]; };
-  auto mg = [&](int k, int o) { return tab->u2lob[k][o]; };
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-  for (int v = 0; v < nobs; ++v) {
-    const double* u = u0 + v * B3;
-    contract1<B, B, B, 0, B, L>(u, a, ml);
-    contract1<B, B, B, 0, B, B>(u, g1, mg);
-    __syncthreads();
-    contract1<L, B, B, 1, B, L>(a, b, ml);
-    contract1<B, B, B, 1, B, B>(g1, g2, mg);
-    __syncthreads();
-    contract1<L, L, B, 2, B, L>(b, c, ml);
-    contract1<B, B, B, 2, B, B>(g2, g1, mg);
-    __syncthreads();
-    double mn = 1.0e300, mx = -1.0e300;
-    for (int e = threadIdx.x; e < B3; e += blockDim.x) {
-      const double p = u[e], q = g1[e];
-      mn = fmin(mn, fmin(p, q));
-      mx = fmax(mx, fmax(p, q));
+__global__ void __launch_bounds__(64) k_cell_min_max_w(const double* __restrict__ luh, double* __restrict__ out,
+                                                       const LimTables* __restrict__ tab, int nobs,
+                                                       const int* __restrict__ only_if) {
+  constexpr int L = 2 * B - 1, B2 = B * B, B3 = B2 * B;
+  __shared__ double a[B2 * L], b[B * L * L], g1[B3], g2[B3];
+  if (only_if && !*only_if) return;
+  const long long item = blockIdx.x;
+  const long long cell = item / nobs;
+  const int v = (int)(item % nobs);
+  const double* u = luh + ((size_t)cell * V + v) * B3;
+  const int lane = threadIdx.x;
+  double mn = 1.0e300, mx = -1.0e300;
+  // x: rows (z, y); the GL values themselves enter the extrema here
+  for (int r = lane; r < B2; r += 64) {
+    double x[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      x[k] = u[r * B + k];
+      mn = fmin(mn, x[k]);
+      mx = fmax(mx, x[k]);
     }
-    for (int e = threadIdx.x; e < L * L * L; e += blockDim.x) {
-      mn = fmin(mn, c[e]);
-      mx = fmax(mx, c[e]);
+#pragma unroll
+    for (int o = 0; o < L; ++o) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < B; ++k) acc += x[k] * tab->u2l[k][o];
+      a[r * L + o] = acc;
     }
-    mn = wave_min(mn);
-    mx = -wave_min(-mx);
-    if (lane == 0) {
-      red[wid] = mn;
-      red[4 + wid] = mx;
+#pragma unroll
+    for (int o = 0; o < B; ++o) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < B; ++k) acc += x[k] * tab->u2lob[k][o];
+      g1[r * B + o] = acc;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int w = 1; w < nw; ++w) {
-        mn = fmin(mn, red[w]);
-        mx = fmax(mx, red[4 + w]);
-      }
-      out[((size_t)cell * 2 + 0) * nobs + v] = mn;
-      out[((size_t)cell * 2 + 1) * nobs + v] = mx;
+  }
+  __syncthreads();
+  // y: columns (z, x) of a [z][y][L] into b [z][L][L], of g1 into g2
+  for (int c = lane; c < B * L; c += 64) {
+    const int z = c / L, i = c % L;
+    double x[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) x[k] = a[(z * B + k) * L + i];
+#pragma unroll
+    for (int o = 0; o < L; ++o) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < B; ++k) acc += x[k] * tab->u2l[k][o];
+      b[(z * L + o) * L + i] = acc;
     }
-    __syncthreads();
+  }
+  for (int c = lane; c < B2; c += 64) {
+    const int z = c / B, i = c % B;
+    double x[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) x[k] = g1[(z * B + k) * B + i];
+#pragma unroll
+    for (int o = 0; o < B; ++o) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < B; ++k) acc += x[k] * tab->u2lob[k][o];
+      g2[(z * B + o) * B + i] = acc;
+    }
+  }
+  __syncthreads();
+  // z: pencils (y, x); subcell averages and Lobatto values go to the extrema
+  for (int c = lane; c < L * L; c += 64) {
+    double x[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) x[k] = b[k * L * L + c];
+#pragma unroll
+    for (int o = 0; o < L; ++o) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < B; ++k) acc += x[k] * tab->u2l[k][o];
+      mn = fmin(mn, acc);
+      mx = fmax(mx, acc);
+    }
+  }
+  for (int c = lane; c < B2; c += 64) {
+    double x[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) x[k] = g2[k * B2 + c];
+#pragma unroll
+    for (int o = 0; o < B; ++o) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < B; ++k) acc += x[k] * tab->u2lob[k][o];
+      mn = fmin(mn, acc);
+      mx = fmax(mx, acc);
+    }
+  }
+  mn = wave_min(mn);
+  mx = -wave_min(-mx);
+  if (lane == 0) {
+    out[((size_t)cell * 2 + 0) * nobs + v] = mn;
+    out[((size_t)cell * 2 + 1) * nobs + v] = mx;
   }
 }
 
