@@ -243,16 +243,16 @@ def main():
     ncells = n ** 3
     dof = ncells * NVAR * (N + 1) ** 3
 
-    # the FP64 state is the reference of the mixed runs (orders <= 7 only)
+    # the FP64 state is the reference of the mixed runs
     r64 = run_gpu(N, n, args.steps, args.warmup, E.PREC_FP64, args.waves_per_simd, args.contraction,
-                  keep_state=N <= 7)
+                  keep_state=True)
     rate64 = dof * args.steps / r64["wall"]
     mixed = {}
     for name, prec, sweeps in MIXED_VARIANTS:
-        if N > 7:  # orders 8 and 9 run the FP64 predictor only
+        if N > 7 and prec != E.PREC_FP32:  # orders 8 and 9: FP32 sweeps on the staged predictor only
             continue
-        # the staged form is FP64 only: the mixed runs take their tuned form
-        cm = 0 if args.contraction == 4 else args.contraction
+        # the staged form takes FP64 and FP32 sweeps; the matrix-core forms their tuned form
+        cm = 0 if (args.contraction == 4 and prec != E.PREC_FP32) else args.contraction
         r = run_gpu(N, n, args.steps, args.warmup, prec, args.waves_per_simd, cm, sweeps)
         err = error_norms(r["state"], r64["state"], N, ncells)
         l2 = err["l2"]

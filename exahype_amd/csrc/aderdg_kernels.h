@@ -912,12 +912,22 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
 //   then every node owner applies iK1 to its own rhs history, takes the
 //   residual against q and overwrites the history with the new iterate; the
 //   two arrays swap roles between sweeps.
-template <int N, bool DEBUG, bool SRC = false>
+//
+// MIX: mixed precision (EXA_PREC_FP32).  The same sweep first runs on FP32
+// copies -- iterate and rhs as FP32 arrays in the first half of the slot,
+// flux slice and pencil constants as FP32 in the LDS, FP32 tables -- until
+// the residual reaches the FP32 rounding floor of the cell (or 2(N+1)
+// sweeps); the iterate is then widened into the slot's second array and at
+// most fp64_sweeps FP64 sweeps follow (stopping early on the FP64
+// criterion), from which the averages, extrapolation and volume integral
+// come, as in k_predictor's mixed form.
+template <int N, bool DEBUG, bool SRC = false, bool MIX = false>
 __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
     const double* __restrict__ luh, double* __restrict__ lduh, double* __restrict__ qf, double* __restrict__ ff,
     int* __restrict__ iters, const double* __restrict__ dtp, double dx0, double dx1, double dx2, double gamma,
     long long ncells, double* __restrict__ stage, unsigned long long* __restrict__ sweep_count,
-    PredictorDebug dbg, Ens ens) {
+    PredictorDebug dbg, Ens ens, int fp64_sweeps) {
+  static_assert(!(MIX && SRC), "the gravity source runs in the FP64 predictor only");
   using O = Ord<N>;
   constexpr int B = O::B, B2 = O::B2, B3 = O::B3, PS = O::PS, BLK = O::BLK;
   constexpr int NITEMS = 3 * V * B2;
@@ -959,16 +969,147 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
     for (int v = 0; v < V; ++v) u[v] = act ? luh[cell * V * B3 + v * B3 + s] : 1.0;
     double* q = slot;
     double* r = slot + SQ;
-#pragma unroll 1
-    for (int t = 0; t < B; ++t) {
-      asm volatile("" : "+s"(tz));
-      const int sl = s + tz;
-      if (act)
-#pragma unroll
-        for (int v = 0; v < V; ++v) q[(t * V + v) * B3 + sl] = u[v];
-    }
-    __syncthreads();
     int it = 0;
+    if constexpr (MIX) {
+      // ---- FP32 phase ----
+      float* qL = reinterpret_cast<float*>(slot);
+      float* rL = qL + SQ;
+      float* sFf = reinterpret_cast<float*>(sm);
+      float* scf = sFf + 3 * V * BLK;  // inside the flux slice's second half: the FP64 constants stay
+      __syncthreads();                 // the FP64 constants are written (pencil_constants above)
+      pencil_constants<N, float>(scf, dt, dx0, dx1, dx2);
+      const float gf = (float)gamma;
+      float uf[V];
+      double um = 0.0;
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        uf[v] = (float)u[v];
+        um = fmax(um, act ? fabs(u[v]) : 0.0);
+      }
+      um = -block_min(-um, red);
+      // rounding floor of sum(dq^2) over the cell in FP32: count * (4 eps |u|max)^2
+      const double floor_lp = (double)(V * O::B4) * (4.0 * 5.96e-8 * um) * (4.0 * 5.96e-8 * um);
+      const double tol_lp = fmax(1e-7 * 1e-7, floor_lp);
+      const float w3f = (float)w3, iw3f = (float)iw3;
+#pragma unroll 1
+      for (int t = 0; t < B; ++t) {
+        asm volatile("" : "+s"(tz));
+        const int sl = s + tz;
+        if (act)
+#pragma unroll
+          for (int v = 0; v < V; ++v) qL[(t * V + v) * B3 + sl] = uf[v];
+      }
+      __syncthreads();
+      bool done_lp = false;
+      while (!done_lp) {
+        ++it;
+#pragma unroll 1
+        for (int t = 0; t < B; ++t) {
+          asm volatile("" : "+s"(tz));
+          const int sl = s + tz;
+          if (act) {
+            float Q[V], F[3][V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) Q[v] = qL[(t * V + v) * B3 + sl];
+            EulerPDE::flux<float>(Q, F, gf);
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+              sFf[(0 * V + v) * BLK + ox] = F[0][v];
+              sFf[(1 * V + v) * BLK + oy] = F[1][v];
+              sFf[(2 * V + v) * BLK + oz] = F[2][v];
+            }
+          }
+          __syncthreads();
+#pragma unroll
+          for (int k = 0; k < ROUNDS; ++k) {
+            asm volatile("" : "+s"(tz));
+            const int i = s + k * O::NT;
+            if (i < NITEMS) {
+              const int dv = i / B2, ab = i - dv * B2, d = dv / V;
+              const float c = scf[(d * B + t) * B2 + ab];
+              float* pf = sFf + dv * BLK + ab * PS;
+              float f[B];
+#pragma unroll
+              for (int n = 0; n < B; ++n) f[n] = c * pf[n];
+#pragma unroll
+              for (int l = 0; l < B; ++l) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int n = 0; n < B; ++n) acc += f[n] * c_tabf[N + tz].Kxi[n][l];
+                pf[l] = acc;
+              }
+            }
+          }
+          __syncthreads();
+          if (act) {
+            const float c0 = w3f * c_tabf[N + tz].F0[t];
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+              const float rr = ((c0 * uf[v] - sFf[(0 * V + v) * BLK + ox]) - sFf[(1 * V + v) * BLK + oy]) -
+                               sFf[(2 * V + v) * BLK + oz];
+              rL[(t * V + v) * B3 + sl] = iw3f * rr;
+            }
+          }
+          __syncthreads();
+        }
+        double res = 0.0;
+        const int sa = act ? s : 0;
+#pragma unroll 1
+        for (int v = 0; v < V; ++v) {
+          asm volatile("" : "+s"(tz));
+          const int sv = sa + tz;
+          float ri[B];
+#pragma unroll
+          for (int t = 0; t < B; ++t) ri[t] = rL[(t * V + v) * B3 + sv];
+#pragma unroll
+          for (int t2 = 0; t2 < B; ++t2) {
+            asm volatile("" : "+s"(tz));
+            float acc = 0.0f;
+#pragma unroll
+            for (int t = 0; t < B; ++t) acc += ri[t] * c_tabf[N + tz].iK1[t2][t];
+            const double dd = (double)(qL[(t2 * V + v) * B3 + sv] - acc);
+            if (act) {
+              res += dd * dd;
+              rL[(t2 * V + v) * B3 + sv] = acc;
+            }
+          }
+        }
+        res = block_sum(res, red);
+        __syncthreads();
+        done_lp = __builtin_amdgcn_readfirstlane((res < tol_lp || it == MaxIterations) ? 1 : 0) != 0;
+        if (!done_lp) {
+          float* tmp = qL;
+          qL = rL;
+          rL = tmp;
+        }
+      }
+      // the FP32 iterate (rL, in the slot's first half) widened into the
+      // second FP64 array, which becomes the input of the first FP64 sweep;
+      // every word is read and written by its own node's thread
+#pragma unroll 1
+      for (int t = 0; t < B; ++t) {
+        asm volatile("" : "+s"(tz));
+        const int sl = s + tz;
+        if (act)
+#pragma unroll
+          for (int v = 0; v < V; ++v) r[(t * V + v) * B3 + sl] = (double)rL[(t * V + v) * B3 + sl];
+      }
+      double* tmp = q;
+      q = r;
+      r = tmp;
+      __syncthreads();  // the flux slice is free again
+    } else {
+#pragma unroll 1
+      for (int t = 0; t < B; ++t) {
+        asm volatile("" : "+s"(tz));
+        const int sl = s + tz;
+        if (act)
+#pragma unroll
+          for (int v = 0; v < V; ++v) q[(t * V + v) * B3 + sl] = u[v];
+      }
+      __syncthreads();
+    }
+    const int max64 = MIX ? it + fp64_sweeps : MaxIterations;
     bool done = false;
     while (!done) {
       ++it;
@@ -1056,7 +1197,7 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
       __syncthreads();  // every thread has read red before the flux slice reuses it
       // block-uniform (every thread holds the same sum); readfirstlane makes
       // that visible to the compiler, which keeps the loops scalar
-      done = __builtin_amdgcn_readfirstlane((res < 1e-7 * 1e-7 || it == MaxIterations) ? 1 : 0) != 0;
+      done = __builtin_amdgcn_readfirstlane((res < 1e-7 * 1e-7 || it == max64) ? 1 : 0) != 0;
       if (!done) {
         double* tmp = q;
         q = r;

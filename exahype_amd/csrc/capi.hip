@@ -269,10 +269,11 @@ static int predictor_form(const exa_aderdg_config& c, int N, int dev) {
 extern "C" int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out) {
   EXA_ARG(cfg && out, "null pointer");
   EXA_ARG(cfg->order >= 1 && cfg->order <= 9, "order must be 1..9");
-  EXA_ARG(cfg->order <= 7 || cfg->precision == EXA_PREC_FP64, "orders 8 and 9 run the FP64 predictor only");
+  EXA_ARG(cfg->order <= 7 || cfg->precision <= EXA_PREC_FP32,
+          "orders 8 and 9 run the staged predictor: FP64 or FP32 sweeps with FP64 correction");
   EXA_ARG(cfg->contraction != EXA_CONTRACT_STAGED ||
-              (cfg->precision == EXA_PREC_FP64 && cfg->order >= EXA_STAGED_MIN_ORDER),
-          "the staged predictor is FP64 and built for orders 5..9");
+              (cfg->precision <= EXA_PREC_FP32 && cfg->order >= EXA_STAGED_MIN_ORDER),
+          "the staged predictor is FP64 or FP32 + FP64 and built for orders 5..9");
   EXA_ARG((cfg->pde == EXA_PDE_EULER || cfg->pde == EXA_PDE_EULER_GRAVITY || exa_pde_generic(cfg->pde)) &&
               cfg->nvar == 5,
           "the compiled PDEs are 3D Euler (optionally with gravity or the test non-conservative product) "
@@ -496,10 +497,13 @@ static int space_time_predictor(exa_solver* s, double* lQhbnd, double* lFhbnd, d
     using O = exa::Ord<NN>;
     if (predictor_staged(s->cfg, s->device)) {
       EXA_TRY(s->stage.ensure(staged_slot_bytes(NN)));
-      hipLaunchKernelGGL(exa_pred_staged_kernel(NN, true, s->cfg.pde == EXA_PDE_EULER_GRAVITY), dim3(1), dim3(O::NT),
+      hipLaunchKernelGGL(exa_pred_staged_kernel(NN, true, s->cfg.pde == EXA_PDE_EULER_GRAVITY,
+                                                s->cfg.precision == EXA_PREC_FP32),
+                         dim3(1), dim3(O::NT),
                          (size_t)exa_pred_staged_lds_bytes(NN), s->stream, s->luh.as<double>(), s->lduh.as<double>(),
                          s->qf.as<double>(), s->ff.as<double>(), s->iters.as<int>(), s->dt.as<double>(), dx[0],
-                         dx[1], dx[2], gamma, 1LL, s->stage.as<double>(), nullptr, dbg, launch_extras(s->cfg));
+                         dx[1], dx[2], gamma, 1LL, s->stage.as<double>(), nullptr, dbg, launch_extras(s->cfg),
+                         s->cfg.fp64_sweeps);
     } else {
       const int form = predictor_form(s->cfg, NN, s->device);
       PredFn kern = s->cfg.pde == EXA_PDE_EULER_GRAVITY ? predictor_kernel_src<NN>(true)
@@ -885,10 +889,13 @@ static int batch_run_predictions(exa_solver* s) {
       int slots = 0;
       EXA_TRY(staged_slots(NN, s->device, n, &slots));
       EXA_TRY(s->b_stage.ensure(staged_slot_bytes(NN) * (size_t)slots));
-      hipLaunchKernelGGL(exa_pred_staged_kernel(NN, false, s->cfg.pde == EXA_PDE_EULER_GRAVITY), dim3((unsigned)slots), dim3(O::NT),
+      hipLaunchKernelGGL(exa_pred_staged_kernel(NN, false, s->cfg.pde == EXA_PDE_EULER_GRAVITY,
+                                                s->cfg.precision == EXA_PREC_FP32),
+                         dim3((unsigned)slots), dim3(O::NT),
                          (size_t)exa_pred_staged_lds_bytes(NN), s->stream, s->b_luh.as<double>(),
                          s->b_lduh.as<double>(), s->b_qf.as<double>(), s->b_ff.as<double>(), s->b_iters.as<int>(),
-                         nullptr, 0.0, 0.0, 0.0, s->cfg.gamma, n, s->b_stage.as<double>(), nullptr, nodbg, ens);
+                         nullptr, 0.0, 0.0, 0.0, s->cfg.gamma, n, s->b_stage.as<double>(), nullptr, nodbg, ens,
+                         s->cfg.fp64_sweeps);
     } else {
       const int wpe = predictor_waves_per_simd(s->cfg, NN, s->device);
       const int form = predictor_form(s->cfg, NN, s->device);
@@ -1350,11 +1357,13 @@ static int enqueue_step(exa_grid* g, bool timed) {
   } else EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
     if (g->staged) {
-      hipLaunchKernelGGL(exa_pred_staged_kernel(NN, false, g->s->cfg.pde == EXA_PDE_EULER_GRAVITY), dim3((unsigned)g->nslots), dim3(O::NT),
+      hipLaunchKernelGGL(exa_pred_staged_kernel(NN, false, g->s->cfg.pde == EXA_PDE_EULER_GRAVITY,
+                                                g->s->cfg.precision == EXA_PREC_FP32),
+                         dim3((unsigned)g->nslots), dim3(O::NT),
                          (size_t)exa_pred_staged_lds_bytes(NN), g->stream, g->luh.as<double>(), g->lduh.as<double>(),
                          g->qf.as<double>(), g->ff.as<double>(), g->iters.as<int>(), g->dt_cur(), g->dx[0],
                          g->dx[1], g->dx[2], g->s->cfg.gamma, g->nc, g->stage.as<double>(), g->iter_sum(), nodbg,
-                         launch_extras(g->s->cfg));
+                         launch_extras(g->s->cfg), g->s->cfg.fp64_sweeps);
     } else {
       const int wpe = predictor_waves_per_simd(g->s->cfg, NN, g->s->device);
       const int form = predictor_form(g->s->cfg, NN, g->s->device);

@@ -38,8 +38,9 @@ EXA_PRED_TU_DECL(7)
 // slot, cells strided by the grid size).
 #define EXA_STAGED_MIN_ORDER 5
 using StagedFn = void (*)(const double*, double*, double*, double*, int*, const double*, double, double, double,
-                          double, long long, double*, unsigned long long*, exa::PredictorDebug, exa::Ens);
-StagedFn exa_pred_staged_kernel(int order, bool debug, bool src = false);  // src: gravity source
+                          double, long long, double*, unsigned long long*, exa::PredictorDebug, exa::Ens, int);
+// src: the gravity source (FP64); mixed: an FP32 phase before the last argument's FP64 sweeps
+StagedFn exa_pred_staged_kernel(int order, bool debug, bool src = false, bool mixed = false);
 int exa_pred_staged_lds_bytes(int order);
 hipError_t exa_pred_staged_init(const DGTables* tab, const DGTablesF* tabf);
 

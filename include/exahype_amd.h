@@ -106,7 +106,7 @@ int exa_selftest_mfma(int kind, const float* A, const float* B, float* D);
 typedef struct exa_solver exa_solver;
 
 typedef struct {
-  int order;        /* N, 1..9 (8, 9: FP64 only)                   */
+  int order;        /* N, 1..9 (8, 9: EXA_PREC_FP64 or EXA_PREC_FP32) */
   int nvar;         /* must be 5 for EXA_PDE_EULER                 */
   int npar;         /* material parameters (passed through)        */
   double cfl;       /* CFL factor (0.9 in the reference)           */
@@ -145,8 +145,10 @@ typedef struct {
 #define EXA_CONTRACT_PENCIL 1   /* pencil owners contract, two barriers per time slice    */
 #define EXA_CONTRACT_NODE 2     /* node owners contract their pencils, one barrier per slice */
 #define EXA_CONTRACT_PENCIL_PAIR 3 /* pencil owners, two slices per phase, contraction in place */
-#define EXA_CONTRACT_STAGED 4   /* space-time iterate staged in global memory (FP64 only,
-                                   orders >= 5; the only form of orders 8 and 9)           */
+#define EXA_CONTRACT_STAGED 4   /* space-time iterate staged in global memory (FP64, or
+                                   EXA_PREC_FP32: an FP32 phase on FP32 staging arrays, then
+                                   FP64 correction sweeps; orders >= 5; the only form of
+                                   orders 8 and 9)                                          */
 
 int exa_aderdg_create(const exa_aderdg_config* cfg, exa_solver** out);
 

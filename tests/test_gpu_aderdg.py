@@ -169,18 +169,19 @@ def test_staged_form_at_lower_orders(N):
     assert rel_err(gr.download(), og.luh()) <= TOL
 
 
-def test_staged_form_rejects_low_orders_and_ensembles():
+def test_staged_form_rejects_low_orders_and_matrix_core_precisions():
     E = exa()
     with pytest.raises(RuntimeError, match="staged"):
         E.ADERDGSolver(3, contraction=E.CONTRACT_STAGED)
     with pytest.raises(RuntimeError, match="staged"):
-        E.ADERDGSolver(5, contraction=E.CONTRACT_STAGED, precision=E.PREC_FP32)
+        E.ADERDGSolver(5, contraction=E.CONTRACT_STAGED, precision=E.PREC_BF16_MFMA)
 
 
-def test_staged_orders_reject_mixed_precision():
+def test_staged_orders_reject_matrix_core_precisions():
     E = exa()
-    with pytest.raises(RuntimeError, match="FP64"):
-        E.ADERDGSolver(9, precision=E.PREC_FP32)
+    with pytest.raises(RuntimeError, match="FP32"):
+        E.ADERDGSolver(9, precision=E.PREC_FP32_MFMA)
+    E.ADERDGSolver(9, precision=E.PREC_FP32)  # FP32 sweeps + FP64 correction are built for 8 and 9
 
 
 @pytest.mark.parametrize("N", [2, 3, 5, 7, 9])

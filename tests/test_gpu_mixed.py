@@ -11,6 +11,8 @@ contraction operands leave a low-precision fixed point only ~1e-3 accurate;
 with two FP64 sweeps its 50-step L2 is 1e-5..1e-4 (three sweeps: ~2e-6), so
 BF16 misses the 1e-6 bar and is tested at 1e-3 (per-cell outputs likewise):
 the measured accuracy of that path, not the acceptance bar of the headline.
+Orders 8 and 9 (and the staged form at 5..7) take FP32 sweeps on FP32
+staging arrays before the FP64 correction, to the same bars.
 """
 import numpy as np
 import pytest
@@ -102,3 +104,43 @@ def test_mixed_50_steps_l2_vs_fp64(N, n, prec, sweeps, l2bar, cellbar):
     assert l2.max() <= l2bar, l2
     # dt follows the state, so the simulated times differ at the solution's level
     assert abs(ta - tb) <= l2bar * ta
+
+
+@pytest.mark.parametrize("N,contraction", [(8, 0), (9, 0), (6, 4)])
+def test_mixed_staged_predictor_vs_oracle(N, contraction):
+    import exahype_amd as E
+    s = E.ADERDGSolver(N, precision=E.PREC_FP32, fp64_sweeps=1, contraction=contraction)
+    c = O.cfg(N)
+    rng = np.random.default_rng(N)
+    ic = O.make_ic(0, **_ic_kw(3))
+    for _ in range(2):
+        centre = rng.uniform(0.3, 0.7, 3)
+        luh = O.solution_adjustment(c, ic, centre, (1 / 27,) * 3)
+        dx = np.full(3, 1 / 27)
+        dt = 0.9 * O.stable_time_step_size(c, luh, dx)
+        g = s.spaceTimePredictor(luh, dx, dt)
+        o = O.space_time_predictor(c, luh, dx, dt)
+        assert g["iters"] >= 2  # at least one FP32 and one FP64 sweep
+        for k in ("lQhbnd", "lFhbnd", "lQhi"):
+            a, b = g[k].reshape(-1, 5), o[k].reshape(-1, 5)
+            err = np.abs(a - b).max() / np.abs(b).max()
+            assert err <= 5e-6, (N, k, err)
+
+
+@pytest.mark.parametrize("N,n,contraction", [(8, 3, 0), (9, 3, 0), (7, 4, 4)])
+def test_mixed_staged_50_steps_l2_vs_fp64(N, n, contraction):
+    import exahype_amd as E
+    print("mixed staged 50 steps N", N, flush=True)
+    out = []
+    for p in (E.PREC_FP64, E.PREC_FP32):
+        s = E.ADERDGSolver(N, precision=p, fp64_sweeps=1, contraction=contraction)
+        g = E.Grid(s, n, n, n)
+        g.set_initial_conditions(E.make_ic(0, **_ic_kw(7)))
+        g.compute_dt()
+        g.step(50)
+        t, _ = g.sync()
+        out.append((g.download(), t))
+    (a, ta), (b, tb) = out
+    l2 = l2_per_variable(b, a, N, n ** 3, 1.0 / n ** 3)
+    assert l2.max() <= 1e-6, l2
+    assert abs(ta - tb) <= 1e-6 * ta
