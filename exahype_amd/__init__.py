@@ -74,7 +74,15 @@ class BoundaryCondition(C.Structure):
     _fields_ = [("kind", C.c_int), ("exact", InitialCondition)]
 
 
-BC_OUTFLOW, BC_REFLECTING, BC_EXACT = 1, 2, 3
+BC_OUTFLOW, BC_REFLECTING, BC_EXACT, BC_USER = 1, 2, 3, 4
+PDE_USER_FIRST = 16
+
+
+def pde_lookup(name):
+    """The id of a PDE registered by a project (exa_pde_register)."""
+    pid = C.c_int(0)
+    _check(load_library().exa_pde_lookup(name.encode(), C.byref(pid)))
+    return pid.value
 
 
 def make_bc(kind, exact=None):
@@ -93,6 +101,7 @@ EXPORTS = [
     "exa_aderdg_space_time_predictor", "exa_aderdg_space_time_predictor_ps", "exa_aderdg_point_source", "exa_aderdg_volume_integral", "exa_aderdg_surface_integral",
     "exa_aderdg_riemann_solver", "exa_aderdg_solution_update", "exa_aderdg_stable_time_step_size",
     "exa_aderdg_adjust_solution", "exa_aderdg_boundary_conditions", "exa_grid_set_boundary",
+    "exa_pde_register", "exa_pde_lookup",
     "exa_aderdg_batch_prediction", "exa_aderdg_batch_riemann",
     "exa_aderdg_batch_update", "exa_aderdg_batch_pending", "exa_aderdg_batch_flush", "exa_grid_create", "exa_grid_destroy", "exa_grid_num_cells",
     "exa_grid_upload", "exa_grid_download", "exa_grid_set_initial_conditions", "exa_grid_sample_cartesian",
@@ -157,6 +166,8 @@ def load_library(path=LIB_PATH):
     L.exa_aderdg_boundary_conditions.argtypes = [C.c_void_p, P(BoundaryCondition), _D, _D, _D, _D, _D, _D,
                                                  C.c_double, C.c_double, I, I]
     L.exa_grid_set_boundary.argtypes = [C.c_void_p, P(I), P(BoundaryCondition)]
+    L.exa_pde_register.argtypes = [C.c_char_p, C.c_void_p, P(I)]
+    L.exa_pde_lookup.argtypes = [C.c_char_p, P(I)]
     L.exa_grid_create.argtypes = [C.c_void_p, I, I, I, _D, _D, P(C.c_void_p)]
     L.exa_grid_destroy.argtypes = [C.c_void_p]
     L.exa_grid_num_cells.argtypes = [C.c_void_p, P(LL)]
@@ -491,7 +502,8 @@ class Grid:
         return out.reshape(self.ncells, self.solver.B ** 3, self.solver.nvar)
 
     def set_initial_conditions(self, ic):
-        _check(_lib.exa_grid_set_initial_conditions(self._h, C.byref(ic)))
+        """ic: an InitialCondition, or None for a registered PDE's adjustPointSolution."""
+        _check(_lib.exa_grid_set_initial_conditions(self._h, None if ic is None else C.byref(ic)))
 
     def set_boundary(self, periodic, bc=None):
         """periodic: three flags; the faces of a non-periodic axis take bc

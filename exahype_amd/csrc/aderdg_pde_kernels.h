@@ -343,6 +343,7 @@ __global__ __launch_bounds__(256) void k_riemann_pde(const double* __restrict__ 
   const int d = (int)(wave / ncells);
   const int cl = (int)(wave % ncells);
   const int cr = nbr[(size_t)d * ncells + cl];
+  if (cr < 0) return;  // a domain face of a non-periodic axis
   const PdeParams P{gamma, {ens.adv[0], ens.adv[1], ens.adv[2]}};
   riemann_face_pde<N, PDE>(qf + ((size_t)cl * 6 + 2 * d + 1) * V * B2, qf + ((size_t)cr * 6 + 2 * d) * V * B2,
                            ff + ((size_t)cl * 6 + 2 * d + 1) * V * B2, ff + ((size_t)cr * 6 + 2 * d) * V * B2, d, P,

@@ -53,6 +53,7 @@ extern "C" const char* exa_version(void) { return "exahype_amd 0.1 gfx950"; }
 // device initialisation: tables to constant memory, large-LDS attributes
 static std::mutex g_init_mu;
 static unsigned long long g_init_mask = 0;
+static DGTables g_host_tab[EXA_MAX_ORDER + 1];  // the DG tables uploaded by exa_init
 
 // Predictor kernels live in the per-order units pred_n<N>.hip.
 template <int N>
@@ -126,8 +127,8 @@ extern "C" int exa_init(int device) {
     exa_set_error("device %d is %s; this library is built for gfx950 only", device, prop.gcnArchName);
     return EXA_ERR_UNSUPPORTED;
   }
-  static DGTables host[EXA_MAX_ORDER + 1];
-  std::memset(host, 0, sizeof(host));
+  DGTables* host = g_host_tab;
+  std::memset(host, 0, sizeof(g_host_tab));
   for (int N = 0; N <= EXA_MAX_ORDER; ++N) {
     for (int i = 0; i < EXA_NB; ++i) {
       host[N].w[i] = EXA_TAB_gl_weights[N][i];
@@ -143,7 +144,7 @@ extern "C" int exa_init(int device) {
       }
     }
   }
-  EXA_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_tab), host, sizeof(host)));
+  EXA_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_tab), host, sizeof(g_host_tab)));
   static DGTablesF hostf[EXA_MAX_ORDER + 1];
   std::memset(hostf, 0, sizeof(hostf));
   for (int N = 0; N <= EXA_MAX_ORDER; ++N)
@@ -165,6 +166,10 @@ extern "C" int exa_init(int device) {
   EXA_CHECK(exa_pred_init_7(host, hostf));
   EXA_CHECK(exa_pred_staged_init(host, hostf));
   EXA_CHECK(exa_pde_init(host));
+  if (exa_pde_user_init_all(host) != 0) {
+    exa_set_error("uploading the DG tables of a registered PDE failed on device %d", device);
+    return EXA_ERR_HIP;
+  }
   EXA_TRY(exa_amr_noalloc_setup(device));
   EXA_CHECK(exa_amr_grid_init(host));
   EXA_CHECK(set_lds_attrs<1>());
@@ -177,6 +182,44 @@ extern "C" int exa_init(int device) {
   EXA_CHECK(set_lds_attrs<8>());
   EXA_CHECK(set_lds_attrs<9>());
   g_init_mask |= (1ull << device);
+  return EXA_OK;
+}
+
+extern "C" int exa_pde_register(const char* name, const exa_pde_kernels* k, int* pde) {
+  EXA_ARG(name && name[0] && k && pde, "null pointer or empty name");
+  EXA_ARG(std::strlen(name) < 64, "PDE names are shorter than 64 characters");
+  EXA_ARG(k->version == EXA_PDE_KERNELS_VERSION, "exa_pde_kernels of another version: rebuild against this header");
+  EXA_ARG(k->init && k->max_order >= 1 && k->max_order <= 7, "kernel table without init or with orders beyond 1..7");
+  for (int n = 1; n <= k->max_order; ++n)
+    EXA_ARG(k->predictor[n][0] && k->predictor[n][1] && k->riemann[n] && k->riemann_list[n] && k->update_dt[n] &&
+                k->update_full[n] && k->initial_conditions[n] && k->boundary_faces[n] && k->boundary_ref[n],
+            "kernel table with missing entries");
+  int dev = 0;
+  EXA_CHECK(hipGetDevice(&dev));
+  EXA_TRY(exa_init(dev));
+  const int slot = exa_pde_user_add(name, k);
+  EXA_ARG(slot >= 0, "every PDE slot is taken (EXA_PDE_USER_SLOTS)");
+  const int id = EXA_PDE_USER_FIRST + slot;
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  for (int d = 0; d < 64; ++d) {
+    if (!(g_init_mask & (1ull << d))) continue;
+    EXA_CHECK(hipSetDevice(d));
+    if (exa_pde_user_init(id, g_host_tab) != 0) {
+      (void)hipSetDevice(dev);
+      exa_set_error("uploading the DG tables of PDE '%s' failed on device %d", name, d);
+      return EXA_ERR_HIP;
+    }
+  }
+  EXA_CHECK(hipSetDevice(dev));
+  *pde = id;
+  return EXA_OK;
+}
+
+extern "C" int exa_pde_lookup(const char* name, int* pde) {
+  EXA_ARG(name && pde, "null pointer");
+  const int slot = exa_pde_user_find(name);
+  EXA_ARG(slot >= 0, "no PDE of that name is registered");
+  *pde = EXA_PDE_USER_FIRST + slot;
   return EXA_OK;
 }
 
@@ -699,6 +742,13 @@ extern "C" int exa_aderdg_riemann_solver(exa_solver* s, double* FL, double* FR, 
 }
 
 static bool bc_needs_euler(const exa_bc* bc) { return bc->kind == EXA_BC_REFLECTING || bc->kind == EXA_BC_EXACT; }
+// Kernels of a registered PDE (exahype_amd_pde.h), by their launch signature.
+using UserIcFn = void (*)(double*, int, int, double, double, double, double, double, double);
+using UserBoundaryFn = void (*)(double*, double*, const int*, int, double*, double*, const double*, const double*,
+                                double, double, double, double, double, double, int, int, double, exa::Ens);
+using UserBoundaryRefFn = void (*)(double*, double*, const double*, const double*, double, double, double, double,
+                                   double, double, double, double, int);
+
 static bool pde_is_euler(int pde) { return pde == EXA_PDE_EULER || pde == EXA_PDE_EULER_GRAVITY; }
 
 extern "C" int exa_aderdg_boundary_conditions(exa_solver* s, const exa_bc* bc, double* FOut, double* QOut,
@@ -708,9 +758,12 @@ extern "C" int exa_aderdg_boundary_conditions(exa_solver* s, const exa_bc* bc, d
   EXA_ARG(s && bc && FOut && QOut && FIn && QIn && centre && size, "null pointer");
   EXA_ARG(normalNonZero >= 0 && normalNonZero < 3, "normalNonZero must be 0, 1 or 2");
   EXA_ARG(faceIndex == 2 * normalNonZero || faceIndex == 2 * normalNonZero + 1, "faceIndex must be 2 normalNonZero + 0 or 1");
-  EXA_ARG(bc->kind >= EXA_BC_OUTFLOW && bc->kind <= EXA_BC_EXACT, "boundary condition kind");
+  const bool user = bc->kind == EXA_BC_USER;
+  EXA_ARG((bc->kind >= EXA_BC_OUTFLOW && bc->kind <= EXA_BC_EXACT) || user, "boundary condition kind");
   EXA_ARG(!bc_needs_euler(bc) || pde_is_euler(s->cfg.pde), "reflecting and exact boundaries need an Euler PDE");
+  EXA_ARG(!user || exa_pde_user(s->cfg.pde), "EXA_BC_USER needs a registered PDE");
   EXA_ARG(s->cfg.nvar == exa::V, "boundary values of 5 variables");
+  EXA_ARG(!user || s->cfg.npar == 0, "a registered PDE carries no parameters");
   std::lock_guard<std::mutex> lk(s->mu);
   EXA_CHECK(hipSetDevice(s->device));
   const int N = s->cfg.order, B2 = (N + 1) * (N + 1), V = exa::V, nd = s->cfg.nvar + s->cfg.npar;
@@ -721,6 +774,17 @@ extern "C" int exa_aderdg_boundary_conditions(exa_solver* s, const exa_bc* bc, d
   double* df = s->ff.as<double>();
   EXA_CHECK(hipMemcpyAsync(dq, QIn, sizeof(double) * nq, hipMemcpyHostToDevice, s->stream));
   EXA_CHECK(hipMemcpyAsync(df, FIn, sizeof(double) * nf, hipMemcpyHostToDevice, s->stream));
+  if (user) {
+    const auto k = reinterpret_cast<UserBoundaryRefFn>(
+        const_cast<void*>(exa_pde_user_kernel(s->cfg.pde, N, EXA_USER_BOUNDARY_REF)));
+    EXA_ARG(k, "the registered PDE has no kernels for this order");
+    hipLaunchKernelGGL(k, dim3(1), dim3(64), 0, s->stream, df + nf, dq + nq, df, dq, centre[0], centre[1], centre[2],
+                       size[0], size[1], size[2], t, dt, faceIndex);
+    EXA_CHECK(hipGetLastError());
+    EXA_CHECK(hipMemcpyAsync(QOut, dq + nq, sizeof(double) * nq, hipMemcpyDeviceToHost, s->stream));
+    EXA_CHECK(hipMemcpyAsync(FOut, df + nf, sizeof(double) * nf, hipMemcpyDeviceToHost, s->stream));
+    return sync_stream(s);
+  }
   EXA_ORDER_SWITCH(N, {
     hipLaunchKernelGGL((exa::k_boundary_conditions_ref<NN>), dim3(1), dim3(64), 0, s->stream, df + nf, dq + nq, df,
                        dq, *bc, s->cfg.gamma, centre[0], centre[1], centre[2], size[0], size[1], size[2], t, dt,
@@ -1219,10 +1283,21 @@ extern "C" int exa_grid_sample_cartesian(exa_grid* g, double* out) {
 }
 
 extern "C" int exa_grid_set_initial_conditions(exa_grid* g, const exa_ic* ic) {
-  EXA_ARG(g && ic, "null pointer");
+  const bool user = g && exa_pde_user(g->s->cfg.pde);
+  EXA_ARG(g && (ic || user), "null pointer (only a registered PDE takes its own initial conditions)");
   EXA_CHECK(hipSetDevice(g->s->device));
   ++g->version;
   const int N = g->s->cfg.order;
+  if (user && !ic) {
+    const auto k = reinterpret_cast<UserIcFn>(const_cast<void*>(exa_pde_user_kernel(g->s->cfg.pde, N, EXA_USER_IC)));
+    EXA_ARG(k, "the registered PDE has no kernels for this order");
+    hipLaunchKernelGGL(k, dim3((unsigned)g->nc), dim3(round_up64((N + 1) * (N + 1) * (N + 1))), 0, g->stream,
+                       g->luh.as<double>(), g->nx, g->ny, g->off[0], g->off[1], g->off[2], g->dx[0], g->dx[1],
+                       g->dx[2]);
+    EXA_CHECK(hipGetLastError());
+    EXA_CHECK(hipStreamSynchronize(g->stream));
+    return EXA_OK;
+  }
   EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
     hipLaunchKernelGGL((exa::k_initial_conditions<NN>), dim3((unsigned)g->nc), dim3(O::NT), 0, g->stream,
@@ -1385,6 +1460,15 @@ static int enqueue_step(exa_grid* g, bool timed) {
                        dim3(blocks), dim3(256), 0, g->stream, g->qf.as<double>(),
                        g->ff.as<double>(), g->nbr.as<int>(), (int)g->nc, g->s->cfg.gamma, 0LL, waves,
                        launch_extras(g->s->cfg));
+    if (g->nb > 0) {  // only a registered PDE has closed axes here
+      const auto k = reinterpret_cast<UserBoundaryFn>(
+          const_cast<void*>(exa_pde_user_kernel(g->s->cfg.pde, N, EXA_USER_BOUNDARY_FACES)));
+      EXA_ARG(k, "the PDE has no boundary kernel for this order");
+      hipLaunchKernelGGL(k, dim3((unsigned)((g->nb * 64LL + 255) / 256)), dim3(256), 0, g->stream,
+                         g->qf.as<double>(), g->ff.as<double>(), g->bfaces.as<int>(), g->nb, g->bq.as<double>(),
+                         g->bf.as<double>(), g->dt_cur(), g->time(), g->off[0], g->off[1], g->off[2], g->dx[0],
+                         g->dx[1], g->dx[2], g->nx, g->ny, g->s->cfg.gamma, launch_extras(g->s->cfg));
+    }
   } else EXA_ORDER_SWITCH(N, {
     hipLaunchKernelGGL((exa::k_riemann<NN>), dim3(blocks), dim3(256), 0, g->stream, g->qf.as<double>(),
                        g->ff.as<double>(), g->nbr.as<int>(), (int)g->nc, g->s->cfg.gamma, 0LL, waves, exa::Ens{});
@@ -1409,9 +1493,13 @@ extern "C" int exa_grid_set_boundary(exa_grid* g, const int periodic[3], const e
   const bool all = periodic[0] && periodic[1] && periodic[2];
   if (!all) {
     EXA_ARG(bc, "a grid with a non-periodic axis needs boundary conditions");
-    EXA_ARG(bc->kind >= EXA_BC_OUTFLOW && bc->kind <= EXA_BC_EXACT, "boundary condition kind");
-    EXA_ARG(pde_is_euler(g->s->cfg.pde) && !g->s->cfg.linear,
-            "domain boundaries of the device grid are implemented for the Euler PDEs");
+    if (exa_pde_user(g->s->cfg.pde)) {
+      EXA_ARG(bc->kind == EXA_BC_USER, "a registered PDE takes its own boundaryValues (EXA_BC_USER)");
+    } else {
+      EXA_ARG(bc->kind >= EXA_BC_OUTFLOW && bc->kind <= EXA_BC_EXACT, "boundary condition kind");
+      EXA_ARG(pde_is_euler(g->s->cfg.pde) && !g->s->cfg.linear,
+              "domain boundaries of the device grid are implemented for the Euler PDEs and registered PDEs");
+    }
   }
   EXA_CHECK(hipSetDevice(g->s->device));
   EXA_CHECK(hipStreamSynchronize(g->stream));

@@ -1,5 +1,8 @@
 // Kernels of the generic PDE arms: linear advection in flux and in
 // non-conservative form, Euler with the test non-conservative product.
+#include <cstring>
+#include <mutex>
+
 #include "aderdg_pde_kernels.h"
 #include "predictor_tu.h"
 
@@ -68,11 +71,96 @@ PdePredFn lin_pred(bool debug, bool ps) {
   }
 }  // namespace
 
+// ---- PDEs registered by a project (exahype_amd_pde.h) ----
+namespace {
+struct UserSlot {
+  bool used = false;
+  char name[64] = {0};
+  exa_pde_kernels k{};
+};
+UserSlot g_user[EXA_PDE_USER_SLOTS];
+std::mutex g_user_mu;
+
+const exa_pde_kernels* user_kernels(int pde, int order) {
+  const int i = pde - EXA_PDE_USER_FIRST;
+  if (i < 0 || i >= EXA_PDE_USER_SLOTS) return nullptr;
+  std::lock_guard<std::mutex> lk(g_user_mu);
+  if (!g_user[i].used || order < 1 || order > g_user[i].k.max_order) return nullptr;
+  return &g_user[i].k;
+}
+}  // namespace
+
+bool exa_pde_user(int pde) {
+  const int i = pde - EXA_PDE_USER_FIRST;
+  if (i < 0 || i >= EXA_PDE_USER_SLOTS) return false;
+  std::lock_guard<std::mutex> lk(g_user_mu);
+  return g_user[i].used;
+}
+
+int exa_pde_user_max_order(int pde) {
+  const int i = pde - EXA_PDE_USER_FIRST;
+  if (i < 0 || i >= EXA_PDE_USER_SLOTS) return 0;
+  std::lock_guard<std::mutex> lk(g_user_mu);
+  return g_user[i].used ? g_user[i].k.max_order : 0;
+}
+
+int exa_pde_user_add(const char* name, const exa_pde_kernels* k) {
+  std::lock_guard<std::mutex> lk(g_user_mu);
+  int free_slot = -1;
+  for (int i = 0; i < EXA_PDE_USER_SLOTS; ++i) {
+    if (g_user[i].used && std::strncmp(g_user[i].name, name, sizeof(g_user[i].name)) == 0) {
+      g_user[i].k = *k;
+      return i;
+    }
+    if (!g_user[i].used && free_slot < 0) free_slot = i;
+  }
+  if (free_slot < 0) return -1;
+  g_user[free_slot].used = true;
+  std::strncpy(g_user[free_slot].name, name, sizeof(g_user[free_slot].name) - 1);
+  g_user[free_slot].k = *k;
+  return free_slot;
+}
+
+int exa_pde_user_find(const char* name) {
+  std::lock_guard<std::mutex> lk(g_user_mu);
+  for (int i = 0; i < EXA_PDE_USER_SLOTS; ++i)
+    if (g_user[i].used && std::strncmp(g_user[i].name, name, sizeof(g_user[i].name)) == 0) return i;
+  return -1;
+}
+
+int exa_pde_user_init_all(const void* tables) {
+  std::lock_guard<std::mutex> lk(g_user_mu);
+  for (int i = 0; i < EXA_PDE_USER_SLOTS; ++i)
+    if (g_user[i].used && g_user[i].k.init(tables) != 0) return -1;
+  return 0;
+}
+
+int exa_pde_user_init(int pde, const void* tables) {
+  const int i = pde - EXA_PDE_USER_FIRST;
+  std::lock_guard<std::mutex> lk(g_user_mu);
+  if (i < 0 || i >= EXA_PDE_USER_SLOTS || !g_user[i].used) return -1;
+  return g_user[i].k.init(tables);
+}
+
+const void* exa_pde_user_kernel(int pde, int order, int which) {
+  const exa_pde_kernels* k = user_kernels(pde, order);
+  if (!k) return nullptr;
+  switch (which) {
+    case EXA_USER_IC: return k->initial_conditions[order];
+    case EXA_USER_BOUNDARY_FACES: return k->boundary_faces[order];
+    case EXA_USER_BOUNDARY_REF: return k->boundary_ref[order];
+    default: return nullptr;
+  }
+}
+
 bool exa_pde_generic(int pde) {
-  return pde == EXA_PDE_ADVECTION || pde == EXA_PDE_ADVECTION_NCP || pde == EXA_PDE_EULER_NCP_TEST;
+  return pde == EXA_PDE_ADVECTION || pde == EXA_PDE_ADVECTION_NCP || pde == EXA_PDE_EULER_NCP_TEST ||
+         exa_pde_user(pde);
 }
 
 PdePredFn exa_pde_predictor(int pde, int order, bool debug) {
+  if (const exa_pde_kernels* k = user_kernels(pde, order))
+    return reinterpret_cast<PdePredFn>(const_cast<void*>(k->predictor[order][debug ? 1 : 0]));
   PdePredFn f = nullptr;
 #define F_(P, N) f = pred<P, N>(debug)
   EXA_PDE_DISPATCH(pde, order, F_);
@@ -94,6 +182,8 @@ int exa_pde_predictor_lds_bytes(int order) {
 }
 
 PdeRiemannFn exa_pde_riemann(int pde, int order) {
+  if (const exa_pde_kernels* k = user_kernels(pde, order))
+    return reinterpret_cast<PdeRiemannFn>(const_cast<void*>(k->riemann[order]));
   PdeRiemannFn f = nullptr;
 #define F_(P, N) f = &exa::k_riemann_pde<N, P>
   EXA_PDE_DISPATCH(pde, order, F_);
@@ -102,6 +192,8 @@ PdeRiemannFn exa_pde_riemann(int pde, int order) {
 }
 
 PdeRiemannListFn exa_pde_riemann_list(int pde, int order) {
+  if (const exa_pde_kernels* k = user_kernels(pde, order))
+    return reinterpret_cast<PdeRiemannListFn>(const_cast<void*>(k->riemann_list[order]));
   PdeRiemannListFn f = nullptr;
 #define F_(P, N) f = &exa::k_riemann_list_pde<N, P>
   EXA_PDE_DISPATCH(pde, order, F_);
@@ -110,6 +202,11 @@ PdeRiemannListFn exa_pde_riemann_list(int pde, int order) {
 }
 
 PdeUpdateFn exa_pde_update(int pde, int order, int mode) {
+  if (const exa_pde_kernels* k = user_kernels(pde, order)) {
+    constexpr int FULL = exa::UPD_SURFACE | exa::UPD_UPDATE | exa::UPD_DT;
+    const void* f = mode == exa::UPD_DT ? k->update_dt[order] : (mode == FULL ? k->update_full[order] : nullptr);
+    return reinterpret_cast<PdeUpdateFn>(const_cast<void*>(f));
+  }
   PdeUpdateFn f = nullptr;
 #define F_(P, N) f = upd<P, N>(mode)
   EXA_PDE_DISPATCH(pde, order, F_);

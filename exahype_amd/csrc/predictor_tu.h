@@ -77,3 +77,14 @@ PdeRiemannFn exa_pde_linear_riemann(int pde, int order);
 PdeUpdateFn exa_pde_linear_update(int pde, int order, int mode);
 LinearVolumeFn exa_pde_linear_volume(int order);
 hipError_t exa_pde_init(const DGTables* tab);
+// Registered PDEs (exahype_amd_pde.h): ids EXA_PDE_USER_FIRST + slot.  Their
+// predictor, Riemann and update kernels come through the exa_pde_* getters
+// above; the rest through exa_pde_user_kernel.
+bool exa_pde_user(int pde);
+int exa_pde_user_max_order(int pde);
+int exa_pde_user_add(const char* name, const exa_pde_kernels* k);  // slot, or -1 when full
+int exa_pde_user_find(const char* name);                            // slot, or -1
+int exa_pde_user_init_all(const void* tables);                      // every slot, current device
+int exa_pde_user_init(int pde, const void* tables);
+enum { EXA_USER_IC = 0, EXA_USER_BOUNDARY_FACES = 1, EXA_USER_BOUNDARY_REF = 2 };
+const void* exa_pde_user_kernel(int pde, int order, int which);

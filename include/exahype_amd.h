@@ -256,6 +256,10 @@ typedef struct {
   exa_ic exact; /* EXA_BC_EXACT only */
 } exa_bc;
 
+/* EXA_BC_USER: the boundaryValues of a PDE registered through
+ * exahype_amd_pde.h (exa_pde_register). */
+#define EXA_BC_USER 4
+
 /* ADERDGSolver::boundaryConditions (E/solvers/ADERDGSolver.h:1061-1070;
  * kernels/aderdg/generic/c/3d/boundaryConditions.cpph:24-85): the outside
  * state QOut [B^2][nData] and normal flux FOut [B^2][nVar] of one domain
@@ -294,6 +298,39 @@ int exa_aderdg_batch_pending(exa_solver* s, long long counts[3]);
 int exa_aderdg_batch_flush(exa_solver* s);
 
 /* ---------------------------------------------------------------------- */
+/* PDEs compiled by a project into the kernels (exahype_amd_pde.h).  No
+ * counterpart in the C ABI of the reference, whose generated solver passes
+ * the user class to the kernel templates (TT/GenericADERDGSolverHeader.template:29-128):
+ * here the project's translation unit instantiates the kernel templates for
+ * its PDE and hands the kernels to the library, which dispatches to them for
+ * the returned PDE id.  Fill the table with exa::register_pde<PDE>() rather
+ * than by hand; the entries are __global__ function pointers of that unit,
+ * indexed by order (1..max_order), and init uploads the DG tables given to it
+ * into the unit's constant memory on the current device. */
+#define EXA_PDE_USER_FIRST 16  /* ids of registered PDEs: EXA_PDE_USER_FIRST + slot */
+#define EXA_PDE_USER_SLOTS 8
+#define EXA_PDE_KERNELS_VERSION 1
+typedef struct {
+  int version;    /* EXA_PDE_KERNELS_VERSION */
+  int max_order;  /* highest order instantiated (<= 7) */
+  int (*init)(const void* tables);
+  const void* predictor[10][2];  /* [order][debug]: FP64 Picard predictor + volume integral */
+  const void* riemann[10];       /* every face of a grid */
+  const void* riemann_list[10];  /* a list of independent faces */
+  const void* update_dt[10];     /* CFL time step */
+  const void* update_full[10];   /* surface integral + update + CFL time step */
+  const void* initial_conditions[10]; /* adjustPointSolution on a grid */
+  const void* boundary_faces[10];     /* boundaryValues + Riemann on a grid's domain faces */
+  const void* boundary_ref[10];       /* boundaryValues of one face, reference layout */
+  int predictor_lds_bytes[10];
+} exa_pde_kernels;
+/* Registers (or, under a name already registered, replaces) a PDE; *pde
+ * receives its id.  Call after exa_init on every device that will run it. */
+int exa_pde_register(const char* name, const exa_pde_kernels* kernels, int* pde);
+/* The id of a registered PDE (EXA_ERR_ARG if the name is unknown). */
+int exa_pde_lookup(const char* name, int* pde);
+
+/* ---------------------------------------------------------------------- */
 /* Device-resident periodic grid. */
 typedef struct exa_grid exa_grid;
 
@@ -305,13 +342,16 @@ int exa_grid_num_cells(const exa_grid* g, long long* n);
  * reference layout [z][y][x][nData]. */
 int exa_grid_upload(exa_grid* g, const double* luh);
 int exa_grid_download(exa_grid* g, double* luh);
+/* ic may be null for a registered PDE: its adjustPointSolution then sets the state. */
 int exa_grid_set_initial_conditions(exa_grid* g, const exa_ic* ic);
 /* Domain boundaries: axis d stays periodic where periodic[d] != 0 (the
  * default); on the two domain faces of every other axis each step calls
  * boundaryConditions with bc and solves the Riemann problem with the
  * outside state on the outer side (applyBoundaryConditions,
- * E/solvers/ADERDGSolver.cpp:2854-2931).  Euler PDEs only (EXA_ERR_ARG
- * otherwise, unless every axis is periodic).  Rebuilds the step's graph. */
+ * E/solvers/ADERDGSolver.cpp:2854-2931).  The Euler PDEs take the
+ * EXA_BC_OUTFLOW..EXA_BC_EXACT kinds, a registered PDE EXA_BC_USER (its
+ * boundaryValues); other PDEs stay periodic (EXA_ERR_ARG).  Rebuilds the
+ * step's graph. */
 int exa_grid_set_boundary(exa_grid* g, const int periodic[3], const exa_bc* bc);
 /* The values a Cartesian-vertices plotter writes
  * (exahype::plotters::ADERDG2CartesianVTK::plotVertexData,
