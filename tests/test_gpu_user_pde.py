@@ -7,8 +7,7 @@ exa_pde_register) -- the example of tests/user_pde/user_pdes.hip:
   boundaryValues) against the oracle: per-cell predictor and face
   boundaryConditions, and a grid closed on every axis over three steps, to
   1e-12 with equal Picard counts;
-* the user advection's exact boundary values on a closed grid track the
-  periodic run of the same periodic solution (size-independent check).
+* the boundary kind of a registered PDE and the built-in kinds do not mix.
 """
 import ctypes as C
 import os
