@@ -1,0 +1,112 @@
+"""Glue generation (exahype_amd/glue.py): the Abstract<Solver>.cpp of a
+specification's solvers with the kernel virtuals routed to the C ABI.
+
+* the handle configuration follows the specification (order / patch size,
+  variables, parameters, kernel family, scheme);
+* every method the toolkit's implementation template defines is defined
+  once, with the template's parameter list (checked against the templates
+  where the reference sources are present);
+* every entry point the glue calls is declared in include/exahype_amd.h.
+"""
+import os
+import re
+
+import pytest
+
+from exahype_amd import glue as G
+from exahype_amd import spec as S
+
+from test_spec import DG_SPEC, FV_SPEC
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TEMPLATES = "/root/reference/Toolkit/src/eu/exahype/solvers/templates"
+
+
+def _defined(text, cls):
+    return re.findall(r"\b(?:void|double|bool|int)\s+\w+::%s::(\w+)\(" % cls, text)
+
+
+def test_aderdg_handle_follows_the_spec():
+    sp = S.Spec(DG_SPEC)
+    text = G.generate(sp, 0)
+    assert "c.order = 5;" in text and "c.nvar = 5;" in text and "c.linear = 0;" in text
+    assert "Vortex::AbstractEulerSolver::spaceTimePredictor(" in text
+    lin = G.generate(sp, 1, precision="fp32", boundary="reflecting")
+    assert "c.order = 3;" in lin and "c.linear = 1;" in lin and "EXA_PREC_FP32" in lin and "EXA_BC_REFLECTING" in lin
+
+
+def test_fv_handle_follows_the_spec():
+    text = G.generate(FV_SPEC, 0)
+    assert "c.patch = 10;" in text and "EXA_FV_GODUNOV" in text
+    assert "EulerPatches::AbstractPatchSolver::ghostLayerFilling(" in text
+    muscl = G.generate(FV_SPEC.replace("generic::godunov", "generic::musclhancock"), 0)
+    assert "EXA_FV_MUSCLHANCOCK" in muscl
+
+
+def test_unsupported_choices_are_errors():
+    with pytest.raises(S.SpecError):
+        G.generate(DG_SPEC, 0, boundary="exact")
+    with pytest.raises(S.SpecError):
+        G.generate(DG_SPEC, 0, precision="bf16")
+
+
+def test_every_method_defined_once():
+    dg = _defined(G.generate(DG_SPEC, 0), "AbstractEulerSolver")
+    fv = _defined(G.generate(FV_SPEC, 0), "AbstractPatchSolver")
+    for names in (dg, fv):
+        # pointSource is overloaded (the kernel and the user function)
+        counts = {n: names.count(n) for n in names}
+        assert all(c == 1 or n == "pointSource" for n, c in counts.items()), counts
+    for n in ("spaceTimePredictor", "solutionUpdate", "volumeIntegral", "surfaceIntegral", "riemannSolver",
+              "boundaryConditions", "stableTimeStepSize", "adjustSolution", "faceUnknownsProlongation",
+              "faceUnknownsRestriction", "volumeUnknownsProlongation", "volumeUnknownsRestriction"):
+        assert n in dg, n
+    for n in ("solutionUpdate", "stableTimeStepSize", "adjustSolution", "boundaryConditions", "ghostLayerFilling",
+              "ghostLayerFillingAtBoundary", "boundaryLayerExtraction", "riemannSolver"):
+        assert n in fv, n
+
+
+def _names(params):
+    """parameter names of a C++ parameter list (template arguments removed)"""
+    params = re.sub(r"<[^<>]*>", "", params)
+    return [p.strip().split()[-1].lstrip("*&") for p in params.split(",") if p.strip()]
+
+
+def _template_methods(path):
+    text = open(path).read()
+    out = {}
+    for m in re.finditer(r"\b(?:void|double|bool|int)\s+\{\{Project\}\}::Abstract\{\{Solver\}\}::(\w+)\(([^)]*)\)",
+                         text):
+        out.setdefault(m.group(1), []).append(_names(m.group(2)))
+    return out
+
+
+@pytest.mark.skipif(not os.path.isdir(TEMPLATES), reason="reference toolkit templates not present")
+@pytest.mark.parametrize("tmpl,spec_text,cls", [
+    ("AbstractGenericADERDGSolverImplementation.template", DG_SPEC, "AbstractEulerSolver"),
+    ("AbstractGenericFiniteVolumesSolverInCImplementation.template", FV_SPEC, "AbstractPatchSolver")])
+def test_methods_and_parameters_match_the_toolkit_templates(tmpl, spec_text, cls):
+    ref = _template_methods(os.path.join(TEMPLATES, tmpl))
+    text = G.generate(spec_text, 0)
+    ours = {}
+    for m in re.finditer(r"\b(?:void|double|bool|int)\s+\w+::%s::(\w+)\(([^)]*)\)" % cls, text):
+        ours.setdefault(m.group(1), []).append(_names(m.group(2)))
+    ref.pop("adjustPatchSolution", None)  # a user hook the glue does not define
+    assert set(ref) <= set(ours), set(ref) - set(ours)
+    for name, sigs in ref.items():
+        for sig in sigs:
+            assert sig in ours[name], (name, sig, ours[name])
+
+
+def test_called_entry_points_are_declared():
+    header = open(os.path.join(ROOT, "include", "exahype_amd.h")).read()
+    for text in (G.generate(DG_SPEC, 0), G.generate(FV_SPEC, 0)):
+        for fn in set(re.findall(r"\b(exa_\w+)\(", text)):
+            if fn in ("exa_ok", "exa_handle", "exa_boundary", "exa_position", "exa_vec"):
+                continue
+            assert re.search(r"\b%s\(" % fn, header), fn
+
+
+def test_write_emits_one_file_per_solver(tmp_path):
+    paths = G.write(DG_SPEC, str(tmp_path))
+    assert sorted(os.path.basename(p) for p in paths) == ["AbstractEulerSolver.cpp", "AbstractLinearSolver.cpp"]
