@@ -8,6 +8,8 @@
   every cell (``vtu::Legendre::vertices``, E/plotters/VTK/ADERDG2LegendreVTK.cpp).
 * ``fv_cells_vtu``: the interior volumes of every finite-volume patch as
   cells with cell data (E/plotters/VTK/FiniteVolumes2VTK.cpp).
+* ``aderdg_cartesian_peano`` / ``write_peano_patch``: Peano patch files (see
+  the section below).
 
 Each cell (patch) writes its own points and sub-cells, as the reference's
 patch writers do.  Cells are numbered x fastest, as the grids store them.
@@ -131,3 +133,103 @@ def fv_cells_vtu(path, fvgrid, time=None, binary=False, name="Q"):
     quads = base[:, None] + np.array([0, 1, m + 1, m])[None, :]
     cells = (np.arange(nx * ny)[:, None, None] * m * m + quads[None, :, :]).reshape(-1, 4)
     write_vtu(path, points, cells, VTK_QUAD, cell_data={name: inner.reshape(-1, 5)}, time=time, binary=binary)
+
+
+# ---------------------------------------------------------------------------
+# Peano patch files (the reference's Peano::Cartesian::{vertices,cells}::ascii
+# plotters, E/plotters/ADERDG2CartesianPeanoPatchFileFormat.cpp:136-372).
+# Those plotters hand every cell to Peano's text patch writer as one patch
+# (its offset and size) with the values at the patch's (N+1)^d equidistant
+# vertices (x fastest) or N^d sub-cell centres, a "Q" dataset of the written
+# unknowns and a "time" dataset holding the cell's time stamp at every
+# vertex / sub-cell.  The writer itself belongs to Peano, which the reference
+# does not vendor; the layout below is a plain-text rendering of that
+# structure: a header naming the format, dimension and patch size, the
+# declaration of every dataset, then one block per patch.
+
+def write_peano_patch(path, offsets, sizes, patch_size, datasets, kind="vertex", dims=3):
+    """Writes a Peano text patch file.  offsets, sizes: (npatch, dims);
+    patch_size: sub-cells per direction; datasets: {name: array of shape
+    (npatch, points per patch, unknowns)} with points = (patch_size+1)^dims
+    (kind "vertex") or patch_size^dims (kind "cell"), x fastest."""
+    offsets, sizes = np.asarray(offsets, float), np.asarray(sizes, float)
+    npatch = offsets.shape[0]
+    npts = (patch_size + 1) ** dims if kind == "vertex" else patch_size ** dims
+    tag = "%s-values" % kind
+    lines = ["# ", "# Peano patch file ", "# Version 0.1 ", "# ", "format ASCII ", "",
+             "dimensions %d " % dims, "patch-size " + " ".join([str(patch_size)] * dims) + " ", ""]
+    data = {}
+    for name, arr in datasets.items():
+        a = np.asarray(arr, float).reshape(npatch, npts, -1)
+        data[name] = a
+        lines += ['begin %s "%s"' % (tag, name), "  number-of-unknowns %d" % a.shape[2], "end %s" % tag, ""]
+    for p in range(npatch):
+        lines.append("begin patch")
+        lines.append("  offset " + " ".join(repr(float(x)) for x in offsets[p]))
+        lines.append("  size " + " ".join(repr(float(x)) for x in sizes[p]))
+        for name, a in data.items():
+            lines.append('  begin %s "%s"' % (tag, name))
+            lines.append("    " + " ".join(repr(float(x)) for x in a[p].ravel()))
+            lines.append("  end %s" % tag)
+        lines.append("end patch")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def read_peano_patch(path):
+    """Parses a file written by write_peano_patch: dict with dims,
+    patch_size, kind, unknowns {name: n}, offsets, sizes and the datasets
+    {name: (npatch, points, unknowns)}."""
+    dims = ps = None
+    kind = None
+    unknowns, offsets, sizes, values = {}, [], [], {}
+    cur, in_patch = None, False
+    with open(path) as f:
+        for raw in f:
+            line = raw.strip()
+            if not line or line.startswith("#"):
+                continue
+            words = line.split()
+            if words[0] == "dimensions":
+                dims = int(words[1])
+            elif words[0] == "patch-size":
+                ps = int(words[1])
+            elif words[0] == "begin" and words[1] == "patch":
+                in_patch = True
+            elif words[0] == "end" and words[1] == "patch":
+                in_patch = False
+            elif words[0] == "begin":
+                kind = words[1].split("-")[0]
+                cur = words[2].strip('"')
+                if in_patch:
+                    values.setdefault(cur, [])
+            elif words[0] == "end":
+                cur = None
+            elif words[0] == "number-of-unknowns":
+                unknowns[cur] = int(words[1])
+            elif words[0] == "offset":
+                offsets.append([float(x) for x in words[1:]])
+            elif words[0] == "size":
+                sizes.append([float(x) for x in words[1:]])
+            elif cur is not None and in_patch:
+                values[cur].append([float(x) for x in words])
+    npts = (ps + 1) ** dims if kind == "vertex" else ps ** dims
+    data = {k: np.asarray(v).reshape(len(v), npts, unknowns[k]) for k, v in values.items()}
+    return dict(dims=dims, patch_size=ps, kind=kind, unknowns=unknowns, offsets=np.asarray(offsets),
+                sizes=np.asarray(sizes), data=data)
+
+
+def aderdg_cartesian_peano(path, grid, time=0.0, name="Q"):
+    """Peano::Cartesian::vertices::ascii plot of an ADER-DG grid: every cell
+    a patch of N^3 sub-cells with the solution at its (N+1)^3 equidistant
+    vertices (sampled on the device) and the time stamp at every vertex."""
+    N = grid.solver.order
+    n = (grid.nx, grid.ny, grid.nz)
+    dx = np.array([grid.width[d] / n[d] for d in range(3)])
+    iz, iy, ix = np.meshgrid(np.arange(n[2]), np.arange(n[1]), np.arange(n[0]), indexing="ij")
+    idx = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], axis=1)
+    offsets = np.asarray(grid.offset)[None, :] + idx * dx[None, :]
+    sizes = np.tile(dx, (len(offsets), 1))
+    q = grid.sample_cartesian()
+    t = np.full(q.shape[:2] + (1,), float(time))
+    write_peano_patch(path, offsets, sizes, N, {name: q, "time": t}, kind="vertex", dims=3)

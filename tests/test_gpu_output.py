@@ -1,6 +1,6 @@
 """Device sampling for the Cartesian-vertices output against the oracle's
 restatement of ADERDG2CartesianVTK::plotVertexData, its exactness on
-polynomials, and VTU files written from live grids."""
+polynomials, and VTU and Peano patch files written from live grids."""
 import numpy as np
 import pytest
 
@@ -70,3 +70,22 @@ def test_fv_cells_vtu(tmp_path):
     assert ncell == 12 * 36 and npts == 12 * 49
     luh = g.download().reshape(12, 8, 8, 5)[:, 1:7, 1:7]
     assert np.array_equal(arr["Q"].reshape(-1, 5), luh.reshape(-1, 5))
+
+
+def test_aderdg_peano_patch_file(tmp_path):
+    """Every cell one patch at its offset and size, its vertex values the
+    device sampling, the time stamp at every vertex."""
+    import exahype_amd as E
+    from exahype_amd import output as OUT
+    N, dims = 3, (3, 2, 2)
+    s = E.ADERDGSolver(N)
+    g = E.Grid(s, *dims, offset=(0.1, 0.2, 0.3), width=(0.6, 0.4, 0.8))
+    g.set_initial_conditions(E.make_ic(4, rho_inf=1.0, p_inf=1.0, amp=0.2, uinf=(0.7, -0.4, 0.3), nmodes=1,
+                                       k=[(1, 2, 1)], a=[1.0], phi=[0.4]))
+    f = tmp_path / "dg.peano-patch-file"
+    OUT.aderdg_cartesian_peano(str(f), g, time=0.125)
+    r = OUT.read_peano_patch(str(f))
+    assert r["dims"] == 3 and r["patch_size"] == N and r["kind"] == "vertex"
+    assert np.array_equal(r["data"]["Q"], g.sample_cartesian())
+    assert (r["data"]["time"] == 0.125).all()
+    assert np.allclose(r["sizes"], [0.2, 0.2, 0.4]) and np.allclose(r["offsets"][4], [0.1 + 0.2, 0.2 + 0.2, 0.3])

@@ -60,3 +60,22 @@ def test_lattice_tiles_each_cell():
         vol += np.prod(hi - lo)
     assert vol == pytest.approx(2 * np.prod(dx))
     assert pts[:, 0].min() == 1.0 and pts[:, 0].max() == pytest.approx(2.0)
+
+
+@pytest.mark.parametrize("kind,dims", [("vertex", 3), ("cell", 2)])
+def test_peano_patch_round_trip(tmp_path, kind, dims):
+    from exahype_amd import output as OUT
+    rng = np.random.default_rng(3)
+    P, npatch = 3, 4
+    npts = (P + 1) ** dims if kind == "vertex" else P ** dims
+    offsets, sizes = rng.uniform(0, 1, (npatch, dims)), rng.uniform(0.1, 0.2, (npatch, dims))
+    q, t = rng.standard_normal((npatch, npts, 5)), np.full((npatch, npts, 1), 0.25)
+    f = tmp_path / "p.peano-patch-file"
+    OUT.write_peano_patch(str(f), offsets, sizes, P, {"Q": q, "time": t}, kind=kind, dims=dims)
+    r = OUT.read_peano_patch(str(f))
+    assert r["dims"] == dims and r["patch_size"] == P and r["kind"] == kind
+    assert r["unknowns"] == {"Q": 5, "time": 1}
+    assert np.array_equal(r["offsets"], offsets) and np.array_equal(r["sizes"], sizes)
+    assert np.array_equal(r["data"]["Q"], q) and np.array_equal(r["data"]["time"], t)
+    head = f.read_text().splitlines()[:8]
+    assert "# Peano patch file " in head and "format ASCII " in head
