@@ -271,6 +271,12 @@ def main():
 
     B = N + 1
     pred_ms = head["times"][0] / max(head["counts"][0], 1)
+    # the fused Riemann + surface integral + update + CFL kernel: reads the
+    # folded predictor output and the cell's own face states and fluxes,
+    # writes the solution (the neighbours' face blocks are the other cells'
+    # own, counted once)
+    tail_ms = head["times"][2] / max(head["counts"][2], 1)
+    tail_bytes = ncells * 8 * 5 * (2 * (N + 1) ** 3 + 12 * (N + 1) ** 2)
     sweeps = head["iters"]
     if mixed_ok:
         # low-precision sweeps at their peak (FP32 vector ALU 157.3 TF/s with
@@ -320,8 +326,13 @@ def main():
                      "kernel": kname, "avg_launch_ms": pred_ms,
                      "picard_sweeps_per_cell_step": sweeps, "flop_per_launch": flops},
         "phase_ms_per_step": {"predictor": pred_ms,
-                              "riemann": head["times"][1] / max(head["counts"][1], 1),
-                              "update_surface_dt": head["times"][2] / max(head["counts"][2], 1)},
+                              "boundary_faces": head["times"][1] / max(head["counts"][1], 1),
+                              "riemann_surface_update_dt": tail_ms},
+        "tail_roofline": {"bound": "hbm", "achieved": tail_bytes / tail_ms * 1e-6, "peak": PEAK_HBM_GBPS,
+                          "unit": "GB/s", "frac": tail_bytes / tail_ms * 1e-6 / PEAK_HBM_GBPS,
+                          "kernel": "k_riemann_update<%d>" % N, "avg_launch_ms": tail_ms,
+                          "bytes_per_launch": tail_bytes,
+                          "bytes_with_neighbour_face_reads": tail_bytes + ncells * 8 * 5 * 12 * (N + 1) ** 2},
         "precisions": dict(
             fp64={"dof_updates_per_s": rate64, "ms_per_step": r64["wall"] / args.steps * 1e3,
                   "predictor_ms": r64["times"][0] / max(r64["counts"][0], 1), "sweeps_per_cell": r64["iters"]},

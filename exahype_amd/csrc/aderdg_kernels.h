@@ -117,6 +117,10 @@ struct Ens {
   double grav;        // gravity g along -z of the Euler-with-gravity PDE; 0: no source
   double adv[3];      // advection velocity of the linear-advection PDEs
   PointSrc ps;        // point source of the linear predictor (mode 0: none)
+  // 1: the Euler predictors write lduh = luh + (dt / w^3) * (volume
+  // integral) -- the update's volume part, for k_riemann_update, which then
+  // needs neither luh nor the bare volume integral; 0: lduh = volume integral
+  int fold_update;
 };
 
 // Algebraic source of Euler under gravity g along -z (the useSource arm of
@@ -881,6 +885,11 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
       out[3] += weight * sh[0];
       out[4] += weight * sh[1];
     }
+    if (ens.fold_update) {
+      const double updateSize = dt / (T.w[z] * T.w[y] * T.w[x]);
+#pragma unroll
+      for (int v = 0; v < V; ++v) out[v] = u[v] + out[v] * updateSize;
+    }
 #pragma unroll
     for (int v = 0; v < V; ++v) lduh[cell * V * B3 + v * B3 + s] = out[v];
   }
@@ -1275,6 +1284,11 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
         out[3] += weight * sh[0];
         out[4] += weight * sh[1];
       }
+      if (ens.fold_update) {
+        const double updateSize = dt / w3;
+#pragma unroll
+        for (int v = 0; v < V; ++v) out[v] = u[v] + out[v] * updateSize;
+      }
 #pragma unroll
       for (int v = 0; v < V; ++v) lduh[cell * V * B3 + v * B3 + s] = out[v];
     }
@@ -1327,12 +1341,13 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_volume_integral_ref(const double
 }
 
 // ---------------------------------------------------------------------------
-// One face, one wavefront (lane = threadIdx.x % 64): Rusanov flux from the
-// face-averaged wave speeds; FL and FR enter holding the extrapolated normal
-// fluxes and both leave holding the numerical flux.
+// One face, one wavefront (lane = threadIdx.x % 64): the Rusanov wave speed
+// of a face, the larger maximum eigenvalue of the two face-averaged states
+// (riemannSolverNonlinear.cpph:52-83; per-lane partial sums, then a
+// wavefront sum).
 template <int N>
-__device__ inline void riemann_face(const double* __restrict__ QL, const double* __restrict__ QR,
-                                    double* __restrict__ FL, double* __restrict__ FR, int d, double gamma, int lane) {
+__device__ inline double riemann_face_speed(const double* __restrict__ QL, const double* __restrict__ QR, int d,
+                                            double gamma, int lane) {
   using O = Ord<N>;
   constexpr int B = O::B, B2 = O::B2;
   constexpr int NPL = (B2 + 63) / 64;
@@ -1359,7 +1374,18 @@ __device__ inline void riemann_face(const double* __restrict__ QL, const double*
   }
   const double sL = EulerPDE::max_abs_eigenvalue(aL, d, gamma);
   const double sR = EulerPDE::max_abs_eigenvalue(aR, d, gamma);
-  const double smax = sL < sR ? sR : sL;
+  return sL < sR ? sR : sL;
+}
+
+// One face, one wavefront: Rusanov flux from the face-averaged wave speeds;
+// FL and FR enter holding the extrapolated normal fluxes and both leave
+// holding the numerical flux.
+template <int N>
+__device__ inline void riemann_face(const double* __restrict__ QL, const double* __restrict__ QR,
+                                    double* __restrict__ FL, double* __restrict__ FR, int d, double gamma, int lane) {
+  constexpr int B2 = Ord<N>::B2;
+  constexpr int NPL = (B2 + 63) / 64;
+  const double smax = riemann_face_speed<N>(QL, QR, d, gamma, lane);
 #pragma unroll
   for (int k = 0; k < NPL; ++k) {
     const int node = lane + 64 * k;
@@ -1647,6 +1673,207 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_update(double* __restrict__ luh,
     dtn = block_min(dtn, red);
     if (s == 0) atomic_min_pos_double(dt_next, dtn);
   }
+}
+
+// Riemann problems of a cell's six faces, surface integral, solution update
+// and CFL minimum in one launch (one workgroup per cell), after a predictor
+// that folded the volume part of the update into lduh (Ens::fold_update:
+// lduh = luh + dt / w^3 * volume integral; the new luh is lduh - dt / w^3 *
+// surface integral, so luh is not read).  Face f = 2d + side
+// belongs to half-wavefront f (32 lanes, nodes lane, lane + 32, ...): its
+// Rusanov flux from this cell's extrapolated state and flux on f and the
+// neighbour's on the opposite face, into LDS, the six faces concurrently.
+// Every interior face is thus solved by both of its cells, with the same
+// operations on the same inputs, so both see the same flux, and the face
+// arrays are only read.  A domain face (no neighbour) takes the flux the
+// boundary kernel left in ff.  The surface integral, update and time step
+// are then k_update's (surfaceIntegralNonlinear.cpph:21-86,
+// solutionUpdate.cpph:23-47, stableTimeStepSize.cpph:21-61).
+// Neighbours: nx > 0 -- a box of nx * ny * nz cells in x-fastest order,
+// wrapped along the axes whose bit is set in periodic, none past the ends of
+// the others (computed, so that every load of the cell is issued at once);
+// nx = 0 -- the table nbr[d * nc + c] (+d), nbr[(3 + d) * nc + c] (-d), -1
+// for none.
+__device__ inline double half_wave_sum(double v) {
+#pragma unroll
+  for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int N>
+struct RiemannUpdate {
+  // at most 256 threads per cell: the faces take six half-wavefronts, and the
+  // nodes of the update loop over the threads, so that high orders keep
+  // several cells in flight per CU
+  static constexpr int NT = Ord<N>::NT > 256 ? 256 : Ord<N>::NT;
+  static constexpr int NIT = (Ord<N>::B3 + NT - 1) / NT;  // update nodes per thread
+};
+
+template <int N>
+__global__ __launch_bounds__(RiemannUpdate<N>::NT) void k_riemann_update(double* __restrict__ luh,
+                                                                         const double* __restrict__ lduh,
+                                                                         const double* __restrict__ qf,
+                                                                         const double* __restrict__ ff,
+                                                                         const int* __restrict__ nbr,
+                                                                         long long ncells, int nx, int ny, int nz,
+                                                                         int periodic,
+                                                                         const double* __restrict__ dtp,
+                                                                         unsigned long long* __restrict__ dt_next,
+                                                                         double dx0, double dx1, double dx2,
+                                                                         double gamma, double cfl, Ens ens) {
+  using O = Ord<N>;
+  constexpr int B = O::B, B2 = O::B2, B3 = O::B3, NT = RiemannUpdate<N>::NT, NIT = RiemannUpdate<N>::NIT;
+  constexpr int NH = NT / 32, NP = (B2 + 31) / 32;
+  __shared__ double sF[6 * V * B2];
+  __shared__ double red[16];
+  const DGTables& T = c_tab[N];
+  const long long cell = blockIdx.x;
+  if (ens.cell_inst) {
+    InstParams& P = ens.inst[ens.cell_inst[cell]];
+    dtp = &P.dt_cur;
+    dt_next = &P.dt_next;
+    dx0 = P.dx[0];
+    dx1 = P.dx[1];
+    dx2 = P.dx[2];
+    gamma = P.gamma;
+  }
+  const int s = threadIdx.x, hw = s >> 5, hl = s & 31;
+  double p0[V];  // one node per thread: loaded before the faces
+  if (NIT == 1 && s < B3) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) p0[v] = lduh[cell * V * B3 + v * B3 + s];
+  }
+  // the two half-waves of a wavefront take faces 2k and 2k + 1 (NH is even),
+  // so both take the same number of faces and every lane of a half-wave is
+  // active at its shuffles
+  for (int f = hw; f < 6; f += NH) {
+    const int d = f >> 1, side = f & 1;
+    long long nb;
+    if (nx > 0) {  // cell indices fit 32 bits: a grid of 2^31 cells would not fit the device
+      const unsigned c = (unsigned)cell;
+      int ix = (int)(c % (unsigned)nx), iy = (int)((c / (unsigned)nx) % (unsigned)ny);
+      int iz = (int)(c / ((unsigned)nx * (unsigned)ny));
+      const int nd = d == 0 ? nx : (d == 1 ? ny : nz);
+      int j = (d == 0 ? ix : (d == 1 ? iy : iz)) + (side ? 1 : -1);
+      bool none = false;
+      if (j < 0 || j >= nd) {
+        if ((periodic >> d) & 1)
+          j = j < 0 ? nd - 1 : 0;
+        else
+          none = true;
+      }
+      if (d == 0)
+        ix = j;
+      else if (d == 1)
+        iy = j;
+      else
+        iz = j;
+      nb = none ? -1 : ((long long)iz * ny + iy) * nx + ix;
+    } else {
+      nb = nbr[(size_t)(side ? d : 3 + d) * ncells + cell];
+    }
+    const double* Qo = qf + ((size_t)cell * 6 + f) * V * B2;
+    const double* Fo = ff + ((size_t)cell * 6 + f) * V * B2;
+    double* out = sF + f * V * B2;
+    if (nb < 0) {
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        const int node = hl + 32 * k;
+        if (node < B2) {
+#pragma unroll
+          for (int v = 0; v < V; ++v) out[v * B2 + node] = Fo[v * B2 + node];
+        }
+      }
+      continue;
+    }
+    const double* Qn = qf + ((size_t)nb * 6 + (f ^ 1)) * V * B2;
+    const double* Fn = ff + ((size_t)nb * 6 + (f ^ 1)) * V * B2;
+    const double* QL = side ? Qo : Qn;
+    const double* QR = side ? Qn : Qo;
+    const double* FL = side ? Fo : Fn;
+    const double* FR = side ? Fn : Fo;
+    double aL[V], aR[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) aL[v] = aR[v] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      const int node = hl + 32 * k;
+      if (node < B2) {
+        const double w = T.w[node / B] * T.w[node % B];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          aL[v] += w * QL[v * B2 + node];
+          aR[v] += w * QR[v * B2 + node];
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      aL[v] = half_wave_sum(aL[v]);
+      aR[v] = half_wave_sum(aR[v]);
+    }
+    const double sL = EulerPDE::max_abs_eigenvalue(aL, d, gamma);
+    const double sR = EulerPDE::max_abs_eigenvalue(aR, d, gamma);
+    const double smax = sL < sR ? sR : sL;
+    // several nodes per lane: the states are read again (from cache) rather
+    // than held through the sums
+    if (NP > 1) asm volatile("" ::: "memory");
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      const int node = hl + 32 * k;
+      if (node < B2) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const int i = v * B2 + node;
+          out[i] = 0.5 * (FR[i] + FL[i]) - 0.5 * smax * (QR[i] - QL[i]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  double dtn = 1.0e300;
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int n = s + it * NT;
+    if (n < B3) {
+      const int z = n / B2, y = (n / B) % B, x = n % B;
+      double du[V], u[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        u[v] = NIT == 1 ? p0[v] : lduh[cell * V * B3 + v * B3 + n];
+        du[v] = 0.0;
+      }
+      const double usx = (T.w[z] * T.w[y]) / dx0;
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        du[v] -= (sF[(1 * V + v) * B2 + z * B + y] * T.FR[x] - sF[(0 * V + v) * B2 + z * B + y] * T.FL[x]) * usx;
+      const double usy = (T.w[z] * T.w[x]) / dx1;
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        du[v] -= (sF[(3 * V + v) * B2 + z * B + x] * T.FR[y] - sF[(2 * V + v) * B2 + z * B + x] * T.FL[y]) * usy;
+      const double usz = (T.w[y] * T.w[x]) / dx2;
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+        du[v] -= (sF[(5 * V + v) * B2 + y * B + x] * T.FR[z] - sF[(4 * V + v) * B2 + y * B + x] * T.FL[z]) * usz;
+      const double weight = T.w[z] * T.w[y] * T.w[x];
+      const double updateSize = *dtp / weight;
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        u[v] += du[v] * updateSize;
+        luh[cell * V * B3 + v * B3 + n] = u[v];
+      }
+      double m[3];
+      PdeEuler::max_abs_eigenvalues3(u, PdeParams{gamma, {ens.adv[0], ens.adv[1], ens.adv[2]}}, m);
+      double den = 0.0;
+      den += m[0] / dx0;
+      den += m[1] / dx1;
+      den += m[2] / dx2;
+      const double cand = exa_dt_sane(cfl * exa_pnpm(N) / den);
+      dtn = cand < dtn ? cand : dtn;
+    }
+  }
+  dtn = block_min(dtn, red);
+  if (s == 0) atomic_min_pos_double(dt_next, dtn);
 }
 
 // The solution at the (N+1)^3 equidistant vertices of every cell, as a

@@ -1101,13 +1101,13 @@ struct exa_grid {
   unsigned long long* iter_sum() const { return reinterpret_cast<unsigned long long*>(state.as<double>() + 3); }
 };
 
-// +d neighbours (nbr[d * nc + c]), wrapped on periodic axes and -1 past the
-// last cell of the others, and the list of domain faces of the non-periodic
-// axes (cell * 6 + face).
+// +d neighbours (nbr[d * nc + c]) and -d neighbours (nbr[(3 + d) * nc + c]),
+// wrapped on periodic axes and -1 past the first / last cell of the others,
+// and the list of domain faces of the non-periodic axes (cell * 6 + face).
 static hipError_t grid_neighbours(exa_grid* g) {
   const int nx = g->nx, ny = g->ny, nz = g->nz;
   const int n[3] = {nx, ny, nz};
-  std::vector<int> nbr(3 * g->nc), bfaces;
+  std::vector<int> nbr(6 * g->nc), bfaces;
   for (long long c = 0; c < g->nc; ++c) {
     const int ix = (int)(c % nx), iy = (int)((c / nx) % ny), iz = (int)(c / ((long long)nx * ny));
     const int id[3] = {ix, iy, iz};
@@ -1116,6 +1116,9 @@ static hipError_t grid_neighbours(exa_grid* g) {
       j[d] = (id[d] + 1) % n[d];
       const bool edge = id[d] == n[d] - 1 && !g->periodic[d];
       nbr[d * g->nc + c] = edge ? -1 : (int)(((long long)j[2] * ny + j[1]) * nx + j[0]);
+      j[d] = (id[d] + n[d] - 1) % n[d];
+      const bool first = id[d] == 0 && !g->periodic[d];
+      nbr[(3 + d) * g->nc + c] = first ? -1 : (int)(((long long)j[2] * ny + j[1]) * nx + j[0]);
       if (!g->periodic[d]) {
         if (id[d] == 0) bfaces.push_back((int)(c * 6 + 2 * d));
         if (id[d] == n[d] - 1) bfaces.push_back((int)(c * 6 + 2 * d + 1));
@@ -1158,7 +1161,7 @@ extern "C" int exa_grid_create(exa_solver* s, int nx, int ny, int nz, const doub
       (r = g->lduh.ensure(sizeof(double) * g->nc * V * B3)) != EXA_OK ||
       (r = g->qf.ensure(sizeof(double) * g->nc * 6 * V * B2)) != EXA_OK ||
       (r = g->ff.ensure(sizeof(double) * g->nc * 6 * V * B2)) != EXA_OK ||
-      (r = g->iters.ensure(sizeof(int) * g->nc)) != EXA_OK || (r = g->nbr.ensure(sizeof(int) * 3 * g->nc)) != EXA_OK ||
+      (r = g->iters.ensure(sizeof(int) * g->nc)) != EXA_OK || (r = g->nbr.ensure(sizeof(int) * 6 * g->nc)) != EXA_OK ||
       (r = g->state.ensure(sizeof(double) * 8)) != EXA_OK) {
     delete g;
     return r;
@@ -1400,6 +1403,26 @@ static exa::Ens grid_extras(const exa_grid* g) {
   return e;
 }
 
+// Riemann problems, surface integral, update and CFL minimum of every cell in
+// one launch (k_riemann_update), after a predictor launched with
+// Ens::fold_update.  box: nx, ny, nz and the periodic axes of a uniform grid
+// (neighbours computed in the kernel), or null for the table.
+static int launch_riemann_update(int N, double* luh, const double* lduh, const double* qf, const double* ff,
+                                 const int* nbr, long long nc, const int* box, const double* dt_cur,
+                                 unsigned long long* dt_next, const double dx[3], double gamma, double cfl,
+                                 hipStream_t st, const exa::Ens& en) {
+  EXA_ARG(en.fold_update, "the fused update needs the folded predictor output");
+  EXA_ARG(nc < (1LL << 32) - 1, "too many cells for one launch");
+  const int nx = box ? box[0] : 0, ny = box ? box[1] : 0, nz = box ? box[2] : 0, per = box ? box[3] : 0;
+  EXA_ORDER_SWITCH(N, {
+    hipLaunchKernelGGL((exa::k_riemann_update<NN>), dim3((unsigned)nc), dim3(exa::RiemannUpdate<NN>::NT), 0, st,
+                       luh, lduh, qf, ff, nbr, nc, nx, ny, nz, per, dt_cur, dt_next, dx[0], dx[1], dx[2], gamma, cfl,
+                       en);
+  });
+  EXA_CHECK(hipGetLastError());
+  return EXA_OK;
+}
+
 static int enqueue_step(exa_grid* g, bool timed) {
   const int N = g->s->cfg.order;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1431,6 +1454,8 @@ static int enqueue_step(exa_grid* g, bool timed) {
                        g->iter_sum(), nodbg, grid_extras(g));
   } else EXA_ORDER_SWITCH(N, {
     using O = exa::Ord<NN>;
+    exa::Ens ex = launch_extras(g->s->cfg);
+    ex.fold_update = 1;  // k_riemann_update follows
     if (g->staged) {
       hipLaunchKernelGGL(exa_pred_staged_kernel(NN, false, g->s->cfg.pde == EXA_PDE_EULER_GRAVITY,
                                                 g->s->cfg.precision == EXA_PREC_FP32),
@@ -1438,7 +1463,7 @@ static int enqueue_step(exa_grid* g, bool timed) {
                          (size_t)exa_pred_staged_lds_bytes(NN), g->stream, g->luh.as<double>(), g->lduh.as<double>(),
                          g->qf.as<double>(), g->ff.as<double>(), g->iters.as<int>(), g->dt_cur(), g->dx[0],
                          g->dx[1], g->dx[2], g->s->cfg.gamma, g->nc, g->stage.as<double>(), g->iter_sum(), nodbg,
-                         launch_extras(g->s->cfg), g->s->cfg.fp64_sweeps);
+                         ex, g->s->cfg.fp64_sweeps);
     } else {
       const int wpe = predictor_waves_per_simd(g->s->cfg, NN, g->s->device);
       const int form = predictor_form(g->s->cfg, NN, g->s->device);
@@ -1447,7 +1472,7 @@ static int enqueue_step(exa_grid* g, bool timed) {
       hipLaunchKernelGGL(kern, dim3((unsigned)g->nc), dim3(O::NT), O::LDS_DOUBLES * sizeof(double), g->stream,
                          g->luh.as<double>(), g->lduh.as<double>(), g->qf.as<double>(), g->ff.as<double>(),
                          g->iters.as<int>(), g->dt_cur(), g->dx[0], g->dx[1], g->dx[2], g->s->cfg.gamma,
-                         g->s->cfg.fp64_sweeps, g->iter_sum(), nodbg, launch_extras(g->s->cfg));
+                         g->s->cfg.fp64_sweeps, g->iter_sum(), nodbg, ex);
     }
   });
   EXA_CHECK(hipGetLastError());
@@ -1470,8 +1495,7 @@ static int enqueue_step(exa_grid* g, bool timed) {
                          g->dx[1], g->dx[2], g->nx, g->ny, g->s->cfg.gamma, launch_extras(g->s->cfg));
     }
   } else EXA_ORDER_SWITCH(N, {
-    hipLaunchKernelGGL((exa::k_riemann<NN>), dim3(blocks), dim3(256), 0, g->stream, g->qf.as<double>(),
-                       g->ff.as<double>(), g->nbr.as<int>(), (int)g->nc, g->s->cfg.gamma, 0LL, waves, exa::Ens{});
+    // domain faces first: the fused kernel takes their fluxes from ff
     if (g->nb > 0)
       hipLaunchKernelGGL((exa::k_boundary_faces<NN>), dim3((unsigned)((g->nb * 64LL + 255) / 256)), dim3(256), 0,
                          g->stream, g->qf.as<double>(), g->ff.as<double>(), g->bfaces.as<int>(), g->nb,
@@ -1481,7 +1505,16 @@ static int enqueue_step(exa_grid* g, bool timed) {
   EXA_CHECK(hipGetLastError());
   EXA_TRY(tstop(1));
   EXA_TRY(tstart(2));
-  EXA_TRY(launch_update<exa::UPD_SURFACE | exa::UPD_UPDATE | exa::UPD_DT>(g));
+  if (generic) {
+    EXA_TRY(launch_update<exa::UPD_SURFACE | exa::UPD_UPDATE | exa::UPD_DT>(g));
+  } else {
+    const int box[4] = {g->nx, g->ny, g->nz, (g->periodic[0] ? 1 : 0) | (g->periodic[1] ? 2 : 0) | (g->periodic[2] ? 4 : 0)};
+    exa::Ens ex{};
+    ex.fold_update = 1;
+    EXA_TRY(launch_riemann_update(N, g->luh.as<double>(), g->lduh.as<double>(), g->qf.as<double>(),
+                                  g->ff.as<double>(), g->nbr.as<int>(), g->nc, box, g->dt_cur(), g->dt_next(), g->dx,
+                                  g->s->cfg.gamma, g->s->cfg.cfl, g->stream, ex));
+  }
   EXA_TRY(tstop(2));
   hipLaunchKernelGGL(exa::k_dt_advance, dim3(1), dim3(1), 0, g->stream, g->dt_cur(), g->dt_next(), g->time());
   EXA_CHECK(hipGetLastError());
@@ -1733,7 +1766,7 @@ extern "C" int exa_ensemble_create(const exa_ensemble_member* m, int n, const ex
         (r = gr->qf.ensure(sizeof(double) * gr->nc * 6 * V * B2)) != EXA_OK ||
         (r = gr->ff.ensure(sizeof(double) * gr->nc * 6 * V * B2)) != EXA_OK ||
         (r = gr->iters.ensure(sizeof(int) * gr->nc)) != EXA_OK ||
-        (r = gr->nbr.ensure(sizeof(int) * 3 * gr->nc)) != EXA_OK ||
+        (r = gr->nbr.ensure(sizeof(int) * 6 * gr->nc)) != EXA_OK ||
         (r = gr->cell_inst.ensure(sizeof(int) * gr->nc)) != EXA_OK ||
         (r = gr->params.ensure(sizeof(exa::InstParams) * mem.size())) != EXA_OK ||
         (r = gr->ics.ensure(sizeof(exa_ic) * mem.size())) != EXA_OK ||
@@ -1741,7 +1774,7 @@ extern "C" int exa_ensemble_create(const exa_ensemble_member* m, int n, const ex
       exa_ensemble_destroy(e);
       return r;
     }
-    std::vector<int> nbr(3 * gr->nc), ci(gr->nc);
+    std::vector<int> nbr(6 * gr->nc), ci(gr->nc);
     for (size_t j = 0; j < mem.size(); ++j) {
       const exa::InstParams& p = gr->hp[j];
       const long long nci = (long long)p.nx * p.ny * p.nz;
@@ -1751,6 +1784,9 @@ extern "C" int exa_ensemble_create(const exa_ensemble_member* m, int n, const ex
         nbr[0 * gr->nc + g] = (int)(p.base + ((long long)iz * p.ny + iy) * p.nx + (ix + 1) % p.nx);
         nbr[1 * gr->nc + g] = (int)(p.base + ((long long)iz * p.ny + (iy + 1) % p.ny) * p.nx + ix);
         nbr[2 * gr->nc + g] = (int)(p.base + ((long long)((iz + 1) % p.nz) * p.ny + iy) * p.nx + ix);
+        nbr[3 * gr->nc + g] = (int)(p.base + ((long long)iz * p.ny + iy) * p.nx + (ix + p.nx - 1) % p.nx);
+        nbr[4 * gr->nc + g] = (int)(p.base + ((long long)iz * p.ny + (iy + p.ny - 1) % p.ny) * p.nx + ix);
+        nbr[5 * gr->nc + g] = (int)(p.base + ((long long)((iz + p.nz - 1) % p.nz) * p.ny + iy) * p.nx + ix);
         ci[g] = (int)j;
       }
     }
@@ -1821,7 +1857,8 @@ static int enqueue_ensemble_step(exa_ensemble* e) {
   exa::PredictorDebug nodbg;
   nodbg.lQi = nodbg.lFi = nodbg.lQhi = nodbg.lFhi = nullptr;
   for (EnsGroup* gr : e->groups) {
-    const exa::Ens en = ens_of(gr);
+    exa::Ens en = ens_of(gr);
+    en.fold_update = 1;  // k_riemann_update follows
     const exa_aderdg_config& cfg = gr->s->cfg;
     EXA_ORDER_SWITCH(gr->order, {
       using O = exa::Ord<NN>;
@@ -1832,15 +1869,11 @@ static int enqueue_ensemble_step(exa_ensemble* e) {
                          gr->luh.as<double>(), gr->lduh.as<double>(), gr->qf.as<double>(), gr->ff.as<double>(),
                          gr->iters.as<int>(), nullptr, 1.0, 1.0, 1.0, 1.4, cfg.fp64_sweeps,
                          gr->iter_sum.as<unsigned long long>(), nodbg, en);
-      const long long waves = 3LL * gr->nc;
-      const unsigned blocks = (unsigned)((waves * 64 + 255) / 256);
-      hipLaunchKernelGGL((exa::k_riemann<NN>), dim3(blocks), dim3(256), 0, e->stream, gr->qf.as<double>(),
-                         gr->ff.as<double>(), gr->nbr.as<int>(), (int)gr->nc, 1.4, 0LL, waves, en);
-      hipLaunchKernelGGL((exa::k_update<NN, exa::UPD_SURFACE | exa::UPD_UPDATE | exa::UPD_DT>),
-                         dim3((unsigned)gr->nc), dim3(O::NT), 0, e->stream, gr->luh.as<double>(),
-                         gr->lduh.as<double>(), gr->ff.as<double>(), nullptr, nullptr, 1.0, 1.0, 1.0, 1.4, cfg.cfl,
-                         en);
     });
+    const double unit[3] = {1.0, 1.0, 1.0};  // per member from the instance table
+    EXA_TRY(launch_riemann_update(gr->order, gr->luh.as<double>(), gr->lduh.as<double>(), gr->qf.as<double>(),
+                                  gr->ff.as<double>(), gr->nbr.as<int>(), gr->nc, nullptr, nullptr, nullptr, unit, 1.4,
+                                  cfg.cfl, e->stream, en));
     const int ni = (int)gr->members.size();
     hipLaunchKernelGGL(exa::k_ens_dt_advance, dim3((ni + 255) / 256), dim3(256), 0, e->stream, en.inst, ni);
     EXA_CHECK(hipGetLastError());

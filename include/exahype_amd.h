@@ -309,7 +309,7 @@ int exa_aderdg_batch_flush(exa_solver* s);
  * into the unit's constant memory on the current device. */
 #define EXA_PDE_USER_FIRST 16  /* ids of registered PDEs: EXA_PDE_USER_FIRST + slot */
 #define EXA_PDE_USER_SLOTS 8
-#define EXA_PDE_KERNELS_VERSION 1
+#define EXA_PDE_KERNELS_VERSION 2
 typedef struct {
   int version;    /* EXA_PDE_KERNELS_VERSION */
   int max_order;  /* highest order instantiated (<= 7) */
@@ -374,7 +374,10 @@ int exa_grid_picard_iterations(exa_grid* g, int* iters);
 int exa_grid_use_graph(exa_grid* g, int enable);
 /* Kernel timing: when enabled, HIP events bracket every launch of each
  * phase on the grid's stream; times[k] = summed milliseconds, counts[k] =
- * launches, for k = 0 predictor, 1 riemann, 2 update(+surface+dt).
+ * launches, for k = 0 predictor, 1 riemann, 2 update(+surface+dt).  The
+ * Euler PDEs solve every interior face's Riemann problem inside the update
+ * launch (one kernel per step for Riemann, surface integral, update and
+ * CFL): phase 1 then holds the domain faces of closed axes only.
  * Only meaningful with graphs disabled. */
 int exa_grid_enable_timing(exa_grid* g, int enable);
 int exa_grid_kernel_times(exa_grid* g, double times[3], long long counts[3]);
