@@ -1285,9 +1285,13 @@ __global__ __launch_bounds__(Ord<N>::NT) void k_predictor_staged(
         out[4] += weight * sh[1];
       }
       if (ens.fold_update) {
+        // luh read again through an opaque index, so that the state is not
+        // held in registers through the sweeps
+        int so = s;
+        asm volatile("" : "+v"(so));
         const double updateSize = dt / w3;
 #pragma unroll
-        for (int v = 0; v < V; ++v) out[v] = u[v] + out[v] * updateSize;
+        for (int v = 0; v < V; ++v) out[v] = luh[cell * V * B3 + v * B3 + so] + out[v] * updateSize;
       }
 #pragma unroll
       for (int v = 0; v < V; ++v) lduh[cell * V * B3 + v * B3 + s] = out[v];
