@@ -330,6 +330,7 @@ def main():
                               "riemann_surface_update_dt": tail_ms},
         "tail_roofline": {"bound": "hbm", "achieved": tail_bytes / tail_ms * 1e-6, "peak": PEAK_HBM_GBPS,
                           "unit": "GB/s", "frac": tail_bytes / tail_ms * 1e-6 / PEAK_HBM_GBPS,
+                          "traffic": measured_traffic("k_riemann_update<%d>" % N, n, N),
                           "kernel": "k_riemann_update<%d>" % N, "avg_launch_ms": tail_ms,
                           "bytes_per_launch": tail_bytes,
                           "bytes_with_neighbour_face_reads": tail_bytes + ncells * 8 * 5 * 12 * (N + 1) ** 2},
