@@ -106,7 +106,8 @@ EXPORTS = [
     "exa_aderdg_batch_update", "exa_aderdg_batch_pending", "exa_aderdg_batch_flush", "exa_grid_create", "exa_grid_destroy", "exa_grid_num_cells",
     "exa_grid_upload", "exa_grid_download", "exa_grid_set_initial_conditions", "exa_grid_sample_cartesian",
     "exa_grid_compute_dt",
-    "exa_grid_set_dt", "exa_grid_step", "exa_grid_sync", "exa_grid_picard_iterations",
+    "exa_grid_set_dt", "exa_grid_set_fused_time_stepping", "exa_grid_fused_reruns", "exa_grid_step", "exa_grid_sync",
+    "exa_grid_picard_iterations",
     "exa_grid_use_graph", "exa_grid_enable_timing", "exa_grid_kernel_times",
     "exa_grid_total_picard_iterations",
     "exa_fv_create", "exa_fv_destroy", "exa_fv_solution_update", "exa_fv_stable_time_step_size",
@@ -177,6 +178,8 @@ def load_library(path=LIB_PATH):
     L.exa_grid_set_initial_conditions.argtypes = [C.c_void_p, P(InitialCondition)]
     L.exa_grid_compute_dt.argtypes = [C.c_void_p, _D]
     L.exa_grid_set_dt.argtypes = [C.c_void_p, C.c_double]
+    L.exa_grid_set_fused_time_stepping.argtypes = [C.c_void_p, I, C.c_double]
+    L.exa_grid_fused_reruns.argtypes = [C.c_void_p, P(LL)]
     L.exa_grid_step.argtypes = [C.c_void_p, I]
     L.exa_grid_sync.argtypes = [C.c_void_p, _D, _D]
     L.exa_grid_picard_iterations.argtypes = [C.c_void_p, P(I)]
@@ -518,6 +521,19 @@ class Grid:
 
     def set_dt(self, dt):
         _check(_lib.exa_grid_set_dt(self._h, float(dt)))
+
+    def set_fused_time_stepping(self, enable=True, weight=1.0):
+        """The reference's fused stepping (`fuse-algorithmic-steps = on`,
+        `fuse-algorithmic-steps-factor = weight`): optimistic time steps,
+        weight * stable after an unstable one."""
+        _check(_lib.exa_grid_set_fused_time_stepping(self._h, 1 if enable else 0, float(weight)))
+
+    def fused_reruns(self):
+        """Steps whose optimistic dt exceeded the stable one (the reference
+        reran the predictor there)."""
+        n = C.c_longlong(0)
+        _check(_lib.exa_grid_fused_reruns(self._h, C.byref(n)))
+        return n.value
 
     def step(self, nsteps=1):
         _check(_lib.exa_grid_step(self._h, int(nsteps)))

@@ -1091,6 +1091,8 @@ struct exa_grid {
   hipGraphExec_t gexec = nullptr;
   bool timing = false;
   bool dt_valid = false;  // a dt has been computed or set (sync checks it from then on)
+  bool fused = false;     // fused time stepping (exa_grid_set_fused_time_stepping)
+  double fuse_w = 1.0;    // its fuse-algorithmic-steps-factor
   long long version = 0;  // bumped whenever luh changes (steps, uploads, initial conditions)
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
   double times[3] = {0, 0, 0};
@@ -1099,6 +1101,8 @@ struct exa_grid {
   unsigned long long* dt_next() const { return reinterpret_cast<unsigned long long*>(state.as<double>() + 1); }
   double* time() const { return state.as<double>() + 2; }
   unsigned long long* iter_sum() const { return reinterpret_cast<unsigned long long*>(state.as<double>() + 3); }
+  double* dt_pred() const { return state.as<double>() + 4; }  // fused stepping: the optimistic dt
+  unsigned long long* reruns() const { return reinterpret_cast<unsigned long long*>(state.as<double>() + 5); }
 };
 
 // +d neighbours (nbr[d * nc + c]) and -d neighbours (nbr[(3 + d) * nc + c]),
@@ -1343,6 +1347,8 @@ extern "C" int exa_grid_compute_dt(exa_grid* g, double* dt) {
   EXA_TRY(launch_update<exa::UPD_DT>(g));
   hipLaunchKernelGGL(exa::k_dt_take, dim3(1), dim3(1), 0, g->stream, g->dt_cur(), g->dt_next());
   hipLaunchKernelGGL(exa::k_dt_reset, dim3(1), dim3(1), 0, g->stream, g->dt_next());
+  if (g->fused)  // fused stepping starts again from this state's stable dt
+    hipLaunchKernelGGL(exa::k_dt_fused_start, dim3(1), dim3(1), 0, g->stream, g->dt_cur(), g->dt_pred(), g->fuse_w);
   EXA_CHECK(hipGetLastError());
   double h = 0.0;
   EXA_CHECK(hipMemcpyAsync(&h, g->dt_cur(), sizeof(double), hipMemcpyDeviceToHost, g->stream));
@@ -1356,8 +1362,43 @@ extern "C" int exa_grid_set_dt(exa_grid* g, double dt) {
   EXA_ARG(g && dt > 0.0, "dt must be positive");
   EXA_CHECK(hipSetDevice(g->s->device));
   EXA_CHECK(hipMemcpyAsync(g->dt_cur(), &dt, sizeof(double), hipMemcpyHostToDevice, g->stream));
+  // fused stepping: the prescribed dt is also the optimistic one
+  if (g->fused) EXA_CHECK(hipMemcpyAsync(g->dt_pred(), &dt, sizeof(double), hipMemcpyHostToDevice, g->stream));
   EXA_CHECK(hipStreamSynchronize(g->stream));
   g->dt_valid = true;
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_set_fused_time_stepping(exa_grid* g, int enable, double weight) {
+  EXA_ARG(g, "null pointer");
+  EXA_ARG(!enable || (weight > 0.0 && weight <= 1.0), "fuse-algorithmic-steps-factor must lie in (0, 1]");
+  EXA_CHECK(hipSetDevice(g->s->device));
+  g->fused = enable != 0;
+  g->fuse_w = enable ? weight : 1.0;
+  const double zero = 0.0;
+  const unsigned long long nz = 0;
+  EXA_CHECK(hipMemcpyAsync(g->dt_pred(), &zero, sizeof(zero), hipMemcpyHostToDevice, g->stream));
+  EXA_CHECK(hipMemcpyAsync(g->reruns(), &nz, sizeof(nz), hipMemcpyHostToDevice, g->stream));
+  // a dt in hand is taken as the current state's stable dt
+  if (g->fused && g->dt_valid)
+    hipLaunchKernelGGL(exa::k_dt_fused_start, dim3(1), dim3(1), 0, g->stream, g->dt_cur(), g->dt_pred(), g->fuse_w);
+  EXA_CHECK(hipGetLastError());
+  EXA_CHECK(hipStreamSynchronize(g->stream));
+  // the captured step ends with the other time-step kernel
+  if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  g->gexec = nullptr;
+  g->graph = nullptr;
+  return EXA_OK;
+}
+
+extern "C" int exa_grid_fused_reruns(exa_grid* g, long long* reruns) {
+  EXA_ARG(g && reruns, "null pointer");
+  EXA_CHECK(hipSetDevice(g->s->device));
+  unsigned long long h = 0;
+  EXA_CHECK(hipMemcpyAsync(&h, g->reruns(), sizeof(h), hipMemcpyDeviceToHost, g->stream));
+  EXA_CHECK(hipStreamSynchronize(g->stream));
+  *reruns = (long long)h;
   return EXA_OK;
 }
 
@@ -1516,7 +1557,11 @@ static int enqueue_step(exa_grid* g, bool timed) {
                                   g->s->cfg.gamma, g->s->cfg.cfl, g->stream, ex));
   }
   EXA_TRY(tstop(2));
-  hipLaunchKernelGGL(exa::k_dt_advance, dim3(1), dim3(1), 0, g->stream, g->dt_cur(), g->dt_next(), g->time());
+  if (g->fused)
+    hipLaunchKernelGGL(exa::k_dt_advance_fused, dim3(1), dim3(1), 0, g->stream, g->dt_cur(), g->dt_next(), g->time(),
+                       g->dt_pred(), g->reruns(), g->fuse_w);
+  else
+    hipLaunchKernelGGL(exa::k_dt_advance, dim3(1), dim3(1), 0, g->stream, g->dt_cur(), g->dt_next(), g->time());
   EXA_CHECK(hipGetLastError());
   return EXA_OK;
 }

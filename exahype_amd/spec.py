@@ -251,6 +251,27 @@ class Spec:
     def steps_till_cured(self, k):
         return atoi(self._solver_key(k, "steps-till-cured"))
 
+    # --- optimisation section -------------------------------------------------
+    def fuse_algorithmic_steps(self):
+        """`fuse-algorithmic-steps` (E/Parser.cpp:452-469): on / off, off when
+        absent."""
+        tok = self.token_after("optimisation", 1, "fuse-algorithmic-steps")
+        if tok == NO_TOKEN:
+            return False
+        if tok not in ("on", "off"):
+            raise SpecError("fuse-algorithmic-steps has to be either on or off: %r" % tok)
+        return tok == "on"
+
+    def fuse_algorithmic_steps_factor(self):
+        """`fuse-algorithmic-steps-factor` (E/Parser.cpp:486-501): strtod of
+        the token, an error unless a number in [0, 1]."""
+        tok = self.token_after("optimisation", 1, "fuse-algorithmic-steps-factor")
+        m = re.match(r"\s*([+-]?(\d+\.?\d*|\.\d+)([eE][+-]?\d+)?)", tok)
+        v = float(m.group(1)) if m else 0.0
+        if not m or v < 0.0 or v > 1.0:
+            raise SpecError("fuse-algorithmic-steps-factor must lie in [0, 1]: %r" % tok)
+        return v
+
     # --- the regular mesh the runner builds ----------------------------------
     def mesh(self, k):
         """(cells per axis, mesh width h, scaled domain width) of solver k's
@@ -315,4 +336,6 @@ def create(spec, k=0, pde=None, velocity=(0.0, 0.0, 0.0), gamma=1.4, device=0):
     s = E.ADERDGSolver(cfg["order"], nvar=cfg["nvar"], npar=cfg["npar"], gamma=gamma, pde=pde, velocity=velocity,
                        linear=cfg["linear"], device=device)
     g = E.Grid(s, n[0], n[1], n[2], offset=off, width=width)
+    if spec.fuse_algorithmic_steps():  # Runner.cpp:506-509
+        g.set_fused_time_stepping(True, spec.fuse_algorithmic_steps_factor())
     return s, g

@@ -21,7 +21,8 @@
  *     structure-of-arrays blocks, advanced by whole time steps (predictor,
  *     Riemann solves, surface integral + update + CFL reduction) without host
  *     round trips.  The step is the reference's non-fused stepping
- *     (E/runners/Runner.cpp:1125-1190).
+ *     (E/runners/Runner.cpp:1125-1190) or, on request, its fused stepping's
+ *     time step control (exa_grid_set_fused_time_stepping).
  *
  * Every entry point returns EXA_OK (0) or a negative status; exa_last_error()
  * gives the message of the last failure on the calling thread.  No entry
@@ -362,9 +363,30 @@ int exa_grid_sample_cartesian(exa_grid* g, double* out);
 /* Stable dt of the current state (global minimum over cells); also
  * becomes the dt of the next step.  Synchronous. */
 int exa_grid_compute_dt(exa_grid* g, double* dt);
-/* Overrides the dt of the next step. */
+/* Overrides the dt of the next step (with fused stepping also the
+ * optimistic one). */
 int exa_grid_set_dt(exa_grid* g, double dt);
-/* Enqueues nsteps time steps; each uses the stable dt of its start state. */
+/* Fused time stepping, the reference's `fuse-algorithmic-steps = on` with
+ * `fuse-algorithmic-steps-factor = weight` in (0, 1] (global time stepping;
+ * exahype::runners::Runner::runOneTimeStepWithFusedAlgorithmicSteps,
+ * E/runners/Runner.cpp:1069-1122; the time step data of
+ * TimeStepSizeComputation.cpp:144-187, ADERDGSolver.cpp:556-570 and
+ * FusedTimeSteppingInitialisation.cpp:80-100).  Starting (here with a dt in
+ * hand, or at the next exa_grid_compute_dt) the first step takes
+ * 0.5 (s + weight s), s the stable dt; after each step, with `used` the
+ * step's optimistic predictor dt and `stable` that of the new state: used >
+ * stable -> the next step takes weight * stable (where the reference reruns
+ * the predictor; counted by exa_grid_fused_reruns), otherwise it takes
+ * `used` and the next optimistic dt is 0.5 (used + weight stable).  The
+ * predictor runs once per step with the final dt: the minimum over the
+ * cells is known on the device before it is launched.  enable = 0 returns
+ * to the step-by-step stable dt. */
+int exa_grid_set_fused_time_stepping(exa_grid* g, int enable, double weight);
+/* Steps since fused stepping started whose optimistic dt exceeded the stable
+ * one (the reference's predictor reruns). */
+int exa_grid_fused_reruns(exa_grid* g, long long* reruns);
+/* Enqueues nsteps time steps; each uses the stable dt of its start state
+ * (fused stepping: the dt chosen as above). */
 int exa_grid_step(exa_grid* g, int nsteps);
 /* Waits for queued work; returns simulated time and the dt of the next step. */
 int exa_grid_sync(exa_grid* g, double* time, double* next_dt);

@@ -400,6 +400,53 @@ class OracleGrid:
                                      nthreads or os.cpu_count())
 
 
+class FusedTimeStepping:
+    """Time step data of the reference's fused algorithmic steps (global time
+    stepping), restated: dt_c is the dt of the next step (its predictor and
+    corrector), dt_p the optimistic predictor dt, reruns the steps whose
+    optimistic dt was unstable.
+
+    start(stable): ADERDGSolver::initSolver zeroes the predictor dt
+    (E/solvers/ADERDGSolver.cpp:754-756); the first time-step-size
+    computation then takes used = stable (TimeStepSizeComputation.cpp:166-168),
+    is not unstable and sets next = min(stable, 0.5 (used + w stable))
+    (:182-183, updateMinNextPredictorTimeStepSize :657-670), which
+    startNewTimeStep makes the predictor dt (ADERDGSolver.cpp:556-570) and
+    the fused initialisation the corrector dt
+    (FusedTimeSteppingInitialisation.cpp:96-99).
+
+    after_step(stable): the end of a fused iteration, stable the minimum over
+    the updated cells (TimeStepSizeComputation.cpp:158-187 then
+    startNewTimeStep): used > stable sets the predictor dt, and with it the
+    next corrector dt, to w stable and the next optimistic dt to
+    min(stable, w stable), and the predictor is recomputed with it
+    (Runner.cpp:1114-1120); otherwise the corrector keeps used and the next
+    optimistic dt is min(stable, 0.5 (used + w stable))."""
+
+    def __init__(self, weight):
+        self.w = weight
+        self.dt_c = 0.0
+        self.dt_p = 0.0
+        self.reruns = 0
+
+    def start(self, stable):
+        used = stable
+        nxt = min(stable, 0.5 * (used + self.w * stable))
+        self.dt_p = nxt
+        self.dt_c = nxt
+
+    def after_step(self, stable):
+        used = self.dt_p if self.dt_p != 0.0 else stable
+        if used > stable:
+            self.dt_c = self.w * stable
+            self.dt_p = min(stable, self.w * stable)
+            self.reruns += 1
+        else:
+            self.dt_c = used
+            self.dt_p = min(stable, 0.5 * (used + self.w * stable))
+        return self.dt_c
+
+
 def fv_rusanov(c, dim, qL, qR, normal):
     fL, fR = np.zeros(16), np.zeros(16)
     s = lib().orc_fv_rusanov(C.byref(c), dim, ptr(fL), ptr(fR), ptr(dvec(qL)), ptr(dvec(qR)), normal)

@@ -50,3 +50,28 @@ def test_aderdg_project_from_spec():
     g3.step(2)
     g3.sync()
     assert np.all(np.isfinite(g3.download()))
+
+
+def test_fused_project_from_spec():
+    """fuse-algorithmic-steps = on gives the grid fused stepping with the
+    spec's factor: bitwise the directly built fused grid, with a dt sequence
+    other than the step-by-step stable one."""
+    import exahype_amd as E
+    from exahype_amd import spec as S
+    from test_spec import _with_optimisation
+    sp = S.Spec(_with_optimisation("on", "0.8"))
+    ic = E.make_ic(0, xc=(0.5, 0.5, 0.5), L=0.1, eps=5.0, uinf=(1.0, 1.0, 0.0), rho_inf=1.0, p_inf=1.0)
+    s1, g1 = S.create(sp, 0)
+    s2 = E.ADERDGSolver(5)
+    g2 = E.Grid(s2, 27, 27, 27)
+    g2.set_fused_time_stepping(True, 0.8)
+    s3 = E.ADERDGSolver(5)
+    g3 = E.Grid(s3, 27, 27, 27)
+    out = []
+    for g in (g1, g2, g3):
+        g.set_initial_conditions(ic)
+        g.compute_dt()
+        g.step(2)
+        out.append((g.sync(), g.download()))
+    assert out[0][0] == out[1][0] and np.array_equal(out[0][1], out[1][1])
+    assert out[0][0][0] < out[2][0][0]  # fused: 0.5 (1 + 0.8) of the stable dt per step here

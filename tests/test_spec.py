@@ -136,3 +136,22 @@ def test_reference_demonstrator_specs():
     assert cfg["cells"] == (81, 81) and sp.end_time == 0.4
     tp = S.Spec.from_file(os.path.join(REF, "TrivialProject", "TrivialProject.exahype"))
     assert tp.num_solvers == 0 and tp.dimension == 2 and tp.end_time == 10.0
+
+
+def _with_optimisation(fuse, factor):
+    block = ("  optimisation\n    fuse-algorithmic-steps        = %s\n"
+             "    fuse-algorithmic-steps-factor = %s\n  end optimisation\n" % (fuse, factor))
+    return DG_SPEC.replace("  solver ADER-DG EulerSolver", block + "  solver ADER-DG EulerSolver", 1)
+
+
+def test_fused_time_stepping_options():
+    sp = S.Spec(_with_optimisation("on", "0.99"))
+    assert sp.fuse_algorithmic_steps() is True and sp.fuse_algorithmic_steps_factor() == 0.99
+    assert S.Spec(_with_optimisation("off", "0.5")).fuse_algorithmic_steps() is False
+    assert S.Spec(DG_SPEC).fuse_algorithmic_steps() is False  # absent: off (E/Parser.cpp:459-460)
+    assert sp.solver_config(0)["order"] == 5  # the section does not disturb the solver queries
+    for bad in ("1.5", "-0.1", "x"):
+        with pytest.raises(S.SpecError):
+            S.Spec(_with_optimisation("on", bad)).fuse_algorithmic_steps_factor()
+    with pytest.raises(S.SpecError):
+        S.Spec(_with_optimisation("maybe", "0.5")).fuse_algorithmic_steps()
