@@ -33,6 +33,13 @@ void fill_tables(int N, AmrTables& t) {
     for (int i = 0; i <= N; ++i)
       for (int j = 0; j <= N; ++j) t.P[s][i][j] = EXA_TAB_fineGridProjector1d[N][s][i][j];
   for (int i = 0; i <= N; ++i) t.w[i] = EXA_TAB_gl_weights[N][i];
+  // level_op of one level, the same expressions
+  for (int s = 0; s < 3; ++s)
+    for (int o = 0; o <= N; ++o)
+      for (int i = 0; i <= N; ++i) {
+        t.Op[0][s][o][i] = t.P[s][i][o];
+        t.Op[1][s][o][i] = t.w[i] * t.P[s][o][i] / (t.w[o] * 3.0);
+      }
 }
 
 // Dynamic-LDS limit of every transfer kernel instance, once per device.
@@ -80,18 +87,28 @@ int check_transfer(const exa_amr_transfer& t, int dim) {
   return EXA_OK;
 }
 
-// Launches one group per workgroup on the null stream; items/groups already on
-// the device.
+// Launches the groups on a stream (the null stream by default); items/groups
+// already on the device.  ntransfers (all groups; < 0: not known): with small
+// groups each workgroup takes several in turn, about 16 transfers, so that
+// the register-staged kernel has the next block in flight; groups of 8 or
+// more transfers get a workgroup each.
 int launch(int dim, bool prolong, const AmrTables* tab, const Transfer* items, const int* groups, int ngroups,
-           double* fine, double* coarse, int B, int nd, hipStream_t st = nullptr) {
+           double* fine, double* coarse, int B, int nd, hipStream_t st = nullptr, long long ntransfers = -1,
+           bool one_level = false) {
   if (ngroups == 0) return EXA_OK;
-  const dim3 grid(ngroups), block(exa::amr::THREADS);
+  long long nwg = ngroups;
+  if (ntransfers >= 0 && ntransfers < 8LL * ngroups) {
+    const long long want = (ntransfers + 15) / 16;
+    nwg = want < 2048 ? 2048 : want;
+    if (nwg > ngroups) nwg = ngroups;
+  }
+  const dim3 grid((unsigned)nwg), block(exa::amr::THREADS);
   // LDS: two buffers of one value chunk (as many values per node as fit)
   const int n = dim == 3 ? B * B * B : B * B;
   const size_t lds = 2 * sizeof(double) * (size_t)n * exa::amr::amr_chunk(B, dim, nd);
   bool ok = false;
   auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, grid, block, lds, st, tab, items, groups, fine, coarse, nd);
+    hipLaunchKernelGGL(kern, grid, block, lds, st, tab, items, groups, fine, coarse, nd, ngroups, one_level ? 1 : 0);
     ok = true;
   };
   switch (B) {
@@ -185,6 +202,12 @@ int build_list(int dim, bool prolong, long long blk, long long n_fine, long long
   return EXA_OK;
 }
 
+bool all_one_level(const exa_amr_transfer* transfers, int count) {
+  for (int i = 0; i < count; ++i)
+    if (transfers[i].levels != 1) return false;
+  return true;
+}
+
 int batched(int dim, bool prolong, int order, int nd, double* d_fine, long long n_fine, double* d_coarse,
             long long n_coarse, const exa_amr_transfer* transfers, int count) {
   EXA_ARG(order >= 0 && order <= EXA_MAX_ORDER, "order must be in 0..9");
@@ -215,7 +238,8 @@ int batched(int dim, bool prolong, int order, int nd, double* d_fine, long long 
     EXA_CHECK(hipEventCreate(&c.e1));
   }
   EXA_CHECK(hipEventRecord(c.e0, nullptr));
-  EXA_TRY(launch(dim, prolong, tab, c.items.as<Transfer>(), c.groups.as<int>(), ng, d_fine, d_coarse, B, nd));
+  EXA_TRY(launch(dim, prolong, tab, c.items.as<Transfer>(), c.groups.as<int>(), ng, d_fine, d_coarse, B, nd, nullptr,
+                 count, all_one_level(transfers, count)));
   EXA_CHECK(hipEventRecord(c.e1, nullptr));
   EXA_CHECK(hipDeviceSynchronize());
   float ms = 0.f;
@@ -382,7 +406,8 @@ extern "C" int exa_amr_restriction_batched_noalloc(int dim, int order, int ndata
   EXA_CHECK(hipMemcpy(&sp->tab, &t, sizeof(t), hipMemcpyHostToDevice));
   EXA_CHECK(hipMemcpy(sp->items, h_items, sizeof(Transfer) * count, hipMemcpyHostToDevice));
   EXA_CHECK(hipMemcpy(sp->groups, h_groups, sizeof(int) * (ng + 1), hipMemcpyHostToDevice));
-  EXA_TRY(launch(dim, false, &sp->tab, sp->items, sp->groups, ng, const_cast<double*>(d_fine), d_coarse, B, ndata));
+  EXA_TRY(launch(dim, false, &sp->tab, sp->items, sp->groups, ng, const_cast<double*>(d_fine), d_coarse, B, ndata,
+                 nullptr, count, all_one_level(transfers, count)));
   EXA_CHECK(hipDeviceSynchronize());
   return EXA_OK;
 }
