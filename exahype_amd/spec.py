@@ -311,11 +311,16 @@ class Spec:
         return cfg
 
 
-def create(spec, k=0, pde=None, velocity=(0.0, 0.0, 0.0), gamma=1.4, device=0):
+def create(spec, k=0, pde=None, velocity=(0.0, 0.0, 0.0), gamma=1.4, device=0, refine=None, patch_capacity=0):
     """The solver and resident grid of solver k of a Spec, with the user PDE
     given by `pde` (a PDE_* id of the compiled PDEs; the spec names none).
-    Returns (solver, grid).  ADER-DG runs in 3D; a Limiting-ADER-DG solver's
-    status machine is not built."""
+    Returns (solver, grid).  ADER-DG runs in 3D.  A Limiting-ADER-DG solver
+    gives the adaptive limiting grid (exahype_amd.AMRGrid) on the unit cube:
+    the regular mesh as level 0, at most maximum-mesh-depth finer levels
+    (two at most), the spec's DMP parameters and steps-till-cured; the
+    refinement criterion is user code in the reference, here `refine` =
+    (centre, radius, half widths per refined level) of the grid's refinement
+    shell (None: no refinement)."""
     import exahype_amd as E
     cfg = spec.solver_config(k)
     n = cfg["cells"]
@@ -329,10 +334,22 @@ def create(spec, k=0, pde=None, velocity=(0.0, 0.0, 0.0), gamma=1.4, device=0):
         else:
             g = E.FVGrid(s, n[0], n[1], n[2], offset=off, width=width)
         return s, g
-    if cfg["type"] == "Limiting-ADER-DG":
-        raise SpecError("Limiting-ADER-DG: the limiter status machine is not built")
     if cfg["dimension"] != 3:
         raise SpecError("ADER-DG kernels are built for 3 dimensions")
+    if cfg["type"] == "Limiting-ADER-DG":
+        if any(o != 0.0 for o in off) or any(w != 1.0 for w in width) or len(set(n)) != 1:
+            raise SpecError("the adaptive limiting grid covers the unit cube with n^3 cells (offset 0, width 1)")
+        if cfg["linear"]:
+            raise SpecError("Limiting-ADER-DG: the limiting grid runs the nonlinear kernels")
+        depth = min(spec.maximum_mesh_depth(k), 2)
+        centre, radius, widths = refine if refine is not None else ((0.5, 0.5, 0.5), 0.0, ())
+        levels = 1 + (min(depth, len(widths)) if refine is not None else 0)
+        s = E.ADERDGSolver(cfg["order"], nvar=cfg["nvar"], npar=cfg["npar"], gamma=gamma, pde=pde,
+                           velocity=velocity, device=device)
+        acfg = E.amr_config(n[0], levels, centre, radius, list(widths)[:levels - 1],
+                            relax=cfg["dmp_relaxation_parameter"], scaling=cfg["dmp_difference_scaling"],
+                            steps_till_cured=cfg["steps_till_cured"], patch_capacity=patch_capacity)
+        return s, E.AMRGrid(s, acfg)
     s = E.ADERDGSolver(cfg["order"], nvar=cfg["nvar"], npar=cfg["npar"], gamma=gamma, pde=pde, velocity=velocity,
                        linear=cfg["linear"], device=device)
     g = E.Grid(s, n[0], n[1], n[2], offset=off, width=width)

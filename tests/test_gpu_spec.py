@@ -75,3 +75,29 @@ def test_fused_project_from_spec():
         out.append((g.sync(), g.download()))
     assert out[0][0] == out[1][0] and np.array_equal(out[0][1], out[1][1])
     assert out[0][0][0] < out[2][0][0]  # fused: 0.5 (1 + 0.8) of the stable dt per step here
+
+
+def test_limiting_project_from_spec():
+    """A Limiting-ADER-DG solver gives the adaptive limiting grid with the
+    spec's mesh and DMP parameters: bitwise the directly built grid."""
+    import exahype_amd as E
+    from exahype_amd import spec as S
+    from test_spec import LIMITING_SPEC
+    sp = S.Spec(LIMITING_SPEC)
+    refine = ((0.5, 0.5, 0.5), 0.25, (0.05, 0.05))
+    s1, g1 = S.create(sp, 0, refine=refine)
+    s2 = E.ADERDGSolver(3)
+    g2 = E.AMRGrid(s2, E.amr_config(9, 3, (0.5, 0.5, 0.5), 0.25, [0.05, 0.05], relax=1e-3, scaling=1e-2,
+                                    steps_till_cured=0))
+    assert g1.ncells == g2.ncells and np.array_equal(g1.cells(), g2.cells())
+    ic = E.make_ic(kind=1, xc=(0.5, 0.5, 0.5), r0=0.25, rho_in=1.0, p_in=1.0, rho_out=0.125, p_out=0.1)
+    out = []
+    for g in (g1, g2):
+        g.set_initial_conditions(ic)
+        g.compute_dt()
+        g.step(2)
+        out.append((g.sync(), g.download(), g.limiter_status()))
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
+    with pytest.raises(S.SpecError):
+        S.create(S.Spec(LIMITING_SPEC.replace("offset          = 0.0, 0.0, 0.0", "offset          = 0.1, 0.0, 0.0")))

@@ -155,3 +155,40 @@ def test_fused_time_stepping_options():
             S.Spec(_with_optimisation("on", bad)).fuse_algorithmic_steps_factor()
     with pytest.raises(S.SpecError):
         S.Spec(_with_optimisation("maybe", "0.5")).fuse_algorithmic_steps()
+
+
+LIMITING_SPEC = """exahype-project SodSphere
+  computational-domain
+    dimension const = 3
+    width           = 1.0, 1.0, 1.0
+    offset          = 0.0, 0.0, 0.0
+    end-time        = 0.1
+  end computational-domain
+  optimisation
+    fuse-algorithmic-steps        = on
+    fuse-algorithmic-steps-factor = 0.99
+  end optimisation
+  solver Limiting-ADER-DG EulerSolver
+    variables const          = 5
+    order const              = 3
+    maximum-mesh-size        = 0.12
+    maximum-mesh-depth       = 2
+    time-stepping            = global
+    kernel const             = generic::fluxes::nonlinear
+    language const           = C
+    limiter-kernel const     = generic::musclhancock
+    limiter-language const   = C
+    dmp-relaxation-parameter = 1e-3
+    dmp-difference-scaling   = 1e-2
+    steps-till-cured         = 0
+  end solver
+end exahype-project
+"""
+
+
+def test_limiting_solver_queries():
+    sp = S.Spec(LIMITING_SPEC)
+    c = sp.solver_config(0)
+    assert c["type"] == "Limiting-ADER-DG" and c["order"] == 3 and c["cells"] == (9, 9, 9)
+    assert (c["dmp_relaxation_parameter"], c["dmp_difference_scaling"], c["steps_till_cured"]) == (1e-3, 1e-2, 0)
+    assert sp.maximum_mesh_depth(0) == 2 and c["limiter_kernel"] == "generic::musclhancock"
