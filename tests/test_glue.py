@@ -110,3 +110,39 @@ def test_called_entry_points_are_declared():
 def test_write_emits_one_file_per_solver(tmp_path):
     paths = G.write(DG_SPEC, str(tmp_path))
     assert sorted(os.path.basename(p) for p in paths) == ["AbstractEulerSolver.cpp", "AbstractLinearSolver.cpp"]
+
+
+def test_limiting_solver_has_two_generated_solvers(tmp_path):
+    from test_spec import LIMITING_SPEC
+    dg = G.generate(LIMITING_SPEC, 0, part="ADERDG")
+    fv = G.generate(LIMITING_SPEC, 0, part="FV")
+    assert "c.order = 3;" in dg and "SodSphere::AbstractEulerSolver_ADERDG::spaceTimePredictor(" in dg
+    assert "c.patch = 7;" in fv and "EXA_FV_MUSCLHANCOCK" in fv  # 2N+1 subcells, the limiter-kernel's scheme
+    assert "SodSphere::AbstractEulerSolver_FV::ghostLayerFilling(" in fv
+    with pytest.raises(S.SpecError):
+        G.generate(LIMITING_SPEC, 0)
+    with pytest.raises(S.SpecError):
+        G.generate(DG_SPEC, 0, part="FV")
+    paths = G.write(LIMITING_SPEC, str(tmp_path))
+    assert sorted(os.path.basename(p) for p in paths) == ["AbstractEulerSolver_ADERDG.cpp",
+                                                           "AbstractEulerSolver_FV.cpp"]
+
+
+@pytest.mark.skipif(not os.path.isdir(TEMPLATES), reason="reference toolkit templates not present")
+@pytest.mark.parametrize("tmpl,part", [("AbstractGenericADERDGSolverImplementation.template", "ADERDG"),
+                                       ("AbstractGenericFiniteVolumesSolverInCImplementation.template", "FV")])
+def test_limiting_parts_match_the_toolkit_templates(tmpl, part):
+    from test_spec import LIMITING_SPEC
+    ref = _template_methods(os.path.join(TEMPLATES, tmpl))
+    text = G.generate(LIMITING_SPEC, 0, part=part)
+    cls = "AbstractEulerSolver_" + part
+    ours = {}
+    for m in re.finditer(r"\b(?:void|double|bool|int)\s+\w+::%s::(\w+)\(([^)]*)\)" % cls, text):
+        ours.setdefault(m.group(1), []).append(_names(m.group(2)))
+    ref.pop("adjustPatchSolution", None)
+    assert set(ref) <= set(ours), set(ref) - set(ours)
+    header = open(os.path.join(ROOT, "include", "exahype_amd.h")).read()
+    for fn in set(re.findall(r"\b(exa_\w+)\(", text)):
+        if fn in ("exa_ok", "exa_handle", "exa_boundary", "exa_position", "exa_vec"):
+            continue
+        assert re.search(r"\b%s\(" % fn, header), fn
