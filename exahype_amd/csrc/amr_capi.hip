@@ -23,6 +23,7 @@ struct AmrCache {
   bool ready[EXA_MAX_ORDER + 1] = {};
   DevBuf items, groups, a, b, c, d;  // staging of the per-call and batched forms
   hipEvent_t e0 = nullptr, e1 = nullptr;  // around the last batched launch
+  hipStream_t st = nullptr;               // the batched launches' stream (events on it time the kernel alone)
   bool lds_set = false;
   float last_ms = 0.f;
 };
@@ -236,12 +237,15 @@ int batched(int dim, bool prolong, int order, int nd, double* d_fine, long long 
   if (!c.e0) {
     EXA_CHECK(hipEventCreate(&c.e0));
     EXA_CHECK(hipEventCreate(&c.e1));
+    EXA_CHECK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
   }
-  EXA_CHECK(hipEventRecord(c.e0, nullptr));
-  EXA_TRY(launch(dim, prolong, tab, c.items.as<Transfer>(), c.groups.as<int>(), ng, d_fine, d_coarse, B, nd, nullptr,
-                 count, all_one_level(transfers, count)));
-  EXA_CHECK(hipEventRecord(c.e1, nullptr));
+  // the caller's data may come from any stream: start after all of it
   EXA_CHECK(hipDeviceSynchronize());
+  EXA_CHECK(hipEventRecord(c.e0, c.st));
+  EXA_TRY(launch(dim, prolong, tab, c.items.as<Transfer>(), c.groups.as<int>(), ng, d_fine, d_coarse, B, nd, c.st,
+                 count, all_one_level(transfers, count)));
+  EXA_CHECK(hipEventRecord(c.e1, c.st));
+  EXA_CHECK(hipStreamSynchronize(c.st));
   float ms = 0.f;
   EXA_CHECK(hipEventElapsedTime(&ms, c.e0, c.e1));
   c.last_ms = ms;
