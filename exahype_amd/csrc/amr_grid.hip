@@ -279,7 +279,7 @@ int with_order(int N, F f) {
 
 size_t proj_lds(int B) {
   const int L = 2 * B - 1;
-  return sizeof(double) * (size_t)(B * B * L + B * L * L + L * L * L);
+  return sizeof(double) * (size_t)V * (B * B * L + B * L * L);  // lim::proj_t_lds_doubles<B>
 }
 size_t muscl_lds(int P) { return sizeof(double) * V * (size_t)P * P * P; }
 
@@ -349,9 +349,12 @@ int enqueue_build(exa_amr_grid* g, int mode, int k, const int* st_old, const int
 }
 
 int enqueue_project_fv(exa_amr_grid* g, const double* src, double* pool, int k) {
-  hipLaunchKernelGGL(k_project_fv_list, dim3((unsigned)g->cap), dim3(256), proj_lds(g->B), g->stream, src, pool,
-                     g->list_cells(k), g->list_slots(k), g->count(k),
-                     static_cast<const lim::LimTables*>(g->lim_tab), g->B);
+  EXA_TRY(with_patch(g->P, [&](auto p) {
+    constexpr int PP = decltype(p)::value;
+    hipLaunchKernelGGL(k_project_fv_list<PP>, dim3((unsigned)g->cap), dim3(256), proj_lds(g->B), g->stream, src, pool,
+                       g->list_cells(k), g->list_slots(k), g->count(k),
+                       static_cast<const lim::LimTables*>(g->lim_tab));
+  }));
   EXA_CHECK(hipGetLastError());
   return EXA_OK;
 }
@@ -501,8 +504,12 @@ int enqueue_step(exa_amr_grid* g, int par, bool timed) {
       hipLaunchKernelGGL(k_muscl_list<PP>, dim3(capb), dim3(256), muscl_lds(PP), g->stream, pool_old, pool_new,
                          g->list_slots(k), g->count(k), hf, g->ip(), g->finest, gamma);
     }));
-    hipLaunchKernelGGL(k_project_dg_list, dim3(capb), dim3(256), proj_lds(B), g->stream, pool_new, g->luh.as<double>(),
-                       g->list_cells(k), g->list_slots(k), g->count(k), ltab, B, g->fv_done.as<int>(), cnt);
+    EXA_TRY(with_patch(P, [&](auto p) {
+      constexpr int PP = decltype(p)::value;
+      hipLaunchKernelGGL(k_project_dg_list<PP>, dim3(capb), dim3(256), proj_lds(B), g->stream, pool_new,
+                         g->luh.as<double>(), g->list_cells(k), g->list_slots(k), g->count(k), ltab,
+                         g->fv_done.as<int>(), cnt);
+    }));
     EXA_CHECK(hipGetLastError());
     return EXA_OK;
   };
@@ -795,16 +802,17 @@ extern "C" int exa_amr_grid_create(exa_solver* s, const exa_amr_config* cfg, exa
   EXA_TRY(exa_amr_tables_dev(g->device, g->N, &g->amr_tab));
   EXA_TRY(exa_lim_tables_dev(g->device, g->N, &g->lim_tab));
   // LDS limits of the list kernels
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_project_fv_list),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)proj_lds(g->B));
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_project_dg_list),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)proj_lds(g->B));
-  EXA_CHECK(e);
+  hipError_t e = hipSuccess;
   EXA_TRY(with_patch(g->P, [&](auto p) {
     constexpr int PP = decltype(p)::value;
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_muscl_list<PP>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)muscl_lds(PP));
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_project_fv_list<PP>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)proj_lds(g->B));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_project_dg_list<PP>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)proj_lds(g->B));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_muscl_list<PP>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)muscl_lds(PP));
   }));
   EXA_CHECK(e);
   EXA_CHECK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));

@@ -327,25 +327,30 @@ __global__ __launch_bounds__(256) void k_muscl_list(const double* __restrict__ p
 }
 
 // DG <-> FV projections of the listed cells (Limiter.cpp:23-112).
-__global__ void k_project_fv_list(const double* __restrict__ luh, double* __restrict__ pool,
-                                  const int* __restrict__ cells, const int* __restrict__ slots,
-                                  const int* __restrict__ count, const lim::LimTables* tab, int B) {
+template <int P>
+__global__ __launch_bounds__(256) void k_project_fv_list(const double* __restrict__ luh, double* __restrict__ pool,
+                                                         const int* __restrict__ cells, const int* __restrict__ slots,
+                                                         const int* __restrict__ count,
+                                                         const lim::LimTables* __restrict__ tab) {
   if ((int)blockIdx.x >= *count) return;
   extern __shared__ double sm[];
-  const int L = 2 * B - 1, S = L + 2 * GHOST;
-  lim::project_on_fv_block(luh + (size_t)cells[blockIdx.x] * V * B * B * B,
-                           pool + (size_t)slots[blockIdx.x] * V * S * S * S, tab, B, GHOST, sm);
+  constexpr int B = (P + 1) / 2, S = P + 2 * GHOST;
+  lim::project_on_fv_block_t<B>(luh + (size_t)cells[blockIdx.x] * V * B * B * B,
+                                pool + (size_t)slots[blockIdx.x] * V * S * S * S, tab, GHOST, sm);
 }
-__global__ void k_project_dg_list(const double* __restrict__ pool, double* __restrict__ luh,
-                                  const int* __restrict__ cells, const int* __restrict__ slots,
-                                  const int* __restrict__ count, const lim::LimTables* tab, int B,
-                                  int* __restrict__ fv_done, unsigned long long* __restrict__ cnt) {
+template <int P>
+__global__ __launch_bounds__(256) void k_project_dg_list(const double* __restrict__ pool, double* __restrict__ luh,
+                                                         const int* __restrict__ cells, const int* __restrict__ slots,
+                                                         const int* __restrict__ count,
+                                                         const lim::LimTables* __restrict__ tab,
+                                                         int* __restrict__ fv_done,
+                                                         unsigned long long* __restrict__ cnt) {
   if ((int)blockIdx.x >= *count) return;
   extern __shared__ double sm[];
-  const int L = 2 * B - 1, S = L + 2 * GHOST;
+  constexpr int B = (P + 1) / 2, S = P + 2 * GHOST;
   const int q = cells[blockIdx.x];
-  lim::project_on_dg_block(pool + (size_t)slots[blockIdx.x] * V * S * S * S, luh + (size_t)q * V * B * B * B, tab, B,
-                           GHOST, sm);
+  lim::project_on_dg_block_t<B>(pool + (size_t)slots[blockIdx.x] * V * S * S * S, luh + (size_t)q * V * B * B * B,
+                                tab, GHOST, sm);
   if (threadIdx.x == 0) {
     fv_done[q] = 1;
     atomicAdd(&cnt[CNT_FV_UPDATES], 1ull);

@@ -4,6 +4,7 @@
 
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "dg_tables_data.h"
@@ -75,6 +76,52 @@ static size_t proj_lds_bytes(int B) {
   return sizeof(double) * (size_t)(B * B * L + B * L * L + L * L * L);
 }
 
+// A projection launch, cell list or none (block b: cell b): the order-fixed
+// all-variable kernels up to B = 9, the per-variable kernel at B = 10.
+static int launch_projection(bool on_fv, int B, int nblocks, hipStream_t st, const double* in, double* out,
+                             const LimTables* tab, int ghost, const int* cells) {
+  bool done = false;
+  auto go = [&](auto bb) -> int {
+    constexpr int BB = decltype(bb)::value;
+    const int bytes = exa::lim::proj_t_lds_doubles<BB>() * (int)sizeof(double);
+    const void* f = on_fv ? reinterpret_cast<const void*>(&exa::lim::k_project_on_fv_t<BB>)
+                          : reinterpret_cast<const void*>(&exa::lim::k_project_on_dg_t<BB>);
+    EXA_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    if (on_fv)
+      hipLaunchKernelGGL(exa::lim::k_project_on_fv_t<BB>, dim3(nblocks), dim3(256), bytes, st, in, out, tab, ghost,
+                         cells);
+    else
+      hipLaunchKernelGGL(exa::lim::k_project_on_dg_t<BB>, dim3(nblocks), dim3(256), bytes, st, in, out, tab, ghost,
+                         cells);
+    done = true;
+    return EXA_OK;
+  };
+  switch (B) {
+    case 2: EXA_TRY(go(std::integral_constant<int, 2>{})); break;
+    case 3: EXA_TRY(go(std::integral_constant<int, 3>{})); break;
+    case 4: EXA_TRY(go(std::integral_constant<int, 4>{})); break;
+    case 5: EXA_TRY(go(std::integral_constant<int, 5>{})); break;
+    case 6: EXA_TRY(go(std::integral_constant<int, 6>{})); break;
+    case 7: EXA_TRY(go(std::integral_constant<int, 7>{})); break;
+    case 8: EXA_TRY(go(std::integral_constant<int, 8>{})); break;
+    case 9: EXA_TRY(go(std::integral_constant<int, 9>{})); break;
+    default: break;
+  }
+  if (!done) {
+    const void* f = on_fv ? reinterpret_cast<const void*>(&exa::lim::k_project_on_fv)
+                          : reinterpret_cast<const void*>(&exa::lim::k_project_on_dg);
+    EXA_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)proj_lds_bytes(B)));
+    if (on_fv)
+      hipLaunchKernelGGL(exa::lim::k_project_on_fv, dim3(nblocks), dim3(256), proj_lds_bytes(B), st, in, out, tab, B,
+                         ghost, cells);
+    else
+      hipLaunchKernelGGL(exa::lim::k_project_on_dg, dim3(nblocks), dim3(256), proj_lds_bytes(B), st, in, out, tab, B,
+                         ghost, cells);
+  }
+  EXA_CHECK(hipGetLastError());
+  return EXA_OK;
+}
+
 struct LimScratch {
   DevBuf a, b, c, d;
   std::mutex mu;
@@ -109,11 +156,7 @@ extern "C" int exa_limiter_project_on_fv(int order, const double* luh, int ghost
   EXA_TRY(sc.b.ensure(sizeof(double) * hout.size()));
   EXA_CHECK(hipMemcpy(sc.a.p, hin.data(), sizeof(double) * hin.size(), hipMemcpyHostToDevice));
   EXA_CHECK(hipMemcpy(sc.b.p, hout.data(), sizeof(double) * hout.size(), hipMemcpyHostToDevice));
-  EXA_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::lim::k_project_on_fv),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)proj_lds_bytes(B)));
-  hipLaunchKernelGGL(exa::lim::k_project_on_fv, dim3(1), dim3(256), proj_lds_bytes(B), 0, sc.a.as<double>(),
-                     sc.b.as<double>(), tab, B, ghost, nullptr);
-  EXA_CHECK(hipGetLastError());
+  EXA_TRY(launch_projection(true, B, 1, nullptr, sc.a.as<double>(), sc.b.as<double>(), tab, ghost, nullptr));
   EXA_CHECK(hipDeviceSynchronize());
   EXA_CHECK(hipMemcpy(hout.data(), sc.b.p, sizeof(double) * hout.size(), hipMemcpyDeviceToHost));
   for (int n = 0; n < S * S * S; ++n)
@@ -139,11 +182,7 @@ extern "C" int exa_limiter_project_on_dg(int order, const double* lim, int ghost
   EXA_TRY(sc.a.ensure(sizeof(double) * hin.size()));
   EXA_TRY(sc.b.ensure(sizeof(double) * hout.size()));
   EXA_CHECK(hipMemcpy(sc.a.p, hin.data(), sizeof(double) * hin.size(), hipMemcpyHostToDevice));
-  EXA_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::lim::k_project_on_dg),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)proj_lds_bytes(B)));
-  hipLaunchKernelGGL(exa::lim::k_project_on_dg, dim3(1), dim3(256), proj_lds_bytes(B), 0, sc.a.as<double>(),
-                     sc.b.as<double>(), tab, B, ghost, nullptr);
-  EXA_CHECK(hipGetLastError());
+  EXA_TRY(launch_projection(false, B, 1, nullptr, sc.a.as<double>(), sc.b.as<double>(), tab, ghost, nullptr));
   EXA_CHECK(hipDeviceSynchronize());
   EXA_CHECK(hipMemcpy(hout.data(), sc.b.p, sizeof(double) * hout.size(), hipMemcpyDeviceToHost));
   for (int n = 0; n < B * B * B; ++n)
@@ -375,11 +414,7 @@ extern "C" int exa_grid_limiter_project(exa_grid* g, int ghost, double* d_lim, l
   const int B = v.order + 1, S = 2 * v.order + 1 + 2 * ghost;
   EXA_ARG((long long)n * exa::lim::V * S * S * S <= capacity, "d_lim holds fewer patches than troubled cells");
   if (n > 0) {
-    EXA_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&exa::lim::k_project_on_fv),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)proj_lds_bytes(B)));
-    hipLaunchKernelGGL(exa::lim::k_project_on_fv, dim3(n), dim3(256), proj_lds_bytes(B), v.stream, v.luh, d_lim, tab,
-                       B, ghost, L->list.as<int>());
-    EXA_CHECK(hipGetLastError());
+    EXA_TRY(launch_projection(true, B, n, v.stream, v.luh, d_lim, tab, ghost, L->list.as<int>()));
   }
   if (cells && n > 0) EXA_CHECK(hipMemcpyAsync(cells, L->list.p, sizeof(int) * n, hipMemcpyDeviceToHost, v.stream));
   EXA_CHECK(hipStreamSynchronize(v.stream));

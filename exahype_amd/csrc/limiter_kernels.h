@@ -170,6 +170,143 @@ __device__ inline void project_on_fv_block(const double* __restrict__ cellu, dou
   }
 }
 
+// The two projections with the order a compile-time constant: all variables
+// at once, pencil owners per axis with the operator entries read from the
+// tables at constant offsets (scalar loads), the first axis reading the
+// source block from memory and the last writing the result to memory; two
+// barriers per block instead of four per variable.  The same products and
+// sums in the same order as project_on_fv_block / project_on_dg_block.
+template <int B>
+constexpr int proj_t_lds_doubles() {
+  return V * (B * B * (2 * B - 1) + B * (2 * B - 1) * (2 * B - 1));
+}
+
+template <int B>
+__device__ inline void project_on_fv_block_t(const double* __restrict__ cellu, double* __restrict__ patch,
+                                             const LimTables* __restrict__ tab, int ghost, double* sm) {
+  constexpr int L = 2 * B - 1;
+  const int S = L + 2 * ghost;
+  double* a = sm;                  // [v][z][y][L]
+  double* b = sm + V * B * B * L;  // [v][z][L][L]
+  for (int it = threadIdx.x; it < V * B * B; it += blockDim.x) {  // x: pencils (v, z, y) of the cell
+    double q[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) q[k] = cellu[it * B + k];
+#pragma unroll
+    for (int o = 0; o < L; ++o) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < B; ++k) acc += q[k] * tab->u2l[k][o];
+      a[it * L + o] = acc;
+    }
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < V * B * L; it += blockDim.x) {  // y: pencils (v, z, x)
+    const int vz = it / L, x = it - vz * L;
+    double q[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) q[k] = a[(vz * B + k) * L + x];
+#pragma unroll
+    for (int o = 0; o < L; ++o) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < B; ++k) acc += q[k] * tab->u2l[k][o];
+      b[(vz * L + o) * L + x] = acc;
+    }
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < V * L * L; it += blockDim.x) {  // z: pencils (v, y, x), into the patch
+    const int v = it / (L * L), yx = it - v * L * L, y = yx / L, x = yx - y * L;
+    double q[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) q[k] = b[((v * B + k) * L + y) * L + x];
+    double* o_ = patch + (size_t)v * S * S * S + (size_t)(y + ghost) * S + x + ghost;
+#pragma unroll
+    for (int o = 0; o < L; ++o) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < B; ++k) acc += q[k] * tab->u2l[k][o];
+      o_[(size_t)(o + ghost) * S * S] = acc;
+    }
+  }
+}
+
+template <int B>
+__device__ inline void project_on_dg_block_t(const double* __restrict__ patch, double* __restrict__ cellu,
+                                             const LimTables* __restrict__ tab, int ghost, double* sm) {
+  constexpr int L = 2 * B - 1;
+  const int S = L + 2 * ghost;
+  double* a = sm;                  // [v][z][y][B] over the subcells' z, y
+  double* b = sm + V * L * L * B;  // [v][z][B][B]
+  for (int it = threadIdx.x; it < V * L * L; it += blockDim.x) {  // x: pencils (v, z, y) of the patch interior
+    const int v = it / (L * L), zy = it - v * L * L, z = zy / L, y = zy - z * L;
+    const double* in = patch + (size_t)v * S * S * S + ((size_t)(z + ghost) * S + y + ghost) * S + ghost;
+    double q[L];
+#pragma unroll
+    for (int k = 0; k < L; ++k) q[k] = in[k];
+#pragma unroll
+    for (int o = 0; o < B; ++o) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < L; ++k) acc += q[k] * tab->l2u[k][o];
+      a[it * B + o] = acc;
+    }
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < V * L * B; it += blockDim.x) {  // y: pencils (v, z, x)
+    const int vz = it / B, x = it - vz * B;
+    double q[L];
+#pragma unroll
+    for (int k = 0; k < L; ++k) q[k] = a[(vz * L + k) * B + x];
+#pragma unroll
+    for (int o = 0; o < B; ++o) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < L; ++k) acc += q[k] * tab->l2u[k][o];
+      b[(vz * B + o) * B + x] = acc;
+    }
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < V * B * B; it += blockDim.x) {  // z: pencils (v, y, x), into the cell
+    const int v = it / (B * B), yx = it - v * B * B;
+    double q[L];
+#pragma unroll
+    for (int k = 0; k < L; ++k) q[k] = b[(v * L + k) * B * B + yx];
+#pragma unroll
+    for (int o = 0; o < B; ++o) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < L; ++k) acc += q[k] * tab->l2u[k][o];
+      cellu[(size_t)v * B * B * B + o * B * B + yx] = acc;
+    }
+  }
+}
+
+// Block-per-cell projections with the order fixed (B <= 9: the two
+// all-variable buffers must fit the LDS).
+template <int B>
+__global__ __launch_bounds__(256) void k_project_on_fv_t(const double* __restrict__ luh, double* __restrict__ lim,
+                                                         const LimTables* __restrict__ tab, int ghost,
+                                                         const int* __restrict__ cells) {
+  extern __shared__ double sm[];
+  constexpr int L = 2 * B - 1;
+  const int S = L + 2 * ghost;
+  const int cell = cells ? cells[blockIdx.x] : blockIdx.x;
+  project_on_fv_block_t<B>(luh + (size_t)cell * V * B * B * B, lim + (size_t)blockIdx.x * V * S * S * S, tab, ghost,
+                           sm);
+}
+template <int B>
+__global__ __launch_bounds__(256) void k_project_on_dg_t(const double* __restrict__ lim, double* __restrict__ luh,
+                                                         const LimTables* __restrict__ tab, int ghost,
+                                                         const int* __restrict__ cells) {
+  extern __shared__ double sm[];
+  constexpr int L = 2 * B - 1;
+  const int S = L + 2 * ghost;
+  const int cell = cells ? cells[blockIdx.x] : blockIdx.x;
+  project_on_dg_block_t<B>(lim + (size_t)blockIdx.x * V * S * S * S, luh + (size_t)cell * V * B * B * B, tab, ghost,
+                           sm);
+}
+
 static __global__ void k_project_on_fv(const double* __restrict__ luh, double* __restrict__ lim, const LimTables* tab,
                                 int B, int ghost, const int* __restrict__ cells) {
   extern __shared__ double sm[];
