@@ -314,7 +314,7 @@ __global__ __launch_bounds__(256) void k_ghost_fill(double* __restrict__ pool, c
 
 // MUSCL-Hancock update of the listed patches: old pool -> new pool.
 template <int P>
-__global__ __launch_bounds__(256) void k_muscl_list(const double* __restrict__ pool_old, double* __restrict__ pool_new,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fv::muscl_min_waves(P)))) void k_muscl_list(const double* __restrict__ pool_old, double* __restrict__ pool_new,
                                                     const int* __restrict__ slots, const int* __restrict__ count,
                                                     double dx, const InstParams* __restrict__ inst, int level,
                                                     double gamma) {

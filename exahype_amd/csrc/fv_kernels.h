@@ -157,7 +157,9 @@ __device__ inline double minmod(double a, double b) {
 //            touches -- its own and its neighbours' w = q -+ h/2 slope, plus
 //            dt/2 Qt where the owner is interior -- and applies the Rusanov
 //            fluxes on (w_R[i], w_L[i+1]) (:215-281); the first ghost layer
-//            receives the patch-boundary face updates as in the reference.
+//            receives the patch-boundary face updates as in the reference,
+//            from the interior subcell next to it, which has just formed
+//            that face's flux for its own update.
 // The states are recomputed from the patch values instead of being stored
 // (six state arrays per patch would be 6x the patch's HBM traffic); they are
 // the same operations on the same inputs, hence the same values.
@@ -231,6 +233,7 @@ __device__ inline double muscl_patch(const double* __restrict__ u, double* __res
     int i, j, k;
     interior_coords(g, c, i, j, k);
     const int idx = box_index(g, i, j, k);
+    const int ijk[3] = {i, j, k};
     double acc[V], a[V], b[V], f[V];
     load_q(u, nbox, idx, acc);
 #pragma unroll
@@ -243,6 +246,11 @@ __device__ inline double muscl_patch(const double* __restrict__ u, double* __res
       dtm = cand < dtm ? cand : dtm;
 #pragma unroll
       for (int v = 0; v < V; ++v) acc[v] += fac * f[v];
+      if (ijk[d] == g.G) {  // the lower ghost's face with this subcell
+        const int gi = idx - g.stride(d);
+#pragma unroll
+        for (int v = 0; v < V; ++v) un[v * nbox + gi] = un[v * nbox + gi] - fac * f[v];
+      }
       muscl_face_state(u, nbox, g, sQt, idx, d, 1, h[d], dt, a);
       muscl_face_state(u, nbox, g, sQt, idx + g.stride(d), d, 0, h[d], dt, b);
       s = rusanov<PDE>(a, b, d, gamma, f);
@@ -250,30 +258,27 @@ __device__ inline double muscl_patch(const double* __restrict__ u, double* __res
       dtm = cand < dtm ? cand : dtm;
 #pragma unroll
       for (int v = 0; v < V; ++v) acc[v] -= fac * f[v];
+      if (ijk[d] == g.G + g.P - 1) {  // the upper ghost's
+        const int gi = idx + g.stride(d);
+#pragma unroll
+        for (int v = 0; v < V; ++v) un[v * nbox + gi] = un[v * nbox + gi] + fac * f[v];
+      }
     }
 #pragma unroll
     for (int v = 0; v < V; ++v) un[v * nbox + idx] = acc[v];
   }
-  const int nghost = 6 * g.P * g.P;
-  for (int it = threadIdx.x; it < nghost; it += blockDim.x) {
-    int d, side, idx, inner;
-    ghost_layer_cell(g, it, d, side, idx, inner);
-    double a[V], b[V], f[V];
-    muscl_face_state(u, nbox, g, sQt, side ? inner : idx, d, 1, h[d], dt, a);
-    muscl_face_state(u, nbox, g, sQt, side ? idx : inner, d, 0, h[d], dt, b);
-    rusanov<PDE>(a, b, d, gamma, f);
-    const double fac = dt / h[d];
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-      const double o = un[v * nbox + idx];
-      un[v * nbox + idx] = side ? o + fac * f[v] : o - fac * f[v];
-    }
-  }
   return dtm;
 }
 
+// Waves per SIMD the MUSCL-Hancock kernels are compiled for: up to 11^3
+// subcells three workgroups' half-step terms fit the LDS (53 KB each at
+// P = 11), and the kernel is then held to 168 VGPRs to let them all run
+// (4.33 against 4.55 ms per launch on 27^3 patches of 11^3); larger patches
+// keep the compiler's choice.
+__host__ __device__ constexpr int muscl_min_waves(int P) { return P <= 11 ? 3 : 1; }
+
 template <class PDE, int PC>
-__global__ __launch_bounds__(256) void k_muscl(const double* __restrict__ luh, double* __restrict__ luh_new,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(muscl_min_waves(PC)))) void k_muscl(const double* __restrict__ luh, double* __restrict__ luh_new,
                                                double* __restrict__ dt_patch, unsigned long long* dt_min,
                                                PatchGeom, double dx0, double dx1, double dx2,
                                                const double* __restrict__ dtp, double gamma, double cfl) {
