@@ -51,10 +51,6 @@ extern "C" {
                                  FP64 correction sweeps                                        */
 #define EXA_PREC_BF16_MFMA 3  /* as EXA_PREC_FP32_MFMA with BF16 contraction operands (FP32
                                  accumulation)                                                 */
-/* In the mixed precisions (orders 1..7), a cell whose nodes differ from one
-   state by less than the low precision resolves (4 eps max|u|; exactly
-   uniform for BF16) takes the FP64 sweeps alone, with the FP64 predictor's
-   stopping rule and outputs. */
 
 /* PDEs compiled into the kernels */
 #define EXA_PDE_EULER 0   /* 3D compressible Euler, 5 variables (gamma from config) */

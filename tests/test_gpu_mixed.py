@@ -144,37 +144,3 @@ def test_mixed_staged_50_steps_l2_vs_fp64(N, n, contraction):
     l2 = l2_per_variable(b, a, N, n ** 3, 1.0 / n ** 3)
     assert l2.max() <= 1e-6, l2
     assert abs(ta - tb) <= 1e-6 * ta
-
-
-@pytest.mark.parametrize("prec", [1, 2, 3])
-@pytest.mark.parametrize("N", [3, 5])
-def test_uniform_cell_takes_the_fp64_sweeps(N, prec):
-    """A cell whose nodes all hold one state is the fixed point of the
-    iteration already: the mixed predictor gives it the FP64 sweeps alone, so
-    its outputs and sweep count are the FP64 predictor's, bit for bit; so does
-    a cell whose nodes differ by less than the FP32 resolution of the state
-    (FP32 operands; BF16 operands only for an exactly uniform cell).  A cell
-    one node of which differs by more takes the low-precision sweeps first."""
-    import exahype_amd as E
-    B = N + 1
-    state = np.array([1.2, 0.3, -0.2, 0.1, 2.5])
-    luh = np.tile(state, B ** 3)
-    near = luh.copy()
-    near[5 * 3 + 1] += 1e-13
-    dx = np.full(3, 1 / 27)
-    dt = 1e-3
-    for u in ((luh,) if prec == 3 else (luh, near)):
-        f64 = E.ADERDGSolver(N, precision=E.PREC_FP64, fp64_sweeps=1).spaceTimePredictor(u, dx, dt)
-        mix = E.ADERDGSolver(N, precision=prec, fp64_sweeps=1).spaceTimePredictor(u, dx, dt)
-        assert mix["iters"] == f64["iters"], (mix["iters"], f64["iters"])
-        for k in ("lQhbnd", "lFhbnd", "lQhi", "lFhi"):
-            assert np.array_equal(mix[k], f64[k]), k
-    bumped = luh.copy()
-    bumped[5 * 7] += 1e-3
-    # BF16 operands take two FP64 sweeps to reach 1e-3 (as above)
-    g = E.ADERDGSolver(N, precision=prec, fp64_sweeps=2 if prec == 3 else 1).spaceTimePredictor(bumped, dx, dt)
-    o = O.space_time_predictor(O.cfg(N), bumped, dx, dt)
-    assert g["iters"] >= 2  # at least one low-precision and one FP64 sweep
-    for k in ("lQhbnd", "lFhbnd", "lQhi"):
-        a, b = g[k].reshape(-1, 5), o[k].reshape(-1, 5)
-        assert np.abs(a - b).max() / np.abs(b).max() <= (1e-3 if prec == 3 else 5e-6), k
