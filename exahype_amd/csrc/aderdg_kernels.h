@@ -749,7 +749,13 @@ enum : int { FORM_PENCIL = 0, FORM_NODE = 1, FORM_PENCIL_PAIR = 2 };
 
 // SRC: the PDE has the gravity source (Euler with gravity, FP64 only); the
 // source code is compiled out of the plain Euler kernels.
-template <int N, bool DEBUG, int PREC, int WPE, int FORM, bool SRC = false>
+// USKIP (FP32 sweeps only): a cell whose nodes differ from one state by less
+// than FP32 resolves (4 eps |u|max) gives the FP32 sweeps nothing to do and
+// takes the FP64 sweeps alone, exactly as the FP64 predictor (a uniform cell
+// starts at the fixed point and stops after one sweep).  A separate variant
+// because the check's registers cost the plain mixed kernel ~1 %; the
+// adaptive mesh, mostly uniform away from its features, uses it.
+template <int N, bool DEBUG, int PREC, int WPE, int FORM, bool SRC = false, bool USKIP = false>
 __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_predictor(
     const double* __restrict__ luh, double* __restrict__ lduh, double* __restrict__ qf,
     double* __restrict__ ff, int* __restrict__ iters, const double* __restrict__ dtp, double dx0,
@@ -795,42 +801,58 @@ __global__ __launch_bounds__(Ord<N>::NT) __attribute__((amdgpu_waves_per_eu(WPE,
   constexpr int MaxIterations = 2 * (N + 1);
   int it = 0;
   if constexpr (MIXED) {
-    float qs[B][V];
-#pragma unroll
-    for (int t = 0; t < B; ++t)
-#pragma unroll
-      for (int v = 0; v < V; ++v) qs[t][v] = (float)u[v];
+    static_assert(!USKIP || PREC == PRED_FP32, "the uniform-cell variant is built for FP32 sweeps");
     // rounding floor of sum((dq)^2) over the cell in the low precision:
     // count * (4 eps * |u|max)^2, eps the unit roundoff of the contraction
     // operands (FP32 2^-24, BF16 2^-8)
-    double um = 0.0;
+    double um = 0.0, dev = 0.0;
 #pragma unroll
-    for (int v = 0; v < V; ++v) um = fmax(um, fabs(u[v]));
+    for (int v = 0; v < V; ++v) {
+      um = fmax(um, fabs(u[v]));
+      if constexpr (USKIP)
+        if (act) dev = fmax(dev, fabs(u[v] - luh[cell * V * B3 + v * B3]));
+    }
     um = -block_min(-um, red);
     const double eps = PREC == PRED_BF16_MFMA ? 3.90625e-3 : 5.96e-8;
-    const double floor_lp = (double)(V * O::B4) * (4.0 * eps * um) * (4.0 * eps * um);
-    const double tol_lp = fmax(1e-7 * 1e-7, floor_lp);
-    float* sFf = reinterpret_cast<float*>(sm);
-    if constexpr (PREC == PRED_FP32_MFMA || PREC == PRED_BF16_MFMA)
-      it = picard_sweeps_mfma<N, PREC == PRED_BF16_MFMA>(qs, u, sFf, sFf + 3 * V * B3, scf, red, act, s, z, y, x,
-                                                         gamma, MaxIterations, tol_lp);
-    else if constexpr (FORM == FORM_NODE)
-      it = picard_sweeps_node<N, float, false, false>(qs, u, sFf, scf, red, act, s, z, y, x, gamma, MaxIterations,
-                                                      tol_lp, fh, dbg);
-    else
-      it = picard_sweeps<N, float, false, false, FORM == FORM_PENCIL_PAIR>(qs, u, sFf, sFf + 3 * V * BLK, scf, red,
-                                                                           act, s, z, y, x, gamma, MaxIterations,
-                                                                           tol_lp, fh, dbg);
+    bool uniform = false;
+    if constexpr (USKIP) uniform = -block_min(-dev, red) <= 4.0 * eps * um;
+    int max64 = fp64_sweeps;
+    if (uniform) {
 #pragma unroll
-    for (int t = 0; t < B; ++t)
+      for (int t = 0; t < B; ++t)
 #pragma unroll
-      for (int v = 0; v < V; ++v) q[t][v] = (double)qs[t][v];
+        for (int v = 0; v < V; ++v) q[t][v] = u[v];
+      max64 = MaxIterations;
+    } else {
+      float qs[B][V];
+#pragma unroll
+      for (int t = 0; t < B; ++t)
+#pragma unroll
+        for (int v = 0; v < V; ++v) qs[t][v] = (float)u[v];
+      const double floor_lp = (double)(V * O::B4) * (4.0 * eps * um) * (4.0 * eps * um);
+      const double tol_lp = fmax(1e-7 * 1e-7, floor_lp);
+      float* sFf = reinterpret_cast<float*>(sm);
+      if constexpr (PREC == PRED_FP32_MFMA || PREC == PRED_BF16_MFMA)
+        it = picard_sweeps_mfma<N, PREC == PRED_BF16_MFMA>(qs, u, sFf, sFf + 3 * V * B3, scf, red, act, s, z, y, x,
+                                                           gamma, MaxIterations, tol_lp);
+      else if constexpr (FORM == FORM_NODE)
+        it = picard_sweeps_node<N, float, false, false>(qs, u, sFf, scf, red, act, s, z, y, x, gamma, MaxIterations,
+                                                        tol_lp, fh, dbg);
+      else
+        it = picard_sweeps<N, float, false, false, FORM == FORM_PENCIL_PAIR>(qs, u, sFf, sFf + 3 * V * BLK, scf, red,
+                                                                             act, s, z, y, x, gamma, MaxIterations,
+                                                                             tol_lp, fh, dbg);
+#pragma unroll
+      for (int t = 0; t < B; ++t)
+#pragma unroll
+        for (int v = 0; v < V; ++v) q[t][v] = (double)qs[t][v];
+    }
     if constexpr (FORM == FORM_NODE)
-      it += picard_sweeps_node<N, double, true, DEBUG>(q, u, sm, scd, red, act, s, z, y, x, gamma, fp64_sweeps,
+      it += picard_sweeps_node<N, double, true, DEBUG>(q, u, sm, scd, red, act, s, z, y, x, gamma, max64,
                                                        1e-7 * 1e-7, fh, dbg, grav, dt, sh);
     else
       it += picard_sweeps<N, double, true, DEBUG, FORM == FORM_PENCIL_PAIR>(q, u, sm, sm + 3 * V * BLK, scd, red, act,
-                                                                            s, z, y, x, gamma, fp64_sweeps,
+                                                                            s, z, y, x, gamma, max64,
                                                                             1e-7 * 1e-7, fh, dbg, grav, dt, sh);
   } else {
 #pragma unroll

@@ -71,15 +71,15 @@ static PredFn predictor_kernel_src(bool debug) {
 }
 
 template <int N>
-static PredFn predictor_kernel(bool debug, int prec, int wpe, int form) {
+static PredFn predictor_kernel(bool debug, int prec, int wpe, int form, bool uskip = false) {
   switch (N) {
-    case 1: return exa_pred_kernel_1(debug, prec, wpe, form);
-    case 2: return exa_pred_kernel_2(debug, prec, wpe, form);
-    case 3: return exa_pred_kernel_3(debug, prec, wpe, form);
-    case 4: return exa_pred_kernel_4(debug, prec, wpe, form);
-    case 5: return exa_pred_kernel_5(debug, prec, wpe, form);
-    case 6: return exa_pred_kernel_6(debug, prec, wpe, form);
-    case 7: return exa_pred_kernel_7(debug, prec, wpe, form);
+    case 1: return exa_pred_kernel_1(debug, prec, wpe, form, uskip);
+    case 2: return exa_pred_kernel_2(debug, prec, wpe, form, uskip);
+    case 3: return exa_pred_kernel_3(debug, prec, wpe, form, uskip);
+    case 4: return exa_pred_kernel_4(debug, prec, wpe, form, uskip);
+    case 5: return exa_pred_kernel_5(debug, prec, wpe, form, uskip);
+    case 6: return exa_pred_kernel_6(debug, prec, wpe, form, uskip);
+    case 7: return exa_pred_kernel_7(debug, prec, wpe, form, uskip);
     default: return nullptr;  // orders 8, 9: exa_pred_staged_kernel
   }
 }
@@ -1422,7 +1422,9 @@ int exa_solver_predictor(const exa_solver* s, PredFn* fn, int* threads, size_t* 
     using O = exa::Ord<NN>;
     const int wpe = predictor_waves_per_simd(s->cfg, NN, s->device);
     const int form = predictor_form(s->cfg, NN, s->device);
-    *fn = predictor_kernel<NN>(false, s->cfg.precision, wpe, form);
+    // the adaptive mesh is mostly uniform away from its features: FP32
+    // sweeps skip the cells they cannot improve
+    *fn = predictor_kernel<NN>(false, s->cfg.precision, wpe, form, true);
     *threads = O::NT;
     *lds = O::LDS_DOUBLES * sizeof(double);
   });

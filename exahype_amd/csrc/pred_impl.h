@@ -27,11 +27,23 @@ PredFn by_form(int prec, int form) {
     default: return by_prec<DEBUG, WPE, exa::FORM_PENCIL>(prec);
   }
 }
+
+// FP32 sweeps, the uniform-cell variant
+template <int WPE>
+PredFn uskip_by_form(int form) {
+  switch (form) {
+    case exa::FORM_NODE: return &exa::k_predictor<NP, false, exa::PRED_FP32, WPE, exa::FORM_NODE, false, true>;
+    case exa::FORM_PENCIL_PAIR:
+      return &exa::k_predictor<NP, false, exa::PRED_FP32, WPE, exa::FORM_PENCIL_PAIR, false, true>;
+    default: return &exa::k_predictor<NP, false, exa::PRED_FP32, WPE, exa::FORM_PENCIL, false, true>;
+  }
+}
 }  // namespace
 
-PredFn EXA_CAT(exa_pred_kernel_, EXA_PRED_ORDER)(bool debug, int prec, int wpe, int form) {
-  if (debug) return by_form<true, 1>(prec, form);
+PredFn EXA_CAT(exa_pred_kernel_, EXA_PRED_ORDER)(bool debug, int prec, int wpe, int form, bool uskip) {
   constexpr int W2 = NP <= 5 ? 2 : 1;
+  if (uskip && !debug && prec == EXA_PREC_FP32) return wpe == 2 ? uskip_by_form<W2>(form) : uskip_by_form<1>(form);
+  if (debug) return by_form<true, 1>(prec, form);
   if (wpe == 2) return by_form<false, W2>(prec, form);
   return by_form<false, 1>(prec, form);
 }
@@ -58,6 +70,13 @@ hipError_t EXA_CAT(exa_pred_init_, EXA_PRED_ORDER)(const DGTables* tab, const DG
               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
           if (e != hipSuccess) return e;
         }
+  for (int w = 1; w <= 2; ++w)
+    for (int form = 0; form < 3; ++form) {
+      e = hipFuncSetAttribute(
+          reinterpret_cast<const void*>(EXA_CAT(exa_pred_kernel_, EXA_PRED_ORDER)(false, EXA_PREC_FP32, w, form, true)),
+          hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+      if (e != hipSuccess) return e;
+    }
   for (int dbg = 0; dbg < 2; ++dbg) {
     e = hipFuncSetAttribute(reinterpret_cast<const void*>(EXA_CAT(exa_pred_kernel_src_, EXA_PRED_ORDER)(dbg)),
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);

@@ -17,10 +17,11 @@ using PredFn = void (*)(const double*, double*, double*, double*, int*, const do
 // Kernel of one variant: per-cell debug outputs (one wave per SIMD only),
 // precision EXA_PREC_*, waves per SIMD (2 is built for N <= 5), contraction
 // form exa::FORM_* (pencil owners, node owners, pencil owners two slices at
-// a time).  exa_pred_kernel_src_N: the gravity-source variant (FP64, pencil
-// owners, one wave per SIMD).
+// a time); uskip: with FP32 sweeps, the variant in which cells uniform to
+// FP32 resolution take the FP64 sweeps alone.  exa_pred_kernel_src_N: the
+// gravity-source variant (FP64, pencil owners, one wave per SIMD).
 #define EXA_PRED_TU_DECL(N)                                                   \
-  PredFn exa_pred_kernel_##N(bool debug, int prec, int wpe, int form);        \
+  PredFn exa_pred_kernel_##N(bool debug, int prec, int wpe, int form, bool uskip = false); \
   PredFn exa_pred_kernel_src_##N(bool debug);                                 \
   hipError_t exa_pred_init_##N(const DGTables* tab, const DGTablesF* tabf);
 EXA_PRED_TU_DECL(1)

@@ -51,6 +51,10 @@ extern "C" {
                                  FP64 correction sweeps                                        */
 #define EXA_PREC_BF16_MFMA 3  /* as EXA_PREC_FP32_MFMA with BF16 contraction operands (FP32
                                  accumulation)                                                 */
+/* On the adaptive mesh (exa_amr_grid_*) with EXA_PREC_FP32, a cell whose
+   nodes differ from one state by less than FP32 resolves (4 eps max|u|)
+   takes the FP64 sweeps alone, with the FP64 predictor's stopping rule and
+   outputs. */
 
 /* PDEs compiled into the kernels */
 #define EXA_PDE_EULER 0   /* 3D compressible Euler, 5 variables (gamma from config) */

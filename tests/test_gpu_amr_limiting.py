@@ -164,3 +164,23 @@ def test_constant_state_stays_constant_across_levels():
     g.sync()
     # to the 13-digit Kxi literals, which annihilate constants to ~1e-13 per 1/dx
     assert np.abs(g.download() - u).max() <= 1e-11
+
+
+def test_mixed_predictor_on_uniform_cells_is_the_fp64_step():
+    """With FP32 sweeps, the adaptive mesh's predictor gives cells whose nodes
+    differ from one state by less than FP32 resolves the FP64 sweeps alone,
+    exactly as the FP64 predictor: on a constant state (which the step keeps
+    constant to rounding) the mixed steps equal the FP64 steps bit for bit."""
+    import exahype_amd as E
+    N = 3
+    q0 = (1.0, 0.1, -0.2, 0.3, 2.5)
+    out = []
+    for p in (E.PREC_FP64, E.PREC_FP32):
+        s = E.ADERDGSolver(N, precision=p, fp64_sweeps=1)
+        g = E.AMRGrid(s, E.amr_config(3, 3, (0.5, 0.5, 0.5), 0.0, [0.01, 0.01]))
+        g.upload(np.tile(np.array(q0), g.ncells * (N + 1) ** 3))
+        g.compute_dt()
+        g.step(3)
+        g.sync()
+        out.append(g.download())
+    assert np.array_equal(out[0], out[1])
